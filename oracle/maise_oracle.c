@@ -1,0 +1,840 @@
+/*
+ * maise_oracle.c -- FP64 CPU restatement of MAISE's neural-network potential
+ * evaluation (energy, forces, stress from Behler-Parrinello G2/G4 descriptors and
+ * a per-species D-H1-H2-1 tanh MLP).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the GPU library is checked
+ * against: only tests/, the smoke check and the CPU-baseline leg of bench.py may
+ * load it.  The product library (maise_amd/libmaise_gpu.so) never links or calls it.
+ *
+ * Each routine cites the reference function it restates (paths are relative to the
+ * MAISE v3.0.00 source tree).  The code is written from the mathematics those
+ * functions implement; where bit-level agreement with the reference matters (the
+ * FP64 wrap and neighbour test that decide neighbour SETS) the floating-point
+ * expression order is kept identical and the file is compiled without FMA
+ * contraction (-ffp-contract=off).
+ *
+ * Pinning (see tests/test_oracle_*.py):
+ *   - model + basis reader ........ interface/c/test OUTCAR energies, golden vectors
+ *   - JAR / LIST .................. neighbour sets vs the reference build (oracle/_ref)
+ *   - descriptors ................. test/parse/ref/e* descriptor files (exact text values)
+ *   - MLP energy .................. interface/c/test/OUTCAR, test/relax/ref/OUTCAR,
+ *                                   examples/03-rlx/README, examples/07-nvt/README
+ *   - forces / virial ............. finite differences of the energy (positions and
+ *                                   strain) and the reference build's FRC/STR outputs
+ *   - UREP ........................ finite differences + reference build (UREP>0 case)
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_MAXSPC 4
+#define ORC_MAXSYM 128
+#define ORC_NPAR 7     /* parameter rows: Rc, Rs, eta2, eta4, k, zeta, lambda */
+#define ORC_NVAL 10    /* values per parameter row */
+
+static const double ORC_PI = 3.1415926535897932384626433832795; /* src/util.c:3 */
+static const double ORC_EV2GPA = 160.2176565;                     /* src/util.c:8 */
+
+typedef struct {
+  int gf[ORC_MAXSYM][8];               /* function table: type, Rc, Rs, n1, n2, k, z, l idx */
+  double gp[ORC_NPAR][ORC_NVAL];       /* scaled parameters */
+  int gt[ORC_NPAR];                    /* number of values per row */
+} orc_basis;
+
+typedef struct {
+  int nspc, spcz[ORC_MAXSPC];
+  int nl, nu[4], nw;
+  int D, nsym, ng2, ng4;
+  double *rmin, *rmax, *dr;            /* [nspc][D] after the READ_ANN fold */
+  double *B[ORC_MAXSPC][3];            /* biases per layer */
+  double *W[ORC_MAXSPC][3];            /* weights per layer, [out][in] */
+  orc_basis w[10];                     /* indexed by s0*s0+s1*s1, as src/nprs.c:137 */
+  double rc_list;                      /* LIST cutoff (src/nprs.c:456-462) */
+} orc_model;
+
+/* ---------------------------------------------------------------- utilities */
+static void set_err(char *err, int errlen, const char *msg) {
+  if (err && errlen > 0) { strncpy(err, msg, errlen - 1); err[errlen - 1] = 0; }
+}
+
+/* version parse, restating check_ver (src/util.c:1948-1996): "maise.n.k.d" -> n*1000+k*100+d */
+static int orc_check_ver(const char *path) {
+  FILE *in = fopen(path, "r");
+  char s[256], t[256];
+  int n = 0, k = 0, d = 0;
+  if (!in) return -2;
+  if (!fgets(s, 200, in) || !fgets(s, 200, in)) { fclose(in); return -1; }
+  {
+    char w[256] = {0};
+    sscanf(s + 2, "%255s", w);
+    if (strncmp(w, "neural", 6) != 0) { fclose(in); return -1; }
+  }
+  s[0] = 0;
+  while (fgets(s, 200, in))
+    if (strncmp(s, "|  maise version", 16) == 0) break;
+  t[0] = 0;
+  sscanf(s, "|  maise version | %255s |", t);
+  /* t looks like "maise.2.4.m5" / "maise.3.0.00": numbers after the 6-char prefix */
+  {
+    size_t i = 6, L = strlen(t);
+    int *dst[3] = {&n, &k, &d};
+    for (int f = 0; f < 3 && i < L; f++) {
+      char part[64];
+      size_t len = strcspn(t + i, ".");
+      if (len > 63) len = 63;
+      memcpy(part, t + i, len);
+      part[len] = 0;
+      sscanf(part, "%d", dst[f]);
+      i += len + 1;
+    }
+  }
+  fclose(in);
+  return n * 1000 + k * 100 + d;
+}
+
+/* ---------------------------------------------------------- model reading */
+/* Restates READ_MODEL (src/util.c:1233-1335), READ_ANN (src/nutl.c:502-633),
+ * V2W (src/nmlp.c:591-608) and Build_PRS (src/nprs.c:75-257). */
+static int read_header_int(FILE *in, const char *key, int *val) {
+  char s[512];
+  while (fgets(s, 500, in))
+    if (strncmp(s, key, 25) == 0) { sscanf(s + 26, "%d", val); return 1; }
+  return 0;
+}
+
+static int read_header_ints(FILE *in, const char *key, int n, int *vals) {
+  char s[512];
+  while (fgets(s, 500, in))
+    if (strncmp(s, key, 25) == 0) {
+      int k = 26, c = 0;
+      for (int i = 0; i < n; i++, k += c) sscanf(s + k, "%d%n", &vals[i], &c);
+      return 1;
+    }
+  return 0;
+}
+
+void orc_free(orc_model *m) {
+  if (!m) return;
+  free(m->rmin); free(m->rmax); free(m->dr);
+  for (int s = 0; s < ORC_MAXSPC; s++)
+    for (int k = 0; k < 3; k++) { free(m->B[s][k]); free(m->W[s][k]); }
+  free(m);
+}
+
+static int read_basis_block(FILE *in, orc_basis *b, int nsym) {
+  char buf[512], name[64];
+  double a = 0, r = 0;
+  if (!fgets(buf, 500, in)) return 0;
+  if (sscanf(buf, "%lf %lf", &a, &r) != 2) return 0;
+  for (int i = 0; i < 3; i++) if (!fgets(buf, 500, in)) return 0;
+  for (int n = 0; n < nsym; n++) {
+    int idx;
+    if (!fgets(buf, 500, in)) return 0;
+    if (sscanf(buf, "%d %d %d %d %d %d %d %d %d", &idx, &b->gf[n][0], &b->gf[n][1], &b->gf[n][2],
+               &b->gf[n][3], &b->gf[n][4], &b->gf[n][5], &b->gf[n][6], &b->gf[n][7]) != 9)
+      return 0;
+  }
+  if (!fgets(buf, 500, in)) return 0;
+  for (int p = 0; p < ORC_NPAR; p++) {
+    double *g = b->gp[p];
+    if (!fgets(buf, 500, in)) return 0;
+    for (int i = 0; i < ORC_NVAL; i++) g[i] = 0.0;
+    int got = sscanf(buf, "%63s %d %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", name, &b->gt[p], &g[0], &g[1],
+                     &g[2], &g[3], &g[4], &g[5], &g[6], &g[7], &g[8], &g[9]);
+    if (got < 3) return 0;
+  }
+  /* unit scaling, src/nprs.c:183-189: distances * a*r, inverse squares / (a*r)^2 */
+  for (int p = 0; p < 2; p++)
+    for (int i = 0; i < ORC_NVAL; i++) b->gp[p][i] *= a * r;
+  for (int p = 2; p < 4; p++)
+    for (int i = 0; i < ORC_NVAL; i++) b->gp[p][i] /= a * a * r * r;
+  return 1;
+}
+
+int orc_load(const char *path, orc_model **out, char *err, int errlen) {
+  char s[512];
+  FILE *in;
+  int ver = orc_check_ver(path);
+  *out = NULL;
+  if (ver == -2) { set_err(err, errlen, "cannot open model file"); return -1; }
+  if (ver < 2400) { set_err(err, errlen, "model file is not a neural model of version >= 2.4.00"); return -1; }
+  in = fopen(path, "r");
+  orc_model *m = (orc_model *)calloc(1, sizeof(orc_model));
+  if (!read_header_int(in, "|  number of species    |", &m->nspc) || m->nspc < 1 || m->nspc > ORC_MAXSPC) {
+    set_err(err, errlen, "bad number of species"); fclose(in); orc_free(m); return -1;
+  }
+  read_header_ints(in, "|  species types        |", m->nspc, m->spcz);
+  if (!read_header_int(in, "|  number of layers     |", &m->nl) || m->nl != 4) {
+    set_err(err, errlen, "only 4-layer (D-H1-H2-1) networks are supported"); fclose(in); orc_free(m); return -1;
+  }
+  read_header_ints(in, "|  architecture         |", m->nl, m->nu);
+  read_header_int(in, "|  number of weights    |", &m->nw);
+  m->D = m->nu[0];
+  while (fgets(s, 500, in))
+    if (strncmp(s, "        minimum          ", 25) == 0) break;
+  m->rmin = (double *)calloc((size_t)m->nspc * m->D, sizeof(double));
+  m->rmax = (double *)calloc((size_t)m->nspc * m->D, sizeof(double));
+  m->dr = (double *)calloc((size_t)m->nspc * m->D, sizeof(double));
+  for (int sp = 0; sp < m->nspc; sp++)
+    for (int n = 0; n < m->D; n++) {
+      size_t o = (size_t)sp * m->D + n;
+      if (fscanf(in, "%lf %lf %lf\n", &m->rmin[o], &m->rmax[o], &m->dr[o]) != 3) {
+        set_err(err, errlen, "bad input-range block"); fclose(in); orc_free(m); return -1;
+      }
+    }
+  for (int i = 0; i < 3; i++) fgets(s, 500, in);
+  {
+    int nw = 0;
+    for (int k = 0; k < m->nl - 1; k++) nw += (m->nu[k] + 1) * m->nu[k + 1];
+    nw *= m->nspc;
+    if (nw != m->nw) { set_err(err, errlen, "number of weights does not match architecture"); fclose(in); orc_free(m); return -1; }
+  }
+  double *V = (double *)malloc(sizeof(double) * m->nw);
+  for (int n = 0; n < m->nw; n++)
+    if (fscanf(in, "%lf\n", &V[n]) != 1) { set_err(err, errlen, "short weight list"); free(V); fclose(in); orc_free(m); return -1; }
+  /* weight-vector order: per species, all biases layer by layer, then weights [out][in] (src/nmlp.c:591-608) */
+  {
+    int i = 0;
+    for (int sp = 0; sp < m->nspc; sp++) {
+      for (int k = 0; k < m->nl - 1; k++) {
+        m->B[sp][k] = (double *)malloc(sizeof(double) * m->nu[k + 1]);
+        for (int o = 0; o < m->nu[k + 1]; o++) m->B[sp][k][o] = V[i++];
+      }
+      for (int k = 0; k < m->nl - 1; k++) {
+        m->W[sp][k] = (double *)malloc(sizeof(double) * m->nu[k + 1] * m->nu[k]);
+        for (int o = 0; o < m->nu[k + 1]; o++)
+          for (int n = 0; n < m->nu[k]; n++) m->W[sp][k][o * m->nu[k] + n] = V[i++];
+      }
+    }
+  }
+  free(V);
+  /* fold the input scaling into the first layer, src/nutl.c:621-630 */
+  for (int sp = 0; sp < m->nspc; sp++) {
+    for (int o = 0; o < m->nu[1]; o++)
+      for (int n = 0; n < m->D; n++) m->W[sp][0][o * m->D + n] *= m->dr[(size_t)sp * m->D + n];
+    for (int n = 0; n < m->D; n++) m->rmin[(size_t)sp * m->D + n] /= m->dr[(size_t)sp * m->D + n];
+  }
+  /* basis: count NSYM as the lines after "  B2A" + 4 up to " Rc  " (src/util.c:1310-1321) */
+  rewind(in);
+  while (fgets(s, 500, in))
+    if (strncmp(s, "  B2A", 5) == 0) break;
+  for (int i = 0; i < 4; i++) fgets(s, 500, in);
+  {
+    int i = -1;
+    while (fgets(s, 500, in)) {
+      if (strncmp(s, " Rc  ", 5) == 0) break;
+      i++;
+    }
+    m->nsym = i;
+  }
+  if (m->nsym < 1 || m->nsym > ORC_MAXSYM) { set_err(err, errlen, "bad basis"); fclose(in); orc_free(m); return -1; }
+  /* per-species blocks; a single block serves all species (src/nprs.c:221-230) */
+  rewind(in);
+  while (fgets(s, 500, in))
+    if (strncmp(s, "  B2A", 5) == 0) break;
+  for (int w0 = 0; w0 < m->nspc; w0++) {
+    orc_basis *b = &m->w[2 * w0 * w0];
+    if (!read_basis_block(in, b, m->nsym)) {
+      set_err(err, errlen, "cannot parse basis block"); fclose(in); orc_free(m); return -1;
+    }
+    fgets(s, 500, in);
+    if (w0 < m->nspc - 1 && fgets(s, 500, in) == NULL) {
+      rewind(in);
+      while (fgets(s, 500, in))
+        if (strncmp(s, "  B2A", 5) == 0) break;
+    }
+  }
+  fclose(in);
+  {
+    orc_basis *b0 = &m->w[0];
+    m->ng2 = m->ng4 = 0;
+    for (int n = 0; n < m->nsym; n++) {
+      if (b0->gf[n][0] == 2) m->ng2++;
+      if (b0->gf[n][0] == 4) m->ng4++;
+    }
+    int c = 0;
+    for (int n = m->nspc; n > 0; n--) c += n;
+    if (m->ng2 * m->nspc + m->ng4 * c != m->D) {
+      set_err(err, errlen, "descriptor width does not match basis (NCMP)"); orc_free(m); return -1;
+    }
+    if (m->ng2 + m->ng4 != m->nsym) { set_err(err, errlen, "only G2/G4 functions are supported"); orc_free(m); return -1; }
+  }
+  /* species-pair mixing, src/nprs.c:233-252 */
+  for (int w0 = 0; w0 < m->nspc; w0++)
+    for (int w1 = 0; w1 < m->nspc; w1++)
+      if (w0 != w1) {
+        orc_basis *a = &m->w[2 * w0 * w0], *b = &m->w[2 * w1 * w1], *c = &m->w[w0 * w0 + w1 * w1];
+        memcpy(c->gf, m->w[0].gf, sizeof(c->gf));
+        memcpy(c->gt, m->w[0].gt, sizeof(c->gt));
+        for (int p = 0; p < ORC_NPAR; p++)
+          for (int i = 0; i < ORC_NVAL; i++) c->gp[p][i] = m->w[0].gp[p][i];
+        for (int p = 0; p < 2; p++)
+          for (int i = 0; i < ORC_NVAL; i++) c->gp[p][i] = 0.5 * (a->gp[p][i] + b->gp[p][i]);
+        for (int p = 2; p < 4; p++)
+          for (int i = 0; i < ORC_NVAL; i++)
+            c->gp[p][i] = pow(0.5 * (pow(a->gp[p][i], -0.5) + pow(b->gp[p][i], -0.5)), -2.0);
+      }
+  /* LIST cutoff: smallest function cutoff over same-species blocks (src/nprs.c:456-462) */
+  {
+    orc_basis *b0 = &m->w[0];
+    double rc = b0->gp[0][b0->gf[0][1]];
+    for (int s0 = 0; s0 < m->nspc; s0++)
+      for (int n = 0; n < m->nsym; n++) {
+        double v = m->w[2 * s0 * s0].gp[0][b0->gf[n][1]];
+        if (rc > v) rc = v;
+      }
+    m->rc_list = rc;
+  }
+  *out = m;
+  return 0;
+}
+
+/* model summary for the Python side: [nspc, D, H1, H2, nsym, ng2, ng4, nw, spcz0..3] */
+void orc_info(const orc_model *m, int *info, double *rc) {
+  info[0] = m->nspc; info[1] = m->D; info[2] = m->nu[1]; info[3] = m->nu[2];
+  info[4] = m->nsym; info[5] = m->ng2; info[6] = m->ng4; info[7] = m->nw;
+  for (int s = 0; s < ORC_MAXSPC; s++) info[8 + s] = s < m->nspc ? m->spcz[s] : 0;
+  *rc = m->rc_list;
+}
+
+/* ------------------------------------------------------------- geometry */
+/* CAST_CELL lattice/position handling, src/cmod.c:1084-1121 */
+void orc_cast(int N, int XT, const double *lat, const double *x_in, double *L, double *X) {
+  for (int i = 0; i < 3; i++)
+    for (int q = 0; q < 3; q++) L[3 * i + q] = lat[3 * i + q] * 1.0;
+  for (int i = 0; i < N; i++) {
+    for (int q = 0; q < 3; q++) X[3 * i + q] = x_in[3 * i + q];
+    if (XT == 0) {
+      double x[3];
+      for (int q = 0; q < 3; q++) {
+        x[q] = 0.0;
+        for (int q1 = 0; q1 < 3; q1++) x[q] += X[3 * i + q1] * L[3 * q1 + q];
+      }
+      for (int q = 0; q < 3; q++) X[3 * i + q] = x[q];
+    }
+  }
+}
+
+/* reciprocal vectors as rows, src/cell.c:432-444 with VectorProd/CrossProd (src/util.c:186-219) */
+void orc_reciprocal(const double *L, double *R) {
+  const double *a = L, *b = L + 3, *c = L + 6;
+  R[6] = a[1] * b[2] - a[2] * b[1];
+  R[7] = a[2] * b[0] - a[0] * b[2];
+  R[8] = a[0] * b[1] - a[1] * b[0];
+  R[0] = b[1] * c[2] - b[2] * c[1];
+  R[1] = b[2] * c[0] - b[0] * c[2];
+  R[2] = b[0] * c[1] - b[1] * c[0];
+  R[3] = c[1] * a[2] - c[2] * a[1];
+  R[4] = c[2] * a[0] - c[0] * a[2];
+  R[5] = c[0] * a[1] - c[1] * a[0];
+  double t = 1.0 / (a[0] * b[1] * c[2] + a[1] * b[2] * c[0] + a[2] * b[0] * c[1] - a[0] * b[2] * c[1] -
+                    a[1] * b[0] * c[2] - a[2] * b[1] * c[0]);
+  for (int i = 0; i < 9; i++) R[i] *= t;
+}
+
+/* JAR: Cartesian -> fractional -> wrap into [0,1) -> Cartesian, src/cell.c:446-508 */
+void orc_jar(int N, const double *L, double *X) {
+  double R[9];
+  orc_reciprocal(L, R);
+  for (int i = 0; i < N; i++) {
+    double x[3];
+    for (int q = 0; q < 3; q++) {
+      x[q] = 0.0;
+      for (int j = 0; j < 3; j++) x[q] += X[3 * i + j] * R[3 * q + j];
+    }
+    for (int q = 0; q < 3; q++) {
+      while (x[q] < 0.0) x[q] += 1.0;
+      while (x[q] >= 1.0) x[q] -= 1.0;
+    }
+    for (int q = 0; q < 3; q++) {
+      X[3 * i + q] = 0.0;
+      for (int j = 0; j < 3; j++) X[3 * i + q] += x[j] * L[3 * j + q];
+    }
+  }
+}
+
+double orc_cell_vol(const double *L) {
+  /* src/cutl.c:178-188 */
+  double V = L[0] * L[4] * L[8] + L[2] * L[3] * L[7] + L[1] * L[5] * L[6] - L[0] * L[5] * L[7] -
+             L[1] * L[3] * L[8] - L[2] * L[4] * L[6];
+  return fabs(V);
+}
+
+typedef struct {
+  int n;
+  int *j;          /* neighbour atom */
+  int *ic;         /* image, 3 ints per entry */
+  double *r2;      /* squared distance as tested */
+  double *d;       /* X_j - X_i + shift, 3 per entry (DX, src/cell.c:205-209) */
+  double *r;       /* sqrt(r2), NDX */
+} orc_row;
+
+/* Brute-force neighbour list restating LIST (src/cell.c:82-185) with O=0: scan order j, ic0, ic1, ic2,
+ * FP64 test r < Rc^2, then the nearest neighbour is swapped into slot 0. */
+static int orc_list_rows(int N, int ND, const double *L, const double *X, double Rc, int cap, orc_row *rows) {
+  double R[9];
+  int Nq[3] = {0, 0, 0};
+  double RC = Rc * Rc;
+  orc_reciprocal(L, R);
+  for (int q = 0; q < ND; q++) {
+    double a = 0.0;
+    for (int k = 0; k < 3; k++) a += R[3 * q + k] * R[3 * q + k];
+    Nq[q] = (int)ceil(Rc * sqrt(a));
+  }
+  int overflow = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : overflow)
+  for (int i = 0; i < N; i++) {
+    orc_row *w = &rows[i];
+    w->n = 0;
+    w->j = (int *)malloc(sizeof(int) * cap);
+    w->ic = (int *)malloc(sizeof(int) * 3 * cap);
+    w->r2 = (double *)malloc(sizeof(double) * cap);
+    w->d = (double *)malloc(sizeof(double) * 3 * cap);
+    w->r = (double *)malloc(sizeof(double) * cap);
+    for (int j = 0; j < N && !overflow; j++) {
+      int ic[3];
+      for (ic[0] = -Nq[0]; ic[0] <= Nq[0]; ic[0]++)
+        for (ic[1] = -Nq[1]; ic[1] <= Nq[1]; ic[1]++)
+          for (ic[2] = -Nq[2]; ic[2] <= Nq[2]; ic[2]++) {
+            if (i == j && ic[0] == 0 && ic[1] == 0 && ic[2] == 0) continue;
+            double r = 0.0;
+            for (int q = 0; q < 3; q++) {
+              double x = 0.0;
+              for (int q1 = 0; q1 < 3; q1++) x += (double)(ic[q1]) * L[3 * q1 + q];
+              x += X[3 * j + q] - X[3 * i + q];
+              r += x * x;
+            }
+            if (r < RC) {
+              if (w->n == cap) { overflow = 1; break; }
+              int k = w->n++;
+              w->j[k] = j;
+              w->r2[k] = r;
+              for (int q = 0; q < 3; q++) {
+                w->ic[3 * k + q] = ic[q];
+              }
+            }
+          }
+    }
+    /* nearest into slot 0 (src/cell.c:157-165) */
+    for (int k = 1; k < w->n; k++)
+      if (w->r2[k] < w->r2[0]) {
+        int t = w->j[k]; w->j[k] = w->j[0]; w->j[0] = t;
+        for (int q = 0; q < 3; q++) { t = w->ic[3 * k + q]; w->ic[3 * k + q] = w->ic[q]; w->ic[q] = t; }
+        double u = w->r2[k]; w->r2[k] = w->r2[0]; w->r2[0] = u;
+      }
+    for (int k = 0; k < w->n; k++) {
+      w->r[k] = sqrt(w->r2[k]);
+      for (int q = 0; q < 3; q++) {
+        double s = 0.0;
+        for (int q1 = 0; q1 < 3; q1++) s += (double)(w->ic[3 * k + q1]) * L[3 * q1 + q];
+        w->d[3 * k + q] = X[3 * w->j[k] + q] - X[3 * i + q] + s;
+      }
+    }
+  }
+  return overflow ? -1 : 0;
+}
+
+static void orc_free_rows(int N, orc_row *rows) {
+  for (int i = 0; i < N; i++) {
+    free(rows[i].j); free(rows[i].ic); free(rows[i].r2); free(rows[i].d); free(rows[i].r);
+  }
+  free(rows);
+}
+
+/* Public neighbour-set export: prepares the cell (CAST_CELL + JAR when ND>0), runs LIST and returns the
+ * per-atom counts plus, when the buffers are large enough, (j, ic0, ic1, ic2) per entry in row order. */
+int orc_neighbors(int N, int ND, int XT, const double *lat, const double *x_in, double Rc, int cap,
+                  int *counts, int *entries, long long entries_cap, double *Xw) {
+  double L[9];
+  double *X = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
+  orc_cast(N, XT, lat, x_in, L, X);
+  if (ND > 0) orc_jar(N, L, X);
+  if (Xw) memcpy(Xw, X, sizeof(double) * 3 * N);
+  orc_row *rows = (orc_row *)calloc(N > 0 ? N : 1, sizeof(orc_row));
+  int rc = orc_list_rows(N, ND, L, X, Rc, cap, rows);
+  long long o = 0;
+  for (int i = 0; i < N; i++) {
+    counts[i] = rows[i].n;
+    for (int k = 0; k < rows[i].n; k++, o++)
+      if (entries && o < entries_cap) {
+        entries[4 * o + 0] = rows[i].j[k];
+        for (int q = 0; q < 3; q++) entries[4 * o + 1 + q] = rows[i].ic[3 * k + q];
+      }
+  }
+  orc_free_rows(N, rows);
+  free(X);
+  return rc;
+}
+
+/* ----------------------------------------------------------- descriptors */
+static inline double fc_(double Rc, double r) { /* src/nprs.c:266-271 */
+  if (r > Rc) return 0.0;
+  return 0.5 * (cos(ORC_PI * r / Rc) + 1.0);
+}
+static inline double dfc_(double Rc, double r) { /* src/nprs.c:259-264 */
+  if (r > Rc) return 0.0;
+  return -0.5 * ORC_PI / Rc * sin(ORC_PI * r / Rc);
+}
+
+/* species-block offsets ind2/ind4 (src/nprs.c:484-498) */
+static void block_offsets(int ng2, int ng4, int *ind2, int *ind4) {
+  for (int i = 0; i < 20; i++) ind2[i] = ind4[i] = -1;
+  ind2[0] = 0;
+  ind2[1] = 1 * ng2 + 1 * ng4;
+  ind2[2] = 2 * ng2 + 3 * ng4;
+  ind2[3] = 3 * ng2 + 6 * ng4;
+  ind4[0] = 1 * ng2;
+  ind4[1] = 2 * ng2 + 1 * ng4;
+  ind4[2] = 2 * ng2 + 2 * ng4;
+  ind4[4] = 3 * ng2 + 3 * ng4;
+  ind4[8] = 3 * ng2 + 4 * ng4;
+  ind4[5] = 3 * ng2 + 5 * ng4;
+  ind4[9] = 4 * ng2 + 6 * ng4;
+  ind4[18] = 4 * ng2 + 7 * ng4;
+  ind4[10] = 4 * ng2 + 8 * ng4;
+  ind4[13] = 4 * ng2 + 9 * ng4;
+}
+
+typedef struct {
+  int ind2[20], ind4[20];
+  int g2first;         /* index of the first G4 function in the symbol table */
+  double powz[ORC_NVAL];
+} orc_tabs;
+
+static void make_tabs(const orc_model *m, orc_tabs *t) {
+  block_offsets(m->ng2, m->ng4, t->ind2, t->ind4);
+  const orc_basis *b0 = &m->w[0];
+  t->g2first = 0;
+  for (int n = 0; n < m->nsym; n++) if (b0->gf[n][0] == 2) t->g2first++;
+  for (int z = 0; z < b0->gt[5]; z++) t->powz[z] = pow(2.0, 1.0 - b0->gp[5][z]);
+}
+
+/* G2/G4 descriptors for atom i, restating the forward loop of PRS_BP (src/nprs.c:504-566). */
+static void descriptors_atom(const orc_model *m, const orc_tabs *t, const int *atmn, const orc_row *w, int i,
+                             double *G) {
+  const orc_basis *b0 = &m->w[0];
+  const int S = m->nspc;
+  const int g2 = t->g2first;
+  int s0 = atmn[i];
+  for (int n = 0; n < m->D; n++) G[n] = 0.0;
+  for (int jj = 0; jj < w->n; jj++) {
+    int s1 = atmn[w->j[jj]];
+    int s01 = s0 * s0 + s1 * s1;
+    const orc_basis *B01 = &m->w[s01];
+    double R01 = B01->gp[0][b0->gf[0][1]];
+    int ds1 = (s1 - s0 + S) % S;
+    double rij = w->r[jj];
+    if (!(rij < R01)) continue;
+    for (int n = 0, c2 = 0; n < m->nsym; n++)
+      if (b0->gf[n][0] == 2) {
+        double eta = B01->gp[2][b0->gf[n][3]], rs = B01->gp[1][b0->gf[n][2]];
+        G[c2 + t->ind2[ds1]] += fc_(R01, rij) * exp(-eta * pow(rij - rs, 2.0));
+        c2++;
+      }
+    if (!(g2 < m->D && rij < B01->gp[0][b0->gf[g2][1]])) continue;
+    for (int k = jj + 1; k < w->n; k++) {
+      int s2 = atmn[w->j[k]];
+      int s02 = s0 * s0 + s2 * s2, s12 = s1 * s1 + s2 * s2;
+      double R02 = m->w[s02].gp[0][b0->gf[g2][1]];
+      double R12 = m->w[s12].gp[0][b0->gf[g2][1]];
+      int ds2 = (s2 - s0 + S) % S;
+      double rik = w->r[k];
+      double dd[3], cs = 0.0, rjk2 = 0.0;
+      for (int q = 0; q < 3; q++) cs += w->d[3 * jj + q] * w->d[3 * k + q];
+      cs = cs / (rij * rik);
+      for (int q = 0; q < 3; q++) dd[q] = w->d[3 * jj + q] - w->d[3 * k + q];
+      for (int q = 0; q < 3; q++) rjk2 += dd[q] * dd[q];
+      double rjk = sqrt(rjk2);
+      if (!(rik < R02 && rjk < R12)) continue;
+      double fcijk = 2.0 * fc_(R01, rij) * fc_(R02, rik) * fc_(R12, rjk);
+      double n2[ORC_NVAL], lz[ORC_NVAL][ORC_NVAL];
+      for (int e = 0; e < b0->gt[3]; e++) {
+        double n01 = m->w[s01].gp[3][e], n02 = m->w[s02].gp[3][e], n12 = m->w[s12].gp[3][e];
+        n2[e] = fcijk * exp(-rij * rij * n01 - rik * rik * n02 - rjk * rjk * n12);
+      }
+      for (int z = 0; z < b0->gt[5]; z++)
+        for (int l = 0; l < b0->gt[6]; l++) lz[z][l] = t->powz[z] * pow(1.0 + b0->gp[6][l] * cs, b0->gp[5][z]);
+      int ds = ds1 * ds1 + ds2 * ds2;
+      for (int n = 0, c4 = 0; n < m->nsym; n++)
+        if (b0->gf[n][0] == 4) {
+          G[c4 + t->ind4[ds]] += lz[b0->gf[n][6]][b0->gf[n][7]] * n2[b0->gf[n][4]];
+          c4++;
+        }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- MLP */
+/* forward (ENE_ANN / FRC_ANN, src/nmlp.c:367-431) and the input gradient dE/dG (the chain that
+ * FRC_ANN contracts with dG/dx, src/nmlp.c:444-478, evaluated once per atom). */
+static double mlp_atom(const orc_model *m, int sp, const double *G, double *dEdG) {
+  const int D = m->D, H1 = m->nu[1], H2 = m->nu[2];
+  double x[1024], h1[256], h2[256], d2[256], d1[256];
+  for (int n = 0; n < D; n++) x[n] = (G[n] - m->rmin[(size_t)sp * D + n] * m->dr[(size_t)sp * D + n]) - 0.0;
+  const double *W0 = m->W[sp][0], *W1 = m->W[sp][1], *W2 = m->W[sp][2];
+  for (int o = 0; o < H1; o++) {
+    double b = m->B[sp][0][o];
+    for (int n = 0; n < D; n++) b += x[n] * W0[o * D + n];
+    h1[o] = tanh(b);
+  }
+  for (int o = 0; o < H2; o++) {
+    double b = m->B[sp][1][o];
+    for (int n = 0; n < H1; n++) b += h1[n] * W1[o * H1 + n];
+    h2[o] = tanh(b);
+  }
+  double e = m->B[sp][2][0];
+  for (int n = 0; n < H2; n++) e += h2[n] * W2[n];
+  if (dEdG) {
+    for (int o = 0; o < H2; o++) d2[o] = W2[o] * (1.0 - h2[o] * h2[o]);
+    for (int n = 0; n < H1; n++) {
+      double s = 0.0;
+      for (int o = 0; o < H2; o++) s += d2[o] * W1[o * H1 + n];
+      d1[n] = s * (1.0 - h1[n] * h1[n]);
+    }
+    for (int l = 0; l < D; l++) {
+      double s = 0.0;
+      for (int n = 0; n < H1; n++) s += d1[n] * W0[n * D + l];
+      dEdG[l] = s;
+    }
+  }
+  return e;
+}
+
+/* ----------------------------------------------------- forces and virial */
+/* Gradient of sum_n g_n G_n(i) with respect to the neighbour vectors of atom i.  The result is scattered as
+ * -dE/dx onto the atoms (F) and as U_ab -= (dE/dd)_a d_b into the virial (xx,yy,zz,xy,yz,zx), matching
+ * the sign and component order of Fn/Sn in PRS_BP (src/nprs.c:568-770) contracted in FRC_ANN
+ * (src/nmlp.c:433-484). */
+static void force_atom(const orc_model *m, const orc_tabs *t, const int *atmn, const orc_row *w, int i,
+                       const double *g, double *F, double *U) {
+  const orc_basis *b0 = &m->w[0];
+  const int S = m->nspc;
+  const int g2 = t->g2first;
+  int s0 = atmn[i];
+  double fi[3] = {0, 0, 0};
+#define ADDV(dst, vec, sc) do { (dst)[0] += (sc) * (vec)[0]; (dst)[1] += (sc) * (vec)[1]; (dst)[2] += (sc) * (vec)[2]; } while (0)
+#define VIR(gv, dv) do { U[0] -= (gv)[0] * (dv)[0]; U[1] -= (gv)[1] * (dv)[1]; U[2] -= (gv)[2] * (dv)[2]; \
+    U[3] -= 0.5 * ((gv)[0] * (dv)[1] + (gv)[1] * (dv)[0]); U[4] -= 0.5 * ((gv)[1] * (dv)[2] + (gv)[2] * (dv)[1]); \
+    U[5] -= 0.5 * ((gv)[2] * (dv)[0] + (gv)[0] * (dv)[2]); } while (0)
+  for (int jj = 0; jj < w->n; jj++) {
+    int j = w->j[jj];
+    int s1 = atmn[j];
+    int s01 = s0 * s0 + s1 * s1;
+    const orc_basis *B01 = &m->w[s01];
+    double R01 = B01->gp[0][b0->gf[0][1]];
+    int ds1 = (s1 - s0 + S) % S;
+    double rij = w->r[jj];
+    const double *a = &w->d[3 * jj];
+    if (!(rij < R01)) continue;
+    /* G2: dT/dr = sum_n g_n exp(-eta (r-rs)^2) (-2 eta (r-rs) fc + dfc) */
+    double dTdr = 0.0;
+    double fcij = fc_(R01, rij), dfcij = dfc_(R01, rij);
+    for (int n = 0, c2 = 0; n < m->nsym; n++)
+      if (b0->gf[n][0] == 2) {
+        double eta = B01->gp[2][b0->gf[n][3]], rs = B01->gp[1][b0->gf[n][2]];
+        double ex = exp(-eta * pow(rij - rs, 2.0));
+        dTdr += g[c2 + t->ind2[ds1]] * ex * (-2.0 * eta * (rij - rs) * fcij + dfcij);
+        c2++;
+      }
+    {
+      double ga[3]; /* dT/da */
+      for (int q = 0; q < 3; q++) ga[q] = dTdr * a[q] / rij;
+      ADDV(&F[3 * j], ga, -1.0);
+      ADDV(fi, ga, 1.0);
+      VIR(ga, a);
+    }
+    if (!(g2 < m->D && rij < B01->gp[0][b0->gf[g2][1]])) continue;
+    for (int k = jj + 1; k < w->n; k++) {
+      int kk = w->j[k];
+      int s2 = atmn[kk];
+      int s02 = s0 * s0 + s2 * s2, s12 = s1 * s1 + s2 * s2;
+      double R02 = m->w[s02].gp[0][b0->gf[g2][1]];
+      double R12 = m->w[s12].gp[0][b0->gf[g2][1]];
+      int ds2 = (s2 - s0 + S) % S;
+      double rik = w->r[k];
+      const double *b = &w->d[3 * k];
+      double c3[3], cs = 0.0, rjk2 = 0.0;
+      for (int q = 0; q < 3; q++) cs += a[q] * b[q];
+      cs = cs / (rij * rik);
+      for (int q = 0; q < 3; q++) c3[q] = b[q] - a[q];
+      for (int q = 0; q < 3; q++) rjk2 += c3[q] * c3[q];
+      double rjk = sqrt(rjk2);
+      if (!(rik < R02 && rjk < R12)) continue;
+      double f1 = fc_(R01, rij), f2 = fc_(R02, rik), f3 = fc_(R12, rjk);
+      double d1 = dfc_(R01, rij), d2 = dfc_(R02, rik), d3 = dfc_(R12, rjk);
+      double Fp = f1 * f2 * f3;
+      int ds = ds1 * ds1 + ds2 * ds2;
+      const double *gb = g + t->ind4[ds];
+      /* T = 2 sum_n g_n P_n(cos) exp(-(e01 r1^2 + e02 r2^2 + e12 r3^2)) f1 f2 f3 */
+      double X = 0, Y1 = 0, Y2 = 0, Y3 = 0, Zc = 0;
+      for (int n = 0, c4 = 0; n < m->nsym; n++)
+        if (b0->gf[n][0] == 4) {
+          int e = b0->gf[n][4], z = b0->gf[n][6], l = b0->gf[n][7];
+          double lam = b0->gp[6][l], zeta = b0->gp[5][z];
+          double u = 1.0 + lam * cs;
+          double P = t->powz[z] * pow(u, zeta);
+          double Pd = t->powz[z] * zeta * lam * pow(u, zeta - 1.0);
+          double e01 = m->w[s01].gp[3][e], e02 = m->w[s02].gp[3][e], e12 = m->w[s12].gp[3][e];
+          double Bq = 2.0 * exp(-rij * rij * e01 - rik * rik * e02 - rjk * rjk * e12);
+          double gn = gb[c4];
+          X += gn * P * Bq;
+          Y1 += gn * P * Bq * e01;
+          Y2 += gn * P * Bq * e02;
+          Y3 += gn * P * Bq * e12;
+          Zc += gn * Pd * Bq;
+          c4++;
+        }
+      double dT1 = -2.0 * rij * Fp * Y1 + d1 * f2 * f3 * X;
+      double dT2 = -2.0 * rik * Fp * Y2 + f1 * d2 * f3 * X;
+      double dT3 = -2.0 * rjk * Fp * Y3 + f1 * f2 * d3 * X;
+      double dTc = Fp * Zc;
+      double ga[3], gbv[3];
+      for (int q = 0; q < 3; q++) {
+        double dca = b[q] / (rij * rik) - cs * a[q] / (rij * rij);
+        double dcb = a[q] / (rij * rik) - cs * b[q] / (rik * rik);
+        ga[q] = dT1 * a[q] / rij - dT3 * c3[q] / rjk + dTc * dca;
+        gbv[q] = dT2 * b[q] / rik + dT3 * c3[q] / rjk + dTc * dcb;
+      }
+      ADDV(&F[3 * j], ga, -1.0);
+      ADDV(&F[3 * kk], gbv, -1.0);
+      ADDV(fi, ga, 1.0);
+      ADDV(fi, gbv, 1.0);
+      VIR(ga, a);
+      VIR(gbv, b);
+    }
+  }
+  ADDV(&F[3 * i], fi, 1.0);
+#undef ADDV
+#undef VIR
+}
+
+/* short-range repulsion CELL_UREP (src/cmod.c:61-126) with FC/dFC (src/cpot.c:4-15) */
+static double FCu(double R1, double R2, double r) {
+  if (r < R1) return 1.0;
+  if (r > R2) return 0.0;
+  return 0.5 * (1.0 + cos(ORC_PI * (r - R1) / (R2 - R1)));
+}
+static double dFCu(double R1, double R2, double r) {
+  if (r < R1 || r > R2) return 0.0;
+  return -0.5 * sin(ORC_PI * (r - R1) / (R2 - R1)) * ORC_PI / (R2 - R1);
+}
+
+/* mindist table (src/util.c:13), needed only for the UREP radius */
+static const double orc_mindist[96] = {0.0,1.0,1.0,1.040,0.773,0.584,0.499,1.0,1.0,1.0,1.0,1.266,1.114,1.0,1.0,1.0,1.0,1.0,1.0,1.610,1.366,1.123,1.0,1.0,1.0,1.0,1.0,1.0,1.0,0.895,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,0.979,1.030,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,0.984,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0,1.0};
+
+/* ------------------------------------------------------------ evaluation */
+/* Full E/F/S evaluation restating CALL_MAISE -> CAST_CELL -> CELL_FRC (src/cmod.c:153-191,1084-1224):
+ *   efs = 0: energy only (CELL_ENE), 1: energy + forces + stress.
+ *   p   : pressure in eV/A^3 (already divided by eV2GPa), used for H = E + pV when ND == 3.
+ *   urep: UREP prefactor in eV (0 disables).
+ *   thr : apply the |x| < 1e-8 -> 0 thresholding of CELL_FRC.
+ * Outputs (any may be NULL): H, E, EA[N], F[3N], U[6], G[N*D], dEdG[N*D], Xw[3N] (wrapped positions),
+ * nnb[N] (neighbour counts).  Returns 0, or -1 on neighbour overflow. */
+int orc_eval(const orc_model *m, int N, int ND, int XT, const int *atmn, const double *lat, const double *x_in,
+             double p, double urep, int efs, int thr, int cap, double *H, double *E, double *EA, double *F,
+             double *U, double *G, double *dEdG, double *Xw, int *nnb) {
+  double L[9];
+  const int D = m->D;
+  double *X = (double *)malloc(sizeof(double) * 3 * (N > 0 ? N : 1));
+  orc_cast(N, XT, lat, x_in, L, X);
+  if (ND > 0) orc_jar(N, L, X);
+  if (Xw) memcpy(Xw, X, sizeof(double) * 3 * N);
+  orc_row *rows = (orc_row *)calloc(N > 0 ? N : 1, sizeof(orc_row));
+  if (orc_list_rows(N, ND, L, X, m->rc_list, cap, rows) != 0) {
+    orc_free_rows(N, rows); free(X); return -1;
+  }
+  if (nnb) for (int i = 0; i < N; i++) nnb[i] = rows[i].n;
+  orc_tabs t;
+  make_tabs(m, &t);
+  double *Gl = (double *)malloc(sizeof(double) * (size_t)N * D + 1);
+  double *gl = (double *)malloc(sizeof(double) * (size_t)N * D + 1);
+  double *ea = (double *)malloc(sizeof(double) * (N > 0 ? N : 1));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    descriptors_atom(m, &t, atmn, &rows[i], i, Gl + (size_t)i * D);
+    ea[i] = mlp_atom(m, atmn[i], Gl + (size_t)i * D, gl + (size_t)i * D);
+  }
+  double e = 0.0;
+  for (int i = 0; i < N; i++) e += ea[i];
+  double u6[6] = {0, 0, 0, 0, 0, 0};
+  double *f = (double *)calloc((size_t)3 * (N > 0 ? N : 1), sizeof(double));
+  if (efs) {
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    double *fth = (double *)calloc((size_t)nth * 3 * N + 1, sizeof(double));
+    double *uth = (double *)calloc((size_t)nth * 6, sizeof(double));
+#pragma omp parallel num_threads(nth)
+    {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+#pragma omp for schedule(static)
+      for (int i = 0; i < N; i++)
+        force_atom(m, &t, atmn, &rows[i], i, gl + (size_t)i * D, fth + (size_t)tid * 3 * N, uth + 6 * tid);
+    }
+    for (int th = 0; th < nth; th++) {
+      for (int k = 0; k < 3 * N; k++) f[k] += fth[(size_t)th * 3 * N + k];
+      for (int q = 0; q < 6; q++) u6[q] += uth[6 * th + q];
+    }
+    free(fth); free(uth);
+  }
+  if (urep > 0.0) {
+    for (int i = 0; i < N; i++) {
+      int zi = m->spcz[atmn[i]];
+      for (int k = 0; k < rows[i].n; k++) {
+        int j = rows[i].j[k];
+        int zj = m->spcz[atmn[j]];
+        double R = orc_mindist[zi] + orc_mindist[zj];
+        double r = rows[i].r[k];
+        if (r < R) {
+          e += urep * FCu(0.0, R, r) * R / r;
+          if (efs) {
+            double tt = -2.0 * urep * R / (r * r) * (-FCu(0.0, R, r) / r + dFCu(0.0, R, r));
+            const double *d = &rows[i].d[3 * k];
+            for (int q = 0; q < 3; q++) f[3 * i + q] -= tt * d[q];
+            for (int q = 0; q < 3; q++) u6[q] += 0.5 * tt * d[q] * d[q];
+            for (int q = 0; q < 3; q++) u6[q + 3] += 0.5 * tt * d[q] * d[(q + 1) % 3];
+          }
+        }
+      }
+    }
+  }
+  if (efs && thr) {
+    for (int k = 0; k < 3 * N; k++) if (fabs(f[k]) < 1e-8) f[k] = 0.0;
+    for (int q = 0; q < 6; q++) if (fabs(u6[q]) < 1e-8) u6[q] = 0.0;
+  }
+  double h = e;
+  if (ND == 3) h += p * orc_cell_vol(L);
+  if (H) *H = h;
+  if (E) *E = e;
+  if (EA) memcpy(EA, ea, sizeof(double) * N);
+  if (F) memcpy(F, f, sizeof(double) * 3 * N);
+  if (U) memcpy(U, u6, sizeof(double) * 6);
+  if (G) memcpy(G, Gl, sizeof(double) * (size_t)N * D);
+  if (dEdG) memcpy(dEdG, gl, sizeof(double) * (size_t)N * D);
+  free(f); free(Gl); free(gl); free(ea);
+  orc_free_rows(N, rows);
+  free(X);
+  return 0;
+}
+
+/* MLP-only entry for tests: E_i and dE/dG for given descriptors */
+double orc_mlp(const orc_model *m, int sp, const double *G, double *dEdG) { return mlp_atom(m, sp, G, dEdG); }
+
+double orc_ev2gpa(void) { return ORC_EV2GPA; }
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

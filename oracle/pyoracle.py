@@ -1,0 +1,183 @@
+"""ctypes access to the FP64 CPU oracle (liboracle.so) and to the reference build (oracle/_ref/maise_ref).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by bench.py's CPU-baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_EXE = os.path.join(HERE, "_ref", "maise_ref")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -f oracle/Makefile)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.orc_load.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int]
+        L.orc_load.restype = C.c_int
+        L.orc_free.argtypes = [vp]
+        L.orc_info.argtypes = [vp, ip, dp]
+        L.orc_eval.argtypes = [vp, C.c_int, C.c_int, C.c_int, ip, dp, dp, C.c_double, C.c_double, C.c_int,
+                               C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp, ip]
+        L.orc_eval.restype = C.c_int
+        L.orc_neighbors.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, ip, ip,
+                                    C.c_longlong, dp]
+        L.orc_neighbors.restype = C.c_int
+        L.orc_mlp.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_mlp.restype = C.c_double
+        L.orc_num_threads.restype = C.c_int
+        L.orc_jar.argtypes = [C.c_int, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int)) if a is not None else None
+
+
+class OracleModel:
+    def __init__(self, path: str):
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(256)
+        if lib().orc_load(path.encode(), C.byref(self._h), err, 256) != 0:
+            raise RuntimeError(f"oracle cannot load {path}: {err.value.decode()}")
+        info = np.zeros(12, dtype=np.int32)
+        rc = C.c_double()
+        lib().orc_info(self._h, _ip(info), C.byref(rc))
+        self.nspc, self.D, self.H1, self.H2, self.nsym, self.ng2, self.ng4, self.nw = (int(v) for v in info[:8])
+        self.spcz = [int(v) for v in info[8:8 + self.nspc]]
+        self.rc = rc.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().orc_free(self._h)
+        except Exception:
+            pass
+
+    def evaluate(self, spc, lat, x, nd=3, xt=1, p=0.0, urep=0.0, forces=True, threshold=True, cap=4096,
+                 want_g=False):
+        """FP64 E/F/S of one structure.  p is in eV/A^3 (CALL_MAISE divides its GPa input by eV2GPa)."""
+        spc = np.ascontiguousarray(spc, dtype=np.int32)
+        lat = np.ascontiguousarray(lat, dtype=np.float64).reshape(9)
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        N = spc.size
+        H, E = C.c_double(), C.c_double()
+        EA = np.zeros(N)
+        F = np.zeros(3 * N)
+        U = np.zeros(6)
+        G = np.zeros(N * self.D) if want_g else None
+        g = np.zeros(N * self.D) if want_g else None
+        Xw = np.zeros(3 * N)
+        nn = np.zeros(N, dtype=np.int32)
+        rc = lib().orc_eval(self._h, N, nd, xt, _ip(spc), _dp(lat), _dp(x), p, urep, 1 if forces else 0,
+                            1 if threshold else 0, cap, C.byref(H), C.byref(E), _dp(EA), _dp(F), _dp(U), _dp(G),
+                            _dp(g), _dp(Xw), _ip(nn))
+        if rc != 0:
+            raise RuntimeError("oracle: neighbour capacity exceeded")
+        out = dict(H=H.value, E=E.value, EA=EA, F=F.reshape(N, 3), U=U, X=Xw.reshape(N, 3), nn=nn)
+        if want_g:
+            out["G"] = G.reshape(N, self.D)
+            out["dEdG"] = g.reshape(N, self.D)
+        return out
+
+    def mlp(self, sp, G):
+        G = np.ascontiguousarray(G, dtype=np.float64)
+        g = np.zeros_like(G)
+        e = lib().orc_mlp(self._h, int(sp), _dp(G), _dp(g))
+        return e, g
+
+
+def neighbors(lat, x, rc, nd=3, xt=1, cap=4096):
+    """Reference-order neighbour list: (counts[N], entries[M,4] of (j, ic0, ic1, ic2), wrapped X)."""
+    lat = np.ascontiguousarray(lat, dtype=np.float64).reshape(9)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    N = x.size // 3
+    counts = np.zeros(N, dtype=np.int32)
+    Xw = np.zeros(3 * N)
+    total_cap = N * cap
+    ent = np.zeros(4 * min(total_cap, 64 * 1024 * 1024), dtype=np.int32)
+    rcode = lib().orc_neighbors(N, nd, xt, _dp(lat), _dp(x), rc, cap, _ip(counts), _ip(ent), ent.size // 4, _dp(Xw))
+    if rcode != 0:
+        raise RuntimeError("oracle: neighbour capacity exceeded")
+    M = int(counts.sum())
+    return counts, ent[:4 * M].reshape(M, 4), Xw.reshape(N, 3)
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+# ------------------------------------------------------------------ reference build
+def have_reference() -> bool:
+    return os.path.exists(REF_EXE)
+
+
+def run_reference(model_path, spc, lat, x, nd=3, xt=1, nm=500, urep=0.0, p_gpa=0.0, repeats=1, threads=1,
+                  workdir=None, timeout=3600):
+    """Run the reference's CALL_MAISE (compiled from its sources) on one structure; parse its dump."""
+    spc = np.asarray(spc, dtype=np.int64)
+    lat = np.asarray(lat, dtype=np.float64).reshape(9)
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+    N = spc.size
+    tmp = workdir or tempfile.mkdtemp(prefix="maise_ref_")
+    mp = os.path.join(tmp, "model")
+    if os.path.abspath(model_path) != os.path.abspath(mp):
+        with open(model_path) as src, open(mp, "w") as dst:
+            dst.write(src.read())
+    with open(os.path.join(tmp, "input"), "w") as f:
+        f.write(f"{N} {nd} {xt} {nm} {float(urep)!r} {float(p_gpa)!r}\n")
+        f.write(" ".join(repr(float(v)) for v in lat) + "\n")
+        for i in range(N):
+            f.write(f"{int(spc[i])} {float(x[i, 0])!r} {float(x[i, 1])!r} {float(x[i, 2])!r}\n")
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    res = subprocess.run([REF_EXE, "input", "output", str(repeats), str(threads)], cwd=tmp, env=env,
+                         capture_output=True, text=True, timeout=timeout)
+    if res.returncode != 0:
+        raise RuntimeError(f"reference run failed ({res.returncode}): {res.stderr[-2000:]}")
+    out = {}
+    for line in res.stdout.splitlines():
+        if line.startswith("TIME"):
+            out["time"] = float(line.split()[1])
+    with open(os.path.join(tmp, "output")) as f:
+        lines = f.read().split("\n")
+    it = iter(lines)
+    for line in it:
+        if not line:
+            continue
+        tag = line.split()[0]
+        if tag in ("H", "E"):
+            out[tag] = float(line.split()[1])
+        elif tag == "EA":
+            out["EA"] = np.array([float(next(it)) for _ in range(N)])
+        elif tag == "F":
+            out["F"] = np.array([[float(v) for v in next(it).split()] for _ in range(N)])
+        elif tag == "U":
+            out["U"] = np.array([float(next(it)) for _ in range(6)])
+        elif tag == "NN":
+            out["nn"] = np.array([int(next(it)) for _ in range(N)])
+        elif tag == "NBR":
+            M = int(out["nn"].sum())
+            out["nbr"] = np.array([[int(v) for v in next(it).split()] for _ in range(M)], dtype=np.int64).reshape(M, 4)
+        elif tag == "G":
+            out["G"] = np.array([[float(v) for v in next(it).split()] for _ in range(N)])
+        elif tag == "X":
+            out["X"] = np.array([[float(v) for v in next(it).split()] for _ in range(N)])
+    return out
