@@ -1,0 +1,91 @@
+/*
+ * maise_gpu.h -- C ABI of the MI355X-native MAISE neural-network potential engine (libmaise_gpu.so).
+ *
+ * Plain C types only: host pointers and sizes, opaque handles.  Every entry point returns 0 on success and
+ * a negative code on failure; mg_last_error() gives the message of the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (MAISE v3.0.00, paths relative to its source tree):
+ *   CALL_MAISE ........................ declared in mlib.h (inc/cmod.h:30, src/cmod.c:1164-1224)
+ *   mg_model_load ..................... READ_MODEL + Build_ANN + READ_ANN + Build_PRS
+ *                                       (src/util.c:1233-1335, src/nutl.c:305-357,502-633, src/nprs.c:75-257)
+ *   mg_set_structure .................. CAST_CELL (src/cmod.c:1084-1135)
+ *   mg_eval ........................... CELL_FRC / CELL_ENE (src/cmod.c:128-191): JAR + LIST + PRS_BP +
+ *                                       FRC_ANN/ENE_ANN + CELL_UREP (src/cell.c:82-185,490-508,
+ *                                       src/nprs.c:445-773, src/nmlp.c:367-488, src/cmod.c:61-126)
+ *   mg_get_neighbors .................. the LIST arrays Nn/Ni/S (src/cell.c:96-156), as a CSR export
+ *   mg_get_descriptors ................ L->Cn after PRS_BP (src/nprs.c:504-566)
+ */
+#ifndef MAISE_GPU_H
+#define MAISE_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mg_model mg_model;
+typedef struct mg_ctx mg_ctx;
+
+/* error codes */
+#define MG_OK 0
+#define MG_ERR_ARG -1
+#define MG_ERR_MODEL -2
+#define MG_ERR_HIP -3
+#define MG_ERR_NEIGHBOR -4   /* neighbour capacity exceeded / inconsistent list */
+#define MG_ERR_UNSUPPORTED -5
+
+/* mg_eval flags */
+#define MG_EVAL_FORCES 1     /* compute forces and the virial as well as energies */
+#define MG_EVAL_KEEP_G 2     /* keep descriptors G[N][D] for mg_get_descriptors */
+
+/* MLP precision of mg_ctx_set_precision */
+#define MG_PREC_FP32 0       /* descriptors, MLP and forces in FP32; energy/virial sums in FP64 */
+#define MG_PREC_BF16 1       /* hidden-layer GEMMs in BF16 on MFMA with FP32 accumulation */
+
+const char *mg_last_error(void);
+
+/* --- model --- */
+/* Load a MAISE nn_* model file (architecture D-H1-H2-1, G2/G4 basis appended). */
+int mg_model_load(const char *path, mg_model **out);
+void mg_model_free(mg_model *m);
+/* info[0..11] = nspc, D, H1, H2, nsym, ng2, ng4, nw, spcz[0..3]; *rc = neighbour-list cutoff (A). */
+int mg_model_info(const mg_model *m, int *info, double *rc);
+
+/* --- evaluation context (one device stream, one structure or a batch of structures) --- */
+int mg_ctx_create(const mg_model *m, int device, mg_ctx **out);
+void mg_ctx_free(mg_ctx *c);
+/* UREP short-range repulsion prefactor in eV (src/cmod.c:61-126); default 0 for the native API. */
+int mg_ctx_set_urep(mg_ctx *c, double urep);
+int mg_ctx_set_precision(mg_ctx *c, int prec);
+
+/* Upload one structure (CAST_CELL semantics): N atoms, nd = 0 (cluster) or 3 (periodic), xt = 0 fractional
+ * or 1 Cartesian (A), spc[N] species indices into the model's species list, lat[9] lattice rows (A),
+ * x[3N] positions. */
+int mg_set_structure(mg_ctx *c, int N, int nd, int xt, const int *spc, const double *lat, const double *x);
+
+/* Upload a batch of S structures in CSR form: atom_off[S+1] offsets into spc/x, nd[S], lat[9S]; x Cartesian. */
+int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const double *lat, const int *spc,
+                 const double *x);
+
+/* Evaluate on the device.  All inputs already resident; results stay on the device until mg_get*. */
+int mg_eval(mg_ctx *c, int flags);
+/* Copy results of structure s (0 for a single structure): E (eV), EA[natoms], F[3*natoms] (eV/A),
+ * U[6] (eV, xx yy zz xy yz zx, U = -dE/dstrain), X[3*natoms] (wrapped positions).  NULL skips a field. */
+int mg_get(mg_ctx *c, int s, double *E, double *EA, double *F, double *U, double *X);
+/* Batch results: E[S], U[6S], F[3*Ntot] (NULL skips). */
+int mg_get_batch(mg_ctx *c, double *E, double *U, double *F);
+
+/* Neighbour list of the last evaluation: row_off[N+1]; j[M], img[3M] (image indices), M = row_off[N].
+ * Call with j == NULL to get M via *nedges only. */
+int mg_get_neighbors(mg_ctx *c, int *nedges, int *row_off, int *j, int *img);
+/* Descriptors G[N][D] of the last mg_eval with MG_EVAL_KEEP_G. */
+int mg_get_descriptors(mg_ctx *c, float *G);
+/* Timing of the last mg_eval per stage in milliseconds (device events on the context stream):
+ * t[0] neighbour build, t[1] descriptors, t[2] MLP, t[3] forces+assembly, t[4] total. */
+int mg_get_timing(mg_ctx *c, float *t);
+/* Enable (1) / disable (0) per-stage event timing (small overhead). */
+int mg_ctx_set_timing(mg_ctx *c, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,227 @@
+"""maise_amd -- MI355X-native evaluation of MAISE neural-network potentials.
+
+Python mirror of the C ABI in include/maise_gpu.h (libmaise_gpu.so, loaded with ctypes).  The library is the
+product: every call here goes to the HIP kernels; there is no CPU fallback, and importing the bindings fails
+loudly when the library has not been built (python -m maise_amd.build, or __graft_entry__.build()).
+
+    m = Model("models/nn_Cu_d3_v1")                 # READ_MODEL + READ_ANN + Build_PRS
+    c = Context(m)                                  # one device stream
+    c.set_structure(spc, lat, x, nd=3, xt=1)         # CAST_CELL
+    r = c.evaluate()                                # CELL_FRC: {"E", "EA", "F", "U", "X"}
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmaise_gpu.so")
+
+MG_EVAL_FORCES = 1
+MG_EVAL_KEEP_G = 2
+MG_PREC_FP32 = 0
+MG_PREC_BF16 = 1
+EV2GPA = 160.2176565
+
+_lib = None
+
+
+class MaiseError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmaise_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MaiseError(f"{LIB_PATH} is missing: build it with `python -m maise_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp, fp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float)
+        sig = {
+            "mg_last_error": ([], C.c_char_p),
+            "mg_model_load": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+            "mg_model_free": ([vp], None),
+            "mg_model_info": ([vp, ip, dp], C.c_int),
+            "mg_ctx_create": ([vp, C.c_int, C.POINTER(vp)], C.c_int),
+            "mg_ctx_free": ([vp], None),
+            "mg_ctx_set_urep": ([vp, C.c_double], C.c_int),
+            "mg_ctx_set_precision": ([vp, C.c_int], C.c_int),
+            "mg_ctx_set_timing": ([vp, C.c_int], C.c_int),
+            "mg_set_structure": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp], C.c_int),
+            "mg_set_batch": ([vp, C.c_int, ip, ip, dp, ip, dp], C.c_int),
+            "mg_eval": ([vp, C.c_int], C.c_int),
+            "mg_get": ([vp, C.c_int, dp, dp, dp, dp, dp], C.c_int),
+            "mg_get_batch": ([vp, dp, dp, dp], C.c_int),
+            "mg_get_neighbors": ([vp, ip, ip, ip, ip], C.c_int),
+            "mg_get_descriptors": ([vp, fp], C.c_int),
+            "mg_get_timing": ([vp, fp], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().mg_last_error()
+        raise MaiseError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Model:
+    """An nn_* model loaded by the GPU library (format of src/nutl.c:394-498)."""
+
+    def __init__(self, path: str):
+        self._h = C.c_void_p()
+        _check(lib().mg_model_load(os.fsencode(path), C.byref(self._h)), f"loading model {path}")
+        info = np.zeros(12, dtype=np.int32)
+        rc = C.c_double()
+        _check(lib().mg_model_info(self._h, _ip(info), C.byref(rc)), "mg_model_info")
+        self.nspc, self.D, self.H1, self.H2, self.nsym, self.ng2, self.ng4, self.nw = (int(v) for v in info[:8])
+        self.spcz = [int(v) for v in info[8:8 + self.nspc]]
+        self.rc = rc.value
+        self.path = path
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().mg_model_free(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Context:
+    """One evaluation context: device buffers and one HIP stream; a single structure or a batch."""
+
+    def __init__(self, model: Model, device: int = 0, urep: float = 0.0, precision: int = MG_PREC_FP32):
+        self.model = model
+        self._h = C.c_void_p()
+        _check(lib().mg_ctx_create(model._h, device, C.byref(self._h)), "mg_ctx_create")
+        self.set_urep(urep)
+        _check(lib().mg_ctx_set_precision(self._h, precision), "mg_ctx_set_precision")
+        self.N = 0
+        self.S = 0
+        self._atom_off = None
+
+    def close(self):
+        if self._h:
+            lib().mg_ctx_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_urep(self, urep: float):
+        _check(lib().mg_ctx_set_urep(self._h, float(urep)), "mg_ctx_set_urep")
+
+    def set_precision(self, prec: int):
+        _check(lib().mg_ctx_set_precision(self._h, int(prec)), "mg_ctx_set_precision")
+
+    def set_timing(self, on: bool):
+        _check(lib().mg_ctx_set_timing(self._h, 1 if on else 0), "mg_ctx_set_timing")
+
+    def set_structure(self, spc, lat, x, nd: int = 3, xt: int = 1):
+        spc = np.ascontiguousarray(spc, dtype=np.int32)
+        lat = np.ascontiguousarray(lat, dtype=np.float64).reshape(9)
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if x.size != 3 * spc.size:
+            raise ValueError("x must hold 3 coordinates per atom")
+        _check(lib().mg_set_structure(self._h, spc.size, nd, xt, _ip(spc), _dp(lat), _dp(x)), "mg_set_structure")
+        self.N, self.S = spc.size, 1
+        self._atom_off = np.array([0, spc.size], dtype=np.int32)
+
+    def set_batch(self, structures):
+        """structures: list of (spc, lat, x_cartesian[, nd]) tuples."""
+        offs, nds, lats, spcs, xs = [0], [], [], [], []
+        for s in structures:
+            spc, lat, x = s[0], s[1], s[2]
+            nd = s[3] if len(s) > 3 else 3
+            spcs.append(np.asarray(spc, dtype=np.int32))
+            lats.append(np.asarray(lat, dtype=np.float64).reshape(9))
+            xs.append(np.asarray(x, dtype=np.float64).reshape(-1))
+            nds.append(nd)
+            offs.append(offs[-1] + len(spc))
+        off = np.array(offs, dtype=np.int32)
+        nd = np.array(nds, dtype=np.int32)
+        lat = np.ascontiguousarray(np.concatenate(lats))
+        spc = np.ascontiguousarray(np.concatenate(spcs))
+        x = np.ascontiguousarray(np.concatenate(xs))
+        _check(lib().mg_set_batch(self._h, len(structures), _ip(off), _ip(nd), _dp(lat), _ip(spc), _dp(x)), "mg_set_batch")
+        self.N, self.S = int(off[-1]), len(structures)
+        self._atom_off = off
+
+    def eval(self, forces: bool = True, keep_g: bool = False):
+        flags = (MG_EVAL_FORCES if forces else 0) | (MG_EVAL_KEEP_G if keep_g else 0)
+        _check(lib().mg_eval(self._h, flags), "mg_eval")
+
+    def result(self, s: int = 0, forces: bool = True) -> dict:
+        n = int(self._atom_off[s + 1] - self._atom_off[s])
+        E = C.c_double()
+        EA = np.zeros(n)
+        X = np.zeros(3 * n)
+        F = np.zeros(3 * n) if forces else None
+        U = np.zeros(6) if forces else None
+        _check(lib().mg_get(self._h, s, C.byref(E), _dp(EA), _dp(F), _dp(U), _dp(X)), "mg_get")
+        out = {"E": E.value, "EA": EA, "X": X.reshape(n, 3)}
+        if forces:
+            out["F"] = F.reshape(n, 3)
+            out["U"] = U
+        return out
+
+    def evaluate(self, forces: bool = True) -> dict:
+        self.eval(forces=forces)
+        return self.result(0, forces=forces)
+
+    def batch_result(self, forces: bool = True) -> dict:
+        E = np.zeros(self.S)
+        U = np.zeros(6 * self.S) if forces else None
+        F = np.zeros(3 * self.N) if forces else None
+        _check(lib().mg_get_batch(self._h, _dp(E), _dp(U), _dp(F)), "mg_get_batch")
+        out = {"E": E}
+        if forces:
+            out["U"] = U.reshape(self.S, 6)
+            out["F"] = F.reshape(self.N, 3)
+        return out
+
+    def neighbors(self):
+        """(row_off[N+1], j[M], img[M,3]) of the last evaluation; rows sorted by (species, j, image)."""
+        M = C.c_int()
+        _check(lib().mg_get_neighbors(self._h, C.byref(M), None, None, None), "mg_get_neighbors")
+        row = np.zeros(self.N + 1, dtype=np.int32)
+        j = np.zeros(max(M.value, 1), dtype=np.int32)
+        img = np.zeros(3 * max(M.value, 1), dtype=np.int32)
+        _check(lib().mg_get_neighbors(self._h, C.byref(M), _ip(row), _ip(j), _ip(img)), "mg_get_neighbors")
+        return row, j[:M.value], img[:3 * M.value].reshape(-1, 3)
+
+    def descriptors(self) -> np.ndarray:
+        G = np.zeros(self.N * self.model.D, dtype=np.float32)
+        _check(lib().mg_get_descriptors(self._h, G.ctypes.data_as(C.POINTER(C.c_float))), "mg_get_descriptors")
+        return G.reshape(self.N, self.model.D)
+
+    def timing(self) -> np.ndarray:
+        t = np.zeros(5, dtype=np.float32)
+        _check(lib().mg_get_timing(self._h, t.ctypes.data_as(C.POINTER(C.c_float))), "mg_get_timing")
+        return t
+
+
+def call_maise_symbol():
+    """The exported drop-in entry point (for ABI tests)."""
+    return lib().CALL_MAISE

@@ -1,0 +1,66 @@
+"""In-tree build of libmaise_gpu.so (hipcc for the gfx950 kernels, gcc for the C host code).
+
+Objects go to maise_amd/_build/, the shared library to maise_amd/libmaise_gpu.so.  Every call rebuilds.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libmaise_gpu.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ARCH = os.environ.get("MAISE_GPU_ARCH", "gfx950")
+
+HIPCC_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
+               # packed FP32 instructions are disabled for all device code
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-Wno-unused-command-line-argument"]
+CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+            f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
+
+HIP_SOURCES = ["mg_kernels.hip"]
+C_SOURCES = ["mg_model.c", "mg_api.c"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:3])} ...")
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    for src in C_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".c", ".o"))
+        _run(["gcc", *CC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lm"], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> None:
+    """Build the CPU oracle and, where the reference tree is present, its reference build (test only)."""
+    _run(["make", "-s", "-f", os.path.join(ROOT, "oracle", "Makefile"), "all"], verbose)
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    build_oracle(verbose=True)
+    print("built", LIB)

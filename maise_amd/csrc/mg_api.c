@@ -1,0 +1,519 @@
+/*
+ * mg_api.c -- host side of libmaise_gpu: evaluation contexts, structure upload (CAST_CELL semantics),
+ * the kernel chain of one evaluation, result download, and the CALL_MAISE drop-in.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mg_host.h"
+#include "../../include/mlib.h"
+
+/* ------------------------------------------------------------------ errors */
+static __thread char g_err[512];
+
+int mg_fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+const char *mg_last_error(void) { return g_err; }
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+/* ------------------------------------------------------------------ context */
+enum { EV_START, EV_NBR, EV_DESC, EV_MLP, EV_END, EV_N };
+
+struct mg_ctx {
+  const mg_model *m;
+  int device;
+  hipStream_t st;
+  double urep;
+  int prec;
+  int timing;
+  hipEvent_t ev[EV_N];
+  float tms[5];
+  /* structures */
+  int S, N;
+  int *h_atom_off, *h_nd;
+  double *h_lat;
+  mg_struct_dev *h_str;
+  int nbins;
+  /* device */
+  mg_basis_dev *d_basis;
+  float *d_mlp;
+  mg_struct_dev *d_str;
+  int *d_atom_str, *d_spc;
+  double *d_xin, *d_x;
+  int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur, *d_bin_atoms;
+  int *d_cnt, *d_rowcnt, *d_row_off, *d_misc; /* misc: [0] maxrow, [1] error flag */
+  void *d_key_u, *d_vec_u, *d_key, *d_vec;
+  int *d_rev;
+  float *d_G, *d_dEdG, *d_Ei, *d_erep, *d_vir;
+  void *d_fself, *d_fe;
+  double *d_F, *d_E, *d_U;
+  void *d_tmp;
+  size_t tmp_bytes;
+  /* capacities (elements) */
+  size_t cap_atoms, cap_str, cap_bins, cap_edges;
+  int nedges, maxrow;
+  int have_forces, have_g;
+};
+
+static int dev_alloc(void **p, size_t bytes) {
+  if (*p) hipFree(*p);
+  *p = NULL;
+  if (bytes == 0) bytes = 16;
+  HIPCHK(hipMalloc(p, bytes));
+  return MG_OK;
+}
+
+#define ALLOC(ptr, bytes) do { int r_ = dev_alloc((void **)&(ptr), (bytes)); if (r_) return r_; } while (0)
+
+static int ensure_atoms(mg_ctx *c, size_t N) {
+  if (N <= c->cap_atoms) return MG_OK;
+  size_t n = N + N / 4 + 16;
+  const int D = c->m->D;
+  ALLOC(c->d_atom_str, n * sizeof(int));
+  ALLOC(c->d_spc, n * sizeof(int));
+  ALLOC(c->d_xin, n * 3 * sizeof(double));
+  ALLOC(c->d_x, n * 3 * sizeof(double));
+  ALLOC(c->d_bin, n * sizeof(int));
+  ALLOC(c->d_bin_atoms, n * sizeof(int));
+  ALLOC(c->d_cnt, n * 4 * sizeof(int));
+  ALLOC(c->d_rowcnt, n * sizeof(int));
+  ALLOC(c->d_row_off, (n + 1) * sizeof(int));
+  ALLOC(c->d_G, n * D * sizeof(float));
+  ALLOC(c->d_dEdG, n * D * sizeof(float));
+  ALLOC(c->d_Ei, n * sizeof(float));
+  ALLOC(c->d_erep, n * sizeof(float));
+  ALLOC(c->d_vir, n * 6 * sizeof(float));
+  ALLOC(c->d_fself, n * 16);
+  ALLOC(c->d_F, n * 3 * sizeof(double));
+  c->cap_atoms = n;
+  /* scan scratch for the larger of the atom and bin scans */
+  return MG_OK;
+}
+
+static int ensure_scan_tmp(mg_ctx *c, int n) {
+  size_t need = 0;
+  if (mgk_scan(NULL, NULL, n, NULL, &need, c->st) != 0) return mg_fail(MG_ERR_HIP, "scan size query failed");
+  if (need > c->tmp_bytes) {
+    ALLOC(c->d_tmp, need);
+    c->tmp_bytes = need;
+  }
+  return MG_OK;
+}
+
+static int ensure_str(mg_ctx *c, size_t S) {
+  if (S <= c->cap_str) return MG_OK;
+  size_t n = S + 4;
+  ALLOC(c->d_str, n * sizeof(mg_struct_dev));
+  ALLOC(c->d_E, n * sizeof(double));
+  ALLOC(c->d_U, n * 6 * sizeof(double));
+  c->h_str = (mg_struct_dev *)realloc(c->h_str, n * sizeof(mg_struct_dev));
+  c->h_atom_off = (int *)realloc(c->h_atom_off, (n + 1) * sizeof(int));
+  c->h_nd = (int *)realloc(c->h_nd, n * sizeof(int));
+  c->h_lat = (double *)realloc(c->h_lat, n * 9 * sizeof(double));
+  c->cap_str = n;
+  return MG_OK;
+}
+
+static int ensure_bins(mg_ctx *c, size_t nb) {
+  if (nb <= c->cap_bins) return MG_OK;
+  size_t n = nb + nb / 4 + 16;
+  ALLOC(c->d_bin_count, (n + 1) * sizeof(int));
+  ALLOC(c->d_bin_start, (n + 1) * sizeof(int));
+  ALLOC(c->d_bin_cur, (n + 1) * sizeof(int));
+  c->cap_bins = n;
+  return MG_OK;
+}
+
+static int ensure_edges(mg_ctx *c, size_t M) {
+  if (M <= c->cap_edges) return MG_OK;
+  size_t n = M + M / 4 + 64;
+  ALLOC(c->d_key_u, n * 8);
+  ALLOC(c->d_vec_u, n * 16);
+  ALLOC(c->d_key, n * 8);
+  ALLOC(c->d_vec, n * 16);
+  ALLOC(c->d_rev, n * sizeof(int));
+  ALLOC(c->d_fe, n * 16);
+  c->cap_edges = n;
+  return MG_OK;
+}
+
+int mg_ctx_create(const mg_model *m, int device, mg_ctx **out) {
+  if (!m || !out) return mg_fail(MG_ERR_ARG, "mg_ctx_create: null argument");
+  *out = NULL;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return mg_fail(MG_ERR_ARG, "device %d not present (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  mg_ctx *c = (mg_ctx *)calloc(1, sizeof(mg_ctx));
+  c->m = m;
+  c->device = device;
+  if (hipStreamCreate(&c->st) != hipSuccess) { free(c); return mg_fail(MG_ERR_HIP, "hipStreamCreate failed"); }
+  for (int k = 0; k < EV_N; k++) hipEventCreate(&c->ev[k]);
+  int r;
+  if ((r = dev_alloc((void **)&c->d_basis, sizeof(mg_basis_dev))) ||
+      (r = dev_alloc((void **)&c->d_mlp, (size_t)m->wstride * m->nspc * sizeof(float))) ||
+      (r = dev_alloc((void **)&c->d_misc, 16 * sizeof(int)))) {
+    mg_ctx_free(c);
+    return r;
+  }
+  HIPCHK(hipMemcpy(c->d_basis, &m->basis, sizeof(mg_basis_dev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_mlp, m->mlp, (size_t)m->wstride * m->nspc * sizeof(float), hipMemcpyHostToDevice));
+  *out = c;
+  return MG_OK;
+}
+
+void mg_ctx_free(mg_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  void *ptrs[] = {c->d_basis, c->d_mlp, c->d_str, c->d_atom_str, c->d_spc, c->d_xin, c->d_x, c->d_bin, c->d_bin_count,
+                  c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
+                  c->d_key_u, c->d_vec_u, c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
+                  c->d_vir, c->d_fself, c->d_fe, c->d_F, c->d_E, c->d_U, c->d_tmp};
+  for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
+    if (ptrs[k]) hipFree(ptrs[k]);
+  for (int k = 0; k < EV_N; k++) hipEventDestroy(c->ev[k]);
+  if (c->st) hipStreamDestroy(c->st);
+  free(c->h_str); free(c->h_atom_off); free(c->h_nd); free(c->h_lat);
+  free(c);
+}
+
+int mg_ctx_set_urep(mg_ctx *c, double urep) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  c->urep = urep;
+  return MG_OK;
+}
+
+int mg_ctx_set_precision(mg_ctx *c, int prec) {
+  if (!c || (prec != MG_PREC_FP32 && prec != MG_PREC_BF16)) return mg_fail(MG_ERR_ARG, "bad precision");
+  c->prec = prec;
+  return MG_OK;
+}
+
+int mg_ctx_set_timing(mg_ctx *c, int on) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  c->timing = on;
+  return MG_OK;
+}
+
+/* reciprocal rows R = (L1xL2, L2xL0, L0xL1) / (L0.(L1xL2)) with the reference's operation order
+ * (src/cell.c:432-444, src/util.c:186-219); compiled without FMA contraction */
+static void reciprocal(const double *L, double *R) {
+  const double *a = L, *b = L + 3, *c = L + 6;
+  R[6] = a[1] * b[2] - a[2] * b[1];
+  R[7] = a[2] * b[0] - a[0] * b[2];
+  R[8] = a[0] * b[1] - a[1] * b[0];
+  R[0] = b[1] * c[2] - b[2] * c[1];
+  R[1] = b[2] * c[0] - b[0] * c[2];
+  R[2] = b[0] * c[1] - b[1] * c[0];
+  R[3] = c[1] * a[2] - c[2] * a[1];
+  R[4] = c[2] * a[0] - c[0] * a[2];
+  R[5] = c[0] * a[1] - c[1] * a[0];
+  double t = 1.0 / (a[0] * b[1] * c[2] + a[1] * b[2] * c[0] + a[2] * b[0] * c[1] - a[0] * b[2] * c[1] -
+                    a[1] * b[0] * c[2] - a[2] * b[1] * c[0]);
+  for (int i = 0; i < 9; i++) R[i] *= t;
+}
+
+static double cell_volume(const double *L) { /* src/cutl.c:178-188 */
+  double V = L[0] * L[4] * L[8] + L[2] * L[3] * L[7] + L[1] * L[5] * L[6] - L[0] * L[5] * L[7] - L[1] * L[3] * L[8] -
+             L[2] * L[4] * L[6];
+  return fabs(V);
+}
+
+/* bins and stencil of one structure: bins at least rc thick (perpendicular to each lattice plane), stencil
+ * half width K_q >= rc |R_q| nb_q so every pair within rc lies in the stencil. */
+static void plan_bins(mg_struct_dev *s, double rc, int natom) {
+  const double eps = 1e-6;
+  if (s->nd == 0) {
+    s->nb[0] = s->nb[1] = s->nb[2] = 1;
+    s->K[0] = s->K[1] = s->K[2] = 0;
+  } else {
+    double h[3];
+    for (int q = 0; q < 3; q++) {
+      double a = 0.0;
+      for (int k = 0; k < 3; k++) a += s->R[3 * q + k] * s->R[3 * q + k];
+      h[q] = rc * sqrt(a) * (1.0 + eps); /* rc in units of the lattice-plane spacing */
+      double nb = floor(1.0 / h[q]);
+      s->nb[q] = nb < 1.0 ? 1 : (nb > 1024.0 ? 1024 : (int)nb);
+    }
+    /* keep the bin count O(natom) for sparse cells */
+    while ((double)s->nb[0] * s->nb[1] * s->nb[2] > 4.0 * natom + 64) {
+      int q = 0;
+      for (int k = 1; k < 3; k++) if (s->nb[k] > s->nb[q]) q = k;
+      s->nb[q] = (s->nb[q] + 1) / 2;
+    }
+    for (int q = 0; q < 3; q++) {
+      int K = (int)ceil(h[q] * s->nb[q]);
+      s->K[q] = K < 1 ? 1 : K;
+    }
+  }
+  s->nbins = s->nb[0] * s->nb[1] * s->nb[2];
+}
+
+static int upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *xt, const double *lat, const int *spc,
+                  const double *x) {
+  HIPCHK(hipSetDevice(c->device));
+  int N = atom_off[S];
+  for (int i = 0; i < N; i++)
+    if (spc[i] < 0 || spc[i] >= c->m->nspc) return mg_fail(MG_ERR_ARG, "atom %d: species %d outside the model's %d", i, spc[i], c->m->nspc);
+  int r;
+  if ((r = ensure_str(c, S)) || (r = ensure_atoms(c, N))) return r;
+  int *atom_str = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+  int bin0 = 0;
+  const double rc = c->m->rc_list;
+  for (int s = 0; s < S; s++) {
+    mg_struct_dev *d = &c->h_str[s];
+    memset(d, 0, sizeof *d);
+    for (int k = 0; k < 9; k++) d->L[k] = lat[9 * s + k] * 1.0;
+    reciprocal(d->L, d->R);
+    d->rc2 = rc * rc;
+    d->atom0 = atom_off[s];
+    d->natom = atom_off[s + 1] - atom_off[s];
+    d->nd = nd[s];
+    d->xt = xt[s];
+    if (d->nd != 0 && d->nd != 3) { free(atom_str); return mg_fail(MG_ERR_ARG, "structure %d: ND must be 0 or 3", s); }
+    plan_bins(d, rc, d->natom);
+    d->bin0 = bin0;
+    bin0 += d->nbins;
+    for (int i = d->atom0; i < d->atom0 + d->natom; i++) atom_str[i] = s;
+    c->h_nd[s] = nd[s];
+    memcpy(c->h_lat + 9 * s, d->L, 9 * sizeof(double));
+    c->h_atom_off[s] = atom_off[s];
+  }
+  c->h_atom_off[S] = N;
+  c->nbins = bin0;
+  if ((r = ensure_bins(c, bin0))) { free(atom_str); return r; }
+  int nscan = N > bin0 ? N : bin0;
+  if ((r = ensure_scan_tmp(c, nscan))) { free(atom_str); return r; }
+  HIPCHK(hipMemcpyAsync(c->d_str, c->h_str, S * sizeof(mg_struct_dev), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_atom_str, atom_str, N * sizeof(int), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_spc, spc, N * sizeof(int), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_xin, x, 3 * N * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  free(atom_str);
+  c->S = S;
+  c->N = N;
+  c->have_forces = c->have_g = 0;
+  return MG_OK;
+}
+
+int mg_set_structure(mg_ctx *c, int N, int nd, int xt, const int *spc, const double *lat, const double *x) {
+  if (!c || N < 0 || (N > 0 && (!spc || !x)) || !lat) return mg_fail(MG_ERR_ARG, "mg_set_structure: bad argument");
+  int off[2] = {0, N};
+  return upload(c, 1, off, &nd, &xt, lat, spc, x);
+}
+
+int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const double *lat, const int *spc, const double *x) {
+  if (!c || S < 1 || !atom_off || !nd || !lat || !spc || !x) return mg_fail(MG_ERR_ARG, "mg_set_batch: bad argument");
+  int *xt = (int *)malloc(sizeof(int) * S);
+  for (int s = 0; s < S; s++) xt[s] = 1;
+  int r = upload(c, S, atom_off, nd, xt, lat, spc, x);
+  free(xt);
+  return r;
+}
+
+/* ------------------------------------------------------------------ evaluation */
+#define KCHK(x) do { if ((x) != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, hipGetErrorString(hipGetLastError())); } while (0)
+
+int mg_eval(mg_ctx *c, int flags) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  if (c->S == 0) return mg_fail(MG_ERR_ARG, "no structure set");
+  HIPCHK(hipSetDevice(c->device));
+  const mg_model *m = c->m;
+  const int N = c->N;
+  hipStream_t st = c->st;
+  const int forces = (flags & MG_EVAL_FORCES) != 0;
+  const float urep = (float)c->urep;
+  if (c->prec != MG_PREC_FP32) return mg_fail(MG_ERR_UNSUPPORTED, "BF16 precision is not available in this build");
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_START], st));
+  /* neighbour list */
+  HIPCHK(hipMemsetAsync(c->d_bin_count, 0, (c->nbins + 1) * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(c->d_bin_cur, 0, (c->nbins + 1) * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(c->d_misc, 0, 16 * sizeof(int), st));
+  KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, st));
+  KCHK(mgk_scan(c->d_bin_count, c->d_bin_start, c->nbins, c->d_tmp, &c->tmp_bytes, st));
+  KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, st));
+  KCHK(mgk_nbr_count(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_atoms, c->d_cnt,
+                     c->d_rowcnt, c->d_misc, st));
+  KCHK(mgk_scan(c->d_rowcnt, c->d_row_off, N, c->d_tmp, &c->tmp_bytes, st));
+  int hm[2];
+  HIPCHK(hipMemcpyAsync(&hm[0], c->d_row_off + N, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&hm[1], c->d_misc, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  c->nedges = hm[0];
+  c->maxrow = hm[1];
+  int r;
+  if ((r = ensure_edges(c, (size_t)c->nedges))) return r;
+  KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_atoms, c->d_row_off,
+                    c->d_key_u, c->d_vec_u, c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, c->maxrow, st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
+  /* descriptors, MLP, forces */
+  KCHK(mgk_desc(N, m->nspc, m->basis.ne, c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
+                urep, c->maxrow, st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_DESC], st));
+  KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_MLP], st));
+  if (forces)
+    KCHK(mgk_force(N, m->nspc, m->basis.ne, c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
+                   c->d_fe, c->d_vir, urep, c->maxrow, c->d_rev, c->d_F, st));
+  KCHK(mgk_reduce(c->S, c->d_str, c->d_Ei, c->d_erep, c->d_vir, forces, c->d_E, c->d_U, st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_END], st));
+  int err = 0;
+  HIPCHK(hipMemcpyAsync(&err, c->d_misc + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (err) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)");
+  if (c->timing) {
+    hipEventElapsedTime(&c->tms[0], c->ev[EV_START], c->ev[EV_NBR]);
+    hipEventElapsedTime(&c->tms[1], c->ev[EV_NBR], c->ev[EV_DESC]);
+    hipEventElapsedTime(&c->tms[2], c->ev[EV_DESC], c->ev[EV_MLP]);
+    hipEventElapsedTime(&c->tms[3], c->ev[EV_MLP], c->ev[EV_END]);
+    hipEventElapsedTime(&c->tms[4], c->ev[EV_START], c->ev[EV_END]);
+  }
+  c->have_forces = forces;
+  c->have_g = 1;
+  return MG_OK;
+}
+
+int mg_get(mg_ctx *c, int s, double *E, double *EA, double *F, double *U, double *X) {
+  if (!c || s < 0 || s >= c->S) return mg_fail(MG_ERR_ARG, "mg_get: bad structure index");
+  HIPCHK(hipSetDevice(c->device));
+  const int a0 = c->h_atom_off[s], n = c->h_atom_off[s + 1] - a0;
+  if (E) HIPCHK(hipMemcpy(E, c->d_E + s, sizeof(double), hipMemcpyDeviceToHost));
+  if (U) {
+    if (!c->have_forces) return mg_fail(MG_ERR_ARG, "mg_get: virial requested but the last eval had no forces");
+    HIPCHK(hipMemcpy(U, c->d_U + 6 * s, 6 * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (F) {
+    if (!c->have_forces) return mg_fail(MG_ERR_ARG, "mg_get: forces requested but the last eval had no forces");
+    HIPCHK(hipMemcpy(F, c->d_F + 3 * (size_t)a0, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (X) HIPCHK(hipMemcpy(X, c->d_x + 3 * (size_t)a0, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  if (EA && n > 0) {
+    float *tmp = (float *)malloc(sizeof(float) * n);
+    hipError_t e = hipMemcpy(tmp, c->d_Ei + a0, sizeof(float) * n, hipMemcpyDeviceToHost);
+    for (int i = 0; i < n; i++) EA[i] = tmp[i];
+    free(tmp);
+    if (e != hipSuccess) return mg_fail(MG_ERR_HIP, "mg_get: %s", hipGetErrorString(e));
+  }
+  return MG_OK;
+}
+
+int mg_get_batch(mg_ctx *c, double *E, double *U, double *F) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  if (E) HIPCHK(hipMemcpy(E, c->d_E, c->S * sizeof(double), hipMemcpyDeviceToHost));
+  if ((U || F) && !c->have_forces) return mg_fail(MG_ERR_ARG, "mg_get_batch: last eval had no forces");
+  if (U) HIPCHK(hipMemcpy(U, c->d_U, 6 * c->S * sizeof(double), hipMemcpyDeviceToHost));
+  if (F) HIPCHK(hipMemcpy(F, c->d_F, 3 * (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost));
+  return MG_OK;
+}
+
+int mg_get_neighbors(mg_ctx *c, int *nedges, int *row_off, int *j, int *img) {
+  if (!c || !nedges) return mg_fail(MG_ERR_ARG, "mg_get_neighbors: bad argument");
+  HIPCHK(hipSetDevice(c->device));
+  *nedges = c->nedges;
+  if (row_off) HIPCHK(hipMemcpy(row_off, c->d_row_off, (c->N + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  if (j || img) {
+    unsigned long long *k = (unsigned long long *)malloc(8 * (size_t)(c->nedges > 0 ? c->nedges : 1));
+    hipError_t e = hipMemcpy(k, c->d_key, 8 * (size_t)c->nedges, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { free(k); return mg_fail(MG_ERR_HIP, "%s", hipGetErrorString(e)); }
+    for (int t = 0; t < c->nedges; t++) {
+      if (j) j[t] = (int)((k[t] >> 30) & 0x3fffffffull);
+      if (img) {
+        unsigned p = (unsigned)(k[t] & 0x3fffffffull);
+        img[3 * t] = (int)(p & 1023u) - MG_IMG_OFF;
+        img[3 * t + 1] = (int)((p >> 10) & 1023u) - MG_IMG_OFF;
+        img[3 * t + 2] = (int)((p >> 20) & 1023u) - MG_IMG_OFF;
+      }
+    }
+    free(k);
+  }
+  return MG_OK;
+}
+
+int mg_get_descriptors(mg_ctx *c, float *G) {
+  if (!c || !G) return mg_fail(MG_ERR_ARG, "mg_get_descriptors: bad argument");
+  if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_get_descriptors: no evaluation yet");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(G, c->d_G, sizeof(float) * (size_t)c->N * c->m->D, hipMemcpyDeviceToHost));
+  return MG_OK;
+}
+
+int mg_get_timing(mg_ctx *c, float *t) {
+  if (!c || !t) return mg_fail(MG_ERR_ARG, "mg_get_timing: bad argument");
+  for (int k = 0; k < 5; k++) t[k] = c->tms[k];
+  return MG_OK;
+}
+
+/* ------------------------------------------------------------------ CALL_MAISE drop-in */
+typedef struct {
+  mg_model *m;
+  mg_ctx *c;
+} cm_state;
+
+static void cm_die(const char *what) {
+  fprintf(stderr, "%s: %s\n", what, mg_last_error());
+  exit(1);
+}
+
+double CALL_MAISE(ANN *R, PRS *P, PRS *W, LNK *L, Cell *C, int CODE, int N, int NM, int ND, int NP, int XT, int *ATMN,
+                  double *LAT, double *X, double *FRC, double *STR) {
+  (void)R; (void)P; (void)W; (void)CODE; (void)NP;
+  const double eV2GPa = 160.2176565;
+  C->p /= eV2GPa; /* every call, as src/cmod.c:1181 */
+  cm_state *s = L->B != 0 ? (cm_state *)L->storage_[0] : NULL;
+  if (!s) {
+    s = (cm_state *)calloc(1, sizeof(cm_state));
+    const char *path = getenv("MAISE_MODEL");
+    if (!path || !*path) path = "model";
+    if (mg_model_load(path, &s->m) != MG_OK) cm_die("ERROR reading model");
+    if (mg_ctx_create(s->m, 0, &s->c) != MG_OK) cm_die("ERROR creating GPU context");
+    const char *u = getenv("MAISE_UREP");
+    mg_ctx_set_urep(s->c, (u && *u) ? atof(u) : 100.0);
+    L->storage_[0] = s;
+    L->B = 1;
+  }
+  if (mg_set_structure(s->c, N, ND, XT, ATMN, LAT, X) != MG_OK) cm_die("CALL_MAISE");
+  if (mg_eval(s->c, MG_EVAL_FORCES) != MG_OK) cm_die("CALL_MAISE");
+  const char *strict = getenv("MAISE_NM_STRICT");
+  if (strict && *strict == '1' && s->c->maxrow >= NM) {
+    fprintf(stderr, "Increase 'Max number of nearest neighbors' (%d needed)\n", s->c->maxrow + 1);
+    exit(1);
+  }
+  double E, U[6];
+  if (mg_get(s->c, 0, &E, NULL, FRC, U, NULL) != MG_OK) cm_die("CALL_MAISE");
+  for (int k = 0; k < 3 * N; k++) if (fabs(FRC[k]) < 1e-8) FRC[k] = 0.0;
+  for (int q = 0; q < 6; q++) STR[q] = fabs(U[q]) < 1e-8 ? 0.0 : U[q];
+  double H = E;
+  if (ND == 3) {
+    double Lr[9];
+    for (int k = 0; k < 9; k++) Lr[k] = LAT[k] * 1.0;
+    H += C->p * cell_volume(Lr);
+  }
+  return H;
+}
+
+void maise_gpu_free(LNK *L) {
+  if (!L) return;
+  cm_state *s = (cm_state *)L->storage_[0];
+  if (s) {
+    mg_ctx_free(s->c);
+    mg_model_free(s->m);
+    free(s);
+  }
+  L->storage_[0] = NULL;
+  L->B = 0;
+}

@@ -191,3 +191,19 @@ def write_model(path: str, net: Network, basis: Basis, model_id: str = "5EED0000
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
         f.write(basis.text())
+
+
+def read_basis(path: str) -> Basis:
+    """Read the basis block (after the ``  B2A`` header) of a model or basis file, unscaled."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    at = next(i for i, l in enumerate(lines) if l.startswith("  B2A"))
+    b2a, scale = (float(v) for v in lines[at + 1].split()[:2])
+    rc_row = next(i for i in range(at + 5, len(lines)) if lines[i].startswith(" Rc  "))
+    nsym = rc_row - (at + 5) - 1
+    funcs = [tuple(int(v) for v in lines[at + 5 + n].split()[1:9]) for n in range(nsym)]
+    params = {}
+    for k in range(7):
+        toks = lines[rc_row + k].split()
+        params[toks[0]] = [float(v) for v in toks[2:2 + int(toks[1])]]
+    return Basis(funcs=funcs, params=params, b2a=b2a, scale=scale)
