@@ -80,10 +80,28 @@ int mg_get_neighbors(mg_ctx *c, int *nedges, int *row_off, int *j, int *img);
 /* Descriptors G[N][D] of the last mg_eval with MG_EVAL_KEEP_G. */
 int mg_get_descriptors(mg_ctx *c, float *G);
 /* Timing of the last mg_eval per stage in milliseconds (device events on the context stream):
- * t[0] neighbour build, t[1] descriptors, t[2] MLP, t[3] forces+assembly, t[4] total. */
+ * t[0] neighbour build, t[1] descriptor kernel, t[2] MLP, t[3] force/virial kernel, t[4] force gather + FP64
+ * reductions, t[5] total.  t must hold 6 floats. */
 int mg_get_timing(mg_ctx *c, float *t);
 /* Enable (1) / disable (0) per-stage event timing (small overhead). */
 int mg_ctx_set_timing(mg_ctx *c, int on);
+
+/* --- device-resident dynamics (state stays on the device between steps) --- */
+/* NVT molecular dynamics, replacing Dynamics/Maxwell (src/cmod.c:224-549): velocity Verlet with a Nose-Hoover
+ * thermostat xi' = (2K/(3N kB T) - 1)/tau^2 (the reference's equation of motion, src/cmod.c:403-407).
+ * mg_md_init draws seeded Maxwell-Boltzmann velocities at T (K), removes the centre-of-mass drift and rescales
+ * to exactly T.  mg_md_run advances nsteps of dt (fs) at thermostat temperature T with time constant tau (fs;
+ * tau <= 0 gives NVE); epot[nsteps] and ekin[nsteps] (eV, NULL skips) receive the energies after each step. */
+int mg_md_init(mg_ctx *c, double T, unsigned long long seed);
+int mg_md_set_velocities(mg_ctx *c, const double *v);
+int mg_md_get_velocities(mg_ctx *c, double *v);
+int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *epot, double *ekin);
+/* Batched FIRE relaxation of every structure of the context (positions only), replacing the CELL_RELX /
+ * CELL_MIN caller (src/cmod.c:901-977, src/cmin.c:177-301; the reference minimiser is GSL BFGS2).  A structure
+ * stops when max_i |F_i| < fmax (eV/A); dt0/dtmax are FIRE's initial and maximum step, maxstep (A) caps the
+ * per-structure displacement norm of one step.  steps[S] and done[S] (NULL skips) report iterations and
+ * convergence; the final positions/energies/forces are read with mg_get / mg_get_batch. */
+int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,11 @@ def lib():
             "mg_get_neighbors": ([vp, ip, ip, ip, ip], C.c_int),
             "mg_get_descriptors": ([vp, fp], C.c_int),
             "mg_get_timing": ([vp, fp], C.c_int),
+            "mg_md_init": ([vp, C.c_double, C.c_ulonglong], C.c_int),
+            "mg_md_set_velocities": ([vp, dp], C.c_int),
+            "mg_md_get_velocities": ([vp, dp], C.c_int),
+            "mg_md_run": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp], C.c_int),
+            "mg_relax": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, ip, ip], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -217,9 +222,39 @@ class Context:
         return G.reshape(self.N, self.model.D)
 
     def timing(self) -> np.ndarray:
-        t = np.zeros(5, dtype=np.float32)
+        t = np.zeros(6, dtype=np.float32)
         _check(lib().mg_get_timing(self._h, t.ctypes.data_as(C.POINTER(C.c_float))), "mg_get_timing")
         return t
+
+
+    # ------------------------------------------------------------- dynamics
+    def md_init(self, T: float, seed: int = 1):
+        """Seeded Maxwell-Boltzmann velocities at T (K), zero centre-of-mass momentum, exact T."""
+        _check(lib().mg_md_init(self._h, float(T), int(seed)), "mg_md_init")
+
+    def md_set_velocities(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        _check(lib().mg_md_set_velocities(self._h, _dp(v)), "mg_md_set_velocities")
+
+    def md_velocities(self) -> np.ndarray:
+        v = np.zeros(3 * self.N)
+        _check(lib().mg_md_get_velocities(self._h, _dp(v)), "mg_md_get_velocities")
+        return v.reshape(self.N, 3)
+
+    def md_run(self, nsteps: int, dt: float = 1.0, T: float = 300.0, tau: float = 100.0):
+        """NVT (tau > 0) or NVE (tau <= 0) velocity Verlet on the device; returns per-step (Epot, Ekin) in eV."""
+        ep, ek = np.zeros(max(nsteps, 1)), np.zeros(max(nsteps, 1))
+        _check(lib().mg_md_run(self._h, int(nsteps), float(dt), float(T), float(tau), _dp(ep), _dp(ek)), "mg_md_run")
+        return ep[:nsteps], ek[:nsteps]
+
+    def relax(self, max_steps: int = 1000, fmax: float = 0.01, dt0: float = 0.1, dtmax: float = 1.0,
+              maxstep: float = 0.2):
+        """Batched FIRE over all structures of the context; returns (steps[S], converged[S])."""
+        steps = np.zeros(self.S, dtype=np.int32)
+        done = np.zeros(self.S, dtype=np.int32)
+        _check(lib().mg_relax(self._h, int(max_steps), float(fmax), float(dt0), float(dtmax), float(maxstep),
+                              _ip(steps), _ip(done)), "mg_relax")
+        return steps, done.astype(bool)
 
 
 def call_maise_symbol():

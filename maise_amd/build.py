@@ -23,8 +23,8 @@ HIPCC_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-munsafe
 CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
             f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["mg_kernels.hip"]
-C_SOURCES = ["mg_model.c", "mg_api.c"]
+HIP_SOURCES = ["mg_kernels.hip", "mg_dyn.hip"]
+C_SOURCES = ["mg_model.c", "mg_api.c", "mg_dyn.c"]
 
 
 def _run(cmd, verbose):
@@ -42,7 +42,7 @@ def build(verbose: bool = False) -> str:
     objs = []
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     for src in HIP_SOURCES:
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        obj = os.path.join(BUILD, src.replace(".hip", "_hip.o"))
         _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
     for src in C_SOURCES:

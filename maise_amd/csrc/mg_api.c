@@ -24,52 +24,9 @@ int mg_fail(int code, const char *fmt, ...) {
 
 const char *mg_last_error(void) { return g_err; }
 
-#define HIPCHK(x)                                                                   \
-  do {                                                                              \
-    hipError_t e_ = (x);                                                            \
-    if (e_ != hipSuccess) return mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
-  } while (0)
 
 /* ------------------------------------------------------------------ context */
-enum { EV_START, EV_NBR, EV_DESC, EV_MLP, EV_END, EV_N };
-
-struct mg_ctx {
-  const mg_model *m;
-  int device;
-  hipStream_t st;
-  double urep;
-  int prec;
-  int timing;
-  hipEvent_t ev[EV_N];
-  float tms[5];
-  /* structures */
-  int S, N;
-  int *h_atom_off, *h_nd;
-  double *h_lat;
-  mg_struct_dev *h_str;
-  int nbins;
-  /* device */
-  mg_basis_dev *d_basis;
-  float *d_mlp;
-  mg_struct_dev *d_str;
-  int *d_atom_str, *d_spc;
-  double *d_xin, *d_x;
-  int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur, *d_bin_atoms;
-  int *d_cnt, *d_rowcnt, *d_row_off, *d_misc; /* misc: [0] maxrow, [1] error flag */
-  void *d_key_u, *d_vec_u, *d_key, *d_vec;
-  int *d_rev;
-  float *d_G, *d_dEdG, *d_Ei, *d_erep, *d_vir;
-  void *d_fself, *d_fe;
-  double *d_F, *d_E, *d_U;
-  void *d_tmp;
-  size_t tmp_bytes;
-  /* capacities (elements) */
-  size_t cap_atoms, cap_str, cap_bins, cap_edges;
-  int nedges, maxrow;
-  int have_forces, have_g;
-};
-
-static int dev_alloc(void **p, size_t bytes) {
+int mg_dev_alloc(void **p, size_t bytes) {
   if (*p) hipFree(*p);
   *p = NULL;
   if (bytes == 0) bytes = 16;
@@ -77,7 +34,7 @@ static int dev_alloc(void **p, size_t bytes) {
   return MG_OK;
 }
 
-#define ALLOC(ptr, bytes) do { int r_ = dev_alloc((void **)&(ptr), (bytes)); if (r_) return r_; } while (0)
+#define ALLOC(ptr, bytes) do { int r_ = mg_dev_alloc((void **)&(ptr), (bytes)); if (r_) return r_; } while (0)
 
 static int ensure_atoms(mg_ctx *c, size_t N) {
   if (N <= c->cap_atoms) return MG_OK;
@@ -164,9 +121,9 @@ int mg_ctx_create(const mg_model *m, int device, mg_ctx **out) {
   if (hipStreamCreate(&c->st) != hipSuccess) { free(c); return mg_fail(MG_ERR_HIP, "hipStreamCreate failed"); }
   for (int k = 0; k < EV_N; k++) hipEventCreate(&c->ev[k]);
   int r;
-  if ((r = dev_alloc((void **)&c->d_basis, sizeof(mg_basis_dev))) ||
-      (r = dev_alloc((void **)&c->d_mlp, (size_t)m->wstride * m->nspc * sizeof(float))) ||
-      (r = dev_alloc((void **)&c->d_misc, 16 * sizeof(int)))) {
+  if ((r = mg_dev_alloc((void **)&c->d_basis, sizeof(mg_basis_dev))) ||
+      (r = mg_dev_alloc((void **)&c->d_mlp, (size_t)m->wstride * m->nspc * sizeof(float))) ||
+      (r = mg_dev_alloc((void **)&c->d_misc, 16 * sizeof(int)))) {
     mg_ctx_free(c);
     return r;
   }
@@ -182,7 +139,8 @@ void mg_ctx_free(mg_ctx *c) {
   void *ptrs[] = {c->d_basis, c->d_mlp, c->d_str, c->d_atom_str, c->d_spc, c->d_xin, c->d_x, c->d_bin, c->d_bin_count,
                   c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
                   c->d_key_u, c->d_vec_u, c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
-                  c->d_vir, c->d_fself, c->d_fe, c->d_F, c->d_E, c->d_U, c->d_tmp};
+                  c->d_vir, c->d_fself, c->d_fe, c->d_F, c->d_E, c->d_U, c->d_tmp,
+                  c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
     if (ptrs[k]) hipFree(ptrs[k]);
   for (int k = 0; k < EV_N; k++) hipEventDestroy(c->ev[k]);
@@ -361,14 +319,19 @@ int mg_eval(mg_ctx *c, int flags) {
                     c->d_key_u, c->d_vec_u, c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, c->maxrow, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
   /* descriptors, MLP, forces */
-  KCHK(mgk_desc(N, m->nspc, m->basis.ne, c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
+  KCHK(mgk_desc(N, m->nspc, m->pat, c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
                 urep, c->maxrow, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_DESC], st));
   KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_MLP], st));
-  if (forces)
-    KCHK(mgk_force(N, m->nspc, m->basis.ne, c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
-                   c->d_fe, c->d_vir, urep, c->maxrow, c->d_rev, c->d_F, st));
+  if (forces) {
+    KCHK(mgk_force(N, m->nspc, m->pat, c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
+                   c->d_fe, c->d_vir, urep, c->maxrow, st));
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
+    KCHK(mgk_gather(N, c->d_row_off, c->d_rev, c->d_fself, c->d_fe, c->d_F, st));
+  } else if (c->timing) {
+    HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
+  }
   KCHK(mgk_reduce(c->S, c->d_str, c->d_Ei, c->d_erep, c->d_vir, forces, c->d_E, c->d_U, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_END], st));
   int err = 0;
@@ -379,8 +342,9 @@ int mg_eval(mg_ctx *c, int flags) {
     hipEventElapsedTime(&c->tms[0], c->ev[EV_START], c->ev[EV_NBR]);
     hipEventElapsedTime(&c->tms[1], c->ev[EV_NBR], c->ev[EV_DESC]);
     hipEventElapsedTime(&c->tms[2], c->ev[EV_DESC], c->ev[EV_MLP]);
-    hipEventElapsedTime(&c->tms[3], c->ev[EV_MLP], c->ev[EV_END]);
-    hipEventElapsedTime(&c->tms[4], c->ev[EV_START], c->ev[EV_END]);
+    hipEventElapsedTime(&c->tms[3], c->ev[EV_MLP], c->ev[EV_FORCE]);
+    hipEventElapsedTime(&c->tms[4], c->ev[EV_FORCE], c->ev[EV_END]);
+    hipEventElapsedTime(&c->tms[5], c->ev[EV_START], c->ev[EV_END]);
   }
   c->have_forces = forces;
   c->have_g = 1;
@@ -454,7 +418,7 @@ int mg_get_descriptors(mg_ctx *c, float *G) {
 
 int mg_get_timing(mg_ctx *c, float *t) {
   if (!c || !t) return mg_fail(MG_ERR_ARG, "mg_get_timing: bad argument");
-  for (int k = 0; k < 5; k++) t[k] = c->tms[k];
+  for (int k = 0; k < 6; k++) t[k] = c->tms[k];
   return MG_OK;
 }
 

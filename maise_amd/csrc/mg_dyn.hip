@@ -1,0 +1,277 @@
+// mg_dyn.hip -- device-resident dynamics: NVT molecular dynamics (velocity Verlet with a Nose-Hoover
+// thermostat) and batched FIRE relaxation.  State (positions, velocities, thermostat, FIRE scalars) never
+// leaves the device between steps; the host only sequences launches on the context stream.
+//
+// Reference callers replaced: Dynamics/Maxwell/CELL_KIN (src/cmod.c:12-23,224-280,283-549) and the relaxation
+// driver CELL_RELX/CELL_MIN (src/cmod.c:901-977, src/cmin.c:177-301).  The reference integrates NVT with one
+// force call per step (x += v dt + f dt^2/2, v += f dt, f = F/m - xi v, xi += dt (2K/(3N kT) - 1)/tau^2) and
+// relaxes with GSL BFGS2; this build uses velocity Verlet with the same thermostat equation of motion
+// (half-step splitting) and FIRE (Bitzek et al., PRL 97, 170201 (2006)).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mg_internal.h"
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  // splitmix64 finaliser (Steele, Lea, Flood 2014)
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t h) { return ((h >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+
+// Maxwell-Boltzmann velocities: v_q = sqrt(kT/m) * N(0,1) by Box-Muller on a counter-based hash
+__global__ void k_md_maxwell(int N, unsigned long long seed, const double* __restrict__ mass, double kT,
+                             double* __restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double sd = sqrt(kT / mass[i]);
+  for (int q = 0; q < 3; q++) {
+    uint64_t h1 = mix64(seed ^ mix64(2ull * (3ull * i + q)));
+    uint64_t h2 = mix64(seed ^ mix64(2ull * (3ull * i + q) + 1ull));
+    double r = sqrt(-2.0 * log(u01(h1)));
+    v[3 * i + q] = sd * r * cos(6.283185307179586 * u01(h2));
+  }
+}
+
+// one block: remove the centre-of-mass velocity and rescale to exactly the target temperature
+__global__ __launch_bounds__(1024) void k_md_fix(int N, const double* __restrict__ mass, double* __restrict__ v,
+                                                 double kT, double ndof) {
+  __shared__ double sh[4][1024];
+  const int t = threadIdx.x;
+  double p[4] = {0, 0, 0, 0};
+  for (int i = t; i < N; i += 1024) {
+    p[0] += mass[i] * v[3 * i]; p[1] += mass[i] * v[3 * i + 1]; p[2] += mass[i] * v[3 * i + 2]; p[3] += mass[i];
+  }
+  for (int k = 0; k < 4; k++) sh[k][t] = p[k];
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) for (int k = 0; k < 4; k++) sh[k][t] += sh[k][t + w];
+    __syncthreads();
+  }
+  const double vx = sh[0][0] / sh[3][0], vy = sh[1][0] / sh[3][0], vz = sh[2][0] / sh[3][0];
+  __syncthreads();
+  double k2 = 0.0;
+  for (int i = t; i < N; i += 1024) {
+    v[3 * i] -= vx; v[3 * i + 1] -= vy; v[3 * i + 2] -= vz;
+    k2 += mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+  }
+  sh[0][t] = k2;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) sh[0][t] += sh[0][t + w];
+    __syncthreads();
+  }
+  const double s = sh[0][0] > 0.0 ? sqrt(ndof * kT / sh[0][0]) : 0.0;
+  for (int i = t; i < N; i += 1024) { v[3 * i] *= s; v[3 * i + 1] *= s; v[3 * i + 2] *= s; }
+}
+
+// one block: kinetic energy K, half-step thermostat update xi += dt/2 (2K/(g kT) - 1)/tau^2 (skipped when
+// tau <= 0), velocity scale s = exp(-xi dt/2).  dyn = {xi, K, s}; trace[slot] = K when slot >= 0.
+__global__ __launch_bounds__(1024) void k_md_thermo(int N, const double* __restrict__ mass, const double* __restrict__ v,
+                                                    double* __restrict__ dyn, double dt, double kT, double ndof,
+                                                    double tau, double* __restrict__ trace, int slot) {
+  __shared__ double sh[1024];
+  const int t = threadIdx.x;
+  double k2 = 0.0;
+  for (int i = t; i < N; i += 1024)
+    k2 += mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+  sh[t] = k2;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) sh[t] += sh[t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    double K = 0.5 * sh[0];
+    double xi = dyn[0];
+    if (tau > 0.0) xi += 0.5 * dt * (2.0 * K / (ndof * kT) - 1.0) / (tau * tau);
+    dyn[0] = xi;
+    dyn[1] = K;
+    dyn[2] = tau > 0.0 ? exp(-0.5 * dt * xi) : 1.0;
+    if (trace && slot >= 0) trace[slot] = K;
+  }
+}
+
+// v <- v s + dt/2 F/m ; x_next <- x + dt v
+__global__ void k_md_kick_drift(int N, double dt, const double* __restrict__ mass, const double* __restrict__ F,
+                                double* __restrict__ v, const double* __restrict__ x, double* __restrict__ xin,
+                                const double* __restrict__ dyn) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double s = dyn[2], h = 0.5 * dt / mass[i];
+  for (int q = 0; q < 3; q++) {
+    double vv = v[3 * i + q] * s + h * F[3 * i + q];
+    v[3 * i + q] = vv;
+    xin[3 * i + q] = x[3 * i + q] + dt * vv;
+  }
+}
+
+// v <- (v + dt/2 F/m) exp(-xi dt/2)
+__global__ void k_md_kick_scale(int N, double dt, const double* __restrict__ mass, const double* __restrict__ F,
+                                double* __restrict__ v, const double* __restrict__ dyn, double tau) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double s = tau > 0.0 ? exp(-0.5 * dt * dyn[0]) : 1.0, h = 0.5 * dt / mass[i];
+  for (int q = 0; q < 3; q++) v[3 * i + q] = (v[3 * i + q] + h * F[3 * i + q]) * s;
+}
+
+__global__ void k_copy_d(const double* __restrict__ src, double* __restrict__ dst, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// FIRE, one block per structure.  state[s] = {dt, alpha, npos, done, steps}.  Unit masses, as in the FIRE
+// paper's formulation for relaxation; the step dr = dt v is capped at maxstep (A) per structure.
+#define FIRE_NMIN 5
+#define FIRE_FINC 1.1
+#define FIRE_FDEC 0.5
+#define FIRE_ASTART 0.1
+#define FIRE_FA 0.99
+__global__ __launch_bounds__(256) void k_fire(int S, const mg_struct_dev* __restrict__ str, const double* __restrict__ F,
+                                              double* __restrict__ v, const double* __restrict__ x,
+                                              double* __restrict__ xin, double* __restrict__ state, double fmax,
+                                              double dtmax, double maxstep) {
+  __shared__ double sh[4][256];
+  const int s = blockIdx.x, t = threadIdx.x;
+  const int a0 = str[s].atom0, n = str[s].natom;
+  double* st = state + 5 * s;
+  if (st[3] != 0.0) {  // converged earlier: keep still
+    for (int i = a0 + t; i < a0 + n; i += 256)
+      for (int q = 0; q < 3; q++) { xin[3 * i + q] = x[3 * i + q]; v[3 * i + q] = 0.0; }
+    return;
+  }
+  double vf = 0, ff = 0, vv = 0, fm = 0;
+  for (int i = a0 + t; i < a0 + n; i += 256) {
+    double f2 = 0;
+    for (int q = 0; q < 3; q++) {
+      double f = F[3 * i + q], u = v[3 * i + q];
+      vf += f * u; f2 += f * f; vv += u * u;
+    }
+    ff += f2;
+    fm = f2 > fm ? f2 : fm;
+  }
+  sh[0][t] = vf; sh[1][t] = ff; sh[2][t] = vv; sh[3][t] = fm;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[0][t] += sh[0][t + w]; sh[1][t] += sh[1][t + w]; sh[2][t] += sh[2][t + w];
+      sh[3][t] = sh[3][t] > sh[3][t + w] ? sh[3][t] : sh[3][t + w];
+    }
+    __syncthreads();
+  }
+  vf = sh[0][0]; ff = sh[1][0]; vv = sh[2][0]; fm = sh[3][0];
+  __syncthreads();
+  if (fm < fmax * fmax) {
+    for (int i = a0 + t; i < a0 + n; i += 256)
+      for (int q = 0; q < 3; q++) { xin[3 * i + q] = x[3 * i + q]; v[3 * i + q] = 0.0; }
+    if (t == 0) st[3] = 1.0;
+    return;
+  }
+  double dt = st[0], alpha = st[1], npos = st[2];
+  bool reset = false;
+  double c1 = 1.0, c2 = 0.0;
+  if (vf > 0.0) {
+    c1 = 1.0 - alpha;
+    c2 = ff > 0.0 ? alpha * sqrt(vv / ff) : 0.0;
+    if (npos > FIRE_NMIN) {
+      dt = fmin(dt * FIRE_FINC, dtmax);
+      alpha *= FIRE_FA;
+    }
+    npos += 1.0;
+  } else {
+    reset = true;
+    alpha = FIRE_ASTART;
+    dt *= FIRE_FDEC;
+    npos = 0.0;
+  }
+  double dr2 = 0.0;
+  for (int i = a0 + t; i < a0 + n; i += 256)
+    for (int q = 0; q < 3; q++) {
+      double u = reset ? 0.0 : c1 * v[3 * i + q] + c2 * F[3 * i + q];
+      u += dt * F[3 * i + q];
+      v[3 * i + q] = u;
+      dr2 += (dt * u) * (dt * u);
+    }
+  sh[0][t] = dr2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sh[0][t] += sh[0][t + w];
+    __syncthreads();
+  }
+  const double nrm = sqrt(sh[0][0]);
+  const double sc = nrm > maxstep ? maxstep / nrm : 1.0;
+  for (int i = a0 + t; i < a0 + n; i += 256)
+    for (int q = 0; q < 3; q++) xin[3 * i + q] = x[3 * i + q] + sc * dt * v[3 * i + q];
+  if (t == 0) {
+    st[0] = dt; st[1] = alpha; st[2] = npos; st[4] += 1.0;
+  }
+}
+
+__global__ void k_fire_init(int S, double* __restrict__ state, double dt0) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double* st = state + 5 * s;
+  st[0] = dt0; st[1] = FIRE_ASTART; st[2] = 0.0; st[3] = 0.0; st[4] = 0.0;
+}
+
+__global__ void k_fire_count(int S, const double* __restrict__ state, int* __restrict__ out) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < S && state[5 * s + 3] != 0.0) atomicAdd(out, 1);
+}
+
+extern "C" {
+
+int mgk_md_maxwell(int N, unsigned long long seed, const double* mass, double kT, double* v, hipStream_t st) {
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_md_maxwell, dim3((N + 255) / 256), dim3(256), 0, st, N, seed, mass, kT, v);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_md_fix(int N, const double* mass, double* v, double kT, double ndof, hipStream_t st) {
+  hipLaunchKernelGGL(k_md_fix, dim3(1), dim3(1024), 0, st, N, mass, v, kT, ndof);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_md_thermo(int N, const double* mass, const double* v, double* dyn, double dt, double kT, double ndof,
+                  double tau, double* trace, int slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_md_thermo, dim3(1), dim3(1024), 0, st, N, mass, v, dyn, dt, kT, ndof, tau, trace, slot);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_md_kick_drift(int N, double dt, const double* mass, const double* F, double* v, const double* x, double* xin,
+                      const double* dyn, hipStream_t st) {
+  hipLaunchKernelGGL(k_md_kick_drift, dim3((N + 255) / 256), dim3(256), 0, st, N, dt, mass, F, v, x, xin, dyn);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_md_kick_scale(int N, double dt, const double* mass, const double* F, double* v, const double* dyn, double tau,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_md_kick_scale, dim3((N + 255) / 256), dim3(256), 0, st, N, dt, mass, F, v, dyn, tau);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_copy_d(const double* src, double* dst, int n, hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_copy_d, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_fire_init(int S, double* state, double dt0, hipStream_t st) {
+  hipLaunchKernelGGL(k_fire_init, dim3((S + 255) / 256), dim3(256), 0, st, S, state, dt0);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_fire(int S, const mg_struct_dev* str, const double* F, double* v, const double* x, double* xin, double* state,
+             double fmax, double dtmax, double maxstep, hipStream_t st) {
+  hipLaunchKernelGGL(k_fire, dim3(S), dim3(256), 0, st, S, str, F, v, x, xin, state, fmax, dtmax, maxstep);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_fire_count(int S, const double* state, int* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_fire_count, dim3((S + 255) / 256), dim3(256), 0, st, S, state, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // extern "C"

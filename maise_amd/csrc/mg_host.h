@@ -19,9 +19,63 @@ struct mg_model {
   float *mlp;           /* per species: W0' [H1][D], b0, W1 [H2][H1], b1, W2 [H2], b2, xoff [D] */
   double *mlp64;        /* same layout in FP64 */
   int wstride;          /* floats per species block (multiple of 4) */
+  int pat;              /* G4 kernel specialisation: 0 generic, 1 43-function, 2 22-function table */
 };
 
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+enum { EV_START, EV_NBR, EV_DESC, EV_MLP, EV_FORCE, EV_END, EV_N };
+
+struct mg_ctx {
+  const mg_model *m;
+  int device;
+  hipStream_t st;
+  double urep;
+  int prec;
+  int timing;
+  hipEvent_t ev[EV_N];
+  float tms[6];
+  /* structures */
+  int S, N;
+  int *h_atom_off, *h_nd;
+  double *h_lat;
+  mg_struct_dev *h_str;
+  int nbins;
+  /* device */
+  mg_basis_dev *d_basis;
+  float *d_mlp;
+  mg_struct_dev *d_str;
+  int *d_atom_str, *d_spc;
+  double *d_xin, *d_x;
+  int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur, *d_bin_atoms;
+  int *d_cnt, *d_rowcnt, *d_row_off, *d_misc; /* misc: [0] maxrow, [1] error flag */
+  void *d_key_u, *d_vec_u, *d_key, *d_vec;
+  int *d_rev;
+  float *d_G, *d_dEdG, *d_Ei, *d_erep, *d_vir;
+  void *d_fself, *d_fe;
+  double *d_F, *d_E, *d_U;
+  void *d_tmp;
+  size_t tmp_bytes;
+  /* capacities (elements) */
+  size_t cap_atoms, cap_str, cap_bins, cap_edges;
+  int nedges, maxrow;
+  int have_forces, have_g;
+  /* dynamics state (mg_dyn.c) */
+  double *d_v;          /* velocities [3N] (A/fs) */
+  double *d_mass;       /* per-atom mass [N] (eV fs^2 / A^2) */
+  double *d_dyn;        /* scalars: [0] Nose-Hoover xi, [1] kinetic energy, ... */
+  double *d_fire;       /* per-structure FIRE state [S][4]: dt, alpha, npos, done */
+  double *d_trace;      /* per-step energies */
+  size_t cap_dyn_atoms, cap_dyn_str, cap_trace;
+};
+
+
 int mg_fail(int code, const char *fmt, ...);
+int mg_dev_alloc(void **p, size_t bytes);
 
 /* kernel launchers (mg_kernels.hip) */
 int mgk_scan(const int *in, int *out, int n, void *tmp, size_t *tmp_bytes, hipStream_t st);
@@ -33,13 +87,14 @@ int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, c
 int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
                  const int *bin_start, const int *bin_atoms, const int *row_off, void *key_u, void *vec_u, void *key,
                  void *vec, int *rev, int *err, int maxrow, hipStream_t st);
-int mgk_desc(int N, int S, int ne, const int *spc, const int *row_off, const int *cnt, const void *vec,
+int mgk_desc(int N, int S, int pat, const int *spc, const int *row_off, const int *cnt, const void *vec,
              const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, hipStream_t st);
 int mgk_mlp(int N, const int *spc, const float *G, const float *W, int wstride, int D, int H1, int H2, float *Ei,
             float *dEdG, hipStream_t st);
-int mgk_force(int N, int S, int ne, const int *spc, const int *row_off, const int *cnt, const void *vec,
+int mgk_force(int N, int S, int pat, const int *spc, const int *row_off, const int *cnt, const void *vec,
               const mg_basis_dev *bp, const float *dEdG, void *fself, void *fe, float *vir, float urep, int maxrow,
-              const int *rev, double *F, hipStream_t st);
+              hipStream_t st);
+int mgk_gather(int N, const int *row_off, const int *rev, const void *fself, const void *fe, double *F, hipStream_t st);
 int mgk_reduce(int nstr, const mg_struct_dev *str, const float *Ei, const float *erep, const float *vir, int have_vir,
                double *E, double *U, hipStream_t st);
 

@@ -303,7 +303,11 @@ __device__ __forceinline__ float sinpi_(float t) { return __builtin_amdgcn_sinf(
 
 struct RowInfo {
   int off, M, si;
-  int m[MG_MAXSPC], o[MG_MAXSPC];
+  int m0, m1, m2, m3;  // neighbours of each species (rows are sorted by species)
+  __device__ __forceinline__ int m(int s) const { return s == 0 ? m0 : (s == 1 ? m1 : (s == 2 ? m2 : m3)); }
+  __device__ __forceinline__ int o(int s) const {
+    return s == 0 ? 0 : (s == 1 ? m0 : (s == 2 ? m0 + m1 : m0 + m1 + m2));
+  }
 };
 
 __device__ __forceinline__ void row_info(RowInfo& ri, int i, const int* __restrict__ spc, const int* __restrict__ row_off,
@@ -311,13 +315,10 @@ __device__ __forceinline__ void row_info(RowInfo& ri, int i, const int* __restri
   ri.off = row_off[i];
   ri.M = row_off[i + 1] - ri.off;
   ri.si = spc[i];
-  int o = 0;
-#pragma unroll
-  for (int s = 0; s < MG_MAXSPC; s++) {
-    ri.m[s] = s < S ? cnt[4 * i + s] : 0;
-    ri.o[s] = o;
-    o += ri.m[s];
-  }
+  ri.m0 = cnt[4 * i];
+  ri.m1 = S > 1 ? cnt[4 * i + 1] : 0;
+  ri.m2 = S > 2 ? cnt[4 * i + 2] : 0;
+  ri.m3 = S > 3 ? cnt[4 * i + 3] : 0;
 }
 
 // stage the row of atom i into LDS
@@ -328,7 +329,7 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
     float4 v = vec[ri.off + t];
     int s = 0;
 #pragma unroll
-    for (int q = 1; q < MG_MAXSPC; q++) s += (q < S && t >= ri.o[q]) ? 1 : 0;
+    for (int q = 1; q < MG_MAXSPC; q++) s += (q < S && t >= ri.o(q)) ? 1 : 0;
     int p = ri.si * MG_MAXSPC + s;
     float rc = bp->rc[p];
     float tt = v.w / rc;
@@ -345,70 +346,130 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
   }
 }
 
-// Enumerate unordered pairs {j, k} of one species class.  Triangle (same species): circulant order
-// (j, j + d mod m) for d = 1 .. m/2, each pair once, balanced over lanes.  Rectangle: all (j, k).
-template <typename F>
-__device__ __forceinline__ void for_pairs_tri(int o, int m, int lane, F&& f) {
-  if (m < 2) return;
-  int j = lane, d = 1;
-  while (j >= m) { j -= m; d++; }
-  const int half = m >> 1;
-  const bool even = (m & 1) == 0;
-  while (d <= half) {
-    if (!(even && d == half && j >= half)) {
-      int k = j + d;
-      if (k >= m) k -= m;
-      f(o + j, o + k);
-    }
-    j += WAVE;
-    while (j >= m) { j -= m; d++; }
-  }
-}
-template <typename F>
-__device__ __forceinline__ void for_pairs_rect(int oa, int ma, int ob, int mb, int lane, F&& f) {
-  // always f(j in class a, k in class b); the longer side is the lane-fast index
-  if (ma == 0 || mb == 0) return;
-  if (ma >= mb) {
-    int j = lane, k = 0;
-    while (j >= ma) { j -= ma; k++; }
-    while (k < mb) {
-      f(oa + j, ob + k);
-      j += WAVE;
-      while (j >= ma) { j -= ma; k++; }
-    }
-  } else {
-    int k = lane, j = 0;
-    while (k >= mb) { k -= mb; j++; }
-    while (j < ma) {
-      f(oa + j, ob + k);
-      k += WAVE;
-      while (k >= mb) { k -= mb; j++; }
+// ---- G4 function patterns on the (eta, zeta, lambda) grid: bitmask of the used (zeta, lambda) cells of each eta
+// (cell q = z*2 + l).  PAT 0 is generic (dense grid, runtime integer zeta); PAT 1 is the 43-function table of the
+// shipped v0/v1 models, PAT 2 the 22-function table of the shipped vp models (both zeta = {1,2,4,16},
+// lambda = {-1,+1}).  Unused cells cost nothing: the masks prune them at compile time.
+template <int PAT> struct Pat;
+template <> struct Pat<0> {
+  static constexpr int NE = 8;
+  static constexpr bool ZSTD = false;
+  static constexpr unsigned mask(int e) { return 0xFFu; }
+};
+template <> struct Pat<1> {
+  static constexpr int NE = 7;
+  static constexpr bool ZSTD = true;
+  static constexpr unsigned mask(int e) { return e < 3 ? 0x0Fu : (e < 6 ? 0xFFu : 0x7Fu); }
+};
+template <> struct Pat<2> {
+  static constexpr int NE = 6;
+  static constexpr bool ZSTD = true;
+  static constexpr unsigned mask(int e) { return e < 2 ? 0x0Fu : (e == 2 ? 0x0Au : 0xAAu); }
+};
+
+// P_zl(c) = 2^(1-zeta) (1 + lambda c)^zeta and dP/dc, cell q = z*2 + l
+template <int PAT>
+__device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, float cs, float* P, float* Pd) {
+#pragma unroll
+  for (int l = 0; l < MG_MAXL; l++) {
+    const float lam = bp->lam[l];
+    const float u = 1.0f + lam * cs;
+    if (Pat<PAT>::ZSTD) {
+      const float u2 = u * u, u4 = u2 * u2, u8 = u4 * u4, u16 = u8 * u8;
+      P[0 + l] = bp->pref[0] * u;
+      P[2 + l] = bp->pref[1] * u2;
+      P[4 + l] = bp->pref[2] * u4;
+      P[6 + l] = bp->pref[3] * u16;
+      if (Pd) {
+        Pd[0 + l] = bp->pref[0] * lam;
+        Pd[2 + l] = bp->pref[1] * 2.0f * lam * u;
+        Pd[4 + l] = bp->pref[2] * 4.0f * lam * u2 * u;
+        Pd[6 + l] = bp->pref[3] * 16.0f * lam * (u8 * u4) * (u2 * u);
+      }
+    } else {
+#pragma unroll
+      for (int z = 0; z < MG_MAXZ; z++) {
+        int zt = bp->zeta[z];
+        float pm1 = ipowf(u, zt - 1);
+        P[z * MG_MAXL + l] = bp->pref[z] * pm1 * u;
+        if (Pd) Pd[z * MG_MAXL + l] = bp->pref[z] * (float)zt * lam * pm1;
+      }
     }
   }
 }
 
-// P_z,l(c) = 2^(1-zeta) (1 + lambda c)^zeta and its derivative in c
-__device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, float cs, float* P, float* Pd) {
-#pragma unroll
-  for (int l = 0; l < MG_MAXL; l++) {
-    float lam = bp->lam[l];
-    float u = 1.0f + lam * cs;
-#pragma unroll
-    for (int z = 0; z < MG_MAXZ; z++) {
-      int zt = bp->zeta[z];
-      float pm1 = ipowf(u, zt - 1);
-      P[z * MG_MAXL + l] = bp->pref[z] * pm1 * u;
-      if (Pd) Pd[z * MG_MAXL + l] = bp->pref[z] * (float)zt * lam * pm1;
+// Unordered neighbour pairs {j, k} of one species class with r_jk < rc.  Enumeration: triangle (same
+// species) in circulant order (j, j + d mod m), d = 1 .. m/2, each pair once; rectangle (a x b) with the longer
+// side as the lane-fast index.  Passing pairs are compacted into an LDS list (ballot + lane prefix count) and
+// processed with all 64 lanes busy; about half of the enumerated pairs fail the r_jk test in bulk metals.
+#define PCH 512
+template <typename F>
+__device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bool tri, int oa, int ma, int ob, int mb,
+                                          float rc2, int lane, F&& body) {
+  int mf, ms;  // fast / slow extents
+  if (tri) {
+    if (ma < 2) return;
+    mf = ma; ms = ma >> 1;
+  } else {
+    if (ma == 0 || mb == 0) return;
+    mf = ma >= mb ? ma : mb; ms = ma >= mb ? mb : ma;
+  }
+  const bool even = (ma & 1) == 0;
+  int f = lane, sl = tri ? 1 : 0;
+  while (f >= mf) { f -= mf; sl++; }
+  bool live = tri ? sl <= ms : sl < ms;
+  int cnt = 0;
+  for (;;) {
+    bool ok = false;
+    unsigned pk = 0;
+    if (live) {
+      int jj, kk;
+      bool use = true;
+      if (tri) {
+        use = !(even && sl == ms && f >= ms);
+        int k = f + sl;
+        if (k >= ma) k -= ma;
+        jj = oa + f; kk = oa + k;
+      } else if (ma >= mb) {
+        jj = oa + f; kk = ob + sl;
+      } else {
+        jj = oa + sl; kk = ob + f;
+      }
+      if (use) {
+        float4 a = rec[NREC * jj], b = rec[NREC * kk];
+        float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
+        ok = cx * cx + cy * cy + cz * cz < rc2;
+        pk = (unsigned)jj | ((unsigned)kk << 16);
+      }
+      f += WAVE;
+      while (f >= mf) { f -= mf; sl++; }
+      live = tri ? sl <= ms : sl < ms;
+    }
+    const unsigned long long bm = __ballot(ok);
+    const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+    if (ok) plist[cnt + below] = pk;
+    cnt += __popcll(bm);
+    const bool more = __ballot(live) != 0ull;
+    if (cnt > PCH - WAVE || !more) {
+      __syncthreads();
+      for (int t = lane; t < cnt; t += WAVE) {
+        unsigned p = plist[t];
+        body((int)(p & 0xffffu), (int)(p >> 16));
+      }
+      __syncthreads();
+      cnt = 0;
+      if (!more) break;
     }
   }
 }
 
 // ---------------------------------------------------------------------------------------------- descriptors
-template <int NE>
+template <int PAT>
 __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                              const int* __restrict__ cnt, const float4* __restrict__ vec,
                                              const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
                                              float* __restrict__ erep, float urep, int cap) {
+  constexpr int NE = Pat<PAT>::NE;
   extern __shared__ float4 smem_d[];
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
@@ -416,6 +477,7 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
   RowInfo ri;
   row_info(ri, i, spc, row_off, cnt, S);
   float4* rec = smem_d;
+  unsigned* plist = (unsigned*)(smem_d + NREC * cap);
   stage_row<NE>(rec, ri, vec, bp, lane, S);
   __syncthreads();
   float* Gi = G + (size_t)i * D;
@@ -423,8 +485,9 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
   // ---- G2 (+ UREP energy) per neighbour species block
   float eu = 0.0f;
   for (int s = 0; s < S; s++) {
-    if (ri.m[s] == 0) {
-      if (lane < bp->ng2) Gi[bp->ind2[(s - ri.si + S) % S] + lane] = 0.0f;
+    const int blk = bp->ind2[(s - ri.si + S) % S];
+    if (ri.m(s) == 0) {
+      if (lane < bp->ng2) Gi[blk + lane] = 0.0f;
       continue;
     }
     const int p = ri.si * MG_MAXSPC + s;
@@ -432,7 +495,7 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
 #pragma unroll
     for (int c = 0; c < MG_MAXG2; c++) acc[c] = 0.0f;
     const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[s];
-    for (int t = ri.o[s] + lane; t < ri.o[s] + ri.m[s]; t += WAVE) {
+    for (int t = ri.o(s) + lane; t < ri.o(s) + ri.m(s); t += WAVE) {
       float4 v = rec[NREC * t];
       float fc = rec[NREC * t + 1].y;
       float r = v.w;
@@ -444,7 +507,6 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
         }
       if (urep > 0.0f && r < Rrep) eu += urep * 0.5f * (1.0f + cospi_(r / Rrep)) * Rrep / r;
     }
-    const int blk = bp->ind2[(s - ri.si + S) % S];
 #pragma unroll
     for (int c = 0; c < MG_MAXG2; c++)
       if (c < bp->ng2) {
@@ -464,7 +526,7 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
       const int blk = bp->ind4[ds1][ds2];
       const int pjk = sa * MG_MAXSPC + sb;
       const float rc = bp->rc[pjk];
-      const float rc2 = rc * rc;
+      const float rinv = 1.0f / rc;
       float ejk[NE];
 #pragma unroll
       for (int e = 0; e < NE; e++) ejk[e] = -(float)MG_LOG2E * bp->eta4[pjk][e];
@@ -475,28 +537,26 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
         float4 a = rec[NREC * j], b = rec[NREC * k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
-        if (!(r3s < rc2)) return;
         float4 ia = rec[NREC * j + 1], ib = rec[NREC * k + 1];
         float r3 = __builtin_sqrtf(r3s);
-        float fc3 = 0.5f * (cospi_(r3 / rc) + 1.0f);
+        float fc3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * ia.x * ib.x;
         float F = 2.0f * ia.y * ib.y * fc3;
         float4 Aj0 = rec[NREC * j + 2], Aj1 = rec[NREC * j + 3], Ak0 = rec[NREC * k + 2], Ak1 = rec[NREC * k + 3];
         float Aj[8] = {Aj0.x, Aj0.y, Aj0.z, Aj0.w, Aj1.x, Aj1.y, Aj1.z, Aj1.w};
         float Ak[8] = {Ak0.x, Ak0.y, Ak0.z, Ak0.w, Ak1.x, Ak1.y, Ak1.z, Ak1.w};
         float P[MG_MAXZ * MG_MAXL];
-        angular(bp, cs, P, nullptr);
+        angular<PAT>(bp, cs, P, nullptr);
 #pragma unroll
         for (int e = 0; e < NE; e++) {
           float B = F * Aj[e] * Ak[e] * fast_exp2(ejk[e] * r3s);
 #pragma unroll
-          for (int q = 0; q < MG_MAXZ * MG_MAXL; q++) V[e * MG_MAXZ * MG_MAXL + q] += B * P[q];
+          for (int q = 0; q < MG_MAXZ * MG_MAXL; q++)
+            if (Pat<PAT>::mask(e) & (1u << q)) V[e * MG_MAXZ * MG_MAXL + q] += B * P[q];
         }
       };
-      if (sa == sb) for_pairs_tri(ri.o[sa], ri.m[sa], lane, body);
-      else for_pairs_rect(ri.o[sa], ri.m[sa], ri.o[sb], ri.m[sb], lane, body);
+      for_pairs(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
       float mine = reduce_scatter64(V, lane);  // lane l: grid cell l
-      // lane n < ng4 fetches its function's grid cell
       int cell = lane < bp->ng4 ? bp->g4cell[lane] : 0;
       float v = __shfl(mine, cell, WAVE);
       if (lane < bp->ng4) Gi[blk + lane] = v;
@@ -569,12 +629,13 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 //   dT/dr1 = -2 r1 F sum_e e_ij S_e B_e + f1' f2 f3 sum_e S_e B_e (r2, r3 likewise), dT/dcos = F sum_e S'_e B_e,
 //   dT/da = dT/dr1 a/r1 - dT/dr3 c/r3 + dT/dcos (b/(r1 r2) - cos a/r1^2), dT/db symmetric.
 // Forces: F_j -= dT/da, F_k -= dT/db, F_i += dT/da + dT/db; virial U -= dT/da (x) a + dT/db (x) b.
-template <int NE>
+template <int PAT>
 __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                               const int* __restrict__ cnt, const float4* __restrict__ vec,
                                               const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
                                               float4* __restrict__ fself, float4* __restrict__ fe,
                                               float* __restrict__ vir, float urep, int cap) {
+  constexpr int NE = Pat<PAT>::NE;
   extern __shared__ float4 smem_f[];
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
@@ -582,7 +643,8 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
   RowInfo ri;
   row_info(ri, i, spc, row_off, cnt, S);
   float4* rec = smem_f;
-  float* fnb = (float*)(smem_f + NREC * cap);  // [cap][4]
+  float* fnb = (float*)(smem_f + NREC * cap);     // [cap][4]
+  unsigned* plist = (unsigned*)(fnb + 4 * cap);  // [PCH]
   stage_row<NE>(rec, ri, vec, bp, lane, S);
   for (int t = lane; t < ri.M; t += WAVE) {
     fnb[4 * t] = 0.0f; fnb[4 * t + 1] = 0.0f; fnb[4 * t + 2] = 0.0f; fnb[4 * t + 3] = 0.0f;
@@ -594,11 +656,11 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
 
   // ---- pair terms (G2) and UREP
   for (int s = 0; s < S; s++) {
-    if (ri.m[s] == 0) continue;
+    if (ri.m(s) == 0) continue;
     const int p = ri.si * MG_MAXSPC + s;
     const int blk = bp->ind2[(s - ri.si + S) % S];
     const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[s];
-    for (int t = ri.o[s] + lane; t < ri.o[s] + ri.m[s]; t += WAVE) {
+    for (int t = ri.o(s) + lane; t < ri.o(s) + ri.m(s); t += WAVE) {
       float4 v = rec[NREC * t];
       float4 w = rec[NREC * t + 1];
       float r = v.w, fc = w.y, dfc = w.z;
@@ -636,8 +698,8 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
       const int blk = bp->ind4[ds1][ds2];
       const int pjk = sa * MG_MAXSPC + sb, pij = ri.si * MG_MAXSPC + sa, pik = ri.si * MG_MAXSPC + sb;
       const float rc = bp->rc[pjk];
-      const float rc2 = rc * rc;
-      const float dfk = -0.5f * 3.14159265358979f / rc;
+      const float rinv = 1.0f / rc;
+      const float dfk = -0.5f * 3.14159265358979f * rinv;
       float ejk[NE], eij[NE], eik[NE], e3[NE];
 #pragma unroll
       for (int e = 0; e < NE; e++) {
@@ -649,36 +711,40 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
       float gt[NE * MG_MAXZ * MG_MAXL];
 #pragma unroll
       for (int q = 0; q < NE * MG_MAXZ * MG_MAXL; q++) {
-        int n = bp->g4map[q];
-        gt[q] = n >= 0 ? gi[blk + n] : 0.0f;
+        if (Pat<PAT>::mask(q / (MG_MAXZ * MG_MAXL)) & (1u << (q % (MG_MAXZ * MG_MAXL)))) {
+          int n = bp->g4map[q];
+          gt[q] = n >= 0 ? gi[blk + n] : 0.0f;
+        } else {
+          gt[q] = 0.0f;
+        }
       }
       auto body = [&](int j, int k) {
         float4 a = rec[NREC * j], b = rec[NREC * k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
-        if (!(r3s < rc2)) return;
         float4 ia = rec[NREC * j + 1], ib = rec[NREC * k + 1];
         float r3 = __builtin_sqrtf(r3s);
         float i3 = 1.0f / r3;
-        float f3 = 0.5f * (cospi_(r3 / rc) + 1.0f);
-        float d3 = dfk * sinpi_(r3 / rc);
+        float f3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
+        float d3 = dfk * sinpi_(r3 * rinv);
         float i1 = ia.x, i2 = ib.x, f1 = ia.y, f2 = ib.y, d1 = ia.z, d2 = ib.z;
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * i1 * i2;
         float4 Aj0 = rec[NREC * j + 2], Aj1 = rec[NREC * j + 3], Ak0 = rec[NREC * k + 2], Ak1 = rec[NREC * k + 3];
         float Aj[8] = {Aj0.x, Aj0.y, Aj0.z, Aj0.w, Aj1.x, Aj1.y, Aj1.z, Aj1.w};
         float Ak[8] = {Ak0.x, Ak0.y, Ak0.z, Ak0.w, Ak1.x, Ak1.y, Ak1.z, Ak1.w};
         float P[MG_MAXZ * MG_MAXL], Pd[MG_MAXZ * MG_MAXL];
-        angular(bp, cs, P, Pd);
+        angular<PAT>(bp, cs, P, Pd);
         float X = 0.f, Y1 = 0.f, Y2 = 0.f, Y3 = 0.f, Zc = 0.f;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
           float B = 2.0f * Aj[e] * Ak[e] * fast_exp2(ejk[e] * r3s);
           float Se = 0.f, Sd = 0.f;
 #pragma unroll
-          for (int q = 0; q < MG_MAXZ * MG_MAXL; q++) {
-            Se += gt[e * MG_MAXZ * MG_MAXL + q] * P[q];
-            Sd += gt[e * MG_MAXZ * MG_MAXL + q] * Pd[q];
-          }
+          for (int q = 0; q < MG_MAXZ * MG_MAXL; q++)
+            if (Pat<PAT>::mask(e) & (1u << q)) {
+              Se += gt[e * MG_MAXZ * MG_MAXL + q] * P[q];
+              Sd += gt[e * MG_MAXZ * MG_MAXL + q] * Pd[q];
+            }
           float SB = Se * B;
           X += SB;
           Y1 += eij[e] * SB;
@@ -709,8 +775,7 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
         u4 -= 0.5f * (gay * a.z + gaz * a.y + gby * b.z + gbz * b.y);
         u5 -= 0.5f * (gaz * a.x + gax * a.z + gbz * b.x + gbx * b.z);
       };
-      if (sa == sb) for_pairs_tri(ri.o[sa], ri.m[sa], lane, body);
-      else for_pairs_rect(ri.o[sa], ri.m[sa], ri.o[sb], ri.m[sb], lane, body);
+      for_pairs(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
     }
   fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
   u0 = wave_sum(u0); u1 = wave_sum(u1); u2 = wave_sum(u2);
@@ -814,15 +879,17 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int mgk_desc(int N, int S, int ne, const int* spc, const int* row_off, const int* cnt, const void* vec,
+int mgk_desc(int N, int S, int pat, const int* spc, const int* row_off, const int* cnt, const void* vec,
              const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, hipStream_t st) {
   if (N == 0) return 0;
   int cap = maxrow < 1 ? 1 : maxrow;
-  size_t sm = (size_t)cap * NREC * sizeof(float4);
-  if (ne <= 7)
-    hipLaunchKernelGGL(k_desc<7>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
+  size_t sm = (size_t)cap * NREC * sizeof(float4) + PCH * sizeof(unsigned);
+  if (pat == 1)
+    hipLaunchKernelGGL(k_desc<1>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
+  else if (pat == 2)
+    hipLaunchKernelGGL(k_desc<2>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
   else
-    hipLaunchKernelGGL(k_desc<8>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
+    hipLaunchKernelGGL(k_desc<0>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -834,18 +901,26 @@ int mgk_mlp(int N, const int* spc, const float* G, const float* W, int wstride, 
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int mgk_force(int N, int S, int ne, const int* spc, const int* row_off, const int* cnt, const void* vec,
+int mgk_force(int N, int S, int pat, const int* spc, const int* row_off, const int* cnt, const void* vec,
               const mg_basis_dev* bp, const float* dEdG, void* fself, void* fe, float* vir, float urep, int maxrow,
-              const int* rev, double* F, hipStream_t st) {
+              hipStream_t st) {
   if (N == 0) return 0;
   int cap = maxrow < 1 ? 1 : maxrow;
-  size_t sm = (size_t)cap * (NREC + 1) * sizeof(float4);
-  if (ne <= 7)
-    hipLaunchKernelGGL(k_force<7>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG,
+  size_t sm = (size_t)cap * (NREC + 1) * sizeof(float4) + PCH * sizeof(unsigned);
+  if (pat == 1)
+    hipLaunchKernelGGL(k_force<1>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG,
+                       (float4*)fself, (float4*)fe, vir, urep, cap);
+  else if (pat == 2)
+    hipLaunchKernelGGL(k_force<2>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG,
                        (float4*)fself, (float4*)fe, vir, urep, cap);
   else
-    hipLaunchKernelGGL(k_force<8>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG,
+    hipLaunchKernelGGL(k_force<0>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG,
                        (float4*)fself, (float4*)fe, vir, urep, cap);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mgk_gather(int N, const int* row_off, const int* rev, const void* fself, const void* fe, double* F, hipStream_t st) {
+  if (N == 0) return 0;
   hipLaunchKernelGGL(k_gather, dim3((N + 255) / 256), dim3(256), 0, st, N, row_off, rev, (const float4*)fself,
                      (const float4*)fe, F);
   return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -302,6 +302,23 @@ int mg_model_load(const char *path, mg_model **out) {
         B->g4cell[c] = cell;
         c++;
       }
+    /* kernel specialisation: the two G4 tables of the shipped models (43 and 22 functions) with
+       zeta = {1,2,4,16}, lambda = {-1,+1}; anything else runs the generic dense-grid kernels */
+    m->pat = 0;
+    if (B->nz == 4 && B->nl == 2 && B->zeta[0] == 1 && B->zeta[1] == 2 && B->zeta[2] == 4 && B->zeta[3] == 16 &&
+        B->lam[0] == -1.0f && B->lam[1] == 1.0f) {
+      static const unsigned m43[8] = {0x0F, 0x0F, 0x0F, 0xFF, 0xFF, 0xFF, 0x7F, 0x00};
+      static const unsigned m22[8] = {0x0F, 0x0F, 0x0A, 0xAA, 0xAA, 0xAA, 0x00, 0x00};
+      for (int cand = 1; cand <= 2 && m->pat == 0; cand++) {
+        const unsigned *mk = cand == 1 ? m43 : m22;
+        int ok = (cand == 1 && B->ne == 7) || (cand == 2 && B->ne == 6);
+        for (int q = 0; q < MG_GRID && ok; q++) {
+          int used = (mk[q / 8] >> (q % 8)) & 1u;
+          if (used != (B->g4map[q] >= 0)) ok = 0;
+        }
+        if (ok) m->pat = cand;
+      }
+    }
     for (int s = 0; s < m->nspc; s++) B->urep_rm[s] = (m->spcz[s] >= 0 && m->spcz[s] < 96) ? (float)MG_MINDIST[m->spcz[s]] : 1.0f;
   }
   free(bs);

@@ -181,3 +181,39 @@ def run_reference(model_path, spc, lat, x, nd=3, xt=1, nm=500, urep=0.0, p_gpa=0
         elif tag == "X":
             out["X"] = np.array([[float(v) for v in next(it).split()] for _ in range(N)])
     return out
+
+
+def fire_relax(model: "OracleModel", spc, lat, x, fmax=0.01, max_steps=2000, dt0=0.1, dtmax=1.0, maxstep=0.2,
+               urep=0.0):
+    """FIRE (Bitzek et al., PRL 97, 170201 (2006)) with unit masses, per-structure step cap, on FP64 oracle
+    forces: the restatement the GPU batched relaxation is checked against.  Returns (x, E, steps, converged)."""
+    x = np.array(x, dtype=np.float64).reshape(-1, 3).copy()
+    v = np.zeros_like(x)
+    dt, alpha, npos = dt0, 0.1, 0
+    for it in range(max_steps):
+        o = model.evaluate(spc, lat, x, urep=urep, threshold=False)
+        F = o["F"]
+        x = o["X"]
+        if np.max(np.sum(F * F, axis=1)) < fmax * fmax:
+            return x, o["E"], it, True
+        vf = float(np.sum(F * v))
+        if vf > 0:
+            ff, vv = float(np.sum(F * F)), float(np.sum(v * v))
+            v = (1 - alpha) * v + (alpha * np.sqrt(vv / ff) if ff > 0 else 0.0) * F
+            if npos > 5:
+                dt = min(dt * 1.1, dtmax)
+                alpha *= 0.99
+            npos += 1
+        else:
+            v[:] = 0.0
+            alpha = 0.1
+            dt *= 0.5
+            npos = 0
+        v = v + dt * F
+        dr = dt * v
+        nrm = np.sqrt(np.sum(dr * dr))
+        if nrm > maxstep:
+            dr *= maxstep / nrm
+        x = x + dr
+    o = model.evaluate(spc, lat, x, urep=urep)
+    return o["X"], o["E"], max_steps, False

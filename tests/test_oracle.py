@@ -1,0 +1,147 @@
+"""The FP64 CPU oracle against the reference's own outputs, fixtures and known-answer properties.
+
+Pins (each item fails for a plausible mistake in the restatement):
+  * golden vectors of the reference build (compiled from its sources, NP=1): H, EA, F, U, G, wrapped X and
+    neighbour sets, on clusters, fractional inputs, pressure, shipped 1/2/3-species models, UREP, small skewed
+    triclinic cells, borderline-Rc shells and the synthetic workload models;
+  * the reference regression fixture test/parse (136 structures, 82-wide descriptors printed to 1e-16);
+  * OUTCAR/README values of interface/c/test, test/relax (start and relaxed minimum), examples/03-rlx and
+    examples/07-nvt;
+  * F = -dE/dx and U = -dE/dstrain by central differences, translation invariance, sum F = 0 and supercell
+    additivity (known-answer properties that do not depend on the reference).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, model_path, nbr_sets
+from oracle import pyoracle as po
+from maise_amd import poscar, workloads as wl
+
+EV2GPA = 160.2176565
+
+
+def _eval(case, **kw):
+    m = po.OracleModel(model_path(case["model"]))
+    return m, m.evaluate(case["spc"], case["lat"], case["x"], nd=int(case["nd"]), xt=int(case["xt"]),
+                         p=float(case["p_gpa"]) / EV2GPA, urep=float(case["urep"]), want_g=True, **kw)
+
+
+@pytest.mark.parametrize("name", ["iface", "relax", "relax_p", "cu", "cuag", "cupdag_tri", "urep", "small_tri",
+                                  "mgca", "config1", "config1_w64", "shell", "config3_like"])
+def test_oracle_matches_reference_build(ref_cases, name):
+    c = ref_cases[name]
+    m, o = _eval(c)
+    N = len(c["spc"])
+    assert o["H"] == pytest.approx(float(c["H"]), rel=1e-12, abs=1e-10)
+    np.testing.assert_allclose(o["EA"], c["EA"], rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(o["F"], c["F"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(o["U"], c["U"], rtol=1e-9, atol=1e-8)
+    np.testing.assert_array_equal(o["X"], c["X"])  # JAR wrap is bit-exact
+    if "G" in c:
+        np.testing.assert_allclose(o["G"], c["G"], rtol=1e-13, atol=1e-13)
+    cnt, ent, _ = po.neighbors(c["lat"], c["x"], m.rc, nd=int(c["nd"]), xt=int(c["xt"]))
+    np.testing.assert_array_equal(cnt, c["nn"])
+    assert nbr_sets(cnt, ent) == nbr_sets(c["nn"], c["nbr"])
+
+
+def test_parse_fixture_descriptors(parse_fixture):
+    m = po.OracleModel(os.path.join(GOLDEN, "models", "nn_CuAg_parse_basis"))
+    worst = 0.0
+    for name, d in parse_fixture.items():
+        o = m.evaluate(d["spc"], d["lat"], d["frac"], nd=3, xt=0, forces=False, want_g=True)
+        worst = max(worst, float(np.max(np.abs(o["G"] - d["G"]) / (1.0 + np.abs(d["G"])))))
+    assert len(parse_fixture) == 136
+    assert worst < 2e-15
+
+
+def test_interface_outcar():
+    """interface/c/test/OUTCAR:22-32"""
+    m = po.OracleModel(os.path.join(GOLDEN, "models", "nn_CuAg_d3_v0_iface"))
+    x = np.array([[2, 0, 0], [-2, 0, 0], [0, 2, 0], [0, -2, 0]], float)
+    o = m.evaluate(np.zeros(4, np.int32), np.eye(3) * 20, x, nd=0)
+    np.testing.assert_allclose(o["EA"], -1.589919661072, atol=6e-13)
+    assert o["H"] == pytest.approx(-6.35967864, abs=6e-9)
+    np.testing.assert_allclose(np.abs(o["F"]).max(axis=1), 1.141127, atol=6e-7)
+    assert o["F"][0, 0] == pytest.approx(-1.141127, abs=6e-7)
+
+
+def test_relax_outcar_start_and_minimum():
+    """test/relax/ref/OUTCAR:22-32 (a=4.0) and :44-54 (relaxed a=3.72890112684628, zero stress)"""
+    m = po.OracleModel(os.path.join(GOLDEN, "models", "nn_CuPd_d3_vp_relax"))
+    frac = np.array([[0, 0, 0], [0, .5, .5], [.5, .5, 0], [.5, 0, .5]])
+    spc = np.array([0, 0, 0, 1], np.int32)
+    o = m.evaluate(spc, np.eye(3) * 4.0, frac, nd=3, xt=0)
+    np.testing.assert_allclose(o["EA"], [-4.268428721950] * 3 + [-4.450024593365], atol=6e-13)
+    np.testing.assert_allclose(o["U"][:3] / 64.0, -0.10966481, atol=6e-9)
+    a = 3.72890112684628
+    o = m.evaluate(spc, np.eye(3) * a, frac, nd=3, xt=0)
+    np.testing.assert_allclose(o["EA"], [-4.648947790797] * 3 + [-4.134534446656], atol=2e-12)
+    assert np.all(np.abs(o["U"] / a ** 3) < 5e-9)
+
+
+def test_examples_03rlx_and_07nvt():
+    """examples/03-rlx/README:40 start energy; examples/07-nvt/README:36 step-0 U and IP"""
+    p = poscar.read_poscar(os.path.join(GOLDEN, "structures", "POSCAR_03rlx"))
+    m = po.OracleModel(os.path.join(GOLDEN, "models", "nn_Cu_03rlx"))
+    o = m.evaluate(p.species, p.lattice, p.frac, nd=3, xt=0)
+    assert o["E"] == pytest.approx(-12.9467487748680767, abs=2e-12)
+    p = poscar.read_poscar(os.path.join(GOLDEN, "structures", "POSCAR_07nvt"))
+    m = po.OracleModel(os.path.join(GOLDEN, "models", "nn_CuAg_07nvt"))
+    o = m.evaluate(p.species, p.lattice, p.frac, nd=3, xt=0)
+    assert o["E"] / 16 == pytest.approx(-3.280776, abs=6e-7)
+    V = abs(np.linalg.det(p.lattice))
+    assert o["U"][:3].sum() / (3 * V) * EV2GPA == pytest.approx(13.787533, abs=6e-6)
+
+
+def _small_alloy():
+    rng = np.random.default_rng(3)
+    return wl.random_triclinic(10, 2, rng)
+
+
+def test_forces_are_minus_energy_gradient():
+    m = po.OracleModel(os.path.join(GOLDEN, "models", "nn_CuAg_d3_v1"))
+    spc, lat, x = _small_alloy()
+    o = m.evaluate(spc, lat, x, threshold=False)
+    h = 1e-5
+    for a in (0, 3, 7):
+        for q in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[a, q] += h
+            xm[a, q] -= h
+            d = (m.evaluate(spc, lat, xp, forces=False)["E"] - m.evaluate(spc, lat, xm, forces=False)["E"]) / (2 * h)
+            assert o["F"][a, q] == pytest.approx(-d, abs=2e-6)
+
+
+def test_virial_is_minus_strain_derivative():
+    m = po.OracleModel(os.path.join(GOLDEN, "models", "nn_CuAg_d3_v1"))
+    spc, lat, x = _small_alloy()
+    o = m.evaluate(spc, lat, x, threshold=False)
+    h = 1e-6
+    comps = [(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)]
+    for k, (a, b) in enumerate(comps):
+        eps = np.zeros((3, 3))
+        eps[a, b] = eps[b, a] = h if a != b else h
+        def e(sign):
+            T = np.eye(3) + sign * eps
+            return m.evaluate(spc, lat @ T.T if False else lat @ T, x @ T, forces=False)["E"]
+        d = (e(1) - e(-1)) / (2 * h)
+        if a != b:
+            d /= 2.0  # symmetric shear moves both off-diagonal components
+        assert o["U"][k] == pytest.approx(-d, rel=1e-5, abs=2e-5)
+
+
+def test_invariants_translation_sumF_additivity():
+    m = po.OracleModel(os.path.join(GOLDEN, "models", "nn_Cu_d3_v1"))
+    spc, lat, x = wl.cu_fcc(3, 3.61, 0.08, seed=9)
+    a = m.evaluate(spc, lat, x)
+    b = m.evaluate(spc, lat, x + np.array([0.37, -1.2, 2.9]))
+    assert a["E"] == pytest.approx(b["E"], abs=1e-9)
+    np.testing.assert_allclose(a["F"], b["F"], atol=1e-8)
+    assert np.abs(a["F"].sum(axis=0)).max() < 1e-9
+    s1, l1, x1 = wl.cu_fcc(2, 3.61, 0.0)
+    s2, l2, x2 = wl.cu_fcc(4, 3.61, 0.0)
+    e1 = m.evaluate(s1, l1, x1, forces=False)["E"] / len(s1)
+    e2 = m.evaluate(s2, l2, x2, forces=False)["E"] / len(s2)
+    assert e1 == pytest.approx(e2, abs=1e-10)
