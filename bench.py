@@ -92,7 +92,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=20, help="fcc supercell edge (4 n^3 atoms)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 256 MiB (5 times): the reference point that scales the "
+                         "FETCH_SIZE/WRITE_SIZE counters of a profiling run to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4, "dispatches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
               "design-only.", file=sys.stderr)
@@ -153,6 +165,27 @@ def main():
     except Exception as e:  # the error report is informative; the oracle may be absent
         err = {"unavailable": str(e)[:200]}
 
+    # same workload with the compensated-BF16 MLP on MFMA
+    bf = None
+    try:
+        cb = maise_amd.Context(model, device=0, precision=maise_amd.MG_PREC_BF16X3)
+        cb.set_structure(spc, lat, x)
+        for _ in range(args.warmup):
+            cb.eval(forces=True)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            cb.eval(forces=True)
+        sync()
+        dtb = time.perf_counter() - t0
+        rb = cb.result(0)
+        bf = {"value": N * args.steps / dtb, "ms_per_step": 1e3 * dtb / args.steps,
+              "dE_per_atom_vs_fp32_eV": abs(rb["E"] - res["E"]) / N,
+              "dF_max_vs_fp32_eV_A": float(np.abs(rb["F"] - res["F"]).max())}
+        cb.close()
+    except Exception as e:
+        bf = {"unavailable": str(e)[:200]}
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -174,6 +207,7 @@ def main():
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu and "value" in cpu else None,
         "error_fp64": err,
+        "bf16x3_mlp": bf,
         "energy_eV": res["E"],
     }
     print(json.dumps(out))

@@ -24,6 +24,7 @@ MG_EVAL_FORCES = 1
 MG_EVAL_KEEP_G = 2
 MG_PREC_FP32 = 0
 MG_PREC_BF16 = 1
+MG_PREC_BF16X3 = 2
 EV2GPA = 160.2176565
 
 _lib = None

@@ -98,8 +98,6 @@ static int ensure_bins(mg_ctx *c, size_t nb) {
 static int ensure_edges(mg_ctx *c, size_t M) {
   if (M <= c->cap_edges) return MG_OK;
   size_t n = M + M / 4 + 64;
-  ALLOC(c->d_key_u, n * 8);
-  ALLOC(c->d_vec_u, n * 16);
   ALLOC(c->d_key, n * 8);
   ALLOC(c->d_vec, n * 16);
   ALLOC(c->d_rev, n * sizeof(int));
@@ -129,6 +127,19 @@ int mg_ctx_create(const mg_model *m, int device, mg_ctx **out) {
   }
   HIPCHK(hipMemcpy(c->d_basis, &m->basis, sizeof(mg_basis_dev), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_mlp, m->mlp, (size_t)m->wstride * m->nspc * sizeof(float), hipMemcpyHostToDevice));
+  if (m->mfma_ok) {
+    if ((r = mg_dev_alloc((void **)&c->d_frag, m->frag_n * sizeof(float))) ||
+        (r = mg_dev_alloc((void **)&c->d_frag_bf, m->frag_bf_n * sizeof(uint16_t))) ||
+        (r = mg_dev_alloc((void **)&c->d_frag_lo, m->frag_bf_n * sizeof(uint16_t))) ||
+        (r = mg_dev_alloc((void **)&c->d_fvec, m->fvec_n * sizeof(float)))) {
+      mg_ctx_free(c);
+      return r;
+    }
+    HIPCHK(hipMemcpy(c->d_frag, m->frag, m->frag_n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_frag_bf, m->frag_bf, m->frag_bf_n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_frag_lo, m->frag_lo, m->frag_bf_n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_fvec, m->fvec, m->fvec_n * sizeof(float), hipMemcpyHostToDevice));
+  }
   *out = c;
   return MG_OK;
 }
@@ -138,8 +149,8 @@ void mg_ctx_free(mg_ctx *c) {
   hipSetDevice(c->device);
   void *ptrs[] = {c->d_basis, c->d_mlp, c->d_str, c->d_atom_str, c->d_spc, c->d_xin, c->d_x, c->d_bin, c->d_bin_count,
                   c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
-                  c->d_key_u, c->d_vec_u, c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
-                  c->d_vir, c->d_fself, c->d_fe, c->d_F, c->d_E, c->d_U, c->d_tmp,
+                  c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
+                  c->d_vir, c->d_fself, c->d_fe, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
     if (ptrs[k]) hipFree(ptrs[k]);
@@ -156,7 +167,7 @@ int mg_ctx_set_urep(mg_ctx *c, double urep) {
 }
 
 int mg_ctx_set_precision(mg_ctx *c, int prec) {
-  if (!c || (prec != MG_PREC_FP32 && prec != MG_PREC_BF16)) return mg_fail(MG_ERR_ARG, "bad precision");
+  if (!c || prec < MG_PREC_FP32 || prec > MG_PREC_BF16X3) return mg_fail(MG_ERR_ARG, "bad precision");
   c->prec = prec;
   return MG_OK;
 }
@@ -230,7 +241,7 @@ static int upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const in
   int r;
   if ((r = ensure_str(c, S)) || (r = ensure_atoms(c, N))) return r;
   int *atom_str = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
-  int bin0 = 0;
+  int bin0 = 0, nchunk = 0;
   const double rc = c->m->rc_list;
   for (int s = 0; s < S; s++) {
     mg_struct_dev *d = &c->h_str[s];
@@ -246,6 +257,9 @@ static int upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const in
     plan_bins(d, rc, d->natom);
     d->bin0 = bin0;
     bin0 += d->nbins;
+    d->chunk0 = nchunk;
+    d->nchunk = (d->natom + MG_RCHUNK - 1) / MG_RCHUNK;
+    nchunk += d->nchunk;
     for (int i = d->atom0; i < d->atom0 + d->natom; i++) atom_str[i] = s;
     c->h_nd[s] = nd[s];
     memcpy(c->h_lat + 9 * s, d->L, 9 * sizeof(double));
@@ -253,6 +267,53 @@ static int upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const in
   }
   c->h_atom_off[S] = N;
   c->nbins = bin0;
+  if ((size_t)nchunk > c->cap_chunk) {
+    if ((r = mg_dev_alloc((void **)&c->d_chunk_str, (nchunk + 16) * sizeof(int))) ||
+        (r = mg_dev_alloc((void **)&c->d_part, (nchunk + 16) * 7 * sizeof(double)))) { free(atom_str); return r; }
+    c->cap_chunk = nchunk + 16;
+  }
+  {
+    int *cs = (int *)malloc(sizeof(int) * (nchunk > 0 ? nchunk : 1));
+    for (int s = 0; s < S; s++)
+      for (int k = 0; k < c->h_str[s].nchunk; k++) cs[c->h_str[s].chunk0 + k] = s;
+    hipError_t e = hipMemcpy(c->d_chunk_str, cs, sizeof(int) * nchunk, hipMemcpyHostToDevice);
+    free(cs);
+    if (e != hipSuccess) { free(atom_str); return mg_fail(MG_ERR_HIP, "chunk table: %s", hipGetErrorString(e)); }
+  }
+  c->nchunk = nchunk;
+  /* species-sorted permutation and MLP tiles of <= 32 same-species atoms */
+  {
+    const int S_ = c->m->nspc;
+    int *perm = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+    int cntS[MG_MAXSPC] = {0, 0, 0, 0}, off[MG_MAXSPC + 1];
+    for (int i = 0; i < N; i++) cntS[spc[i]]++;
+    off[0] = 0;
+    for (int q = 0; q < S_; q++) off[q + 1] = off[q] + cntS[q];
+    int pos[MG_MAXSPC];
+    for (int q = 0; q < S_; q++) pos[q] = off[q];
+    for (int i = 0; i < N; i++) perm[pos[spc[i]]++] = i;
+    int nt = 0;
+    for (int q = 0; q < S_; q++) nt += (cntS[q] + 31) / 32;
+    int *tiles = (int *)malloc(sizeof(int) * 4 * (nt > 0 ? nt : 1));
+    nt = 0;
+    for (int q = 0; q < S_; q++)
+      for (int a = off[q]; a < off[q + 1]; a += 32) {
+        tiles[4 * nt] = q; tiles[4 * nt + 1] = a; tiles[4 * nt + 2] = off[q + 1] - a < 32 ? off[q + 1] - a : 32; tiles[4 * nt + 3] = 0;
+        nt++;
+      }
+    if ((size_t)nt > c->cap_tiles || (size_t)N > c->cap_tiles * 32 || !c->d_perm) {
+      size_t cap = (size_t)nt + N / 32 + 64;
+      if ((r = mg_dev_alloc((void **)&c->d_tiles, cap * 4 * sizeof(int))) ||
+          (r = mg_dev_alloc((void **)&c->d_perm, (cap * 32 + 64) * sizeof(int)))) { free(perm); free(tiles); free(atom_str); return r; }
+      c->cap_tiles = cap;
+    }
+    hipError_t e1 = hipMemcpy(c->d_perm, perm, sizeof(int) * N, hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(c->d_tiles, tiles, sizeof(int) * 4 * nt, hipMemcpyHostToDevice);
+    free(perm);
+    free(tiles);
+    if (e1 != hipSuccess || e2 != hipSuccess) { free(atom_str); return mg_fail(MG_ERR_HIP, "tile upload failed"); }
+    c->ntiles = nt;
+  }
   if ((r = ensure_bins(c, bin0))) { free(atom_str); return r; }
   int nscan = N > bin0 ? N : bin0;
   if ((r = ensure_scan_tmp(c, nscan))) { free(atom_str); return r; }
@@ -295,7 +356,7 @@ int mg_eval(mg_ctx *c, int flags) {
   hipStream_t st = c->st;
   const int forces = (flags & MG_EVAL_FORCES) != 0;
   const float urep = (float)c->urep;
-  if (c->prec != MG_PREC_FP32) return mg_fail(MG_ERR_UNSUPPORTED, "BF16 precision is not available in this build");
+  if (c->prec != MG_PREC_FP32 && !m->mfma_ok) return mg_fail(MG_ERR_UNSUPPORTED, "BF16 MLP needs hidden layers <= 64 wide");
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_START], st));
   /* neighbour list */
   HIPCHK(hipMemsetAsync(c->d_bin_count, 0, (c->nbins + 1) * sizeof(int), st));
@@ -316,23 +377,28 @@ int mg_eval(mg_ctx *c, int flags) {
   int r;
   if ((r = ensure_edges(c, (size_t)c->nedges))) return r;
   KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_atoms, c->d_row_off,
-                    c->d_key_u, c->d_vec_u, c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, c->maxrow, st));
+                    c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, c->maxrow, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
   /* descriptors, MLP, forces */
-  KCHK(mgk_desc(N, m->nspc, m->pat, c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
+  KCHK(mgk_desc(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
                 urep, c->maxrow, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_DESC], st));
-  KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
+  if (m->mfma_ok)
+    KCHK(mgk_mlp_mfma(c->ntiles, c->d_tiles, c->d_perm, c->d_G, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, m->fdesc,
+                      c->prec, c->d_Ei, c->d_dEdG, st));
+  else
+    KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_MLP], st));
   if (forces) {
-    KCHK(mgk_force(N, m->nspc, m->pat, c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
+    KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
                    c->d_fe, c->d_vir, urep, c->maxrow, st));
     if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
     KCHK(mgk_gather(N, c->d_row_off, c->d_rev, c->d_fself, c->d_fe, c->d_F, st));
   } else if (c->timing) {
     HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
   }
-  KCHK(mgk_reduce(c->S, c->d_str, c->d_Ei, c->d_erep, c->d_vir, forces, c->d_E, c->d_U, st));
+  KCHK(mgk_reduce(c->S, c->nchunk, c->d_chunk_str, c->d_str, c->d_Ei, c->d_erep, c->d_vir, forces, c->d_part, c->d_E,
+                  c->d_U, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_END], st));
   int err = 0;
   HIPCHK(hipMemcpyAsync(&err, c->d_misc + 1, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -7,6 +7,7 @@
 #endif
 #include <hip/hip_runtime_api.h>
 #include <stddef.h>
+#include <stdint.h>
 
 #include "../../include/maise_gpu.h"
 #include "mg_internal.h"
@@ -20,6 +21,14 @@ struct mg_model {
   double *mlp64;        /* same layout in FP64 */
   int wstride;          /* floats per species block (multiple of 4) */
   int pat;              /* G4 kernel specialisation: 0 generic, 1 43-function, 2 22-function table */
+  int uni;              /* 1 when every species pair has the same G4 etas */
+  /* MFMA MLP: weight fragments in the kernel's operand order (FP32 and BF16 sets) and per-unit vectors */
+  int mfma_ok;          /* H1, H2 <= 64 */
+  int fdesc[20];
+  float *frag;
+  uint16_t *frag_bf, *frag_lo;
+  float *fvec;
+  size_t frag_n, frag_bf_n, fvec_n;
 };
 
 #define HIPCHK(x)                                                                   \
@@ -53,11 +62,17 @@ struct mg_ctx {
   double *d_xin, *d_x;
   int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur, *d_bin_atoms;
   int *d_cnt, *d_rowcnt, *d_row_off, *d_misc; /* misc: [0] maxrow, [1] error flag */
-  void *d_key_u, *d_vec_u, *d_key, *d_vec;
+  void *d_key, *d_vec;
   int *d_rev;
   float *d_G, *d_dEdG, *d_Ei, *d_erep, *d_vir;
   void *d_fself, *d_fe;
-  double *d_F, *d_E, *d_U;
+  double *d_F, *d_E, *d_U, *d_part;
+  float *d_frag, *d_fvec;
+  void *d_frag_bf, *d_frag_lo;
+  int *d_perm, *d_tiles, ntiles;
+  size_t cap_tiles;
+  int *d_chunk_str, nchunk;
+  size_t cap_chunk;
   void *d_tmp;
   size_t tmp_bytes;
   /* capacities (elements) */
@@ -85,17 +100,20 @@ int mgk_bins(int N, int nbins, const int *bin, const int *bin_start, int *bin_cu
 int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
                   const int *bin_start, const int *bin_atoms, int *cnt, int *rowcnt, int *maxrow, hipStream_t st);
 int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
-                 const int *bin_start, const int *bin_atoms, const int *row_off, void *key_u, void *vec_u, void *key,
-                 void *vec, int *rev, int *err, int maxrow, hipStream_t st);
-int mgk_desc(int N, int S, int pat, const int *spc, const int *row_off, const int *cnt, const void *vec,
+                 const int *bin_start, const int *bin_atoms, const int *row_off, void *key, void *vec, int *rev, int *err,
+                 int maxrow, hipStream_t st);
+int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
              const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, hipStream_t st);
+int mgk_mlp_mfma(int ntiles, const void *tiles, const int *perm, const float *G, const float *frag, const void *frag_bf,
+                 const void *frag_lo, const float *vecs, const int *fdesc, int prec, float *Ei, float *dEdG,
+                 hipStream_t st);
 int mgk_mlp(int N, const int *spc, const float *G, const float *W, int wstride, int D, int H1, int H2, float *Ei,
             float *dEdG, hipStream_t st);
-int mgk_force(int N, int S, int pat, const int *spc, const int *row_off, const int *cnt, const void *vec,
+int mgk_force(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
               const mg_basis_dev *bp, const float *dEdG, void *fself, void *fe, float *vir, float urep, int maxrow,
               hipStream_t st);
 int mgk_gather(int N, const int *row_off, const int *rev, const void *fself, const void *fe, double *F, hipStream_t st);
-int mgk_reduce(int nstr, const mg_struct_dev *str, const float *Ei, const float *erep, const float *vir, int have_vir,
-               double *E, double *U, hipStream_t st);
+int mgk_reduce(int nstr, int nchunk, const int *chunk_str, const mg_struct_dev *str, const float *Ei, const float *erep,
+               const float *vir, int have_vir, double *part, double *E, double *U, hipStream_t st);
 
 #endif

@@ -27,6 +27,7 @@
 #define MG_MAXG4 MG_GRID
 #define MG_IMG_OFF 512
 #define MG_LOG2E 1.4426950408889634
+#define MG_RCHUNK 2048              /* atoms per FP64 reduction chunk */
 
 #ifdef __cplusplus
 extern "C" {
@@ -58,6 +59,7 @@ typedef struct {
   int nb[3];                   /* bins per lattice direction */
   int K[3];                    /* stencil half width per direction */
   int bin0, nbins;
+  int chunk0, nchunk;           /* reduction chunks (MG_RCHUNK atoms each) of this structure */
 } mg_struct_dev;
 
 #ifdef __cplusplus

@@ -3,11 +3,10 @@
 // Pipeline for one evaluation (all structures of a context at once):
 //   k_prepare      CAST_CELL fractional->Cartesian, JAR wrap in FP64 (src/cmod.c:1112-1121, src/cell.c:446-508),
 //                  cell-list bin of every atom
-//   k_bin_fill / k_bin_sort    counting sort of atoms into bins (deterministic order inside a bin)
-//   k_nbr<false>   per-atom neighbour count per species over the bin stencil, FP64 test r < Rc^2 in the
-//                  reference's expression order (src/cell.c:109-123) so the neighbour SETS are bit-exact
-//   k_nbr<true>    fill: 64-bit key (species, j, image) + FP32 (d, r) per edge
-//   k_row_sort     one wave per row: bitonic sort of the row by key (canonical, bin-independent order)
+//   k_bin_fill     counting sort of atoms into bins (order inside a bin is free: rows are sorted later)
+//   k_nbr<false>   one wave per atom: neighbour count per species over the bin stencil, FP64 test r < Rc^2 in
+//                  the reference's expression order (src/cell.c:109-123) so the neighbour SETS are bit-exact
+//   k_nbr<true>    fill: 64-bit key (species, j, image) + FP32 (d, r) per edge, row sorted by key in LDS
 //   k_rev          reverse-edge index by binary search in the neighbour's row
 //   k_desc<NE>     one wave per atom: G2 over neighbours, G4 over neighbour pairs of each species class,
 //                  neighbour tile staged in LDS, (eta, zeta, lambda) grid accumulators in VGPRs,
@@ -96,139 +95,158 @@ __global__ void k_bin_fill(int N, const int* __restrict__ bin, const int* __rest
   bin_atoms[p] = i;
 }
 
-__global__ void k_bin_sort(int nbins, const int* __restrict__ bin_start, int* __restrict__ bin_atoms) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  int s = bin_start[b], e = bin_start[b + 1];
-  for (int p = s + 1; p < e; p++) {
-    int v = bin_atoms[p], q = p - 1;
-    while (q >= s && bin_atoms[q] > v) { bin_atoms[q + 1] = bin_atoms[q]; q--; }
-    bin_atoms[q + 1] = v;
-  }
-}
-
 // ---------------------------------------------------------------------------------------------- neighbours
-// One thread per atom.  Stencil over bins within K_q in each direction; the image of a candidate follows
-// from the periodic wrap of the bin coordinate.  The FP64 distance is formed exactly as LIST does.
+// One wave per atom.  The stencil of (2K0+1)(2K1+1)(2K2+1) bins around the atom's bin (each with the image
+// shift its periodic wrap implies) is processed in chunks of 64 entries: lane l builds entry l (bin start,
+// atom count, FP64 image shift), a wave prefix scan over the counts flattens the chunk into one candidate
+// range, and the lanes then test candidates c = lane, lane + 64, ... (entry found by binary search in LDS).
+// The FP64 distance uses LIST's expression order (src/cell.c:115-123), so the accepted set is bit-exact.
+// FILL: accepted entries are appended to an LDS row (ballot + lane prefix), the row is sorted by key
+// (species, j, image) with a bitonic sort and written once: canonical order, independent of the binning.
+#define NBR_CHUNK 64
 template <bool FILL>
-__global__ void k_nbr(int N, const double* __restrict__ x, const int* __restrict__ spc, const int* __restrict__ atom_str,
-                      const mg_struct_dev* __restrict__ str, const int* __restrict__ bin, const int* __restrict__ bin_start,
-                      const int* __restrict__ bin_atoms, int* __restrict__ cnt, int* __restrict__ rowcnt,
-                      int* __restrict__ maxrow, const int* __restrict__ row_off, u64* __restrict__ key_u,
-                      float4* __restrict__ vec_u) {
+__global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x, const int* __restrict__ spc,
+                                            const int* __restrict__ atom_str, const mg_struct_dev* __restrict__ str,
+                                            const int* __restrict__ bin, const int* __restrict__ bin_start,
+                                            const int* __restrict__ bin_atoms, int* __restrict__ cnt,
+                                            int* __restrict__ rowcnt, int* __restrict__ maxrow,
+                                            const int* __restrict__ row_off, u64* __restrict__ key,
+                                            float4* __restrict__ vec, int cap) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  __shared__ int e_start[NBR_CHUNK], e_pre[NBR_CHUNK + 1];
+  __shared__ unsigned e_img[NBR_CHUNK];
+  __shared__ double e_sh[NBR_CHUNK][3];
+  extern __shared__ unsigned char smem_n[];
+  u64* rk = (u64*)smem_n;             // [cap] keys of the row (FILL)
+  float4* rv = (float4*)(rk + cap);   // [cap] vectors
+  int* rp = (int*)(rv + cap);         // [cap] permutation
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
   const mg_struct_dev* st = &str[atom_str[i]];
   const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
   const double RC = st->rc2;
   const int nb0 = st->nb[0], nb1 = st->nb[1], nb2 = st->nb[2];
-  int local = bin[i] - st->bin0;
+  const int K0 = st->K[0], K1 = st->K[1], K2 = st->K[2];
+  const int w1n = 2 * K1 + 1, w2n = 2 * K2 + 1;
+  const int nsten = (2 * K0 + 1) * w1n * w2n;
+  const int local = bin[i] - st->bin0;
   const int b2 = local % nb2, b1 = (local / nb2) % nb1, b0 = local / (nb1 * nb2);
-  int c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // per-species counts
-  int pos = FILL ? row_off[i] : 0;
-  for (int d0 = -st->K[0]; d0 <= st->K[0]; d0++) {
-    int w0 = b0 + d0, ic0 = floordiv_i(w0, nb0);
-    w0 -= ic0 * nb0;
-    for (int d1 = -st->K[1]; d1 <= st->K[1]; d1++) {
+  const unsigned img0 = pack_img(0, 0, 0);
+  int c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // per-species counts (wave-uniform)
+  int n = 0;                           // accepted so far (wave-uniform)
+  for (int base = 0; base < nsten; base += NBR_CHUNK) {
+    const int t = base + lane;
+    int count = 0;
+    if (t < nsten) {
+      const int d2 = t % w2n - K2, d1 = (t / w2n) % w1n - K1, d0 = t / (w2n * w1n) - K0;
+      int w0 = b0 + d0, ic0 = floordiv_i(w0, nb0);
+      w0 -= ic0 * nb0;
       int w1 = b1 + d1, ic1 = floordiv_i(w1, nb1);
       w1 -= ic1 * nb1;
-      for (int d2 = -st->K[2]; d2 <= st->K[2]; d2++) {
-        int w2 = b2 + d2, ic2 = floordiv_i(w2, nb2);
-        w2 -= ic2 * nb2;
-        double sh[3];
-        for (int q = 0; q < 3; q++) {
-          double v = 0.0;
-          v += (double)ic0 * st->L[q];
-          v += (double)ic1 * st->L[3 + q];
-          v += (double)ic2 * st->L[6 + q];
-          sh[q] = v;
+      int w2 = b2 + d2, ic2 = floordiv_i(w2, nb2);
+      w2 -= ic2 * nb2;
+      const int w = st->bin0 + (w0 * nb1 + w1) * nb2 + w2;
+      e_start[lane] = bin_start[w];
+      count = bin_start[w + 1] - bin_start[w];
+      e_img[lane] = pack_img(ic0, ic1, ic2);
+      for (int q = 0; q < 3; q++) {
+        double v = 0.0;
+        v += (double)ic0 * st->L[q];
+        v += (double)ic1 * st->L[3 + q];
+        v += (double)ic2 * st->L[6 + q];
+        e_sh[lane][q] = v;
+      }
+    }
+    int inc = count;  // inclusive scan of the counts across the wave
+#pragma unroll
+    for (int k = 1; k < WAVE; k <<= 1) {
+      int y = __shfl_up(inc, k, WAVE);
+      if (lane >= k) inc += y;
+    }
+    e_pre[lane + 1] = inc;
+    if (lane == 0) e_pre[0] = 0;
+    const int total = __shfl(inc, WAVE - 1, WAVE);
+    __syncthreads();
+    for (int cb = 0; cb < total; cb += WAVE) {
+      const int c = cb + lane;
+      bool ok = false;
+      int j = 0, sj = 0;
+      unsigned img = 0;
+      double a0 = 0, a1 = 0, a2 = 0, r = 0;
+      if (c < total) {
+        int lo = 0, hi = NBR_CHUNK - 1;  // last entry e with e_pre[e] <= c
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (e_pre[mid] <= c) lo = mid; else hi = mid - 1;
         }
-        const bool zero_img = (ic0 == 0 && ic1 == 0 && ic2 == 0);
-        int w = st->bin0 + (w0 * nb1 + w1) * nb2 + w2;
-        int s = bin_start[w], e = bin_start[w + 1];
-        for (int p = s; p < e; p++) {
-          int j = bin_atoms[p];
-          if (j == i && zero_img) continue;
-          double a0 = sh[0]; a0 += x[3 * j] - xi0;
-          double a1 = sh[1]; a1 += x[3 * j + 1] - xi1;
-          double a2 = sh[2]; a2 += x[3 * j + 2] - xi2;
-          double r = 0.0;
+        j = bin_atoms[e_start[lo] + (c - e_pre[lo])];
+        img = e_img[lo];
+        if (!(j == i && img == img0)) {
+          a0 = e_sh[lo][0]; a0 += x[3 * j] - xi0;
+          a1 = e_sh[lo][1]; a1 += x[3 * j + 1] - xi1;
+          a2 = e_sh[lo][2]; a2 += x[3 * j + 2] - xi2;
+          r = 0.0;
           r += a0 * a0;
           r += a1 * a1;
           r += a2 * a2;
-          if (r < RC) {
-            int sj = spc[j];
-            if (FILL) {
-              key_u[pos] = make_key(sj, j, pack_img(ic0, ic1, ic2));
-              vec_u[pos] = make_float4((float)a0, (float)a1, (float)a2, (float)sqrt(r));
-              pos++;
-            } else {
-              c0 += sj == 0; c1 += sj == 1; c2 += sj == 2; c3 += sj == 3;
-            }
-          }
+          ok = r < RC;
+          if (ok) sj = spc[j];
         }
       }
+      const u64 bm = __ballot(ok);
+      if (FILL) {
+        const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+        if (ok && n + below < cap) {
+          rk[n + below] = make_key(sj, j, img);
+          rv[n + below] = make_float4((float)a0, (float)a1, (float)a2, (float)sqrt(r));
+        }
+      } else {
+        c0 += __popcll(__ballot(ok && sj == 0));
+        c1 += __popcll(__ballot(ok && sj == 1));
+        c2 += __popcll(__ballot(ok && sj == 2));
+        c3 += __popcll(__ballot(ok && sj == 3));
+      }
+      n += __popcll(bm);
     }
+    __syncthreads();
   }
   if (!FILL) {
-    cnt[4 * i] = c0; cnt[4 * i + 1] = c1; cnt[4 * i + 2] = c2; cnt[4 * i + 3] = c3;
-    int t = c0 + c1 + c2 + c3;
-    rowcnt[i] = t;
-    atomicMax(maxrow, t);
-  }
-}
-
-// One wave (block of 64) per row: bitonic sort of the keys in LDS, then gather (key, vec) into place.
-__global__ __launch_bounds__(64) void k_row_sort(int N, const int* __restrict__ row_off, const u64* __restrict__ key_u,
-                                                 const float4* __restrict__ vec_u, u64* __restrict__ key,
-                                                 float4* __restrict__ vec, int cap) {
-  extern __shared__ unsigned char smem_sort[];
-  u64* sk = (u64*)smem_sort;
-  int* si = (int*)(sk + cap);
-  const int i = blockIdx.x;
-  const int lane = threadIdx.x;
-  const int off = row_off[i], M = row_off[i + 1] - off;
-  if (M > cap) {  // long row: one lane sorts in place (rare: only when the LDS tile is too small)
     if (lane == 0) {
-      for (int t = 0; t < M; t++) { key[off + t] = key_u[off + t]; vec[off + t] = vec_u[off + t]; }
-      for (int p = 1; p < M; p++) {
-        u64 kv = key[off + p];
-        float4 vv = vec[off + p];
-        int q = p - 1;
-        while (q >= 0 && key[off + q] > kv) { key[off + q + 1] = key[off + q]; vec[off + q + 1] = vec[off + q]; q--; }
-        key[off + q + 1] = kv;
-        vec[off + q + 1] = vv;
-      }
+      cnt[4 * i] = c0; cnt[4 * i + 1] = c1; cnt[4 * i + 2] = c2; cnt[4 * i + 3] = c3;
+      rowcnt[i] = n;
+      // a plain read first: almost every row is shorter than the running maximum, so few atomics contend
+      if (n > __atomic_load_n(maxrow, __ATOMIC_RELAXED)) atomicMax(maxrow, n);
     }
     return;
   }
+  const int M = n < cap ? n : cap;  // n <= cap by construction (cap >= longest row)
   int n2 = 1;
   while (n2 < M) n2 <<= 1;
   for (int t = lane; t < n2; t += WAVE) {
-    sk[t] = t < M ? key_u[off + t] : ~0ull;
-    si[t] = t;
+    if (t >= M) rk[t] = ~0ull;
+    rp[t] = t;
   }
   __syncthreads();
   for (int k = 2; k <= n2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
       for (int t = lane; t < n2; t += WAVE) {
-        int p = t ^ j;
+        const int p = t ^ jj;
         if (p > t) {
-          bool asc = (t & k) == 0;
-          u64 a = sk[t], b = sk[p];
+          const bool asc = (t & k) == 0;
+          const u64 a = rk[t], b = rk[p];
           if ((a > b) == asc) {
-            sk[t] = b; sk[p] = a;
-            int ia = si[t]; si[t] = si[p]; si[p] = ia;
+            rk[t] = b; rk[p] = a;
+            const int ia = rp[t]; rp[t] = rp[p]; rp[p] = ia;
           }
         }
       }
       __syncthreads();
     }
   }
+  const int off = row_off[i];
   for (int t = lane; t < M; t += WAVE) {
-    key[off + t] = sk[t];
-    vec[off + t] = vec_u[off + si[t]];
+    key[off + t] = rk[t];
+    vec[off + t] = rv[rp[t]];
   }
 }
 
@@ -298,8 +316,6 @@ __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_ex
 __device__ __forceinline__ float cospi_(float t) { return __builtin_amdgcn_cosf(0.5f * t); }
 __device__ __forceinline__ float sinpi_(float t) { return __builtin_amdgcn_sinf(0.5f * t); }
 
-// Per-neighbour LDS record (64 B): (dx, dy, dz, r), (1/r, fc, dfc, -), A[0..7] = exp(-eta4(i,j)_e r^2)
-#define NREC 4
 
 struct RowInfo {
   int off, M, si;
@@ -321,10 +337,13 @@ __device__ __forceinline__ void row_info(RowInfo& ri, int i, const int* __restri
   ri.m3 = S > 3 ? cnt[4 * i + 3] : 0;
 }
 
-// stage the row of atom i into LDS
-template <int NE>
+// Stage the row of atom i into LDS.  Record per neighbour: (dx, dy, dz, r), (1/r, fc, dfc, r^2) and, unless
+// every species pair shares the G4 etas (UNI), A[0..7] = exp(-eta4(i,j)_e r^2).  With shared etas a triplet's
+// radial factor is one exp(-eta_e (r_ij^2 + r_ik^2 + r_jk^2)) and the A tables are not needed.
+template <int NE, bool UNI>
 __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const float4* __restrict__ vec,
                                           const mg_basis_dev* __restrict__ bp, int lane, int S) {
+  constexpr int R = UNI ? 2 : 4;
   for (int t = lane; t < ri.M; t += WAVE) {
     float4 v = vec[ri.off + t];
     int s = 0;
@@ -335,14 +354,16 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
     float tt = v.w / rc;
     float fc = v.w < rc ? 0.5f * (cospi_(tt) + 1.0f) : 0.0f;
     float dfc = v.w < rc ? -0.5f * 3.14159265358979f / rc * sinpi_(tt) : 0.0f;
-    rec[NREC * t + 0] = v;
-    rec[NREC * t + 1] = make_float4(1.0f / v.w, fc, dfc, 0.0f);
-    float A[8];
     float r2 = v.w * v.w;
+    rec[R * t + 0] = v;
+    rec[R * t + 1] = make_float4(__builtin_amdgcn_rcpf(v.w), fc, dfc, r2);
+    if (!UNI) {
+      float A[8];
 #pragma unroll
-    for (int e = 0; e < 8; e++) A[e] = e < NE ? fast_exp2(-(float)MG_LOG2E * bp->eta4[p][e] * r2) : 0.0f;
-    rec[NREC * t + 2] = make_float4(A[0], A[1], A[2], A[3]);
-    rec[NREC * t + 3] = make_float4(A[4], A[5], A[6], A[7]);
+      for (int e = 0; e < 8; e++) A[e] = e < NE ? fast_exp2(-(float)MG_LOG2E * bp->eta4[p][e] * r2) : 0.0f;
+      rec[R * t + 2] = make_float4(A[0], A[1], A[2], A[3]);
+      rec[R * t + 3] = make_float4(A[4], A[5], A[6], A[7]);
+    }
   }
 }
 
@@ -403,7 +424,7 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
 // side as the lane-fast index.  Passing pairs are compacted into an LDS list (ballot + lane prefix count) and
 // processed with all 64 lanes busy; about half of the enumerated pairs fail the r_jk test in bulk metals.
 #define PCH 512
-template <typename F>
+template <int R, typename F>
 __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bool tri, int oa, int ma, int ob, int mb,
                                           float rc2, int lane, F&& body) {
   int mf, ms;  // fast / slow extents
@@ -436,7 +457,7 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
         jj = oa + sl; kk = ob + f;
       }
       if (use) {
-        float4 a = rec[NREC * jj], b = rec[NREC * kk];
+        float4 a = rec[R * jj], b = rec[R * kk];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         ok = cx * cx + cy * cy + cz * cz < rc2;
         pk = (unsigned)jj | ((unsigned)kk << 16);
@@ -464,12 +485,13 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
 }
 
 // ---------------------------------------------------------------------------------------------- descriptors
-template <int PAT>
+template <int PAT, bool UNI>
 __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                              const int* __restrict__ cnt, const float4* __restrict__ vec,
                                              const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
                                              float* __restrict__ erep, float urep, int cap) {
   constexpr int NE = Pat<PAT>::NE;
+  constexpr int R = UNI ? 2 : 4;
   extern __shared__ float4 smem_d[];
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
@@ -477,8 +499,8 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
   RowInfo ri;
   row_info(ri, i, spc, row_off, cnt, S);
   float4* rec = smem_d;
-  unsigned* plist = (unsigned*)(smem_d + NREC * cap);
-  stage_row<NE>(rec, ri, vec, bp, lane, S);
+  unsigned* plist = (unsigned*)(smem_d + R * cap);
+  stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
   __syncthreads();
   float* Gi = G + (size_t)i * D;
 
@@ -496,8 +518,8 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
     for (int c = 0; c < MG_MAXG2; c++) acc[c] = 0.0f;
     const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[s];
     for (int t = ri.o(s) + lane; t < ri.o(s) + ri.m(s); t += WAVE) {
-      float4 v = rec[NREC * t];
-      float fc = rec[NREC * t + 1].y;
+      float4 v = rec[R * t];
+      float fc = rec[R * t + 1].y;
       float r = v.w;
 #pragma unroll
       for (int c = 0; c < MG_MAXG2; c++)
@@ -534,28 +556,35 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
 #pragma unroll
       for (int q = 0; q < MG_GRID; q++) V[q] = 0.0f;
       auto body = [&](int j, int k) {
-        float4 a = rec[NREC * j], b = rec[NREC * k];
+        float4 a = rec[R * j], b = rec[R * k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
-        float4 ia = rec[NREC * j + 1], ib = rec[NREC * k + 1];
+        float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
         float r3 = __builtin_sqrtf(r3s);
         float fc3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * ia.x * ib.x;
         float F = 2.0f * ia.y * ib.y * fc3;
-        float4 Aj0 = rec[NREC * j + 2], Aj1 = rec[NREC * j + 3], Ak0 = rec[NREC * k + 2], Ak1 = rec[NREC * k + 3];
-        float Aj[8] = {Aj0.x, Aj0.y, Aj0.z, Aj0.w, Aj1.x, Aj1.y, Aj1.z, Aj1.w};
-        float Ak[8] = {Ak0.x, Ak0.y, Ak0.z, Ak0.w, Ak1.x, Ak1.y, Ak1.z, Ak1.w};
         float P[MG_MAXZ * MG_MAXL];
         angular<PAT>(bp, cs, P, nullptr);
+        float Bq[NE];
+        if (UNI) {
+          const float rs = ia.w + ib.w + r3s;
 #pragma unroll
-        for (int e = 0; e < NE; e++) {
-          float B = F * Aj[e] * Ak[e] * fast_exp2(ejk[e] * r3s);
+          for (int e = 0; e < NE; e++) Bq[e] = F * fast_exp2(ejk[e] * rs);
+        } else {
+          float4 Aj0 = rec[R * j + 2], Aj1 = rec[R * j + 3], Ak0 = rec[R * k + 2], Ak1 = rec[R * k + 3];
+          float Aj[8] = {Aj0.x, Aj0.y, Aj0.z, Aj0.w, Aj1.x, Aj1.y, Aj1.z, Aj1.w};
+          float Ak[8] = {Ak0.x, Ak0.y, Ak0.z, Ak0.w, Ak1.x, Ak1.y, Ak1.z, Ak1.w};
+#pragma unroll
+          for (int e = 0; e < NE; e++) Bq[e] = F * Aj[e] * Ak[e] * fast_exp2(ejk[e] * r3s);
+        }
+#pragma unroll
+        for (int e = 0; e < NE; e++)
 #pragma unroll
           for (int q = 0; q < MG_MAXZ * MG_MAXL; q++)
-            if (Pat<PAT>::mask(e) & (1u << q)) V[e * MG_MAXZ * MG_MAXL + q] += B * P[q];
-        }
+            if (Pat<PAT>::mask(e) & (1u << q)) V[e * MG_MAXZ * MG_MAXL + q] += Bq[e] * P[q];
       };
-      for_pairs(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
+      for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
       float mine = reduce_scatter64(V, lane);  // lane l: grid cell l
       int cell = lane < bp->ng4 ? bp->g4cell[lane] : 0;
       float v = __shfl(mine, cell, WAVE);
@@ -629,13 +658,17 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 //   dT/dr1 = -2 r1 F sum_e e_ij S_e B_e + f1' f2 f3 sum_e S_e B_e (r2, r3 likewise), dT/dcos = F sum_e S'_e B_e,
 //   dT/da = dT/dr1 a/r1 - dT/dr3 c/r3 + dT/dcos (b/(r1 r2) - cos a/r1^2), dT/db symmetric.
 // Forces: F_j -= dT/da, F_k -= dT/db, F_i += dT/da + dT/db; virial U -= dT/da (x) a + dT/db (x) b.
-template <int PAT>
+// The dE/dG grid g(e,z,l) of the current species block is wave-uniform; it sits in LDS and is read with
+// broadcast loads (kept out of SGPRs, which would spill inside the triplet loop).
+template <int PAT, bool UNI>
 __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                               const int* __restrict__ cnt, const float4* __restrict__ vec,
                                               const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
                                               float4* __restrict__ fself, float4* __restrict__ fe,
                                               float* __restrict__ vir, float urep, int cap) {
   constexpr int NE = Pat<PAT>::NE;
+  constexpr int R = UNI ? 2 : 4;
+  constexpr int ZL = MG_MAXZ * MG_MAXL;
   extern __shared__ float4 smem_f[];
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
@@ -643,9 +676,10 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
   RowInfo ri;
   row_info(ri, i, spc, row_off, cnt, S);
   float4* rec = smem_f;
-  float* fnb = (float*)(smem_f + NREC * cap);     // [cap][4]
+  float4* gts4 = smem_f + R * cap;               // [MG_GRID / 4]
+  float* fnb = (float*)(gts4 + MG_GRID / 4);     // [cap][4]
   unsigned* plist = (unsigned*)(fnb + 4 * cap);  // [PCH]
-  stage_row<NE>(rec, ri, vec, bp, lane, S);
+  stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
   for (int t = lane; t < ri.M; t += WAVE) {
     fnb[4 * t] = 0.0f; fnb[4 * t + 1] = 0.0f; fnb[4 * t + 2] = 0.0f; fnb[4 * t + 3] = 0.0f;
   }
@@ -661,8 +695,8 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
     const int blk = bp->ind2[(s - ri.si + S) % S];
     const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[s];
     for (int t = ri.o(s) + lane; t < ri.o(s) + ri.m(s); t += WAVE) {
-      float4 v = rec[NREC * t];
-      float4 w = rec[NREC * t + 1];
+      float4 v = rec[R * t];
+      float4 w = rec[R * t + 1];
       float r = v.w, fc = w.y, dfc = w.z;
       float dT = 0.0f;
 #pragma unroll
@@ -674,11 +708,7 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
           dT += gi[blk + c] * ex * (-2.0f * eta * dr * fc + dfc);
         }
       float sc = dT * w.x;
-      float gx = sc * v.x, gy = sc * v.y, gz = sc * v.z;
-      fnb[4 * t] -= gx; fnb[4 * t + 1] -= gy; fnb[4 * t + 2] -= gz;
-      fx += gx; fy += gy; fz += gz;
-      u0 -= gx * v.x; u1 -= gy * v.y; u2 -= gz * v.z;
-      u3 -= 0.5f * (gx * v.y + gy * v.x); u4 -= 0.5f * (gy * v.z + gz * v.y); u5 -= 0.5f * (gz * v.x + gx * v.z);
+      fnb[4 * t] -= sc * v.x; fnb[4 * t + 1] -= sc * v.y; fnb[4 * t + 2] -= sc * v.z;
       if (urep > 0.0f && r < Rrep) {
         // per ordered pair, acting on atom i only, as CELL_UREP (src/cmod.c:95-116)
         float FC = 0.5f * (1.0f + cospi_(r / Rrep));
@@ -700,58 +730,64 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
       const float rc = bp->rc[pjk];
       const float rinv = 1.0f / rc;
       const float dfk = -0.5f * 3.14159265358979f * rinv;
-      float ejk[NE], eij[NE], eik[NE], e3[NE];
-#pragma unroll
-      for (int e = 0; e < NE; e++) {
-        ejk[e] = -(float)MG_LOG2E * bp->eta4[pjk][e];
-        e3[e] = bp->eta4[pjk][e];
-        eij[e] = bp->eta4[pij][e];
-        eik[e] = bp->eta4[pik][e];
+      {
+        int n = bp->g4map[lane];
+        ((float*)gts4)[lane] = n >= 0 ? gi[blk + n] : 0.0f;
       }
-      float gt[NE * MG_MAXZ * MG_MAXL];
-#pragma unroll
-      for (int q = 0; q < NE * MG_MAXZ * MG_MAXL; q++) {
-        if (Pat<PAT>::mask(q / (MG_MAXZ * MG_MAXL)) & (1u << (q % (MG_MAXZ * MG_MAXL)))) {
-          int n = bp->g4map[q];
-          gt[q] = n >= 0 ? gi[blk + n] : 0.0f;
-        } else {
-          gt[q] = 0.0f;
-        }
-      }
+      __syncthreads();
       auto body = [&](int j, int k) {
-        float4 a = rec[NREC * j], b = rec[NREC * k];
+        float4 a = rec[R * j], b = rec[R * k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
-        float4 ia = rec[NREC * j + 1], ib = rec[NREC * k + 1];
+        float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
         float r3 = __builtin_sqrtf(r3s);
-        float i3 = 1.0f / r3;
+        float i3 = __builtin_amdgcn_rcpf(r3);
         float f3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
         float d3 = dfk * sinpi_(r3 * rinv);
         float i1 = ia.x, i2 = ib.x, f1 = ia.y, f2 = ib.y, d1 = ia.z, d2 = ib.z;
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * i1 * i2;
-        float4 Aj0 = rec[NREC * j + 2], Aj1 = rec[NREC * j + 3], Ak0 = rec[NREC * k + 2], Ak1 = rec[NREC * k + 3];
-        float Aj[8] = {Aj0.x, Aj0.y, Aj0.z, Aj0.w, Aj1.x, Aj1.y, Aj1.z, Aj1.w};
-        float Ak[8] = {Ak0.x, Ak0.y, Ak0.z, Ak0.w, Ak1.x, Ak1.y, Ak1.z, Ak1.w};
-        float P[MG_MAXZ * MG_MAXL], Pd[MG_MAXZ * MG_MAXL];
+        float P[ZL], Pd[ZL];
         angular<PAT>(bp, cs, P, Pd);
         float X = 0.f, Y1 = 0.f, Y2 = 0.f, Y3 = 0.f, Zc = 0.f;
+        float Aj[8], Ak[8];
+        if (!UNI) {
+          float4 Aj0 = rec[R * j + 2], Aj1 = rec[R * j + 3], Ak0 = rec[R * k + 2], Ak1 = rec[R * k + 3];
+          Aj[0] = Aj0.x; Aj[1] = Aj0.y; Aj[2] = Aj0.z; Aj[3] = Aj0.w; Aj[4] = Aj1.x; Aj[5] = Aj1.y; Aj[6] = Aj1.z; Aj[7] = Aj1.w;
+          Ak[0] = Ak0.x; Ak[1] = Ak0.y; Ak[2] = Ak0.z; Ak[3] = Ak0.w; Ak[4] = Ak1.x; Ak[5] = Ak1.y; Ak[6] = Ak1.z; Ak[7] = Ak1.w;
+        }
+        const float rsum = ia.w + ib.w + r3s;
 #pragma unroll
         for (int e = 0; e < NE; e++) {
-          float B = 2.0f * Aj[e] * Ak[e] * fast_exp2(ejk[e] * r3s);
+          const float eta3 = bp->eta4[pjk][e];
+          float B = UNI ? 2.0f * fast_exp2(-(float)MG_LOG2E * eta3 * rsum)
+                        : 2.0f * Aj[e] * Ak[e] * fast_exp2(-(float)MG_LOG2E * eta3 * r3s);
           float Se = 0.f, Sd = 0.f;
+          if (Pat<PAT>::mask(e) & 0x0Fu) {
+            const float4 g = gts4[e * 2];
+            const float gg[4] = {g.x, g.y, g.z, g.w};
 #pragma unroll
-          for (int q = 0; q < MG_MAXZ * MG_MAXL; q++)
-            if (Pat<PAT>::mask(e) & (1u << q)) {
-              Se += gt[e * MG_MAXZ * MG_MAXL + q] * P[q];
-              Sd += gt[e * MG_MAXZ * MG_MAXL + q] * Pd[q];
-            }
+            for (int q = 0; q < 4; q++)
+              if (Pat<PAT>::mask(e) & (1u << q)) { Se += gg[q] * P[q]; Sd += gg[q] * Pd[q]; }
+          }
+          if (Pat<PAT>::mask(e) & 0xF0u) {
+            const float4 g = gts4[e * 2 + 1];
+            const float gg[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              if (Pat<PAT>::mask(e) & (1u << (q + 4))) { Se += gg[q] * P[q + 4]; Sd += gg[q] * Pd[q + 4]; }
+          }
           float SB = Se * B;
           X += SB;
-          Y1 += eij[e] * SB;
-          Y2 += eik[e] * SB;
-          Y3 += e3[e] * SB;
+          if (UNI) {
+            Y1 += eta3 * SB;
+          } else {
+            Y1 += bp->eta4[pij][e] * SB;
+            Y2 += bp->eta4[pik][e] * SB;
+            Y3 += eta3 * SB;
+          }
           Zc += Sd * B;
         }
+        if (UNI) { Y2 = Y1; Y3 = Y1; }
         float r1 = a.w, r2 = b.w;
         float Fp = f1 * f2 * f3;
         float dT1 = -2.0f * r1 * Fp * Y1 + d1 * f2 * f3 * X;
@@ -767,16 +803,21 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
         float gbx = cbb * b.x + cb * a.x + c3c * cx, gby = cbb * b.y + cb * a.y + c3c * cy, gbz = cbb * b.z + cb * a.z + c3c * cz;
         atomicAdd(&fnb[4 * j], -gax); atomicAdd(&fnb[4 * j + 1], -gay); atomicAdd(&fnb[4 * j + 2], -gaz);
         atomicAdd(&fnb[4 * k], -gbx); atomicAdd(&fnb[4 * k + 1], -gby); atomicAdd(&fnb[4 * k + 2], -gbz);
-        fx += gax + gbx; fy += gay + gby; fz += gaz + gbz;
-        u0 -= gax * a.x + gbx * b.x;
-        u1 -= gay * a.y + gby * b.y;
-        u2 -= gaz * a.z + gbz * b.z;
-        u3 -= 0.5f * (gax * a.y + gay * a.x + gbx * b.y + gby * b.x);
-        u4 -= 0.5f * (gay * a.z + gaz * a.y + gby * b.z + gbz * b.y);
-        u5 -= 0.5f * (gaz * a.x + gax * a.z + gbz * b.x + gbx * b.z);
       };
-      for_pairs(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
+      for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
+      __syncthreads();
     }
+  // every term of T_i depends on the neighbour vectors d_e only, and fnb[e] = -dT_i/dd_e summed over them: so
+  // F_i = -sum_e fnb[e] and the virial U = -sum_e dT/dd_e (x) d_e = sum_e fnb[e] (x) d_e (symmetric)
+  __syncthreads();
+  for (int t = lane; t < ri.M; t += WAVE) {
+    const float gx = fnb[4 * t], gy = fnb[4 * t + 1], gz = fnb[4 * t + 2];
+    const float4 v = rec[R * t];
+    fx -= gx; fy -= gy; fz -= gz;
+    u0 += gx * v.x; u1 += gy * v.y; u2 += gz * v.z;
+    u3 += 0.5f * (gx * v.y + gy * v.x); u4 += 0.5f * (gy * v.z + gz * v.y); u5 += 0.5f * (gz * v.x + gx * v.z);
+    fe[ri.off + t] = make_float4(gx, gy, gz, 0.0f);
+  }
   fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
   u0 = wave_sum(u0); u1 = wave_sum(u1); u2 = wave_sum(u2);
   u3 = wave_sum(u3); u4 = wave_sum(u4); u5 = wave_sum(u5);
@@ -785,32 +826,42 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
     float* vv = vir + 6 * (size_t)i;
     vv[0] = u0; vv[1] = u1; vv[2] = u2; vv[3] = u3; vv[4] = u4; vv[5] = u5;
   }
-  __syncthreads();
-  for (int t = lane; t < ri.M; t += WAVE) fe[ri.off + t] = make_float4(fnb[4 * t], fnb[4 * t + 1], fnb[4 * t + 2], 0.0f);
 }
 
-__global__ void k_gather(int N, const int* __restrict__ row_off, const int* __restrict__ rev,
-                         const float4* __restrict__ fself, const float4* __restrict__ fe, double* __restrict__ F) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= N) return;
-  float4 s = fself[a];
-  double fx = s.x, fy = s.y, fz = s.z;
-  for (int e = row_off[a]; e < row_off[a + 1]; e++) {
-    float4 v = fe[rev[e]];
-    fx += v.x; fy += v.y; fz += v.z;
+// F_a = own term + sum of the reverse edges' neighbour terms; 4 lanes per atom, fixed order, FP64 accumulation
+__global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ row_off, const int* __restrict__ rev,
+                                                const float4* __restrict__ fself, const float4* __restrict__ fe,
+                                                double* __restrict__ F) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = g >> 2, sub = g & 3;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (a < N) {
+    for (int e = row_off[a] + sub; e < row_off[a + 1]; e += 4) {
+      float4 v = fe[rev[e]];
+      fx += v.x; fy += v.y; fz += v.z;
+    }
   }
-  F[3 * a] = fx; F[3 * a + 1] = fy; F[3 * a + 2] = fz;
+  fx += __shfl_xor(fx, 1, 4); fy += __shfl_xor(fy, 1, 4); fz += __shfl_xor(fz, 1, 4);
+  fx += __shfl_xor(fx, 2, 4); fy += __shfl_xor(fy, 2, 4); fz += __shfl_xor(fz, 2, 4);
+  if (a < N && sub == 0) {
+    float4 s = fself[a];
+    F[3 * a] = fx + s.x; F[3 * a + 1] = fy + s.y; F[3 * a + 2] = fz + s.z;
+  }
 }
 
-// one block of 256 per structure; FP64, fixed summation order
-__global__ __launch_bounds__(256) void k_reduce(int nstr, const mg_struct_dev* __restrict__ str, const float* __restrict__ Ei,
-                                                const float* __restrict__ erep, const float* __restrict__ vir, int have_vir,
-                                                double* __restrict__ Eout, double* __restrict__ Uout) {
+// Per-structure FP64 energy and virial in two fixed-order passes: chunk partials, then a sum over the chunks.
+__global__ __launch_bounds__(256) void k_reduce_part(int nchunk, const int* __restrict__ chunk_str,
+                                                     const mg_struct_dev* __restrict__ str, const float* __restrict__ Ei,
+                                                     const float* __restrict__ erep, const float* __restrict__ vir,
+                                                     int have_vir, double* __restrict__ part) {
   __shared__ double sh[7][256];
-  const int s = blockIdx.x, t = threadIdx.x;
-  const int a0 = str[s].atom0, n = str[s].natom;
+  const int ch = blockIdx.x, t = threadIdx.x;
+  const mg_struct_dev* st = &str[chunk_str[ch]];
+  const int a0 = st->atom0 + (ch - st->chunk0) * MG_RCHUNK;
+  int a1 = a0 + MG_RCHUNK;
+  if (a1 > st->atom0 + st->natom) a1 = st->atom0 + st->natom;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int a = a0 + t; a < a0 + n; a += 256) {
+  for (int a = a0 + t; a < a1; a += 256) {
     acc[0] += (double)Ei[a] + (erep ? (double)erep[a] : 0.0);
     if (have_vir)
       for (int q = 0; q < 6; q++) acc[1 + q] += (double)vir[6 * (size_t)a + q];
@@ -822,10 +873,18 @@ __global__ __launch_bounds__(256) void k_reduce(int nstr, const mg_struct_dev* _
       for (int q = 0; q < 7; q++) sh[q][t] += sh[q][t + w];
     __syncthreads();
   }
-  if (t == 0) {
-    Eout[s] = sh[0][0];
-    for (int q = 0; q < 6; q++) Uout[6 * s + q] = sh[1 + q][0];
-  }
+  if (t < 7) part[7 * (size_t)ch + t] = sh[t][0];
+}
+
+__global__ void k_reduce_final(int nstr, const mg_struct_dev* __restrict__ str, const double* __restrict__ part,
+                               double* __restrict__ Eout, double* __restrict__ Uout) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nstr) return;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int c = str[s].chunk0; c < str[s].chunk0 + str[s].nchunk; c++)
+    for (int q = 0; q < 7; q++) acc[q] += part[7 * (size_t)c + q];
+  Eout[s] = acc[0];
+  for (int q = 0; q < 6; q++) Uout[6 * s + q] = acc[1 + q];
 }
 
 // ---------------------------------------------------------------------------------------------- launchers
@@ -852,44 +911,52 @@ int mgk_prepare(int N, const double* xin, const int* atom_str, const mg_struct_d
 int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_cur, int* bin_atoms, hipStream_t st) {
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_bin_fill, dim3((N + 255) / 256), dim3(256), 0, st, N, bin, bin_start, bin_cur, bin_atoms);
-  hipLaunchKernelGGL(k_bin_sort, dim3((nbins + 255) / 256), dim3(256), 0, st, nbins, bin_start, bin_atoms);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
                   const int* bin_start, const int* bin_atoms, int* cnt, int* rowcnt, int* maxrow, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_nbr<false>, dim3((N + 127) / 128), dim3(128), 0, st, N, x, spc, atom_str, str, bin, bin_start,
-                     bin_atoms, cnt, rowcnt, maxrow, (const int*)nullptr, (u64*)nullptr, (float4*)nullptr);
+  hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, st, N, x, spc, atom_str, str, bin, bin_start, bin_atoms, cnt,
+                     rowcnt, maxrow, (const int*)nullptr, (u64*)nullptr, (float4*)nullptr, 0);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Fill and sort the rows (LDS tile of cap entries: a power of two >= the longest row) and build the reverse-edge
+// index.  Rows longer than 2048 neighbours are rejected (-2).
 int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
-                 const int* bin_start, const int* bin_atoms, const int* row_off, void* key_u, void* vec_u,
-                 void* key, void* vec, int* rev, int* err, int maxrow, hipStream_t st) {
+                 const int* bin_start, const int* bin_atoms, const int* row_off, void* key, void* vec, int* rev,
+                 int* err, int maxrow, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_nbr<true>, dim3((N + 127) / 128), dim3(128), 0, st, N, x, spc, atom_str, str, bin, bin_start,
-                     bin_atoms, (int*)nullptr, (int*)nullptr, (int*)nullptr, row_off, (u64*)key_u, (float4*)vec_u);
   int cap = 64;
-  while (cap < maxrow && cap < 1024) cap <<= 1;
-  size_t sm = (size_t)cap * (sizeof(u64) + sizeof(int));
-  hipLaunchKernelGGL(k_row_sort, dim3(N), dim3(64), sm, st, N, row_off, (const u64*)key_u, (const float4*)vec_u,
-                     (u64*)key, (float4*)vec, cap);
+  while (cap < maxrow) cap <<= 1;
+  if (cap > 2048) return -2;
+  size_t sm = (size_t)cap * (sizeof(u64) + sizeof(float4) + sizeof(int));
+  hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), sm, st, N, x, spc, atom_str, str, bin, bin_start, bin_atoms,
+                     (int*)nullptr, (int*)nullptr, (int*)nullptr, row_off, (u64*)key, (float4*)vec, cap);
   hipLaunchKernelGGL(k_rev, dim3(N), dim3(64), 0, st, N, spc, row_off, (const u64*)key, rev, err);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int mgk_desc(int N, int S, int pat, const int* spc, const int* row_off, const int* cnt, const void* vec,
+#define LAUNCH_PU(KER, PAT, UNI, SM, ...)                                                                   \
+  hipLaunchKernelGGL((KER<PAT, UNI>), dim3(N), dim3(64), SM, st, __VA_ARGS__)
+#define DISPATCH_PU(KER, SM, ...)                                  \
+  do {                                                             \
+    const int pat_ = variant & 15;                                 \
+    const bool uni_ = (variant >> 4) & 1;                          \
+    if (pat_ == 1) { if (uni_) LAUNCH_PU(KER, 1, true, SM, __VA_ARGS__); else LAUNCH_PU(KER, 1, false, SM, __VA_ARGS__); } \
+    else if (pat_ == 2) { if (uni_) LAUNCH_PU(KER, 2, true, SM, __VA_ARGS__); else LAUNCH_PU(KER, 2, false, SM, __VA_ARGS__); } \
+    else { if (uni_) LAUNCH_PU(KER, 0, true, SM, __VA_ARGS__); else LAUNCH_PU(KER, 0, false, SM, __VA_ARGS__); } \
+  } while (0)
+
+// variant = G4 pattern (bits 0-3) | shared-eta flag (bit 4)
+int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
              const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, hipStream_t st) {
   if (N == 0) return 0;
-  int cap = maxrow < 1 ? 1 : maxrow;
-  size_t sm = (size_t)cap * NREC * sizeof(float4) + PCH * sizeof(unsigned);
-  if (pat == 1)
-    hipLaunchKernelGGL(k_desc<1>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
-  else if (pat == 2)
-    hipLaunchKernelGGL(k_desc<2>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
-  else
-    hipLaunchKernelGGL(k_desc<0>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
+  const int cap = maxrow < 1 ? 1 : maxrow;
+  const int R = ((variant >> 4) & 1) ? 2 : 4;
+  size_t sm = (size_t)cap * R * sizeof(float4) + PCH * sizeof(unsigned);
+  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -901,36 +968,235 @@ int mgk_mlp(int N, const int* spc, const float* G, const float* W, int wstride, 
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int mgk_force(int N, int S, int pat, const int* spc, const int* row_off, const int* cnt, const void* vec,
+int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
               const mg_basis_dev* bp, const float* dEdG, void* fself, void* fe, float* vir, float urep, int maxrow,
               hipStream_t st) {
   if (N == 0) return 0;
-  int cap = maxrow < 1 ? 1 : maxrow;
-  size_t sm = (size_t)cap * (NREC + 1) * sizeof(float4) + PCH * sizeof(unsigned);
-  if (pat == 1)
-    hipLaunchKernelGGL(k_force<1>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG,
-                       (float4*)fself, (float4*)fe, vir, urep, cap);
-  else if (pat == 2)
-    hipLaunchKernelGGL(k_force<2>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG,
-                       (float4*)fself, (float4*)fe, vir, urep, cap);
-  else
-    hipLaunchKernelGGL(k_force<0>, dim3(N), dim3(64), sm, st, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG,
-                       (float4*)fself, (float4*)fe, vir, urep, cap);
+  const int cap = maxrow < 1 ? 1 : maxrow;
+  const int R = ((variant >> 4) & 1) ? 2 : 4;
+  size_t sm = (size_t)cap * (R + 1) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
+  DISPATCH_PU(k_force, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
+              urep, cap);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int mgk_gather(int N, const int* row_off, const int* rev, const void* fself, const void* fe, double* F, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_gather, dim3((N + 255) / 256), dim3(256), 0, st, N, row_off, rev, (const float4*)fself,
+  hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rev, (const float4*)fself,
                      (const float4*)fe, F);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int mgk_reduce(int nstr, const mg_struct_dev* str, const float* Ei, const float* erep, const float* vir, int have_vir,
-               double* E, double* U, hipStream_t st) {
+int mgk_reduce(int nstr, int nchunk, const int* chunk_str, const mg_struct_dev* str, const float* Ei, const float* erep,
+               const float* vir, int have_vir, double* part, double* E, double* U, hipStream_t st) {
   if (nstr == 0) return 0;
-  hipLaunchKernelGGL(k_reduce, dim3(nstr), dim3(256), 0, st, nstr, str, Ei, erep, vir, have_vir, E, U);
+  hipLaunchKernelGGL(k_reduce_part, dim3(nchunk), dim3(256), 0, st, nchunk, chunk_str, str, Ei, erep, vir, have_vir, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3((nstr + 255) / 256), dim3(256), 0, st, nstr, str, part, E, U);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------------- MLP on MFMA
+// One wave per tile of up to 32 atoms of one species (atoms reached through a species-sorted permutation).
+// Layers are GEMMs with hidden units as rows and atoms as columns: out[u][atom] = W[u][k] x[k][atom], on
+// v_mfma_f32_32x32x2_f32 (FP32) or v_mfma_f32_32x32x16_bf16 (BF16 inputs, FP32 accumulation).
+// Accumulator layout (32x32 blocks): lane l, register r holds row 8(r/4) + 4(l/32) + r%4, column l%32.  Each
+// lane therefore keeps all hidden units of one atom spread over its registers, and those registers are fed
+// straight back as the next layer's B operand; the host packs the weight fragments (A operand) in the same
+// k order, so no data movement is needed between layers.  Forward tanh layers, E_i = W2.h2 + b2, and the
+// backward chain dE/dG = W0'^T[(W1^T[W2 (1-h2^2)]) (1-h1^2)] (src/nmlp.c:367-431 and the chain of 433-478).
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ int acc_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// PREC 0: FP32 MFMA (K=2).  PREC 1: BF16 MFMA (K=16), one pass.  PREC 2: compensated BF16 with split operands
+// a = a_hi + a_lo, b = b_hi + b_lo (each a BF16 value) and three passes a_lo b_hi + a_hi b_lo + a_hi b_hi into the
+// FP32 accumulator: single-pass BF16 rounds the weights to 2^-9 relative, which alone moves energies by ~1e-3
+// eV/atom on the 51-10-10-1 workload; the split keeps the error near FP32 (~1e-6 eV/atom).
+template <int PREC>
+__device__ __forceinline__ f32x16 mfma_step(const float* __restrict__ afrag, const __bf16* __restrict__ afrag_bf,
+                                            const __bf16* __restrict__ afrag_lo, int idx, const float* bvals,
+                                            f32x16 acc) {
+  if (PREC == 0) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(afrag[idx], bvals[0], acc, 0, 0, 0);
+  } else {
+    const bf16x8 a = *(const bf16x8*)(afrag_bf + 8 * (size_t)idx);
+    bf16x8 b;
+#pragma unroll
+    for (int v = 0; v < 8; v++) b[v] = (__bf16)bvals[v];
+    if (PREC == 2) {
+      const bf16x8 al = *(const bf16x8*)(afrag_lo + 8 * (size_t)idx);
+      bf16x8 bl;
+#pragma unroll
+      for (int v = 0; v < 8; v++) bl[v] = (__bf16)(bvals[v] - (float)b[v]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl, acc, 0, 0, 0);
+    }
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+}
+
+// Fragment offsets (in A-operand slots of 64 lanes; BF16 slots hold 8 values) of one species, see mg_model.c
+struct MlpFrag {
+  int D, H1, H2, hb1, hb2, db, k1;   // k1: layer-1 steps
+  int o_l1, o_l2, o_b1, o_g0;        // slot offsets of the four A-operand sets
+  int o_vec;                          // float offset of b0[hb1*32], b1[hb2*32], w2[hb2*32], b2, xoff[D]
+  int stride_slots, stride_vec;       // per-species strides
+};
+
+template <int PREC, int HB1, int HB2>
+__global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles, const int* __restrict__ perm,
+                                                 const float* __restrict__ G, const float* __restrict__ frag,
+                                                 const __bf16* __restrict__ frag_bf, const __bf16* __restrict__ frag_lo,
+                                                 const float* __restrict__ vecs, MlpFrag f, float* __restrict__ Ei,
+                                                 float* __restrict__ dEdG) {
+  constexpr bool BF16 = PREC != 0;
+  constexpr int KS = BF16 ? 8 : 1;   // B values per lane per step
+  const int4 tl = tiles[blockIdx.x];
+  const int sp = tl.x, start = tl.y, count = tl.z;
+  const int lane = threadIdx.x, col = lane & 31, h = lane >> 5;
+  const int atom = col < count ? perm[start + col] : -1;
+  const int D = f.D;
+  const size_t so = (size_t)sp * f.stride_slots;
+  const float* fa = frag + so * 64;
+  const __bf16* fb = frag_bf + so * 64 * 8;
+  const __bf16* fl = frag_lo + so * 64 * 8;
+  const float* vv = vecs + (size_t)sp * f.stride_vec;
+  const float* b0 = vv;
+  const float* b1 = b0 + HB1 * 32;
+  const float* w2 = b1 + HB2 * 32;
+  const float* xo = w2 + HB2 * 32 + 1;
+  const float b2 = w2[HB2 * 32];
+  const float* g = atom >= 0 ? G + (size_t)atom * D : nullptr;
+
+  // ---- layer 1: a1[u][atom] = b0 + W0' x
+  f32x16 a1[HB1];
+#pragma unroll
+  for (int rb = 0; rb < HB1; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) a1[rb][r] = b0[rb * 32 + acc_row(r, h)];
+  for (int st = 0; st < f.k1; st++) {
+    float xb[KS];
+#pragma unroll
+    for (int v = 0; v < KS; v++) {
+      const int k = BF16 ? 16 * st + 8 * h + v : 2 * st + h;
+      xb[v] = (g && k < D) ? g[k] - xo[k] : 0.0f;
+    }
+#pragma unroll
+    for (int rb = 0; rb < HB1; rb++) a1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l1 * 64 + (rb * f.k1 + st) * 64 + lane, xb, a1[rb]);
+  }
+#pragma unroll
+  for (int rb = 0; rb < HB1; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) a1[rb][r] = tanh_f(a1[rb][r]);
+
+  // ---- layer 2: a2 = tanh(b1 + W1 h1); energy; delta2 = W2 (1 - h2^2)
+  constexpr int S2 = BF16 ? 2 : 16;  // steps per 32-unit block
+  f32x16 a2[HB2];
+#pragma unroll
+  for (int rb = 0; rb < HB2; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) a2[rb][r] = b1[rb * 32 + acc_row(r, h)];
+#pragma unroll
+  for (int rb1 = 0; rb1 < HB1; rb1++)
+#pragma unroll
+    for (int q = 0; q < S2; q++) {
+      float xb[KS];
+#pragma unroll
+      for (int v = 0; v < KS; v++) xb[v] = a1[rb1][BF16 ? 8 * q + v : q];
+      const int st = rb1 * S2 + q;
+#pragma unroll
+      for (int rb = 0; rb < HB2; rb++)
+        a2[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l2 * 64 + (rb * HB1 * S2 + st) * 64 + lane, xb, a2[rb]);
+    }
+  float e = 0.0f;
+#pragma unroll
+  for (int rb = 0; rb < HB2; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const float w = w2[rb * 32 + acc_row(r, h)];
+      const float t = tanh_f(a2[rb][r]);
+      e += w * t;
+      a2[rb][r] = w * (1.0f - t * t);  // delta2
+    }
+  e += __shfl_xor(e, 32, WAVE);
+  if (atom >= 0 && h == 0) Ei[atom] = e + b2;
+
+  // ---- delta1 = (W1^T delta2) (1 - h1^2)
+  f32x16 d1[HB1];
+#pragma unroll
+  for (int rb = 0; rb < HB1; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) d1[rb][r] = 0.0f;
+#pragma unroll
+  for (int rb2 = 0; rb2 < HB2; rb2++)
+#pragma unroll
+    for (int q = 0; q < S2; q++) {
+      float xb[KS];
+#pragma unroll
+      for (int v = 0; v < KS; v++) xb[v] = a2[rb2][BF16 ? 8 * q + v : q];
+      const int st = rb2 * S2 + q;
+#pragma unroll
+      for (int rb = 0; rb < HB1; rb++)
+        d1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_b1 * 64 + (rb * HB2 * S2 + st) * 64 + lane, xb, d1[rb]);
+    }
+#pragma unroll
+  for (int rb = 0; rb < HB1; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) d1[rb][r] *= 1.0f - a1[rb][r] * a1[rb][r];
+
+  // ---- dE/dG = W0'^T delta1, 32 features per block
+  float* out = atom >= 0 ? dEdG + (size_t)atom * D : nullptr;
+  for (int rbd = 0; rbd < f.db; rbd++) {
+    f32x16 gacc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) gacc[r] = 0.0f;
+#pragma unroll
+    for (int rb1 = 0; rb1 < HB1; rb1++)
+#pragma unroll
+      for (int q = 0; q < S2; q++) {
+        float xb[KS];
+#pragma unroll
+        for (int v = 0; v < KS; v++) xb[v] = d1[rb1][BF16 ? 8 * q + v : q];
+        gacc = mfma_step<PREC>(fa, fb, fl, f.o_g0 * 64 + (rbd * HB1 * S2 + rb1 * S2 + q) * 64 + lane, xb, gacc);
+      }
+    if (out)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int k = rbd * 32 + acc_row(r, h);
+        if (k < D) out[k] = gacc[r];
+      }
+  }
+}
+
+extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, const float* G, const float* frag,
+                            const void* frag_bf, const void* frag_lo, const float* vecs, const int* fdesc, int prec,
+                            float* Ei, float* dEdG, hipStream_t st) {
+  if (ntiles == 0) return 0;
+  const int bf16 = prec != 0;
+  MlpFrag f;
+  f.D = fdesc[0]; f.H1 = fdesc[1]; f.H2 = fdesc[2]; f.hb1 = fdesc[3]; f.hb2 = fdesc[4]; f.db = fdesc[5];
+  f.k1 = bf16 ? fdesc[6] : fdesc[7];
+  const int* o = bf16 ? fdesc + 8 : fdesc + 14;
+  f.o_l1 = o[0]; f.o_l2 = o[1]; f.o_b1 = o[2]; f.o_g0 = o[3]; f.stride_slots = o[4]; f.o_vec = 0;
+  f.stride_vec = o[5];
+  const int4* t = (const int4*)tiles;
+  const __bf16* fb = (const __bf16*)frag_bf;
+  const __bf16* fl = (const __bf16*)frag_lo;
+#define MLP_L(P, H1B, H2B) hipLaunchKernelGGL((k_mlp_mfma<P, H1B, H2B>), dim3(ntiles), dim3(64), 0, st, t, perm, G, frag, fb, fl, vecs, f, Ei, dEdG)
+#define MLP_P(P)                                  \
+  do {                                            \
+    if (f.hb1 == 1 && f.hb2 == 1) MLP_L(P, 1, 1); \
+    else if (f.hb1 == 1) MLP_L(P, 1, 2);          \
+    else if (f.hb2 == 1) MLP_L(P, 2, 1);          \
+    else MLP_L(P, 2, 2);                          \
+  } while (0)
+  if (prec == 2) MLP_P(2);
+  else if (prec == 1) MLP_P(1);
+  else MLP_P(0);
+#undef MLP_P
+#undef MLP_L
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

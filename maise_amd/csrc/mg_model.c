@@ -14,6 +14,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <stdint.h>
 #include <string.h>
 
 #include "mg_host.h"
@@ -132,8 +133,123 @@ void mg_model_free(mg_model *m) {
   if (!m) return;
   free(m->mlp);
   free(m->mlp64);
+  free(m->frag);
+  free(m->frag_bf);
+  free(m->frag_lo);
+  free(m->fvec);
   free(m);
 }
+
+static inline int acc_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+
+static uint16_t f32_to_bf16(float f) {
+  uint32_t b;
+  memcpy(&b, &f, 4);
+  if ((b & 0x7f800000u) == 0x7f800000u) return (uint16_t)(b >> 16); /* inf/nan */
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+
+static float bf16_to_f32(uint16_t h) {
+  uint32_t b = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+
+/* a = hi + lo with hi, lo BF16 (lo rounds the residual) */
+static void put_split(uint16_t *hi, uint16_t *lo, size_t k, float a) {
+  hi[k] = f32_to_bf16(a);
+  lo[k] = f32_to_bf16(a - bf16_to_f32(hi[k]));
+}
+
+/* Weight fragments (A operands) for the MFMA MLP in the k order of the kernel (mg_kernels.hip, k_mlp_mfma):
+ * accumulator register r of lane l holds unit 8(r/4) + 4(l/32) + r%4 of atom l%32 within a 32-unit block. */
+static int pack_mfma(mg_model *m) {
+  const int D = m->D, H1 = m->H1, H2 = m->H2;
+  m->mfma_ok = H1 <= 64 && H2 <= 64;
+  if (!m->mfma_ok) return MG_OK;
+  const int hb1 = (H1 + 31) / 32, hb2 = (H2 + 31) / 32, db = (D + 31) / 32;
+  const int k1f = (D + 1) / 2, k1b = (D + 15) / 16;
+  int *fd = m->fdesc;
+  fd[0] = D; fd[1] = H1; fd[2] = H2; fd[3] = hb1; fd[4] = hb2; fd[5] = db; fd[6] = k1b; fd[7] = k1f;
+  /* bf16 set: offsets in slots */
+  int ob[4], of[4];
+  ob[0] = 0; ob[1] = ob[0] + hb1 * k1b; ob[2] = ob[1] + hb2 * hb1 * 2; ob[3] = ob[2] + hb1 * hb2 * 2;
+  const int sb = ob[3] + db * hb1 * 2;
+  of[0] = 0; of[1] = of[0] + hb1 * k1f; of[2] = of[1] + hb2 * hb1 * 16; of[3] = of[2] + hb1 * hb2 * 16;
+  const int sf = of[3] + db * hb1 * 16;
+  const int sv = hb1 * 32 + 2 * hb2 * 32 + 1 + D;
+  for (int k = 0; k < 4; k++) { fd[8 + k] = ob[k]; fd[14 + k] = of[k]; }
+  fd[12] = sb; fd[13] = sv; fd[18] = sf; fd[19] = sv;
+  const int S = m->nspc;
+  m->frag = (float *)calloc((size_t)S * sf * 64, sizeof(float));
+  m->frag_bf = (uint16_t *)calloc((size_t)S * sb * 64 * 8, sizeof(uint16_t));
+  m->frag_lo = (uint16_t *)calloc((size_t)S * sb * 64 * 8, sizeof(uint16_t));
+  m->fvec = (float *)calloc((size_t)S * sv, sizeof(float));
+  m->frag_n = (size_t)S * sf * 64;
+  m->frag_bf_n = (size_t)S * sb * 64 * 8;
+  m->fvec_n = (size_t)S * sv;
+  for (int s = 0; s < S; s++) {
+    const double *w = m->mlp64 + (size_t)s * m->wstride;
+    const double *W0 = w, *b0 = W0 + H1 * D, *W1 = b0 + H1, *b1 = W1 + H2 * H1, *W2 = b1 + H2, *b2 = W2 + H2, *xo = b2 + 1;
+#define W0F(u, k) ((u) < H1 && (k) < D ? W0[(u) * D + (k)] : 0.0)
+#define W1F(o, u) ((o) < H2 && (u) < H1 ? W1[(o) * H1 + (u)] : 0.0)
+    float *F = m->frag + (size_t)s * sf * 64;
+    uint16_t *B = m->frag_bf + (size_t)s * sb * 64 * 8;
+    uint16_t *Bl = m->frag_lo + (size_t)s * sb * 64 * 8;
+    for (int l = 0; l < 64; l++) {
+      const int c = l & 31, h = l >> 5;
+      /* FP32 (K = 2 per step, one value per lane) */
+      for (int rb = 0; rb < hb1; rb++)
+        for (int st = 0; st < k1f; st++) F[(of[0] + rb * k1f + st) * 64 + l] = (float)W0F(rb * 32 + c, 2 * st + h);
+      for (int rb2 = 0; rb2 < hb2; rb2++)
+        for (int rb1 = 0; rb1 < hb1; rb1++)
+          for (int r = 0; r < 16; r++)
+            F[(of[1] + rb2 * hb1 * 16 + rb1 * 16 + r) * 64 + l] = (float)W1F(rb2 * 32 + c, rb1 * 32 + acc_row(r, h));
+      for (int rb1 = 0; rb1 < hb1; rb1++)
+        for (int rb2 = 0; rb2 < hb2; rb2++)
+          for (int r = 0; r < 16; r++)
+            F[(of[2] + rb1 * hb2 * 16 + rb2 * 16 + r) * 64 + l] = (float)W1F(rb2 * 32 + acc_row(r, h), rb1 * 32 + c);
+      for (int rbd = 0; rbd < db; rbd++)
+        for (int rb1 = 0; rb1 < hb1; rb1++)
+          for (int r = 0; r < 16; r++)
+            F[(of[3] + rbd * hb1 * 16 + rb1 * 16 + r) * 64 + l] = (float)W0F(rb1 * 32 + acc_row(r, h), rbd * 32 + c);
+      /* BF16 (K = 16 per step, eight values per lane) */
+      for (int v = 0; v < 8; v++) {
+        for (int rb = 0; rb < hb1; rb++)
+          for (int st = 0; st < k1b; st++)
+            put_split(B, Bl, ((ob[0] + rb * k1b + st) * 64 + l) * 8 + v, (float)W0F(rb * 32 + c, 16 * st + 8 * h + v));
+        for (int rb2 = 0; rb2 < hb2; rb2++)
+          for (int rb1 = 0; rb1 < hb1; rb1++)
+            for (int q = 0; q < 2; q++)
+              put_split(B, Bl, ((ob[1] + rb2 * hb1 * 2 + rb1 * 2 + q) * 64 + l) * 8 + v,
+                        (float)W1F(rb2 * 32 + c, rb1 * 32 + acc_row(8 * q + v, h)));
+        for (int rb1 = 0; rb1 < hb1; rb1++)
+          for (int rb2 = 0; rb2 < hb2; rb2++)
+            for (int q = 0; q < 2; q++)
+              put_split(B, Bl, ((ob[2] + rb1 * hb2 * 2 + rb2 * 2 + q) * 64 + l) * 8 + v,
+                        (float)W1F(rb2 * 32 + acc_row(8 * q + v, h), rb1 * 32 + c));
+        for (int rbd = 0; rbd < db; rbd++)
+          for (int rb1 = 0; rb1 < hb1; rb1++)
+            for (int q = 0; q < 2; q++)
+              put_split(B, Bl, ((ob[3] + rbd * hb1 * 2 + rb1 * 2 + q) * 64 + l) * 8 + v,
+                        (float)W0F(rb1 * 32 + acc_row(8 * q + v, h), rbd * 32 + c));
+      }
+    }
+#undef W0F
+#undef W1F
+    float *V = m->fvec + (size_t)s * sv;
+    for (int u = 0; u < hb1 * 32; u++) V[u] = u < H1 ? (float)b0[u] : 0.0f;
+    for (int u = 0; u < hb2 * 32; u++) V[hb1 * 32 + u] = u < H2 ? (float)b1[u] : 0.0f;
+    for (int u = 0; u < hb2 * 32; u++) V[hb1 * 32 + hb2 * 32 + u] = u < H2 ? (float)W2[u] : 0.0f;
+    V[hb1 * 32 + 2 * hb2 * 32] = (float)b2[0];
+    for (int k = 0; k < D; k++) V[hb1 * 32 + 2 * hb2 * 32 + 1 + k] = (float)xo[k];
+  }
+  return MG_OK;
+}
+
+static int pack_mfma(mg_model *m);
 
 int mg_model_load(const char *path, mg_model **out) {
   if (!path || !out) return mg_fail(MG_ERR_ARG, "mg_model_load: null argument");
@@ -319,6 +435,12 @@ int mg_model_load(const char *path, mg_model **out) {
         if (ok) m->pat = cand;
       }
     }
+    /* all species pairs share the G4 etas (single-basis models): one exp per triplet and eta */
+    m->uni = 1;
+    for (int s0 = 0; s0 < m->nspc; s0++)
+      for (int s1 = 0; s1 < m->nspc; s1++)
+        for (int e = 0; e < B->ne && e < MG_MAXE; e++)
+          if (B->eta4[s0 * MG_MAXSPC + s1][e] != B->eta4[0][e]) m->uni = 0;
     for (int s = 0; s < m->nspc; s++) B->urep_rm[s] = (m->spcz[s] >= 0 && m->spcz[s] < 96) ? (float)MG_MINDIST[m->spcz[s]] : 1.0f;
   }
   free(bs);
@@ -346,6 +468,7 @@ int mg_model_load(const char *path, mg_model **out) {
       for (int k = 0; k < stride; k++) m->mlp[(size_t)s * stride + k] = (float)w[k];
     }
   }
+  if ((rc = pack_mfma(m)) != MG_OK) goto done;
   rc = MG_OK;
 done:
   free(V); free(rmin); free(rmax); free(dr);

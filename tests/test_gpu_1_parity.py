@@ -140,3 +140,26 @@ def test_full_size_32k_properties():
     assert np.abs(a["F"] - b["F"]).max() < F_TOL
     assert np.abs(a["F"].sum(axis=0)).max() < 1e-2
     c.close()
+
+
+@pytest.mark.parametrize("name", ["config1", "config1_w64", "cuag", "cupdag_tri", "mgca"])
+def test_bf16_mlp_error_bounds(ref_cases, name):
+    """Compensated BF16 MLP on MFMA (split hi/lo operands, FP32 accumulation, FP32 descriptors): energy within
+    1 meV/atom and forces within 10 meV/A of the FP64 reference build (measured ~1e-6 eV/atom).  The one-pass BF16
+    mode is only reported, not bounded: its weight rounding alone gives ~2e-3 eV/atom on the 51-10-10-1 model
+    (emulated on the CPU with the same rounding) and up to ~2e-2 eV/atom, 0.3 eV/A on the shipped models."""
+    import maise_amd
+    c = ref_cases[name]
+    print(f"[gpu case] bf16 {name}", flush=True)
+    m = maise_amd.Model(model_path(c["model"]))
+    N = len(c["spc"])
+    for prec, etol, ftol in ((maise_amd.MG_PREC_BF16X3, 1e-3, 1e-2), (maise_amd.MG_PREC_BF16, np.inf, np.inf)):
+        ctx = maise_amd.Context(m, urep=float(c["urep"]), precision=prec)
+        ctx.set_structure(c["spc"], c["lat"], c["x"], nd=int(c["nd"]), xt=int(c["xt"]))
+        r = ctx.evaluate()
+        de = abs(r["E"] - float(c["E"])) / N
+        df = np.abs(r["F"] - c["F"]).max()
+        print(f"bf16 prec={prec} {name}: dE/atom={de:.2e} dF={df:.2e}")
+        assert de < etol
+        assert df < ftol
+        ctx.close()
