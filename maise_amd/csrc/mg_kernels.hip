@@ -274,6 +274,33 @@ __global__ void k_rev(int N, const int* __restrict__ spc, const int* __restrict_
 }
 
 // ---------------------------------------------------------------------------------------------- wave helpers
+// LDS ordering between the lanes of one wave (several independent waves share a workgroup, so no block barrier)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// fnb[s] += g for every active lane, without atomics: lanes claim their slot (the last store to a slot wins),
+// the unique winners do a plain float4 read-modify-write, the others retry.  LDS float atomics serialise per lane
+// and cost ~3x the whole triplet arithmetic; duplicate slots within one 64-lane batch are rare, so this usually
+// finishes in one round.  Slots of one wave are private to that wave.
+__device__ __forceinline__ void lds_accum(float4* fnb, int* claim, int slot, float gx, float gy, float gz, bool active,
+                                          int lane) {
+  bool pending = active;
+  while (__ballot(pending) != 0ull) {
+    if (pending) claim[slot] = lane;
+    wave_sync();
+    if (pending && claim[slot] == lane) {
+      float4 v = fnb[slot];
+      v.x += gx; v.y += gy; v.z += gz;
+      fnb[slot] = v;
+      pending = false;
+    }
+    wave_sync();
+  }
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, WAVE);
@@ -472,12 +499,14 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
     cnt += __popcll(bm);
     const bool more = __ballot(live) != 0ull;
     if (cnt > PCH - WAVE || !more) {
-      __syncthreads();
-      for (int t = lane; t < cnt; t += WAVE) {
-        unsigned p = plist[t];
-        body((int)(p & 0xffffu), (int)(p >> 16));
+      wave_sync();
+      for (int tb = 0; tb < cnt; tb += WAVE) {
+        const int t = tb + lane;
+        const bool valid = t < cnt;
+        const unsigned p = valid ? plist[t] : 0u;
+        body((int)(p & 0xffffu), (int)(p >> 16), valid);
       }
-      __syncthreads();
+      wave_sync();
       cnt = 0;
       if (!more) break;
     }
@@ -485,23 +514,25 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
 }
 
 // ---------------------------------------------------------------------------------------------- descriptors
-template <int PAT, bool UNI>
-__global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
-                                             const int* __restrict__ cnt, const float4* __restrict__ vec,
-                                             const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
-                                             float* __restrict__ erep, float urep, int cap) {
+template <int PAT, bool UNI, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_desc(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+                                                   const int* __restrict__ cnt, const float4* __restrict__ vec,
+                                                   const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
+                                                   float* __restrict__ erep, float urep, int cap) {
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   extern __shared__ float4 smem_d[];
-  const int i = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * WPB + wv;
+  const int lane = threadIdx.x & 63;
+  if (i >= N) return;
   const int D = bp->D;
   RowInfo ri;
   row_info(ri, i, spc, row_off, cnt, S);
-  float4* rec = smem_d;
-  unsigned* plist = (unsigned*)(smem_d + R * cap);
+  float4* rec = smem_d + (size_t)wv * (R * cap + PCH / 4);
+  unsigned* plist = (unsigned*)(rec + R * cap);
   stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
-  __syncthreads();
+  wave_sync();
   float* Gi = G + (size_t)i * D;
 
   // ---- G2 (+ UREP energy) per neighbour species block
@@ -555,7 +586,8 @@ __global__ __launch_bounds__(64) void k_desc(int N, int S, const int* __restrict
       float V[MG_GRID];
 #pragma unroll
       for (int q = 0; q < MG_GRID; q++) V[q] = 0.0f;
-      auto body = [&](int j, int k) {
+      auto body = [&](int j, int k, bool valid) {
+        if (!valid) return;
         float4 a = rec[R * j], b = rec[R * k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
@@ -660,30 +692,34 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 // Forces: F_j -= dT/da, F_k -= dT/db, F_i += dT/da + dT/db; virial U -= dT/da (x) a + dT/db (x) b.
 // The dE/dG grid g(e,z,l) of the current species block is wave-uniform; it sits in LDS and is read with
 // broadcast loads (kept out of SGPRs, which would spill inside the triplet loop).
-template <int PAT, bool UNI>
-__global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
-                                              const int* __restrict__ cnt, const float4* __restrict__ vec,
-                                              const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
-                                              float4* __restrict__ fself, float4* __restrict__ fe,
-                                              float* __restrict__ vir, float urep, int cap) {
+template <int PAT, bool UNI, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+                                                    const int* __restrict__ cnt, const float4* __restrict__ vec,
+                                                    const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
+                                                    float4* __restrict__ fself, float4* __restrict__ fe,
+                                                    float* __restrict__ vir, float urep, int cap) {
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   constexpr int ZL = MG_MAXZ * MG_MAXL;
   extern __shared__ float4 smem_f[];
-  const int i = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * WPB + wv;
+  const int lane = threadIdx.x & 63;
+  if (i >= N) return;
   const int D = bp->D;
   RowInfo ri;
   row_info(ri, i, spc, row_off, cnt, S);
-  float4* rec = smem_f;
-  float4* gts4 = smem_f + R * cap;               // [MG_GRID / 4]
-  float* fnb = (float*)(gts4 + MG_GRID / 4);     // [cap][4]
-  unsigned* plist = (unsigned*)(fnb + 4 * cap);  // [PCH]
+  float4* rec = smem_f + (size_t)wv * ((R + 1) * cap + MG_GRID / 4 + PCH / 4 + ((cap + 3) & ~3) / 4);
+  float4* gts4 = rec + R * cap;                  // [MG_GRID / 4]
+  float4* fnb4 = gts4 + MG_GRID / 4;             // [cap] force on each neighbour slot
+  float* fnb = (float*)fnb4;
+  unsigned* plist = (unsigned*)(fnb4 + cap);     // [PCH]
+  int* claim = (int*)(plist + PCH);              // [cap]
   stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
   for (int t = lane; t < ri.M; t += WAVE) {
     fnb[4 * t] = 0.0f; fnb[4 * t + 1] = 0.0f; fnb[4 * t + 2] = 0.0f; fnb[4 * t + 3] = 0.0f;
   }
-  __syncthreads();
+  wave_sync();
   const float* gi = dEdG + (size_t)i * D;
   float fx = 0.f, fy = 0.f, fz = 0.f;
   float u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f, u4 = 0.f, u5 = 0.f;
@@ -734,8 +770,18 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
         int n = bp->g4map[lane];
         ((float*)gts4)[lane] = n >= 0 ? gi[blk + n] : 0.0f;
       }
-      __syncthreads();
-      auto body = [&](int j, int k) {
+      wave_sync();
+      // the grid of this species block, loop-invariant: kept in registers for the whole class
+      float gv[NE * ZL];
+#pragma unroll
+      for (int e = 0; e < NE; e++) {
+        const float4 g0 = (Pat<PAT>::mask(e) & 0x0Fu) ? gts4[e * 2] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 g1 = (Pat<PAT>::mask(e) & 0xF0u) ? gts4[e * 2 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        gv[e * ZL + 0] = g0.x; gv[e * ZL + 1] = g0.y; gv[e * ZL + 2] = g0.z; gv[e * ZL + 3] = g0.w;
+        gv[e * ZL + 4] = g1.x; gv[e * ZL + 5] = g1.y; gv[e * ZL + 6] = g1.z; gv[e * ZL + 7] = g1.w;
+      }
+      auto body = [&](int j, int k, bool valid) {
+        if (!valid) { j = 0; k = 1; }  // a harmless slot pair; its gradient is discarded below
         float4 a = rec[R * j], b = rec[R * k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
@@ -762,20 +808,9 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
           float B = UNI ? 2.0f * fast_exp2(-(float)MG_LOG2E * eta3 * rsum)
                         : 2.0f * Aj[e] * Ak[e] * fast_exp2(-(float)MG_LOG2E * eta3 * r3s);
           float Se = 0.f, Sd = 0.f;
-          if (Pat<PAT>::mask(e) & 0x0Fu) {
-            const float4 g = gts4[e * 2];
-            const float gg[4] = {g.x, g.y, g.z, g.w};
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-              if (Pat<PAT>::mask(e) & (1u << q)) { Se += gg[q] * P[q]; Sd += gg[q] * Pd[q]; }
-          }
-          if (Pat<PAT>::mask(e) & 0xF0u) {
-            const float4 g = gts4[e * 2 + 1];
-            const float gg[4] = {g.x, g.y, g.z, g.w};
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-              if (Pat<PAT>::mask(e) & (1u << (q + 4))) { Se += gg[q] * P[q + 4]; Sd += gg[q] * Pd[q + 4]; }
-          }
+          for (int q = 0; q < ZL; q++)
+            if (Pat<PAT>::mask(e) & (1u << q)) { Se += gv[e * ZL + q] * P[q]; Sd += gv[e * ZL + q] * Pd[q]; }
           float SB = Se * B;
           X += SB;
           if (UNI) {
@@ -801,15 +836,15 @@ __global__ __launch_bounds__(64) void k_force(int N, int S, const int* __restric
         float cbb = dT2 * i2 - dTc * cs * i2 * i2; // coefficient of b in dT/db
         float gax = ca * a.x + cb * b.x - c3c * cx, gay = ca * a.y + cb * b.y - c3c * cy, gaz = ca * a.z + cb * b.z - c3c * cz;
         float gbx = cbb * b.x + cb * a.x + c3c * cx, gby = cbb * b.y + cb * a.y + c3c * cy, gbz = cbb * b.z + cb * a.z + c3c * cz;
-        atomicAdd(&fnb[4 * j], -gax); atomicAdd(&fnb[4 * j + 1], -gay); atomicAdd(&fnb[4 * j + 2], -gaz);
-        atomicAdd(&fnb[4 * k], -gbx); atomicAdd(&fnb[4 * k + 1], -gby); atomicAdd(&fnb[4 * k + 2], -gbz);
+        lds_accum(fnb4, claim, j, -gax, -gay, -gaz, valid, lane);
+        lds_accum(fnb4, claim, k, -gbx, -gby, -gbz, valid, lane);
       };
       for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
-      __syncthreads();
+      wave_sync();
     }
   // every term of T_i depends on the neighbour vectors d_e only, and fnb[e] = -dT_i/dd_e summed over them: so
   // F_i = -sum_e fnb[e] and the virial U = -sum_e dT/dd_e (x) d_e = sum_e fnb[e] (x) d_e (symmetric)
-  __syncthreads();
+  wave_sync();
   for (int t = lane; t < ri.M; t += WAVE) {
     const float gx = fnb[4 * t], gy = fnb[4 * t + 1], gz = fnb[4 * t + 2];
     const float4 v = rec[R * t];
@@ -938,8 +973,9 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+#define MG_WPB 4  /* atoms (waves) per workgroup of the wave-per-atom kernels */
 #define LAUNCH_PU(KER, PAT, UNI, SM, ...)                                                                   \
-  hipLaunchKernelGGL((KER<PAT, UNI>), dim3(N), dim3(64), SM, st, __VA_ARGS__)
+  hipLaunchKernelGGL((KER<PAT, UNI, MG_WPB>), dim3((N + MG_WPB - 1) / MG_WPB), dim3(64 * MG_WPB), (SM) * MG_WPB, st, __VA_ARGS__)
 #define DISPATCH_PU(KER, SM, ...)                                  \
   do {                                                             \
     const int pat_ = variant & 15;                                 \
@@ -974,7 +1010,8 @@ int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, con
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
-  size_t sm = (size_t)cap * (R + 1) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
+  size_t sm = (size_t)cap * (R + 1) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned) +
+              (size_t)((cap + 3) & ~3) * sizeof(int);
   DISPATCH_PU(k_force, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
               urep, cap);
   return hipGetLastError() == hipSuccess ? 0 : -1;
