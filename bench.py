@@ -85,6 +85,69 @@ def cpu_baseline(threads):
             "sample": f"FP64 oracle on Cu fcc 6x6x6 = {len(s)} atoms"}
 
 
+RELAX_POT = os.path.join(ROOT, "tests", "golden", "models", "nn_CuPdAg_d3_v1")
+MD_POT = os.path.join(ROOT, "tests", "golden", "models", "nn_CuAg_d3_v1")
+
+
+def relax_leg(maise_amd, batch, args):
+    """Batched FIRE of the Cu-Pd-Ag batch (one context, every structure at once, converged ones compacted away)
+    against the same FIRE run one structure per context (the plain path), to fmax 0.01 eV/A."""
+    model = maise_amd.Model(RELAX_POT)
+    max_steps, fmax = 3000, 0.01
+    c = maise_amd.Context(model, device=0, urep=100.0)
+    c.set_batch(batch[:64])
+    c.relax(max_steps=50, fmax=fmax)           # warm-up: code objects, allocations
+    t0 = time.perf_counter()
+    c.set_batch(batch)
+    steps, done = c.relax(max_steps=max_steps, fmax=fmax)
+    dt = time.perf_counter() - t0
+    c.close()
+    nsingle = min(args.relax_single, len(batch))
+    c1 = maise_amd.Context(model, device=0, urep=100.0)
+    t1 = time.perf_counter()
+    s1 = []
+    for k in range(nsingle):
+        c1.set_batch([batch[k]])
+        st, _ = c1.relax(max_steps=max_steps, fmax=fmax)
+        s1.append(int(st[0]))
+    dt1 = time.perf_counter() - t1
+    c1.close()
+    return {"workload": f"{len(batch)} Cu-Pd-Ag structures, 8-64 atoms ({sum(len(b[0]) for b in batch)} atoms), "
+                        "random triclinic cells, 15-20 A^3/atom, dmin 2.0 A, seed 3; shipped potential "
+                        "nn_CuPdAg_d3_v1 (282-10-10-1, Rc 7.5 A), UREP 100 eV; FIRE to fmax 0.01 eV/A, cap 3000",
+            "relaxed_structures_per_s": float(done.sum()) / dt, "seconds": dt,
+            "converged": int(done.sum()), "structures": len(batch),
+            "fire_steps_mean": float(steps.mean()), "fire_steps_max": int(steps.max()),
+            "single_structure_path": {"structures": nsingle, "seconds": dt1,
+                                      "relaxed_structures_per_s": nsingle / dt1,
+                                      "steps_equal_batched": bool(np.array_equal(np.array(s1), steps[:nsingle]))},
+            "batched_over_single": (float(done.sum()) / dt) / (nsingle / dt1)}
+
+
+def md_leg(maise_amd, args):
+    """NVT molecular dynamics of the Cu-Ag 50/50 fcc 12x12x12 alloy (6912 atoms, a = 3.8 A), 600 K
+    Maxwell-Boltzmann start (seed 1), 1 fs steps, state on the device."""
+    from maise_amd import workloads as wl
+    s, l, x = wl.alloy_fcc(12, 3.8, 2, 0.0, seed=2)
+    c = maise_amd.Context(maise_amd.Model(MD_POT), device=0)
+    c.set_structure(s, l, x)
+    c.md_init(600.0, seed=1)
+    c.md_run(10, dt=1.0, T=600.0, tau=50.0)     # warm-up
+    c.md_init(600.0, seed=1)
+    t0 = time.perf_counter()
+    ep, ek = c.md_run(args.md_steps, dt=1.0, T=600.0, tau=50.0)
+    dt = time.perf_counter() - t0
+    c.close()
+    T = ek / (1.5 * len(s) / 11604.0)
+    n = len(s)
+    return {"workload": f"Cu-Ag 50/50 fcc 12x12x12 ({n} atoms), a=3.8 A, NVT 600 K, dt 1 fs, "
+                        f"{args.md_steps} steps; shipped potential nn_CuAg_d3_v1 (145-10-10-1, Rc 7.5 A)",
+            "steps_per_s": args.md_steps / dt, "atom_steps_per_s": n * args.md_steps / dt,
+            "ms_per_step": 1e3 * dt / args.md_steps,
+            "T_mean_last_half_K": float(T[len(T) // 2:].mean()),
+            "epot_per_atom_first_last_eV": [float(ep[0] / n), float(ep[-1] / n)]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,10 +155,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=20, help="fcc supercell edge (4 n^3 atoms)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--relax", type=int, default=4096,
+                    help="structures in the batched FIRE leg (0 skips it)")
+    ap.add_argument("--relax-single", type=int, default=48,
+                    help="structures relaxed one at a time for the unbatched comparison")
+    ap.add_argument("--md-steps", type=int, default=1000, help="NVT steps of the MD leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (5 times): the reference point that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profiling run to bytes")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
+              "design-only.", file=sys.stderr)
+        sys.exit(2)
+    batch = None
+    if args.relax > 0:
+        from maise_amd import workloads as wl0
+        # built before the GPU is touched: the generator forks worker processes
+        batch = wl0.cupdag_batch(args.relax, seed=3)
     if args.copy_calibration:
         import torch
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -105,10 +182,6 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": src.numel() * 4, "dispatches": 5}))
         return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
-              "design-only.", file=sys.stderr)
-        sys.exit(2)
 
     import maise_amd
     from maise_amd import workloads as wl
@@ -150,19 +223,22 @@ def main():
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
             "flops_per_launch": fl, "kernel_ms": round(t * 1e3, 4), "sum_M": M, "sum_K": K}
 
-    # parity of the FP32 path against FP64 on the 256-atom cell
+    # FP32 path against the reference's own FP64 output (CALL_MAISE compiled from its sources, committed as a golden
+    # vector) on the 256-atom cell with the same potential
     err = None
     try:
-        from oracle import pyoracle as po
-        s1, l1, x1 = wl.cu_fcc(4, 3.61, 0.05, seed=1)
-        c1 = maise_amd.Context(model)
-        c1.set_structure(s1, l1, x1)
+        z = np.load(os.path.join(ROOT, "tests", "golden", "ref_cases.npz"))
+        k = "config1_w64"
+        c1 = maise_amd.Context(maise_amd.Model(os.path.join(ROOT, str(z[k + "/model"]))))
+        c1.set_structure(z[k + "/spc"], z[k + "/lat"], z[k + "/x"], nd=int(z[k + "/nd"]), xt=int(z[k + "/xt"]))
         g = c1.evaluate()
-        o = po.OracleModel(POT).evaluate(s1, l1, x1)
-        err = {"dE_per_atom_eV": abs(g["E"] - o["E"]) / len(s1), "dF_max_eV_A": float(np.abs(g["F"] - o["F"]).max()),
-               "dU_max_eV_per_atom": float(np.abs(g["U"] - o["U"]).max() / len(s1))}
+        n1 = len(z[k + "/spc"])
+        err = {"case": "Cu fcc 4x4x4 (256 atoms), same potential, reference FP64 output",
+               "dE_per_atom_eV": abs(g["E"] - float(z[k + "/E"])) / n1,
+               "dF_max_eV_A": float(np.abs(g["F"] - z[k + "/F"].reshape(-1, 3)).max()),
+               "dU_max_eV_per_atom": float(np.abs(g["U"] - z[k + "/U"]).max() / n1)}
         c1.close()
-    except Exception as e:  # the error report is informative; the oracle may be absent
+    except Exception as e:  # informative only
         err = {"unavailable": str(e)[:200]}
 
     # same workload with the compensated-BF16 MLP on MFMA
@@ -186,6 +262,19 @@ def main():
     except Exception as e:
         bf = {"unavailable": str(e)[:200]}
 
+    relax = None
+    if batch is not None:
+        try:
+            relax = relax_leg(maise_amd, batch, args)
+        except Exception as e:
+            relax = {"unavailable": str(e)[:300]}
+    md = None
+    if args.md_steps > 0:
+        try:
+            md = md_leg(maise_amd, args)
+        except Exception as e:
+            md = {"unavailable": str(e)[:300]}
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -208,6 +297,8 @@ def main():
         "gpu_vs_cpu": (value / cpu["value"]) if cpu and "value" in cpu else None,
         "error_fp64": err,
         "bf16x3_mlp": bf,
+        "relax": relax,
+        "md": md,
         "energy_eV": res["E"],
     }
     print(json.dumps(out))

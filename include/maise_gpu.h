@@ -86,6 +86,12 @@ int mg_get_descriptors(mg_ctx *c, float *G);
 int mg_get_timing(mg_ctx *c, float *t);
 /* Enable (1) / disable (0) per-stage event timing (small overhead). */
 int mg_ctx_set_timing(mg_ctx *c, int on);
+/* Launch shape of the wave-per-atom descriptor and force kernels: atoms (64-lane waves) per workgroup, 1, 2 or 4;
+ * 0 (the default) picks the largest whose LDS tile fits 64 KiB.  No counterpart in the reference (CPU only). */
+int mg_ctx_set_wpb(mg_ctx *c, int wpb_desc, int wpb_force);
+/* Autotuner: times 1, 2 and 4 waves per workgroup on the structure/batch currently set (reps evaluations each)
+ * and keeps the fastest shape per kernel; wpb_out[2] (may be NULL) receives (descriptor, force). */
+int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out);
 
 /* --- device-resident dynamics (state stays on the device between steps) --- */
 /* NVT molecular dynamics, replacing Dynamics/Maxwell (src/cmod.c:224-549): velocity Verlet with a Nose-Hoover

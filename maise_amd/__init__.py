@@ -52,6 +52,8 @@ def lib():
             "mg_ctx_set_urep": ([vp, C.c_double], C.c_int),
             "mg_ctx_set_precision": ([vp, C.c_int], C.c_int),
             "mg_ctx_set_timing": ([vp, C.c_int], C.c_int),
+            "mg_ctx_set_wpb": ([vp, C.c_int, C.c_int], C.c_int),
+            "mg_ctx_autotune": ([vp, C.c_int, ip], C.c_int),
             "mg_set_structure": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp], C.c_int),
             "mg_set_batch": ([vp, C.c_int, ip, ip, dp, ip, dp], C.c_int),
             "mg_eval": ([vp, C.c_int], C.c_int),
@@ -143,6 +145,16 @@ class Context:
 
     def set_timing(self, on: bool):
         _check(lib().mg_ctx_set_timing(self._h, 1 if on else 0), "mg_ctx_set_timing")
+
+    def set_wpb(self, wpb_desc: int = 0, wpb_force: int = 0):
+        """Atoms (waves) per workgroup of the descriptor and force kernels (0 = largest that fits 64 KiB LDS)."""
+        _check(lib().mg_ctx_set_wpb(self._h, int(wpb_desc), int(wpb_force)), "mg_ctx_set_wpb")
+
+    def autotune(self, reps: int = 3):
+        """Time 1/2/4 waves per workgroup on the current structure or batch; keep the fastest per kernel."""
+        out = np.zeros(2, dtype=np.int32)
+        _check(lib().mg_ctx_autotune(self._h, int(reps), _ip(out)), "mg_ctx_autotune")
+        return int(out[0]), int(out[1])
 
     def set_structure(self, spc, lat, x, nd: int = 3, xt: int = 1):
         spc = np.ascontiguousarray(spc, dtype=np.int32)

@@ -178,6 +178,53 @@ int mg_ctx_set_timing(mg_ctx *c, int on) {
   return MG_OK;
 }
 
+int mg_ctx_set_wpb(mg_ctx *c, int wpb_desc, int wpb_force) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  if ((wpb_desc != 0 && wpb_desc != 1 && wpb_desc != 2 && wpb_desc != 4) ||
+      (wpb_force != 0 && wpb_force != 1 && wpb_force != 2 && wpb_force != 4))
+    return mg_fail(MG_ERR_ARG, "waves per block must be 0 (automatic), 1, 2 or 4");
+  c->wpb_desc = wpb_desc;
+  c->wpb_force = wpb_force;
+  return MG_OK;
+}
+
+/* Time the descriptor and force kernels at 1, 2 and 4 waves per block on the structure (or batch) now set,
+ * reps evaluations each, and keep the fastest per kernel.  Candidates whose LDS does not fit are skipped. */
+int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  if (c->S == 0) return mg_fail(MG_ERR_ARG, "mg_ctx_autotune: no structure set");
+  if (reps < 1) reps = 1;
+  const int cand[3] = {1, 2, 4};
+  const int timing0 = c->timing, wd0 = c->wpb_desc, wf0 = c->wpb_force;
+  float best_d = 1e30f, best_f = 1e30f;
+  int wd = 0, wf = 0;
+  c->timing = 1;
+  for (int k = 0; k < 3; k++) {
+    c->wpb_desc = c->wpb_force = cand[k];
+    if (mg_eval(c, MG_EVAL_FORCES) != MG_OK) continue;  /* LDS does not fit: skip this shape */
+    float td = 0.f, tf = 0.f;
+    int ok = 1;
+    for (int r = 0; r < reps && ok; r++) {
+      if (mg_eval(c, MG_EVAL_FORCES) != MG_OK) ok = 0;
+      td += c->tms[1];
+      tf += c->tms[3];
+    }
+    if (!ok) continue;
+    if (td < best_d) { best_d = td; wd = cand[k]; }
+    if (tf < best_f) { best_f = tf; wf = cand[k]; }
+  }
+  c->timing = timing0;
+  if (!wd || !wf) {
+    c->wpb_desc = wd0;
+    c->wpb_force = wf0;
+    return mg_fail(MG_ERR_HIP, "mg_ctx_autotune: no launch shape ran");
+  }
+  c->wpb_desc = wd;
+  c->wpb_force = wf;
+  if (wpb_out) { wpb_out[0] = wd; wpb_out[1] = wf; }
+  return MG_OK;
+}
+
 /* reciprocal rows R = (L1xL2, L2xL0, L0xL1) / (L0.(L1xL2)) with the reference's operation order
  * (src/cell.c:432-444, src/util.c:186-219); compiled without FMA contraction */
 static void reciprocal(const double *L, double *R) {
@@ -232,8 +279,8 @@ static void plan_bins(mg_struct_dev *s, double rc, int natom) {
   s->nbins = s->nb[0] * s->nb[1] * s->nb[2];
 }
 
-static int upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *xt, const double *lat, const int *spc,
-                  const double *x) {
+int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *xt, const double *lat, const int *spc,
+               const double *x) {
   HIPCHK(hipSetDevice(c->device));
   int N = atom_off[S];
   for (int i = 0; i < N; i++)
@@ -332,20 +379,25 @@ static int upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const in
 int mg_set_structure(mg_ctx *c, int N, int nd, int xt, const int *spc, const double *lat, const double *x) {
   if (!c || N < 0 || (N > 0 && (!spc || !x)) || !lat) return mg_fail(MG_ERR_ARG, "mg_set_structure: bad argument");
   int off[2] = {0, N};
-  return upload(c, 1, off, &nd, &xt, lat, spc, x);
+  return mgh_upload(c, 1, off, &nd, &xt, lat, spc, x);
 }
 
 int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const double *lat, const int *spc, const double *x) {
   if (!c || S < 1 || !atom_off || !nd || !lat || !spc || !x) return mg_fail(MG_ERR_ARG, "mg_set_batch: bad argument");
   int *xt = (int *)malloc(sizeof(int) * S);
   for (int s = 0; s < S; s++) xt[s] = 1;
-  int r = upload(c, S, atom_off, nd, xt, lat, spc, x);
+  int r = mgh_upload(c, S, atom_off, nd, xt, lat, spc, x);
   free(xt);
   return r;
 }
 
 /* ------------------------------------------------------------------ evaluation */
-#define KCHK(x) do { if ((x) != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, hipGetErrorString(hipGetLastError())); } while (0)
+#define KCHK(x)                                                                                                     \
+  do {                                                                                                              \
+    int kr_ = (x);                                                                                                  \
+    if (kr_ == -3) return mg_fail(MG_ERR_NEIGHBOR, "neighbour rows too long for the LDS of one CU (%d)", c->maxrow); \
+    if (kr_ != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, hipGetErrorString(hipGetLastError())); \
+  } while (0)
 
 int mg_eval(mg_ctx *c, int flags) {
   if (!c) return mg_fail(MG_ERR_ARG, "null context");
@@ -381,7 +433,7 @@ int mg_eval(mg_ctx *c, int flags) {
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
   /* descriptors, MLP, forces */
   KCHK(mgk_desc(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
-                urep, c->maxrow, st));
+                urep, c->maxrow, c->wpb_desc, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_DESC], st));
   if (m->mfma_ok)
     KCHK(mgk_mlp_mfma(c->ntiles, c->d_tiles, c->d_perm, c->d_G, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, m->fdesc,
@@ -391,7 +443,7 @@ int mg_eval(mg_ctx *c, int flags) {
   if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_MLP], st));
   if (forces) {
     KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
-                   c->d_fe, c->d_vir, urep, c->maxrow, st));
+                   c->d_fe, c->d_vir, urep, c->maxrow, c->wpb_force, st));
     if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
     KCHK(mgk_gather(N, c->d_row_off, c->d_rev, c->d_fself, c->d_fe, c->d_F, st));
   } else if (c->timing) {

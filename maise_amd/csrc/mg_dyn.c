@@ -131,32 +131,152 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
   return MG_OK;
 }
 
+/* Batched FIRE.  Every 8 iterations the converged structures are counted on the device; once the atoms of the
+ * structures still moving fall to half of those being evaluated, the finished ones are copied out and the batch
+ * is re-packed to the unconverged structures (positions, velocities and FIRE state carried over), so the tail
+ * of the relaxation no longer evaluates structures that have stopped.  At the end the full batch is restored
+ * with the final positions and evaluated once more, so mg_get / mg_get_batch see every structure. */
+typedef struct {
+  int S0, N0;
+  int *off0, *nd0, *spc0;
+  double *lat0, *x0, *st0;
+  int *act;   /* active slot -> original structure */
+  int S;      /* active structures */
+} relax_book;
+
+static void book_free(relax_book *b) {
+  free(b->off0); free(b->nd0); free(b->spc0); free(b->lat0); free(b->x0); free(b->st0); free(b->act);
+}
+
+/* copy the current device state of the active structures into the book; keep (re-pack) those with keep[k] */
+static int compact(mg_ctx *c, relax_book *b, int repack) {
+  const int S = c->S, N = c->N;
+  double *x = (double *)malloc(sizeof(double) * 3 * N), *v = (double *)malloc(sizeof(double) * 3 * N);
+  double *st = (double *)malloc(sizeof(double) * 5 * S);
+  int r = MG_OK;
+  if (hipMemcpyAsync(x, c->d_xin, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipMemcpyAsync(v, c->d_v, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipMemcpyAsync(st, c->d_fire, sizeof(double) * 5 * S, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess) {
+    r = mg_fail(MG_ERR_HIP, "mg_relax: state download failed");
+    goto out;
+  }
+  for (int k = 0; k < S; k++) {
+    const int o = b->act[k], n = b->off0[o + 1] - b->off0[o];
+    memcpy(b->x0 + 3 * b->off0[o], x + 3 * c->h_atom_off[k], sizeof(double) * 3 * n);
+    memcpy(b->st0 + 5 * o, st + 5 * k, sizeof(double) * 5);
+  }
+  if (repack) {
+    int S1 = 0, N1 = 0;
+    for (int k = 0; k < S; k++)
+      if (st[5 * k + 3] == 0.0) { b->act[S1++] = b->act[k]; }
+    int *off = (int *)malloc(sizeof(int) * (S1 + 1)), *nd = (int *)malloc(sizeof(int) * S1);
+    int *xt = (int *)malloc(sizeof(int) * S1), *spc = (int *)malloc(sizeof(int) * b->N0);
+    double *lat = (double *)malloc(sizeof(double) * 9 * S1), *xn = (double *)malloc(sizeof(double) * 3 * b->N0);
+    double *vn = (double *)malloc(sizeof(double) * 3 * b->N0), *sn = (double *)malloc(sizeof(double) * 5 * S1);
+    int kk = 0;
+    for (int k = 0; k < S; k++) {
+      if (st[5 * k + 3] != 0.0) continue;
+      const int o = b->act[kk], n = b->off0[o + 1] - b->off0[o];
+      off[kk] = N1;
+      nd[kk] = b->nd0[o];
+      xt[kk] = 1;
+      memcpy(lat + 9 * kk, b->lat0 + 9 * o, sizeof(double) * 9);
+      memcpy(spc + N1, b->spc0 + b->off0[o], sizeof(int) * n);
+      memcpy(xn + 3 * N1, x + 3 * c->h_atom_off[k], sizeof(double) * 3 * n);
+      memcpy(vn + 3 * N1, v + 3 * c->h_atom_off[k], sizeof(double) * 3 * n);
+      memcpy(sn + 5 * kk, st + 5 * k, sizeof(double) * 5);
+      N1 += n;
+      kk++;
+    }
+    off[S1] = N1;
+    r = mgh_upload(c, S1, off, nd, xt, lat, spc, xn);
+    if (r == MG_OK && (r = ensure_dyn(c)) == MG_OK) {
+      if (hipMemcpy(c->d_v, vn, sizeof(double) * 3 * N1, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(c->d_fire, sn, sizeof(double) * 5 * S1, hipMemcpyHostToDevice) != hipSuccess)
+        r = mg_fail(MG_ERR_HIP, "mg_relax: state upload failed");
+    }
+    b->S = S1;
+    free(off); free(nd); free(xt); free(spc); free(lat); free(xn); free(vn); free(sn);
+  }
+out:
+  free(x); free(v); free(st);
+  return r;
+}
+
 int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done) {
   if (!c || c->S < 1 || max_steps < 0 || fmax <= 0.0) return mg_fail(MG_ERR_ARG, "mg_relax: bad argument");
   HIPCHK(hipSetDevice(c->device));
   int r;
   if ((r = ensure_dyn(c))) return r;
-  HIPCHK(hipMemsetAsync(c->d_v, 0, sizeof(double) * 3 * c->N, c->st));
-  KCHK(mgk_fire_init(c->S, c->d_fire, dt0, c->st));
-  int ndone = 0, it = 0;
-  for (it = 0; it < max_steps; it++) {
-    if ((r = mg_eval(c, MG_EVAL_FORCES))) return r;
-    KCHK(mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, fmax, dtmax, maxstep, c->st));
+  relax_book b;
+  memset(&b, 0, sizeof b);
+  b.S0 = b.S = c->S;
+  b.N0 = c->N;
+  b.off0 = (int *)malloc(sizeof(int) * (b.S0 + 1));
+  b.nd0 = (int *)malloc(sizeof(int) * b.S0);
+  b.spc0 = (int *)malloc(sizeof(int) * (b.N0 > 0 ? b.N0 : 1));
+  b.lat0 = (double *)malloc(sizeof(double) * 9 * b.S0);
+  b.x0 = (double *)malloc(sizeof(double) * 3 * (b.N0 > 0 ? b.N0 : 1));
+  b.st0 = (double *)malloc(sizeof(double) * 5 * b.S0);
+  b.act = (int *)malloc(sizeof(int) * b.S0);
+  memcpy(b.off0, c->h_atom_off, sizeof(int) * (b.S0 + 1));
+  memcpy(b.nd0, c->h_nd, sizeof(int) * b.S0);
+  memcpy(b.lat0, c->h_lat, sizeof(double) * 9 * b.S0);
+  for (int s = 0; s < b.S0; s++) b.act[s] = s;
+  if (hipMemcpy(b.spc0, c->d_spc, sizeof(int) * b.N0, hipMemcpyDeviceToHost) != hipSuccess) {
+    book_free(&b);
+    return mg_fail(MG_ERR_HIP, "mg_relax: species download failed");
+  }
+  int compacted = 0;
+#define RCHK(x) do { if ((r = (x)) != MG_OK) { book_free(&b); return r; } } while (0)
+#define RKCHK(x) do { if ((x) != 0) { book_free(&b); return mg_fail(MG_ERR_HIP, "kernel launch failed: %s", #x); } } while (0)
+#define RHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { book_free(&b); return mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+  RHIP(hipMemsetAsync(c->d_v, 0, sizeof(double) * 3 * c->N, c->st));
+  RKCHK(mgk_fire_init(c->S, c->d_fire, dt0, c->st));
+  int ndone = 0;
+  for (int it = 0; it < max_steps; it++) {
+    RCHK(mg_eval(c, MG_EVAL_FORCES));
+    RKCHK(mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, fmax, dtmax, maxstep, c->st));
     if (it == 0) {
       int any_frac = 0;
       for (int s = 0; s < c->S; s++) if (c->h_str[s].xt == 0) { c->h_str[s].xt = 1; any_frac = 1; }
-      if (any_frac) HIPCHK(hipMemcpyAsync(c->d_str, c->h_str, sizeof(mg_struct_dev) * c->S, hipMemcpyHostToDevice, c->st));
+      if (any_frac) RHIP(hipMemcpyAsync(c->d_str, c->h_str, sizeof(mg_struct_dev) * c->S, hipMemcpyHostToDevice, c->st));
     }
     if ((it % 8) == 7) {
-      HIPCHK(hipMemsetAsync(c->d_misc + 2, 0, sizeof(int), c->st));
-      KCHK(mgk_fire_count(c->S, c->d_fire, c->d_misc + 2, c->st));
-      HIPCHK(hipMemcpyAsync(&ndone, c->d_misc + 2, sizeof(int), hipMemcpyDeviceToHost, c->st));
-      HIPCHK(hipStreamSynchronize(c->st));
+      RHIP(hipMemsetAsync(c->d_misc + 2, 0, sizeof(int), c->st));
+      RKCHK(mgk_fire_count(c->S, c->d_fire, c->d_misc + 2, c->st));
+      RHIP(hipMemcpyAsync(&ndone, c->d_misc + 2, sizeof(int), hipMemcpyDeviceToHost, c->st));
+      RHIP(hipStreamSynchronize(c->st));
       if (ndone == c->S) break;
+      if (ndone > 0 && c->S > 1) {
+        /* atoms still moving, from the per-structure done flags */
+        double *st = (double *)malloc(sizeof(double) * 5 * c->S);
+        RHIP(hipMemcpy(st, c->d_fire, sizeof(double) * 5 * c->S, hipMemcpyDeviceToHost));
+        long moving = 0;
+        for (int k = 0; k < c->S; k++)
+          if (st[5 * k + 3] == 0.0) moving += c->h_atom_off[k + 1] - c->h_atom_off[k];
+        free(st);
+        if (2 * moving <= c->N) {
+          RCHK(compact(c, &b, 1));
+          compacted = 1;
+        }
+      }
     }
   }
+  if (compacted) {
+    /* bring the last active structures into the book and restore the full batch with the final positions */
+    RCHK(compact(c, &b, 0));
+    int *xt = (int *)malloc(sizeof(int) * b.S0);
+    for (int s = 0; s < b.S0; s++) xt[s] = 1;
+    r = mgh_upload(c, b.S0, b.off0, b.nd0, xt, b.lat0, b.spc0, b.x0);
+    free(xt);
+    if (r != MG_OK) { book_free(&b); return r; }
+    RCHK(ensure_dyn(c));
+    RHIP(hipMemcpy(c->d_fire, b.st0, sizeof(double) * 5 * b.S0, hipMemcpyHostToDevice));
+  }
   /* energies and forces at the final positions */
-  if ((r = mg_eval(c, MG_EVAL_FORCES))) return r;
+  RCHK(mg_eval(c, MG_EVAL_FORCES));
   if (steps || done) {
     double *state = (double *)malloc(sizeof(double) * 5 * c->S);
     hipError_t e = hipMemcpy(state, c->d_fire, sizeof(double) * 5 * c->S, hipMemcpyDeviceToHost);
@@ -165,7 +285,11 @@ int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, do
       if (done) done[s] = state[5 * s + 3] != 0.0;
     }
     free(state);
-    if (e != hipSuccess) return mg_fail(MG_ERR_HIP, "mg_relax: %s", hipGetErrorString(e));
+    if (e != hipSuccess) { book_free(&b); return mg_fail(MG_ERR_HIP, "mg_relax: %s", hipGetErrorString(e)); }
   }
+  book_free(&b);
+#undef RCHK
+#undef RKCHK
+#undef RHIP
   return MG_OK;
 }

@@ -39,6 +39,11 @@ struct mg_model {
 
 enum { EV_START, EV_NBR, EV_DESC, EV_MLP, EV_FORCE, EV_END, EV_N };
 
+struct mg_ctx;
+/* (Re)load S structures into the context: per-structure tables, bins, species tiles, positions (xt 0 = fractional) */
+int mgh_upload(struct mg_ctx *c, int S, const int *atom_off, const int *nd, const int *xt, const double *lat,
+               const int *spc, const double *x);
+
 struct mg_ctx {
   const mg_model *m;
   int device;
@@ -46,6 +51,7 @@ struct mg_ctx {
   double urep;
   int prec;
   int timing;
+  int wpb_desc, wpb_force; /* waves (atoms) per block of the descriptor and force kernels; 0 = by LDS size */
   hipEvent_t ev[EV_N];
   float tms[6];
   /* structures */
@@ -103,7 +109,7 @@ int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, co
                  const int *bin_start, const int *bin_atoms, const int *row_off, void *key, void *vec, int *rev, int *err,
                  int maxrow, hipStream_t st);
 int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
-             const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, hipStream_t st);
+             const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, hipStream_t st);
 int mgk_mlp_mfma(int ntiles, const void *tiles, const int *perm, const float *G, const float *frag, const void *frag_bf,
                  const void *frag_lo, const float *vecs, const int *fdesc, int prec, float *Ei, float *dEdG,
                  hipStream_t st);
@@ -111,7 +117,7 @@ int mgk_mlp(int N, const int *spc, const float *G, const float *W, int wstride, 
             float *dEdG, hipStream_t st);
 int mgk_force(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
               const mg_basis_dev *bp, const float *dEdG, void *fself, void *fe, float *vir, float urep, int maxrow,
-              hipStream_t st);
+              int wpb, hipStream_t st);
 int mgk_gather(int N, const int *row_off, const int *rev, const void *fself, const void *fe, double *F, hipStream_t st);
 int mgk_reduce(int nstr, int nchunk, const int *chunk_str, const mg_struct_dev *str, const float *Ei, const float *erep,
                const float *vir, int have_vir, double *part, double *E, double *U, hipStream_t st);

@@ -514,8 +514,8 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
 }
 
 // ---------------------------------------------------------------------------------------------- descriptors
-template <int PAT, bool UNI, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_desc(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+template <int PAT, bool UNI>
+__global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                                    const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                    const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
                                                    float* __restrict__ erep, float urep, int cap) {
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(64 * WPB) void k_desc(int N, int S, const int* __re
   constexpr int R = UNI ? 2 : 4;
   extern __shared__ float4 smem_d[];
   const int wv = threadIdx.x >> 6;
-  const int i = blockIdx.x * WPB + wv;
+  const int i = blockIdx.x * (blockDim.x >> 6) + wv;  // one wave per atom, 1-4 waves per block
   const int lane = threadIdx.x & 63;
   if (i >= N) return;
   const int D = bp->D;
@@ -692,8 +692,8 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 // Forces: F_j -= dT/da, F_k -= dT/db, F_i += dT/da + dT/db; virial U -= dT/da (x) a + dT/db (x) b.
 // The dE/dG grid g(e,z,l) of the current species block is wave-uniform; it sits in LDS and is read with
 // broadcast loads (kept out of SGPRs, which would spill inside the triplet loop).
-template <int PAT, bool UNI, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+template <int PAT, bool UNI>
+__global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                                     const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
                                                     float4* __restrict__ fself, float4* __restrict__ fe,
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(64 * WPB) void k_force(int N, int S, const int* __r
   constexpr int ZL = MG_MAXZ * MG_MAXL;
   extern __shared__ float4 smem_f[];
   const int wv = threadIdx.x >> 6;
-  const int i = blockIdx.x * WPB + wv;
+  const int i = blockIdx.x * (blockDim.x >> 6) + wv;  // one wave per atom, 1-4 waves per block
   const int lane = threadIdx.x & 63;
   if (i >= N) return;
   const int D = bp->D;
@@ -973,11 +973,28 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-#define MG_WPB 4  /* atoms (waves) per workgroup of the wave-per-atom kernels */
-#define LAUNCH_PU(KER, PAT, UNI, SM, ...)                                                                   \
-  hipLaunchKernelGGL((KER<PAT, UNI, MG_WPB>), dim3((N + MG_WPB - 1) / MG_WPB), dim3(64 * MG_WPB), (SM) * MG_WPB, st, __VA_ARGS__)
+// One wave per atom; wpb waves (atoms) per block.  wpb = 0 picks the largest of 4, 2, 1 whose LDS fits 64 KiB;
+// a single wave may use up to the 160 KiB of a gfx950 CU.
+#define MG_LDS_BLOCK (64 * 1024)
+#define MG_LDS_CU (160 * 1024)
+static int pick_wpb(int wpb, size_t sm_wave) {
+  if (wpb == 1 || wpb == 2 || wpb == 4) return (size_t)wpb * sm_wave <= MG_LDS_CU ? wpb : -1;
+  for (int w = 4; w >= 1; w >>= 1)
+    if ((size_t)w * sm_wave <= MG_LDS_BLOCK) return w;
+  return sm_wave <= MG_LDS_CU ? 1 : -1;
+}
+#define LAUNCH_PU(KER, PAT, UNI, SM, ...)                                                                            \
+  do {                                                                                                               \
+    if ((size_t)(SM) * wpb_ > MG_LDS_BLOCK &&                                                                        \
+        hipFuncSetAttribute((const void*)&KER<PAT, UNI>, hipFuncAttributeMaxDynamicSharedMemorySize,                 \
+                            (int)((SM) * wpb_)) != hipSuccess)                                                       \
+      return -1;                                                                                                     \
+    hipLaunchKernelGGL((KER<PAT, UNI>), dim3((N + wpb_ - 1) / wpb_), dim3(64 * wpb_), (SM) * wpb_, st, __VA_ARGS__); \
+  } while (0)
 #define DISPATCH_PU(KER, SM, ...)                                  \
   do {                                                             \
+    const int wpb_ = pick_wpb(wpb, SM);                            \
+    if (wpb_ < 0) return -3;                                       \
     const int pat_ = variant & 15;                                 \
     const bool uni_ = (variant >> 4) & 1;                          \
     if (pat_ == 1) { if (uni_) LAUNCH_PU(KER, 1, true, SM, __VA_ARGS__); else LAUNCH_PU(KER, 1, false, SM, __VA_ARGS__); } \
@@ -987,7 +1004,7 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
 
 // variant = G4 pattern (bits 0-3) | shared-eta flag (bit 4)
 int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
-             const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, hipStream_t st) {
+             const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, int wpb, hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
@@ -1006,7 +1023,7 @@ int mgk_mlp(int N, const int* spc, const float* G, const float* W, int wstride, 
 
 int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
               const mg_basis_dev* bp, const float* dEdG, void* fself, void* fe, float* vir, float urep, int maxrow,
-              hipStream_t st) {
+              int wpb, hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;

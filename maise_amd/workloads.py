@@ -71,27 +71,53 @@ def random_triclinic(nat: int, nspc: int, rng: np.random.Generator, vol_range=(1
         rec = np.linalg.inv(lat).T
         if np.min(1.0 / np.linalg.norm(rec, axis=1)) < dmin:
             continue
-        frac = []
+        # sequential random placement; the periodic distance to every atom placed so far is the minimum over the
+        # 27 images around the rounded fractional difference
+        shifts = np.array([(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], dtype=float)
+        frac = np.zeros((nat, 3))
+        n = 0
         tries = 0
-        while len(frac) < nat and tries < max_tries * nat:
+        while n < nat and tries < max_tries * nat:
             tries += 1
             f = rng.random(3)
-            ok = True
-            for h in frac:
-                d = f - h
+            if n:
+                d = f - frac[:n]
                 d -= np.round(d)
-                if np.linalg.norm(d @ lat) < dmin:
-                    ok = False
-                    break
-            if ok:
-                frac.append(f)
-        if len(frac) < nat:
+                cart = (d[:, None, :] + shifts[None, :, :]) @ lat
+                if np.einsum("kiq,kiq->ki", cart, cart).min() < dmin * dmin:
+                    continue
+            frac[n] = f
+            n += 1
+        if n < nat:
             continue
         w = rng.dirichlet(np.ones(nspc))
         counts = np.floor(w * nat).astype(int)
         counts[np.argmax(w)] += nat - counts.sum()
         spc = np.repeat(np.arange(nspc), counts).astype(np.int32)
-        return spc, lat, np.array(frac) @ lat
+        return spc, lat, frac @ lat
+
+
+def _cupdag_one(args):
+    seq, lo, hi = args
+    rng = np.random.default_rng(seq)
+    return random_triclinic(int(rng.integers(lo, hi + 1)), 3, rng)
+
+
+def cupdag_batch(count: int = 4096, seed: int = 3, nat_range=(8, 64), workers: int = 0):
+    """The batched-relaxation workload: `count` ternary Cu-Pd-Ag structures with atom counts uniform in
+    nat_range, random triclinic cells (15-20 A^3/atom, angles 60-120 deg), random positions with minimum
+    periodic pair distance 2.0 A and compositions uniform on the simplex.  Structure k is drawn from its own
+    stream (SeedSequence(seed).spawn), so the batch does not depend on how many worker processes build it
+    (workers = 0: up to 16 host cores; call before the GPU is initialised in this process)."""
+    seqs = np.random.SeedSequence(seed).spawn(count)
+    jobs = [(q, nat_range[0], nat_range[1]) for q in seqs]
+    if workers == 0:
+        workers = min(16, os.cpu_count() or 1)
+    if workers <= 1 or count < 64:
+        return [_cupdag_one(j) for j in jobs]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_cupdag_one, jobs, chunksize=32)
 
 
 def synthetic_model(path: str, spcz: Sequence[int], hidden: Tuple[int, int] = (10, 10), seed: int = 7,
