@@ -93,6 +93,43 @@ int mg_ctx_set_wpb(mg_ctx *c, int wpb_desc, int wpb_force);
  * and keeps the fastest shape per kernel; wpb_out[2] (may be NULL) receives (descriptor, force). */
 int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out);
 
+/* --- screening surrogate (no counterpart in the reference; its training derivative follows DE_ANN_PARA,
+ * src/nmlp.c:292-354, and its training loop replaces the GSL minimiser of src/nmin.c:361-432 with Adam) ---
+ * A nin-64-64-1 tanh network trained on the GPU with mean-squared error and Adam (b1 0.9, b2 0.999, eps 1e-8).
+ * Parameters (floats, mg_sur_nparams(nin) of them): W0[64][nin] b0[64] W1[64][64] b1[64] w2[64] b2.
+ * Inputs are standardised with (mu, sig) and the target with (ymu, ysig) set by mg_sur_set_norm. */
+typedef struct mg_sur mg_sur;
+int mg_sur_nparams(int nin);
+/* nin <= 128; weights ~ N(0, 1/fan_in) from the seed, biases 0, identity normalisation */
+int mg_sur_create(int device, int nin, unsigned long long seed, mg_sur **out);
+void mg_sur_free(mg_sur *s);
+int mg_sur_get_params(mg_sur *s, float *p);
+/* replaces the parameters and resets the Adam state */
+int mg_sur_set_params(mg_sur *s, const float *p);
+int mg_sur_set_norm(mg_sur *s, const float *mu, const float *sig, float ymu, float ysig);
+/* epochs of seeded-shuffle mini-batch Adam over X[n][nin], y[n]; loss = final mean squared error (normalised) */
+int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, int batch, float lr,
+                 unsigned long long seed, float *loss);
+/* gradient (mg_sur_nparams floats) and value of the mean squared error over all n samples, parameters unchanged */
+int mg_sur_grad(mg_sur *s, const float *X, const float *y, int n, float *grad, float *loss);
+int mg_sur_predict(mg_sur *s, const float *X, int n, float *y);
+/* Features of every structure of the context's last evaluation: per species q, the mean over that structure's
+ * atoms of species q of the same-species descriptor block (G2 block at ind2[0], then the G4 block at ind4[0][0];
+ * ng2 + ng4 values), zero when q is absent.  feat holds S * nspc * (ng2 + ng4) floats. */
+int mg_features(mg_ctx *c, float *feat);
+
+/* --- evolutionary operators (host C; no GPU work) ---
+ * Cut-and-splice crossover after BULK_MATE (src/efnc.c:2021-2310) with MATE_LT's lattice choice
+ * (src/eutl.c:139-172), then a Gaussian displacement of every atom (sigma, A) and a periodic minimum-distance
+ * test (>= dmin).  Parents: npar structures of N species-grouped atoms (spc[npar*N], lat[9*npar] rows,
+ * x[3*npar*N] Cartesian) drawn with the given weights; count children go to out_* with the same layout.  A slot
+ * whose operators fail max_tries times is flagged in failed[count].  Returns the number of failed slots. */
+int mg_evo_offspring(int npar, int N, int nspc, const int *spc, const double *lat, const double *x,
+                     const double *weights, int count, double sigma, double dmin, unsigned long long seed,
+                     int max_tries, int *out_spc, double *out_lat, double *out_x, int *failed);
+/* smallest periodic interatomic distance of one structure (lattice rows L[9], Cartesian x[3n]) */
+double mg_evo_min_distance(const double *L, const double *x, int n);
+
 /* --- device-resident dynamics (state stays on the device between steps) --- */
 /* NVT molecular dynamics, replacing Dynamics/Maxwell (src/cmod.c:224-549): velocity Verlet with a Nose-Hoover
  * thermostat xi' = (2K/(3N kB T) - 1)/tau^2 (the reference's equation of motion, src/cmod.c:403-407).

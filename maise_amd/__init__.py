@@ -67,6 +67,19 @@ def lib():
             "mg_md_get_velocities": ([vp, dp], C.c_int),
             "mg_md_run": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp], C.c_int),
             "mg_relax": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, ip, ip], C.c_int),
+            "mg_sur_nparams": ([C.c_int], C.c_int),
+            "mg_sur_create": ([C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)], C.c_int),
+            "mg_sur_free": ([vp], None),
+            "mg_sur_get_params": ([vp, fp], C.c_int),
+            "mg_sur_set_params": ([vp, fp], C.c_int),
+            "mg_sur_set_norm": ([vp, fp, fp, C.c_float, C.c_float], C.c_int),
+            "mg_sur_train": ([vp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp], C.c_int),
+            "mg_sur_grad": ([vp, fp, fp, C.c_int, fp, fp], C.c_int),
+            "mg_sur_predict": ([vp, fp, C.c_int, fp], C.c_int),
+            "mg_features": ([vp, fp], C.c_int),
+            "mg_evo_offspring": ([C.c_int, C.c_int, C.c_int, ip, dp, dp, dp, C.c_int, C.c_double, C.c_double,
+                                  C.c_ulonglong, C.c_int, ip, dp, dp, ip], C.c_int),
+            "mg_evo_min_distance": ([dp, dp, C.c_int], C.c_double),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -88,6 +101,10 @@ def _dp(a):
 
 def _ip(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 class Model:
@@ -259,6 +276,14 @@ class Context:
         ep, ek = np.zeros(max(nsteps, 1)), np.zeros(max(nsteps, 1))
         _check(lib().mg_md_run(self._h, int(nsteps), float(dt), float(T), float(tau), _dp(ep), _dp(ek)), "mg_md_run")
         return ep[:nsteps], ek[:nsteps]
+
+    def features(self) -> np.ndarray:
+        """Per-structure surrogate inputs of the last evaluation: [S, nspc * (ng2 + ng4)] species means of the
+        same-species descriptor block."""
+        m = self.model
+        f = np.zeros((self.S, m.nspc * (m.ng2 + m.ng4)), dtype=np.float32)
+        _check(lib().mg_features(self._h, _fp(f)), "mg_features")
+        return f
 
     def relax(self, max_steps: int = 1000, fmax: float = 0.01, dt0: float = 0.1, dtmax: float = 1.0,
               maxstep: float = 0.2):

@@ -23,8 +23,8 @@ HIPCC_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-munsafe
 CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
             f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["mg_kernels.hip", "mg_dyn.hip"]
-C_SOURCES = ["mg_model.c", "mg_api.c", "mg_dyn.c"]
+HIP_SOURCES = ["mg_kernels.hip", "mg_dyn.hip", "mg_sur.hip"]
+C_SOURCES = ["mg_model.c", "mg_api.c", "mg_dyn.c", "mg_sur.c", "mg_evo.c"]
 
 
 def _run(cmd, verbose):

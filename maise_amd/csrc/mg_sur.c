@@ -1,0 +1,272 @@
+/*
+ * mg_sur.c -- host side of the screening surrogate (kernels in mg_sur.hip): parameter state, seeded
+ * initialisation, normalisation, mini-batch Adam training, full-batch gradients and prediction, and the
+ * per-structure descriptor features drawn from a context's last evaluation.
+ *
+ * Training data lives on the device for the duration of one mg_sur_train call; the parameters, the Adam
+ * moments and the normalisation stay on the device between calls.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mg_host.h"
+
+#define SUR_H 64
+#define SUR_MAXIN 128
+#define SUR_CHUNK 4
+
+int mgk_sur_features(int S, const mg_struct_dev *str, const int *spc, const float *G, int D, int nspc, int ng2,
+                     int ng4, int off2, int off4, float *feat, hipStream_t st);
+int mgk_sur_grad(int B, int chunk, int nin, const int *idx, const float *X, const float *Y, const float *p,
+                 const float *norm, float *part, float *part_loss, hipStream_t st);
+int mgk_sur_adam(int P, int nblk, const float *part, float *p, float *m, float *v, float *grad, float lr, float b1,
+                 float b2, float eps, float c1, float c2, hipStream_t st);
+int mgk_sur_predict(int n, int nin, const float *X, const float *p, const float *norm, float *out, hipStream_t st);
+
+struct mg_sur {
+  int device, nin, P;
+  long step;                   /* Adam steps taken */
+  hipStream_t st;
+  float *d_p, *d_m, *d_v, *d_norm, *d_grad;
+  float *d_part, *d_ploss;
+  size_t cap_blk;
+  float *d_X, *d_Y, *d_out;
+  int *d_idx;
+  size_t cap_n;
+};
+
+static unsigned long long splitmix64(unsigned long long *s) {
+  unsigned long long z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static double unif(unsigned long long *s) { return ((splitmix64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+
+static double gauss(unsigned long long *s) {
+  const double u = unif(s), v = unif(s);
+  return sqrt(-2.0 * log(u)) * cos(6.283185307179586 * v);
+}
+
+int mg_sur_nparams(int nin) { return SUR_H * nin + SUR_H * SUR_H + 3 * SUR_H + 1; }
+
+int mg_sur_create(int device, int nin, unsigned long long seed, mg_sur **out) {
+  if (!out || nin < 1 || nin > SUR_MAXIN) return mg_fail(MG_ERR_ARG, "mg_sur_create: nin must be 1..%d", SUR_MAXIN);
+  HIPCHK(hipSetDevice(device));
+  mg_sur *s = (mg_sur *)calloc(1, sizeof *s);
+  s->device = device;
+  s->nin = nin;
+  s->P = mg_sur_nparams(nin);
+  int r;
+  if ((r = mg_dev_alloc((void **)&s->d_p, sizeof(float) * s->P)) || (r = mg_dev_alloc((void **)&s->d_m, sizeof(float) * s->P)) ||
+      (r = mg_dev_alloc((void **)&s->d_v, sizeof(float) * s->P)) || (r = mg_dev_alloc((void **)&s->d_grad, sizeof(float) * s->P)) ||
+      (r = mg_dev_alloc((void **)&s->d_norm, sizeof(float) * (2 * nin + 2)))) {
+    mg_sur_free(s);
+    return r;
+  }
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    mg_sur_free(s);
+    return mg_fail(MG_ERR_HIP, "mg_sur_create: stream");
+  }
+  /* weights ~ N(0, 1/fan_in), biases 0 (as the synthetic potentials) */
+  float *p = (float *)calloc(s->P, sizeof(float));
+  unsigned long long st = seed;
+  for (int k = 0; k < SUR_H * nin; k++) p[k] = (float)(gauss(&st) / sqrt((double)nin));
+  float *W1 = p + SUR_H * nin + SUR_H;
+  for (int k = 0; k < SUR_H * SUR_H; k++) W1[k] = (float)(gauss(&st) / sqrt((double)SUR_H));
+  float *w2 = W1 + SUR_H * SUR_H + SUR_H;
+  for (int k = 0; k < SUR_H; k++) w2[k] = (float)(gauss(&st) / sqrt((double)SUR_H));
+  float *norm = (float *)malloc(sizeof(float) * (2 * nin + 2));
+  for (int k = 0; k < nin; k++) { norm[k] = 0.0f; norm[nin + k] = 1.0f; }
+  norm[2 * nin] = 0.0f;
+  norm[2 * nin + 1] = 1.0f;
+  hipError_t e1 = hipMemcpy(s->d_p, p, sizeof(float) * s->P, hipMemcpyHostToDevice);
+  hipError_t e2 = hipMemcpy(s->d_norm, norm, sizeof(float) * (2 * nin + 2), hipMemcpyHostToDevice);
+  hipError_t e3 = hipMemset(s->d_m, 0, sizeof(float) * s->P);
+  hipError_t e4 = hipMemset(s->d_v, 0, sizeof(float) * s->P);
+  free(p);
+  free(norm);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+    mg_sur_free(s);
+    return mg_fail(MG_ERR_HIP, "mg_sur_create: upload");
+  }
+  *out = s;
+  return MG_OK;
+}
+
+void mg_sur_free(mg_sur *s) {
+  if (!s) return;
+  hipSetDevice(s->device);
+  float *bufs[] = {s->d_p, s->d_m, s->d_v, s->d_norm, s->d_grad, s->d_part, s->d_ploss, s->d_X, s->d_Y, s->d_out};
+  for (size_t k = 0; k < sizeof bufs / sizeof bufs[0]; k++)
+    if (bufs[k]) hipFree(bufs[k]);
+  if (s->d_idx) hipFree(s->d_idx);
+  if (s->st) hipStreamDestroy(s->st);
+  free(s);
+}
+
+int mg_sur_get_params(mg_sur *s, float *p) {
+  if (!s || !p) return mg_fail(MG_ERR_ARG, "mg_sur_get_params: bad argument");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipMemcpy(p, s->d_p, sizeof(float) * s->P, hipMemcpyDeviceToHost));
+  return MG_OK;
+}
+
+int mg_sur_set_params(mg_sur *s, const float *p) {
+  if (!s || !p) return mg_fail(MG_ERR_ARG, "mg_sur_set_params: bad argument");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipMemcpy(s->d_p, p, sizeof(float) * s->P, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(s->d_m, 0, sizeof(float) * s->P));
+  HIPCHK(hipMemset(s->d_v, 0, sizeof(float) * s->P));
+  s->step = 0;
+  return MG_OK;
+}
+
+int mg_sur_set_norm(mg_sur *s, const float *mu, const float *sig, float ymu, float ysig) {
+  if (!s || !mu || !sig || !(ysig > 0.0f)) return mg_fail(MG_ERR_ARG, "mg_sur_set_norm: bad argument");
+  for (int k = 0; k < s->nin; k++)
+    if (!(sig[k] > 0.0f)) return mg_fail(MG_ERR_ARG, "mg_sur_set_norm: sig[%d] must be positive", k);
+  float *norm = (float *)malloc(sizeof(float) * (2 * s->nin + 2));
+  memcpy(norm, mu, sizeof(float) * s->nin);
+  memcpy(norm + s->nin, sig, sizeof(float) * s->nin);
+  norm[2 * s->nin] = ymu;
+  norm[2 * s->nin + 1] = ysig;
+  HIPCHK(hipSetDevice(s->device));
+  hipError_t e = hipMemcpy(s->d_norm, norm, sizeof(float) * (2 * s->nin + 2), hipMemcpyHostToDevice);
+  free(norm);
+  if (e != hipSuccess) return mg_fail(MG_ERR_HIP, "mg_sur_set_norm: %s", hipGetErrorString(e));
+  return MG_OK;
+}
+
+#define SUR_MAXBLK 256  /* blocks of a full-data gradient pass (chunk grows with n) */
+
+static int ensure_data(mg_sur *s, int n, int batch) {
+  int r;
+  if ((size_t)n > s->cap_n) {
+    size_t cap = (size_t)n + 256;
+    if ((r = mg_dev_alloc((void **)&s->d_X, sizeof(float) * cap * s->nin)) ||
+        (r = mg_dev_alloc((void **)&s->d_Y, sizeof(float) * cap)) ||
+        (r = mg_dev_alloc((void **)&s->d_out, sizeof(float) * cap)) ||
+        (r = mg_dev_alloc((void **)&s->d_idx, sizeof(int) * cap)))
+      return r;
+    s->cap_n = cap;
+  }
+  size_t nblk = (size_t)(batch + SUR_CHUNK - 1) / SUR_CHUNK;
+  if (nblk < SUR_MAXBLK) nblk = SUR_MAXBLK;
+  if (nblk > s->cap_blk) {
+    if ((r = mg_dev_alloc((void **)&s->d_part, sizeof(float) * nblk * s->P)) ||
+        (r = mg_dev_alloc((void **)&s->d_ploss, sizeof(float) * nblk)))
+      return r;
+    s->cap_blk = nblk;
+  }
+  return MG_OK;
+}
+
+static int upload_xy(mg_sur *s, const float *X, const float *y, int n) {
+  HIPCHK(hipMemcpyAsync(s->d_X, X, sizeof(float) * (size_t)n * s->nin, hipMemcpyHostToDevice, s->st));
+  if (y) HIPCHK(hipMemcpyAsync(s->d_Y, y, sizeof(float) * n, hipMemcpyHostToDevice, s->st));
+  return MG_OK;
+}
+
+/* mean squared error (normalised units) of the current parameters over the n samples now on the device */
+static int full_loss(mg_sur *s, int n, float *loss, float *grad) {
+  int chunk = (n + SUR_MAXBLK - 1) / SUR_MAXBLK;
+  if (chunk < SUR_CHUNK) chunk = SUR_CHUNK;
+  const int nblk = (n + chunk - 1) / chunk;
+  if (mgk_sur_grad(n, chunk, s->nin, NULL, s->d_X, s->d_Y, s->d_p, s->d_norm, s->d_part, s->d_ploss, s->st) ||
+      mgk_sur_adam(s->P, nblk, s->d_part, s->d_p, s->d_m, s->d_v, s->d_grad, 0.0f, 0.f, 0.f, 0.f, 1.f, 1.f, s->st))
+    return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
+  float *pl = (float *)malloc(sizeof(float) * nblk);
+  hipError_t e = hipMemcpyAsync(pl, s->d_ploss, sizeof(float) * nblk, hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess && grad) e = hipMemcpyAsync(grad, s->d_grad, sizeof(float) * s->P, hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  double sum = 0.0;
+  for (int b = 0; b < nblk; b++) sum += pl[b];
+  free(pl);
+  if (e != hipSuccess) return mg_fail(MG_ERR_HIP, "surrogate loss: %s", hipGetErrorString(e));
+  if (loss) *loss = (float)(sum / n);
+  return MG_OK;
+}
+
+int mg_sur_grad(mg_sur *s, const float *X, const float *y, int n, float *grad, float *loss) {
+  if (!s || !X || !y || n < 1) return mg_fail(MG_ERR_ARG, "mg_sur_grad: bad argument");
+  HIPCHK(hipSetDevice(s->device));
+  int r;
+  if ((r = ensure_data(s, n, 1)) || (r = upload_xy(s, X, y, n))) return r;
+  return full_loss(s, n, loss, grad);
+}
+
+int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, int batch, float lr,
+                 unsigned long long seed, float *loss) {
+  if (!s || !X || !y || n < 1 || epochs < 0 || batch < 1 || !(lr > 0.0f))
+    return mg_fail(MG_ERR_ARG, "mg_sur_train: bad argument");
+  HIPCHK(hipSetDevice(s->device));
+  int r;
+  if (batch > n) batch = n;
+  if ((r = ensure_data(s, n, batch)) || (r = upload_xy(s, X, y, n))) return r;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int *perm = (int *)malloc(sizeof(int) * n);
+  unsigned long long st = seed;
+  for (int ep = 0; ep < epochs; ep++) {
+    for (int i = 0; i < n; i++) perm[i] = i;
+    for (int i = n - 1; i > 0; i--) { /* Fisher-Yates, seeded */
+      int j = (int)(splitmix64(&st) % (unsigned long long)(i + 1));
+      int t = perm[i]; perm[i] = perm[j]; perm[j] = t;
+    }
+    /* the previous epoch's kernels read d_idx: order the copy after them on the same stream */
+    if (hipMemcpyAsync(s->d_idx, perm, sizeof(int) * n, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+        hipStreamSynchronize(s->st) != hipSuccess) {
+      free(perm);
+      return mg_fail(MG_ERR_HIP, "mg_sur_train: index upload");
+    }
+    for (int b0 = 0; b0 < n; b0 += batch) {
+      const int B = n - b0 < batch ? n - b0 : batch;
+      const int nblk = (B + SUR_CHUNK - 1) / SUR_CHUNK;
+      s->step++;
+      const float c1 = 1.0f - powf(b1, (float)s->step), c2 = 1.0f - powf(b2, (float)s->step);
+      if (mgk_sur_grad(B, SUR_CHUNK, s->nin, s->d_idx + b0, s->d_X, s->d_Y, s->d_p, s->d_norm, s->d_part, s->d_ploss, s->st) ||
+          mgk_sur_adam(s->P, nblk, s->d_part, s->d_p, s->d_m, s->d_v, NULL, lr, b1, b2, eps, c1, c2, s->st)) {
+        free(perm);
+        return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
+      }
+    }
+  }
+  free(perm);
+  return full_loss(s, n, loss, NULL);
+}
+
+int mg_sur_predict(mg_sur *s, const float *X, int n, float *y) {
+  if (!s || !X || !y || n < 0) return mg_fail(MG_ERR_ARG, "mg_sur_predict: bad argument");
+  if (n == 0) return MG_OK;
+  HIPCHK(hipSetDevice(s->device));
+  int r;
+  if ((r = ensure_data(s, n, 1)) || (r = upload_xy(s, X, NULL, n))) return r;
+  if (mgk_sur_predict(n, s->nin, s->d_X, s->d_p, s->d_norm, s->d_out, s->st))
+    return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
+  HIPCHK(hipMemcpyAsync(y, s->d_out, sizeof(float) * n, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return MG_OK;
+}
+
+int mg_features(mg_ctx *c, float *feat) {
+  if (!c || !feat) return mg_fail(MG_ERR_ARG, "mg_features: bad argument");
+  if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_features: no evaluation yet");
+  const mg_model *m = c->m;
+  HIPCHK(hipSetDevice(c->device));
+  const int nf = m->nspc * (m->ng2 + m->ng4);
+  float *d_feat = NULL;
+  int r;
+  if ((r = mg_dev_alloc((void **)&d_feat, sizeof(float) * (size_t)c->S * nf))) return r;
+  if (mgk_sur_features(c->S, c->d_str, c->d_spc, c->d_G, m->D, m->nspc, m->ng2, m->ng4, m->basis.ind2[0],
+                       m->basis.ind4[0][0], d_feat, c->st)) {
+    hipFree(d_feat);
+    return mg_fail(MG_ERR_HIP, "feature kernel launch failed");
+  }
+  hipError_t e = hipMemcpyAsync(feat, d_feat, sizeof(float) * (size_t)c->S * nf, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  hipFree(d_feat);
+  if (e != hipSuccess) return mg_fail(MG_ERR_HIP, "mg_features: %s", hipGetErrorString(e));
+  return MG_OK;
+}

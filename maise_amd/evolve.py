@@ -1,0 +1,267 @@
+"""Evolutionary structure search with the learned screening surrogate.
+
+The generation loop of the reference's EVOS caller (src/emod.c:484-757) with its bulk operators:
+  crossover  cut-and-splice after BULK_MATE (src/efnc.c:2021-2310): each parent is shifted by a random
+             fractional vector, the first parent is cut by a random plane through a point near the cell centre,
+             the second parent's complementary slab is taken with the plane offset stepped until the atom count
+             and composition come out right; the child keeps the lattice of one parent (MATE_LT,
+             src/eutl.c:139-172: the second parent's lattice, cyclically permuted to best match the first, with
+             probability equal to the first parent's share of atoms) and is rejected when its volume differs by
+             10% or more from the parents' mean;
+  mutation   Gaussian displacement of every atom (sigma in A), then the minimum-distance check;
+  selection  the population is ranked by relaxed energy per atom (RANK_TR, src/emod.c:304-444), near-duplicate
+             energies are dropped, and parents are drawn with weights that fall off with rank.
+All structures are relaxed on the GPU in one batch (mg_relax, positions only, cell fixed).
+
+Surrogate screening: each generation's offspring are evaluated once unrelaxed (one batched descriptor pass),
+the surrogate predicts their relaxed energy per atom from the species means of the same-species descriptor
+block, and only the predicted best `keep_top` fraction plus a random `keep_random` fraction of the rest are
+relaxed.  The surrogate is retrained after every generation on all (unrelaxed features, relaxed energy) pairs
+relaxed so far.  With `audit=True` every offspring is also relaxed (separately timed) so that top-k recall and
+the surrogate's energy error are measured against full relaxation.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import Context, Model, _check, _dp, _ip, lib
+from .surrogate import Surrogate
+
+Structure = Tuple[np.ndarray, np.ndarray, np.ndarray]  # (species[N], lattice rows[3,3], Cartesian x[N,3])
+
+
+@dataclass
+class GAConfig:
+    pop: int = 512
+    generations: int = 30
+    counts: Tuple[int, ...] = (16, 8)          # atoms per species (Mg16Ca8: Mg:Ca = 2:1)
+    vol_range: Tuple[float, float] = (24.0, 32.0)  # A^3 per atom of the initial random cells
+    sigma: float = 0.3                         # Gaussian mutation (A)
+    dmin: float = 2.0                          # minimum interatomic distance of a new structure (A)
+    keep_top: float = 0.25
+    keep_random: float = 0.10
+    fmax: float = 0.01
+    max_steps: int = 1500
+    urep: float = 100.0
+    seed: int = 5
+    epochs: int = 60
+    batch: int = 64
+    lr: float = 1e-3
+    audit: bool = True                         # also relax every offspring to measure recall and error
+
+
+@dataclass
+class GenerationReport:
+    gen: int
+    offspring: int
+    relaxed: int
+    seconds_screened: float                    # features + prediction + relaxation of the kept + training
+    seconds_full: Optional[float] = None       # relaxation of every offspring (audit)
+    recall_top10: Optional[float] = None
+    recall_top25: Optional[float] = None
+    mae_eV_atom: Optional[float] = None
+    best_eV_atom: float = math.nan
+    train_loss: Optional[float] = None
+
+
+@dataclass
+class GAResult:
+    reports: List[GenerationReport] = field(default_factory=list)
+    population_E: Optional[np.ndarray] = None
+    best: Optional[Structure] = None
+
+
+# ------------------------------------------------------------------ geometry helpers
+def lattice_params(lat: np.ndarray):
+    a, b, c = (np.linalg.norm(v) for v in lat)
+    al = math.acos(np.dot(lat[1], lat[2]) / (b * c))
+    be = math.acos(np.dot(lat[0], lat[2]) / (a * c))
+    ga = math.acos(np.dot(lat[0], lat[1]) / (a * b))
+    return np.array([a, b, c]), np.array([al, be, ga])
+
+
+def lattice_from_params(L, A) -> np.ndarray:
+    a, b, c = L
+    al, be, ga = A
+    ca, cb, cg, sg = math.cos(al), math.cos(be), math.cos(ga), math.sin(ga)
+    cx = c * cb
+    cy = c * (ca - cb * cg) / sg
+    cz2 = c * c - cx * cx - cy * cy
+    if cz2 <= 1e-6:
+        return None
+    return np.array([[a, 0.0, 0.0], [b * cg, b * sg, 0.0], [cx, cy, math.sqrt(cz2)]])
+
+
+def min_distance(lat: np.ndarray, x: np.ndarray) -> float:
+    """Smallest periodic interatomic distance (native, mg_evo_min_distance)."""
+    lat = np.ascontiguousarray(lat, dtype=np.float64).reshape(9)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    return float(lib().mg_evo_min_distance(_dp(lat), _dp(x), len(x) // 3))
+
+
+def random_structure(cfg: GAConfig, rng: np.random.Generator) -> Structure:
+    from .workloads import random_triclinic
+    n = sum(cfg.counts)
+    _, lat, x = random_triclinic(n, 1, rng, vol_range=cfg.vol_range, dmin=cfg.dmin)
+    spc = np.repeat(np.arange(len(cfg.counts)), cfg.counts).astype(np.int32)
+    return spc, lat, x
+
+
+# ------------------------------------------------------------------ operators (native: mg_evo_offspring)
+def make_offspring(parents: List[Structure], weights: np.ndarray, cfg: GAConfig, rng: np.random.Generator,
+                   count: int, max_tries: int = 500) -> List[Structure]:
+    """count children by crossover + Gaussian mutation + minimum-distance test; a slot whose operators keep
+    failing gets a fresh random structure."""
+    N = len(parents[0][0])
+    spc = np.ascontiguousarray(np.concatenate([p[0] for p in parents]), dtype=np.int32)
+    lat = np.ascontiguousarray(np.concatenate([np.asarray(p[1]).reshape(9) for p in parents]), dtype=np.float64)
+    x = np.ascontiguousarray(np.concatenate([np.asarray(p[2]).reshape(-1) for p in parents]), dtype=np.float64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    os_ = np.zeros(count * N, dtype=np.int32)
+    ol = np.zeros(count * 9)
+    ox = np.zeros(count * 3 * N)
+    failed = np.zeros(count, dtype=np.int32)
+    nspc = len(cfg.counts)
+    rc = lib().mg_evo_offspring(len(parents), N, nspc, _ip(spc), _dp(lat), _dp(x), _dp(w), count, float(cfg.sigma),
+                                float(cfg.dmin), int(rng.integers(1 << 62)), int(max_tries), _ip(os_), _dp(ol), _dp(ox),
+                                _ip(failed))
+    if rc < 0:
+        _check(rc, "mg_evo_offspring")
+    out: List[Structure] = []
+    for k in range(count):
+        if failed[k]:
+            out.append(random_structure(cfg, rng))
+        else:
+            out.append((os_[k * N:(k + 1) * N].copy(), ol[9 * k:9 * k + 9].reshape(3, 3).copy(),
+                        ox[3 * N * k:3 * N * (k + 1)].reshape(N, 3).copy()))
+    return out
+
+
+# ------------------------------------------------------------------ the search
+class SurrogateGA:
+    def __init__(self, model_path: str, cfg: GAConfig, device: int = 0):
+        self.cfg = cfg
+        self.model = Model(model_path)
+        self.device = device
+        self.rng = np.random.default_rng(cfg.seed)
+        self.nin = self.model.nspc * (self.model.ng2 + self.model.ng4)
+        self.sur = Surrogate(self.nin, device=device, seed=cfg.seed)
+        self.trained = False
+        self.X: List[np.ndarray] = []
+        self.y: List[np.ndarray] = []
+        self._ctx_eval = Context(self.model, device=device, urep=cfg.urep)
+        self._ctx_relax = Context(self.model, device=device, urep=cfg.urep)
+
+    def close(self):
+        self.sur.close()
+        self._ctx_eval.close()
+        self._ctx_relax.close()
+
+    def features(self, structs: List[Structure]) -> np.ndarray:
+        c = self._ctx_eval
+        c.set_batch(structs)
+        c.eval(forces=False)
+        return c.features()
+
+    def relax(self, structs: List[Structure]):
+        """Relax in one batch; returns (relaxed structures, energy per atom, converged flags)."""
+        c = self._ctx_relax
+        c.set_batch(structs)
+        _, done = c.relax(max_steps=self.cfg.max_steps, fmax=self.cfg.fmax)
+        E = c.batch_result(forces=False)["E"]
+        out, epa = [], np.zeros(len(structs))
+        for k, (spc, lat, _) in enumerate(structs):
+            X = c.result(k, forces=False)["X"]
+            out.append((spc, lat, X))
+            epa[k] = E[k] / len(spc)
+        return out, epa, done
+
+    def train(self) -> float:
+        X = np.concatenate(self.X)
+        y = np.concatenate(self.y)
+        self.sur.fit_norm(X, y)
+        loss = self.sur.train(X, y, epochs=self.cfg.epochs, batch=self.cfg.batch, lr=self.cfg.lr,
+                              seed=int(self.rng.integers(1 << 62)))
+        self.trained = True
+        return loss
+
+    def select(self, pred: np.ndarray) -> np.ndarray:
+        n = len(pred)
+        ntop = int(round(self.cfg.keep_top * n))
+        nrand = int(round(self.cfg.keep_random * n))
+        order = np.argsort(pred, kind="stable")
+        top = order[:ntop]
+        rest = order[ntop:]
+        extra = self.rng.choice(rest, size=min(nrand, len(rest)), replace=False) if nrand else np.array([], int)
+        return np.sort(np.concatenate([top, extra]).astype(int))
+
+    def update_population(self, pop: List[Structure], popE: np.ndarray, new: List[Structure], newE: np.ndarray,
+                          tol: float = 1e-4):
+        allS = pop + new
+        allE = np.concatenate([popE, newE])
+        order = np.argsort(allE, kind="stable")
+        keep, last = [], None
+        for k in order:
+            if not np.isfinite(allE[k]):
+                continue
+            if last is not None and abs(allE[k] - last) < tol:
+                continue                           # near-duplicate energy: treated as the same structure
+            keep.append(k)
+            last = allE[k]
+            if len(keep) == self.cfg.pop:
+                break
+        return [allS[k] for k in keep], allE[keep]
+
+    def run(self, log=None) -> GAResult:
+        cfg = self.cfg
+        res = GAResult()
+        # generation 0: random structures, all relaxed, first training set
+        t0 = time.perf_counter()
+        init = [random_structure(cfg, self.rng) for _ in range(cfg.pop)]
+        feat = self.features(init)
+        relaxed, epa, done = self.relax(init)
+        self.X.append(feat)
+        self.y.append(epa.astype(np.float32))
+        loss = self.train()
+        pop, popE = self.update_population([], np.zeros(0), relaxed, np.where(done, epa, np.inf))
+        res.reports.append(GenerationReport(0, len(init), len(init), time.perf_counter() - t0,
+                                            best_eV_atom=float(popE[0]), train_loss=loss))
+        if log:
+            log(res.reports[-1])
+        for g in range(1, cfg.generations + 1):
+            ranks = np.arange(len(pop))
+            weights = np.exp(-ranks / max(1.0, 0.25 * len(pop)))
+            kids = make_offspring(pop, weights, cfg, self.rng, cfg.pop)
+            t0 = time.perf_counter()
+            feat = self.features(kids)
+            pred = self.sur.predict(feat)
+            sel = self.select(pred)
+            relaxed, epa, done = self.relax([kids[k] for k in sel])
+            self.X.append(feat[sel])
+            self.y.append(epa.astype(np.float32))
+            loss = self.train()
+            ts = time.perf_counter() - t0
+            rep = GenerationReport(g, len(kids), len(sel), ts, train_loss=loss)
+            if cfg.audit:
+                t1 = time.perf_counter()
+                _, epa_all, done_all = self.relax(kids)
+                rep.seconds_full = time.perf_counter() - t1
+                order = np.argsort(epa_all, kind="stable")
+                sset = set(sel.tolist())
+                for frac_k, name in ((0.10, "recall_top10"), (0.25, "recall_top25")):
+                    k = max(1, int(round(frac_k * len(kids))))
+                    setattr(rep, name, len(sset.intersection(order[:k].tolist())) / k)
+                rep.mae_eV_atom = float(np.mean(np.abs(pred.astype(np.float64) - epa_all)))
+            pop, popE = self.update_population(pop, popE, relaxed, np.where(done, epa, np.inf))
+            rep.best_eV_atom = float(popE[0])
+            res.reports.append(rep)
+            if log:
+                log(rep)
+        res.population_E = popE
+        res.best = pop[0]
+        return res
