@@ -89,6 +89,12 @@ int mg_ctx_set_timing(mg_ctx *c, int on);
 /* Launch shape of the wave-per-atom descriptor and force kernels: atoms (64-lane waves) per workgroup, 1, 2 or 4;
  * 0 (the default) picks the largest whose LDS tile fits 64 KiB.  No counterpart in the reference (CPU only). */
 int mg_ctx_set_wpb(mg_ctx *c, int wpb_desc, int wpb_force);
+/* Relaxation and MD replay one captured iteration (evaluation chain + integrator step) as a HIP graph, with the
+ * neighbour-edge buffers sized from the last synchronised evaluation plus `headroom` (fraction; default 0.125
+ * and 8 extra row slots, 0 = exact).  A step that outgrows them is detected on the device, rolled back to the
+ * last checkpoint (every 8 iterations) and repeated with larger buffers, so results do not depend on this
+ * setting.  enable = 0 runs every iteration synchronised.  No counterpart in the reference (CPU only). */
+int mg_ctx_set_graph(mg_ctx *c, int enable, double headroom);
 /* Autotuner: times 1, 2 and 4 waves per workgroup on the structure/batch currently set (reps evaluations each)
  * and keeps the fastest shape per kernel; wpb_out[2] (may be NULL) receives (descriptor, force). */
 int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out);

@@ -54,6 +54,7 @@ def lib():
             "mg_ctx_set_timing": ([vp, C.c_int], C.c_int),
             "mg_ctx_set_wpb": ([vp, C.c_int, C.c_int], C.c_int),
             "mg_ctx_autotune": ([vp, C.c_int, ip], C.c_int),
+            "mg_ctx_set_graph": ([vp, C.c_int, C.c_double], C.c_int),
             "mg_set_structure": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp], C.c_int),
             "mg_set_batch": ([vp, C.c_int, ip, ip, dp, ip, dp], C.c_int),
             "mg_eval": ([vp, C.c_int], C.c_int),
@@ -166,6 +167,12 @@ class Context:
     def set_wpb(self, wpb_desc: int = 0, wpb_force: int = 0):
         """Atoms (waves) per workgroup of the descriptor and force kernels (0 = largest that fits 64 KiB LDS)."""
         _check(lib().mg_ctx_set_wpb(self._h, int(wpb_desc), int(wpb_force)), "mg_ctx_set_wpb")
+
+    def set_graph(self, enable: bool = True, headroom: float = 0.125):
+        """Relaxation/MD replay a captured HIP graph per iteration (edge buffers sized with this headroom;
+        overflow is detected and rolled back, so results do not depend on it); enable=False synchronises every
+        iteration."""
+        _check(lib().mg_ctx_set_graph(self._h, 1 if enable else 0, float(headroom)), "mg_ctx_set_graph")
 
     def autotune(self, reps: int = 3):
         """Time 1/2/4 waves per workgroup on the current structure or batch; keep the fastest per kernel."""

@@ -114,6 +114,8 @@ int mg_ctx_create(const mg_model *m, int device, mg_ctx **out) {
   if (device < 0 || device >= ndev) return mg_fail(MG_ERR_ARG, "device %d not present (%d devices)", device, ndev);
   HIPCHK(hipSetDevice(device));
   mg_ctx *c = (mg_ctx *)calloc(1, sizeof(mg_ctx));
+  c->graph_enable = 1;
+  c->graph_headroom = 0.125;
   c->m = m;
   c->device = device;
   if (hipStreamCreate(&c->st) != hipSuccess) { free(c); return mg_fail(MG_ERR_HIP, "hipStreamCreate failed"); }
@@ -151,7 +153,8 @@ void mg_ctx_free(mg_ctx *c) {
                   c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
                   c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
                   c->d_vir, c->d_fself, c->d_fe, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
-                  c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace};
+                  c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
+                  c->d_snap_dyn};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
     if (ptrs[k]) hipFree(ptrs[k]);
   for (int k = 0; k < EV_N; k++) hipEventDestroy(c->ev[k]);
@@ -396,67 +399,90 @@ int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const dou
   do {                                                                                                              \
     int kr_ = (x);                                                                                                  \
     if (kr_ == -3) return mg_fail(MG_ERR_NEIGHBOR, "neighbour rows too long for the LDS of one CU (%d)", c->maxrow); \
-    if (kr_ != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, hipGetErrorString(hipGetLastError())); \
+    if (kr_ != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error());               \
   } while (0)
 
-int mg_eval(mg_ctx *c, int flags) {
-  if (!c) return mg_fail(MG_ERR_ARG, "null context");
-  if (c->S == 0) return mg_fail(MG_ERR_ARG, "no structure set");
-  HIPCHK(hipSetDevice(c->device));
+int mgh_graph_caps(mg_ctx *c, int need_edges, int need_row) {
+  const double h = c->graph_headroom;
+  const size_t ne = (size_t)need_edges + (size_t)(h * need_edges) + (h > 0.0 ? 256 : 0);
+  if (ne > 0x7fffffff) return mg_fail(MG_ERR_ARG, "too many neighbour edges (%zu)", ne);
+  int r;
+  if ((r = ensure_edges(c, ne))) return r;
+  c->gcap_edges = (int)ne;
+  c->gcap_row = need_row + (h > 0.0 ? 8 : 0);
+  return MG_OK;
+}
+
+int mg_ctx_set_graph(mg_ctx *c, int enable, double headroom) {
+  if (!c || headroom < 0.0 || headroom > 10.0) return mg_fail(MG_ERR_ARG, "mg_ctx_set_graph: bad argument");
+  c->graph_enable = enable != 0;
+  c->graph_headroom = headroom;
+  return MG_OK;
+}
+
+int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   const mg_model *m = c->m;
   const int N = c->N;
   hipStream_t st = c->st;
-  const int forces = (flags & MG_EVAL_FORCES) != 0;
   const float urep = (float)c->urep;
-  if (c->prec != MG_PREC_FP32 && !m->mfma_ok) return mg_fail(MG_ERR_UNSUPPORTED, "BF16 MLP needs hidden layers <= 64 wide");
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_START], st));
-  /* neighbour list */
-  HIPCHK(hipMemsetAsync(c->d_bin_count, 0, (c->nbins + 1) * sizeof(int), st));
-  HIPCHK(hipMemsetAsync(c->d_bin_cur, 0, (c->nbins + 1) * sizeof(int), st));
-  HIPCHK(hipMemsetAsync(c->d_misc, 0, 16 * sizeof(int), st));
+  const int timing = c->timing && mode == MG_CHAIN_SYNC;
+  const int *ovf = c->d_misc + 3;
+  if (timing) HIPCHK(hipEventRecord(c->ev[EV_START], st));
+  /* neighbour list: bins, row counts, row offsets; graph mode keeps the flags of misc[1..5] for the caller */
+  KCHK(mgk_clear(c->d_bin_count, c->nbins + 1, c->d_bin_cur, c->nbins + 1, c->d_misc, mode == MG_CHAIN_SYNC ? 16 : 1, st));
   KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, st));
   KCHK(mgk_scan(c->d_bin_count, c->d_bin_start, c->nbins, c->d_tmp, &c->tmp_bytes, st));
   KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, st));
   KCHK(mgk_nbr_count(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_atoms, c->d_cnt,
                      c->d_rowcnt, c->d_misc, st));
   KCHK(mgk_scan(c->d_rowcnt, c->d_row_off, N, c->d_tmp, &c->tmp_bytes, st));
-  int hm[2];
-  HIPCHK(hipMemcpyAsync(&hm[0], c->d_row_off + N, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&hm[1], c->d_misc, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  c->nedges = hm[0];
-  c->maxrow = hm[1];
-  int r;
-  if ((r = ensure_edges(c, (size_t)c->nedges))) return r;
+  int cap_row;
+  if (mode == MG_CHAIN_SYNC) {
+    int hm[2];
+    HIPCHK(hipMemcpyAsync(&hm[0], c->d_row_off + N, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hm[1], c->d_misc, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->nedges = hm[0];
+    c->maxrow = hm[1];
+    int r;
+    if ((r = ensure_edges(c, (size_t)c->nedges))) return r;
+    cap_row = c->maxrow;
+    KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->nedges, cap_row, st));
+  } else {
+    cap_row = c->gcap_row;
+    KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->gcap_edges, cap_row, st));
+  }
   KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_atoms, c->d_row_off,
-                    c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, c->maxrow, st));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
+                    c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, ovf, cap_row, st));
+  if (timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
   /* descriptors, MLP, forces */
   KCHK(mgk_desc(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
-                urep, c->maxrow, c->wpb_desc, st));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_DESC], st));
+                urep, cap_row, c->wpb_desc, ovf, st));
+  if (timing) HIPCHK(hipEventRecord(c->ev[EV_DESC], st));
   if (m->mfma_ok)
     KCHK(mgk_mlp_mfma(c->ntiles, c->d_tiles, c->d_perm, c->d_G, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, m->fdesc,
                       c->prec, c->d_Ei, c->d_dEdG, st));
   else
     KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_MLP], st));
+  if (timing) HIPCHK(hipEventRecord(c->ev[EV_MLP], st));
   if (forces) {
     KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
-                   c->d_fe, c->d_vir, urep, c->maxrow, c->wpb_force, st));
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
-    KCHK(mgk_gather(N, c->d_row_off, c->d_rev, c->d_fself, c->d_fe, c->d_F, st));
-  } else if (c->timing) {
+                   c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, ovf, st));
+    if (timing) HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
+    KCHK(mgk_gather(N, c->d_row_off, c->d_rev, c->d_fself, c->d_fe, c->d_F, ovf, st));
+  } else if (timing) {
     HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
   }
   KCHK(mgk_reduce(c->S, c->nchunk, c->d_chunk_str, c->d_str, c->d_Ei, c->d_erep, c->d_vir, forces, c->d_part, c->d_E,
                   c->d_U, st));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_END], st));
-  int err = 0;
-  HIPCHK(hipMemcpyAsync(&err, c->d_misc + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  if (timing) HIPCHK(hipEventRecord(c->ev[EV_END], st));
+  if (mode == MG_CHAIN_GRAPH) return MG_OK;
+  int flags[4];
+  HIPCHK(hipMemcpyAsync(flags, c->d_misc, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (err) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)");
-  if (c->timing) {
+  if (flags[3]) return mg_fail(MG_ERR_HIP, "edge capacity check failed in a synchronised evaluation");
+  if (flags[1]) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)");
+  if (timing) {
     hipEventElapsedTime(&c->tms[0], c->ev[EV_START], c->ev[EV_NBR]);
     hipEventElapsedTime(&c->tms[1], c->ev[EV_NBR], c->ev[EV_DESC]);
     hipEventElapsedTime(&c->tms[2], c->ev[EV_DESC], c->ev[EV_MLP]);
@@ -464,6 +490,17 @@ int mg_eval(mg_ctx *c, int flags) {
     hipEventElapsedTime(&c->tms[4], c->ev[EV_FORCE], c->ev[EV_END]);
     hipEventElapsedTime(&c->tms[5], c->ev[EV_START], c->ev[EV_END]);
   }
+  return MG_OK;
+}
+
+int mg_eval(mg_ctx *c, int flags) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  if (c->S == 0) return mg_fail(MG_ERR_ARG, "no structure set");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->prec != MG_PREC_FP32 && !c->m->mfma_ok) return mg_fail(MG_ERR_UNSUPPORTED, "BF16 MLP needs hidden layers <= 64 wide");
+  const int forces = (flags & MG_EVAL_FORCES) != 0;
+  int r = mgh_eval_chain(c, forces, MG_CHAIN_SYNC);
+  if (r != MG_OK) return r;
   c->have_forces = forces;
   c->have_g = 1;
   return MG_OK;

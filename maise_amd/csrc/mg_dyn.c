@@ -37,21 +37,25 @@ int mgk_fire(int S, const mg_struct_dev *str, const double *F, double *v, const 
              double fmax, double dtmax, double maxstep, hipStream_t st);
 int mgk_fire_count(int S, const double *state, int *out, hipStream_t st);
 
-#define KCHK(x) do { if ((x) != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s", #x); } while (0)
+#define KCHK(x) do { if ((x) != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error()); } while (0)
 
 static int ensure_dyn(mg_ctx *c) {
   int r;
   if (c->cap_dyn_atoms < (size_t)c->N || !c->d_v) {
     size_t n = c->N + 16;
     if ((r = mg_dev_alloc((void **)&c->d_v, n * 3 * sizeof(double))) ||
-        (r = mg_dev_alloc((void **)&c->d_mass, n * sizeof(double))))
+        (r = mg_dev_alloc((void **)&c->d_mass, n * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_x, n * 3 * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_v, n * 3 * sizeof(double))))
       return r;
     c->cap_dyn_atoms = n;
   }
   if (c->cap_dyn_str < (size_t)c->S || !c->d_fire) {
     size_t n = c->S + 4;
     if ((r = mg_dev_alloc((void **)&c->d_fire, n * 5 * sizeof(double))) ||
-        (r = mg_dev_alloc((void **)&c->d_dyn, 16 * sizeof(double))))
+        (r = mg_dev_alloc((void **)&c->d_snap_fire, n * 5 * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_dyn, 16 * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_dyn, 16 * sizeof(double))))
       return r;
     c->cap_dyn_str = n;
   }
@@ -136,6 +140,60 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
  * is re-packed to the unconverged structures (positions, velocities and FIRE state carried over), so the tail
  * of the relaxation no longer evaluates structures that have stopped.  At the end the full batch is restored
  * with the final positions and evaluated once more, so mg_get / mg_get_batch see every structure. */
+/* ---- graph mode: one iteration (evaluation chain + integrator step) captured once and replayed; the
+ * edge buffers are sized from the last synchronised evaluation with headroom, the kernels raise an overflow
+ * flag instead of overrunning them, and every MG_GRAPH_CHECK replays the host checks the flags and, on
+ * overflow, restores the state saved at the previous check, grows the buffers and re-captures. */
+#define MG_GRAPH_CHECK 8
+
+typedef int (*step_fn)(mg_ctx *c, void *arg);
+
+static int graph_capture(mg_ctx *c, step_fn tail, void *arg, hipGraphExec_t *exec) {
+  hipGraph_t g = NULL;
+  HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  int r = mgh_eval_chain(c, 1, MG_CHAIN_GRAPH);
+  if (r == MG_OK && tail) r = tail(c, arg);
+  hipError_t e = hipStreamEndCapture(c->st, &g);
+  if (r != MG_OK) {
+    if (e == hipSuccess && g) hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return mg_fail(MG_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(exec, g, NULL, NULL, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return mg_fail(MG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  return MG_OK;
+}
+
+/* save / restore positions, velocities, per-structure FIRE state and thermostat scalars */
+static int snapshot(mg_ctx *c, int restore) {
+  const size_t nx = sizeof(double) * 3 * c->N, nf = sizeof(double) * 5 * c->S;
+  hipMemcpyKind k = hipMemcpyDeviceToDevice;
+  if (!restore) {
+    HIPCHK(hipMemcpyAsync(c->d_snap_x, c->d_xin, nx, k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_snap_v, c->d_v, nx, k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_snap_fire, c->d_fire, nf, k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_snap_dyn, c->d_dyn, 16 * sizeof(double), k, c->st));
+  } else {
+    HIPCHK(hipMemcpyAsync(c->d_xin, c->d_snap_x, nx, k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_v, c->d_snap_v, nx, k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_fire, c->d_snap_fire, nf, k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_dyn, c->d_snap_dyn, 16 * sizeof(double), k, c->st));
+  }
+  return MG_OK;
+}
+
+typedef struct {
+  double fmax, dtmax, maxstep;
+} fire_arg;
+
+static int fire_tail(mg_ctx *c, void *arg) {
+  const fire_arg *a = (const fire_arg *)arg;
+  return mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, a->fmax, a->dtmax, a->maxstep, c->st)
+             ? mg_fail(MG_ERR_HIP, "kernel launch failed: mgk_fire (%s)", mgk_last_error())
+             : MG_OK;
+}
+
 typedef struct {
   int S0, N0;
   int *off0, *nd0, *spc0;
@@ -229,41 +287,80 @@ int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, do
     return mg_fail(MG_ERR_HIP, "mg_relax: species download failed");
   }
   int compacted = 0;
-#define RCHK(x) do { if ((r = (x)) != MG_OK) { book_free(&b); return r; } } while (0)
-#define RKCHK(x) do { if ((x) != 0) { book_free(&b); return mg_fail(MG_ERR_HIP, "kernel launch failed: %s", #x); } } while (0)
-#define RHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { book_free(&b); return mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+#define RCHK(x) do { if ((r = (x)) != MG_OK) { if (gx) hipGraphExecDestroy(gx); book_free(&b); return r; } } while (0)
+#define RKCHK(x) do { if ((x) != 0) { if (gx) hipGraphExecDestroy(gx); book_free(&b); return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error()); } } while (0)
+#define RHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { if (gx) hipGraphExecDestroy(gx); book_free(&b); return mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+  hipGraphExec_t gx = NULL;
+  fire_arg fa = {fmax, dtmax, maxstep};
   RHIP(hipMemsetAsync(c->d_v, 0, sizeof(double) * 3 * c->N, c->st));
   RKCHK(mgk_fire_init(c->S, c->d_fire, dt0, c->st));
-  int ndone = 0;
-  for (int it = 0; it < max_steps; it++) {
-    RCHK(mg_eval(c, MG_EVAL_FORCES));
-    RKCHK(mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, fmax, dtmax, maxstep, c->st));
-    if (it == 0) {
-      int any_frac = 0;
-      for (int s = 0; s < c->S; s++) if (c->h_str[s].xt == 0) { c->h_str[s].xt = 1; any_frac = 1; }
-      if (any_frac) RHIP(hipMemcpyAsync(c->d_str, c->h_str, sizeof(mg_struct_dev) * c->S, hipMemcpyHostToDevice, c->st));
-    }
-    if ((it % 8) == 7) {
-      RHIP(hipMemsetAsync(c->d_misc + 2, 0, sizeof(int), c->st));
-      RKCHK(mgk_fire_count(c->S, c->d_fire, c->d_misc + 2, c->st));
-      RHIP(hipMemcpyAsync(&ndone, c->d_misc + 2, sizeof(int), hipMemcpyDeviceToHost, c->st));
-      RHIP(hipStreamSynchronize(c->st));
-      if (ndone == c->S) break;
-      if (ndone > 0 && c->S > 1) {
-        /* atoms still moving, from the per-structure done flags */
-        double *st = (double *)malloc(sizeof(double) * 5 * c->S);
-        RHIP(hipMemcpy(st, c->d_fire, sizeof(double) * 5 * c->S, hipMemcpyDeviceToHost));
-        long moving = 0;
-        for (int k = 0; k < c->S; k++)
-          if (st[5 * k + 3] == 0.0) moving += c->h_atom_off[k + 1] - c->h_atom_off[k];
-        free(st);
-        if (2 * moving <= c->N) {
-          RCHK(compact(c, &b, 1));
-          compacted = 1;
+  int it = 0, need_e = 0, need_r = 0;
+  while (it < max_steps) {
+    int nrun;
+    if (c->graph_enable && gx) {
+      nrun = max_steps - it < MG_GRAPH_CHECK ? max_steps - it : MG_GRAPH_CHECK;
+      for (int k = 0; k < nrun; k++) RHIP(hipGraphLaunch(gx, c->st));
+    } else {
+      /* synchronised iterations: all of them without graphs, else one to size the buffers before a capture */
+      nrun = c->graph_enable ? 1 : (max_steps - it < MG_GRAPH_CHECK ? max_steps - it : MG_GRAPH_CHECK);
+      for (int k = 0; k < nrun; k++) {
+        RCHK(mg_eval(c, MG_EVAL_FORCES));
+        RKCHK(mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, fmax, dtmax, maxstep, c->st));
+        if (it + k == 0) {
+          int any_frac = 0;
+          for (int s = 0; s < c->S; s++) if (c->h_str[s].xt == 0) { c->h_str[s].xt = 1; any_frac = 1; }
+          if (any_frac) RHIP(hipMemcpyAsync(c->d_str, c->h_str, sizeof(mg_struct_dev) * c->S, hipMemcpyHostToDevice, c->st));
         }
+      }
+      if (c->graph_enable) {
+        it += nrun;
+        RCHK(mgh_graph_caps(c, c->nedges > need_e ? c->nedges : need_e, c->maxrow > need_r ? c->maxrow : need_r));
+        RCHK(graph_capture(c, fire_tail, &fa, &gx));
+        RCHK(snapshot(c, 0));
+        RKCHK(mgk_clear(c->d_misc + 3, 3, NULL, 0, NULL, 0, c->st));
+        continue;
+      }
+    }
+    it += nrun;
+    int flags[6];
+    RKCHK(mgk_clear(c->d_misc + 2, 1, NULL, 0, NULL, 0, c->st));
+    RKCHK(mgk_fire_count(c->S, c->d_fire, c->d_misc + 2, c->st));
+    RHIP(hipMemcpyAsync(flags, c->d_misc, sizeof flags, hipMemcpyDeviceToHost, c->st));
+    RHIP(hipStreamSynchronize(c->st));
+    if (flags[3]) {
+      /* a row or the edge total outgrew the capture: redo these iterations with larger buffers */
+      RCHK(snapshot(c, 1));
+      it -= nrun;
+      need_e = flags[4];
+      need_r = flags[5];
+      hipGraphExecDestroy(gx);
+      gx = NULL;
+      continue;
+    }
+    if (flags[1]) RCHK(mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)"));
+    RCHK(snapshot(c, 0));
+    RKCHK(mgk_clear(c->d_misc + 3, 3, NULL, 0, NULL, 0, c->st));
+    const int ndone = flags[2];
+    if (ndone == c->S) break;
+    if (ndone > 0 && c->S > 1) {
+      /* atoms still moving, from the per-structure done flags */
+      double *st = (double *)malloc(sizeof(double) * 5 * c->S);
+      RHIP(hipMemcpy(st, c->d_fire, sizeof(double) * 5 * c->S, hipMemcpyDeviceToHost));
+      long moving = 0;
+      for (int k = 0; k < c->S; k++)
+        if (st[5 * k + 3] == 0.0) moving += c->h_atom_off[k + 1] - c->h_atom_off[k];
+      free(st);
+      if (2 * moving <= c->N) {
+        RCHK(compact(c, &b, 1));
+        compacted = 1;
+        if (gx) hipGraphExecDestroy(gx);  /* the batch changed: capture again */
+        gx = NULL;
+        need_e = need_r = 0;
       }
     }
   }
+  if (gx) hipGraphExecDestroy(gx);
+  gx = NULL;
   if (compacted) {
     /* bring the last active structures into the book and restore the full batch with the final positions */
     RCHK(compact(c, &b, 0));

@@ -137,9 +137,11 @@ __global__ __launch_bounds__(256) void k_fire(int S, const mg_struct_dev* __rest
   const int s = blockIdx.x, t = threadIdx.x;
   const int a0 = str[s].atom0, n = str[s].natom;
   double* st = state + 5 * s;
-  if (st[3] != 0.0) {  // converged earlier: keep still
+  // a converged structure keeps its positions as they are (re-wrapping them every iteration would move them by
+  // rounding, so results would depend on how long the structure stays in the batch)
+  if (st[3] != 0.0) {
     for (int i = a0 + t; i < a0 + n; i += 256)
-      for (int q = 0; q < 3; q++) { xin[3 * i + q] = x[3 * i + q]; v[3 * i + q] = 0.0; }
+      for (int q = 0; q < 3; q++) v[3 * i + q] = 0.0;
     return;
   }
   double vf = 0, ff = 0, vv = 0, fm = 0;
@@ -164,6 +166,7 @@ __global__ __launch_bounds__(256) void k_fire(int S, const mg_struct_dev* __rest
   vf = sh[0][0]; ff = sh[1][0]; vv = sh[2][0]; fm = sh[3][0];
   __syncthreads();
   if (fm < fmax * fmax) {
+    // converged now: store the wrapped Cartesian positions once (inputs may still be fractional at step 0)
     for (int i = a0 + t; i < a0 + n; i += 256)
       for (int q = 0; q < 3; q++) { xin[3 * i + q] = x[3 * i + q]; v[3 * i + q] = 0.0; }
     if (t == 0) st[3] = 1.0;
@@ -226,52 +229,52 @@ extern "C" {
 int mgk_md_maxwell(int N, unsigned long long seed, const double* mass, double kT, double* v, hipStream_t st) {
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_md_maxwell, dim3((N + 255) / 256), dim3(256), 0, st, N, seed, mass, kT, v);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_md_fix(int N, const double* mass, double* v, double kT, double ndof, hipStream_t st) {
   hipLaunchKernelGGL(k_md_fix, dim3(1), dim3(1024), 0, st, N, mass, v, kT, ndof);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_md_thermo(int N, const double* mass, const double* v, double* dyn, double dt, double kT, double ndof,
                   double tau, double* trace, int slot, hipStream_t st) {
   hipLaunchKernelGGL(k_md_thermo, dim3(1), dim3(1024), 0, st, N, mass, v, dyn, dt, kT, ndof, tau, trace, slot);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_md_kick_drift(int N, double dt, const double* mass, const double* F, double* v, const double* x, double* xin,
                       const double* dyn, hipStream_t st) {
   hipLaunchKernelGGL(k_md_kick_drift, dim3((N + 255) / 256), dim3(256), 0, st, N, dt, mass, F, v, x, xin, dyn);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_md_kick_scale(int N, double dt, const double* mass, const double* F, double* v, const double* dyn, double tau,
                       hipStream_t st) {
   hipLaunchKernelGGL(k_md_kick_scale, dim3((N + 255) / 256), dim3(256), 0, st, N, dt, mass, F, v, dyn, tau);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_copy_d(const double* src, double* dst, int n, hipStream_t st) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_copy_d, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, n);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_fire_init(int S, double* state, double dt0, hipStream_t st) {
   hipLaunchKernelGGL(k_fire_init, dim3((S + 255) / 256), dim3(256), 0, st, S, state, dt0);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_fire(int S, const mg_struct_dev* str, const double* F, double* v, const double* x, double* xin, double* state,
              double fmax, double dtmax, double maxstep, hipStream_t st) {
   hipLaunchKernelGGL(k_fire, dim3(S), dim3(256), 0, st, S, str, F, v, x, xin, state, fmax, dtmax, maxstep);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_fire_count(int S, const double* state, int* out, hipStream_t st) {
   hipLaunchKernelGGL(k_fire_count, dim3((S + 255) / 256), dim3(256), 0, st, S, state, out);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 }  // extern "C"

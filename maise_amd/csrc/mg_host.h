@@ -52,6 +52,9 @@ struct mg_ctx {
   int prec;
   int timing;
   int wpb_desc, wpb_force; /* waves (atoms) per block of the descriptor and force kernels; 0 = by LDS size */
+  int gcap_edges, gcap_row; /* graph-mode capacities: edges allocated, longest row the LDS tiles hold */
+  int graph_enable;         /* relaxation / MD replay a captured iteration (default 1) */
+  double graph_headroom;    /* fractional headroom of the graph-mode capacities (default 1/8) */
   hipEvent_t ev[EV_N];
   float tms[6];
   /* structures */
@@ -67,7 +70,9 @@ struct mg_ctx {
   int *d_atom_str, *d_spc;
   double *d_xin, *d_x;
   int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur, *d_bin_atoms;
-  int *d_cnt, *d_rowcnt, *d_row_off, *d_misc; /* misc: [0] maxrow, [1] error flag */
+  /* misc: [0] longest row of the evaluation, [1] error flag (reverse edge missing), [2] converged-structure count,
+   * [3] capacity overflow flag, [4] largest edge total and [5] longest row seen since misc was last cleared */
+  int *d_cnt, *d_rowcnt, *d_row_off, *d_misc;
   void *d_key, *d_vec;
   int *d_rev;
   float *d_G, *d_dEdG, *d_Ei, *d_erep, *d_vir;
@@ -91,6 +96,7 @@ struct mg_ctx {
   double *d_dyn;        /* scalars: [0] Nose-Hoover xi, [1] kinetic energy, ... */
   double *d_fire;       /* per-structure FIRE state [S][4]: dt, alpha, npos, done */
   double *d_trace;      /* per-step energies */
+  double *d_snap_x, *d_snap_v, *d_snap_fire, *d_snap_dyn; /* state at the last graph-mode checkpoint */
   size_t cap_dyn_atoms, cap_dyn_str, cap_trace;
 };
 
@@ -107,9 +113,10 @@ int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, c
                   const int *bin_start, const int *bin_atoms, int *cnt, int *rowcnt, int *maxrow, hipStream_t st);
 int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
                  const int *bin_start, const int *bin_atoms, const int *row_off, void *key, void *vec, int *rev, int *err,
-                 int maxrow, hipStream_t st);
+                 const int *ovf, int maxrow, hipStream_t st);
 int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
-             const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, hipStream_t st);
+             const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, const int *ovf,
+             hipStream_t st);
 int mgk_mlp_mfma(int ntiles, const void *tiles, const int *perm, const float *G, const float *frag, const void *frag_bf,
                  const void *frag_lo, const float *vecs, const int *fdesc, int prec, float *Ei, float *dEdG,
                  hipStream_t st);
@@ -117,8 +124,20 @@ int mgk_mlp(int N, const int *spc, const float *G, const float *W, int wstride, 
             float *dEdG, hipStream_t st);
 int mgk_force(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
               const mg_basis_dev *bp, const float *dEdG, void *fself, void *fe, float *vir, float urep, int maxrow,
-              int wpb, hipStream_t st);
-int mgk_gather(int N, const int *row_off, const int *rev, const void *fself, const void *fe, double *F, hipStream_t st);
+              int wpb, const int *ovf, hipStream_t st);
+int mgk_gather(int N, const int *row_off, const int *rev, const void *fself, const void *fe, double *F, const int *ovf,
+               hipStream_t st);
+int mgk_clear(int *a, int na, int *b, int nb, int *c, int nc, hipStream_t st);
+int mgk_check_caps(int N, const int *row_off, int *misc, int cap_edges, int cap_row, hipStream_t st);
+
+/* the evaluation chain.  MG_CHAIN_SYNC reads the edge count back and sizes the edge buffers exactly (one host
+ * synchronisation in the middle, one at the end); MG_CHAIN_GRAPH uses the capacities gcap_edges / gcap_row set
+ * beforehand, never synchronises or allocates, and leaves the overflow flag misc[3] for the caller to check, so
+ * it can be captured into a HIP graph. */
+enum { MG_CHAIN_SYNC = 0, MG_CHAIN_GRAPH = 1 };
+int mgh_eval_chain(struct mg_ctx *c, int forces, int mode);
+/* size the graph-mode capacities from the last evaluation with headroom, allocating edge buffers */
+int mgh_graph_caps(struct mg_ctx *c, int need_edges, int need_row);
 int mgk_reduce(int nstr, int nchunk, const int *chunk_str, const mg_struct_dev *str, const float *Ei, const float *erep,
                const float *vir, int have_vir, double *part, double *E, double *U, hipStream_t st);
 

@@ -33,6 +33,11 @@
 extern "C" {
 #endif
 
+/* status of the launches just issued: 0, or -1 with the HIP error kept for mgk_last_error() (the error the
+ * runtime reports once is consumed, so messages built afterwards would otherwise read "no error") */
+int mgk_status(void);
+const char *mgk_last_error(void);
+
 /* Basis tables, in FP32, indexed by the ordered species pair p = s0*MG_MAXSPC + s1 (symmetric). */
 typedef struct {
   int nspc, D, ng2, ng4;

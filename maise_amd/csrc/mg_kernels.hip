@@ -17,7 +17,6 @@
 //   k_gather       F_a = own term + sum over reverse edges (deterministic)
 //   k_reduce       per-structure FP64 energy and virial sums (deterministic tree)
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include <stdint.h>
 
 #include "mg_internal.h"
@@ -39,6 +38,87 @@ __device__ __forceinline__ u64 make_key(int s, int j, unsigned img) {
   return ((u64)(unsigned)s << 60) | ((u64)(unsigned)j << 30) | (u64)img;
 }
 __device__ __forceinline__ int key_j(u64 k) { return (int)((k >> 30) & 0x3fffffffull); }
+
+// ---- scans, clears and the capacity check (graph-capturable: kernels only)
+#define SCAN_TILE 1024  // values per block: 256 threads x 4
+
+// exclusive scan of one tile; single != 0: the whole array is one tile and out[0] = 0 is written here
+__global__ __launch_bounds__(256) void k_scan_tile(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                   int* __restrict__ bsum, int single) {
+  __shared__ int ts[256];
+  const int t = threadIdx.x, base = blockIdx.x * SCAN_TILE + 4 * t;
+  int v[4], run = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) { v[q] = base + q < n ? in[base + q] : 0; run += v[q]; }
+  ts[t] = run;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {  // Hillis-Steele inclusive scan of the thread sums
+    int a = t >= o ? ts[t - o] : 0;
+    __syncthreads();
+    ts[t] += a;
+    __syncthreads();
+  }
+  int acc = ts[t] - run;  // exclusive prefix of this thread within the tile
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    acc += v[q];
+    if (base + q < n) out[base + q + 1] = acc;  // inclusive value at q = exclusive value at q + 1
+  }
+  if (t == 255 && bsum) bsum[blockIdx.x] = ts[255];
+  if (single && blockIdx.x == 0 && t == 0) out[0] = 0;
+}
+
+// exclusive scan of the nb tile sums in place (one block, chunks of 256 with a carry); bsum[nb] = total
+__global__ __launch_bounds__(256) void k_scan_top(int* __restrict__ bsum, int nb) {
+  __shared__ int ts[256];
+  __shared__ int carry;
+  const int t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < nb; c0 += 256) {
+    const int v = c0 + t < nb ? bsum[c0 + t] : 0;
+    ts[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      int a = t >= o ? ts[t - o] : 0;
+      __syncthreads();
+      ts[t] += a;
+      __syncthreads();
+    }
+    if (c0 + t < nb) bsum[c0 + t] = carry + ts[t] - v;
+    __syncthreads();
+    if (t == 0) carry += ts[255];
+    __syncthreads();
+  }
+  if (t == 0) bsum[nb] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int* __restrict__ out, int n, const int* __restrict__ bsum) {
+  const int b = blockIdx.x, add = bsum[b];
+  for (int q = threadIdx.x; q < SCAN_TILE; q += 256) {
+    const int k = b * SCAN_TILE + q;
+    if (k < n) out[k + 1] += add;
+  }
+  if (b == 0 && threadIdx.x == 0) out[0] = 0;
+}
+
+__global__ void k_clear3(int* __restrict__ a, int na, int* __restrict__ b, int nb, int* __restrict__ c, int nc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < na) a[k] = 0;
+  if (k < nb) b[k] = 0;
+  if (k < nc) c[k] = 0;
+}
+
+// misc[0] = longest row of this evaluation; raises the overflow flag misc[3] when the edges or the longest row
+// exceed the capacities the edge kernels were sized for, and keeps the largest totals seen in misc[4], misc[5]
+__global__ void k_check_caps(int N, const int* __restrict__ row_off, int* __restrict__ misc, int cap_edges,
+                             int cap_row) {
+  if (threadIdx.x != 0) return;
+  const int ne = row_off[N], mr = misc[0];
+  if (ne > cap_edges || mr > cap_row) misc[3] = 1;
+  if (ne > misc[4]) misc[4] = ne;
+  if (mr > misc[5]) misc[5] = mr;
+}
 
 // ---------------------------------------------------------------------------------------------- prepare
 __global__ void k_prepare(int N, const double* __restrict__ xin, const int* __restrict__ atom_str,
@@ -111,8 +191,9 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
                                             const int* __restrict__ bin_atoms, int* __restrict__ cnt,
                                             int* __restrict__ rowcnt, int* __restrict__ maxrow,
                                             const int* __restrict__ row_off, u64* __restrict__ key,
-                                            float4* __restrict__ vec, int cap) {
+                                            float4* __restrict__ vec, int cap, const int* __restrict__ ovf) {
 #pragma clang fp contract(off)
+  if (FILL && *ovf) return;  // rows exceed the capacities this launch was sized for
   __shared__ int e_start[NBR_CHUNK], e_pre[NBR_CHUNK + 1];
   __shared__ unsigned e_img[NBR_CHUNK];
   __shared__ double e_sh[NBR_CHUNK][3];
@@ -252,7 +333,8 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
 
 // Reverse edge of e = (i -> j, img): the entry (j -> i, -img) of row j, found by binary search on keys.
 __global__ void k_rev(int N, const int* __restrict__ spc, const int* __restrict__ row_off, const u64* __restrict__ key,
-                      int* __restrict__ rev, int* __restrict__ err) {
+                      int* __restrict__ rev, int* __restrict__ err, const int* __restrict__ ovf) {
+  if (*ovf) return;
   int i = blockIdx.x;
   int off = row_off[i], M = row_off[i + 1] - off;
   int si = spc[i];
@@ -518,10 +600,12 @@ template <int PAT, bool UNI>
 __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                                    const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                    const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
-                                                   float* __restrict__ erep, float urep, int cap) {
+                                                   float* __restrict__ erep, float urep, int cap,
+                                                   const int* __restrict__ ovf) {
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   extern __shared__ float4 smem_d[];
+  if (*ovf) return;  // rows may exceed the LDS tile of cap entries
   const int wv = threadIdx.x >> 6;
   const int i = blockIdx.x * (blockDim.x >> 6) + wv;  // one wave per atom, 1-4 waves per block
   const int lane = threadIdx.x & 63;
@@ -697,11 +781,13 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
                                                     const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
                                                     float4* __restrict__ fself, float4* __restrict__ fe,
-                                                    float* __restrict__ vir, float urep, int cap) {
+                                                    float* __restrict__ vir, float urep, int cap,
+                                                    const int* __restrict__ ovf) {
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   constexpr int ZL = MG_MAXZ * MG_MAXL;
   extern __shared__ float4 smem_f[];
+  if (*ovf) return;  // rows may exceed the LDS tile of cap entries
   const int wv = threadIdx.x >> 6;
   const int i = blockIdx.x * (blockDim.x >> 6) + wv;  // one wave per atom, 1-4 waves per block
   const int lane = threadIdx.x & 63;
@@ -866,7 +952,8 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
 // F_a = own term + sum of the reverse edges' neighbour terms; 4 lanes per atom, fixed order, FP64 accumulation
 __global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ row_off, const int* __restrict__ rev,
                                                 const float4* __restrict__ fself, const float4* __restrict__ fe,
-                                                double* __restrict__ F) {
+                                                double* __restrict__ F, const int* __restrict__ ovf) {
+  if (*ovf) return;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int a = g >> 2, sub = g & 3;
   double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -923,54 +1010,84 @@ __global__ void k_reduce_final(int nstr, const mg_struct_dev* __restrict__ str, 
 }
 
 // ---------------------------------------------------------------------------------------------- launchers
+static thread_local hipError_t g_launch_err = hipSuccess;
+
+extern "C" int mgk_status(void) {
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return 0;
+  g_launch_err = e;
+  return -1;
+}
+
+extern "C" const char* mgk_last_error(void) { return hipGetErrorString(g_launch_err); }
+
 extern "C" {
 
 int mgk_scan(const int* in, int* out, int n, void* tmp, size_t* tmp_bytes, hipStream_t st) {
-  // exclusive scan of n values into out[0..n]; out[n] = total.  tmp == NULL queries the scratch size.
-  size_t need = 0;
-  hipcub::DeviceScan::InclusiveSum(nullptr, need, in, out + 1, n, st);
+  // exclusive scan of n values into out[0..n]; out[n] = total.  tmp == NULL queries the scratch size
+  // (one int per SCAN_TILE block).  Three plain launches and no memset, so the scan can sit in a HIP graph.
+  const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  const size_t need = (size_t)(nb + 1) * sizeof(int);
   if (!tmp) { *tmp_bytes = need; return 0; }
   if (*tmp_bytes < need) return -1;
-  if (hipMemsetAsync(out, 0, sizeof(int), st) != hipSuccess) return -1;
-  if (n > 0 && hipcub::DeviceScan::InclusiveSum(tmp, need, in, out + 1, n, st) != hipSuccess) return -1;
-  return 0;
+  int* bsum = (int*)tmp;
+  if (nb <= 1) {
+    hipLaunchKernelGGL(k_scan_tile, dim3(1), dim3(256), 0, st, in, out, n, (int*)nullptr, 1);
+    return mgk_status();
+  }
+  hipLaunchKernelGGL(k_scan_tile, dim3(nb), dim3(256), 0, st, in, out, n, bsum, 0);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, st, out, n, bsum);
+  return mgk_status();
+}
+
+int mgk_clear(int* a, int na, int* b, int nb, int* c, int nc, hipStream_t st) {
+  const int n = na > nb ? (na > nc ? na : nc) : (nb > nc ? nb : nc);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_clear3, dim3((n + 255) / 256), dim3(256), 0, st, a, na, b, nb, c, nc);
+  return mgk_status();
+}
+
+int mgk_check_caps(int N, const int* row_off, int* misc, int cap_edges, int cap_row, hipStream_t st) {
+  hipLaunchKernelGGL(k_check_caps, dim3(1), dim3(64), 0, st, N, row_off, misc, cap_edges, cap_row);
+  return mgk_status();
 }
 
 int mgk_prepare(int N, const double* xin, const int* atom_str, const mg_struct_dev* str, double* x, int* bin,
                 int* bin_count, hipStream_t st) {
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_prepare, dim3((N + 255) / 256), dim3(256), 0, st, N, xin, atom_str, str, x, bin, bin_count);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_cur, int* bin_atoms, hipStream_t st) {
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_bin_fill, dim3((N + 255) / 256), dim3(256), 0, st, N, bin, bin_start, bin_cur, bin_atoms);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
                   const int* bin_start, const int* bin_atoms, int* cnt, int* rowcnt, int* maxrow, hipStream_t st) {
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, st, N, x, spc, atom_str, str, bin, bin_start, bin_atoms, cnt,
-                     rowcnt, maxrow, (const int*)nullptr, (u64*)nullptr, (float4*)nullptr, 0);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+                     rowcnt, maxrow, (const int*)nullptr, (u64*)nullptr, (float4*)nullptr, 0, (const int*)nullptr);
+  return mgk_status();
 }
 
 // Fill and sort the rows (LDS tile of cap entries: a power of two >= the longest row) and build the reverse-edge
 // index.  Rows longer than 2048 neighbours are rejected (-2).
 int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
                  const int* bin_start, const int* bin_atoms, const int* row_off, void* key, void* vec, int* rev,
-                 int* err, int maxrow, hipStream_t st) {
+                 int* err, const int* ovf, int maxrow, hipStream_t st) {
   if (N == 0) return 0;
   int cap = 64;
   while (cap < maxrow) cap <<= 1;
   if (cap > 2048) return -2;
   size_t sm = (size_t)cap * (sizeof(u64) + sizeof(float4) + sizeof(int));
   hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), sm, st, N, x, spc, atom_str, str, bin, bin_start, bin_atoms,
-                     (int*)nullptr, (int*)nullptr, (int*)nullptr, row_off, (u64*)key, (float4*)vec, cap);
-  hipLaunchKernelGGL(k_rev, dim3(N), dim3(64), 0, st, N, spc, row_off, (const u64*)key, rev, err);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+                     (int*)nullptr, (int*)nullptr, (int*)nullptr, row_off, (u64*)key, (float4*)vec, cap, ovf);
+  hipLaunchKernelGGL(k_rev, dim3(N), dim3(64), 0, st, N, spc, row_off, (const u64*)key, rev, err, ovf);
+  return mgk_status();
 }
 
 // One wave per atom; wpb waves (atoms) per block.  wpb = 0 picks the largest of 4, 2, 1 whose LDS fits 64 KiB;
@@ -1004,13 +1121,14 @@ static int pick_wpb(int wpb, size_t sm_wave) {
 
 // variant = G4 pattern (bits 0-3) | shared-eta flag (bit 4)
 int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
-             const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, int wpb, hipStream_t st) {
+             const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, int wpb, const int* ovf,
+             hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * R * sizeof(float4) + PCH * sizeof(unsigned);
-  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap, ovf);
+  return mgk_status();
 }
 
 int mgk_mlp(int N, const int* spc, const float* G, const float* W, int wstride, int D, int H1, int H2, float* Ei,
@@ -1018,27 +1136,28 @@ int mgk_mlp(int N, const int* spc, const float* G, const float* W, int wstride, 
   if (N == 0) return 0;
   size_t sm = (size_t)(H1 + H2) * 64 * sizeof(float);
   hipLaunchKernelGGL(k_mlp_fp32, dim3((N + 63) / 64), dim3(64), sm, st, N, spc, G, W, wstride, D, H1, H2, Ei, dEdG);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
               const mg_basis_dev* bp, const float* dEdG, void* fself, void* fe, float* vir, float urep, int maxrow,
-              int wpb, hipStream_t st) {
+              int wpb, const int* ovf, hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * (R + 1) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned) +
               (size_t)((cap + 3) & ~3) * sizeof(int);
   DISPATCH_PU(k_force, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
-              urep, cap);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+              urep, cap, ovf);
+  return mgk_status();
 }
 
-int mgk_gather(int N, const int* row_off, const int* rev, const void* fself, const void* fe, double* F, hipStream_t st) {
+int mgk_gather(int N, const int* row_off, const int* rev, const void* fself, const void* fe, double* F, const int* ovf,
+               hipStream_t st) {
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rev, (const float4*)fself,
-                     (const float4*)fe, F);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+                     (const float4*)fe, F, ovf);
+  return mgk_status();
 }
 
 int mgk_reduce(int nstr, int nchunk, const int* chunk_str, const mg_struct_dev* str, const float* Ei, const float* erep,
@@ -1046,7 +1165,7 @@ int mgk_reduce(int nstr, int nchunk, const int* chunk_str, const mg_struct_dev* 
   if (nstr == 0) return 0;
   hipLaunchKernelGGL(k_reduce_part, dim3(nchunk), dim3(256), 0, st, nchunk, chunk_str, str, Ei, erep, vir, have_vir, part);
   hipLaunchKernelGGL(k_reduce_final, dim3((nstr + 255) / 256), dim3(256), 0, st, nstr, str, part, E, U);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 }  // extern "C"
@@ -1252,5 +1371,5 @@ extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, cons
   else MLP_P(0);
 #undef MLP_P
 #undef MLP_L
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }

@@ -198,7 +198,7 @@ int mgk_sur_features(int S, const mg_struct_dev* str, const int* spc, const floa
                      int ng4, int off2, int off4, float* feat, hipStream_t st) {
   if (S == 0) return 0;
   hipLaunchKernelGGL(k_sur_features, dim3(S), dim3(128), 0, st, S, str, spc, G, D, nspc, ng2, ng4, off2, off4, feat);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_sur_grad(int B, int chunk, int nin, const int* idx, const float* X, const float* Y, const float* p,
@@ -206,14 +206,14 @@ int mgk_sur_grad(int B, int chunk, int nin, const int* idx, const float* X, cons
   if (nin < 1 || nin > SUR_MAXIN || chunk < 1 || B < 1) return -1;
   hipLaunchKernelGGL(k_sur_grad, dim3((B + chunk - 1) / chunk), dim3(64), 0, st, B, chunk, nin, idx, X, Y, p, norm,
                      part, part_loss);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_sur_adam(int P, int nblk, const float* part, float* p, float* m, float* v, float* grad, float lr, float b1,
                  float b2, float eps, float c1, float c2, hipStream_t st) {
   hipLaunchKernelGGL(k_sur_adam, dim3((P + 255) / 256), dim3(256), 0, st, P, nblk, part, p, m, v, grad, lr, b1, b2,
                      eps, c1, c2);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 int mgk_sur_predict(int n, int nin, const float* X, const float* p, const float* norm, float* out, hipStream_t st) {
@@ -221,7 +221,7 @@ int mgk_sur_predict(int n, int nin, const float* X, const float* p, const float*
   if (nin < 1 || nin > SUR_MAXIN) return -1;
   const int grid = n < 1024 ? n : 1024;
   hipLaunchKernelGGL(k_sur_predict, dim3(grid), dim3(64), 0, st, n, nin, X, p, norm, out);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return mgk_status();
 }
 
 }  // extern "C"
