@@ -399,6 +399,7 @@ int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const dou
   do {                                                                                                              \
     int kr_ = (x);                                                                                                  \
     if (kr_ == -3) return mg_fail(MG_ERR_NEIGHBOR, "neighbour rows too long for the LDS of one CU (%d)", c->maxrow); \
+    if (kr_ == -2) return mg_fail(MG_ERR_NEIGHBOR, "an atom has more than 2048 neighbours (%d)", c->maxrow);        \
     if (kr_ != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error());               \
   } while (0)
 
@@ -508,6 +509,7 @@ int mg_eval(mg_ctx *c, int flags) {
 
 int mg_get(mg_ctx *c, int s, double *E, double *EA, double *F, double *U, double *X) {
   if (!c || s < 0 || s >= c->S) return mg_fail(MG_ERR_ARG, "mg_get: bad structure index");
+  if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_get: the structure has not been evaluated yet");
   HIPCHK(hipSetDevice(c->device));
   const int a0 = c->h_atom_off[s], n = c->h_atom_off[s + 1] - a0;
   if (E) HIPCHK(hipMemcpy(E, c->d_E + s, sizeof(double), hipMemcpyDeviceToHost));
@@ -532,6 +534,7 @@ int mg_get(mg_ctx *c, int s, double *E, double *EA, double *F, double *U, double
 
 int mg_get_batch(mg_ctx *c, double *E, double *U, double *F) {
   if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_get_batch: the batch has not been evaluated yet");
   HIPCHK(hipSetDevice(c->device));
   if (E) HIPCHK(hipMemcpy(E, c->d_E, c->S * sizeof(double), hipMemcpyDeviceToHost));
   if ((U || F) && !c->have_forces) return mg_fail(MG_ERR_ARG, "mg_get_batch: last eval had no forces");
