@@ -363,26 +363,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// fnb[s] += g for every active lane, without atomics: lanes claim their slot (the last store to a slot wins),
-// the unique winners do a plain float4 read-modify-write, the others retry.  LDS float atomics serialise per lane
-// and cost ~3x the whole triplet arithmetic; duplicate slots within one 64-lane batch are rare, so this usually
-// finishes in one round.  Slots of one wave are private to that wave.
-__device__ __forceinline__ void lds_accum(float4* fnb, int* claim, int slot, float gx, float gy, float gz, bool active,
-                                          int lane) {
-  bool pending = active;
-  while (__ballot(pending) != 0ull) {
-    if (pending) claim[slot] = lane;
-    wave_sync();
-    if (pending && claim[slot] == lane) {
-      float4 v = fnb[slot];
-      v.x += gx; v.y += gy; v.z += gz;
-      fnb[slot] = v;
-      pending = false;
-    }
-    wave_sync();
-  }
-}
-
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, WAVE);
@@ -533,47 +513,67 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
 // side as the lane-fast index.  Passing pairs are compacted into an LDS list (ballot + lane prefix count) and
 // processed with all 64 lanes busy; about half of the enumerated pairs fail the r_jk test in bulk metals.
 #define PCH 512
+// Enumerate the neighbour pairs (j, k) of one species class with r_jk < rc, 64 candidates per step, compacting
+// the survivors into plist and handing them to body(j, k, tag, valid) in batches of 64 (valid = false on the
+// lanes past the end of the last batch).
+//   same species (tri): circulant diagonals, pair (f, f + sl mod m) for sl = 1..m/2 (the last diagonal of an even
+//     m only for f < m/2), so every unordered pair appears once;
+//   two species: the longer side's slot f against the shorter side's slot (f mod l + sl) mod l, sl = 0..l-1.
+// tag (8 bits) numbers the groups of pairs in which no slot repeats on either side: a triangle diagonal, or a run of
+// l consecutive f on one rectangle diagonal.  The force kernel accumulates one group at a time without atomics.
+// Packing: j | k << 12 | tag << 24 (slots < 2048).
 template <int R, typename F>
 __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bool tri, int oa, int ma, int ob, int mb,
                                           float rc2, int lane, F&& body) {
-  int mf, ms;  // fast / slow extents
+  int mf, ms, l;  // fast extent, slow extent, group length (shorter side)
   if (tri) {
     if (ma < 2) return;
-    mf = ma; ms = ma >> 1;
+    mf = ma; ms = ma >> 1; l = ma;
   } else {
     if (ma == 0 || mb == 0) return;
-    mf = ma >= mb ? ma : mb; ms = ma >= mb ? mb : ma;
+    mf = ma >= mb ? ma : mb; l = ma >= mb ? mb : ma; ms = l - 1;
   }
+  const int nblk = (mf + l - 1) / l;  // groups per rectangle diagonal
   const bool even = (ma & 1) == 0;
   int f = lane, sl = tri ? 1 : 0;
   while (f >= mf) { f -= mf; sl++; }
-  bool live = tri ? sl <= ms : sl < ms;
+  int fq = 0, fr = f;  // f = fq * l + fr
+  while (fr >= l) { fr -= l; fq++; }
+  bool live = sl <= ms;
   int cnt = 0;
   for (;;) {
     bool ok = false;
     unsigned pk = 0;
     if (live) {
-      int jj, kk;
+      int jj, kk, tag;
       bool use = true;
       if (tri) {
         use = !(even && sl == ms && f >= ms);
         int k = f + sl;
         if (k >= ma) k -= ma;
         jj = oa + f; kk = oa + k;
-      } else if (ma >= mb) {
-        jj = oa + f; kk = ob + sl;
+        tag = sl;
       } else {
-        jj = oa + sl; kk = ob + f;
+        int q = fr + sl;
+        if (q >= l) q -= l;
+        if (ma >= mb) { jj = oa + f; kk = ob + q; } else { jj = oa + q; kk = ob + f; }
+        tag = sl * nblk + fq;
       }
       if (use) {
         float4 a = rec[R * jj], b = rec[R * kk];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         ok = cx * cx + cy * cy + cz * cz < rc2;
-        pk = (unsigned)jj | ((unsigned)kk << 16);
+        pk = (unsigned)jj | ((unsigned)kk << 12) | ((unsigned)(tag & 0xff) << 24);
       }
       f += WAVE;
-      while (f >= mf) { f -= mf; sl++; }
-      live = tri ? sl <= ms : sl < ms;
+      fr += WAVE;
+      while (fr >= l) { fr -= l; fq++; }
+      if (f >= mf) {
+        while (f >= mf) { f -= mf; sl++; }
+        fq = 0; fr = f;
+        while (fr >= l) { fr -= l; fq++; }
+      }
+      live = sl <= ms;
     }
     const unsigned long long bm = __ballot(ok);
     const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
@@ -586,12 +586,32 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
         const int t = tb + lane;
         const bool valid = t < cnt;
         const unsigned p = valid ? plist[t] : 0u;
-        body((int)(p & 0xffffu), (int)(p >> 16), valid);
+        body((int)(p & 0xfffu), (int)((p >> 12) & 0xfffu), (int)(p >> 24), valid);
       }
       wave_sync();
       cnt = 0;
       if (!more) break;
     }
+  }
+}
+
+// fj[j] += gj, fk[k] += gk over one batch of pairs: the pairs of one tag group touch distinct slots on each side, and
+// the two sides accumulate into separate arrays, so each group is one plain read-modify-write of both; the groups of
+// a batch (a few) go one after the other, ordered by the LDS executing one wave's accesses in program order.
+__device__ __forceinline__ void group_accum(float4* fj, float4* fk, int j, float jx, float jy, float jz, int k, float kx,
+                                            float ky, float kz, int tag, bool valid) {
+  unsigned long long rem = __ballot(valid);
+  while (rem) {
+    const int cur = __shfl(tag, (int)__builtin_ctzll(rem), WAVE);
+    const bool mine = valid && tag == cur;
+    if (mine) {
+      float4 v = fj[j], w = fk[k];
+      v.x += jx; v.y += jy; v.z += jz;
+      w.x += kx; w.y += ky; w.z += kz;
+      fj[j] = v;
+      fk[k] = w;
+    }
+    rem &= ~__ballot(mine);
   }
 }
 
@@ -670,7 +690,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
       float V[MG_GRID];
 #pragma unroll
       for (int q = 0; q < MG_GRID; q++) V[q] = 0.0f;
-      auto body = [&](int j, int k, bool valid) {
+      auto body = [&](int j, int k, int, bool valid) {
         if (!valid) return;
         float4 a = rec[R * j], b = rec[R * k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
@@ -795,15 +815,16 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
   const int D = bp->D;
   RowInfo ri;
   row_info(ri, i, spc, row_off, cnt, S);
-  float4* rec = smem_f + (size_t)wv * ((R + 1) * cap + MG_GRID / 4 + PCH / 4 + ((cap + 3) & ~3) / 4);
+  float4* rec = smem_f + (size_t)wv * ((R + 2) * cap + MG_GRID / 4 + PCH / 4);
   float4* gts4 = rec + R * cap;                  // [MG_GRID / 4]
-  float4* fnb4 = gts4 + MG_GRID / 4;             // [cap] force on each neighbour slot
+  float4* fnb4 = gts4 + MG_GRID / 4;             // [cap] force on each neighbour slot (pair terms, triplet j side)
+  float4* fnk4 = fnb4 + cap;                     // [cap] triplet k side
   float* fnb = (float*)fnb4;
-  unsigned* plist = (unsigned*)(fnb4 + cap);     // [PCH]
-  int* claim = (int*)(plist + PCH);              // [cap]
+  unsigned* plist = (unsigned*)(fnk4 + cap);     // [PCH]
   stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
   for (int t = lane; t < ri.M; t += WAVE) {
-    fnb[4 * t] = 0.0f; fnb[4 * t + 1] = 0.0f; fnb[4 * t + 2] = 0.0f; fnb[4 * t + 3] = 0.0f;
+    fnb4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    fnk4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   wave_sync();
   const float* gi = dEdG + (size_t)i * D;
@@ -866,7 +887,7 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
         gv[e * ZL + 0] = g0.x; gv[e * ZL + 1] = g0.y; gv[e * ZL + 2] = g0.z; gv[e * ZL + 3] = g0.w;
         gv[e * ZL + 4] = g1.x; gv[e * ZL + 5] = g1.y; gv[e * ZL + 6] = g1.z; gv[e * ZL + 7] = g1.w;
       }
-      auto body = [&](int j, int k, bool valid) {
+      auto body = [&](int j, int k, int tag, bool valid) {
         if (!valid) { j = 0; k = 1; }  // a harmless slot pair; its gradient is discarded below
         float4 a = rec[R * j], b = rec[R * k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
@@ -922,8 +943,7 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
         float cbb = dT2 * i2 - dTc * cs * i2 * i2; // coefficient of b in dT/db
         float gax = ca * a.x + cb * b.x - c3c * cx, gay = ca * a.y + cb * b.y - c3c * cy, gaz = ca * a.z + cb * b.z - c3c * cz;
         float gbx = cbb * b.x + cb * a.x + c3c * cx, gby = cbb * b.y + cb * a.y + c3c * cy, gbz = cbb * b.z + cb * a.z + c3c * cz;
-        lds_accum(fnb4, claim, j, -gax, -gay, -gaz, valid, lane);
-        lds_accum(fnb4, claim, k, -gbx, -gby, -gbz, valid, lane);
+        group_accum(fnb4, fnk4, j, -gax, -gay, -gaz, k, -gbx, -gby, -gbz, tag, valid);
       };
       for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
       wave_sync();
@@ -932,7 +952,8 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
   // F_i = -sum_e fnb[e] and the virial U = -sum_e dT/dd_e (x) d_e = sum_e fnb[e] (x) d_e (symmetric)
   wave_sync();
   for (int t = lane; t < ri.M; t += WAVE) {
-    const float gx = fnb[4 * t], gy = fnb[4 * t + 1], gz = fnb[4 * t + 2];
+    const float4 g1 = fnb4[t], g2 = fnk4[t];
+    const float gx = g1.x + g2.x, gy = g1.y + g2.y, gz = g1.z + g2.z;
     const float4 v = rec[R * t];
     fx -= gx; fy -= gy; fz -= gz;
     u0 += gx * v.x; u1 += gy * v.y; u2 += gz * v.z;
@@ -1145,8 +1166,7 @@ int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, con
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
-  size_t sm = (size_t)cap * (R + 1) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned) +
-              (size_t)((cap + 3) & ~3) * sizeof(int);
+  size_t sm = (size_t)cap * (R + 2) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
   DISPATCH_PU(k_force, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
               urep, cap, ovf);
   return mgk_status();
