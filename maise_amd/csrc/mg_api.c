@@ -376,6 +376,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
   c->S = S;
   c->N = N;
   c->have_forces = c->have_g = 0;
+  c->gcap_edges = c->gcap_row = 0;  /* capacities are sized again by the next synchronised evaluation */
   return MG_OK;
 }
 
@@ -395,6 +396,7 @@ int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const dou
 }
 
 /* ------------------------------------------------------------------ evaluation */
+#define MG_ERR_NEIGHBOR_CAP 1  /* internal: capacity-mode evaluation outgrew its buffers */
 #define KCHK(x)                                                                                                     \
   do {                                                                                                              \
     int kr_ = (x);                                                                                                  \
@@ -403,9 +405,14 @@ int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const dou
     if (kr_ != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error());               \
   } while (0)
 
+static size_t graph_edge_cap(const mg_ctx *c, int need_edges) {
+  const double h = c->graph_headroom;
+  return (size_t)need_edges + (size_t)(h * need_edges) + (h > 0.0 ? 256 : 0);
+}
+
 int mgh_graph_caps(mg_ctx *c, int need_edges, int need_row) {
   const double h = c->graph_headroom;
-  const size_t ne = (size_t)need_edges + (size_t)(h * need_edges) + (h > 0.0 ? 256 : 0);
+  const size_t ne = graph_edge_cap(c, need_edges);
   if (ne > 0x7fffffff) return mg_fail(MG_ERR_ARG, "too many neighbour edges (%zu)", ne);
   int r;
   if ((r = ensure_edges(c, ne))) return r;
@@ -426,11 +433,11 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   const int N = c->N;
   hipStream_t st = c->st;
   const float urep = (float)c->urep;
-  const int timing = c->timing && mode == MG_CHAIN_SYNC;
+  const int timing = c->timing && mode != MG_CHAIN_GRAPH;
   const int *ovf = c->d_misc + 3;
   if (timing) HIPCHK(hipEventRecord(c->ev[EV_START], st));
   /* neighbour list: bins, row counts, row offsets; graph mode keeps the flags of misc[1..5] for the caller */
-  KCHK(mgk_clear(c->d_bin_count, c->nbins + 1, c->d_bin_cur, c->nbins + 1, c->d_misc, mode == MG_CHAIN_SYNC ? 16 : 1, st));
+  KCHK(mgk_clear(c->d_bin_count, c->nbins + 1, c->d_bin_cur, c->nbins + 1, c->d_misc, mode == MG_CHAIN_GRAPH ? 1 : 16, st));
   KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, st));
   KCHK(mgk_scan(c->d_bin_count, c->d_bin_start, c->nbins, c->d_tmp, &c->tmp_bytes, st));
   KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, st));
@@ -446,7 +453,9 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     c->nedges = hm[0];
     c->maxrow = hm[1];
     int r;
-    if ((r = ensure_edges(c, (size_t)c->nedges))) return r;
+    /* allocate with the capacity-mode headroom now, so sizing the capacities after this evaluation never has to
+     * reallocate (and drop) the edge buffers it has just filled */
+    if ((r = ensure_edges(c, graph_edge_cap(c, c->nedges)))) return r;
     cap_row = c->maxrow;
     KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->nedges, cap_row, st));
   } else {
@@ -478,11 +487,17 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
                   c->d_U, st));
   if (timing) HIPCHK(hipEventRecord(c->ev[EV_END], st));
   if (mode == MG_CHAIN_GRAPH) return MG_OK;
-  int flags[4];
+  int flags[5];
   HIPCHK(hipMemcpyAsync(flags, c->d_misc, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&flags[4], c->d_row_off + N, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (flags[3]) return mg_fail(MG_ERR_HIP, "edge capacity check failed in a synchronised evaluation");
+  if (flags[3]) {
+    if (mode == MG_CHAIN_CAPS) return MG_ERR_NEIGHBOR_CAP;  /* outgrew the capacities: the caller repeats in sync mode */
+    return mg_fail(MG_ERR_HIP, "edge capacity check failed in a synchronised evaluation");
+  }
   if (flags[1]) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)");
+  c->nedges = flags[4];
+  c->maxrow = flags[0];
   if (timing) {
     hipEventElapsedTime(&c->tms[0], c->ev[EV_START], c->ev[EV_NBR]);
     hipEventElapsedTime(&c->tms[1], c->ev[EV_NBR], c->ev[EV_DESC]);
@@ -500,8 +515,16 @@ int mg_eval(mg_ctx *c, int flags) {
   HIPCHK(hipSetDevice(c->device));
   if (c->prec != MG_PREC_FP32 && !c->m->mfma_ok) return mg_fail(MG_ERR_UNSUPPORTED, "BF16 MLP needs hidden layers <= 64 wide");
   const int forces = (flags & MG_EVAL_FORCES) != 0;
-  int r = mgh_eval_chain(c, forces, MG_CHAIN_SYNC);
-  if (r != MG_OK) return r;
+  /* repeated evaluations of the same atoms reuse the capacities of the last one (no host round trip in the middle
+   * of the chain); a step that outgrows them is detected on the device and evaluated again synchronised */
+  int r = MG_ERR_NEIGHBOR_CAP;
+  if (c->graph_enable && c->gcap_edges > 0) r = mgh_eval_chain(c, forces, MG_CHAIN_CAPS);
+  if (r == MG_ERR_NEIGHBOR_CAP) {
+    if ((r = mgh_eval_chain(c, forces, MG_CHAIN_SYNC)) != MG_OK) return r;
+    if ((r = mgh_graph_caps(c, c->nedges, c->maxrow)) != MG_OK) return r;
+  } else if (r != MG_OK) {
+    return r;
+  }
   c->have_forces = forces;
   c->have_g = 1;
   return MG_OK;

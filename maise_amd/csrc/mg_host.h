@@ -134,7 +134,7 @@ int mgk_check_caps(int N, const int *row_off, int *misc, int cap_edges, int cap_
  * synchronisation in the middle, one at the end); MG_CHAIN_GRAPH uses the capacities gcap_edges / gcap_row set
  * beforehand, never synchronises or allocates, and leaves the overflow flag misc[3] for the caller to check, so
  * it can be captured into a HIP graph. */
-enum { MG_CHAIN_SYNC = 0, MG_CHAIN_GRAPH = 1 };
+enum { MG_CHAIN_SYNC = 0, MG_CHAIN_GRAPH = 1, MG_CHAIN_CAPS = 2 };  /* CAPS: graph-mode capacities, not captured */
 int mgh_eval_chain(struct mg_ctx *c, int forces, int mode);
 /* size the graph-mode capacities from the last evaluation with headroom, allocating edge buffers */
 int mgh_graph_caps(struct mg_ctx *c, int need_edges, int need_row);
