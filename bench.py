@@ -189,6 +189,10 @@ def ga_leg(maise_amd, args):
             "recall_top25_mean": float(np.mean([r.recall_top25 for r in reps])),
             "recall_random_selection": cfg.keep_top + cfg.keep_random,
             "mae_eV_atom_mean": float(np.mean([r.mae_eV_atom for r in reps])),
+            "seconds_per_generation": {"screened": ts, "full_relax": tf,
+                                       "predict": float(np.mean([r.seconds_predict for r in reps])),
+                                       "relax_kept": float(np.mean([r.seconds_relax for r in reps])),
+                                       "train": float(np.mean([r.seconds_train for r in reps]))},
             "best_eV_atom_first_last": [res.reports[0].best_eV_atom, res.reports[-1].best_eV_atom],
             "seconds_total": wall}
 
@@ -266,9 +270,21 @@ def main():
     t_desc = stages[1] * 1e-3
     kernel, fl, t = ("k_force", fl_force, t_force) if t_force >= t_desc else ("k_desc", fl_desc, t_desc)
     achieved = fl / t / 1e12
-    roof = {"bound": "mfma", "bound_detail": "FP32 vector ALU (no HBM or matrix-core bound); priced at the FP32 peak",
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "traffic_cu32k_w64_fp32.json")
+    if os.path.exists(tpath) and N == 32000:
+        tj = json.load(open(tpath))
+        kk = tj["kernels_KiB_per_dispatch"].get(kernel)
+        if kk:
+            cal = tj["calibration"]
+            traffic = int(1024 * (kk["FETCH_SIZE"] * cal["fetch_scale"] + kk["WRITE_SIZE"] * cal["write_scale"]))
+            traffic_src = "profiles/traffic_cu32k_w64_fp32.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
+    roof = {"bound": "mfma",
+            "bound_detail": "FP32 vector ALU (neither HBM nor the matrix cores bound it); priced at the published FP32 "
+                            "peak, which counts packed FMA; without packed FP32 (not used) the issue limit is half",
             "kernel": kernel, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_source": traffic_src,
+            "frac_of_unpacked_fp32": round(achieved / (FP32_PEAK_TFLOPS / 2), 4),
             "flops_per_launch": fl, "kernel_ms": round(t * 1e3, 4), "sum_M": M, "sum_K": K}
 
     # FP32 path against the reference's own FP64 output (CALL_MAISE compiled from its sources, committed as a golden

@@ -2,10 +2,7 @@
 #ifndef MG_HOST_H
 #define MG_HOST_H
 
-#ifndef __HIP_PLATFORM_AMD__
-#define __HIP_PLATFORM_AMD__
-#endif
-#include <hip/hip_runtime_api.h>
+#include <hip/hip_runtime_api.h>  /* gcc builds the host code with -D__HIP_PLATFORM_AMD__ (maise_amd/build.py) */
 #include <stddef.h>
 #include <stdint.h>
 

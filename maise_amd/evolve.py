@@ -49,7 +49,7 @@ class GAConfig:
     max_steps: int = 1500
     urep: float = 100.0
     seed: int = 5
-    epochs: int = 60
+    train_steps: int = 300                     # Adam steps per generation (warm start from the last generation)
     batch: int = 64
     lr: float = 1e-3
     audit: bool = True                         # also relax every offspring to measure recall and error
@@ -67,6 +67,9 @@ class GenerationReport:
     mae_eV_atom: Optional[float] = None
     best_eV_atom: float = math.nan
     train_loss: Optional[float] = None
+    seconds_predict: Optional[float] = None    # unrelaxed evaluation + features + prediction + selection
+    seconds_relax: Optional[float] = None      # relaxation of the kept offspring
+    seconds_train: Optional[float] = None      # retraining
 
 
 @dataclass
@@ -182,10 +185,14 @@ class SurrogateGA:
         return out, epa, done
 
     def train(self) -> float:
+        """A fixed budget of Adam steps over everything relaxed so far, warm-started from the last generation's
+        network; the normalisation is fitted once, on the first (random, fully relaxed) generation."""
         X = np.concatenate(self.X)
         y = np.concatenate(self.y)
-        self.sur.fit_norm(X, y)
-        loss = self.sur.train(X, y, epochs=self.cfg.epochs, batch=self.cfg.batch, lr=self.cfg.lr,
+        if not self.trained:
+            self.sur.fit_norm(X, y)
+        epochs = max(1, int(round(self.cfg.train_steps * self.cfg.batch / len(y))))
+        loss = self.sur.train(X, y, epochs=epochs, batch=self.cfg.batch, lr=self.cfg.lr,
                               seed=int(self.rng.integers(1 << 62)))
         self.trained = True
         return loss
@@ -241,12 +248,15 @@ class SurrogateGA:
             feat = self.features(kids)
             pred = self.sur.predict(feat)
             sel = self.select(pred)
+            t1 = time.perf_counter()
             relaxed, epa, done = self.relax([kids[k] for k in sel])
+            t2 = time.perf_counter()
             self.X.append(feat[sel])
             self.y.append(epa.astype(np.float32))
             loss = self.train()
-            ts = time.perf_counter() - t0
-            rep = GenerationReport(g, len(kids), len(sel), ts, train_loss=loss)
+            t3 = time.perf_counter()
+            rep = GenerationReport(g, len(kids), len(sel), t3 - t0, train_loss=loss, seconds_predict=t1 - t0,
+                                   seconds_relax=t2 - t1, seconds_train=t3 - t2)
             if cfg.audit:
                 t1 = time.perf_counter()
                 _, epa_all, done_all = self.relax(kids)

@@ -118,7 +118,7 @@ def test_training_fits_a_smooth_function():
 def test_screened_search_short():
     from maise_amd.evolve import GAConfig, SurrogateGA
     print("[gpu case] screened GA, Mg16Ca8, pop 96, 3 generations", flush=True)
-    cfg = GAConfig(pop=96, generations=3, seed=21, epochs=80, max_steps=1500)
+    cfg = GAConfig(pop=96, generations=3, seed=21, max_steps=1500)
     ga = SurrogateGA(MGCA, cfg)
     res = ga.run()
     ga.close()
