@@ -478,6 +478,8 @@ template <> struct Pat<2> {
 };
 
 // P_zl(c) = 2^(1-zeta) (1 + lambda c)^zeta and dP/dc, cell q = z*2 + l
+// (1 + lambda c)^zeta and its derivative in c, cell q = z*2 + l.  The prefactors 2^(1 - zeta) are applied once
+// outside the pair loops (to the reduced descriptors, and folded into the dE/dG grid in the force kernel).
 template <int PAT>
 __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, float cs, float* P, float* Pd) {
 #pragma unroll
@@ -486,23 +488,23 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
     const float u = 1.0f + lam * cs;
     if (Pat<PAT>::ZSTD) {
       const float u2 = u * u, u4 = u2 * u2, u8 = u4 * u4, u16 = u8 * u8;
-      P[0 + l] = bp->pref[0] * u;
-      P[2 + l] = bp->pref[1] * u2;
-      P[4 + l] = bp->pref[2] * u4;
-      P[6 + l] = bp->pref[3] * u16;
+      P[0 + l] = u;
+      P[2 + l] = u2;
+      P[4 + l] = u4;
+      P[6 + l] = u16;
       if (Pd) {
-        Pd[0 + l] = bp->pref[0] * lam;
-        Pd[2 + l] = bp->pref[1] * 2.0f * lam * u;
-        Pd[4 + l] = bp->pref[2] * 4.0f * lam * u2 * u;
-        Pd[6 + l] = bp->pref[3] * 16.0f * lam * (u8 * u4) * (u2 * u);
+        Pd[0 + l] = lam;
+        Pd[2 + l] = 2.0f * lam * u;
+        Pd[4 + l] = 4.0f * lam * u2 * u;
+        Pd[6 + l] = 16.0f * lam * (u8 * u4) * (u2 * u);
       }
     } else {
 #pragma unroll
       for (int z = 0; z < MG_MAXZ; z++) {
         int zt = bp->zeta[z];
         float pm1 = ipowf(u, zt - 1);
-        P[z * MG_MAXL + l] = bp->pref[z] * pm1 * u;
-        if (Pd) Pd[z * MG_MAXL + l] = bp->pref[z] * (float)zt * lam * pm1;
+        P[z * MG_MAXL + l] = pm1 * u;
+        if (Pd) Pd[z * MG_MAXL + l] = (float)zt * lam * pm1;
       }
     }
   }
@@ -525,72 +527,64 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
 template <int R, typename F>
 __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bool tri, int oa, int ma, int ob, int mb,
                                           float rc2, int lane, F&& body) {
-  int mf, ms, l;  // fast extent, slow extent, group length (shorter side)
+  int mf, l, ntot;  // diagonal length, group length (shorter side), candidates
   if (tri) {
     if (ma < 2) return;
-    mf = ma; ms = ma >> 1; l = ma;
+    mf = ma; l = ma;
+    const int ms = ma >> 1;
+    ntot = (ms - 1) * ma + ((ma & 1) ? ma : ms);  // diagonals 1..ms; an even m's last one holds f < m/2
   } else {
     if (ma == 0 || mb == 0) return;
-    mf = ma >= mb ? ma : mb; l = ma >= mb ? mb : ma; ms = l - 1;
+    mf = ma >= mb ? ma : mb; l = ma >= mb ? mb : ma;
+    ntot = mf * l;
   }
   const int nblk = (mf + l - 1) / l;  // groups per rectangle diagonal
-  const bool even = (ma & 1) == 0;
-  int f = lane, sl = tri ? 1 : 0;
-  while (f >= mf) { f -= mf; sl++; }
-  int fq = 0, fr = f;  // f = fq * l + fr
-  while (fr >= l) { fr -= l; fq++; }
-  bool live = sl <= ms;
+  const float rmf = 1.0f / (float)mf, rl = 1.0f / (float)l;
   int cnt = 0;
-  for (;;) {
+  for (int base = 0; base < ntot; base += WAVE) {
+    const int t = base + lane;  // candidate t = diagonal * mf + f
     bool ok = false;
     unsigned pk = 0;
-    if (live) {
+    if (t < ntot) {
+      int q = (int)(((float)t + 0.5f) * rmf);  // t / mf through the reciprocal, corrected by one step
+      int f = t - q * mf;
+      if (f < 0) { q--; f += mf; } else if (f >= mf) { q++; f -= mf; }
       int jj, kk, tag;
-      bool use = true;
       if (tri) {
-        use = !(even && sl == ms && f >= ms);
+        const int sl = q + 1;
         int k = f + sl;
         if (k >= ma) k -= ma;
         jj = oa + f; kk = oa + k;
         tag = sl;
       } else {
-        int q = fr + sl;
-        if (q >= l) q -= l;
-        if (ma >= mb) { jj = oa + f; kk = ob + q; } else { jj = oa + q; kk = ob + f; }
-        tag = sl * nblk + fq;
+        int fq = (int)(((float)f + 0.5f) * rl);
+        int fr = f - fq * l;
+        if (fr < 0) { fq--; fr += l; } else if (fr >= l) { fq++; fr -= l; }
+        int u = fr + q;
+        if (u >= l) u -= l;
+        if (ma >= mb) { jj = oa + f; kk = ob + u; } else { jj = oa + u; kk = ob + f; }
+        tag = q * nblk + fq;
       }
-      if (use) {
-        float4 a = rec[R * jj], b = rec[R * kk];
-        float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
-        ok = cx * cx + cy * cy + cz * cz < rc2;
-        pk = (unsigned)jj | ((unsigned)kk << 12) | ((unsigned)(tag & 0xff) << 24);
-      }
-      f += WAVE;
-      fr += WAVE;
-      while (fr >= l) { fr -= l; fq++; }
-      if (f >= mf) {
-        while (f >= mf) { f -= mf; sl++; }
-        fq = 0; fr = f;
-        while (fr >= l) { fr -= l; fq++; }
-      }
-      live = sl <= ms;
+      const float4 a = rec[R * jj], b = rec[R * kk];
+      const float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
+      ok = cx * cx + cy * cy + cz * cz < rc2;
+      pk = (unsigned)jj | ((unsigned)kk << 12) | ((unsigned)(tag & 0xff) << 24);
     }
     const unsigned long long bm = __ballot(ok);
     const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
     if (ok) plist[cnt + below] = pk;
     cnt += __popcll(bm);
-    const bool more = __ballot(live) != 0ull;
-    if (cnt > PCH - WAVE || !more) {
+    const bool last = base + WAVE >= ntot;
+    if (cnt > PCH - WAVE || last) {
       wave_sync();
       for (int tb = 0; tb < cnt; tb += WAVE) {
-        const int t = tb + lane;
-        const bool valid = t < cnt;
-        const unsigned p = valid ? plist[t] : 0u;
+        const int tt = tb + lane;
+        const bool valid = tt < cnt;
+        const unsigned p = valid ? plist[tt] : 0u;
         body((int)(p & 0xfffu), (int)((p >> 12) & 0xfffu), (int)(p >> 24), valid);
       }
       wave_sync();
       cnt = 0;
-      if (!more) break;
     }
   }
 }
@@ -696,7 +690,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
         float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
-        float r3 = __builtin_sqrtf(r3s);
+        float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
         float fc3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * ia.x * ib.x;
         float F = 2.0f * ia.y * ib.y * fc3;
@@ -721,7 +715,8 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
             if (Pat<PAT>::mask(e) & (1u << q)) V[e * MG_MAXZ * MG_MAXL + q] += Bq[e] * P[q];
       };
       for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
-      float mine = reduce_scatter64(V, lane);  // lane l: grid cell l
+      // lane l: grid cell l = (e*MAXZ + z)*MAXL + lambda; the zeta prefactor 2^(1 - zeta) goes on here
+      float mine = reduce_scatter64(V, lane) * bp->pref[(lane / MG_MAXL) % MG_MAXZ];
       int cell = lane < bp->ng4 ? bp->g4cell[lane] : 0;
       float v = __shfl(mine, cell, WAVE);
       if (lane < bp->ng4) Gi[blk + lane] = v;
@@ -874,8 +869,8 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
       const float rinv = 1.0f / rc;
       const float dfk = -0.5f * 3.14159265358979f * rinv;
       {
-        int n = bp->g4map[lane];
-        ((float*)gts4)[lane] = n >= 0 ? gi[blk + n] : 0.0f;
+        int n = bp->g4map[lane];  // grid cell lane; the zeta prefactor 2^(1 - zeta) is folded in here
+        ((float*)gts4)[lane] = n >= 0 ? gi[blk + n] * bp->pref[(lane / MG_MAXL) % MG_MAXZ] : 0.0f;
       }
       wave_sync();
       // the grid of this species block, loop-invariant: kept in registers for the whole class
@@ -893,7 +888,7 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
         float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
-        float r3 = __builtin_sqrtf(r3s);
+        float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
         float i3 = __builtin_amdgcn_rcpf(r3);
         float f3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
         float d3 = dfk * sinpi_(r3 * rinv);
