@@ -546,9 +546,11 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
     bool ok = false;
     unsigned pk = 0;
     if (t < ntot) {
-      int q = (int)(((float)t + 0.5f) * rmf);  // t / mf through the reciprocal, corrected by one step
-      int f = t - q * mf;
-      if (f < 0) { q--; f += mf; } else if (f >= mf) { q++; f -= mf; }
+      // t / mf through the reciprocal: t + 0.5 is exact and the two roundings move (t + 0.5)/mf by less than
+      // (t + 0.5) 2^-23 / mf, under the 0.5/mf that separates it from an integer while t < 2^22 (rows are at most
+      // 2048 slots, so t < 2048 * 2047 / 2 for a triangle and t < 1024^2 for a rectangle)
+      const int q = (int)(((float)t + 0.5f) * rmf);
+      const int f = t - q * mf;
       int jj, kk, tag;
       if (tri) {
         const int sl = q + 1;
@@ -557,9 +559,8 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
         jj = oa + f; kk = oa + k;
         tag = sl;
       } else {
-        int fq = (int)(((float)f + 0.5f) * rl);
-        int fr = f - fq * l;
-        if (fr < 0) { fq--; fr += l; } else if (fr >= l) { fq++; fr -= l; }
+        const int fq = (int)(((float)f + 0.5f) * rl);  // exact as above (f < 2048)
+        const int fr = f - fq * l;
         int u = fr + q;
         if (u >= l) u -= l;
         if (ma >= mb) { jj = oa + f; kk = ob + u; } else { jj = oa + u; kk = ob + f; }
