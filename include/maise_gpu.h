@@ -152,6 +152,14 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
  * per-structure displacement norm of one step.  steps[S] and done[S] (NULL skips) report iterations and
  * convergence; the final positions/energies/forces are read with mg_get / mg_get_batch. */
 int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done);
+/* Full-cell relaxation (the reference's RLXT 3, CELL_RELX/CELL_MIN src/cmod.c:901-977, src/cmin.c:90-166: enthalpy
+ * H = E + pV over positions and lattice, there with GSL BFGS2): batched FIRE over positions and the cell strain
+ * (generalized force (U - pV 1)/N), to every |F| < fmax (eV/A) and every |U - pV 1|/V < smax (GPa), at pressure
+ * p (GPa).  Lattices are updated on the device; mg_get_lattice returns them. */
+int mg_relax_cell(mg_ctx *c, double p_gpa, int max_steps, double fmax, double smax_gpa, double dt0, double dtmax,
+                  double maxstep, int *steps, int *done);
+/* Lattice (rows, A) of structure s as last set or relaxed. */
+int mg_get_lattice(mg_ctx *c, int s, double *lat);
 
 #ifdef __cplusplus
 }

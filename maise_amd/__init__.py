@@ -68,6 +68,9 @@ def lib():
             "mg_md_get_velocities": ([vp, dp], C.c_int),
             "mg_md_run": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp], C.c_int),
             "mg_relax": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, ip, ip], C.c_int),
+            "mg_relax_cell": ([vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                               ip, ip], C.c_int),
+            "mg_get_lattice": ([vp, C.c_int, dp], C.c_int),
             "mg_sur_nparams": ([C.c_int], C.c_int),
             "mg_sur_create": ([C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)], C.c_int),
             "mg_sur_free": ([vp], None),
@@ -300,6 +303,21 @@ class Context:
         _check(lib().mg_relax(self._h, int(max_steps), float(fmax), float(dt0), float(dtmax), float(maxstep),
                               _ip(steps), _ip(done)), "mg_relax")
         return steps, done.astype(bool)
+
+    def relax_cell(self, p_gpa: float = 0.0, max_steps: int = 3000, fmax: float = 0.01, smax_gpa: float = 0.01,
+                   dt0: float = 0.1, dtmax: float = 1.0, maxstep: float = 0.2):
+        """Batched FIRE over positions and lattice at pressure p (GPa) to every |F| < fmax (eV/A) and every stress
+        component < smax (GPa); returns (steps[S], converged[S]); lattices via lattice(s)."""
+        steps = np.zeros(self.S, dtype=np.int32)
+        done = np.zeros(self.S, dtype=np.int32)
+        _check(lib().mg_relax_cell(self._h, float(p_gpa), int(max_steps), float(fmax), float(smax_gpa), float(dt0),
+                                   float(dtmax), float(maxstep), _ip(steps), _ip(done)), "mg_relax_cell")
+        return steps, done.astype(bool)
+
+    def lattice(self, s: int = 0) -> np.ndarray:
+        lat = np.zeros(9)
+        _check(lib().mg_get_lattice(self._h, int(s), _dp(lat)), "mg_get_lattice")
+        return lat.reshape(3, 3)
 
 
 def call_maise_symbol():

@@ -154,7 +154,7 @@ void mg_ctx_free(mg_ctx *c) {
                   c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
                   c->d_vir, c->d_fself, c->d_fe, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
-                  c->d_snap_dyn};
+                  c->d_snap_dyn, c->d_cellv, c->d_snap_cellv, c->d_snap_str};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
     if (ptrs[k]) hipFree(ptrs[k]);
   for (int k = 0; k < EV_N; k++) hipEventDestroy(c->ev[k]);
@@ -594,6 +594,12 @@ int mg_get_descriptors(mg_ctx *c, float *G) {
   if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_get_descriptors: no evaluation yet");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpy(G, c->d_G, sizeof(float) * (size_t)c->N * c->m->D, hipMemcpyDeviceToHost));
+  return MG_OK;
+}
+
+int mg_get_lattice(mg_ctx *c, int s, double *lat) {
+  if (!c || !lat || s < 0 || s >= c->S) return mg_fail(MG_ERR_ARG, "mg_get_lattice: bad argument");
+  memcpy(lat, c->h_lat + 9 * s, 9 * sizeof(double));
   return MG_OK;
 }
 

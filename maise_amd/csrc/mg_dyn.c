@@ -36,6 +36,9 @@ int mgk_fire_init(int S, double *state, double dt0, hipStream_t st);
 int mgk_fire(int S, const mg_struct_dev *str, const double *F, double *v, const double *x, double *xin, double *state,
              double fmax, double dtmax, double maxstep, hipStream_t st);
 int mgk_fire_count(int S, const double *state, int *out, hipStream_t st);
+int mgk_fire_cell(int S, mg_struct_dev *str, const double *F, const double *U, double *v, double *cellv,
+                  const double *x, double *xin, double *state, double p, double fmax, double smax, double dtmax,
+                  double maxstep, hipStream_t st);
 
 #define KCHK(x) do { if ((x) != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error()); } while (0)
 
@@ -54,6 +57,9 @@ static int ensure_dyn(mg_ctx *c) {
     size_t n = c->S + 4;
     if ((r = mg_dev_alloc((void **)&c->d_fire, n * 5 * sizeof(double))) ||
         (r = mg_dev_alloc((void **)&c->d_snap_fire, n * 5 * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_cellv, n * 9 * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_cellv, n * 9 * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_str, n * sizeof(mg_struct_dev))) ||
         (r = mg_dev_alloc((void **)&c->d_dyn, 16 * sizeof(double))) ||
         (r = mg_dev_alloc((void **)&c->d_snap_dyn, 16 * sizeof(double))))
       return r;
@@ -174,24 +180,32 @@ static int snapshot(mg_ctx *c, int restore) {
     HIPCHK(hipMemcpyAsync(c->d_snap_v, c->d_v, nx, k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_snap_fire, c->d_fire, nf, k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_snap_dyn, c->d_dyn, 16 * sizeof(double), k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_snap_cellv, c->d_cellv, sizeof(double) * 9 * c->S, k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_snap_str, c->d_str, sizeof(mg_struct_dev) * c->S, k, c->st));
   } else {
     HIPCHK(hipMemcpyAsync(c->d_xin, c->d_snap_x, nx, k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_v, c->d_snap_v, nx, k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_fire, c->d_snap_fire, nf, k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_dyn, c->d_snap_dyn, 16 * sizeof(double), k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_cellv, c->d_snap_cellv, sizeof(double) * 9 * c->S, k, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_str, c->d_snap_str, sizeof(mg_struct_dev) * c->S, k, c->st));
   }
   return MG_OK;
 }
 
 typedef struct {
   double fmax, dtmax, maxstep;
+  int cell;            /* also relax the lattice (enthalpy at pressure p) */
+  double p, smax;      /* eV/A^3 */
 } fire_arg;
 
 static int fire_tail(mg_ctx *c, void *arg) {
   const fire_arg *a = (const fire_arg *)arg;
-  return mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, a->fmax, a->dtmax, a->maxstep, c->st)
-             ? mg_fail(MG_ERR_HIP, "kernel launch failed: mgk_fire (%s)", mgk_last_error())
-             : MG_OK;
+  const int e = a->cell ? mgk_fire_cell(c->S, c->d_str, c->d_F, c->d_U, c->d_v, c->d_cellv, c->d_x, c->d_xin,
+                                        c->d_fire, a->p, a->fmax, a->smax, a->dtmax, a->maxstep, c->st)
+                        : mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, a->fmax, a->dtmax,
+                                   a->maxstep, c->st);
+  return e ? mg_fail(MG_ERR_HIP, "kernel launch failed: FIRE step (%s)", mgk_last_error()) : MG_OK;
 }
 
 typedef struct {
@@ -210,11 +224,14 @@ static void book_free(relax_book *b) {
 static int compact(mg_ctx *c, relax_book *b, int repack) {
   const int S = c->S, N = c->N;
   double *x = (double *)malloc(sizeof(double) * 3 * N), *v = (double *)malloc(sizeof(double) * 3 * N);
-  double *st = (double *)malloc(sizeof(double) * 5 * S);
+  double *st = (double *)malloc(sizeof(double) * 5 * S), *cv = (double *)malloc(sizeof(double) * 9 * S);
+  mg_struct_dev *sd = (mg_struct_dev *)malloc(sizeof(mg_struct_dev) * S);
   int r = MG_OK;
   if (hipMemcpyAsync(x, c->d_xin, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
       hipMemcpyAsync(v, c->d_v, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
       hipMemcpyAsync(st, c->d_fire, sizeof(double) * 5 * S, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipMemcpyAsync(cv, c->d_cellv, sizeof(double) * 9 * S, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipMemcpyAsync(sd, c->d_str, sizeof(mg_struct_dev) * S, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
       hipStreamSynchronize(c->st) != hipSuccess) {
     r = mg_fail(MG_ERR_HIP, "mg_relax: state download failed");
     goto out;
@@ -223,6 +240,7 @@ static int compact(mg_ctx *c, relax_book *b, int repack) {
     const int o = b->act[k], n = b->off0[o + 1] - b->off0[o];
     memcpy(b->x0 + 3 * b->off0[o], x + 3 * c->h_atom_off[k], sizeof(double) * 3 * n);
     memcpy(b->st0 + 5 * o, st + 5 * k, sizeof(double) * 5);
+    memcpy(b->lat0 + 9 * o, sd[k].L, sizeof(double) * 9);  /* the cell relaxation moves the lattices */
   }
   if (repack) {
     int S1 = 0, N1 = 0;
@@ -232,6 +250,7 @@ static int compact(mg_ctx *c, relax_book *b, int repack) {
     int *xt = (int *)malloc(sizeof(int) * S1), *spc = (int *)malloc(sizeof(int) * b->N0);
     double *lat = (double *)malloc(sizeof(double) * 9 * S1), *xn = (double *)malloc(sizeof(double) * 3 * b->N0);
     double *vn = (double *)malloc(sizeof(double) * 3 * b->N0), *sn = (double *)malloc(sizeof(double) * 5 * S1);
+    double *cvn = (double *)malloc(sizeof(double) * 9 * (S1 > 0 ? S1 : 1));
     int kk = 0;
     for (int k = 0; k < S; k++) {
       if (st[5 * k + 3] != 0.0) continue;
@@ -244,6 +263,7 @@ static int compact(mg_ctx *c, relax_book *b, int repack) {
       memcpy(xn + 3 * N1, x + 3 * c->h_atom_off[k], sizeof(double) * 3 * n);
       memcpy(vn + 3 * N1, v + 3 * c->h_atom_off[k], sizeof(double) * 3 * n);
       memcpy(sn + 5 * kk, st + 5 * k, sizeof(double) * 5);
+      memcpy(cvn + 9 * kk, cv + 9 * k, sizeof(double) * 9);
       N1 += n;
       kk++;
     }
@@ -251,19 +271,22 @@ static int compact(mg_ctx *c, relax_book *b, int repack) {
     r = mgh_upload(c, S1, off, nd, xt, lat, spc, xn);
     if (r == MG_OK && (r = ensure_dyn(c)) == MG_OK) {
       if (hipMemcpy(c->d_v, vn, sizeof(double) * 3 * N1, hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(c->d_fire, sn, sizeof(double) * 5 * S1, hipMemcpyHostToDevice) != hipSuccess)
+          hipMemcpy(c->d_fire, sn, sizeof(double) * 5 * S1, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(c->d_cellv, cvn, sizeof(double) * 9 * S1, hipMemcpyHostToDevice) != hipSuccess)
         r = mg_fail(MG_ERR_HIP, "mg_relax: state upload failed");
     }
     b->S = S1;
-    free(off); free(nd); free(xt); free(spc); free(lat); free(xn); free(vn); free(sn);
+    free(off); free(nd); free(xt); free(spc); free(lat); free(xn); free(vn); free(sn); free(cvn);
   }
 out:
-  free(x); free(v); free(st);
+  free(x); free(v); free(st); free(cv); free(sd);
   return r;
 }
 
-int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done) {
-  if (!c || c->S < 1 || max_steps < 0 || fmax <= 0.0) return mg_fail(MG_ERR_ARG, "mg_relax: bad argument");
+static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps, double fmax, double dt0, double dtmax,
+                      double maxstep, int *steps, int *done) {
+  if (!c || c->S < 1 || max_steps < 0 || fmax <= 0.0 || (cell && smax <= 0.0))
+    return mg_fail(MG_ERR_ARG, "mg_relax: bad argument");
   HIPCHK(hipSetDevice(c->device));
   int r;
   if ((r = ensure_dyn(c))) return r;
@@ -291,8 +314,9 @@ int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, do
 #define RKCHK(x) do { if ((x) != 0) { if (gx) hipGraphExecDestroy(gx); book_free(&b); return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error()); } } while (0)
 #define RHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { if (gx) hipGraphExecDestroy(gx); book_free(&b); return mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
   hipGraphExec_t gx = NULL;
-  fire_arg fa = {fmax, dtmax, maxstep};
+  fire_arg fa = {fmax, dtmax, maxstep, cell, p, smax};
   RHIP(hipMemsetAsync(c->d_v, 0, sizeof(double) * 3 * c->N, c->st));
+  RHIP(hipMemsetAsync(c->d_cellv, 0, sizeof(double) * 9 * c->S, c->st));
   RKCHK(mgk_fire_init(c->S, c->d_fire, dt0, c->st));
   int it = 0, need_e = 0, need_r = 0;
   while (it < max_steps) {
@@ -305,7 +329,7 @@ int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, do
       nrun = c->graph_enable ? 1 : (max_steps - it < MG_GRAPH_CHECK ? max_steps - it : MG_GRAPH_CHECK);
       for (int k = 0; k < nrun; k++) {
         RCHK(mg_eval(c, MG_EVAL_FORCES));
-        RKCHK(mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, fmax, dtmax, maxstep, c->st));
+        RCHK(fire_tail(c, &fa));
         if (it + k == 0) {
           int any_frac = 0;
           for (int s = 0; s < c->S; s++) if (c->h_str[s].xt == 0) { c->h_str[s].xt = 1; any_frac = 1; }
@@ -361,6 +385,11 @@ int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, do
   }
   if (gx) hipGraphExecDestroy(gx);
   gx = NULL;
+  if (cell && !compacted) {
+    /* the lattices moved on the device: bring the host tables up to date */
+    RHIP(hipMemcpy(c->h_str, c->d_str, sizeof(mg_struct_dev) * c->S, hipMemcpyDeviceToHost));
+    for (int s = 0; s < c->S; s++) memcpy(c->h_lat + 9 * s, c->h_str[s].L, 9 * sizeof(double));
+  }
   if (compacted) {
     /* bring the last active structures into the book and restore the full batch with the final positions */
     RCHK(compact(c, &b, 0));
@@ -389,4 +418,14 @@ int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, do
 #undef RKCHK
 #undef RHIP
   return MG_OK;
+}
+
+int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done) {
+  return relax_impl(c, 0, 0.0, 0.0, max_steps, fmax, dt0, dtmax, maxstep, steps, done);
+}
+
+int mg_relax_cell(mg_ctx *c, double p_gpa, int max_steps, double fmax, double smax_gpa, double dt0, double dtmax,
+                  double maxstep, int *steps, int *done) {
+  const double ev2gpa = 160.2176565; /* eV/A^3 in GPa, the reference's eV2GPa */
+  return relax_impl(c, 1, p_gpa / ev2gpa, smax_gpa / ev2gpa, max_steps, fmax, dt0, dtmax, maxstep, steps, done);
 }

@@ -212,6 +212,153 @@ __global__ __launch_bounds__(256) void k_fire(int S, const mg_struct_dev* __rest
   }
 }
 
+// FIRE over positions and cell (the reference's full-cell relaxation, RLXT 3: enthalpy H = E + pV minimised over
+// atoms and lattice, src/cmin.c:90-166, there with GSL BFGS2).  The cell degree of freedom is the strain e of the
+// current lattice, scaled by the atom count N (generalized coordinate Y = N e, generalized force
+// G = (U - p V 1) / N with U the virial, so cell and atom steps are of comparable size).  Each step deforms the
+// lattice and the positions by 1 + dY / N, then the lattice, its reciprocal and the neighbour-search stencil of the
+// structure are updated on the device (bin counts stay as planned).  Converged when every |F| < fmax and every
+// |U - p V 1| / V < smax.  Cell velocities: 9 per structure.
+__global__ __launch_bounds__(256) void k_fire_cell(int S, mg_struct_dev* __restrict__ str, const double* __restrict__ F,
+                                                   const double* __restrict__ U, double* __restrict__ v,
+                                                   double* __restrict__ cellv, const double* __restrict__ x,
+                                                   double* __restrict__ xin, double* __restrict__ state, double p,
+                                                   double fmax, double smax, double dtmax, double maxstep) {
+  __shared__ double sh[5][256];
+  __shared__ double Dm[9];
+  const int s = blockIdx.x, t = threadIdx.x;
+  const int a0 = str[s].atom0, n = str[s].natom;
+  const bool cell = str[s].nd == 3;
+  double* st = state + 5 * s;
+  double* cv = cellv + 9 * s;
+  if (st[3] != 0.0) {
+    for (int i = a0 + t; i < a0 + n; i += 256)
+      for (int q = 0; q < 3; q++) v[3 * i + q] = 0.0;
+    return;
+  }
+  // generalized cell force G (symmetric), from the structure's virial xx yy zz xy yz zx
+  double L[9];
+  for (int k = 0; k < 9; k++) L[k] = str[s].L[k];
+  const double V = fabs(L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
+                        L[2] * (L[3] * L[7] - L[4] * L[6]));
+  const double* u = U + 6 * s;
+  double G[9] = {u[0] - p * V, u[3], u[5], u[3], u[1] - p * V, u[4], u[5], u[4], u[2] - p * V};
+  double smx = 0.0;
+  for (int k = 0; k < 9; k++) {
+    const double sk = fabs(G[k]) / V;
+    smx = sk > smx ? sk : smx;
+    G[k] = cell ? G[k] / (double)n : 0.0;
+  }
+  if (!cell) smx = 0.0;
+  double vf = 0, ff = 0, vv = 0, fm = 0;
+  for (int i = a0 + t; i < a0 + n; i += 256) {
+    double f2 = 0;
+    for (int q = 0; q < 3; q++) {
+      double f = F[3 * i + q], w = v[3 * i + q];
+      vf += f * w; f2 += f * f; vv += w * w;
+    }
+    ff += f2;
+    fm = f2 > fm ? f2 : fm;
+  }
+  if (t == 0)
+    for (int k = 0; k < 9; k++) { vf += cv[k] * G[k]; ff += G[k] * G[k]; vv += cv[k] * cv[k]; }
+  sh[0][t] = vf; sh[1][t] = ff; sh[2][t] = vv; sh[3][t] = fm;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[0][t] += sh[0][t + w]; sh[1][t] += sh[1][t + w]; sh[2][t] += sh[2][t + w];
+      sh[3][t] = sh[3][t] > sh[3][t + w] ? sh[3][t] : sh[3][t + w];
+    }
+    __syncthreads();
+  }
+  vf = sh[0][0]; ff = sh[1][0]; vv = sh[2][0]; fm = sh[3][0];
+  __syncthreads();
+  if (fm < fmax * fmax && smx < smax) {
+    for (int i = a0 + t; i < a0 + n; i += 256)
+      for (int q = 0; q < 3; q++) { xin[3 * i + q] = x[3 * i + q]; v[3 * i + q] = 0.0; }
+    if (t == 0) {
+      st[3] = 1.0;
+      for (int k = 0; k < 9; k++) cv[k] = 0.0;
+    }
+    return;
+  }
+  double dt = st[0], alpha = st[1], npos = st[2];
+  bool reset = false;
+  double c1 = 1.0, c2 = 0.0;
+  if (vf > 0.0) {
+    c1 = 1.0 - alpha;
+    c2 = ff > 0.0 ? alpha * sqrt(vv / ff) : 0.0;
+    if (npos > FIRE_NMIN) {
+      dt = fmin(dt * FIRE_FINC, dtmax);
+      alpha *= FIRE_FA;
+    }
+    npos += 1.0;
+  } else {
+    reset = true;
+    alpha = FIRE_ASTART;
+    dt *= FIRE_FDEC;
+    npos = 0.0;
+  }
+  double dr2 = 0.0;
+  for (int i = a0 + t; i < a0 + n; i += 256)
+    for (int q = 0; q < 3; q++) {
+      double w = reset ? 0.0 : c1 * v[3 * i + q] + c2 * F[3 * i + q];
+      w += dt * F[3 * i + q];
+      v[3 * i + q] = w;
+      dr2 += (dt * w) * (dt * w);
+    }
+  double cvn[9];
+  for (int k = 0; k < 9; k++) {
+    double w = reset ? 0.0 : c1 * cv[k] + c2 * G[k];
+    cvn[k] = w + dt * G[k];
+  }
+  if (t == 0)
+    for (int k = 0; k < 9; k++) dr2 += (dt * cvn[k]) * (dt * cvn[k]);
+  sh[0][t] = dr2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sh[0][t] += sh[0][t + w];
+    __syncthreads();
+  }
+  const double nrm = sqrt(sh[0][0]);
+  const double sc = nrm > maxstep ? maxstep / nrm : 1.0;
+  if (t == 0) {
+    // deformation D = 1 + dY / N (symmetric, so no rotation)
+    for (int k = 0; k < 9; k++) Dm[k] = (k % 4 == 0 ? 1.0 : 0.0) + (cell ? sc * dt * cvn[k] / (double)n : 0.0);
+  }
+  __syncthreads();
+  for (int i = a0 + t; i < a0 + n; i += 256) {
+    const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+    for (int q = 0; q < 3; q++)
+      xin[3 * i + q] = x0 * Dm[q] + x1 * Dm[3 + q] + x2 * Dm[6 + q] + sc * dt * v[3 * i + q];
+  }
+  if (t == 0) {
+    for (int k = 0; k < 9; k++) cv[k] = cvn[k];
+    st[0] = dt; st[1] = alpha; st[2] = npos; st[4] += 1.0;
+    if (cell) {
+      mg_struct_dev& d = str[s];
+      double Ln[9];
+      for (int q = 0; q < 3; q++)
+        for (int b = 0; b < 3; b++) Ln[3 * q + b] = L[3 * q] * Dm[b] + L[3 * q + 1] * Dm[3 + b] + L[3 * q + 2] * Dm[6 + b];
+      // reciprocal rows (L1 x L2, L2 x L0, L0 x L1) / det
+      const double* a = Ln; const double* bb = Ln + 3; const double* c = Ln + 6;
+      double R[9];
+      R[0] = bb[1] * c[2] - bb[2] * c[1]; R[1] = bb[2] * c[0] - bb[0] * c[2]; R[2] = bb[0] * c[1] - bb[1] * c[0];
+      R[3] = c[1] * a[2] - c[2] * a[1];   R[4] = c[2] * a[0] - c[0] * a[2];   R[5] = c[0] * a[1] - c[1] * a[0];
+      R[6] = a[1] * bb[2] - a[2] * bb[1]; R[7] = a[2] * bb[0] - a[0] * bb[2]; R[8] = a[0] * bb[1] - a[1] * bb[0];
+      const double det = a[0] * R[0] + a[1] * R[1] + a[2] * R[2];
+      for (int k = 0; k < 9; k++) { d.L[k] = Ln[k]; d.R[k] = R[k] / det; }
+      const double rc = sqrt(d.rc2);
+      for (int q = 0; q < 3; q++) {
+        const double h = rc * sqrt(d.R[3 * q] * d.R[3 * q] + d.R[3 * q + 1] * d.R[3 * q + 1] + d.R[3 * q + 2] * d.R[3 * q + 2]) *
+                         (1.0 + 1e-6);
+        const int K = (int)ceil(h * d.nb[q]);
+        d.K[q] = K < 1 ? 1 : K;
+      }
+    }
+  }
+}
+
 __global__ void k_fire_init(int S, double* __restrict__ state, double dt0) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -269,6 +416,14 @@ int mgk_fire_init(int S, double* state, double dt0, hipStream_t st) {
 int mgk_fire(int S, const mg_struct_dev* str, const double* F, double* v, const double* x, double* xin, double* state,
              double fmax, double dtmax, double maxstep, hipStream_t st) {
   hipLaunchKernelGGL(k_fire, dim3(S), dim3(256), 0, st, S, str, F, v, x, xin, state, fmax, dtmax, maxstep);
+  return mgk_status();
+}
+
+int mgk_fire_cell(int S, mg_struct_dev* str, const double* F, const double* U, double* v, double* cellv,
+                  const double* x, double* xin, double* state, double p, double fmax, double smax, double dtmax,
+                  double maxstep, hipStream_t st) {
+  hipLaunchKernelGGL(k_fire_cell, dim3(S), dim3(256), 0, st, S, str, F, U, v, cellv, x, xin, state, p, fmax, smax,
+                     dtmax, maxstep);
   return mgk_status();
 }
 

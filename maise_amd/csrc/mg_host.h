@@ -93,7 +93,9 @@ struct mg_ctx {
   double *d_dyn;        /* scalars: [0] Nose-Hoover xi, [1] kinetic energy, ... */
   double *d_fire;       /* per-structure FIRE state [S][4]: dt, alpha, npos, done */
   double *d_trace;      /* per-step energies */
-  double *d_snap_x, *d_snap_v, *d_snap_fire, *d_snap_dyn; /* state at the last graph-mode checkpoint */
+  double *d_cellv;       /* cell-relaxation velocities [S][9] */
+  double *d_snap_x, *d_snap_v, *d_snap_fire, *d_snap_dyn, *d_snap_cellv; /* state at the last graph checkpoint */
+  mg_struct_dev *d_snap_str;                                             /* (lattices change in cell relaxation) */
   size_t cap_dyn_atoms, cap_dyn_str, cap_trace;
 };
 

@@ -122,3 +122,54 @@ def test_fire_graph_replay_matches_synchronised_loop():
         assert np.array_equal(out[0][3], out[k][3])
         assert np.array_equal(out[0][4], out[k][4])
     assert out[0][1].all()
+
+
+def test_cell_relax_reaches_reference_minimum():
+    """Full-cell relaxation against the reference's relaxed state (path independent, so the end point pins it
+    although the reference minimises with GSL BFGS2 and this build with FIRE): test/relax, L1_2 Cu3Pd from
+    a = 4.0 A (test/relax/POSCAR) with its model, relaxed to a = 3.72890112684628 A (test/relax/ref/CONTCAR) with
+    per-atom energies -4.648947790797 (Cu) and -4.134534446656 eV (Pd) (test/relax/ref/OUTCAR:44-54)."""
+    import maise_amd
+    print("[gpu case] cell relax test/relax Cu3Pd", flush=True)
+    m = maise_amd.Model(os.path.join(GOLDEN, "models", "nn_CuPd_d3_vp_relax"))
+    frac = np.array([[0, 0, 0], [0, .5, .5], [.5, .5, 0], [.5, 0, .5]])
+    spc = np.array([0, 0, 0, 1], np.int32)
+    c = maise_amd.Context(m)
+    c.set_structure(spc, np.eye(3) * 4.0, frac, xt=0)
+    steps, done = c.relax_cell(p_gpa=0.0, max_steps=4000, fmax=1e-4, smax_gpa=2e-4)
+    assert done[0], steps
+    lat = c.lattice(0)
+    print("relaxed lattice", np.diag(lat), "steps", steps[0], flush=True)
+    np.testing.assert_allclose(np.diag(lat), 3.72890112684628, atol=5e-5)
+    assert np.abs(lat - np.diag(np.diag(lat))).max() < 1e-6
+    r = c.result(0)
+    e_ref = 3 * -4.648947790797 + -4.134534446656
+    assert abs(r["E"] - e_ref) / 4 < 1e-5  # FP32 path, per atom
+    # the relaxed structure itself, evaluated in FP64: the reference's relaxed energies
+    from oracle import pyoracle as po
+    o = po.OracleModel(os.path.join(GOLDEN, "models", "nn_CuPd_d3_vp_relax")).evaluate(spc, lat, r["X"])
+    # (per-atom energies move at first order with the residual displacements; the total is stationary)
+    assert o["E"] == pytest.approx(e_ref, abs=2e-7)
+    c.close()
+
+
+def test_cell_relax_examples_03rlx_final_energy():
+    """examples/03-rlx: Cu4 in a 3.626 x 3.626 x 6.026 A cell with its model relaxes (positions + cell) to the
+    minimum H = -16.3965601100172194 eV (examples/03-rlx/README:48)."""
+    import maise_amd
+    from maise_amd import poscar
+    print("[gpu case] cell relax examples/03-rlx", flush=True)
+    p = poscar.read_poscar(os.path.join(GOLDEN, "structures", "POSCAR_03rlx"))
+    c = maise_amd.Context(maise_amd.Model(os.path.join(GOLDEN, "models", "nn_Cu_03rlx")))
+    c.set_structure(p.species, p.lattice, p.frac, xt=0)
+    steps, done = c.relax_cell(p_gpa=0.0, max_steps=6000, fmax=1e-4, smax_gpa=2e-4)
+    assert done[0], steps
+    E = c.result(0)["E"]
+    print("03-rlx final E", E, "steps", steps[0], "lattice", c.lattice(0).round(6).tolist(), flush=True)
+    assert E == pytest.approx(-16.3965601100172194, abs=4e-5)  # FP32 path: 1e-5 eV/atom
+    # the relaxed structure itself, evaluated in FP64, sits at the reference's minimum
+    from oracle import pyoracle as po
+    o = po.OracleModel(os.path.join(GOLDEN, "models", "nn_Cu_03rlx")).evaluate(p.species, c.lattice(0),
+                                                                               c.result(0)["X"])
+    assert o["E"] == pytest.approx(-16.3965601100172194, abs=1e-7)
+    c.close()
