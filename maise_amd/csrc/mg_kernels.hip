@@ -332,10 +332,20 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
 }
 
 // Reverse edge of e = (i -> j, img): the entry (j -> i, -img) of row j, found by binary search on keys.
+// The 8 XCDs of the chip take workgroups round robin (block b runs on XCD b mod 8), each with its own L2.  Kernels
+// that read the rows of an atom's neighbours remap blocks so every XCD works through one contiguous range of
+// atoms: neighbouring atoms then share their neighbours' rows in that XCD's L2.
+#define MG_NXCD 8
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  const int per = nblk / MG_NXCD, rem = nblk % MG_NXCD;  // XCD x takes per (+1 for x < rem) blocks
+  const int x = b % MG_NXCD, k = b / MG_NXCD;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
 __global__ void k_rev(int N, const int* __restrict__ spc, const int* __restrict__ row_off, const u64* __restrict__ key,
                       int* __restrict__ rev, int* __restrict__ err, const int* __restrict__ ovf) {
   if (*ovf) return;
-  int i = blockIdx.x;
+  const int i = xcd_block(blockIdx.x, gridDim.x);
   int off = row_off[i], M = row_off[i + 1] - off;
   int si = spc[i];
   for (int t = threadIdx.x; t < M; t += blockDim.x) {
@@ -971,7 +981,7 @@ __global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ r
                                                 const float4* __restrict__ fself, const float4* __restrict__ fe,
                                                 double* __restrict__ F, const int* __restrict__ ovf) {
   if (*ovf) return;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int a = g >> 2, sub = g & 3;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (a < N) {
