@@ -48,7 +48,8 @@ const char *mg_last_error(void);
 /* Load a MAISE nn_* model file (architecture D-H1-H2-1, G2/G4 basis appended). */
 int mg_model_load(const char *path, mg_model **out);
 void mg_model_free(mg_model *m);
-/* info[0..11] = nspc, D, H1, H2, nsym, ng2, ng4, nw, spcz[0..3]; *rc = neighbour-list cutoff (A). */
+/* info[0..15] = nspc, D, H1, H2, nsym, ng2, ng4, nw, spcz[0..3], G4 kernel variant (0 generic, 1 the 43-function
+ * table, 2 the 22-function table), shared-eta flag, MFMA MLP flag, 0; *rc = neighbour-list cutoff (A). */
 int mg_model_info(const mg_model *m, int *info, double *rc);
 
 /* --- evaluation context (one device stream, one structure or a batch of structures) --- */

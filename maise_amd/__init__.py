@@ -117,11 +117,12 @@ class Model:
     def __init__(self, path: str):
         self._h = C.c_void_p()
         _check(lib().mg_model_load(os.fsencode(path), C.byref(self._h)), f"loading model {path}")
-        info = np.zeros(12, dtype=np.int32)
+        info = np.zeros(16, dtype=np.int32)
         rc = C.c_double()
         _check(lib().mg_model_info(self._h, _ip(info), C.byref(rc)), "mg_model_info")
         self.nspc, self.D, self.H1, self.H2, self.nsym, self.ng2, self.ng4, self.nw = (int(v) for v in info[:8])
         self.spcz = [int(v) for v in info[8:8 + self.nspc]]
+        self.g4_variant, self.shared_eta, self.mfma = int(info[12]), bool(info[13]), bool(info[14])
         self.rc = rc.value
         self.path = path
 

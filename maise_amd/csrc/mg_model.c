@@ -427,7 +427,8 @@ int mg_model_load(const char *path, mg_model **out) {
       static const unsigned m22[8] = {0x0F, 0x0F, 0x0A, 0xAA, 0xAA, 0xAA, 0x00, 0x00};
       for (int cand = 1; cand <= 2 && m->pat == 0; cand++) {
         const unsigned *mk = cand == 1 ? m43 : m22;
-        int ok = (cand == 1 && B->ne == 7) || (cand == 2 && B->ne == 6);
+        /* eta rows past the table's (7 or 6) must be unused, which the mask comparison below checks */
+        int ok = (cand == 1 && B->ne >= 7) || (cand == 2 && B->ne >= 6);
         for (int q = 0; q < MG_GRID && ok; q++) {
           int used = (mk[q / 8] >> (q % 8)) & 1u;
           if (used != (B->g4map[q] >= 0)) ok = 0;
@@ -483,6 +484,10 @@ int mg_model_info(const mg_model *m, int *info, double *rc) {
   info[0] = m->nspc; info[1] = m->D; info[2] = m->H1; info[3] = m->H2;
   info[4] = m->nsym; info[5] = m->ng2; info[6] = m->ng4; info[7] = m->nw;
   for (int s = 0; s < 4; s++) info[8 + s] = s < m->nspc ? m->spcz[s] : 0;
+  info[12] = m->pat;   /* G4 kernel specialisation: 0 generic, 1 43-function table, 2 22-function table */
+  info[13] = m->uni;   /* 1: every species pair shares the G4 etas (one exp per triplet) */
+  info[14] = m->mfma_ok;
+  info[15] = 0;
   if (rc) *rc = m->rc_list;
   return MG_OK;
 }

@@ -97,6 +97,24 @@ def standard_basis(rc: float = 6.0, eta2=None, eta4=None, zetas=(1.0, 2.0, 4.0, 
     return Basis(funcs=funcs, params=params, b2a=b2a, scale=scale)
 
 
+def generic_basis(rc: float = 6.0, eta2=None, eta4=None, zetas=(1.0, 3.0), lambdas=(-1.0, 1.0), b2a: float = 1.0,
+                  scale: float = 1.0) -> Basis:
+    """A G2 + G4 basis on the full (eta, zeta, lambda) grid with zeta values of the caller's choosing (the
+    default zeta = {1, 3} matches neither shipped G4 table)."""
+    if eta2 is None:
+        eta2 = np.geomspace(0.005, 0.4, 6)
+    if eta4 is None:
+        eta4 = np.geomspace(0.005, 0.2, 3)
+    funcs = [(2, 0, 0, n, 0, 0, 0, 0) for n in range(len(eta2))]
+    for e in range(len(eta4)):
+        for z in range(len(zetas)):
+            for l in range(len(lambdas)):
+                funcs.append((4, 0, 0, 0, e, 0, z, l))
+    params = {"Rc": [rc], "Rs": [0.0], "n1": list(map(float, eta2)), "n2": list(map(float, eta4)),
+              "k": [1.0], "z": list(zetas), "l": list(lambdas)}
+    return Basis(funcs=funcs, params=params, b2a=b2a, scale=scale)
+
+
 @dataclass
 class Network:
     """Per-species D-H1-H2-1 network: ``biases[s][k]`` (length NU[k+1]) and ``weights[s][k]`` ([out][in])."""
@@ -146,14 +164,22 @@ def _row(tag: str, value: str) -> str:
     return body.ljust(76) + "|"
 
 
-def write_model(path: str, net: Network, basis: Basis, model_id: str = "5EED0000",
+def write_model(path: str, net: Network, basis, model_id: str = "5EED0000",
                 version: str = "maise.3.0.00") -> None:
-    """Write an ``nn_*`` model file readable by the reference (format of src/nutl.c:394-498)."""
+    """Write an ``nn_*`` model file readable by the reference (format of src/nutl.c:394-498).
+
+    ``basis`` is one Basis (used for every species) or a list of one Basis per species: species-specific blocks
+    follow each other separated by one blank line, the layout src/nprs.c:154-231 reads (after a block it takes
+    one line, then the next block's header line, then that block's scaling line)."""
     bar = "-" * 77
     S = len(net.spcz)
     D = net.arch[0]
-    if basis.width(S) != D:
-        raise ValueError(f"basis width {basis.width(S)} != input width {D}")
+    blocks = list(basis) if isinstance(basis, (list, tuple)) else [basis]
+    if len(blocks) not in (1, S):
+        raise ValueError("give one basis block, or one per species")
+    for b in blocks:
+        if b.width(S) != D or [tuple(f) for f in b.funcs] != [tuple(f) for f in blocks[0].funcs]:
+            raise ValueError(f"basis width {b.width(S)} != input width {D}, or blocks with different functions")
     lines = [bar, "|                    neural network general information                     |", bar,
              _row("model unique ID", model_id),
              _row("number of species", str(S)),
@@ -190,7 +216,9 @@ def write_model(path: str, net: Network, basis: Basis, model_id: str = "5EED0000
     lines.append(bar)
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
-        f.write(basis.text())
+        f.write(blocks[0].text())
+        for b in blocks[1:]:
+            f.write("\n" + b.text())
 
 
 def read_basis(path: str) -> Basis:

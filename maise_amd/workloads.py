@@ -122,15 +122,17 @@ def cupdag_batch(count: int = 4096, seed: int = 3, nat_range=(8, 64), workers: i
 
 def synthetic_model(path: str, spcz: Sequence[int], hidden: Tuple[int, int] = (10, 10), seed: int = 7,
                     rc: float = 6.0, range_from: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None,
-                    descriptor_max: Optional[np.ndarray] = None) -> modelfile.Network:
+                    descriptor_max: Optional[np.ndarray] = None, basis=None) -> modelfile.Network:
     """Write a seeded synthetic model: standard basis at cutoff rc (B2A=1, scale=1), D-H1-H2-1 tanh.
 
     ``descriptor_max`` (shape [S, D]) sets the input ranges to min 0, max = descriptor_max, range = 1/max, so
     the normalised inputs are O(1) and tanh is not saturated.  Without it the ranges are (0, 1, 1).
+    ``basis``: a Basis or a list of one per species in place of the standard one.
     """
-    basis = modelfile.standard_basis(rc=rc)
+    if basis is None:
+        basis = modelfile.standard_basis(rc=rc)
     S = len(spcz)
-    D = basis.width(S)
+    D = (basis[0] if isinstance(basis, (list, tuple)) else basis).width(S)
     net = modelfile.random_network(spcz, [D, hidden[0], hidden[1], 1], seed)
     if descriptor_max is not None:
         mx = np.maximum(np.asarray(descriptor_max, dtype=np.float64).reshape(S, D), 1e-6)
@@ -140,12 +142,13 @@ def synthetic_model(path: str, spcz: Sequence[int], hidden: Tuple[int, int] = (1
     return net
 
 
-def calibrated_model(path: str, spcz, hidden, seed, sample_structures, descriptor_fn, rc: float = 6.0):
+def calibrated_model(path: str, spcz, hidden, seed, sample_structures, descriptor_fn, rc: float = 6.0, basis=None):
     """Two-step synthetic model: write with unit ranges, compute descriptors of the sample structures with
     ``descriptor_fn(model_path, spc, lat, x) -> G[N, D]``, then rewrite with ranges 1/max(G) per species."""
-    synthetic_model(path, spcz, hidden, seed, rc)
+    synthetic_model(path, spcz, hidden, seed, rc, basis=basis)
     S = len(spcz)
-    first = modelfile.standard_basis(rc=rc).width(S)
+    b0 = modelfile.standard_basis(rc=rc) if basis is None else (basis[0] if isinstance(basis, (list, tuple)) else basis)
+    first = b0.width(S)
     gmax = np.zeros((S, first))
     for spc, lat, x in sample_structures:
         G = descriptor_fn(path, spc, lat, x)
@@ -154,5 +157,5 @@ def calibrated_model(path: str, spcz, hidden, seed, sample_structures, descripto
             if sel.any():
                 gmax[s] = np.maximum(gmax[s], G[sel].max(axis=0))
     gmax[gmax <= 0] = 1.0
-    synthetic_model(path, spcz, hidden, seed, rc, descriptor_max=gmax)
+    synthetic_model(path, spcz, hidden, seed, rc, descriptor_max=gmax, basis=basis)
     return path

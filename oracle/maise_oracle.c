@@ -610,9 +610,18 @@ static double mlp_atom(const orc_model *m, int sp, const double *G, double *dEdG
  * -dE/dx onto the atoms (F) and as U_ab -= (dE/dd)_a d_b into the virial (xx,yy,zz,xy,yz,zx), matching
  * the sign and component order of Fn/Sn in PRS_BP (src/nprs.c:568-770) contracted in FRC_ANN
  * (src/nmlp.c:433-484). */
+/* orc_ref_deriv = 1 reproduces the reference's derivative code as written: PRS_BP's derivative sections
+ * (src/nprs.c:597-738) take every parameter -- cutoffs, G2 eta and Rs, G4 eta -- from P->GP, the species-0 block
+ * (src/nprs.c:193-197), while the forward pass (src/nprs.c:515-551) uses each species pair's block.  With a
+ * single basis block (every shipped model) both agree; with species-specific blocks the reference's forces are
+ * then not -dE/dx.  The default (0) is the exact gradient of the energy. */
+static int orc_ref_deriv = 0;
+void orc_set_ref_derivatives(int on) { orc_ref_deriv = on != 0; }
+
 static void force_atom(const orc_model *m, const orc_tabs *t, const int *atmn, const orc_row *w, int i,
                        const double *g, double *F, double *U) {
   const orc_basis *b0 = &m->w[0];
+  const int q0 = orc_ref_deriv;
   const int S = m->nspc;
   const int g2 = t->g2first;
   int s0 = atmn[i];
@@ -625,7 +634,7 @@ static void force_atom(const orc_model *m, const orc_tabs *t, const int *atmn, c
     int j = w->j[jj];
     int s1 = atmn[j];
     int s01 = s0 * s0 + s1 * s1;
-    const orc_basis *B01 = &m->w[s01];
+    const orc_basis *B01 = q0 ? b0 : &m->w[s01];
     double R01 = B01->gp[0][b0->gf[0][1]];
     int ds1 = (s1 - s0 + S) % S;
     double rij = w->r[jj];
@@ -653,8 +662,9 @@ static void force_atom(const orc_model *m, const orc_tabs *t, const int *atmn, c
       int kk = w->j[k];
       int s2 = atmn[kk];
       int s02 = s0 * s0 + s2 * s2, s12 = s1 * s1 + s2 * s2;
-      double R02 = m->w[s02].gp[0][b0->gf[g2][1]];
-      double R12 = m->w[s12].gp[0][b0->gf[g2][1]];
+      const orc_basis *B02 = q0 ? b0 : &m->w[s02], *B12 = q0 ? b0 : &m->w[s12];
+      double R02 = B02->gp[0][b0->gf[g2][1]];
+      double R12 = B12->gp[0][b0->gf[g2][1]];
       int ds2 = (s2 - s0 + S) % S;
       double rik = w->r[k];
       const double *b = &w->d[3 * k];
@@ -679,7 +689,7 @@ static void force_atom(const orc_model *m, const orc_tabs *t, const int *atmn, c
           double u = 1.0 + lam * cs;
           double P = t->powz[z] * pow(u, zeta);
           double Pd = t->powz[z] * zeta * lam * pow(u, zeta - 1.0);
-          double e01 = m->w[s01].gp[3][e], e02 = m->w[s02].gp[3][e], e12 = m->w[s12].gp[3][e];
+          double e01 = B01->gp[3][e], e02 = B02->gp[3][e], e12 = B12->gp[3][e];
           double Bq = 2.0 * exp(-rij * rij * e01 - rik * rik * e02 - rjk * rjk * e12);
           double gn = gb[c4];
           X += gn * P * Bq;

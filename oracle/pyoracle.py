@@ -39,6 +39,8 @@ def lib():
         L.orc_mlp.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
         L.orc_jar.argtypes = [C.c_int, dp, dp]
+        L.orc_set_ref_derivatives.argtypes = [C.c_int]
+        L.orc_set_ref_derivatives.restype = None
         _lib = L
     return _lib
 
@@ -72,8 +74,10 @@ class OracleModel:
             pass
 
     def evaluate(self, spc, lat, x, nd=3, xt=1, p=0.0, urep=0.0, forces=True, threshold=True, cap=4096,
-                 want_g=False):
-        """FP64 E/F/S of one structure.  p is in eV/A^3 (CALL_MAISE divides its GPa input by eV2GPa)."""
+                 want_g=False, ref_derivatives=False):
+        """FP64 E/F/S of one structure.  p is in eV/A^3 (CALL_MAISE divides its GPa input by eV2GPa).
+        ref_derivatives: forces and virial as the reference's derivative code computes them (species-0 basis
+        parameters for every pair; differs from -dE/dx only for models with species-specific basis blocks)."""
         spc = np.ascontiguousarray(spc, dtype=np.int32)
         lat = np.ascontiguousarray(lat, dtype=np.float64).reshape(9)
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
@@ -86,9 +90,13 @@ class OracleModel:
         g = np.zeros(N * self.D) if want_g else None
         Xw = np.zeros(3 * N)
         nn = np.zeros(N, dtype=np.int32)
-        rc = lib().orc_eval(self._h, N, nd, xt, _ip(spc), _dp(lat), _dp(x), p, urep, 1 if forces else 0,
-                            1 if threshold else 0, cap, C.byref(H), C.byref(E), _dp(EA), _dp(F), _dp(U), _dp(G),
-                            _dp(g), _dp(Xw), _ip(nn))
+        lib().orc_set_ref_derivatives(1 if ref_derivatives else 0)
+        try:
+            rc = lib().orc_eval(self._h, N, nd, xt, _ip(spc), _dp(lat), _dp(x), p, urep, 1 if forces else 0,
+                                1 if threshold else 0, cap, C.byref(H), C.byref(E), _dp(EA), _dp(F), _dp(U), _dp(G),
+                                _dp(g), _dp(Xw), _ip(nn))
+        finally:
+            lib().orc_set_ref_derivatives(0)
         if rc != 0:
             raise RuntimeError("oracle: neighbour capacity exceeded")
         out = dict(H=H.value, E=E.value, EA=EA, F=F.reshape(N, 3), U=U, X=Xw.reshape(N, 3), nn=nn)

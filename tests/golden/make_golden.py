@@ -55,6 +55,13 @@ def make_potentials():
     p = os.path.join(POT, "cupdag_282_10_10.nn")
     wl.calibrated_model(p, [29, 46, 47], (10, 10), 10, c3, oracle_descriptors)
     out["cupdag_282_10_10.nn"] = p
+    # species-specific bases (test data, kept with the fixtures)
+    cs = [wl.alloy_fcc(3, 3.8, 2, 0.1, seed=7)]
+    for name, mk in (("species_generic", modelfile.generic_basis), ("species_std", modelfile.standard_basis)):
+        p = os.path.join(MODELS, f"nn_CuAg_{name}")
+        wl.calibrated_model(p, [29, 47], (10, 10), 11, cs, oracle_descriptors,
+                            basis=[mk(rc=6.0), mk(rc=6.0, scale=1.1)])
+        out[name] = p
     return out
 
 
@@ -108,6 +115,12 @@ def make_ref_cases(pots):
     rng = np.random.default_rng(12)
     spc, lat, x = wl.random_triclinic(40, 3, rng)
     cases.update(ref_case("config3_like", pots["cupdag_282_10_10.nn"], spc, lat, x))
+    # species-specific basis blocks (Ag block scaled by 1.1): cross-species etas differ by pair (the mixing rule
+    # of src/nprs.c:233-252), once with a zeta = {1, 3} basis matching neither shipped G4 table, once with the
+    # 43-function table
+    spc, lat, x = wl.alloy_fcc(3, 3.8, 2, 0.1, seed=8)
+    cases.update(ref_case("species_generic", pots["species_generic"], spc, lat, x))
+    cases.update(ref_case("species_std", pots["species_std"], spc, lat, x))
     np.savez_compressed(os.path.join(HERE, "ref_cases.npz"), **cases)
 
 

@@ -29,10 +29,16 @@ def _eval(case, **kw):
 
 
 @pytest.mark.parametrize("name", ["iface", "relax", "relax_p", "cu", "cuag", "cupdag_tri", "urep", "small_tri",
-                                  "mgca", "config1", "config1_w64", "shell", "config3_like"])
+                                  "mgca", "config1", "config1_w64", "shell", "config3_like", "species_generic",
+                                  "species_std"])
 def test_oracle_matches_reference_build(ref_cases, name):
+    """Energies, descriptors, wrapped positions and neighbour sets against the reference build; forces and virial
+    too -- for the two species-specific-basis models with the oracle reproducing the reference's derivative code
+    (species-0 basis parameters for every pair, src/nprs.c:597-738), whose forces there are not -dE/dx; the
+    oracle's default (exact) gradient for those is pinned by finite differences in the test below."""
     c = ref_cases[name]
-    m, o = _eval(c)
+    species_specific = name.startswith("species_")
+    m, o = _eval(c, ref_derivatives=species_specific)
     N = len(c["spc"])
     assert o["H"] == pytest.approx(float(c["H"]), rel=1e-12, abs=1e-10)
     np.testing.assert_allclose(o["EA"], c["EA"], rtol=1e-12, atol=1e-11)
@@ -112,6 +118,32 @@ def test_forces_are_minus_energy_gradient():
             xm[a, q] -= h
             d = (m.evaluate(spc, lat, xp, forces=False)["E"] - m.evaluate(spc, lat, xm, forces=False)["E"]) / (2 * h)
             assert o["F"][a, q] == pytest.approx(-d, abs=2e-6)
+
+
+@pytest.mark.parametrize("name", ["species_generic", "species_std"])
+def test_species_specific_basis_gradient(ref_cases, name):
+    """Models with species-specific basis blocks: the oracle's energy equals the reference build's (test above),
+    and its default forces and virial are the exact gradient of that energy (central differences), while the
+    reference's own forces differ from -dE/dx by up to ~0.1-0.5 eV/A (its derivative code uses the species-0
+    block for every pair)."""
+    c = ref_cases[name]
+    m = po.OracleModel(model_path(c["model"]))
+    spc, lat, x = c["spc"], c["lat"], c["x"]
+    o = m.evaluate(spc, lat, x, threshold=False)
+    h = 1e-5
+    for a in (0, 1, 60):
+        for q in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[a, q] += h
+            xm[a, q] -= h
+            d = (m.evaluate(spc, lat, xp, forces=False)["E"] - m.evaluate(spc, lat, xm, forces=False)["E"]) / (2 * h)
+            assert o["F"][a, q] == pytest.approx(-d, abs=2e-6)
+    h = 1e-6
+    eps = np.eye(3) * h
+    d = (m.evaluate(spc, lat @ (np.eye(3) + eps), x @ (np.eye(3) + eps), forces=False)["E"] -
+         m.evaluate(spc, lat @ (np.eye(3) - eps), x @ (np.eye(3) - eps), forces=False)["E"]) / (2 * h)
+    assert o["U"][:3].sum() == pytest.approx(-d, rel=1e-5, abs=2e-5)  # trace of the virial vs isotropic strain
+    assert np.abs(o["F"] - c["F"]).max() > 1e-3  # the reference's forces are not this gradient
 
 
 def test_virial_is_minus_strain_derivative():
