@@ -96,6 +96,12 @@ int mg_ctx_set_wpb(mg_ctx *c, int wpb_desc, int wpb_force);
  * last checkpoint (every 8 iterations) and repeated with larger buffers, so results do not depend on this
  * setting.  enable = 0 runs every iteration synchronised.  No counterpart in the reference (CPU only). */
 int mg_ctx_set_graph(mg_ctx *c, int enable, double headroom);
+/* The descriptor kernel stores the neighbour pairs (j, k) with r_jk < Rc that it enumerates for the G4 terms, and
+ * the force kernel reads them back instead of enumerating again (default on; the list takes up to
+ * rows x (rows - 1) / 2 x 4 bytes per atom and is skipped when that exceeds 1/16 of the device memory).  Results do
+ * not depend on this setting.  No counterpart in the reference (PRS_BP enumerates the triplets once per call,
+ * src/nprs.c:515-551, and derives forces in the same pass). */
+int mg_ctx_set_pair_store(mg_ctx *c, int enable);
 /* Autotuner: times 1, 2 and 4 waves per workgroup on the structure/batch currently set (reps evaluations each)
  * and keeps the fastest shape per kernel; wpb_out[2] (may be NULL) receives (descriptor, force). */
 int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out);

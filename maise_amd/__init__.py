@@ -55,6 +55,7 @@ def lib():
             "mg_ctx_set_wpb": ([vp, C.c_int, C.c_int], C.c_int),
             "mg_ctx_autotune": ([vp, C.c_int, ip], C.c_int),
             "mg_ctx_set_graph": ([vp, C.c_int, C.c_double], C.c_int),
+            "mg_ctx_set_pair_store": ([vp, C.c_int], C.c_int),
             "mg_set_structure": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp], C.c_int),
             "mg_set_batch": ([vp, C.c_int, ip, ip, dp, ip, dp], C.c_int),
             "mg_eval": ([vp, C.c_int], C.c_int),
@@ -177,6 +178,11 @@ class Context:
         overflow is detected and rolled back, so results do not depend on it); enable=False synchronises every
         iteration."""
         _check(lib().mg_ctx_set_graph(self._h, 1 if enable else 0, float(headroom)), "mg_ctx_set_graph")
+
+    def set_pair_store(self, enable: bool = True):
+        """The descriptor kernel stores its G4 neighbour-pair lists for the force kernel (default); False makes the
+        force kernel enumerate the pairs itself.  Results do not depend on it."""
+        _check(lib().mg_ctx_set_pair_store(self._h, 1 if enable else 0), "mg_ctx_set_pair_store")
 
     def autotune(self, reps: int = 3):
         """Time 1/2/4 waves per workgroup on the current structure or batch; keep the fastest per kernel."""

@@ -47,6 +47,7 @@ static int ensure_atoms(mg_ctx *c, size_t N) {
   ALLOC(c->d_bin, n * sizeof(int));
   ALLOC(c->d_bin_atoms, n * sizeof(int));
   ALLOC(c->d_cnt, n * 4 * sizeof(int));
+  ALLOC(c->d_npl, n * MG_NCLS * sizeof(int));
   ALLOC(c->d_rowcnt, n * sizeof(int));
   ALLOC(c->d_row_off, (n + 1) * sizeof(int));
   ALLOC(c->d_G, n * D * sizeof(float));
@@ -95,6 +96,30 @@ static int ensure_bins(mg_ctx *c, size_t nb) {
   return MG_OK;
 }
 
+/* Stored G4 pair lists: every unordered pair of a row of up to cap_row neighbours fits the per-atom stride.  The
+ * allocation holds alloc_row-long rows (so the capacities set after a synchronised evaluation reuse it); returns
+ * the stride for use_row, or 0 when the lists would exceed the budget or cannot be allocated (the force kernel then
+ * enumerates the pairs itself). */
+static size_t pair_stride(int cap_row) { return cap_row < 2 ? 1 : (size_t)cap_row * (size_t)(cap_row - 1) / 2; }
+
+static size_t ensure_pairs(mg_ctx *c, int alloc_row, int use_row) {
+  if (!c->pair_store || c->N == 0) return 0;
+  const size_t need = (size_t)c->N * pair_stride(alloc_row > use_row ? alloc_row : use_row);
+  if (need * sizeof(unsigned) > c->pair_budget) return 0;
+  if (need > c->cap_gpl) {
+    if (c->d_gpl) hipFree(c->d_gpl);
+    c->d_gpl = NULL;
+    c->cap_gpl = 0;
+    if (hipMalloc((void **)&c->d_gpl, need * sizeof(unsigned)) != hipSuccess) {
+      (void)hipGetLastError();  /* out of memory here is not an error: the pairs are enumerated twice instead */
+      c->d_gpl = NULL;
+      return 0;
+    }
+    c->cap_gpl = need;
+  }
+  return pair_stride(use_row);
+}
+
 static int ensure_edges(mg_ctx *c, size_t M) {
   if (M <= c->cap_edges) return MG_OK;
   size_t n = M + M / 4 + 64;
@@ -116,6 +141,11 @@ int mg_ctx_create(const mg_model *m, int device, mg_ctx **out) {
   mg_ctx *c = (mg_ctx *)calloc(1, sizeof(mg_ctx));
   c->graph_enable = 1;
   c->graph_headroom = 0.125;
+  c->pair_store = 1;
+  {
+    size_t fr = 0, tot = 0;
+    c->pair_budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 16 : ((size_t)1 << 30);
+  }
   c->m = m;
   c->device = device;
   if (hipStreamCreate(&c->st) != hipSuccess) { free(c); return mg_fail(MG_ERR_HIP, "hipStreamCreate failed"); }
@@ -152,7 +182,7 @@ void mg_ctx_free(mg_ctx *c) {
   void *ptrs[] = {c->d_basis, c->d_mlp, c->d_str, c->d_atom_str, c->d_spc, c->d_xin, c->d_x, c->d_bin, c->d_bin_count,
                   c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
                   c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
-                  c->d_vir, c->d_fself, c->d_fe, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
+                  c->d_vir, c->d_fself, c->d_fe, c->d_gpl, c->d_npl, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
                   c->d_snap_dyn, c->d_cellv, c->d_snap_cellv, c->d_snap_str};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
@@ -377,6 +407,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
   c->N = N;
   c->have_forces = c->have_g = 0;
   c->gcap_edges = c->gcap_row = 0;  /* capacities are sized again by the next synchronised evaluation */
+  c->gpstride = 0;
   return MG_OK;
 }
 
@@ -418,6 +449,15 @@ int mgh_graph_caps(mg_ctx *c, int need_edges, int need_row) {
   if ((r = ensure_edges(c, ne))) return r;
   c->gcap_edges = (int)ne;
   c->gcap_row = need_row + (h > 0.0 ? 8 : 0);
+  c->gpstride = ensure_pairs(c, c->gcap_row, c->gcap_row);
+  return MG_OK;
+}
+
+int mg_ctx_set_pair_store(mg_ctx *c, int enable) {
+  if (!c) return mg_fail(MG_ERR_ARG, "mg_ctx_set_pair_store: null context");
+  c->pair_store = enable != 0;
+  c->gcap_edges = c->gcap_row = 0;  /* the next evaluation sizes the capacities (and the pair lists) again */
+  c->gpstride = 0;
   return MG_OK;
 }
 
@@ -445,6 +485,7 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
                      c->d_rowcnt, c->d_misc, st));
   KCHK(mgk_scan(c->d_rowcnt, c->d_row_off, N, c->d_tmp, &c->tmp_bytes, st));
   int cap_row;
+  size_t pstride;  /* stored pair lists (forces only): entries per atom, 0 = none */
   if (mode == MG_CHAIN_SYNC) {
     int hm[2];
     HIPCHK(hipMemcpyAsync(&hm[0], c->d_row_off + N, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -457,9 +498,11 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
      * reallocate (and drop) the edge buffers it has just filled */
     if ((r = ensure_edges(c, graph_edge_cap(c, c->nedges)))) return r;
     cap_row = c->maxrow;
+    pstride = forces ? ensure_pairs(c, cap_row + (c->graph_headroom > 0.0 ? 8 : 0), cap_row) : 0;
     KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->nedges, cap_row, st));
   } else {
     cap_row = c->gcap_row;
+    pstride = forces ? c->gpstride : 0;
     KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->gcap_edges, cap_row, st));
   }
   KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_atoms, c->d_row_off,
@@ -467,7 +510,7 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   if (timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
   /* descriptors, MLP, forces */
   KCHK(mgk_desc(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
-                urep, cap_row, c->wpb_desc, ovf, st));
+                urep, cap_row, c->wpb_desc, pstride ? c->d_gpl : NULL, c->d_npl, pstride, ovf, st));
   if (timing) HIPCHK(hipEventRecord(c->ev[EV_DESC], st));
   if (m->mfma_ok)
     KCHK(mgk_mlp_mfma(c->ntiles, c->d_tiles, c->d_perm, c->d_G, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, m->fdesc,
@@ -477,7 +520,7 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   if (timing) HIPCHK(hipEventRecord(c->ev[EV_MLP], st));
   if (forces) {
     KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
-                   c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, ovf, st));
+                   c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, pstride ? c->d_gpl : NULL, c->d_npl, pstride, ovf, st));
     if (timing) HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
     KCHK(mgk_gather(N, c->d_row_off, c->d_rev, c->d_fself, c->d_fe, c->d_F, ovf, st));
   } else if (timing) {

@@ -52,6 +52,9 @@ struct mg_ctx {
   int gcap_edges, gcap_row; /* graph-mode capacities: edges allocated, longest row the LDS tiles hold */
   int graph_enable;         /* relaxation / MD replay a captured iteration (default 1) */
   double graph_headroom;    /* fractional headroom of the graph-mode capacities (default 1/8) */
+  int pair_store;           /* the descriptor kernel stores its G4 pair lists for the force kernel (default 1) */
+  size_t pair_budget;       /* largest pair-list allocation in bytes (1/16 of the device memory) */
+  size_t gpstride;          /* pair-list entries per atom at the graph-mode row capacity (0: not stored) */
   hipEvent_t ev[EV_N];
   float tms[6];
   /* structures */
@@ -74,6 +77,9 @@ struct mg_ctx {
   int *d_rev;
   float *d_G, *d_dEdG, *d_Ei, *d_erep, *d_vir;
   void *d_fself, *d_fe;
+  unsigned *d_gpl;          /* stored G4 pair lists, pstride entries per atom (j | k << 12 | tag << 24) */
+  int *d_npl;               /* pairs per atom and species class [N][MG_NCLS] */
+  size_t cap_gpl;
   double *d_F, *d_E, *d_U, *d_part;
   float *d_frag, *d_fvec;
   void *d_frag_bf, *d_frag_lo;
@@ -114,8 +120,8 @@ int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, co
                  const int *bin_start, const int *bin_atoms, const int *row_off, void *key, void *vec, int *rev, int *err,
                  const int *ovf, int maxrow, hipStream_t st);
 int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
-             const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, const int *ovf,
-             hipStream_t st);
+             const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, unsigned *gpl, int *npl,
+             size_t pstride, const int *ovf, hipStream_t st);
 int mgk_mlp_mfma(int ntiles, const void *tiles, const int *perm, const float *G, const float *frag, const void *frag_bf,
                  const void *frag_lo, const float *vecs, const int *fdesc, int prec, float *Ei, float *dEdG,
                  hipStream_t st);
@@ -123,7 +129,7 @@ int mgk_mlp(int N, const int *spc, const float *G, const float *W, int wstride, 
             float *dEdG, hipStream_t st);
 int mgk_force(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
               const mg_basis_dev *bp, const float *dEdG, void *fself, void *fe, float *vir, float urep, int maxrow,
-              int wpb, const int *ovf, hipStream_t st);
+              int wpb, const unsigned *gpl, const int *npl, size_t pstride, const int *ovf, hipStream_t st);
 int mgk_gather(int N, const int *row_off, const int *rev, const void *fself, const void *fe, double *F, const int *ovf,
                hipStream_t st);
 int mgk_clear(int *a, int na, int *b, int nb, int *c, int nc, hipStream_t st);

@@ -19,6 +19,7 @@
 
 #define MG_MAXSPC 4
 #define MG_NPAIR (MG_MAXSPC * MG_MAXSPC)
+#define MG_NCLS (MG_MAXSPC * (MG_MAXSPC + 1) / 2) /* unordered species classes (sa <= sb) of G4 pairs */
 #define MG_MAXG2 16
 #define MG_MAXE 8        /* distinct G4 eta values */
 #define MG_MAXZ 4        /* distinct zeta values */

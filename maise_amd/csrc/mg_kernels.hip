@@ -533,24 +533,25 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
 //   two species: the longer side's slot f against the shorter side's slot (f mod l + sl) mod l, sl = 0..l-1.
 // tag (8 bits) numbers the groups of pairs in which no slot repeats on either side: a triangle diagonal, or a run of
 // l consecutive f on one rectangle diagonal.  The force kernel accumulates one group at a time without atomics.
-// Packing: j | k << 12 | tag << 24 (slots < 2048).
+// Packing: j | k << 12 | tag << 24 (slots < 2048).  With gout, the survivors are also written there in order (the
+// pair list of the class, which the force kernel reads instead of enumerating again); returns their number.
 template <int R, typename F>
-__device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bool tri, int oa, int ma, int ob, int mb,
-                                          float rc2, int lane, F&& body) {
+__device__ __forceinline__ int for_pairs(const float4* rec, unsigned* plist, bool tri, int oa, int ma, int ob, int mb,
+                                         float rc2, int lane, unsigned* __restrict__ gout, F&& body) {
   int mf, l, ntot;  // diagonal length, group length (shorter side), candidates
   if (tri) {
-    if (ma < 2) return;
+    if (ma < 2) return 0;
     mf = ma; l = ma;
     const int ms = ma >> 1;
     ntot = (ms - 1) * ma + ((ma & 1) ? ma : ms);  // diagonals 1..ms; an even m's last one holds f < m/2
   } else {
-    if (ma == 0 || mb == 0) return;
+    if (ma == 0 || mb == 0) return 0;
     mf = ma >= mb ? ma : mb; l = ma >= mb ? mb : ma;
     ntot = mf * l;
   }
   const int nblk = (mf + l - 1) / l;  // groups per rectangle diagonal
   const float rmf = 1.0f / (float)mf, rl = 1.0f / (float)l;
-  int cnt = 0;
+  int cnt = 0, tot = 0;
   for (int base = 0; base < ntot; base += WAVE) {
     const int t = base + lane;  // candidate t = diagonal * mf + f
     bool ok = false;
@@ -583,8 +584,12 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
     }
     const unsigned long long bm = __ballot(ok);
     const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
-    if (ok) plist[cnt + below] = pk;
+    if (ok) {
+      plist[cnt + below] = pk;
+      if (gout) gout[tot + below] = pk;
+    }
     cnt += __popcll(bm);
+    tot += __popcll(bm);
     const bool last = base + WAVE >= ntot;
     if (cnt > PCH - WAVE || last) {
       wave_sync();
@@ -597,6 +602,24 @@ __device__ __forceinline__ void for_pairs(const float4* rec, unsigned* plist, bo
       wave_sync();
       cnt = 0;
     }
+  }
+  return tot;
+}
+
+// The stored pair list of one class (np pairs from pl), handed to body in batches of 64 like for_pairs; the next
+// batch's words are loaded before the current batch runs.  The loads are unconditional (index clamped to the last
+// pair), so the only wait for one is where its batch starts.
+template <typename F>
+__device__ __forceinline__ void for_stored_pairs(const unsigned* __restrict__ pl, int np, int lane, F&& body) {
+  if (np <= 0) return;
+  const int last = np - 1;
+  unsigned pn = pl[lane < np ? lane : last];
+  for (int tb = 0; tb < np; tb += WAVE) {
+    const unsigned p = pn;
+    const bool valid = tb + lane < np;
+    const int tn = tb + WAVE + lane;
+    pn = pl[tn < np ? tn : last];
+    body((int)(p & 0xfffu), (int)((p >> 12) & 0xfffu), (int)(p >> 24), valid);
   }
 }
 
@@ -626,12 +649,13 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
                                                    const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                    const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
                                                    float* __restrict__ erep, float urep, int cap,
+                                                   unsigned* __restrict__ gpl, int* __restrict__ npl, size_t pstride,
                                                    const int* __restrict__ ovf) {
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   extern __shared__ float4 smem_d[];
   if (*ovf) return;  // rows may exceed the LDS tile of cap entries
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: per-atom data in SGPRs
   const int i = blockIdx.x * (blockDim.x >> 6) + wv;  // one wave per atom, 1-4 waves per block
   const int lane = threadIdx.x & 63;
   if (i >= N) return;
@@ -681,9 +705,12 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
     if (lane == 0) erep[i] = v;
   }
 
-  // ---- G4 per species class (sa <= sb)
+  // ---- G4 per species class (sa <= sb); with gpl, the surviving pairs of each class are stored for the force
+  // kernel (class after class from gpl + i * pstride; npl holds each class's count)
+  unsigned* gout = gpl ? gpl + (size_t)i * pstride : nullptr;
+  int cls = 0;
   for (int sa = 0; sa < S; sa++)
-    for (int sb = sa; sb < S; sb++) {
+    for (int sb = sa; sb < S; sb++, cls++) {
       const int ds1 = (sa - ri.si + S) % S, ds2 = (sb - ri.si + S) % S;
       const int blk = bp->ind4[ds1][ds2];
       const int pjk = sa * MG_MAXSPC + sb;
@@ -725,7 +752,11 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
           for (int q = 0; q < MG_MAXZ * MG_MAXL; q++)
             if (Pat<PAT>::mask(e) & (1u << q)) V[e * MG_MAXZ * MG_MAXL + q] += Bq[e] * P[q];
       };
-      for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
+      const int np = for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, gout, body);
+      if (gout) {
+        gout += np;
+        if (lane == 0) npl[(size_t)i * MG_NCLS + cls] = np;
+      }
       // lane l: grid cell l = (e*MAXZ + z)*MAXL + lambda; the zeta prefactor 2^(1 - zeta) goes on here
       float mine = reduce_scatter64(V, lane) * bp->pref[(lane / MG_MAXL) % MG_MAXZ];
       int cell = lane < bp->ng4 ? bp->g4cell[lane] : 0;
@@ -808,13 +839,14 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
                                                     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
                                                     float4* __restrict__ fself, float4* __restrict__ fe,
                                                     float* __restrict__ vir, float urep, int cap,
-                                                    const int* __restrict__ ovf) {
+                                                    const unsigned* __restrict__ gpl, const int* __restrict__ npl,
+                                                    size_t pstride, const int* __restrict__ ovf) {
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   constexpr int ZL = MG_MAXZ * MG_MAXL;
   extern __shared__ float4 smem_f[];
   if (*ovf) return;  // rows may exceed the LDS tile of cap entries
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: per-atom data in SGPRs
   const int i = blockIdx.x * (blockDim.x >> 6) + wv;  // one wave per atom, 1-4 waves per block
   const int lane = threadIdx.x & 63;
   if (i >= N) return;
@@ -870,9 +902,12 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
     }
   }
 
-  // ---- triplet terms (G4) per species class
+  // ---- triplet terms (G4) per species class; the pairs come from the descriptor kernel's stored list when there
+  // is one, else they are enumerated here as there
+  const unsigned* pl = gpl ? gpl + (size_t)i * pstride : nullptr;
+  int cls = 0;
   for (int sa = 0; sa < S; sa++)
-    for (int sb = sa; sb < S; sb++) {
+    for (int sb = sa; sb < S; sb++, cls++) {
       const int ds1 = (sa - ri.si + S) % S, ds2 = (sb - ri.si + S) % S;
       const int blk = bp->ind4[ds1][ds2];
       const int pjk = sa * MG_MAXSPC + sb, pij = ri.si * MG_MAXSPC + sa, pik = ri.si * MG_MAXSPC + sb;
@@ -951,7 +986,13 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
         float gbx = cbb * b.x + cb * a.x + c3c * cx, gby = cbb * b.y + cb * a.y + c3c * cy, gbz = cbb * b.z + cb * a.z + c3c * cz;
         group_accum(fnb4, fnk4, j, -gax, -gay, -gaz, k, -gbx, -gby, -gbz, tag, valid);
       };
-      for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, body);
+      if (pl) {
+        const int np = npl[(size_t)i * MG_NCLS + cls];
+        for_stored_pairs(pl, np, lane, body);
+        pl += np;
+      } else {
+        for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, nullptr, body);
+      }
       wave_sync();
     }
   // every term of T_i depends on the neighbour vectors d_e only, and fnb[e] = -dT_i/dd_e summed over them: so
@@ -1148,13 +1189,13 @@ static int pick_wpb(int wpb, size_t sm_wave) {
 
 // variant = G4 pattern (bits 0-3) | shared-eta flag (bit 4)
 int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
-             const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, int wpb, const int* ovf,
-             hipStream_t st) {
+             const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, int wpb, unsigned* gpl, int* npl,
+             size_t pstride, const int* ovf, hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * R * sizeof(float4) + PCH * sizeof(unsigned);
-  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap, ovf);
+  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap, gpl, npl, pstride, ovf);
   return mgk_status();
 }
 
@@ -1168,13 +1209,13 @@ int mgk_mlp(int N, const int* spc, const float* G, const float* W, int wstride, 
 
 int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
               const mg_basis_dev* bp, const float* dEdG, void* fself, void* fe, float* vir, float urep, int maxrow,
-              int wpb, const int* ovf, hipStream_t st) {
+              int wpb, const unsigned* gpl, const int* npl, size_t pstride, const int* ovf, hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * (R + 2) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
   DISPATCH_PU(k_force, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
-              urep, cap, ovf);
+              urep, cap, gpl, npl, pstride, ovf);
   return mgk_status();
 }
 
