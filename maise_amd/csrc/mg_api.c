@@ -45,7 +45,7 @@ static int ensure_atoms(mg_ctx *c, size_t N) {
   ALLOC(c->d_xin, n * 3 * sizeof(double));
   ALLOC(c->d_x, n * 3 * sizeof(double));
   ALLOC(c->d_bin, n * sizeof(int));
-  ALLOC(c->d_bin_atoms, n * sizeof(int));
+  ALLOC(c->d_bin_rec, n * 32);
   ALLOC(c->d_cnt, n * 4 * sizeof(int));
   ALLOC(c->d_npl, n * MG_NCLS * sizeof(int));
   ALLOC(c->d_rowcnt, n * sizeof(int));
@@ -180,7 +180,7 @@ void mg_ctx_free(mg_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   void *ptrs[] = {c->d_basis, c->d_mlp, c->d_str, c->d_atom_str, c->d_spc, c->d_xin, c->d_x, c->d_bin, c->d_bin_count,
-                  c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
+                  c->d_bin_start, c->d_bin_cur, c->d_bin_rec, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
                   c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
                   c->d_vir, c->d_fself, c->d_fe, c->d_gpl, c->d_npl, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
@@ -480,8 +480,8 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   KCHK(mgk_clear(c->d_bin_count, c->nbins + 1, c->d_bin_cur, c->nbins + 1, c->d_misc, mode == MG_CHAIN_GRAPH ? 1 : 16, st));
   KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, st));
   KCHK(mgk_scan(c->d_bin_count, c->d_bin_start, c->nbins, c->d_tmp, &c->tmp_bytes, st));
-  KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_cur, c->d_bin_atoms, st));
-  KCHK(mgk_nbr_count(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_atoms, c->d_cnt,
+  KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_cur, c->d_x, c->d_spc, c->d_bin_rec, st));
+  KCHK(mgk_nbr_count(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_cnt,
                      c->d_rowcnt, c->d_misc, st));
   KCHK(mgk_scan(c->d_rowcnt, c->d_row_off, N, c->d_tmp, &c->tmp_bytes, st));
   int cap_row;
@@ -505,7 +505,7 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     pstride = forces ? c->gpstride : 0;
     KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->gcap_edges, cap_row, st));
   }
-  KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_atoms, c->d_row_off,
+  KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
                     c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, ovf, cap_row, st));
   if (timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
   /* descriptors, MLP, forces */

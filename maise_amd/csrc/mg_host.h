@@ -69,7 +69,8 @@ struct mg_ctx {
   mg_struct_dev *d_str;
   int *d_atom_str, *d_spc;
   double *d_xin, *d_x;
-  int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur, *d_bin_atoms;
+  int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur;
+  void *d_bin_rec;          /* atoms in bin order: position, species, index (32 B each) */
   /* misc: [0] longest row of the evaluation, [1] error flag (reverse edge missing), [2] converged-structure count,
    * [3] capacity overflow flag, [4] largest edge total and [5] longest row seen since misc was last cleared */
   int *d_cnt, *d_rowcnt, *d_row_off, *d_misc;
@@ -113,11 +114,12 @@ int mg_dev_alloc(void **p, size_t bytes);
 int mgk_scan(const int *in, int *out, int n, void *tmp, size_t *tmp_bytes, hipStream_t st);
 int mgk_prepare(int N, const double *xin, const int *atom_str, const mg_struct_dev *str, double *x, int *bin,
                 int *bin_count, hipStream_t st);
-int mgk_bins(int N, int nbins, const int *bin, const int *bin_start, int *bin_cur, int *bin_atoms, hipStream_t st);
+int mgk_bins(int N, int nbins, const int *bin, const int *bin_start, int *bin_cur, const double *x, const int *spc,
+             void *brec, hipStream_t st);
 int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
-                  const int *bin_start, const int *bin_atoms, int *cnt, int *rowcnt, int *maxrow, hipStream_t st);
+                  const int *bin_start, const void *brec, int *cnt, int *rowcnt, int *maxrow, hipStream_t st);
 int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
-                 const int *bin_start, const int *bin_atoms, const int *row_off, void *key, void *vec, int *rev, int *err,
+                 const int *bin_start, const void *brec, const int *row_off, void *key, void *vec, int *rev, int *err,
                  const int *ovf, int maxrow, hipStream_t st);
 int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
              const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, unsigned *gpl, int *npl,

@@ -166,13 +166,23 @@ __global__ void k_prepare(int N, const double* __restrict__ xin, const int* __re
   atomicAdd(&bin_count[id], 1);
 }
 
+// Atoms in bin order, each with its wrapped position and species, so a neighbour candidate is one 32-byte load.
+struct __align__(16) BinRec {
+  double x, y, z;
+  int s, j;
+};
+
 __global__ void k_bin_fill(int N, const int* __restrict__ bin, const int* __restrict__ bin_start,
-                           int* __restrict__ bin_cur, int* __restrict__ bin_atoms) {
+                           int* __restrict__ bin_cur, const double* __restrict__ x, const int* __restrict__ spc,
+                           BinRec* __restrict__ brec) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int b = bin[i];
   int p = bin_start[b] + atomicAdd(&bin_cur[b], 1);
-  bin_atoms[p] = i;
+  BinRec r;
+  r.x = x[3 * i]; r.y = x[3 * i + 1]; r.z = x[3 * i + 2];
+  r.s = spc[i]; r.j = i;
+  brec[p] = r;
 }
 
 // ---------------------------------------------------------------------------------------------- neighbours
@@ -182,13 +192,13 @@ __global__ void k_bin_fill(int N, const int* __restrict__ bin, const int* __rest
 // range, and the lanes then test candidates c = lane, lane + 64, ... (entry found by binary search in LDS).
 // The FP64 distance uses LIST's expression order (src/cell.c:115-123), so the accepted set is bit-exact.
 // FILL: accepted entries are appended to an LDS row (ballot + lane prefix), the row is sorted by key
-// (species, j, image) with a bitonic sort and written once: canonical order, independent of the binning.
+// (species, j, image) with a rank sort and written once: canonical order, independent of the binning.
 #define NBR_CHUNK 64
 template <bool FILL>
 __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x, const int* __restrict__ spc,
                                             const int* __restrict__ atom_str, const mg_struct_dev* __restrict__ str,
                                             const int* __restrict__ bin, const int* __restrict__ bin_start,
-                                            const int* __restrict__ bin_atoms, int* __restrict__ cnt,
+                                            const BinRec* __restrict__ brec, int* __restrict__ cnt,
                                             int* __restrict__ rowcnt, int* __restrict__ maxrow,
                                             const int* __restrict__ row_off, u64* __restrict__ key,
                                             float4* __restrict__ vec, int cap, const int* __restrict__ ovf) {
@@ -200,7 +210,6 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
   extern __shared__ unsigned char smem_n[];
   u64* rk = (u64*)smem_n;             // [cap] keys of the row (FILL)
   float4* rv = (float4*)(rk + cap);   // [cap] vectors
-  int* rp = (int*)(rv + cap);         // [cap] permutation
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
   const mg_struct_dev* st = &str[atom_str[i]];
@@ -260,18 +269,19 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
           int mid = (lo + hi + 1) >> 1;
           if (e_pre[mid] <= c) lo = mid; else hi = mid - 1;
         }
-        j = bin_atoms[e_start[lo] + (c - e_pre[lo])];
+        const BinRec br = brec[e_start[lo] + (c - e_pre[lo])];
+        j = br.j;
         img = e_img[lo];
         if (!(j == i && img == img0)) {
-          a0 = e_sh[lo][0]; a0 += x[3 * j] - xi0;
-          a1 = e_sh[lo][1]; a1 += x[3 * j + 1] - xi1;
-          a2 = e_sh[lo][2]; a2 += x[3 * j + 2] - xi2;
+          a0 = e_sh[lo][0]; a0 += br.x - xi0;
+          a1 = e_sh[lo][1]; a1 += br.y - xi1;
+          a2 = e_sh[lo][2]; a2 += br.z - xi2;
           r = 0.0;
           r += a0 * a0;
           r += a1 * a1;
           r += a2 * a2;
           ok = r < RC;
-          if (ok) sj = spc[j];
+          sj = br.s;
         }
       }
       const u64 bm = __ballot(ok);
@@ -301,33 +311,22 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
     return;
   }
   const int M = n < cap ? n : cap;  // n <= cap by construction (cap >= longest row)
-  int n2 = 1;
-  while (n2 < M) n2 <<= 1;
-  for (int t = lane; t < n2; t += WAVE) {
-    if (t >= M) rk[t] = ~0ull;
-    rp[t] = t;
-  }
   __syncthreads();
-  for (int k = 2; k <= n2; k <<= 1) {
-    for (int jj = k >> 1; jj > 0; jj >>= 1) {
-      for (int t = lane; t < n2; t += WAVE) {
-        const int p = t ^ jj;
-        if (p > t) {
-          const bool asc = (t & k) == 0;
-          const u64 a = rk[t], b = rk[p];
-          if ((a > b) == asc) {
-            rk[t] = b; rk[p] = a;
-            const int ia = rp[t]; rp[t] = rp[p]; rp[p] = ia;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  // rank sort: the keys of a row are distinct, so the number of smaller keys is an entry's place in the sorted row;
+  // every lane counts against broadcast reads of the whole row and writes its entries to their places
   const int off = row_off[i];
-  for (int t = lane; t < M; t += WAVE) {
-    key[off + t] = rk[t];
-    vec[off + t] = rv[rp[t]];
+  for (int t0 = 0; t0 < M; t0 += 2 * WAVE) {
+    const int ta = t0 + lane, tb = t0 + WAVE + lane;
+    const u64 ka = ta < M ? rk[ta] : ~0ull, kb = tb < M ? rk[tb] : ~0ull;
+    int ra = 0, rb = 0;
+#pragma unroll 4
+    for (int u = 0; u < M; u++) {
+      const u64 ku = rk[u];
+      ra += ku < ka ? 1 : 0;
+      rb += ku < kb ? 1 : 0;
+    }
+    if (ta < M) { key[off + ra] = ka; vec[off + ra] = rv[ta]; }
+    if (tb < M) { key[off + rb] = kb; vec[off + rb] = rv[tb]; }
   }
 }
 
@@ -342,26 +341,60 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
-__global__ void k_rev(int N, const int* __restrict__ spc, const int* __restrict__ row_off, const u64* __restrict__ key,
-                      int* __restrict__ rev, int* __restrict__ err, const int* __restrict__ ovf) {
+// One wave per atom, four atoms per block; each lane looks up two edges at once (slots t and t + 64) with
+// branchless lower-bound searches whose loads are independent of each other, so a row of up to 128 neighbours
+// costs ceil(log2(row)) + 1 dependent loads.  Every load stays inside row j (an empty row searches the lane's own
+// edge instead, and the missing reverse edge is flagged).
+__device__ __forceinline__ int rev_lookup_base(const int* __restrict__ row_off, int j, int own, int& n) {
+  const int b = row_off[j];
+  n = row_off[j + 1] - b;
+  if (n > 0) return b;
+  n = 1;
+  return own;
+}
+
+__global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc, const int* __restrict__ row_off,
+                                             const u64* __restrict__ key, int* __restrict__ rev, int* __restrict__ err,
+                                             const int* __restrict__ ovf) {
   if (*ovf) return;
-  const int i = xcd_block(blockIdx.x, gridDim.x);
-  int off = row_off[i], M = row_off[i + 1] - off;
-  int si = spc[i];
-  for (int t = threadIdx.x; t < M; t += blockDim.x) {
-    int e = off + t;
-    u64 k = key[e];
-    int j = key_j(k);
-    u64 target = make_key(si, i, neg_img((unsigned)(k & 0x3fffffffull)));
-    int lo = row_off[j], hi = row_off[j + 1] - 1, found = -1;
-    while (lo <= hi) {
-      int mid = (lo + hi) >> 1;
-      u64 v = key[mid];
-      if (v == target) { found = mid; break; }
-      if (v < target) lo = mid + 1; else hi = mid - 1;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
+  if (i >= N) return;
+  const int lane = threadIdx.x & 63;
+  const int off = row_off[i], M = row_off[i + 1] - off;
+  const int si = spc[i];
+  for (int t0 = 0; t0 < M; t0 += 2 * WAVE) {
+    const int ta = t0 + lane, tb = t0 + WAVE + lane;
+    const bool va = ta < M, vb = tb < M;
+    const int ea = off + (va ? ta : 0), eb = off + (vb ? tb : 0);
+    const u64 ka = key[ea], kb = key[eb];
+    const u64 xa = make_key(si, i, neg_img((unsigned)(ka & 0x3fffffffull)));
+    const u64 xb = make_key(si, i, neg_img((unsigned)(kb & 0x3fffffffull)));
+    int na, nb;
+    const int ba = rev_lookup_base(row_off, key_j(ka), ea, na);
+    const int bb = rev_lookup_base(row_off, key_j(kb), eb, nb);
+    // lower bounds: pa = number of keys of row j below the target
+    int pa = 0, pb = 0;
+    const int nmax = na > nb ? na : nb;
+    for (int step = 1 << (31 - __clz(nmax)); step > 0; step >>= 1) {
+      const int qa = pa + step, qb = pb + step;
+      const u64 ya = key[ba + (qa < na ? qa : na) - 1];
+      const u64 yb = key[bb + (qb < nb ? qb : nb) - 1];
+      if (qa <= na && ya < xa) pa = qa;
+      if (qb <= nb && yb < xb) pb = qb;
     }
-    if (found < 0) { atomicOr(err, 1); found = e; }
-    rev[e] = found;
+    const u64 za = key[ba + (pa < na ? pa : na - 1)];
+    const u64 zb = key[bb + (pb < nb ? pb : nb - 1)];
+    if (va) {
+      int f = ba + pa;
+      if (pa >= na || za != xa) { atomicOr(err, 1); f = ea; }
+      rev[ea] = f;
+    }
+    if (vb) {
+      int f = bb + pb;
+      if (pb >= nb || zb != xb) { atomicOr(err, 1); f = eb; }
+      rev[eb] = f;
+    }
   }
 }
 
@@ -1128,16 +1161,19 @@ int mgk_prepare(int N, const double* xin, const int* atom_str, const mg_struct_d
   return mgk_status();
 }
 
-int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_cur, int* bin_atoms, hipStream_t st) {
+int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_cur, const double* x, const int* spc,
+             void* brec, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_bin_fill, dim3((N + 255) / 256), dim3(256), 0, st, N, bin, bin_start, bin_cur, bin_atoms);
+  hipLaunchKernelGGL(k_bin_fill, dim3((N + 255) / 256), dim3(256), 0, st, N, bin, bin_start, bin_cur, x, spc,
+                     (BinRec*)brec);
   return mgk_status();
 }
 
 int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
-                  const int* bin_start, const int* bin_atoms, int* cnt, int* rowcnt, int* maxrow, hipStream_t st) {
+                  const int* bin_start, const void* brec, int* cnt, int* rowcnt, int* maxrow, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, st, N, x, spc, atom_str, str, bin, bin_start, bin_atoms, cnt,
+  hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, st, N, x, spc, atom_str, str, bin, bin_start,
+                     (const BinRec*)brec, cnt,
                      rowcnt, maxrow, (const int*)nullptr, (u64*)nullptr, (float4*)nullptr, 0, (const int*)nullptr);
   return mgk_status();
 }
@@ -1145,16 +1181,16 @@ int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, c
 // Fill and sort the rows (LDS tile of cap entries: a power of two >= the longest row) and build the reverse-edge
 // index.  Rows longer than 2048 neighbours are rejected (-2).
 int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
-                 const int* bin_start, const int* bin_atoms, const int* row_off, void* key, void* vec, int* rev,
+                 const int* bin_start, const void* brec, const int* row_off, void* key, void* vec, int* rev,
                  int* err, const int* ovf, int maxrow, hipStream_t st) {
   if (N == 0) return 0;
   int cap = 64;
   while (cap < maxrow) cap <<= 1;
   if (cap > 2048) return -2;
-  size_t sm = (size_t)cap * (sizeof(u64) + sizeof(float4) + sizeof(int));
-  hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), sm, st, N, x, spc, atom_str, str, bin, bin_start, bin_atoms,
+  size_t sm = (size_t)cap * (sizeof(u64) + sizeof(float4));
+  hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), sm, st, N, x, spc, atom_str, str, bin, bin_start, (const BinRec*)brec,
                      (int*)nullptr, (int*)nullptr, (int*)nullptr, row_off, (u64*)key, (float4*)vec, cap, ovf);
-  hipLaunchKernelGGL(k_rev, dim3(N), dim3(64), 0, st, N, spc, row_off, (const u64*)key, rev, err, ovf);
+  hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, row_off, (const u64*)key, rev, err, ovf);
   return mgk_status();
 }
 
