@@ -68,7 +68,11 @@ int mg_set_structure(mg_ctx *c, int N, int nd, int xt, const int *spc, const dou
 int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const double *lat, const int *spc,
                  const double *x);
 
-/* Evaluate on the device.  All inputs already resident; results stay on the device until mg_get*. */
+/* Evaluate on the device.  All inputs already resident; results stay on the device until mg_get*.  Once the
+ * capacities are known (after the first evaluation of a structure or batch) the call only enqueues the work on the
+ * context's stream and returns; the next call that reads results (mg_get*, mg_features, dynamics) waits for it,
+ * checks it, and evaluates again with larger buffers if the neighbour list outgrew them, so back-to-back
+ * evaluations run without host round trips.  Errors of an enqueued evaluation are returned by that next call. */
 int mg_eval(mg_ctx *c, int flags);
 /* Copy results of structure s (0 for a single structure): E (eV), EA[natoms], F[3*natoms] (eV/A),
  * U[6] (eV, xx yy zz xy yz zx, U = -dE/dstrain), X[3*natoms] (wrapped positions).  NULL skips a field. */
@@ -81,11 +85,11 @@ int mg_get_batch(mg_ctx *c, double *E, double *U, double *F);
 int mg_get_neighbors(mg_ctx *c, int *nedges, int *row_off, int *j, int *img);
 /* Descriptors G[N][D] of the last mg_eval with MG_EVAL_KEEP_G. */
 int mg_get_descriptors(mg_ctx *c, float *G);
-/* Timing of the last mg_eval per stage in milliseconds (device events on the context stream):
- * t[0] neighbour build, t[1] descriptor kernel, t[2] MLP, t[3] force/virial kernel, t[4] force gather + FP64
- * reductions, t[5] total.  t must hold 6 floats. */
+/* Stage times in milliseconds (device events on the context stream), averaged over the evaluations since timing
+ * was enabled or last read (reading resets the average): t[0] neighbour build, t[1] descriptor kernel, t[2] MLP,
+ * t[3] force/virial kernel, t[4] force gather + FP64 reductions, t[5] total.  t must hold 6 floats. */
 int mg_get_timing(mg_ctx *c, float *t);
-/* Enable (1) / disable (0) per-stage event timing (small overhead). */
+/* Enable (1) / disable (0) per-stage event timing (small overhead; evaluations stay asynchronous). */
 int mg_ctx_set_timing(mg_ctx *c, int on);
 /* Launch shape of the wave-per-atom descriptor and force kernels: atoms (64-lane waves) per workgroup, 1, 2 or 4;
  * 0 (the default) picks the largest whose LDS tile fits 64 KiB.  No counterpart in the reference (CPU only). */

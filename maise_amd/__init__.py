@@ -269,6 +269,7 @@ class Context:
         return G.reshape(self.N, self.model.D)
 
     def timing(self) -> np.ndarray:
+        """Stage times (ms), averaged over the evaluations since timing was enabled or last read."""
         t = np.zeros(6, dtype=np.float32)
         _check(lib().mg_get_timing(self._h, t.ctypes.data_as(C.POINTER(C.c_float))), "mg_get_timing")
         return t

@@ -149,7 +149,11 @@ int mg_ctx_create(const mg_model *m, int device, mg_ctx **out) {
   c->m = m;
   c->device = device;
   if (hipStreamCreate(&c->st) != hipSuccess) { free(c); return mg_fail(MG_ERR_HIP, "hipStreamCreate failed"); }
-  for (int k = 0; k < EV_N; k++) hipEventCreate(&c->ev[k]);
+  if (hipHostMalloc((void **)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    hipStreamDestroy(c->st);
+    free(c);
+    return mg_fail(MG_ERR_HIP, "hipHostMalloc failed");
+  }
   int r;
   if ((r = mg_dev_alloc((void **)&c->d_basis, sizeof(mg_basis_dev))) ||
       (r = mg_dev_alloc((void **)&c->d_mlp, (size_t)m->wstride * m->nspc * sizeof(float))) ||
@@ -187,7 +191,11 @@ void mg_ctx_free(mg_ctx *c) {
                   c->d_snap_dyn, c->d_cellv, c->d_snap_cellv, c->d_snap_str};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
     if (ptrs[k]) hipFree(ptrs[k]);
-  for (int k = 0; k < EV_N; k++) hipEventDestroy(c->ev[k]);
+  if (c->st) hipStreamSynchronize(c->st);
+  if (c->tev_ok)
+    for (int q = 0; q < MG_TRING; q++)
+      for (int k = 0; k < EV_N; k++) hipEventDestroy(c->tev[q][k]);
+  if (c->h_flags) hipHostFree(c->h_flags);
   if (c->st) hipStreamDestroy(c->st);
   free(c->h_str); free(c->h_atom_off); free(c->h_nd); free(c->h_lat);
   free(c);
@@ -207,7 +215,17 @@ int mg_ctx_set_precision(mg_ctx *c, int prec) {
 
 int mg_ctx_set_timing(mg_ctx *c, int on) {
   if (!c) return mg_fail(MG_ERR_ARG, "null context");
-  c->timing = on;
+  int r;
+  if ((r = mgh_settle(c)) != MG_OK) return r;
+  HIPCHK(hipSetDevice(c->device));
+  if (on && !c->tev_ok) {
+    for (int q = 0; q < MG_TRING; q++)
+      for (int k = 0; k < EV_N; k++) HIPCHK(hipEventCreate(&c->tev[q][k]));
+    c->tev_ok = 1;
+  }
+  c->timing = on != 0;
+  c->tcount = c->tread = c->tn = 0;
+  for (int k = 0; k < 6; k++) c->tsum[k] = 0.0;
   return MG_OK;
 }
 
@@ -229,24 +247,22 @@ int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out) {
   if (reps < 1) reps = 1;
   const int cand[3] = {1, 2, 4};
   const int timing0 = c->timing, wd0 = c->wpb_desc, wf0 = c->wpb_force;
-  float best_d = 1e30f, best_f = 1e30f;
-  int wd = 0, wf = 0;
-  c->timing = 1;
+  float best_d = 1e30f, best_f = 1e30f, t[6];
+  int wd = 0, wf = 0, r;
+  if ((r = mg_ctx_set_timing(c, 1)) != MG_OK) return r;
   for (int k = 0; k < 3; k++) {
     c->wpb_desc = c->wpb_force = cand[k];
-    if (mg_eval(c, MG_EVAL_FORCES) != MG_OK) continue;  /* LDS does not fit: skip this shape */
-    float td = 0.f, tf = 0.f;
+    if (mgh_eval(c, MG_EVAL_FORCES, 0) != MG_OK) continue;  /* LDS does not fit: skip this shape */
+    if ((r = mg_get_timing(c, t)) != MG_OK) return r;       /* drop the warm-up evaluation */
     int ok = 1;
-    for (int r = 0; r < reps && ok; r++) {
-      if (mg_eval(c, MG_EVAL_FORCES) != MG_OK) ok = 0;
-      td += c->tms[1];
-      tf += c->tms[3];
-    }
+    for (int q = 0; q < reps && ok; q++)
+      if (mgh_eval(c, MG_EVAL_FORCES, 0) != MG_OK) ok = 0;
     if (!ok) continue;
-    if (td < best_d) { best_d = td; wd = cand[k]; }
-    if (tf < best_f) { best_f = tf; wf = cand[k]; }
+    if ((r = mg_get_timing(c, t)) != MG_OK) return r;       /* stage means over the reps */
+    if (t[1] < best_d) { best_d = t[1]; wd = cand[k]; }
+    if (t[3] < best_f) { best_f = t[3]; wf = cand[k]; }
   }
-  c->timing = timing0;
+  if ((r = mg_ctx_set_timing(c, timing0)) != MG_OK) return r;
   if (!wd || !wf) {
     c->wpb_desc = wd0;
     c->wpb_force = wf0;
@@ -315,6 +331,10 @@ static void plan_bins(mg_struct_dev *s, double rc, int natom) {
 int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *xt, const double *lat, const int *spc,
                const double *x) {
   HIPCHK(hipSetDevice(c->device));
+  if (c->pending) {  /* the results of an evaluation still running are dropped with the old structures */
+    c->pending = 0;
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
   int N = atom_off[S];
   for (int i = 0; i < N; i++)
     if (spc[i] < 0 || spc[i] >= c->m->nspc) return mg_fail(MG_ERR_ARG, "atom %d: species %d outside the model's %d", i, spc[i], c->m->nspc);
@@ -468,14 +488,60 @@ int mg_ctx_set_graph(mg_ctx *c, int enable, double headroom) {
   return MG_OK;
 }
 
+/* flags of a finished chain (h_flags = misc[0..7]) */
+static int chain_flags(mg_ctx *c, int mode) {
+  const int *f = c->h_flags;
+  if (f[3]) {
+    if (mode != MG_CHAIN_SYNC) return MG_ERR_NEIGHBOR_CAP;  /* outgrew the capacities: repeat in sync mode */
+    return mg_fail(MG_ERR_HIP, "edge capacity check failed in a synchronised evaluation");
+  }
+  if (f[1]) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)");
+  c->nedges = f[6];
+  c->maxrow = f[0];
+  return MG_OK;
+}
+
+/* read the stage events of the oldest unread timed evaluation into the sums */
+static int timing_read_one(mg_ctx *c) {
+  hipEvent_t *e = c->tev[c->tread % MG_TRING];
+  HIPCHK(hipEventSynchronize(e[EV_END]));
+  float t[6];
+  HIPCHK(hipEventElapsedTime(&t[0], e[EV_START], e[EV_NBR]));
+  HIPCHK(hipEventElapsedTime(&t[1], e[EV_NBR], e[EV_DESC]));
+  HIPCHK(hipEventElapsedTime(&t[2], e[EV_DESC], e[EV_MLP]));
+  HIPCHK(hipEventElapsedTime(&t[3], e[EV_MLP], e[EV_FORCE]));
+  HIPCHK(hipEventElapsedTime(&t[4], e[EV_FORCE], e[EV_END]));
+  HIPCHK(hipEventElapsedTime(&t[5], e[EV_START], e[EV_END]));
+  for (int k = 0; k < 6; k++) c->tsum[k] += t[k];
+  c->tn++;
+  c->tread++;
+  return MG_OK;
+}
+
+/* events for the next timed evaluation; the oldest outstanding one is read first when the ring is full */
+static int timing_slot(mg_ctx *c, hipEvent_t **ev) {
+  if (c->tcount - c->tread >= MG_TRING) {
+    int r;
+    if ((r = timing_read_one(c)) != MG_OK) return r;
+  }
+  *ev = c->tev[c->tcount % MG_TRING];
+  c->tcount++;
+  return MG_OK;
+}
+
 int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   const mg_model *m = c->m;
   const int N = c->N;
   hipStream_t st = c->st;
   const float urep = (float)c->urep;
-  const int timing = c->timing && mode != MG_CHAIN_GRAPH;
+  const int timing = c->timing && c->tev_ok && mode != MG_CHAIN_GRAPH;
   const int *ovf = c->d_misc + 3;
-  if (timing) HIPCHK(hipEventRecord(c->ev[EV_START], st));
+  hipEvent_t *ev = NULL;
+  if (timing) {
+    int r;
+    if ((r = timing_slot(c, &ev)) != MG_OK) return r;
+    HIPCHK(hipEventRecord(ev[EV_START], st));
+  }
   /* neighbour list: bins, row counts, row offsets; graph mode keeps the flags of misc[1..5] for the caller */
   KCHK(mgk_clear(c->d_bin_count, c->nbins + 1, c->d_bin_cur, c->nbins + 1, c->d_misc, mode == MG_CHAIN_GRAPH ? 1 : 16, st));
   KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, st));
@@ -507,61 +573,60 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   }
   KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
                     c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, ovf, cap_row, st));
-  if (timing) HIPCHK(hipEventRecord(c->ev[EV_NBR], st));
+  if (timing) HIPCHK(hipEventRecord(ev[EV_NBR], st));
   /* descriptors, MLP, forces */
   KCHK(mgk_desc(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
                 urep, cap_row, c->wpb_desc, pstride ? c->d_gpl : NULL, c->d_npl, pstride, ovf, st));
-  if (timing) HIPCHK(hipEventRecord(c->ev[EV_DESC], st));
+  if (timing) HIPCHK(hipEventRecord(ev[EV_DESC], st));
   if (m->mfma_ok)
     KCHK(mgk_mlp_mfma(c->ntiles, c->d_tiles, c->d_perm, c->d_G, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, m->fdesc,
                       c->prec, c->d_Ei, c->d_dEdG, st));
   else
     KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
-  if (timing) HIPCHK(hipEventRecord(c->ev[EV_MLP], st));
+  if (timing) HIPCHK(hipEventRecord(ev[EV_MLP], st));
   if (forces) {
     KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
                    c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, pstride ? c->d_gpl : NULL, c->d_npl, pstride, ovf, st));
-    if (timing) HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
+    if (timing) HIPCHK(hipEventRecord(ev[EV_FORCE], st));
     KCHK(mgk_gather(N, c->d_row_off, c->d_rev, c->d_fself, c->d_fe, c->d_F, ovf, st));
   } else if (timing) {
-    HIPCHK(hipEventRecord(c->ev[EV_FORCE], st));
+    HIPCHK(hipEventRecord(ev[EV_FORCE], st));
   }
   KCHK(mgk_reduce(c->S, c->nchunk, c->d_chunk_str, c->d_str, c->d_Ei, c->d_erep, c->d_vir, forces, c->d_part, c->d_E,
                   c->d_U, st));
-  if (timing) HIPCHK(hipEventRecord(c->ev[EV_END], st));
+  if (timing) HIPCHK(hipEventRecord(ev[EV_END], st));
   if (mode == MG_CHAIN_GRAPH) return MG_OK;
-  int flags[5];
-  HIPCHK(hipMemcpyAsync(flags, c->d_misc, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&flags[4], c->d_row_off + N, sizeof(int), hipMemcpyDeviceToHost, st));
+  /* misc[0..7] (row maximum, error, overflow, edge total) to pinned host memory; ASYNC leaves the check to
+   * mgh_settle */
+  HIPCHK(hipMemcpyAsync(c->h_flags, c->d_misc, 8 * sizeof(int), hipMemcpyDeviceToHost, st));
+  if (mode == MG_CHAIN_ASYNC) {
+    c->pending = 1;
+    c->pending_forces = forces;
+    return MG_OK;
+  }
   HIPCHK(hipStreamSynchronize(st));
-  if (flags[3]) {
-    if (mode == MG_CHAIN_CAPS) return MG_ERR_NEIGHBOR_CAP;  /* outgrew the capacities: the caller repeats in sync mode */
-    return mg_fail(MG_ERR_HIP, "edge capacity check failed in a synchronised evaluation");
-  }
-  if (flags[1]) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)");
-  c->nedges = flags[4];
-  c->maxrow = flags[0];
-  if (timing) {
-    hipEventElapsedTime(&c->tms[0], c->ev[EV_START], c->ev[EV_NBR]);
-    hipEventElapsedTime(&c->tms[1], c->ev[EV_NBR], c->ev[EV_DESC]);
-    hipEventElapsedTime(&c->tms[2], c->ev[EV_DESC], c->ev[EV_MLP]);
-    hipEventElapsedTime(&c->tms[3], c->ev[EV_MLP], c->ev[EV_FORCE]);
-    hipEventElapsedTime(&c->tms[4], c->ev[EV_FORCE], c->ev[EV_END]);
-    hipEventElapsedTime(&c->tms[5], c->ev[EV_START], c->ev[EV_END]);
-  }
-  return MG_OK;
+  return chain_flags(c, mode);
 }
 
-int mg_eval(mg_ctx *c, int flags) {
+int mgh_eval(mg_ctx *c, int flags, int async) {
   if (!c) return mg_fail(MG_ERR_ARG, "null context");
   if (c->S == 0) return mg_fail(MG_ERR_ARG, "no structure set");
   HIPCHK(hipSetDevice(c->device));
   if (c->prec != MG_PREC_FP32 && !c->m->mfma_ok) return mg_fail(MG_ERR_UNSUPPORTED, "BF16 MLP needs hidden layers <= 64 wide");
   const int forces = (flags & MG_EVAL_FORCES) != 0;
   /* repeated evaluations of the same atoms reuse the capacities of the last one (no host round trip in the middle
-   * of the chain); a step that outgrows them is detected on the device and evaluated again synchronised */
+   * of the chain); a step that outgrows them is detected on the device and evaluated again synchronised.  An
+   * asynchronous evaluation supersedes a pending one: only the last evaluation's results can be read. */
   int r = MG_ERR_NEIGHBOR_CAP;
-  if (c->graph_enable && c->gcap_edges > 0) r = mgh_eval_chain(c, forces, MG_CHAIN_CAPS);
+  c->pending = 0;
+  if (c->graph_enable && c->gcap_edges > 0) {
+    r = mgh_eval_chain(c, forces, async ? MG_CHAIN_ASYNC : MG_CHAIN_CAPS);
+    if (async && r == MG_OK) {
+      c->have_forces = forces;
+      c->have_g = 1;
+      return MG_OK;
+    }
+  }
   if (r == MG_ERR_NEIGHBOR_CAP) {
     if ((r = mgh_eval_chain(c, forces, MG_CHAIN_SYNC)) != MG_OK) return r;
     if ((r = mgh_graph_caps(c, c->nedges, c->maxrow)) != MG_OK) return r;
@@ -573,10 +638,26 @@ int mg_eval(mg_ctx *c, int flags) {
   return MG_OK;
 }
 
+int mgh_settle(mg_ctx *c) {
+  if (!c || !c->pending) return MG_OK;
+  c->pending = 0;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->st));
+  int r = chain_flags(c, MG_CHAIN_CAPS);
+  if (r == MG_ERR_NEIGHBOR_CAP) {
+    if ((r = mgh_eval_chain(c, c->pending_forces, MG_CHAIN_SYNC)) != MG_OK) return r;
+    r = mgh_graph_caps(c, c->nedges, c->maxrow);
+  }
+  return r;
+}
+
+int mg_eval(mg_ctx *c, int flags) { return mgh_eval(c, flags, 1); }
+
 int mg_get(mg_ctx *c, int s, double *E, double *EA, double *F, double *U, double *X) {
   if (!c || s < 0 || s >= c->S) return mg_fail(MG_ERR_ARG, "mg_get: bad structure index");
   if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_get: the structure has not been evaluated yet");
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   const int a0 = c->h_atom_off[s], n = c->h_atom_off[s + 1] - a0;
   if (E) HIPCHK(hipMemcpy(E, c->d_E + s, sizeof(double), hipMemcpyDeviceToHost));
   if (U) {
@@ -602,6 +683,7 @@ int mg_get_batch(mg_ctx *c, double *E, double *U, double *F) {
   if (!c) return mg_fail(MG_ERR_ARG, "null context");
   if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_get_batch: the batch has not been evaluated yet");
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   if (E) HIPCHK(hipMemcpy(E, c->d_E, c->S * sizeof(double), hipMemcpyDeviceToHost));
   if ((U || F) && !c->have_forces) return mg_fail(MG_ERR_ARG, "mg_get_batch: last eval had no forces");
   if (U) HIPCHK(hipMemcpy(U, c->d_U, 6 * c->S * sizeof(double), hipMemcpyDeviceToHost));
@@ -612,6 +694,7 @@ int mg_get_batch(mg_ctx *c, double *E, double *U, double *F) {
 int mg_get_neighbors(mg_ctx *c, int *nedges, int *row_off, int *j, int *img) {
   if (!c || !nedges) return mg_fail(MG_ERR_ARG, "mg_get_neighbors: bad argument");
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   *nedges = c->nedges;
   if (row_off) HIPCHK(hipMemcpy(row_off, c->d_row_off, (c->N + 1) * sizeof(int), hipMemcpyDeviceToHost));
   if (j || img) {
@@ -636,6 +719,7 @@ int mg_get_descriptors(mg_ctx *c, float *G) {
   if (!c || !G) return mg_fail(MG_ERR_ARG, "mg_get_descriptors: bad argument");
   if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_get_descriptors: no evaluation yet");
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   HIPCHK(hipMemcpy(G, c->d_G, sizeof(float) * (size_t)c->N * c->m->D, hipMemcpyDeviceToHost));
   return MG_OK;
 }
@@ -648,7 +732,13 @@ int mg_get_lattice(mg_ctx *c, int s, double *lat) {
 
 int mg_get_timing(mg_ctx *c, float *t) {
   if (!c || !t) return mg_fail(MG_ERR_ARG, "mg_get_timing: bad argument");
-  for (int k = 0; k < 6; k++) t[k] = c->tms[k];
+  int r;
+  if ((r = mgh_settle(c)) != MG_OK) return r;
+  while (c->tread < c->tcount)
+    if ((r = timing_read_one(c)) != MG_OK) return r;
+  for (int k = 0; k < 6; k++) t[k] = c->tn ? (float)(c->tsum[k] / c->tn) : 0.0f;
+  for (int k = 0; k < 6; k++) c->tsum[k] = 0.0;
+  c->tn = 0;
   return MG_OK;
 }
 

@@ -83,6 +83,7 @@ static int ensure_dyn(mg_ctx *c) {
 int mg_md_init(mg_ctx *c, double T, unsigned long long seed) {
   if (!c || c->S != 1) return mg_fail(MG_ERR_ARG, "mg_md_init: needs a context holding one structure");
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   int r;
   if ((r = ensure_dyn(c))) return r;
   const double kT = MG_KB * T;
@@ -96,6 +97,7 @@ int mg_md_init(mg_ctx *c, double T, unsigned long long seed) {
 int mg_md_set_velocities(mg_ctx *c, const double *v) {
   if (!c || !v) return mg_fail(MG_ERR_ARG, "mg_md_set_velocities: bad argument");
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   int r;
   if ((r = ensure_dyn(c))) return r;
   HIPCHK(hipMemcpy(c->d_v, v, sizeof(double) * 3 * c->N, hipMemcpyHostToDevice));
@@ -106,6 +108,7 @@ int mg_md_set_velocities(mg_ctx *c, const double *v) {
 int mg_md_get_velocities(mg_ctx *c, double *v) {
   if (!c || !v || !c->d_v) return mg_fail(MG_ERR_ARG, "mg_md_get_velocities: no velocities");
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   HIPCHK(hipMemcpy(v, c->d_v, sizeof(double) * 3 * c->N, hipMemcpyDeviceToHost));
   return MG_OK;
 }
@@ -114,6 +117,7 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
   if (!c || c->S != 1 || nsteps < 0 || dt <= 0.0) return mg_fail(MG_ERR_ARG, "mg_md_run: bad argument");
   if (!c->d_v) return mg_fail(MG_ERR_ARG, "mg_md_run: call mg_md_init or mg_md_set_velocities first");
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   int r;
   if ((size_t)(2 * nsteps + 2) > c->cap_trace) {
     if ((r = mg_dev_alloc((void **)&c->d_trace, sizeof(double) * (2 * nsteps + 2)))) return r;
@@ -121,7 +125,7 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
   }
   const double kT = MG_KB * T, ndof = 3.0 * c->N;
   /* forces at the starting positions */
-  if ((r = mg_eval(c, MG_EVAL_FORCES))) return r;
+  if ((r = mgh_eval(c, MG_EVAL_FORCES, 0))) return r;
   for (int s = 0; s < nsteps; s++) {
     KCHK(mgk_md_thermo(c->N, c->d_mass, c->d_v, c->d_dyn, dt, kT, ndof, tau, NULL, -1, c->st));
     KCHK(mgk_md_kick_drift(c->N, dt, c->d_mass, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_dyn, c->st));
@@ -130,7 +134,7 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
       c->h_str[0].xt = 1;
       HIPCHK(hipMemcpyAsync(c->d_str, c->h_str, sizeof(mg_struct_dev), hipMemcpyHostToDevice, c->st));
     }
-    if ((r = mg_eval(c, MG_EVAL_FORCES))) return r;
+    if ((r = mgh_eval(c, MG_EVAL_FORCES, 0))) return r;
     KCHK(mgk_md_kick_scale(c->N, dt, c->d_mass, c->d_F, c->d_v, c->d_dyn, tau, c->st));
     KCHK(mgk_md_thermo(c->N, c->d_mass, c->d_v, c->d_dyn, dt, kT, ndof, tau, c->d_trace + nsteps, s, c->st));
     KCHK(mgk_copy_d(c->d_E, c->d_trace + s, 1, c->st));
@@ -289,6 +293,7 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
     return mg_fail(MG_ERR_ARG, "mg_relax: bad argument");
   HIPCHK(hipSetDevice(c->device));
   int r;
+  if ((r = mgh_settle(c)) != MG_OK) return r;
   if ((r = ensure_dyn(c))) return r;
   relax_book b;
   memset(&b, 0, sizeof b);
@@ -328,7 +333,7 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
       /* synchronised iterations: all of them without graphs, else one to size the buffers before a capture */
       nrun = c->graph_enable ? 1 : (max_steps - it < MG_GRAPH_CHECK ? max_steps - it : MG_GRAPH_CHECK);
       for (int k = 0; k < nrun; k++) {
-        RCHK(mg_eval(c, MG_EVAL_FORCES));
+        RCHK(mgh_eval(c, MG_EVAL_FORCES, 0));
         RCHK(fire_tail(c, &fa));
         if (it + k == 0) {
           int any_frac = 0;
@@ -402,7 +407,7 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
     RHIP(hipMemcpy(c->d_fire, b.st0, sizeof(double) * 5 * b.S0, hipMemcpyHostToDevice));
   }
   /* energies and forces at the final positions */
-  RCHK(mg_eval(c, MG_EVAL_FORCES));
+  RCHK(mgh_eval(c, MG_EVAL_FORCES, 0));
   if (steps || done) {
     double *state = (double *)malloc(sizeof(double) * 5 * c->S);
     hipError_t e = hipMemcpy(state, c->d_fire, sizeof(double) * 5 * c->S, hipMemcpyDeviceToHost);

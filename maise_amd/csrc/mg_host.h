@@ -35,6 +35,7 @@ struct mg_model {
   } while (0)
 
 enum { EV_START, EV_NBR, EV_DESC, EV_MLP, EV_FORCE, EV_END, EV_N };
+#define MG_TRING 32  /* evaluations whose stage events can be outstanding before the oldest is read */
 
 struct mg_ctx;
 /* (Re)load S structures into the context: per-structure tables, bins, species tiles, positions (xt 0 = fractional) */
@@ -55,8 +56,14 @@ struct mg_ctx {
   int pair_store;           /* the descriptor kernel stores its G4 pair lists for the force kernel (default 1) */
   size_t pair_budget;       /* largest pair-list allocation in bytes (1/16 of the device memory) */
   size_t gpstride;          /* pair-list entries per atom at the graph-mode row capacity (0: not stored) */
-  hipEvent_t ev[EV_N];
-  float tms[6];
+  hipEvent_t tev[MG_TRING][EV_N]; /* stage events of the last MG_TRING timed evaluations (ring) */
+  int tev_ok, tcount, tread;      /* events created; evaluations recorded; evaluations read */
+  double tsum[6];                 /* stage milliseconds summed over the evaluations read */
+  int tn;
+  /* asynchronous evaluation: the flags of the last enqueued chain land in pinned h_flags; the first call that
+   * reads results checks them (and evaluates again synchronised if the capacities overflowed) */
+  int pending, pending_forces;
+  int *h_flags;
   /* structures */
   int S, N;
   int *h_atom_off, *h_nd;
@@ -72,7 +79,8 @@ struct mg_ctx {
   int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur;
   void *d_bin_rec;          /* atoms in bin order: position, species, index (32 B each) */
   /* misc: [0] longest row of the evaluation, [1] error flag (reverse edge missing), [2] converged-structure count,
-   * [3] capacity overflow flag, [4] largest edge total and [5] longest row seen since misc was last cleared */
+   * [3] capacity overflow flag, [4] largest edge total and [5] longest row seen since misc was last cleared,
+   * [6] edge total of the evaluation */
   int *d_cnt, *d_rowcnt, *d_row_off, *d_misc;
   void *d_key, *d_vec;
   int *d_rev;
@@ -141,8 +149,15 @@ int mgk_check_caps(int N, const int *row_off, int *misc, int cap_edges, int cap_
  * synchronisation in the middle, one at the end); MG_CHAIN_GRAPH uses the capacities gcap_edges / gcap_row set
  * beforehand, never synchronises or allocates, and leaves the overflow flag misc[3] for the caller to check, so
  * it can be captured into a HIP graph. */
-enum { MG_CHAIN_SYNC = 0, MG_CHAIN_GRAPH = 1, MG_CHAIN_CAPS = 2 };  /* CAPS: graph-mode capacities, not captured */
+enum { MG_CHAIN_SYNC = 0, MG_CHAIN_GRAPH = 1, MG_CHAIN_CAPS = 2, MG_CHAIN_ASYNC = 3 };
+/* CAPS: graph-mode capacities, not captured, flags checked at the end; ASYNC: CAPS whose flags are copied to the
+ * host without waiting (checked by mgh_settle) */
 int mgh_eval_chain(struct mg_ctx *c, int forces, int mode);
+/* evaluate; async = 0 waits and checks the flags before returning (every internal caller) */
+int mgh_eval(struct mg_ctx *c, int flags, int async);
+/* finish the pending asynchronous evaluation, if any: wait, check its flags, evaluate again synchronised when the
+ * capacities overflowed */
+int mgh_settle(struct mg_ctx *c);
 /* size the graph-mode capacities from the last evaluation with headroom, allocating edge buffers */
 int mgh_graph_caps(struct mg_ctx *c, int need_edges, int need_row);
 int mgk_reduce(int nstr, int nchunk, const int *chunk_str, const mg_struct_dev *str, const float *Ei, const float *erep,

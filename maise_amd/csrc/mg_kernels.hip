@@ -115,6 +115,7 @@ __global__ void k_check_caps(int N, const int* __restrict__ row_off, int* __rest
                              int cap_row) {
   if (threadIdx.x != 0) return;
   const int ne = row_off[N], mr = misc[0];
+  misc[6] = ne;
   if (ne > cap_edges || mr > cap_row) misc[3] = 1;
   if (ne > misc[4]) misc[4] = ne;
   if (mr > misc[5]) misc[5] = mr;

@@ -255,6 +255,7 @@ int mg_features(mg_ctx *c, float *feat) {
   if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_features: no evaluation yet");
   const mg_model *m = c->m;
   HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   const int nf = m->nspc * (m->ng2 + m->ng4);
   float *d_feat = NULL;
   int r;
