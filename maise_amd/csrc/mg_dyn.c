@@ -32,6 +32,7 @@ int mgk_md_kick_drift(int N, double dt, const double *mass, const double *F, dou
 int mgk_md_kick_scale(int N, double dt, const double *mass, const double *F, double *v, const double *dyn, double tau,
                       hipStream_t st);
 int mgk_copy_d(const double *src, double *dst, int n, hipStream_t st);
+int mgk_md_record(const double *E, double *dyn, double *trace, int nsteps, hipStream_t st);
 int mgk_fire_init(int S, double *state, double dt0, hipStream_t st);
 int mgk_fire(int S, const mg_struct_dev *str, const double *F, double *v, const double *x, double *xin, double *state,
              double fmax, double dtmax, double maxstep, hipStream_t st);
@@ -41,6 +42,11 @@ int mgk_fire_cell(int S, mg_struct_dev *str, const double *F, const double *U, d
                   double maxstep, hipStream_t st);
 
 #define KCHK(x) do { if ((x) != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error()); } while (0)
+
+typedef int (*step_fn)(mg_ctx *c, void *arg);
+static int graph_capture(mg_ctx *c, step_fn head, step_fn tail, void *arg, hipGraphExec_t *exec);
+static int snapshot(mg_ctx *c, int restore);
+#define MG_GRAPH_CHECK 8
 
 static int ensure_dyn(mg_ctx *c) {
   int r;
@@ -113,6 +119,43 @@ int mg_md_get_velocities(mg_ctx *c, double *v) {
   return MG_OK;
 }
 
+typedef struct {
+  double dt, kT, ndof, tau;
+  int nsteps;
+} md_arg;
+
+/* velocity Verlet with the reference's Nose-Hoover xi equation, split around the force evaluation */
+static int md_head(mg_ctx *c, void *arg) {
+  const md_arg *a = (const md_arg *)arg;
+  KCHK(mgk_md_thermo(c->N, c->d_mass, c->d_v, c->d_dyn, a->dt, a->kT, a->ndof, a->tau, NULL, -1, c->st));
+  KCHK(mgk_md_kick_drift(c->N, a->dt, c->d_mass, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_dyn, c->st));
+  return MG_OK;
+}
+
+static int md_tail(mg_ctx *c, void *arg) {
+  const md_arg *a = (const md_arg *)arg;
+  KCHK(mgk_md_kick_scale(c->N, a->dt, c->d_mass, c->d_F, c->d_v, c->d_dyn, a->tau, c->st));
+  KCHK(mgk_md_thermo(c->N, c->d_mass, c->d_v, c->d_dyn, a->dt, a->kT, a->ndof, a->tau, NULL, -1, c->st));
+  KCHK(mgk_md_record(c->d_E, c->d_dyn, c->d_trace, a->nsteps, c->st));  /* step counter dyn[3] */
+  return MG_OK;
+}
+
+/* one step without a graph, with a synchronised evaluation */
+static int md_step_plain(mg_ctx *c, md_arg *a) {
+  int r;
+  if ((r = md_head(c, a))) return r;
+  if (c->h_str[0].xt == 0) {  /* positions advanced: x_in is Cartesian from now on */
+    c->h_str[0].xt = 1;
+    HIPCHK(hipMemcpyAsync(c->d_str, c->h_str, sizeof(mg_struct_dev), hipMemcpyHostToDevice, c->st));
+  }
+  if ((r = mgh_eval(c, MG_EVAL_FORCES, 0))) return r;
+  return md_tail(c, a);
+}
+
+/* NVT steps.  With graphs enabled the first step runs synchronised (sizing the buffers), then one step is captured
+ * and replayed MG_GRAPH_CHECK steps at a time; after each block the host reads the overflow flag and, when a row or
+ * the edge total outgrew the capture, restores the checkpoint taken before the block, evaluates synchronised and
+ * captures again, so the trajectory does not depend on the buffer sizes. */
 int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *epot, double *ekin) {
   if (!c || c->S != 1 || nsteps < 0 || dt <= 0.0) return mg_fail(MG_ERR_ARG, "mg_md_run: bad argument");
   if (!c->d_v) return mg_fail(MG_ERR_ARG, "mg_md_run: call mg_md_init or mg_md_set_velocities first");
@@ -123,26 +166,59 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
     if ((r = mg_dev_alloc((void **)&c->d_trace, sizeof(double) * (2 * nsteps + 2)))) return r;
     c->cap_trace = 2 * nsteps + 2;
   }
-  const double kT = MG_KB * T, ndof = 3.0 * c->N;
-  /* forces at the starting positions */
+  md_arg a = {dt, MG_KB * T, 3.0 * c->N, tau, nsteps};
+  /* forces at the starting positions; step counter to 0 */
   if ((r = mgh_eval(c, MG_EVAL_FORCES, 0))) return r;
-  for (int s = 0; s < nsteps; s++) {
-    KCHK(mgk_md_thermo(c->N, c->d_mass, c->d_v, c->d_dyn, dt, kT, ndof, tau, NULL, -1, c->st));
-    KCHK(mgk_md_kick_drift(c->N, dt, c->d_mass, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_dyn, c->st));
-    /* positions advanced: x_in is Cartesian from now on */
-    if (c->h_str[0].xt == 0) {
-      c->h_str[0].xt = 1;
-      HIPCHK(hipMemcpyAsync(c->d_str, c->h_str, sizeof(mg_struct_dev), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemsetAsync(c->d_dyn + 3, 0, sizeof(double), c->st));
+  hipGraphExec_t gx = NULL;
+  int s = 0, s_ck = 0, need_e = 0, need_r = 0, flags[8];
+#define MCHK(x) do { if ((r = (x)) != MG_OK) goto out; } while (0)
+#define MHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { r = mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); goto out; } } while (0)
+  while (s < nsteps) {
+    if (!c->graph_enable || !gx) {
+      MCHK(md_step_plain(c, &a));
+      s++;
+      if (c->graph_enable && s < nsteps) {
+        MCHK(mgh_graph_caps(c, c->nedges > need_e ? c->nedges : need_e, c->maxrow > need_r ? c->maxrow : need_r));
+        MCHK(graph_capture(c, md_head, md_tail, &a, &gx));
+        MCHK(snapshot(c, 0));
+        s_ck = s;
+        if (mgk_clear(c->d_misc + 3, 3, NULL, 0, NULL, 0, c->st)) { r = mg_fail(MG_ERR_HIP, "misc clear failed"); goto out; }
+      }
+      continue;
     }
-    if ((r = mgh_eval(c, MG_EVAL_FORCES, 0))) return r;
-    KCHK(mgk_md_kick_scale(c->N, dt, c->d_mass, c->d_F, c->d_v, c->d_dyn, tau, c->st));
-    KCHK(mgk_md_thermo(c->N, c->d_mass, c->d_v, c->d_dyn, dt, kT, ndof, tau, c->d_trace + nsteps, s, c->st));
-    KCHK(mgk_copy_d(c->d_E, c->d_trace + s, 1, c->st));
+    const int nrun = nsteps - s < MG_GRAPH_CHECK ? nsteps - s : MG_GRAPH_CHECK;
+    for (int k = 0; k < nrun; k++) MHIP(hipGraphLaunch(gx, c->st));
+    MHIP(hipMemcpyAsync(flags, c->d_misc, sizeof flags, hipMemcpyDeviceToHost, c->st));
+    MHIP(hipStreamSynchronize(c->st));
+    if (flags[3]) {
+      /* restore the checkpoint; the next pass evaluates it synchronised and captures with larger buffers */
+      MCHK(snapshot(c, 1));
+      s = s_ck;
+      need_e = flags[4];
+      need_r = flags[5];
+      hipGraphExecDestroy(gx);
+      gx = NULL;
+      MCHK(mgh_eval(c, MG_EVAL_FORCES, 0));  /* forces at the restored positions */
+      continue;
+    }
+    if (flags[1]) { r = mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)"); goto out; }
+    s += nrun;
+    c->nedges = flags[6];
+    c->maxrow = flags[0];
+    MCHK(snapshot(c, 0));
+    s_ck = s;
+    if (mgk_clear(c->d_misc + 3, 3, NULL, 0, NULL, 0, c->st)) { r = mg_fail(MG_ERR_HIP, "misc clear failed"); goto out; }
   }
-  if (nsteps > 0 && epot) HIPCHK(hipMemcpy(epot, c->d_trace, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
-  if (nsteps > 0 && ekin) HIPCHK(hipMemcpy(ekin, c->d_trace + nsteps, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
-  HIPCHK(hipStreamSynchronize(c->st));
-  return MG_OK;
+  if (nsteps > 0 && epot) MHIP(hipMemcpy(epot, c->d_trace, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
+  if (nsteps > 0 && ekin) MHIP(hipMemcpy(ekin, c->d_trace + nsteps, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
+  MHIP(hipStreamSynchronize(c->st));
+  r = MG_OK;
+out:
+  if (gx) hipGraphExecDestroy(gx);
+  return r;
+#undef MCHK
+#undef MHIP
 }
 
 /* Batched FIRE.  Every 8 iterations the converged structures are counted on the device; once the atoms of the
@@ -154,14 +230,11 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
  * edge buffers are sized from the last synchronised evaluation with headroom, the kernels raise an overflow
  * flag instead of overrunning them, and every MG_GRAPH_CHECK replays the host checks the flags and, on
  * overflow, restores the state saved at the previous check, grows the buffers and re-captures. */
-#define MG_GRAPH_CHECK 8
-
-typedef int (*step_fn)(mg_ctx *c, void *arg);
-
-static int graph_capture(mg_ctx *c, step_fn tail, void *arg, hipGraphExec_t *exec) {
+static int graph_capture(mg_ctx *c, step_fn head, step_fn tail, void *arg, hipGraphExec_t *exec) {
   hipGraph_t g = NULL;
   HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  int r = mgh_eval_chain(c, 1, MG_CHAIN_GRAPH);
+  int r = head ? head(c, arg) : MG_OK;
+  if (r == MG_OK) r = mgh_eval_chain(c, 1, MG_CHAIN_GRAPH);
   if (r == MG_OK && tail) r = tail(c, arg);
   hipError_t e = hipStreamEndCapture(c->st, &g);
   if (r != MG_OK) {
@@ -344,7 +417,7 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
       if (c->graph_enable) {
         it += nrun;
         RCHK(mgh_graph_caps(c, c->nedges > need_e ? c->nedges : need_e, c->maxrow > need_r ? c->maxrow : need_r));
-        RCHK(graph_capture(c, fire_tail, &fa, &gx));
+        RCHK(graph_capture(c, NULL, fire_tail, &fa, &gx));
         RCHK(snapshot(c, 0));
         RKCHK(mgk_clear(c->d_misc + 3, 3, NULL, 0, NULL, 0, c->st));
         continue;

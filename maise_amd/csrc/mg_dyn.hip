@@ -117,6 +117,18 @@ __global__ void k_md_kick_scale(int N, double dt, const double* __restrict__ mas
   for (int q = 0; q < 3; q++) v[3 * i + q] = (v[3 * i + q] + h * F[3 * i + q]) * s;
 }
 
+// end of an MD step replayed from a graph: the step index lives in dyn[3]; trace[s] = E, trace[nsteps + s] = K
+__global__ void k_md_record(const double* __restrict__ E, double* __restrict__ dyn, double* __restrict__ trace,
+                            int nsteps) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int s = (int)dyn[3];
+  if (s >= 0 && s < nsteps) {
+    trace[s] = E[0];
+    trace[nsteps + s] = dyn[1];
+  }
+  dyn[3] = (double)(s + 1);
+}
+
 __global__ void k_copy_d(const double* __restrict__ src, double* __restrict__ dst, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -399,6 +411,11 @@ int mgk_md_kick_drift(int N, double dt, const double* mass, const double* F, dou
 int mgk_md_kick_scale(int N, double dt, const double* mass, const double* F, double* v, const double* dyn, double tau,
                       hipStream_t st) {
   hipLaunchKernelGGL(k_md_kick_scale, dim3((N + 255) / 256), dim3(256), 0, st, N, dt, mass, F, v, dyn, tau);
+  return mgk_status();
+}
+
+int mgk_md_record(const double* E, double* dyn, double* trace, int nsteps, hipStream_t st) {
+  hipLaunchKernelGGL(k_md_record, dim3(1), dim3(64), 0, st, E, dyn, trace, nsteps);
   return mgk_status();
 }
 

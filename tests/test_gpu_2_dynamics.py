@@ -54,6 +54,31 @@ def test_md_nvt_holds_temperature():
     c.close()
 
 
+def test_md_graph_replay_matches_synchronised_loop():
+    """NVT steps replayed from a captured HIP graph, with default headroom and with none (rows outgrow the capture,
+    the checkpoint is restored and the block redone), against every step synchronised: the same kernels in the
+    same order, so the energy traces, positions and velocities agree exactly."""
+    import maise_amd
+    from maise_amd import workloads as wl
+    print("[gpu case] NVT graph vs synchronised, CuAg 256 atoms 900 K", flush=True)
+    m = maise_amd.Model(os.path.join(GOLDEN, "models", "nn_CuAg_d3_v1"))
+    s, l, x = wl.alloy_fcc(4, 3.8, 2, 0.02, seed=6)
+    out = []
+    for enable, headroom in ((True, 0.125), (True, 0.0), (False, 0.125)):
+        c = maise_amd.Context(m)
+        c.set_graph(enable, headroom)
+        c.set_structure(s, l, x)
+        c.md_init(900.0, seed=2)
+        ep, ek = c.md_run(61, dt=2.0, T=900.0, tau=40.0)
+        r = c.result(0)
+        out.append((ep, ek, r["X"], r["F"], c.md_velocities()))
+        c.close()
+    for k in (1, 2):
+        for a, b in zip(out[0], out[k]):
+            assert np.array_equal(a, b)
+    assert np.isfinite(out[0][0]).all()
+
+
 def test_fire_batch_relaxes_and_matches_oracle():
     """Trained ternary potential (random-weight potentials have no minima to relax into), UREP on as in the
     reference executable.  Every structure converges; the FP64 oracle confirms |F| < fmax and the energy at the
