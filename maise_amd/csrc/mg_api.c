@@ -461,14 +461,20 @@ static size_t graph_edge_cap(const mg_ctx *c, int need_edges) {
   return (size_t)need_edges + (size_t)(h * need_edges) + (h > 0.0 ? 256 : 0);
 }
 
+/* edge slots for rows of up to row neighbours at a fixed stride (the layout of evaluations with known capacities) */
+static size_t padded_edges(const mg_ctx *c, int row) { return (size_t)c->N * (size_t)(row > 1 ? row : 1); }
+
 int mgh_graph_caps(mg_ctx *c, int need_edges, int need_row) {
   const double h = c->graph_headroom;
-  const size_t ne = graph_edge_cap(c, need_edges);
-  if (ne > 0x7fffffff) return mg_fail(MG_ERR_ARG, "too many neighbour edges (%zu)", ne);
+  const int row = need_row + (h > 0.0 ? 8 : 0);
+  size_t ne = padded_edges(c, row);
+  const size_t nc = graph_edge_cap(c, need_edges > 0 ? need_edges : 0);
+  if (nc > ne) ne = nc;
+  if (ne > 0x7fffffff) return mg_fail(MG_ERR_ARG, "too many neighbour edge slots (%zu)", ne);
   int r;
   if ((r = ensure_edges(c, ne))) return r;
   c->gcap_edges = (int)ne;
-  c->gcap_row = need_row + (h > 0.0 ? 8 : 0);
+  c->gcap_row = row;
   c->gpstride = ensure_pairs(c, c->gcap_row, c->gcap_row);
   return MG_OK;
 }
@@ -496,7 +502,7 @@ static int chain_flags(mg_ctx *c, int mode) {
     return mg_fail(MG_ERR_HIP, "edge capacity check failed in a synchronised evaluation");
   }
   if (f[1]) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)");
-  c->nedges = f[6];
+  c->nedges = mode == MG_CHAIN_SYNC ? f[6] : -1;  /* padded rows: the total is summed when it is asked for */
   c->maxrow = f[0];
   return MG_OK;
 }
@@ -547,12 +553,13 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, st));
   KCHK(mgk_scan(c->d_bin_count, c->d_bin_start, c->nbins, c->d_tmp, &c->tmp_bytes, st));
   KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_cur, c->d_x, c->d_spc, c->d_bin_rec, st));
-  KCHK(mgk_nbr_count(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_cnt,
-                     c->d_rowcnt, c->d_misc, st));
-  KCHK(mgk_scan(c->d_rowcnt, c->d_row_off, N, c->d_tmp, &c->tmp_bytes, st));
   int cap_row;
   size_t pstride;  /* stored pair lists (forces only): entries per atom, 0 = none */
   if (mode == MG_CHAIN_SYNC) {
+    /* row counts, CSR offsets, one read-back to size the buffers, then the fill at the offsets */
+    KCHK(mgk_nbr_count(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_cnt,
+                       c->d_rowcnt, c->d_misc, st));
+    KCHK(mgk_scan(c->d_rowcnt, c->d_row_off, N, c->d_tmp, &c->tmp_bytes, st));
     int hm[2];
     HIPCHK(hipMemcpyAsync(&hm[0], c->d_row_off + N, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(&hm[1], c->d_misc, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -562,20 +569,27 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     int r;
     /* allocate with the capacity-mode headroom now, so sizing the capacities after this evaluation never has to
      * reallocate (and drop) the edge buffers it has just filled */
-    if ((r = ensure_edges(c, graph_edge_cap(c, c->nedges)))) return r;
+    {
+      size_t ne = graph_edge_cap(c, c->nedges);
+      const size_t np = padded_edges(c, c->maxrow + (c->graph_headroom > 0.0 ? 8 : 0));
+      if (np > ne) ne = np;
+      if ((r = ensure_edges(c, ne))) return r;
+    }
     cap_row = c->maxrow;
     pstride = forces ? ensure_pairs(c, cap_row + (c->graph_headroom > 0.0 ? 8 : 0), cap_row) : 0;
     KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->nedges, cap_row, st));
+    KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
+                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_misc, cap_row, 0, st));
   } else {
+    /* capacities known: one pass, row i at i * gcap_row (the edge buffers hold N * gcap_row slots) */
     cap_row = c->gcap_row;
     pstride = forces ? c->gpstride : 0;
-    KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->gcap_edges, cap_row, st));
+    KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
+                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_misc, cap_row, 1, st));
   }
-  KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
-                    c->d_key, c->d_vec, c->d_rev, c->d_misc + 1, ovf, cap_row, st));
   if (timing) HIPCHK(hipEventRecord(ev[EV_NBR], st));
   /* descriptors, MLP, forces */
-  KCHK(mgk_desc(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
+  KCHK(mgk_desc(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
                 urep, cap_row, c->wpb_desc, pstride ? c->d_gpl : NULL, c->d_npl, pstride, ovf, st));
   if (timing) HIPCHK(hipEventRecord(ev[EV_DESC], st));
   if (m->mfma_ok)
@@ -585,10 +599,10 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
   if (timing) HIPCHK(hipEventRecord(ev[EV_MLP], st));
   if (forces) {
-    KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
+    KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
                    c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, pstride ? c->d_gpl : NULL, c->d_npl, pstride, ovf, st));
     if (timing) HIPCHK(hipEventRecord(ev[EV_FORCE], st));
-    KCHK(mgk_gather(N, c->d_row_off, c->d_rev, c->d_fself, c->d_fe, c->d_F, ovf, st));
+    KCHK(mgk_gather(N, c->d_row_off, c->d_rowcnt, c->d_rev, c->d_fself, c->d_fe, c->d_F, ovf, st));
   } else if (timing) {
     HIPCHK(hipEventRecord(ev[EV_FORCE], st));
   }
@@ -695,23 +709,40 @@ int mg_get_neighbors(mg_ctx *c, int *nedges, int *row_off, int *j, int *img) {
   if (!c || !nedges) return mg_fail(MG_ERR_ARG, "mg_get_neighbors: bad argument");
   HIPCHK(hipSetDevice(c->device));
   { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
+  /* rows are (start, count) on the device: CSR after a synchronised evaluation, a fixed stride after one with known
+   * capacities; the export is CSR either way */
+  const int N = c->N;
+  int *st = (int *)malloc(sizeof(int) * (N > 0 ? N : 1)), *ct = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+  hipError_t e = hipMemcpy(st, c->d_row_off, sizeof(int) * N, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ct, c->d_rowcnt, sizeof(int) * N, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) { free(st); free(ct); return mg_fail(MG_ERR_HIP, "%s", hipGetErrorString(e)); }
+  size_t total = 0, extent = 0;
+  for (int i = 0; i < N; i++) {
+    total += (size_t)ct[i];
+    if ((size_t)st[i] + (size_t)ct[i] > extent) extent = (size_t)st[i] + (size_t)ct[i];
+  }
+  c->nedges = (int)total;
   *nedges = c->nedges;
-  if (row_off) HIPCHK(hipMemcpy(row_off, c->d_row_off, (c->N + 1) * sizeof(int), hipMemcpyDeviceToHost));
-  if (j || img) {
-    unsigned long long *k = (unsigned long long *)malloc(8 * (size_t)(c->nedges > 0 ? c->nedges : 1));
-    hipError_t e = hipMemcpy(k, c->d_key, 8 * (size_t)c->nedges, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) { free(k); return mg_fail(MG_ERR_HIP, "%s", hipGetErrorString(e)); }
-    for (int t = 0; t < c->nedges; t++) {
-      if (j) j[t] = (int)((k[t] >> 30) & 0x3fffffffull);
+  if (!row_off && !j && !img) { free(st); free(ct); return MG_OK; }
+  unsigned long long *k = (unsigned long long *)malloc(8 * (extent > 0 ? extent : 1));
+  if (j || img) e = hipMemcpy(k, c->d_key, 8 * extent, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) { free(st); free(ct); free(k); return mg_fail(MG_ERR_HIP, "%s", hipGetErrorString(e)); }
+  int t = 0;
+  for (int i = 0; i < N; i++) {
+    if (row_off) row_off[i] = t;
+    for (int q = 0; q < ct[i]; q++, t++) {
+      const unsigned long long kk = k[st[i] + q];
+      if (j) j[t] = (int)((kk >> 30) & 0x3fffffffull);
       if (img) {
-        unsigned p = (unsigned)(k[t] & 0x3fffffffull);
+        unsigned p = (unsigned)(kk & 0x3fffffffull);
         img[3 * t] = (int)(p & 1023u) - MG_IMG_OFF;
         img[3 * t + 1] = (int)((p >> 10) & 1023u) - MG_IMG_OFF;
         img[3 * t + 2] = (int)((p >> 20) & 1023u) - MG_IMG_OFF;
       }
     }
-    free(k);
   }
+  if (row_off) row_off[N] = t;
+  free(st); free(ct); free(k);
   return MG_OK;
 }
 

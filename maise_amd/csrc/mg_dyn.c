@@ -204,7 +204,7 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
     }
     if (flags[1]) { r = mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)"); goto out; }
     s += nrun;
-    c->nedges = flags[6];
+    c->nedges = -1;  /* padded rows: summed when asked for */
     c->maxrow = flags[0];
     MCHK(snapshot(c, 0));
     s_ck = s;

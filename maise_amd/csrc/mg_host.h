@@ -80,7 +80,7 @@ struct mg_ctx {
   void *d_bin_rec;          /* atoms in bin order: position, species, index (32 B each) */
   /* misc: [0] longest row of the evaluation, [1] error flag (reverse edge missing), [2] converged-structure count,
    * [3] capacity overflow flag, [4] largest edge total and [5] longest row seen since misc was last cleared,
-   * [6] edge total of the evaluation */
+   * [6] edge total of a synchronised evaluation */
   int *d_cnt, *d_rowcnt, *d_row_off, *d_misc;
   void *d_key, *d_vec;
   int *d_rev;
@@ -125,11 +125,12 @@ int mgk_prepare(int N, const double *xin, const int *atom_str, const mg_struct_d
 int mgk_bins(int N, int nbins, const int *bin, const int *bin_start, int *bin_cur, const double *x, const int *spc,
              void *brec, hipStream_t st);
 int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
-                  const int *bin_start, const void *brec, int *cnt, int *rowcnt, int *maxrow, hipStream_t st);
+                  const int *bin_start, const void *brec, int *cnt, int *rowcnt, int *misc, hipStream_t st);
 int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
-                 const int *bin_start, const void *brec, const int *row_off, void *key, void *vec, int *rev, int *err,
-                 const int *ovf, int maxrow, hipStream_t st);
-int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
+                 const int *bin_start, const void *brec, int *row_off, int *rowcnt, int *cnt, void *key, void *vec,
+                 int *rev, int *misc, int maxrow, int padded, hipStream_t st);
+int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *rowcnt, const int *cnt,
+             const void *vec,
              const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, unsigned *gpl, int *npl,
              size_t pstride, const int *ovf, hipStream_t st);
 int mgk_mlp_mfma(int ntiles, const void *tiles, const int *perm, const float *G, const float *frag, const void *frag_bf,
@@ -137,11 +138,12 @@ int mgk_mlp_mfma(int ntiles, const void *tiles, const int *perm, const float *G,
                  hipStream_t st);
 int mgk_mlp(int N, const int *spc, const float *G, const float *W, int wstride, int D, int H1, int H2, float *Ei,
             float *dEdG, hipStream_t st);
-int mgk_force(int N, int S, int variant, const int *spc, const int *row_off, const int *cnt, const void *vec,
+int mgk_force(int N, int S, int variant, const int *spc, const int *row_off, const int *rowcnt, const int *cnt,
+              const void *vec,
               const mg_basis_dev *bp, const float *dEdG, void *fself, void *fe, float *vir, float urep, int maxrow,
               int wpb, const unsigned *gpl, const int *npl, size_t pstride, const int *ovf, hipStream_t st);
-int mgk_gather(int N, const int *row_off, const int *rev, const void *fself, const void *fe, double *F, const int *ovf,
-               hipStream_t st);
+int mgk_gather(int N, const int *row_off, const int *rowcnt, const int *rev, const void *fself, const void *fe,
+               double *F, const int *ovf, hipStream_t st);
 int mgk_clear(int *a, int na, int *b, int nb, int *c, int nc, hipStream_t st);
 int mgk_check_caps(int N, const int *row_off, int *misc, int cap_edges, int cap_row, hipStream_t st);
 

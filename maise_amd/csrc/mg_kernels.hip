@@ -194,17 +194,22 @@ __global__ void k_bin_fill(int N, const int* __restrict__ bin, const int* __rest
 // The FP64 distance uses LIST's expression order (src/cell.c:115-123), so the accepted set is bit-exact.
 // FILL: accepted entries are appended to an LDS row (ballot + lane prefix), the row is sorted by key
 // (species, j, image) with a rank sort and written once: canonical order, independent of the binning.
+// Rows are (row_off[i], rowcnt[i]).  The count pass (!FILL) writes rowcnt and the per-species counts, and a scan
+// then gives row_off, so FILL writes CSR rows.  With padded, FILL runs without the count pass: row i lies at
+// i * cap_row and the fill writes row_off[i], rowcnt[i] and the species counts itself (every consumer addresses rows
+// through row_off and rowcnt).  A row longer than cap_row raises the overflow flag misc[3] and is not written;
+// misc[5] keeps the longest row seen, to size the next capacities.
 #define NBR_CHUNK 64
 template <bool FILL>
 __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x, const int* __restrict__ spc,
                                             const int* __restrict__ atom_str, const mg_struct_dev* __restrict__ str,
                                             const int* __restrict__ bin, const int* __restrict__ bin_start,
                                             const BinRec* __restrict__ brec, int* __restrict__ cnt,
-                                            int* __restrict__ rowcnt, int* __restrict__ maxrow,
-                                            const int* __restrict__ row_off, u64* __restrict__ key,
-                                            float4* __restrict__ vec, int cap, const int* __restrict__ ovf) {
+                                            int* __restrict__ rowcnt, int* __restrict__ misc,
+                                            int* __restrict__ row_off, u64* __restrict__ key,
+                                            float4* __restrict__ vec, int cap, int padded, int cap_row) {
 #pragma clang fp contract(off)
-  if (FILL && *ovf) return;  // rows exceed the capacities this launch was sized for
+  if (FILL && !padded && misc[3]) return;  // rows exceed the capacities this launch was sized for
   __shared__ int e_start[NBR_CHUNK], e_pre[NBR_CHUNK + 1];
   __shared__ unsigned e_img[NBR_CHUNK];
   __shared__ double e_sh[NBR_CHUNK][3];
@@ -292,7 +297,8 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
           rk[n + below] = make_key(sj, j, img);
           rv[n + below] = make_float4((float)a0, (float)a1, (float)a2, (float)sqrt(r));
         }
-      } else {
+      }
+      if (!FILL || padded) {
         c0 += __popcll(__ballot(ok && sj == 0));
         c1 += __popcll(__ballot(ok && sj == 1));
         c2 += __popcll(__ballot(ok && sj == 2));
@@ -302,20 +308,25 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
     }
     __syncthreads();
   }
-  if (!FILL) {
+  if (!FILL || padded) {
     if (lane == 0) {
       cnt[4 * i] = c0; cnt[4 * i + 1] = c1; cnt[4 * i + 2] = c2; cnt[4 * i + 3] = c3;
       rowcnt[i] = n;
       // a plain read first: almost every row is shorter than the running maximum, so few atomics contend
-      if (n > __atomic_load_n(maxrow, __ATOMIC_RELAXED)) atomicMax(maxrow, n);
+      if (n > __atomic_load_n(misc, __ATOMIC_RELAXED)) atomicMax(misc, n);
+      if (FILL) {
+        row_off[i] = i * cap_row;
+        if (n > __atomic_load_n(misc + 5, __ATOMIC_RELAXED)) atomicMax(misc + 5, n);
+        if (n > cap_row) misc[3] = 1;
+      }
     }
-    return;
+    if (!FILL || n > cap_row) return;  // count pass done, or overflow: nothing of this row is written
   }
   const int M = n < cap ? n : cap;  // n <= cap by construction (cap >= longest row)
   __syncthreads();
   // rank sort: the keys of a row are distinct, so the number of smaller keys is an entry's place in the sorted row;
   // every lane counts against broadcast reads of the whole row and writes its entries to their places
-  const int off = row_off[i];
+  const int off = padded ? i * cap_row : row_off[i];
   for (int t0 = 0; t0 < M; t0 += 2 * WAVE) {
     const int ta = t0 + lane, tb = t0 + WAVE + lane;
     const u64 ka = ta < M ? rk[ta] : ~0ull, kb = tb < M ? rk[tb] : ~0ull;
@@ -346,23 +357,24 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
 // branchless lower-bound searches whose loads are independent of each other, so a row of up to 128 neighbours
 // costs ceil(log2(row)) + 1 dependent loads.  Every load stays inside row j (an empty row searches the lane's own
 // edge instead, and the missing reverse edge is flagged).
-__device__ __forceinline__ int rev_lookup_base(const int* __restrict__ row_off, int j, int own, int& n) {
+__device__ __forceinline__ int rev_lookup_base(const int* __restrict__ row_off, const int* __restrict__ rowcnt, int j,
+                                               int own, int& n) {
   const int b = row_off[j];
-  n = row_off[j + 1] - b;
+  n = rowcnt[j];
   if (n > 0) return b;
   n = 1;
   return own;
 }
 
 __global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc, const int* __restrict__ row_off,
-                                             const u64* __restrict__ key, int* __restrict__ rev, int* __restrict__ err,
-                                             const int* __restrict__ ovf) {
+                                             const int* __restrict__ rowcnt, const u64* __restrict__ key,
+                                             int* __restrict__ rev, int* __restrict__ err, const int* __restrict__ ovf) {
   if (*ovf) return;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
   if (i >= N) return;
   const int lane = threadIdx.x & 63;
-  const int off = row_off[i], M = row_off[i + 1] - off;
+  const int off = row_off[i], M = rowcnt[i];
   const int si = spc[i];
   for (int t0 = 0; t0 < M; t0 += 2 * WAVE) {
     const int ta = t0 + lane, tb = t0 + WAVE + lane;
@@ -372,8 +384,8 @@ __global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc,
     const u64 xa = make_key(si, i, neg_img((unsigned)(ka & 0x3fffffffull)));
     const u64 xb = make_key(si, i, neg_img((unsigned)(kb & 0x3fffffffull)));
     int na, nb;
-    const int ba = rev_lookup_base(row_off, key_j(ka), ea, na);
-    const int bb = rev_lookup_base(row_off, key_j(kb), eb, nb);
+    const int ba = rev_lookup_base(row_off, rowcnt, key_j(ka), ea, na);
+    const int bb = rev_lookup_base(row_off, rowcnt, key_j(kb), eb, nb);
     // lower bounds: pa = number of keys of row j below the target
     int pa = 0, pb = 0;
     const int nmax = na > nb ? na : nb;
@@ -460,9 +472,9 @@ struct RowInfo {
 };
 
 __device__ __forceinline__ void row_info(RowInfo& ri, int i, const int* __restrict__ spc, const int* __restrict__ row_off,
-                                         const int* __restrict__ cnt, int S) {
+                                         const int* __restrict__ rowcnt, const int* __restrict__ cnt, int S) {
   ri.off = row_off[i];
-  ri.M = row_off[i + 1] - ri.off;
+  ri.M = rowcnt[i];
   ri.si = spc[i];
   ri.m0 = cnt[4 * i];
   ri.m1 = S > 1 ? cnt[4 * i + 1] : 0;
@@ -680,7 +692,7 @@ __device__ __forceinline__ void group_accum(float4* fj, float4* fk, int j, float
 // ---------------------------------------------------------------------------------------------- descriptors
 template <int PAT, bool UNI>
 __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
-                                                   const int* __restrict__ cnt, const float4* __restrict__ vec,
+                                                   const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                    const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
                                                    float* __restrict__ erep, float urep, int cap,
                                                    unsigned* __restrict__ gpl, int* __restrict__ npl, size_t pstride,
@@ -695,7 +707,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
   if (i >= N) return;
   const int D = bp->D;
   RowInfo ri;
-  row_info(ri, i, spc, row_off, cnt, S);
+  row_info(ri, i, spc, row_off, rowcnt, cnt, S);
   float4* rec = smem_d + (size_t)wv * (R * cap + PCH / 4);
   unsigned* plist = (unsigned*)(rec + R * cap);
   stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
@@ -869,7 +881,7 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 // broadcast loads (kept out of SGPRs, which would spill inside the triplet loop).
 template <int PAT, bool UNI>
 __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
-                                                    const int* __restrict__ cnt, const float4* __restrict__ vec,
+                                                    const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
                                                     float4* __restrict__ fself, float4* __restrict__ fe,
                                                     float* __restrict__ vir, float urep, int cap,
@@ -886,7 +898,7 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
   if (i >= N) return;
   const int D = bp->D;
   RowInfo ri;
-  row_info(ri, i, spc, row_off, cnt, S);
+  row_info(ri, i, spc, row_off, rowcnt, cnt, S);
   float4* rec = smem_f + (size_t)wv * ((R + 2) * cap + MG_GRID / 4 + PCH / 4);
   float4* gts4 = rec + R * cap;                  // [MG_GRID / 4]
   float4* fnb4 = gts4 + MG_GRID / 4;             // [cap] force on each neighbour slot (pair terms, triplet j side)
@@ -1052,7 +1064,8 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
 }
 
 // F_a = own term + sum of the reverse edges' neighbour terms; 4 lanes per atom, fixed order, FP64 accumulation
-__global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ row_off, const int* __restrict__ rev,
+__global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ row_off, const int* __restrict__ rowcnt,
+                                                const int* __restrict__ rev,
                                                 const float4* __restrict__ fself, const float4* __restrict__ fe,
                                                 double* __restrict__ F, const int* __restrict__ ovf) {
   if (*ovf) return;
@@ -1060,7 +1073,8 @@ __global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ r
   const int a = g >> 2, sub = g & 3;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (a < N) {
-    for (int e = row_off[a] + sub; e < row_off[a + 1]; e += 4) {
+    const int e1 = row_off[a] + rowcnt[a];
+    for (int e = row_off[a] + sub; e < e1; e += 4) {
       float4 v = fe[rev[e]];
       fx += v.x; fy += v.y; fz += v.z;
     }
@@ -1171,27 +1185,27 @@ int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_cu
 }
 
 int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
-                  const int* bin_start, const void* brec, int* cnt, int* rowcnt, int* maxrow, hipStream_t st) {
+                  const int* bin_start, const void* brec, int* cnt, int* rowcnt, int* misc, hipStream_t st) {
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, st, N, x, spc, atom_str, str, bin, bin_start,
-                     (const BinRec*)brec, cnt,
-                     rowcnt, maxrow, (const int*)nullptr, (u64*)nullptr, (float4*)nullptr, 0, (const int*)nullptr);
+                     (const BinRec*)brec, cnt, rowcnt, misc, (int*)nullptr, (u64*)nullptr, (float4*)nullptr, 0, 0, 0);
   return mgk_status();
 }
 
 // Fill and sort the rows (LDS tile of cap entries: a power of two >= the longest row) and build the reverse-edge
-// index.  Rows longer than 2048 neighbours are rejected (-2).
+// index.  padded: no count pass, row i at i * maxrow.  Rows longer than 2048 neighbours are rejected (-2).
 int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
-                 const int* bin_start, const void* brec, const int* row_off, void* key, void* vec, int* rev,
-                 int* err, const int* ovf, int maxrow, hipStream_t st) {
+                 const int* bin_start, const void* brec, int* row_off, int* rowcnt, int* cnt, void* key, void* vec,
+                 int* rev, int* misc, int maxrow, int padded, hipStream_t st) {
   if (N == 0) return 0;
   int cap = 64;
   while (cap < maxrow) cap <<= 1;
   if (cap > 2048) return -2;
   size_t sm = (size_t)cap * (sizeof(u64) + sizeof(float4));
   hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), sm, st, N, x, spc, atom_str, str, bin, bin_start, (const BinRec*)brec,
-                     (int*)nullptr, (int*)nullptr, (int*)nullptr, row_off, (u64*)key, (float4*)vec, cap, ovf);
-  hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, row_off, (const u64*)key, rev, err, ovf);
+                     cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, cap, padded, maxrow);
+  hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, (const int*)row_off, (const int*)rowcnt,
+                     (const u64*)key, rev, misc + 1, (const int*)(misc + 3));
   return mgk_status();
 }
 
@@ -1225,14 +1239,15 @@ static int pick_wpb(int wpb, size_t sm_wave) {
   } while (0)
 
 // variant = G4 pattern (bits 0-3) | shared-eta flag (bit 4)
-int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
+int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* rowcnt, const int* cnt,
+             const void* vec,
              const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, int wpb, unsigned* gpl, int* npl,
              size_t pstride, const int* ovf, hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * R * sizeof(float4) + PCH * sizeof(unsigned);
-  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, G, erep, urep, cap, gpl, npl, pstride, ovf);
+  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, G, erep, urep, cap, gpl, npl, pstride, ovf);
   return mgk_status();
 }
 
@@ -1244,22 +1259,23 @@ int mgk_mlp(int N, const int* spc, const float* G, const float* W, int wstride, 
   return mgk_status();
 }
 
-int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, const int* cnt, const void* vec,
+int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, const int* rowcnt, const int* cnt,
+              const void* vec,
               const mg_basis_dev* bp, const float* dEdG, void* fself, void* fe, float* vir, float urep, int maxrow,
               int wpb, const unsigned* gpl, const int* npl, size_t pstride, const int* ovf, hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * (R + 2) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
-  DISPATCH_PU(k_force, sm, N, S, spc, row_off, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
+  DISPATCH_PU(k_force, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
               urep, cap, gpl, npl, pstride, ovf);
   return mgk_status();
 }
 
-int mgk_gather(int N, const int* row_off, const int* rev, const void* fself, const void* fe, double* F, const int* ovf,
-               hipStream_t st) {
+int mgk_gather(int N, const int* row_off, const int* rowcnt, const int* rev, const void* fself, const void* fe, double* F,
+               const int* ovf, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rev, (const float4*)fself,
+  hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rowcnt, rev, (const float4*)fself,
                      (const float4*)fe, F, ovf);
   return mgk_status();
 }
