@@ -207,7 +207,8 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
                                             const BinRec* __restrict__ brec, int* __restrict__ cnt,
                                             int* __restrict__ rowcnt, int* __restrict__ misc,
                                             int* __restrict__ row_off, u64* __restrict__ key,
-                                            float4* __restrict__ vec, int cap, int padded, int cap_row) {
+                                            float4* __restrict__ vec, int* __restrict__ rev, int cap, int padded,
+                                            int cap_row) {
 #pragma clang fp contract(off)
   if (FILL && !padded && misc[3]) return;  // rows exceed the capacities this launch was sized for
   __shared__ int e_start[NBR_CHUNK], e_pre[NBR_CHUNK + 1];
@@ -337,8 +338,8 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
       ra += ku < ka ? 1 : 0;
       rb += ku < kb ? 1 : 0;
     }
-    if (ta < M) { key[off + ra] = ka; vec[off + ra] = rv[ta]; }
-    if (tb < M) { key[off + rb] = kb; vec[off + rb] = rv[tb]; }
+    if (ta < M) { key[off + ra] = ka; vec[off + ra] = rv[ta]; rev[off + ta] = -1; }
+    if (tb < M) { key[off + rb] = kb; vec[off + rb] = rv[tb]; rev[off + tb] = -1; }
   }
 }
 
@@ -353,10 +354,19 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
-// One wave per atom, four atoms per block; each lane looks up two edges at once (slots t and t + 64) with
-// branchless lower-bound searches whose loads are independent of each other, so a row of up to 128 neighbours
-// costs ceil(log2(row)) + 1 dependent loads.  Every load stays inside row j (an empty row searches the lane's own
-// edge instead, and the missing reverse edge is flagged).
+// LDS ordering between the lanes of one wave (several independent waves share a workgroup, so no block barrier)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Edges come in mutual pairs e = (i -> j, img), e' = (j -> i, -img), with rev[e] = e' and rev[e'] = e, so only
+// the edge of each pair with the smaller (i, img) looks the other up and writes both.  One wave per atom, four atoms
+// per block: the wave compacts the slots it owns into LDS, then each lane looks up two of them at once with
+// branchless lower-bound searches whose loads are independent of each other (a row of up to 128 neighbours costs
+// ceil(log2(row)) + 1 dependent loads).  Every load stays inside row j (an empty row searches the lane's own edge
+// instead, and the missing reverse edge is flagged).
 __device__ __forceinline__ int rev_lookup_base(const int* __restrict__ row_off, const int* __restrict__ rowcnt, int j,
                                                int own, int& n) {
   const int b = row_off[j];
@@ -366,9 +376,11 @@ __device__ __forceinline__ int rev_lookup_base(const int* __restrict__ row_off, 
   return own;
 }
 
+#define REV_TILE 256
 __global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc, const int* __restrict__ row_off,
                                              const int* __restrict__ rowcnt, const u64* __restrict__ key,
                                              int* __restrict__ rev, int* __restrict__ err, const int* __restrict__ ovf) {
+  __shared__ int olist[4][REV_TILE];
   if (*ovf) return;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
@@ -376,49 +388,61 @@ __global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc,
   const int lane = threadIdx.x & 63;
   const int off = row_off[i], M = rowcnt[i];
   const int si = spc[i];
-  for (int t0 = 0; t0 < M; t0 += 2 * WAVE) {
-    const int ta = t0 + lane, tb = t0 + WAVE + lane;
-    const bool va = ta < M, vb = tb < M;
-    const int ea = off + (va ? ta : 0), eb = off + (vb ? tb : 0);
-    const u64 ka = key[ea], kb = key[eb];
-    const u64 xa = make_key(si, i, neg_img((unsigned)(ka & 0x3fffffffull)));
-    const u64 xb = make_key(si, i, neg_img((unsigned)(kb & 0x3fffffffull)));
-    int na, nb;
-    const int ba = rev_lookup_base(row_off, rowcnt, key_j(ka), ea, na);
-    const int bb = rev_lookup_base(row_off, rowcnt, key_j(kb), eb, nb);
-    // lower bounds: pa = number of keys of row j below the target
-    int pa = 0, pb = 0;
-    const int nmax = na > nb ? na : nb;
-    for (int step = 1 << (31 - __clz(nmax)); step > 0; step >>= 1) {
-      const int qa = pa + step, qb = pb + step;
-      const u64 ya = key[ba + (qa < na ? qa : na) - 1];
-      const u64 yb = key[bb + (qb < nb ? qb : nb) - 1];
-      if (qa <= na && ya < xa) pa = qa;
-      if (qb <= nb && yb < xb) pb = qb;
+  int* ol = olist[wv];
+  for (int c0 = 0; c0 < M; c0 += REV_TILE) {
+    // slots of this tile whose edge owns its pair: j > i, or j == i with the smaller packed image
+    int no = 0;
+    for (int t0 = c0; t0 < c0 + REV_TILE && t0 < M; t0 += WAVE) {
+      const int t = t0 + lane;
+      bool own = false;
+      if (t < M) {
+        const u64 k = key[off + t];
+        const int j = key_j(k);
+        const unsigned im = (unsigned)(k & 0x3fffffffull);
+        own = j > i || (j == i && im < neg_img(im));
+      }
+      const u64 bm = __ballot(own);
+      const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+      if (own) ol[no + below] = t;
+      no += __popcll(bm);
     }
-    const u64 za = key[ba + (pa < na ? pa : na - 1)];
-    const u64 zb = key[bb + (pb < nb ? pb : nb - 1)];
-    if (va) {
-      int f = ba + pa;
-      if (pa >= na || za != xa) { atomicOr(err, 1); f = ea; }
-      rev[ea] = f;
+    wave_sync();
+    for (int q0 = 0; q0 < no; q0 += 2 * WAVE) {
+      const int qa = q0 + lane, qb = q0 + WAVE + lane;
+      const bool va = qa < no, vb = qb < no;
+      const int ea = off + (va ? ol[qa] : 0), eb = off + (vb ? ol[qb] : 0);
+      const u64 ka = key[ea], kb = key[eb];
+      const u64 xa = make_key(si, i, neg_img((unsigned)(ka & 0x3fffffffull)));
+      const u64 xb = make_key(si, i, neg_img((unsigned)(kb & 0x3fffffffull)));
+      int na, nb;
+      const int ba = rev_lookup_base(row_off, rowcnt, key_j(ka), ea, na);
+      const int bb = rev_lookup_base(row_off, rowcnt, key_j(kb), eb, nb);
+      // lower bounds: pa = number of keys of row j below the target
+      int pa = 0, pb = 0;
+      const int nmax = na > nb ? na : nb;
+      for (int step = 1 << (31 - __clz(nmax)); step > 0; step >>= 1) {
+        const int ya_i = pa + step, yb_i = pb + step;
+        const u64 ya = key[ba + (ya_i < na ? ya_i : na) - 1];
+        const u64 yb = key[bb + (yb_i < nb ? yb_i : nb) - 1];
+        if (ya_i <= na && ya < xa) pa = ya_i;
+        if (yb_i <= nb && yb < xb) pb = yb_i;
+      }
+      const u64 za = key[ba + (pa < na ? pa : na - 1)];
+      const u64 zb = key[bb + (pb < nb ? pb : nb - 1)];
+      if (va) {
+        if (pa < na && za == xa) { rev[ea] = ba + pa; rev[ba + pa] = ea; }
+        else { atomicOr(err, 1); rev[ea] = ea; }
+      }
+      if (vb) {
+        if (pb < nb && zb == xb) { rev[eb] = bb + pb; rev[bb + pb] = eb; }
+        else { atomicOr(err, 1); rev[eb] = eb; }
+      }
     }
-    if (vb) {
-      int f = bb + pb;
-      if (pb >= nb || zb != xb) { atomicOr(err, 1); f = eb; }
-      rev[eb] = f;
-    }
+    wave_sync();
   }
 }
 
 // ---------------------------------------------------------------------------------------------- wave helpers
-// LDS ordering between the lanes of one wave (several independent waves share a workgroup, so no block barrier)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, WAVE);
@@ -1065,7 +1089,7 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
 
 // F_a = own term + sum of the reverse edges' neighbour terms; 4 lanes per atom, fixed order, FP64 accumulation
 __global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ row_off, const int* __restrict__ rowcnt,
-                                                const int* __restrict__ rev,
+                                                const int* __restrict__ rev, int* __restrict__ err,
                                                 const float4* __restrict__ fself, const float4* __restrict__ fe,
                                                 double* __restrict__ F, const int* __restrict__ ovf) {
   if (*ovf) return;
@@ -1075,7 +1099,9 @@ __global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ r
   if (a < N) {
     const int e1 = row_off[a] + rowcnt[a];
     for (int e = row_off[a] + sub; e < e1; e += 4) {
-      float4 v = fe[rev[e]];
+      const int q = rev[e];
+      if (q < 0) { atomicOr(err, 1); continue; }  // reverse edge missing (flagged; the evaluation fails)
+      float4 v = fe[q];
       fx += v.x; fy += v.y; fz += v.z;
     }
   }
@@ -1188,7 +1214,8 @@ int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, c
                   const int* bin_start, const void* brec, int* cnt, int* rowcnt, int* misc, hipStream_t st) {
   if (N == 0) return 0;
   hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, st, N, x, spc, atom_str, str, bin, bin_start,
-                     (const BinRec*)brec, cnt, rowcnt, misc, (int*)nullptr, (u64*)nullptr, (float4*)nullptr, 0, 0, 0);
+                     (const BinRec*)brec, cnt, rowcnt, misc, (int*)nullptr, (u64*)nullptr, (float4*)nullptr,
+                     (int*)nullptr, 0, 0, 0);
   return mgk_status();
 }
 
@@ -1203,7 +1230,7 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
   if (cap > 2048) return -2;
   size_t sm = (size_t)cap * (sizeof(u64) + sizeof(float4));
   hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), sm, st, N, x, spc, atom_str, str, bin, bin_start, (const BinRec*)brec,
-                     cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, cap, padded, maxrow);
+                     cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev, cap, padded, maxrow);
   hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, (const int*)row_off, (const int*)rowcnt,
                      (const u64*)key, rev, misc + 1, (const int*)(misc + 3));
   return mgk_status();
@@ -1272,10 +1299,10 @@ int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, con
   return mgk_status();
 }
 
-int mgk_gather(int N, const int* row_off, const int* rowcnt, const int* rev, const void* fself, const void* fe, double* F,
-               const int* ovf, hipStream_t st) {
+int mgk_gather(int N, const int* row_off, const int* rowcnt, const int* rev, int* err, const void* fself, const void* fe,
+               double* F, const int* ovf, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rowcnt, rev, (const float4*)fself,
+  hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rowcnt, rev, err, (const float4*)fself,
                      (const float4*)fe, F, ovf);
   return mgk_status();
 }
