@@ -298,8 +298,9 @@ static double cell_volume(const double *L) { /* src/cutl.c:178-188 */
   return fabs(V);
 }
 
-/* bins and stencil of one structure: bins at least rc thick (perpendicular to each lattice plane), stencil
- * half width K_q >= rc |R_q| nb_q so every pair within rc lies in the stencil. */
+/* bins and stencil of one structure: bins at least rc/2 thick (perpendicular to each lattice plane), stencil
+ * half width K_q >= rc |R_q| nb_q so every pair within rc lies in the stencil (the neighbour kernels skip the
+ * stencil bins that lie farther than rc from the atom). */
 static void plan_bins(mg_struct_dev *s, double rc, int natom) {
   const double eps = 1e-6;
   if (s->nd == 0) {
@@ -311,7 +312,7 @@ static void plan_bins(mg_struct_dev *s, double rc, int natom) {
       double a = 0.0;
       for (int k = 0; k < 3; k++) a += s->R[3 * q + k] * s->R[3 * q + k];
       h[q] = rc * sqrt(a) * (1.0 + eps); /* rc in units of the lattice-plane spacing */
-      double nb = floor(1.0 / h[q]);
+      double nb = floor(2.0 / h[q]);
       s->nb[q] = nb < 1.0 ? 1 : (nb > 1024.0 ? 1024 : (int)nb);
     }
     /* keep the bin count O(natom) for sparse cells */

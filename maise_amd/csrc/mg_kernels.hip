@@ -229,6 +229,25 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
   const int local = bin[i] - st->bin0;
   const int b2 = local % nb2, b1 = (local / nb2) % nb1, b0 = local / (nb1 * nb2);
   const unsigned img0 = pack_img(0, 0, 0);
+  // stencil pruning: a lower bound on the distance from atom i to every point of a stencil bin.  Along lattice
+  // direction q the bin is the slab of fractional coordinates [(b_q + d_q)/nb_q, (b_q + d_q + 1)/nb_q); its gap to
+  // the atom's fractional coordinate times the plane spacing 1/|R_q| bounds the distance.  The largest of the three
+  // bounds holds for any cell; for mutually orthogonal lattice rows the slabs are orthogonal and the bounds add in
+  // quadrature.  Bins whose bound exceeds rc (with a relative margin far above rounding) hold no neighbour.
+  double fi[3], sp[3];
+  for (int q = 0; q < 3; q++) {
+    fi[q] = xi0 * st->R[3 * q] + xi1 * st->R[3 * q + 1] + xi2 * st->R[3 * q + 2];
+    sp[q] = 1.0 / sqrt(st->R[3 * q] * st->R[3 * q] + st->R[3 * q + 1] * st->R[3 * q + 1] + st->R[3 * q + 2] * st->R[3 * q + 2]);
+  }
+  bool orth = true;
+  for (int q = 0; q < 3; q++) {
+    const double* u = st->L + 3 * q;
+    const double* v = st->L + 3 * ((q + 1) % 3);
+    const double uv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2], vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (uv * uv > 1e-24 * uu * vv) orth = false;
+  }
+  const double rc2m = RC * (1.0 + 1e-9);
   int c0 = 0, c1 = 0, c2 = 0, c3 = 0;  // per-species counts (wave-uniform)
   int n = 0;                           // accepted so far (wave-uniform)
   for (int base = 0; base < nsten; base += NBR_CHUNK) {
@@ -236,6 +255,17 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
     int count = 0;
     if (t < nsten) {
       const int d2 = t % w2n - K2, d1 = (t / w2n) % w1n - K1, d0 = t / (w2n * w1n) - K0;
+      const int dd[3] = {d0, d1, d2}, bb[3] = {b0, b1, b2}, nn[3] = {nb0, nb1, nb2};
+      double lb[3];
+      for (int q = 0; q < 3; q++) {
+        double g = 0.0;
+        if (dd[q] > 0) g = (double)(bb[q] + dd[q]) / nn[q] - fi[q];
+        else if (dd[q] < 0) g = fi[q] - (double)(bb[q] + dd[q] + 1) / nn[q];
+        lb[q] = g > 0.0 ? g * sp[q] : 0.0;
+      }
+      const double lb2 = orth ? lb[0] * lb[0] + lb[1] * lb[1] + lb[2] * lb[2]
+                              : fmax(lb[0] * lb[0], fmax(lb[1] * lb[1], lb[2] * lb[2]));
+      const bool far = lb2 > rc2m;
       int w0 = b0 + d0, ic0 = floordiv_i(w0, nb0);
       w0 -= ic0 * nb0;
       int w1 = b1 + d1, ic1 = floordiv_i(w1, nb1);
@@ -244,7 +274,7 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
       w2 -= ic2 * nb2;
       const int w = st->bin0 + (w0 * nb1 + w1) * nb2 + w2;
       e_start[lane] = bin_start[w];
-      count = bin_start[w + 1] - bin_start[w];
+      count = far ? 0 : bin_start[w + 1] - bin_start[w];
       e_img[lane] = pack_img(ic0, ic1, ic2);
       for (int q = 0; q < 3; q++) {
         double v = 0.0;
