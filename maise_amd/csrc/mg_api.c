@@ -91,8 +91,8 @@ static int ensure_bins(mg_ctx *c, size_t nb) {
   size_t n = nb + nb / 4 + 16;
   ALLOC(c->d_bin_count, (n + 1) * sizeof(int));
   ALLOC(c->d_bin_start, (n + 1) * sizeof(int));
-  ALLOC(c->d_bin_cur, (n + 1) * sizeof(int));
   c->cap_bins = n;
+  c->bins_dirty = 1;
   return MG_OK;
 }
 
@@ -184,7 +184,7 @@ void mg_ctx_free(mg_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   void *ptrs[] = {c->d_basis, c->d_mlp, c->d_str, c->d_atom_str, c->d_spc, c->d_xin, c->d_x, c->d_bin, c->d_bin_count,
-                  c->d_bin_start, c->d_bin_cur, c->d_bin_rec, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
+                  c->d_bin_start, c->d_bin_rec, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
                   c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
                   c->d_vir, c->d_fself, c->d_fe, c->d_gpl, c->d_npl, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
@@ -336,6 +336,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
     c->pending = 0;
     HIPCHK(hipStreamSynchronize(c->st));
   }
+  c->bins_dirty = 1;  /* the bin layout may change: clear the counts once before the next evaluation */
   int N = atom_off[S];
   for (int i = 0; i < N; i++)
     if (spc[i] < 0 || spc[i] >= c->m->nspc) return mg_fail(MG_ERR_ARG, "atom %d: species %d outside the model's %d", i, spc[i], c->m->nspc);
@@ -452,6 +453,7 @@ int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const dou
 #define KCHK(x)                                                                                                     \
   do {                                                                                                              \
     int kr_ = (x);                                                                                                  \
+    if (kr_ != 0) c->bins_dirty = 1; /* the chain stopped part way: bin counts may be left non-zero */             \
     if (kr_ == -3) return mg_fail(MG_ERR_NEIGHBOR, "neighbour rows too long for the LDS of one CU (%d)", c->maxrow); \
     if (kr_ == -2) return mg_fail(MG_ERR_NEIGHBOR, "an atom has more than 2048 neighbours (%d)", c->maxrow);        \
     if (kr_ != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error());               \
@@ -550,10 +552,16 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     HIPCHK(hipEventRecord(ev[EV_START], st));
   }
   /* neighbour list: bins, row counts, row offsets; graph mode keeps the flags of misc[1..5] for the caller */
-  KCHK(mgk_clear(c->d_bin_count, c->nbins + 1, c->d_bin_cur, c->nbins + 1, c->d_misc, mode == MG_CHAIN_GRAPH ? 1 : 16, st));
-  KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, st));
+  /* bin counts are zero between evaluations (the bin fill counts them back down); they are cleared explicitly
+   * only after an allocation or an evaluation that stopped part way */
+  if (c->bins_dirty) {
+    KCHK(mgk_clear(c->d_bin_count, c->nbins + 1, NULL, 0, NULL, 0, st));
+    c->bins_dirty = 0;
+  }
+  KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, c->d_misc,
+                   mode == MG_CHAIN_GRAPH ? 1 : 16, st));
   KCHK(mgk_scan(c->d_bin_count, c->d_bin_start, c->nbins, c->d_tmp, &c->tmp_bytes, st));
-  KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_cur, c->d_x, c->d_spc, c->d_bin_rec, st));
+  KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_count, c->d_x, c->d_spc, c->d_bin_rec, st));
   int cap_row;
   size_t pstride;  /* stored pair lists (forces only): entries per atom, 0 = none */
   if (mode == MG_CHAIN_SYNC) {

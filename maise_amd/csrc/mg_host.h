@@ -76,7 +76,8 @@ struct mg_ctx {
   mg_struct_dev *d_str;
   int *d_atom_str, *d_spc;
   double *d_xin, *d_x;
-  int *d_bin, *d_bin_count, *d_bin_start, *d_bin_cur;
+  int *d_bin, *d_bin_count, *d_bin_start;
+  int bins_dirty;           /* bin counts must be cleared before the next evaluation */
   void *d_bin_rec;          /* atoms in bin order: position, species, index (32 B each) */
   /* misc: [0] longest row of the evaluation, [1] error flag (reverse edge missing), [2] converged-structure count,
    * [3] capacity overflow flag, [4] largest edge total and [5] longest row seen since misc was last cleared,
@@ -121,8 +122,8 @@ int mg_dev_alloc(void **p, size_t bytes);
 /* kernel launchers (mg_kernels.hip) */
 int mgk_scan(const int *in, int *out, int n, void *tmp, size_t *tmp_bytes, hipStream_t st);
 int mgk_prepare(int N, const double *xin, const int *atom_str, const mg_struct_dev *str, double *x, int *bin,
-                int *bin_count, hipStream_t st);
-int mgk_bins(int N, int nbins, const int *bin, const int *bin_start, int *bin_cur, const double *x, const int *spc,
+                int *bin_count, int *misc, int nclear, hipStream_t st);
+int mgk_bins(int N, int nbins, const int *bin, const int *bin_start, int *bin_count, const double *x, const int *spc,
              void *brec, hipStream_t st);
 int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
                   const int *bin_start, const void *brec, int *cnt, int *rowcnt, int *misc, hipStream_t st);

@@ -28,7 +28,7 @@
 #define MG_MAXG4 MG_GRID
 #define MG_IMG_OFF 512
 #define MG_LOG2E 1.4426950408889634
-#define MG_RCHUNK 2048              /* atoms per FP64 reduction chunk */
+#define MG_RCHUNK 512               /* atoms per FP64 reduction chunk */
 
 #ifdef __cplusplus
 extern "C" {

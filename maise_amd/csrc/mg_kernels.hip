@@ -102,6 +102,30 @@ __global__ __launch_bounds__(256) void k_scan_add(int* __restrict__ out, int n, 
   if (b == 0 && threadIdx.x == 0) out[0] = 0;
 }
 
+// exclusive scan of up to 1024 * SCAN1_PT values in one block: thread t owns a contiguous run of pt values
+#define SCAN1_PT 64
+__global__ __launch_bounds__(1024) void k_scan_one(const int* __restrict__ in, int* __restrict__ out, int n, int pt) {
+  __shared__ int ts[1024];
+  const int t = threadIdx.x, a0 = t * pt;
+  int run = 0;
+  for (int q = 0; q < pt; q++) run += a0 + q < n ? in[a0 + q] : 0;
+  ts[t] = run;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan of the thread sums
+    int a = t >= o ? ts[t - o] : 0;
+    __syncthreads();
+    ts[t] += a;
+    __syncthreads();
+  }
+  int acc = ts[t] - run;
+  for (int q = 0; q < pt; q++)
+    if (a0 + q < n) {
+      acc += in[a0 + q];
+      out[a0 + q + 1] = acc;
+    }
+  if (t == 0) out[0] = 0;
+}
+
 __global__ void k_clear3(int* __restrict__ a, int na, int* __restrict__ b, int nb, int* __restrict__ c, int nc) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < na) a[k] = 0;
@@ -122,11 +146,14 @@ __global__ void k_check_caps(int N, const int* __restrict__ row_off, int* __rest
 }
 
 // ---------------------------------------------------------------------------------------------- prepare
+// Wraps positions and counts atoms per bin (bin_count is zero on entry: the bin fill of the previous evaluation
+// counted it back down).  Also clears the first nclear flags of misc for this evaluation.
 __global__ void k_prepare(int N, const double* __restrict__ xin, const int* __restrict__ atom_str,
                           const mg_struct_dev* __restrict__ str, double* __restrict__ x, int* __restrict__ bin,
-                          int* __restrict__ bin_count) {
+                          int* __restrict__ bin_count, int* __restrict__ misc, int nclear) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nclear) misc[i] = 0;
   if (i >= N) return;
   const mg_struct_dev* st = &str[atom_str[i]];
   double X[3] = {xin[3 * i], xin[3 * i + 1], xin[3 * i + 2]};
@@ -173,13 +200,14 @@ struct __align__(16) BinRec {
   int s, j;
 };
 
+// Places atoms in their bins, taking slots by counting bin_count back down to zero (ready for the next evaluation).
 __global__ void k_bin_fill(int N, const int* __restrict__ bin, const int* __restrict__ bin_start,
-                           int* __restrict__ bin_cur, const double* __restrict__ x, const int* __restrict__ spc,
+                           int* __restrict__ bin_count, const double* __restrict__ x, const int* __restrict__ spc,
                            BinRec* __restrict__ brec) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int b = bin[i];
-  int p = bin_start[b] + atomicAdd(&bin_cur[b], 1);
+  int p = bin_start[b] + atomicSub(&bin_count[b], 1) - 1;
   BinRec r;
   r.x = x[3 * i]; r.y = x[3 * i + 1]; r.z = x[3 * i + 2];
   r.s = spc[i]; r.j = i;
@@ -1170,15 +1198,22 @@ __global__ __launch_bounds__(256) void k_reduce_part(int nchunk, const int* __re
   if (t < 7) part[7 * (size_t)ch + t] = sh[t][0];
 }
 
-__global__ void k_reduce_final(int nstr, const mg_struct_dev* __restrict__ str, const double* __restrict__ part,
-                               double* __restrict__ Eout, double* __restrict__ Uout) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// one wave per structure: lane l sums chunks l, l + 64, ... of the structure, then a fixed butterfly tree
+__global__ __launch_bounds__(256) void k_reduce_final(int nstr, const mg_struct_dev* __restrict__ str,
+                                                      const double* __restrict__ part, double* __restrict__ Eout,
+                                                      double* __restrict__ Uout) {
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (s >= nstr) return;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int c = str[s].chunk0; c < str[s].chunk0 + str[s].nchunk; c++)
-    for (int q = 0; q < 7; q++) acc[q] += part[7 * (size_t)c + q];
-  Eout[s] = acc[0];
-  for (int q = 0; q < 6; q++) Uout[6 * s + q] = acc[1 + q];
+  const int c0 = str[s].chunk0, nc = str[s].nchunk;
+  for (int c = lane; c < nc; c += WAVE)
+    for (int q = 0; q < 7; q++) acc[q] += part[7 * (size_t)(c0 + c) + q];
+  for (int o = 32; o > 0; o >>= 1)
+    for (int q = 0; q < 7; q++) acc[q] += __shfl_xor(acc[q], o, WAVE);
+  if (lane == 0) {
+    Eout[s] = acc[0];
+    for (int q = 0; q < 6; q++) Uout[6 * s + q] = acc[1 + q];
+  }
 }
 
 // ---------------------------------------------------------------------------------------------- launchers
@@ -1207,6 +1242,10 @@ int mgk_scan(const int* in, int* out, int n, void* tmp, size_t* tmp_bytes, hipSt
     hipLaunchKernelGGL(k_scan_tile, dim3(1), dim3(256), 0, st, in, out, n, (int*)nullptr, 1);
     return mgk_status();
   }
+  if (n <= 1024 * SCAN1_PT) {  // one launch
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, st, in, out, n, (n + 1023) / 1024);
+    return mgk_status();
+  }
   hipLaunchKernelGGL(k_scan_tile, dim3(nb), dim3(256), 0, st, in, out, n, bsum, 0);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb);
   hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, st, out, n, bsum);
@@ -1226,16 +1265,18 @@ int mgk_check_caps(int N, const int* row_off, int* misc, int cap_edges, int cap_
 }
 
 int mgk_prepare(int N, const double* xin, const int* atom_str, const mg_struct_dev* str, double* x, int* bin,
-                int* bin_count, hipStream_t st) {
-  if (N == 0) return 0;
-  hipLaunchKernelGGL(k_prepare, dim3((N + 255) / 256), dim3(256), 0, st, N, xin, atom_str, str, x, bin, bin_count);
+                int* bin_count, int* misc, int nclear, hipStream_t st) {
+  const int n = N > nclear ? N : nclear;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_prepare, dim3((n + 255) / 256), dim3(256), 0, st, N, xin, atom_str, str, x, bin, bin_count, misc,
+                     nclear);
   return mgk_status();
 }
 
-int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_cur, const double* x, const int* spc,
+int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_count, const double* x, const int* spc,
              void* brec, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_bin_fill, dim3((N + 255) / 256), dim3(256), 0, st, N, bin, bin_start, bin_cur, x, spc,
+  hipLaunchKernelGGL(k_bin_fill, dim3((N + 255) / 256), dim3(256), 0, st, N, bin, bin_start, bin_count, x, spc,
                      (BinRec*)brec);
   return mgk_status();
 }
@@ -1341,7 +1382,7 @@ int mgk_reduce(int nstr, int nchunk, const int* chunk_str, const mg_struct_dev* 
                const float* vir, int have_vir, double* part, double* E, double* U, hipStream_t st) {
   if (nstr == 0) return 0;
   hipLaunchKernelGGL(k_reduce_part, dim3(nchunk), dim3(256), 0, st, nchunk, chunk_str, str, Ei, erep, vir, have_vir, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3((nstr + 255) / 256), dim3(256), 0, st, nstr, str, part, E, U);
+  hipLaunchKernelGGL(k_reduce_final, dim3((nstr + 3) / 4), dim3(256), 0, st, nstr, str, part, E, U);
   return mgk_status();
 }
 
