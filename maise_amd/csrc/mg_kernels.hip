@@ -622,7 +622,8 @@ template <int PAT>
 __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, float cs, float* P, float* Pd) {
 #pragma unroll
   for (int l = 0; l < MG_MAXL; l++) {
-    const float lam = bp->lam[l];
+    // the two shipped tables have lambda = {-1, +1} (checked when the model is loaded): constants here
+    const float lam = Pat<PAT>::ZSTD ? (l == 0 ? -1.0f : 1.0f) : bp->lam[l];
     const float u = 1.0f + lam * cs;
     if (Pat<PAT>::ZSTD) {
       const float u2 = u * u, u4 = u2 * u2, u8 = u4 * u4, u16 = u8 * u8;
@@ -859,7 +860,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
         float fc3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * ia.x * ib.x;
-        float F = 2.0f * ia.y * ib.y * fc3;
+        float F = ia.y * ib.y * fc3;  // the factor 2 of the G4 definition is applied with the zeta prefactor
         float P[MG_MAXZ * MG_MAXL];
         angular<PAT>(bp, cs, P, nullptr);
         float Bq[NE];
@@ -885,8 +886,9 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         gout += np;
         if (lane == 0) npl[(size_t)i * MG_NCLS + cls] = np;
       }
-      // lane l: grid cell l = (e*MAXZ + z)*MAXL + lambda; the zeta prefactor 2^(1 - zeta) goes on here
-      float mine = reduce_scatter64(V, lane) * bp->pref[(lane / MG_MAXL) % MG_MAXZ];
+      // lane l: grid cell l = (e*MAXZ + z)*MAXL + lambda; the zeta prefactor 2^(1 - zeta) and the factor 2 of the
+      // G4 definition go on here
+      float mine = reduce_scatter64(V, lane) * (2.0f * bp->pref[(lane / MG_MAXL) % MG_MAXZ]);
       int cell = lane < bp->ng4 ? bp->g4cell[lane] : 0;
       float v = __shfl(mine, cell, WAVE);
       if (lane < bp->ng4) Gi[blk + lane] = v;
@@ -954,13 +956,13 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 // ---------------------------------------------------------------------------------------------- forces
 // Per atom i: gradient of T_i = sum_n g_n G_n(i) with respect to its neighbour vectors.  Pair terms
 // act along d_ij; a triplet (j, k) with a = d_ij, b = d_ik, c = b - a, cos = a.b/(|a||b|):
-//   T = F * sum_e S_e B_e, F = f1 f2 f3, B_e = 2 exp(-(e_ij r1^2 + e_ik r2^2 + e_jk r3^2)),
-//   S_e = sum_(z,l) g(e,z,l) P_zl(cos), S'_e with P'_zl,
+//   T = F * sum_e S_e B_e, F = f1 f2 f3, B_e = exp(-(e_ij r1^2 + e_ik r2^2 + e_jk r3^2)),
+//   S_e = sum_(z,l) g(e,z,l) P_zl(cos), S'_e with P'_zl (g carries 2^(1 - zeta) and the factor 2 of G4),
 //   dT/dr1 = -2 r1 F sum_e e_ij S_e B_e + f1' f2 f3 sum_e S_e B_e (r2, r3 likewise), dT/dcos = F sum_e S'_e B_e,
 //   dT/da = dT/dr1 a/r1 - dT/dr3 c/r3 + dT/dcos (b/(r1 r2) - cos a/r1^2), dT/db symmetric.
 // Forces: F_j -= dT/da, F_k -= dT/db, F_i += dT/da + dT/db; virial U -= dT/da (x) a + dT/db (x) b.
-// The dE/dG grid g(e,z,l) of the current species block is wave-uniform; it sits in LDS and is read with
-// broadcast loads (kept out of SGPRs, which would spill inside the triplet loop).
+// The dE/dG grid g(e,z,l) of the current species block is wave-uniform; it is staged through LDS once per class
+// and kept in registers for the class's triplets.
 template <int PAT, bool UNI>
 __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                                     const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
@@ -1043,8 +1045,9 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
       const float rinv = 1.0f / rc;
       const float dfk = -0.5f * 3.14159265358979f * rinv;
       {
-        int n = bp->g4map[lane];  // grid cell lane; the zeta prefactor 2^(1 - zeta) is folded in here
-        ((float*)gts4)[lane] = n >= 0 ? gi[blk + n] * bp->pref[(lane / MG_MAXL) % MG_MAXZ] : 0.0f;
+        // grid cell lane, with the zeta prefactor 2^(1 - zeta) and the factor 2 of the G4 definition folded in
+        int n = bp->g4map[lane];
+        ((float*)gts4)[lane] = n >= 0 ? gi[blk + n] * (2.0f * bp->pref[(lane / MG_MAXL) % MG_MAXZ]) : 0.0f;
       }
       wave_sync();
       // the grid of this species block, loop-invariant: kept in registers for the whole class
@@ -1081,8 +1084,8 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
 #pragma unroll
         for (int e = 0; e < NE; e++) {
           const float eta3 = bp->eta4[pjk][e];
-          float B = UNI ? 2.0f * fast_exp2(-(float)MG_LOG2E * eta3 * rsum)
-                        : 2.0f * Aj[e] * Ak[e] * fast_exp2(-(float)MG_LOG2E * eta3 * r3s);
+          float B = UNI ? fast_exp2(-(float)MG_LOG2E * eta3 * rsum)
+                        : Aj[e] * Ak[e] * fast_exp2(-(float)MG_LOG2E * eta3 * r3s);
           float Se = 0.f, Sd = 0.f;
 #pragma unroll
           for (int q = 0; q < ZL; q++)
