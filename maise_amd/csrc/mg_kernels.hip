@@ -214,6 +214,24 @@ __global__ void k_bin_fill(int N, const int* __restrict__ bin, const int* __rest
   brec[p] = r;
 }
 
+// ---------------------------------------------------------------------------------------------- block order, wave sync
+// The 8 XCDs of the chip take workgroups round robin (block b runs on XCD b mod 8), each with its own L2.  Kernels
+// that read the rows of an atom's neighbours remap blocks so every XCD works through one contiguous range of
+// atoms: neighbouring atoms then share their neighbours' rows in that XCD's L2.
+#define MG_NXCD 8
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  const int per = nblk / MG_NXCD, rem = nblk % MG_NXCD;  // XCD x takes per (+1 for x < rem) blocks
+  const int x = b % MG_NXCD, k = b / MG_NXCD;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
+// LDS ordering between the lanes of one wave (several independent waves share a workgroup, so no block barrier)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------------------------- neighbours
 // One wave per atom.  The stencil of (2K0+1)(2K1+1)(2K2+1) bins around the atom's bin (each with the image
 // shift its periodic wrap implies) is processed in chunks of 64 entries: lane l builds entry l (bin start,
@@ -228,25 +246,32 @@ __global__ void k_bin_fill(int N, const int* __restrict__ bin, const int* __rest
 // through row_off and rowcnt).  A row longer than cap_row raises the overflow flag misc[3] and is not written;
 // misc[5] keeps the longest row seen, to size the next capacities.
 #define NBR_CHUNK 64
+#define NBR_WPB 4  // most atoms (waves) per block; XCD-aware block order, so neighbouring atoms share bins in one L2
 template <bool FILL>
-__global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x, const int* __restrict__ spc,
+__global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __restrict__ x, const int* __restrict__ spc,
                                             const int* __restrict__ atom_str, const mg_struct_dev* __restrict__ str,
                                             const int* __restrict__ bin, const int* __restrict__ bin_start,
                                             const BinRec* __restrict__ brec, int* __restrict__ cnt,
                                             int* __restrict__ rowcnt, int* __restrict__ misc,
                                             int* __restrict__ row_off, u64* __restrict__ key,
                                             float4* __restrict__ vec, int* __restrict__ rev, int cap, int padded,
-                                            int cap_row) {
+                                            int cap_row, int wpb) {
 #pragma clang fp contract(off)
   if (FILL && !padded && misc[3]) return;  // rows exceed the capacities this launch was sized for
-  __shared__ int e_start[NBR_CHUNK], e_pre[NBR_CHUNK + 1];
-  __shared__ unsigned e_img[NBR_CHUNK];
-  __shared__ double e_sh[NBR_CHUNK][3];
+  __shared__ int e_start_s[NBR_WPB][NBR_CHUNK], e_pre_s[NBR_WPB][NBR_CHUNK + 1];
+  __shared__ unsigned e_img_s[NBR_WPB][NBR_CHUNK];
+  __shared__ double e_sh_s[NBR_WPB][NBR_CHUNK][3];
   extern __shared__ unsigned char smem_n[];
-  u64* rk = (u64*)smem_n;             // [cap] keys of the row (FILL)
-  float4* rv = (float4*)(rk + cap);   // [cap] vectors
-  const int i = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = xcd_block(blockIdx.x, gridDim.x) * wpb + wv;
+  const int lane = threadIdx.x & 63;
+  if (i >= N) return;
+  int* e_start = e_start_s[wv];
+  int* e_pre = e_pre_s[wv];
+  unsigned* e_img = e_img_s[wv];
+  double (*e_sh)[3] = e_sh_s[wv];
+  u64* rk = (u64*)smem_n + (size_t)wv * 3 * cap;  // [cap] keys of the row (FILL)
+  float4* rv = (float4*)(rk + cap);                // [cap] vectors
   const mg_struct_dev* st = &str[atom_str[i]];
   const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
   const double RC = st->rc2;
@@ -321,7 +346,7 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
     e_pre[lane + 1] = inc;
     if (lane == 0) e_pre[0] = 0;
     const int total = __shfl(inc, WAVE - 1, WAVE);
-    __syncthreads();
+    wave_sync();
     for (int cb = 0; cb < total; cb += WAVE) {
       const int c = cb + lane;
       bool ok = false;
@@ -365,7 +390,7 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
       }
       n += __popcll(bm);
     }
-    __syncthreads();
+    wave_sync();
   }
   if (!FILL || padded) {
     if (lane == 0) {
@@ -382,7 +407,7 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
     if (!FILL || n > cap_row) return;  // count pass done, or overflow: nothing of this row is written
   }
   const int M = n < cap ? n : cap;  // n <= cap by construction (cap >= longest row)
-  __syncthreads();
+  wave_sync();
   // rank sort: the keys of a row are distinct, so the number of smaller keys is an entry's place in the sorted row;
   // every lane counts against broadcast reads of the whole row and writes its entries to their places
   const int off = padded ? i * cap_row : row_off[i];
@@ -402,23 +427,6 @@ __global__ __launch_bounds__(64) void k_nbr(int N, const double* __restrict__ x,
 }
 
 // Reverse edge of e = (i -> j, img): the entry (j -> i, -img) of row j, found by binary search on keys.
-// The 8 XCDs of the chip take workgroups round robin (block b runs on XCD b mod 8), each with its own L2.  Kernels
-// that read the rows of an atom's neighbours remap blocks so every XCD works through one contiguous range of
-// atoms: neighbouring atoms then share their neighbours' rows in that XCD's L2.
-#define MG_NXCD 8
-__device__ __forceinline__ int xcd_block(int b, int nblk) {
-  const int per = nblk / MG_NXCD, rem = nblk % MG_NXCD;  // XCD x takes per (+1 for x < rem) blocks
-  const int x = b % MG_NXCD, k = b / MG_NXCD;
-  return x * per + (x < rem ? x : rem) + k;
-}
-
-// LDS ordering between the lanes of one wave (several independent waves share a workgroup, so no block barrier)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Edges come in mutual pairs e = (i -> j, img), e' = (j -> i, -img), with rev[e] = e' and rev[e'] = e, so only
 // the edge of each pair with the smaller (i, img) looks the other up and writes both.  One wave per atom, four atoms
 // per block: the wave compacts the slots it owns into LDS, then each lane looks up two of them at once with
@@ -1287,9 +1295,9 @@ int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_co
 int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
                   const int* bin_start, const void* brec, int* cnt, int* rowcnt, int* misc, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_nbr<false>, dim3(N), dim3(64), 0, st, N, x, spc, atom_str, str, bin, bin_start,
-                     (const BinRec*)brec, cnt, rowcnt, misc, (int*)nullptr, (u64*)nullptr, (float4*)nullptr,
-                     (int*)nullptr, 0, 0, 0);
+  hipLaunchKernelGGL(k_nbr<false>, dim3((N + NBR_WPB - 1) / NBR_WPB), dim3(64 * NBR_WPB), 0, st, N, x, spc, atom_str, str,
+                     bin, bin_start, (const BinRec*)brec, cnt, rowcnt, misc, (int*)nullptr, (u64*)nullptr,
+                     (float4*)nullptr, (int*)nullptr, 0, 0, 0, NBR_WPB);
   return mgk_status();
 }
 
@@ -1302,9 +1310,16 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
   int cap = 64;
   while (cap < maxrow) cap <<= 1;
   if (cap > 2048) return -2;
-  size_t sm = (size_t)cap * (sizeof(u64) + sizeof(float4));
-  hipLaunchKernelGGL(k_nbr<true>, dim3(N), dim3(64), sm, st, N, x, spc, atom_str, str, bin, bin_start, (const BinRec*)brec,
-                     cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev, cap, padded, maxrow);
+  // per wave: keys + vectors (3 u64 per slot); fewer waves per block when long rows would not fit 64 KiB
+  int wpb = NBR_WPB;
+  while (wpb > 1 && (size_t)wpb * cap * (sizeof(u64) + sizeof(float4)) > 64 * 1024) wpb >>= 1;
+  const size_t sm = (size_t)wpb * cap * (sizeof(u64) + sizeof(float4));
+  if (sm > 64 * 1024 &&
+      hipFuncSetAttribute((const void*)&k_nbr<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(k_nbr<true>, dim3((N + wpb - 1) / wpb), dim3(64 * wpb), sm, st, N, x, spc, atom_str,
+                     str, bin, bin_start, (const BinRec*)brec,
+                     cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev, cap, padded, maxrow, wpb);
   hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, (const int*)row_off, (const int*)rowcnt,
                      (const u64*)key, rev, misc + 1, (const int*)(misc + 3));
   return mgk_status();
