@@ -102,28 +102,32 @@ __global__ __launch_bounds__(256) void k_scan_add(int* __restrict__ out, int n, 
   if (b == 0 && threadIdx.x == 0) out[0] = 0;
 }
 
-// exclusive scan of up to 1024 * SCAN1_PT values in one block: thread t owns a contiguous run of pt values
+// exclusive scan of up to 1024 * SCAN1_PT values in one block, in coalesced tiles of 1024: a shuffle scan in each
+// wave, the 16 wave totals scanned in LDS, a running carry between tiles
 #define SCAN1_PT 64
-__global__ __launch_bounds__(1024) void k_scan_one(const int* __restrict__ in, int* __restrict__ out, int n, int pt) {
-  __shared__ int ts[1024];
-  const int t = threadIdx.x, a0 = t * pt;
-  int run = 0;
-  for (int q = 0; q < pt; q++) run += a0 + q < n ? in[a0 + q] : 0;
-  ts[t] = run;
+__global__ __launch_bounds__(1024) void k_scan_one(const int* __restrict__ in, int* __restrict__ out, int n) {
+  __shared__ int wsum[16];
+  __shared__ int carry_s;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) { carry_s = 0; out[0] = 0; }
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan of the thread sums
-    int a = t >= o ? ts[t - o] : 0;
+  for (int c0 = 0; c0 < n; c0 += 1024) {
+    const int k = c0 + t;
+    int v = k < n ? in[k] : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(v, o, 64);
+      if (lane >= o) v += y;
+    }
+    if (lane == 63) wsum[w] = v;
     __syncthreads();
-    ts[t] += a;
+    int off = carry_s;
+    for (int q = 0; q < w; q++) off += wsum[q];
+    if (k < n) out[k + 1] = off + v;
+    __syncthreads();
+    if (t == 1023) carry_s = off + v;
     __syncthreads();
   }
-  int acc = ts[t] - run;
-  for (int q = 0; q < pt; q++)
-    if (a0 + q < n) {
-      acc += in[a0 + q];
-      out[a0 + q + 1] = acc;
-    }
-  if (t == 0) out[0] = 0;
 }
 
 __global__ void k_clear3(int* __restrict__ a, int na, int* __restrict__ b, int nb, int* __restrict__ c, int nc) {
@@ -1254,7 +1258,7 @@ int mgk_scan(const int* in, int* out, int n, void* tmp, size_t* tmp_bytes, hipSt
     return mgk_status();
   }
   if (n <= 1024 * SCAN1_PT) {  // one launch
-    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, st, in, out, n, (n + 1023) / 1024);
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, st, in, out, n);
     return mgk_status();
   }
   hipLaunchKernelGGL(k_scan_tile, dim3(nb), dim3(256), 0, st, in, out, n, bsum, 0);
