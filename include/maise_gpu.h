@@ -157,6 +157,14 @@ int mg_md_init(mg_ctx *c, double T, unsigned long long seed);
 int mg_md_set_velocities(mg_ctx *c, const double *v);
 int mg_md_get_velocities(mg_ctx *c, double *v);
 int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *epot, double *ekin);
+/* NPT (tau > 0) or isobaric NVE (tau = 0) steps: as mg_md_run plus the reference's Berendsen barostat
+ * (MDTP 3 / 4, src/cmod.c:371, 408-411, 457-468; CELL_PRS src/cmod.c:552-561).  After every force evaluation the
+ * lattice scales by lambda = (1 - (ktt dt / 3)(p - p_int))^(1/3), p_int = (U_xx + U_yy + U_zz)/(3V), and the
+ * positions by the same factor at the next drift (all atoms; the reference's loop rescales only the first three).
+ * p_gpa: target pressure (GPa); ktt = ICMP / CPLP (1/(GPa fs), the reference's default 0.005 / 50).  Optional
+ * traces per step: epot, ekin (eV), vol (A^3), pint_gpa (GPa).  Periodic single structure only. */
+int mg_md_run_npt(mg_ctx *c, int nsteps, double dt, double T, double tau, double p_gpa, double ktt, double *epot,
+                  double *ekin, double *vol, double *pint_gpa);
 /* Batched FIRE relaxation of every structure of the context (positions only), replacing the CELL_RELX /
  * CELL_MIN caller (src/cmod.c:901-977, src/cmin.c:177-301; the reference minimiser is GSL BFGS2).  A structure
  * stops when max_i |F_i| < fmax (eV/A); dt0/dtmax are FIRE's initial and maximum step, maxstep (A) caps the

@@ -68,6 +68,8 @@ def lib():
             "mg_md_set_velocities": ([vp, dp], C.c_int),
             "mg_md_get_velocities": ([vp, dp], C.c_int),
             "mg_md_run": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp], C.c_int),
+            "mg_md_run_npt": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp],
+                              C.c_int),
             "mg_relax": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, ip, ip], C.c_int),
             "mg_relax_cell": ([vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                ip, ip], C.c_int),
@@ -294,6 +296,16 @@ class Context:
         ep, ek = np.zeros(max(nsteps, 1)), np.zeros(max(nsteps, 1))
         _check(lib().mg_md_run(self._h, int(nsteps), float(dt), float(T), float(tau), _dp(ep), _dp(ek)), "mg_md_run")
         return ep[:nsteps], ek[:nsteps]
+
+    def md_run_npt(self, nsteps: int, dt: float = 1.0, T: float = 300.0, tau: float = 100.0, p_gpa: float = 0.0,
+                   ktt: float = 0.005 / 50.0):
+        """NPT (tau > 0) or isobaric NVE (tau <= 0) with the Berendsen barostat (ktt = compressibility / coupling
+        time, 1/(GPa fs)); returns per-step (Epot eV, Ekin eV, volume A^3, internal pressure GPa)."""
+        n = max(nsteps, 1)
+        ep, ek, vol, pi = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().mg_md_run_npt(self._h, int(nsteps), float(dt), float(T), float(tau), float(p_gpa), float(ktt),
+                                   _dp(ep), _dp(ek), _dp(vol), _dp(pi)), "mg_md_run_npt")
+        return ep[:nsteps], ek[:nsteps], vol[:nsteps], pi[:nsteps]
 
     def features(self) -> np.ndarray:
         """Per-structure surrogate inputs of the last evaluation: [S, nspc * (ng2 + ng4)] species means of the

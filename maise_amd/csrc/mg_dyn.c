@@ -28,11 +28,12 @@ int mgk_md_fix(int N, const double *mass, double *v, double kT, double ndof, hip
 int mgk_md_thermo(int N, const double *mass, const double *v, double *dyn, double dt, double kT, double ndof, double tau,
                   double *trace, int slot, hipStream_t st);
 int mgk_md_kick_drift(int N, double dt, const double *mass, const double *F, double *v, const double *x, double *xin,
-                      const double *dyn, hipStream_t st);
+                      const double *dyn, int baro, hipStream_t st);
+int mgk_md_baro(mg_struct_dev *str, const double *U, double *dyn, double p, double ktt_dt, hipStream_t st);
 int mgk_md_kick_scale(int N, double dt, const double *mass, const double *F, double *v, const double *dyn, double tau,
                       hipStream_t st);
 int mgk_copy_d(const double *src, double *dst, int n, hipStream_t st);
-int mgk_md_record(const double *E, double *dyn, double *trace, int nsteps, hipStream_t st);
+int mgk_md_record(const double *E, double *dyn, double *trace, int nsteps, int baro, hipStream_t st);
 int mgk_fire_init(int S, double *state, double dt0, hipStream_t st);
 int mgk_fire(int S, const mg_struct_dev *str, const double *F, double *v, const double *x, double *xin, double *state,
              double fmax, double dtmax, double maxstep, hipStream_t st);
@@ -122,13 +123,15 @@ int mg_md_get_velocities(mg_ctx *c, double *v) {
 typedef struct {
   double dt, kT, ndof, tau;
   int nsteps;
+  int baro;          /* Berendsen barostat on */
+  double p, ktt_dt;  /* target pressure (eV/A^3), coupling x dt (1/GPa) */
 } md_arg;
 
 /* velocity Verlet with the reference's Nose-Hoover xi equation, split around the force evaluation */
 static int md_head(mg_ctx *c, void *arg) {
   const md_arg *a = (const md_arg *)arg;
   KCHK(mgk_md_thermo(c->N, c->d_mass, c->d_v, c->d_dyn, a->dt, a->kT, a->ndof, a->tau, NULL, -1, c->st));
-  KCHK(mgk_md_kick_drift(c->N, a->dt, c->d_mass, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_dyn, c->st));
+  KCHK(mgk_md_kick_drift(c->N, a->dt, c->d_mass, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_dyn, a->baro, c->st));
   return MG_OK;
 }
 
@@ -136,7 +139,8 @@ static int md_tail(mg_ctx *c, void *arg) {
   const md_arg *a = (const md_arg *)arg;
   KCHK(mgk_md_kick_scale(c->N, a->dt, c->d_mass, c->d_F, c->d_v, c->d_dyn, a->tau, c->st));
   KCHK(mgk_md_thermo(c->N, c->d_mass, c->d_v, c->d_dyn, a->dt, a->kT, a->ndof, a->tau, NULL, -1, c->st));
-  KCHK(mgk_md_record(c->d_E, c->d_dyn, c->d_trace, a->nsteps, c->st));  /* step counter dyn[3] */
+  if (a->baro) KCHK(mgk_md_baro(c->d_str, c->d_U, c->d_dyn, a->p, a->ktt_dt, c->st));
+  KCHK(mgk_md_record(c->d_E, c->d_dyn, c->d_trace, a->nsteps, a->baro, c->st));  /* step counter dyn[3] */
   return MG_OK;
 }
 
@@ -152,21 +156,28 @@ static int md_step_plain(mg_ctx *c, md_arg *a) {
   return md_tail(c, a);
 }
 
-/* NVT steps.  With graphs enabled the first step runs synchronised (sizing the buffers), then one step is captured
- * and replayed MG_GRAPH_CHECK steps at a time; after each block the host reads the overflow flag and, when a row or
+/* NVT (tau > 0) or NVE steps, optionally with the Berendsen barostat (NPT / isobaric).  With graphs enabled the
+ * first step runs synchronised (sizing the buffers), then one step is captured and replayed MG_GRAPH_CHECK steps at
+ * a time; after each block the host reads the overflow flag and, when a row or
  * the edge total outgrew the capture, restores the checkpoint taken before the block, evaluates synchronised and
  * captures again, so the trajectory does not depend on the buffer sizes. */
-int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *epot, double *ekin) {
+static int md_impl(mg_ctx *c, int nsteps, double dt, double T, double tau, int baro, double p, double ktt, double *epot,
+                   double *ekin, double *vol, double *pint) {
   if (!c || c->S != 1 || nsteps < 0 || dt <= 0.0) return mg_fail(MG_ERR_ARG, "mg_md_run: bad argument");
   if (!c->d_v) return mg_fail(MG_ERR_ARG, "mg_md_run: call mg_md_init or mg_md_set_velocities first");
   HIPCHK(hipSetDevice(c->device));
   { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
   int r;
-  if ((size_t)(2 * nsteps + 2) > c->cap_trace) {
-    if ((r = mg_dev_alloc((void **)&c->d_trace, sizeof(double) * (2 * nsteps + 2)))) return r;
-    c->cap_trace = 2 * nsteps + 2;
+  const size_t ntr = (size_t)(baro ? 4 : 2) * nsteps + 2;
+  if (ntr > c->cap_trace) {
+    if ((r = mg_dev_alloc((void **)&c->d_trace, sizeof(double) * ntr))) return r;
+    c->cap_trace = ntr;
   }
-  md_arg a = {dt, MG_KB * T, 3.0 * c->N, tau, nsteps};
+  md_arg a = {dt, MG_KB * T, 3.0 * c->N, tau, nsteps, baro, p, ktt * dt};
+  if (baro) {  /* lambda = 1 until the first evaluation has set it */
+    const double one = 1.0;
+    HIPCHK(hipMemcpy(c->d_dyn + 4, &one, sizeof(double), hipMemcpyHostToDevice));
+  }
   /* forces at the starting positions; step counter to 0 */
   if ((r = mgh_eval(c, MG_EVAL_FORCES, 0))) return r;
   HIPCHK(hipMemsetAsync(c->d_dyn + 3, 0, sizeof(double), c->st));
@@ -212,13 +223,33 @@ int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *ep
   }
   if (nsteps > 0 && epot) MHIP(hipMemcpy(epot, c->d_trace, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
   if (nsteps > 0 && ekin) MHIP(hipMemcpy(ekin, c->d_trace + nsteps, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
+  if (baro && nsteps > 0 && vol) MHIP(hipMemcpy(vol, c->d_trace + 2 * nsteps, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
+  if (baro && nsteps > 0 && pint) MHIP(hipMemcpy(pint, c->d_trace + 3 * nsteps, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
   MHIP(hipStreamSynchronize(c->st));
+  if (baro) {  /* the lattice moved on the device: bring the host tables up to date */
+    MHIP(hipMemcpy(c->h_str, c->d_str, sizeof(mg_struct_dev), hipMemcpyDeviceToHost));
+    memcpy(c->h_lat, c->h_str[0].L, 9 * sizeof(double));
+  }
   r = MG_OK;
 out:
   if (gx) hipGraphExecDestroy(gx);
   return r;
 #undef MCHK
 #undef MHIP
+}
+
+int mg_md_run(mg_ctx *c, int nsteps, double dt, double T, double tau, double *epot, double *ekin) {
+  return md_impl(c, nsteps, dt, T, tau, 0, 0.0, 0.0, epot, ekin, NULL, NULL);
+}
+
+int mg_md_run_npt(mg_ctx *c, int nsteps, double dt, double T, double tau, double p_gpa, double ktt, double *epot,
+                  double *ekin, double *vol, double *pint_gpa) {
+  const double ev2gpa = 160.2176565; /* eV/A^3 in GPa, the reference's eV2GPa */
+  if (!c || ktt < 0.0 || (c->S == 1 && c->h_str[0].nd != 3)) return mg_fail(MG_ERR_ARG, "mg_md_run_npt: bad argument (periodic cell and ktt >= 0 needed)");
+  int r = md_impl(c, nsteps, dt, T, tau, 1, p_gpa / ev2gpa, ktt, epot, ekin, vol, pint_gpa);
+  if (r == MG_OK && pint_gpa)
+    for (int s = 0; s < nsteps; s++) pint_gpa[s] *= ev2gpa;
+  return r;
 }
 
 /* Batched FIRE.  Every 8 iterations the converged structures are counted on the device; once the atoms of the

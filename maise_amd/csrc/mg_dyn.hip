@@ -94,18 +94,49 @@ __global__ __launch_bounds__(1024) void k_md_thermo(int N, const double* __restr
   }
 }
 
-// v <- v s + dt/2 F/m ; x_next <- x + dt v
+// v <- v s + dt/2 F/m ; x_next <- lambda (x + dt v), lambda = dyn[4] with the barostat on (the lattice was scaled
+// by the same factor when lambda was set), else 1
 __global__ void k_md_kick_drift(int N, double dt, const double* __restrict__ mass, const double* __restrict__ F,
                                 double* __restrict__ v, const double* __restrict__ x, double* __restrict__ xin,
-                                const double* __restrict__ dyn) {
+                                const double* __restrict__ dyn, int baro) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double s = dyn[2], h = 0.5 * dt / mass[i];
+  const double s = dyn[2], h = 0.5 * dt / mass[i], lam = baro ? dyn[4] : 1.0;
   for (int q = 0; q < 3; q++) {
     double vv = v[3 * i + q] * s + h * F[3 * i + q];
     v[3 * i + q] = vv;
-    xin[3 * i + q] = x[3 * i + q] + dt * vv;
+    xin[3 * i + q] = baro ? lam * (x[3 * i + q] + dt * vv) : x[3 * i + q] + dt * vv;
   }
+}
+
+// Berendsen barostat of the reference's NPT and isobaric runs (src/cmod.c:371, 408-411, 457-468): after a force
+// evaluation, lambda = (1 - (ktt dt / 3)(p - p_int) eV2GPa)^(1/3) with the internal pressure
+// p_int = (U_xx + U_yy + U_zz) / (3 V) (CELL_PRS, src/cmod.c:552-561; p, p_int in eV/A^3, ktt in 1/(GPa fs)).  The
+// lattice rows scale by lambda now (reciprocal rows by 1/lambda, the bin stencil follows) and every position at the
+// next drift (the reference rescales only X[0..2], src/cmod.c:462-465; all atoms scale here).  The argument is
+// kept >= 1/8 (lambda >= 1/2 per step).  dyn[4] = lambda, dyn[5] = p_int, dyn[6] = new volume.
+__global__ void k_md_baro(mg_struct_dev* __restrict__ str, const double* __restrict__ U, double* __restrict__ dyn,
+                          double p, double ktt_dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  mg_struct_dev& d = str[0];
+  const double* L = d.L;
+  const double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
+                     L[2] * (L[3] * L[7] - L[4] * L[6]);
+  const double V = fabs(det);
+  const double pint = (U[0] + U[1] + U[2]) / (3.0 * V);
+  const double arg = fmax(1.0 - (ktt_dt / 3.0) * (p - pint) * 160.2176565, 0.125);
+  const double lam = cbrt(arg);
+  for (int k = 0; k < 9; k++) { d.L[k] *= lam; d.R[k] /= lam; }
+  const double rc = sqrt(d.rc2);
+  for (int q = 0; q < 3; q++) {
+    const double h = rc * sqrt(d.R[3 * q] * d.R[3 * q] + d.R[3 * q + 1] * d.R[3 * q + 1] + d.R[3 * q + 2] * d.R[3 * q + 2]) *
+                     (1.0 + 1e-6);
+    const int K = (int)ceil(h * d.nb[q]);
+    d.K[q] = K < 1 ? 1 : K;
+  }
+  dyn[4] = lam;
+  dyn[5] = pint;
+  dyn[6] = V * lam * lam * lam;
 }
 
 // v <- (v + dt/2 F/m) exp(-xi dt/2)
@@ -117,14 +148,19 @@ __global__ void k_md_kick_scale(int N, double dt, const double* __restrict__ mas
   for (int q = 0; q < 3; q++) v[3 * i + q] = (v[3 * i + q] + h * F[3 * i + q]) * s;
 }
 
-// end of an MD step replayed from a graph: the step index lives in dyn[3]; trace[s] = E, trace[nsteps + s] = K
+// end of an MD step replayed from a graph: the step index lives in dyn[3]; trace[s] = E, trace[nsteps + s] = K and,
+// with the barostat, trace[2 nsteps + s] = volume, trace[3 nsteps + s] = internal pressure
 __global__ void k_md_record(const double* __restrict__ E, double* __restrict__ dyn, double* __restrict__ trace,
-                            int nsteps) {
+                            int nsteps, int baro) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int s = (int)dyn[3];
   if (s >= 0 && s < nsteps) {
     trace[s] = E[0];
     trace[nsteps + s] = dyn[1];
+    if (baro) {
+      trace[2 * nsteps + s] = dyn[6];
+      trace[3 * nsteps + s] = dyn[5];
+    }
   }
   dyn[3] = (double)(s + 1);
 }
@@ -403,8 +439,13 @@ int mgk_md_thermo(int N, const double* mass, const double* v, double* dyn, doubl
 }
 
 int mgk_md_kick_drift(int N, double dt, const double* mass, const double* F, double* v, const double* x, double* xin,
-                      const double* dyn, hipStream_t st) {
-  hipLaunchKernelGGL(k_md_kick_drift, dim3((N + 255) / 256), dim3(256), 0, st, N, dt, mass, F, v, x, xin, dyn);
+                      const double* dyn, int baro, hipStream_t st) {
+  hipLaunchKernelGGL(k_md_kick_drift, dim3((N + 255) / 256), dim3(256), 0, st, N, dt, mass, F, v, x, xin, dyn, baro);
+  return mgk_status();
+}
+
+int mgk_md_baro(mg_struct_dev* str, const double* U, double* dyn, double p, double ktt_dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_md_baro, dim3(1), dim3(64), 0, st, str, U, dyn, p, ktt_dt);
   return mgk_status();
 }
 
@@ -414,8 +455,8 @@ int mgk_md_kick_scale(int N, double dt, const double* mass, const double* F, dou
   return mgk_status();
 }
 
-int mgk_md_record(const double* E, double* dyn, double* trace, int nsteps, hipStream_t st) {
-  hipLaunchKernelGGL(k_md_record, dim3(1), dim3(64), 0, st, E, dyn, trace, nsteps);
+int mgk_md_record(const double* E, double* dyn, double* trace, int nsteps, int baro, hipStream_t st) {
+  hipLaunchKernelGGL(k_md_record, dim3(1), dim3(64), 0, st, E, dyn, trace, nsteps, baro);
   return mgk_status();
 }
 

@@ -1,8 +1,9 @@
-"""Device-resident dynamics: NVT/NVE molecular dynamics and batched FIRE relaxation.
+"""Device-resident dynamics: NVT/NVE/NPT molecular dynamics and batched FIRE relaxation.
 
 Trajectory parity with the reference is not a bar (its NVT integrator differs and is seeded from the clock,
 src/cmod.c:224-453; its minimiser is GSL BFGS2, absent here).  Checked instead: NVE conservation of Epot + Ekin,
-the thermostat holding the target temperature, Maxwell-Boltzmann initialisation (zero momentum, exact T), and
+the thermostat holding the target temperature, Maxwell-Boltzmann initialisation (zero momentum, exact T), the
+barostat's lambda and its drift to the p = 0 volume of the cell relaxation, and
 for FIRE that every structure ends below the force threshold with the FP64 oracle agreeing at the relaxed
 positions, and that batched results equal one-at-a-time results.
 """
@@ -77,6 +78,90 @@ def test_md_graph_replay_matches_synchronised_loop():
         for a, b in zip(out[0], out[k]):
             assert np.array_equal(a, b)
     assert np.isfinite(out[0][0]).all()
+
+
+EV2GPA = 160.2176565
+
+
+def test_npt_zero_coupling_equals_nvt():
+    """With ktt = 0 the barostat factor is exactly 1: the NPT run reproduces the NVT run bit for bit."""
+    import maise_amd
+    from maise_amd import workloads as wl
+    print("[gpu case] NPT ktt=0 vs NVT, Cu 108 atoms", flush=True)
+    m = maise_amd.Model(os.path.join(GOLDEN, "models", "nn_Cu_d3_v1"))
+    s, l, x = wl.cu_fcc(3, 3.61, 0.02, seed=5)
+    out = []
+    for npt in (False, True):
+        c = maise_amd.Context(m)
+        c.set_structure(s, l, x)
+        c.md_init(300.0, seed=3)
+        if npt:
+            ep, ek, vol, pint = c.md_run_npt(40, dt=1.0, T=300.0, tau=50.0, p_gpa=0.0, ktt=0.0)
+            assert np.all(vol == vol[0])
+        else:
+            ep, ek = c.md_run(40, dt=1.0, T=300.0, tau=50.0)
+        out.append((ep, ek, c.result(0)["X"], c.lattice()))
+        c.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+def test_npt_one_step_lambda():
+    """One NPT step scales the lattice by lambda = (1 - (ktt dt/3)(p - p_int))^(1/3) (GPa), with the internal
+    pressure p_int = tr(U)/(3V) of the evaluation after the first drift (1e-4 A from the start at 1 K: checked
+    against the starting structure's tr(U)/(3V) to 1e-3) in a compressed cell, p_int > 0."""
+    import maise_amd
+    from maise_amd import workloads as wl
+    print("[gpu case] NPT one step, Cu 108 atoms at a = 3.50", flush=True)
+    m = maise_amd.Model(os.path.join(GOLDEN, "models", "nn_Cu_d3_v1"))
+    s, l, x = wl.cu_fcc(3, 3.50, 0.02, seed=5)
+    c = maise_amd.Context(m)
+    c.set_structure(s, l, x)
+    r0 = c.evaluate()
+    V = abs(np.linalg.det(l))
+    pint = (r0["U"][0] + r0["U"][1] + r0["U"][2]) / (3.0 * V)
+    ktt, dt, p_gpa = 2e-3, 1.0, 1.5
+    assert pint * EV2GPA > 5.0  # compressed: the barostat expands
+    c.md_init(1.0, seed=2)
+    ep, ek, vol, pg = c.md_run_npt(1, dt=dt, T=1.0, tau=0.0, p_gpa=p_gpa, ktt=ktt)
+    assert pg[0] == pytest.approx(pint * EV2GPA, rel=1e-3)
+    lam = (1.0 - ktt * dt / 3.0 * (p_gpa - pg[0])) ** (1.0 / 3.0)
+    assert lam > 1.0
+    assert vol[0] == pytest.approx(V * lam ** 3, rel=1e-12)
+    np.testing.assert_allclose(c.lattice(), np.asarray(l) * lam, rtol=1e-13, atol=1e-13)
+    c.close()
+
+
+@pytest.mark.parametrize("a0", [3.50, 3.75])
+def test_npt_relaxes_toward_equilibrium_volume(a0):
+    """Berendsen NPT at p = 0, 300 K from a compressed and an expanded Cu cell: the volume moves toward the p = 0
+    volume of the cell relaxation (within 3% over the last third, thermal expansion included) and the internal
+    pressure averages near zero there."""
+    import maise_amd
+    from maise_amd import workloads as wl
+    print(f"[gpu case] NPT Cu 108 atoms from a = {a0}", flush=True)
+    m = maise_amd.Model(os.path.join(GOLDEN, "models", "nn_Cu_d3_v1"))
+    s, l, x = wl.cu_fcc(3, 3.61, 0.0, seed=5)
+    cr = maise_amd.Context(m)
+    cr.set_structure(s, l, x)
+    cr.relax_cell(p_gpa=0.0, max_steps=3000, fmax=1e-3, smax_gpa=1e-3)
+    V0 = abs(np.linalg.det(cr.lattice())) / len(s)
+    cr.close()
+    s, l, x = wl.cu_fcc(3, a0, 0.02, seed=5)
+    c = maise_amd.Context(m)
+    c.set_structure(s, l, x)
+    c.md_init(300.0, seed=4)
+    ep, ek, vol, pg = c.md_run_npt(3000, dt=2.0, T=300.0, tau=50.0, p_gpa=0.0, ktt=2e-4)
+    v = vol / len(s)
+    assert np.isfinite(ep).all() and np.isfinite(v).all()
+    tail = v[2000:]
+    assert abs(tail.mean() / V0 - 1.0) < 0.03, (tail.mean(), V0)
+    assert abs(pg[2000:].mean()) < 1.5, pg[2000:].mean()
+    if a0 < 3.6:
+        assert tail.mean() > v[0]
+    else:
+        assert tail.mean() < v[0]
+    c.close()
 
 
 def test_fire_batch_relaxes_and_matches_oracle():
