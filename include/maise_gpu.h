@@ -32,6 +32,8 @@ typedef struct mg_ctx mg_ctx;
 #define MG_ERR_HIP -3
 #define MG_ERR_NEIGHBOR -4   /* neighbour capacity exceeded / inconsistent list */
 #define MG_ERR_UNSUPPORTED -5
+#define MG_ERR_IO -6          /* a file could not be opened, read or written */
+#define MG_ERR_ALLOC -7       /* host memory */
 
 /* mg_eval flags */
 #define MG_EVAL_FORCES 1     /* compute forces and the virial as well as energies */
@@ -51,6 +53,9 @@ void mg_model_free(mg_model *m);
 /* info[0..15] = nspc, D, H1, H2, nsym, ng2, ng4, nw, spcz[0..3], G4 kernel variant (0 generic, 1 the 43-function
  * table, 2 the 22-function table), shared-eta flag, MFMA MLP flag, 0; *rc = neighbour-list cutoff (A). */
 int mg_model_info(const mg_model *m, int *info, double *rc);
+/* The model file's unique ID ("|  model unique ID |", read as READ_ANN does, src/nutl.c:530), as OUTCAR headers
+ * print it; empty when the file has none. */
+int mg_model_id(const mg_model *m, char *buf, int len);
 
 /* --- evaluation context (one device stream, one structure or a batch of structures) --- */
 int mg_ctx_create(const mg_model *m, int device, mg_ctx **out);
@@ -179,6 +184,34 @@ int mg_relax_cell(mg_ctx *c, double p_gpa, int max_steps, double fmax, double sm
                   double maxstep, int *steps, int *done);
 /* Lattice (rows, A) of structure s as last set or relaxed. */
 int mg_get_lattice(mg_ctx *c, int s, double *lat);
+
+/* ---- structure and output files (host only; no device needed) -------------------------------------------------
+ * POSCAR/CONTCAR read (READ_CELL, src/util.c:820-953): title (first line, up to 199 characters; buffer >= 200),
+ * lattice rows lat[9] scaled by the file's factor (A), species (symbols required, as in the reference; spcz[10]
+ * atomic numbers, spcn[10] counts), natoms; with cap >= natoms also Cartesian positions x[3 natoms] (direct
+ * coordinates converted with the lattice rows; Cartesian ones taken as written, unscaled, as the reference does;
+ * clusters, nd = 0, centred on the origin as CENTER(C, 0.0)), selective-dynamics flags ff[3 natoms] (1 = free; may
+ * be NULL) and velocities v[3 natoms] when every atom has one (*has_v; v may be NULL).  cap < natoms reads the
+ * header and the count only. */
+int mg_poscar_read(const char *path, int nd, int cap, char *title, double *lat, int *nspc, int *spcz, int *spcn,
+                   int *natoms, double *x, int *ff, double *v, int *has_v);
+/* CONTCAR write in the reference's format (SAVE_CELL, src/util.c:955-1030): lattice divided by scale, species symbols,
+ * counts, "Selective dynamics" when sd, direct coordinates (clusters, nd = 0, centred in the box as CENTER(C, 0.5))
+ * with T/F columns when sd or any flag in ff is 0, then velocities when v is given.  Atoms in species order. */
+int mg_poscar_write(const char *path, const char *title, double scale, const double *lat, int nd, int nspc,
+                    const int *spcz, const int *spcn, int natoms, const double *x, const int *ff, int sd,
+                    const double *v);
+/* One optimisation/dynamics iteration appended to dir/OUTCAR and dir/OSZICAR (CELL_OUT, src/cmod.c:618-687): the
+ * run header when it == 0 (version, model id, optimizer and relaxation types, pressure, dimensionality, species);
+ * the lattice with the volume per atom (CELL_VOL; clusters: NP_VOL, the inertia-tensor volume); with detail,
+ * positions, forces and atom energies and the stress U/V (eV/A^3 and kB); the enthalpy/energy line; OSZICAR gets H. */
+int mg_outcar_write(const char *dir, int it, const char *version, const char *model_id, int mint, int rlxt,
+                    double p_gpa, int nd, int nspc, const int *spcz, const int *spcn, int natoms, const double *lat,
+                    const double *x, const double *F, const double *EA, const double *U, double E, double H,
+                    int detail);
+/* Element symbol of atomic number z (1..100; "--" otherwise) and the atomic number of a symbol (0 if unknown). */
+const char *mg_element_symbol(int z);
+int mg_element_z(const char *symbol);
 
 #ifdef __cplusplus
 }

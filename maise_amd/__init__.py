@@ -47,6 +47,7 @@ def lib():
             "mg_model_load": ([C.c_char_p, C.POINTER(vp)], C.c_int),
             "mg_model_free": ([vp], None),
             "mg_model_info": ([vp, ip, dp], C.c_int),
+            "mg_model_id": ([vp, C.c_char_p, C.c_int], C.c_int),
             "mg_ctx_create": ([vp, C.c_int, C.POINTER(vp)], C.c_int),
             "mg_ctx_free": ([vp], None),
             "mg_ctx_set_urep": ([vp, C.c_double], C.c_int),
@@ -87,6 +88,15 @@ def lib():
             "mg_evo_offspring": ([C.c_int, C.c_int, C.c_int, ip, dp, dp, dp, C.c_int, C.c_double, C.c_double,
                                   C.c_ulonglong, C.c_int, ip, dp, dp, ip], C.c_int),
             "mg_evo_min_distance": ([dp, dp, C.c_int], C.c_double),
+            "mg_poscar_read": ([C.c_char_p, C.c_int, C.c_int, C.c_char_p, dp, ip, ip, ip, ip, dp, ip, dp, ip],
+                               C.c_int),
+            "mg_poscar_write": ([C.c_char_p, C.c_char_p, C.c_double, dp, C.c_int, C.c_int, ip, ip, C.c_int, dp, ip,
+                                 C.c_int, dp], C.c_int),
+            "mg_outcar_write": ([C.c_char_p, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                 C.c_int, ip, ip, C.c_int, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int],
+                                C.c_int),
+            "mg_element_symbol": ([C.c_int], C.c_char_p),
+            "mg_element_z": ([C.c_char_p], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -128,6 +138,9 @@ class Model:
         self.g4_variant, self.shared_eta, self.mfma = int(info[12]), bool(info[13]), bool(info[14])
         self.rc = rc.value
         self.path = path
+        mid = C.create_string_buffer(64)
+        _check(lib().mg_model_id(self._h, mid, 64), "mg_model_id")
+        self.id = mid.value.decode()
 
     def __del__(self):
         try:
@@ -343,3 +356,4 @@ class Context:
 def call_maise_symbol():
     """The exported drop-in entry point (for ABI tests)."""
     return lib().CALL_MAISE
+

@@ -24,7 +24,7 @@ CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wno-un
             f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
 HIP_SOURCES = ["mg_kernels.hip", "mg_dyn.hip", "mg_sur.hip"]
-C_SOURCES = ["mg_model.c", "mg_api.c", "mg_dyn.c", "mg_sur.c", "mg_evo.c"]
+C_SOURCES = ["mg_model.c", "mg_api.c", "mg_dyn.c", "mg_sur.c", "mg_evo.c", "mg_io.c"]
 
 
 def _run(cmd, verbose):

@@ -276,7 +276,7 @@ int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out) {
 
 /* reciprocal rows R = (L1xL2, L2xL0, L0xL1) / (L0.(L1xL2)) with the reference's operation order
  * (src/cell.c:432-444, src/util.c:186-219); compiled without FMA contraction */
-static void reciprocal(const double *L, double *R) {
+void mgh_reciprocal(const double *L, double *R) {
   const double *a = L, *b = L + 3, *c = L + 6;
   R[6] = a[1] * b[2] - a[2] * b[1];
   R[7] = a[2] * b[0] - a[0] * b[2];
@@ -349,7 +349,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
     mg_struct_dev *d = &c->h_str[s];
     memset(d, 0, sizeof *d);
     for (int k = 0; k < 9; k++) d->L[k] = lat[9 * s + k] * 1.0;
-    reciprocal(d->L, d->R);
+    mgh_reciprocal(d->L, d->R);
     d->rc2 = rc * rc;
     d->atom0 = atom_off[s];
     d->natom = atom_off[s + 1] - atom_off[s];

@@ -12,7 +12,7 @@
 
 #include "mg_host.h"
 
-static const double MG_AMU[96] = { /* atomic masses (amu), src/util.c:11 */
+const double MG_AMU_TABLE[96] = { /* atomic masses (amu), src/util.c:11 */
   0.0,1.008,4.002602,6.94,9.0121831,10.81,12.011,14.007,15.999,18.99840316,20.1797,22.98976928,24.305,26.9815384,
   28.085,30.973762,32.06,35.45,39.948,39.0983,40.078,44.955908,47.867,50.9415,51.9961,54.938043,55.845,58.933194,
   58.6934,63.546,65.38,69.723,72.63,74.921595,78.971,79.904,83.798,85.4678,87.62,88.90584,91.224,92.90637,95.95,98,
@@ -78,7 +78,7 @@ static int ensure_dyn(mg_ctx *c) {
   HIPCHK(hipMemcpy(spc, c->d_spc, sizeof(int) * c->N, hipMemcpyDeviceToHost));
   for (int i = 0; i < c->N; i++) {
     int z = c->m->spcz[spc[i]];
-    mass[i] = (z > 0 && z < 96 ? MG_AMU[z] : 1.0) * MG_MASS_UNIT;
+    mass[i] = (z > 0 && z < 96 ? MG_AMU_TABLE[z] : 1.0) * MG_MASS_UNIT;
   }
   hipError_t e = hipMemcpy(c->d_mass, mass, sizeof(double) * c->N, hipMemcpyHostToDevice);
   free(spc);

@@ -11,6 +11,7 @@
 
 struct mg_model {
   int nspc, spcz[MG_MAXSPC];
+  char id[64];          /* model unique ID of the file header */
   int D, H1, H2, nsym, ng2, ng4, nw;
   double rc_list;       /* neighbour-list cutoff (A) */
   mg_basis_dev basis;   /* FP32 basis tables */
@@ -156,6 +157,8 @@ enum { MG_CHAIN_SYNC = 0, MG_CHAIN_GRAPH = 1, MG_CHAIN_CAPS = 2, MG_CHAIN_ASYNC 
 /* CAPS: graph-mode capacities, not captured, flags checked at the end; ASYNC: CAPS whose flags are copied to the
  * host without waiting (checked by mgh_settle) */
 int mgh_eval_chain(struct mg_ctx *c, int forces, int mode);
+/* reciprocal rows R = (L1xL2, L2xL0, L0xL1) / (L0.(L1xL2)) in the reference's operation order */
+void mgh_reciprocal(const double *L, double *R);
 /* evaluate; async = 0 waits and checks the flags before returning (every internal caller) */
 int mgh_eval(struct mg_ctx *c, int flags, int async);
 /* finish the pending asynchronous evaluation, if any: wait, check its flags, evaluate again synchronised when the

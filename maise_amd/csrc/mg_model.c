@@ -262,6 +262,7 @@ int mg_model_load(const char *path, mg_model **out) {
   int ver = model_version(&t);
   if (ver < 2400) { mg_fail(rc, "'%s' is not a neural-network model of version >= 2.4.00", path); goto done; }
   int li;
+  if ((li = find_tag(&t, 0, "|  model unique ID      |", 25)) >= 0) sscanf(t.line[li] + 26, "%63s", m->id);
   if ((li = find_tag(&t, 0, "|  number of species    |", 25)) < 0 || sscanf(t.line[li] + 26, "%d", &m->nspc) != 1 ||
       m->nspc < 1 || m->nspc > MG_MAXSPC) {
     mg_fail(rc, "model: bad or unsupported number of species (1..%d)", MG_MAXSPC); goto done;
@@ -489,5 +490,11 @@ int mg_model_info(const mg_model *m, int *info, double *rc) {
   info[14] = m->mfma_ok;
   info[15] = 0;
   if (rc) *rc = m->rc_list;
+  return MG_OK;
+}
+
+int mg_model_id(const mg_model *m, char *buf, int len) {
+  if (!m || !buf || len < 1) return mg_fail(MG_ERR_ARG, "mg_model_id: bad argument");
+  snprintf(buf, (size_t)len, "%s", m->id);
   return MG_OK;
 }

@@ -1,0 +1,314 @@
+"""Structure and output files (mg_poscar_read / mg_poscar_write / mg_outcar_write) against the reference's own files.
+
+Fixtures under tests/golden/files are the reference's test inputs and outputs, unchanged:
+  relax_*  test/relax/POSCAR and test/relax/ref/{CONTCAR,OUTCAR,OSZICAR} (Cu3Pd cell relaxation, model
+           nn_CuPd_d3_vp_relax, ID 05830545);
+  iface_*  interface/c/test/{POSCAR,CONTCAR,OUTCAR,OSZICAR} (4-atom Cu cluster, model nn_CuAg_d3_v0_iface).
+The OUTCAR blocks are rebuilt from FP64 oracle evaluations of the same structures, so the writer is checked on the
+reference's numbers: line for line, the layout character for character and every numeric field within one unit of
+its last printed digit or 4e-12, whichever is larger (the reference's atom energies at the relaxed cell lie 3e-12
+from the oracle's at the CONTCAR's 14-digit lattice; an atom energy there moves 2.5e-12 eV per 1e-12 A of lattice
+constant).  The run header's date line and the trailer the reference's driver appends after the last block are not
+compared.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from maise_amd import MaiseError, Model
+from maise_amd import poscar as pc
+from oracle import pyoracle as po
+
+FILES = os.path.join(GOLDEN, "files")
+EV2GPA = 160.2176565
+
+
+def _text(name):
+    with open(os.path.join(FILES, name)) as f:
+        return f.read()
+
+
+def _num(tok):
+    try:
+        return float(tok)
+    except ValueError:
+        return None
+
+
+def _compare_lines(got, ref):
+    """Mismatching (line number, got, ref): tokens must end at the same columns; numbers may differ by one unit in
+    their last printed digit or by 4e-12 (so -0.000000 and 0.000000 agree), every other token must be equal."""
+    bad = []
+    for k, (g, r) in enumerate(zip(got, ref)):
+        if g == r:
+            continue
+        gt, rt = g.split(), r.split()
+        ok = len(g.rstrip()) == len(r.rstrip()) and len(gt) == len(rt)
+        if ok:
+            ge = [m_end for m_end in _ends(g)]
+            re_ = [m_end for m_end in _ends(r)]
+            ok = ge == re_
+        if ok:
+            for a, b in zip(gt, rt):
+                na, nb = _num(a), _num(b)
+                if na is None or nb is None:
+                    ok = ok and a == b
+                else:
+                    d = len(b.split(".")[1]) if "." in b else 0
+                    ok = ok and abs(na - nb) <= max(1.0001 * 10.0 ** (-d), 4e-12)
+        if not ok:
+            bad.append((k + 1, g, r))
+    if len(got) != len(ref):
+        bad.append((min(len(got), len(ref)) + 1, f"{len(got)} lines", f"{len(ref)} lines"))
+    return bad
+
+
+def _ends(line):
+    ends, inside = [], False
+    for i, ch in enumerate(line + " "):
+        if ch != " " and not inside:
+            inside = True
+        elif ch == " " and inside:
+            inside = False
+            ends.append(i)
+    return ends
+
+
+# ---- POSCAR / CONTCAR -------------------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("name,nd", [("relax_CONTCAR", 3), ("iface_CONTCAR", 0)])
+def test_contcar_round_trip_is_text_identical(tmp_path, name, nd):
+    """Read a CONTCAR the reference wrote and write it again: the same bytes (title, 14-digit lattice and direct
+    coordinates, species line, trailing blanks, the velocity block of the relaxation run, cluster centring)."""
+    p = pc.read_poscar(os.path.join(FILES, name), nd=nd)
+    out = str(tmp_path / "CONTCAR")
+    pc.write_poscar(out, p.lattice, p.z, p.counts, p.x, comment=p.comment, nd=nd, velocities=p.velocities)
+    assert open(out).read() == _text(name)
+
+
+def test_cluster_poscar_gives_the_reference_contcar(tmp_path):
+    """interface/c/test: the POSCAR's cluster (direct +-0.1 around the origin) comes back centred on the origin
+    (CENTER(C, 0.0): +-2 A) and is written centred in the box (CENTER(C, 0.5)) exactly as the reference's CONTCAR."""
+    p = pc.read_poscar(os.path.join(FILES, "iface_POSCAR"), nd=0)
+    np.testing.assert_allclose(p.x, [[2, 0, 0], [-2, 0, 0], [0, 2, 0], [0, -2, 0]], atol=1e-14)
+    assert p.comment == "" and p.symbols == ["Cu"] and p.counts == [4] and p.velocities is None
+    out = str(tmp_path / "CONTCAR")
+    pc.write_poscar(out, p.lattice, p.z, p.counts, p.x, comment=p.comment, nd=0)
+    assert open(out).read() == _text("iface_CONTCAR")
+
+
+def test_relax_poscar_header_and_positions():
+    p = pc.read_poscar(os.path.join(FILES, "relax_POSCAR"))
+    assert p.comment == "icsd_58525  Li Au                       "
+    assert p.symbols == ["Cu", "Pd"] and p.counts == [3, 1] and list(p.z) == [29, 46]
+    np.testing.assert_array_equal(p.lattice, np.eye(3) * 4.0)
+    np.testing.assert_array_equal(p.x, [[0, 0, 0], [0, 2, 2], [2, 2, 0], [2, 0, 2]])
+    np.testing.assert_array_equal(p.ff, np.ones((4, 3), np.int32))
+    np.testing.assert_array_equal(p.velocities, np.zeros((4, 3)))  # the VASP-style zero block after a blank line
+
+
+def test_relaxed_lattice_in_contcar_is_the_oracle_minimum():
+    """The CONTCAR's lattice is the zero-stress point of the oracle for this model (the relaxation's result)."""
+    p = pc.read_poscar(os.path.join(FILES, "relax_CONTCAR"))
+    assert p.lattice[0, 0] == 3.72890112684628
+    np.testing.assert_array_equal(p.velocities, np.zeros((4, 3)))
+    o = po.OracleModel(os.path.join(GOLDEN, "models", "nn_CuPd_d3_vp_relax")).evaluate(
+        np.array([0, 0, 0, 1], np.int32), p.lattice, p.x, nd=3, xt=1)
+    assert np.all(np.abs(o["U"]) / abs(np.linalg.det(p.lattice)) * EV2GPA < 1e-6)
+
+
+def test_random_triclinic_write_read_round_trip(tmp_path):
+    """Write -> read returns the Cartesian positions to the 14 printed digits, with scale, Cartesian-to-direct
+    conversion, selective-dynamics flags and velocities."""
+    rng = np.random.default_rng(11)
+    lat = np.diag([7.1, 6.3, 8.2]) + rng.uniform(-1.5, 1.5, (3, 3))
+    N = 13
+    x = rng.uniform(0, 1, (N, 3)) @ lat
+    ff = (rng.uniform(size=(N, 3)) > 0.3).astype(np.int32)
+    v = rng.normal(size=(N, 3)) * 0.01
+    out = str(tmp_path / "POSCAR")
+    pc.write_poscar(out, lat, [29, 47, 46], [5, 6, 2], x, comment="tri", scale=2.0, ff=ff, sd=True, velocities=v)
+    p = pc.read_poscar(out)
+    assert p.symbols == ["Cu", "Ag", "Pd"] and p.counts == [5, 6, 2]
+    np.testing.assert_allclose(p.lattice, lat, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(p.x, x, rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(p.ff, ff)
+    np.testing.assert_allclose(p.velocities, v, rtol=0, atol=1e-14)
+    text = open(out).read().splitlines()
+    assert text[1] == " 2.00000000000000" and text[7] == "Selective dynamics" and text[8] == "direct"
+
+
+def test_fixed_flags_without_sd_follow_the_reference(tmp_path):
+    """A fixed coordinate with sd = 0 writes the T/F columns but no "Selective dynamics" line (what SAVE_CELL
+    does with SD = 0); the reader takes the columns back either way."""
+    lat = np.eye(3) * 5.0
+    x = np.array([[0.0, 0.0, 0.0], [2.5, 2.5, 2.5]])
+    ff = np.array([[1, 1, 0], [1, 1, 1]], np.int32)
+    out = str(tmp_path / "POSCAR")
+    pc.write_poscar(out, lat, [29], [2], x, comment="fixed", ff=ff)
+    lines = open(out).read().splitlines()
+    assert lines[6] == "2 " and lines[7] == "direct"
+    assert lines[8].endswith("  T  T  F") and lines[9].endswith("  T  T  T")
+    np.testing.assert_array_equal(pc.read_poscar(out).ff, ff)
+    pc.write_poscar(out, lat, [29], [2], x, comment="free")
+    assert "  T" not in open(out).read()
+
+
+def test_cartesian_poscar_is_read_unscaled(tmp_path):
+    """READ_CELL takes Cartesian coordinates as written (the scale multiplies the lattice only)."""
+    out = tmp_path / "POSCAR"
+    out.write_text("c\n2.0\n2 0 0\n0 2 0\n0 0 2\nAl\n1\ncartesian\n1.0 0.5 0.25\n")
+    p = pc.read_poscar(str(out))
+    np.testing.assert_array_equal(p.lattice, np.eye(3) * 4.0)
+    np.testing.assert_array_equal(p.x, [[1.0, 0.5, 0.25]])
+
+
+def test_reader_errors(tmp_path):
+    with pytest.raises(MaiseError, match="cannot open"):
+        pc.read_poscar(str(tmp_path / "missing"))
+    bad = tmp_path / "POSCAR"
+    bad.write_text("no species\n1.0\n4 0 0\n0 4 0\n0 0 4\n4\ndirect\n0 0 0\n0 .5 .5\n.5 .5 0\n.5 0 .5\n")
+    with pytest.raises(MaiseError, match="species"):
+        pc.read_poscar(str(bad))
+    bad.write_text("short\n1.0\n4 0 0\n0 4 0\n0 0 4\nCu\n4\ndirect\n0 0 0\n0 .5 .5\n")
+    with pytest.raises(MaiseError, match="position"):
+        pc.read_poscar(str(bad))
+    bad.write_text("unknown\n1.0\n4 0 0\n0 4 0\n0 0 4\nQq\n1\ndirect\n0 0 0\n")
+    with pytest.raises(MaiseError, match="unknown species"):
+        pc.read_poscar(str(bad))
+
+
+def test_element_table():
+    """Symbols <-> Z over the whole table: a bijection on 1..100, lower case accepted, known anchors."""
+    for z in range(1, 101):
+        s = pc.element_symbol(z)
+        assert pc.element_z(s) == z and pc.element_z(s.lower()) == z
+    anchors = {"H": 1, "C": 6, "Si": 14, "Cu": 29, "Pd": 46, "Ag": 47, "Au": 79, "U": 92, "Am": 95, "Fm": 100}
+    for s, z in anchors.items():
+        assert pc.element_z(s) == z
+    assert pc.element_z("Xx") == 0 and pc.element_symbol(0) == "--" and pc.element_symbol(101) == "--"
+
+
+# ---- OUTCAR / OSZICAR -------------------------------------------------------------------------------------------
+
+
+def _outcar_block(tmp, it, model, spc, z, counts, lat, x, nd, version, rlxt=3):
+    o = model.evaluate(spc, lat, x, nd=nd, xt=1)
+    pc.write_outcar(str(tmp), it, lat, z, counts, o["E"], o["H"], x=o["X"] if nd else x, F=o["F"], EA=o["EA"],
+                    U=o["U"], version=version, model_id=Model(model.path).id, mint=0, rlxt=rlxt, p_gpa=0.0, nd=nd)
+    return o
+
+
+class _Oracle(po.OracleModel):
+    def __init__(self, path):
+        super().__init__(path)
+        self.path = path
+
+
+def test_relax_outcar_matches_reference(tmp_path):
+    """test/relax/ref/OUTCAR: the header and the three iteration blocks (a = 4.0, then twice the relaxed lattice)
+    and the three OSZICAR lines."""
+    m = _Oracle(os.path.join(GOLDEN, "models", "nn_CuPd_d3_vp_relax"))
+    spc = np.array([0, 0, 0, 1], np.int32)
+    frac = np.array([[0, 0, 0], [0, .5, .5], [.5, .5, 0], [.5, 0, .5]])
+    for it, a in enumerate([4.0, 3.72890112684628, 3.72890112684628]):
+        lat = np.eye(3) * a
+        _outcar_block(tmp_path, it, m, spc, [29, 46], [3, 1], lat, frac @ lat, 3, "maise.2.8.17")
+    ref = _text("relax_OUTCAR").splitlines()
+    got = open(tmp_path / "OUTCAR").read().splitlines()
+    n = len(got)
+    assert n == 80 and ref[n:] == ["  ", " Total CPU time used (sec):      0.43894400     wall time (sec):     0.11669400"]
+    assert got[1].startswith("    Cell optimization    with maise.2.8.17   model 05830545   on  ")
+    assert _compare_lines(got[:1] + got[2:], ref[:1] + ref[2:n]) == []
+    assert _compare_lines(open(tmp_path / "OSZICAR").read().splitlines(), _text("relax_OSZICAR").splitlines()) == []
+
+
+def test_cluster_outcar_matches_reference(tmp_path):
+    """interface/c/test/OUTCAR: cluster header (dimensionality 0), the inertia-tensor volume per atom (NP_VOL with
+    the reference's masses: 16.557647), and the stress divided by that volume."""
+    m = _Oracle(os.path.join(GOLDEN, "models", "nn_CuAg_d3_v0_iface"))
+    p = pc.read_poscar(os.path.join(FILES, "iface_POSCAR"), nd=0)
+    _outcar_block(tmp_path, 0, m, np.zeros(4, np.int32), p.z, p.counts, p.lattice, p.x, 0, "maise.2.6.00")
+    ref = _text("iface_OUTCAR").splitlines()
+    got = open(tmp_path / "OUTCAR").read().splitlines()
+    n = len(got)
+    assert n == 36 and ref[n].strip() == "" and ref[n + 1].startswith(" Total CPU time used")
+    assert _compare_lines(got[:1] + got[2:], ref[:1] + ref[2:n]) == []
+    assert got[2:] == ref[2:n]  # every number of this block prints identically
+    assert open(tmp_path / "OSZICAR").read() == _text("iface_OSZICAR")
+
+
+def test_outcar_without_detail_and_comparator_catches_changes(tmp_path):
+    """COUT < 10 omits positions and stress; the line comparator rejects a shifted column and a wrong digit."""
+    lat = np.eye(3) * 4.0
+    pc.write_outcar(str(tmp_path), 5, lat, [29], [4], -10.0, -9.5, detail=False)
+    lines = open(tmp_path / "OUTCAR").read().splitlines()
+    assert len(lines) == 11 and "POSITION" not in "".join(lines)
+    assert lines[3] == _text("relax_OUTCAR").splitlines()[17]  # the same lattice line of the reference's a = 4 block
+    assert lines[6] == "  iter   5   total enthalpy=    -9.50000000   energy=    -10.00000000      -2.37500000     -2.50000000"
+    assert _compare_lines(["  a  1.000000"], ["  a  1.000001"]) == []
+    assert _compare_lines(["  a  1.000000"], ["  a  1.000002"]) != []
+    assert _compare_lines(["  a   1.000000"], ["  a  1.000000 "]) != []
+    assert _compare_lines(["  b  1.000000"], ["  a  1.000000"]) != []
+
+
+# ---- model files (SAVE_ANN layout) --------------------------------------------------------------------------------
+
+
+def _parse_shipped_model(path, S, arch):
+    """Input ranges and weight vector of an nn_* file, read as READ_ANN does (src/nutl.c:580-617)."""
+    lines = open(path).read().splitlines()
+    at = next(i for i, l in enumerate(lines) if l.startswith("        minimum"))
+    D = arch[0]
+    rng = np.array([[float(v) for v in lines[at + 1 + k].split()] for k in range(S * D)]).reshape(S, D, 3)
+    w0 = at + 1 + S * D + 3
+    nw = S * sum((arch[k] + 1) * arch[k + 1] for k in range(len(arch) - 1))
+    V = np.array([float(lines[w0 + k]) for k in range(nw)])
+    return lines, at, w0, nw, rng, V
+
+
+def _network_from_vector(spcz, arch, V, rng):
+    from maise_amd.modelfile import Network
+    biases, weights, o = [], [], 0
+    for _ in spcz:
+        b, w = [], []
+        for k in range(len(arch) - 1):
+            b.append(V[o:o + arch[k + 1]]); o += arch[k + 1]
+        for k in range(len(arch) - 1):
+            w.append(V[o:o + arch[k + 1] * arch[k]].reshape(arch[k + 1], arch[k])); o += arch[k + 1] * arch[k]
+        biases.append(b); weights.append(w)
+    return Network(spcz=list(spcz), arch=list(arch), biases=biases, weights=weights, rmin=rng[:, :, 0],
+                   rmax=rng[:, :, 1], drange=rng[:, :, 2])
+
+
+@pytest.mark.parametrize("name", ["nn_CuPd_d3_vp_relax", "nn_CuAg_d3_v0_iface"])
+def test_model_writer_reproduces_shipped_numeric_sections(tmp_path, name):
+    """write_model on a shipped model's own weights, ranges and basis: the range and weight sections come out
+    text-identical to the reference's file (the "% 21.16lf % 25.16lf  % 25.16lf" and "% 21.16lf" rows of SAVE_ANN,
+    src/nutl.c:486-493), and the library and the oracle read the rewritten file to the same model."""
+    from maise_amd.modelfile import read_basis, write_model
+    src = os.path.join(GOLDEN, "models", name)
+    m = Model(src)
+    arch = [m.D, m.H1, m.H2, 1]
+    lines, at, w0, nw, rng, V = _parse_shipped_model(src, m.nspc, arch)
+    out = str(tmp_path / name)
+    write_model(out, _network_from_vector(m.spcz, arch, V, rng), read_basis(src), model_id=m.id)
+    got = open(out).read().splitlines()
+    gat = next(i for i, l in enumerate(got) if l.startswith("        minimum"))
+    assert got[gat:gat + 1 + m.nspc * m.D] == lines[at:at + 1 + m.nspc * m.D]
+    gw0 = gat + 1 + m.nspc * m.D + 3
+    assert got[gw0:gw0 + nw] == lines[w0:w0 + nw]
+    m2 = Model(out)
+    assert (m2.id, m2.nspc, m2.D, m2.H1, m2.H2, m2.ng2, m2.ng4, m2.nw, m2.spcz, m2.rc) == \
+        (m.id, m.nspc, m.D, m.H1, m.H2, m.ng2, m.ng4, m.nw, m.spcz, m.rc)
+    rs = np.random.default_rng(5)
+    lat = np.eye(3) * 7.3 + rs.uniform(-0.4, 0.4, (3, 3))
+    spc = rs.integers(0, m.nspc, 12).astype(np.int32)
+    frac = rs.uniform(size=(12, 3))
+    a = po.OracleModel(src).evaluate(spc, lat, frac, nd=3, xt=0)
+    b = po.OracleModel(out).evaluate(spc, lat, frac, nd=3, xt=0)
+    assert a["E"] == b["E"]
+    np.testing.assert_array_equal(a["F"], b["F"])
