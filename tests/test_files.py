@@ -312,3 +312,19 @@ def test_model_writer_reproduces_shipped_numeric_sections(tmp_path, name):
     b = po.OracleModel(out).evaluate(spc, lat, frac, nd=3, xt=0)
     assert a["E"] == b["E"]
     np.testing.assert_array_equal(a["F"], b["F"])
+
+
+def test_relax_job_rejects_foreign_species_and_fixed_atoms(tmp_path):
+    """Checked before any device work: a species the model lacks (READ_CELL's error) and selective dynamics,
+    which the device relaxation does not implement."""
+    from maise_amd import jobs
+    m = Model(os.path.join(GOLDEN, "models", "nn_CuPd_d3_vp_relax"))
+    f = str(tmp_path / "POSCAR")
+    pc.write_poscar(f, np.eye(3) * 4.0, [29, 47], [3, 1], np.array([[0, 0, 0], [0, 2, 2], [2, 2, 0], [2, 0, 2.]]))
+    with pytest.raises(MaiseError, match="species 47"):
+        jobs.relax_job(m, f, str(tmp_path / "out"))
+    pc.write_poscar(f, np.eye(3) * 4.0, [29, 46], [3, 1], np.array([[0, 0, 0], [0, 2, 2], [2, 2, 0], [2, 0, 2.]]),
+                    ff=np.array([[0, 0, 0], [1, 1, 1], [1, 1, 1], [1, 1, 1]]))
+    with pytest.raises(MaiseError, match="selective dynamics"):
+        jobs.relax_job(m, f, str(tmp_path / "out"))
+    assert not os.path.exists(tmp_path / "out")

@@ -263,6 +263,71 @@ def test_cell_relax_reaches_reference_minimum():
     c.close()
 
 
+def _numbers_close(got, ref, atol, skip=()):
+    """Line-by-line: the same tokens where they are not numbers, numbers within atol (per-line override in a
+    dict {line: atol}); lines in skip are not compared.  Returns the mismatching lines."""
+    bad = []
+    for k, (g, r) in enumerate(zip(got, ref)):
+        if k in skip:
+            continue
+        gt, rt = g.split(), r.split()
+        ok = len(gt) == len(rt)
+        tol = atol.get(k, atol[None]) if isinstance(atol, dict) else atol
+        for a, b in zip(gt, rt):
+            try:
+                ok = ok and abs(float(a) - float(b)) <= tol
+            except ValueError:
+                ok = ok and a == b
+        if not ok:
+            bad.append((k, g, r))
+    if len(got) != len(ref):
+        bad.append(("length", len(got), len(ref)))
+    return bad
+
+
+def test_relax_job_files_match_reference(tmp_path):
+    """The file-driven job of test/relax (POSCAR in; OUTCAR, OSZICAR, CONTCAR out) on the device: the CONTCAR has
+    the reference's layout line for line (test/relax/ref/CONTCAR, velocity block included) with the relaxed
+    lattice within 5e-5 A of the reference's; OUTCAR iteration 0 (the start, a = 4.0) and iteration 1 (the relaxed
+    cell) match the reference's blocks 0 and 1 (test/relax/ref/OUTCAR) with the FP32 path's tolerances; the
+    "optimizer type" line differs by design (FIRE is 4, the reference ran BFGS2, 0) and the date line is not
+    compared."""
+    import maise_amd
+    from maise_amd import jobs
+    print("[gpu case] relax job test/relax files", flush=True)
+    files = os.path.join(GOLDEN, "files")
+    m = maise_amd.Model(os.path.join(GOLDEN, "models", "nn_CuPd_d3_vp_relax"))
+    out = str(tmp_path)
+    r = jobs.relax_job(m, os.path.join(files, "relax_POSCAR"), out, rlxt=3, max_steps=4000, fmax=1e-4, smax_gpa=2e-4,
+                       version="maise.2.8.17")
+    assert r["converged"], r["steps"]
+    got = open(os.path.join(out, "CONTCAR")).read().splitlines()
+    ref = open(os.path.join(files, "relax_CONTCAR")).read().splitlines()
+    assert got[0] == ref[0] and got[5:8] == ref[5:8]
+    pos = range(8, 12)  # direct coordinates: compared on the torus (an atom at 0 may come back as 0.99999999...)
+    assert _numbers_close(got, ref, 5e-5, skip=pos) == []
+    d = np.array([[float(v) for v in got[k].split()] for k in pos]) - \
+        np.array([[float(v) for v in ref[k].split()] for k in pos])
+    assert np.abs(d - np.round(d)).max() < 1e-5
+    got = open(os.path.join(out, "OUTCAR")).read().splitlines()
+    ref = open(os.path.join(files, "relax_OUTCAR")).read().splitlines()
+    assert len(got) == 58  # header + two iteration blocks
+    assert got[3] == "   optimizer type          4" and ref[3] == "   optimizer type          0"
+    # FP32 path: atom energies 1e-5 eV, positions/forces 1e-4, lattice 5e-5 A, stress 1e-6 eV/A^3 (kB: x1602)
+    tol = {None: 1e-4, 28: 2e-6, 29: 4e-3, 31: 5e-5, 39: 1e-3, 50: 2e-6, 51: 4e-3, 53: 5e-5}
+    pos = list(range(23, 27)) + list(range(45, 49))  # position rows: Cartesian columns compared on the torus
+    assert _numbers_close(got, ref[:58], tol, skip=[1, 3] + pos) == []
+    for k in pos:
+        g, q = np.array(got[k].split(), float), np.array(ref[k].split(), float)
+        a = 4.0 if k < 40 else 3.72890112684628
+        d = (g[:3] - q[:3]) / a
+        assert np.abs(d - np.round(d)).max() * a < 1e-4, (k, got[k], ref[k])
+        np.testing.assert_allclose(g[3:6], q[3:6], atol=1e-4)
+        assert abs(g[6] - q[6]) < 1e-5, (k, got[k], ref[k])
+    oz = open(os.path.join(out, "OSZICAR")).read().split()
+    np.testing.assert_allclose([float(v) for v in oz], [-17.25531076, -18.08137782], atol=5e-5)
+
+
 def test_cell_relax_examples_03rlx_final_energy():
     """examples/03-rlx: Cu4 in a 3.626 x 3.626 x 6.026 A cell with its model relaxes (positions + cell) to the
     minimum H = -16.3965601100172194 eV (examples/03-rlx/README:48)."""
