@@ -44,7 +44,7 @@ static int ensure_atoms(mg_ctx *c, size_t N) {
   ALLOC(c->d_spc, n * sizeof(int));
   ALLOC(c->d_xin, n * 3 * sizeof(double));
   ALLOC(c->d_x, n * 3 * sizeof(double));
-  ALLOC(c->d_bin, n * sizeof(int));
+  ALLOC(c->d_bin, 2 * n * sizeof(int));  /* bin id, then the packed bin coordinates at N + i */
   ALLOC(c->d_bin_rec, n * 32);
   ALLOC(c->d_cnt, n * 4 * sizeof(int));
   ALLOC(c->d_npl, n * MG_NCLS * sizeof(int));
@@ -504,7 +504,7 @@ static int chain_flags(mg_ctx *c, int mode) {
     if (mode != MG_CHAIN_SYNC) return MG_ERR_NEIGHBOR_CAP;  /* outgrew the capacities: repeat in sync mode */
     return mg_fail(MG_ERR_HIP, "edge capacity check failed in a synchronised evaluation");
   }
-  if (f[1]) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)");
+  if (f[1]) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list error: a reverse edge is missing, or a cell is too thin for the cutoff");
   c->nedges = mode == MG_CHAIN_SYNC ? f[6] : -1;  /* padded rows: the total is summed when it is asked for */
   c->maxrow = f[0];
   return MG_OK;

@@ -213,7 +213,7 @@ static int md_impl(mg_ctx *c, int nsteps, double dt, double T, double tau, int b
       MCHK(mgh_eval(c, MG_EVAL_FORCES, 0));  /* forces at the restored positions */
       continue;
     }
-    if (flags[1]) { r = mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)"); goto out; }
+    if (flags[1]) { r = mg_fail(MG_ERR_NEIGHBOR, "neighbour list error: a reverse edge is missing, or a cell is too thin for the cutoff"); goto out; }
     s += nrun;
     c->nedges = -1;  /* padded rows: summed when asked for */
     c->maxrow = flags[0];
@@ -470,7 +470,7 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
       gx = NULL;
       continue;
     }
-    if (flags[1]) RCHK(mg_fail(MG_ERR_NEIGHBOR, "neighbour list is not symmetric (reverse edge missing)"));
+    if (flags[1]) RCHK(mg_fail(MG_ERR_NEIGHBOR, "neighbour list error: a reverse edge is missing, or a cell is too thin for the cutoff"));
     RCHK(snapshot(c, 0));
     RKCHK(mgk_clear(c->d_misc + 3, 3, NULL, 0, NULL, 0, c->st));
     const int ndone = flags[2];

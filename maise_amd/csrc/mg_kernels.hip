@@ -25,7 +25,10 @@
 
 typedef unsigned long long u64;
 
-__device__ __forceinline__ int floordiv_i(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+// floor(a / b) for |a| < 2^21, 0 < b <= 2^21, with rb = 1.0f / b (correctly rounded): a + 0.5 is exact in
+// float, (a + 0.5) / b lies at least 0.5 / b from an integer, and the two roundings move it by under
+// (|a| + 0.5) 2^-23 / b, less than that while |a| < 2^21
+__device__ __forceinline__ int floordiv_f(int a, float rb) { return (int)floorf(((float)a + 0.5f) * rb); }
 
 __device__ __forceinline__ unsigned pack_img(int a, int b, int c) {
   return (unsigned)(a + MG_IMG_OFF) | ((unsigned)(b + MG_IMG_OFF) << 10) | ((unsigned)(c + MG_IMG_OFF) << 20);
@@ -102,30 +105,61 @@ __global__ __launch_bounds__(256) void k_scan_add(int* __restrict__ out, int n, 
   if (b == 0 && threadIdx.x == 0) out[0] = 0;
 }
 
-// exclusive scan of up to 1024 * SCAN1_PT values in one block, in coalesced tiles of 1024: a shuffle scan in each
-// wave, the 16 wave totals scanned in LDS, a running carry between tiles
+// exclusive scan of up to 1024 * SCAN1_PT values in one block.  Each pass covers 1024 x SCAN1_SEG values: they are
+// loaded coalesced into LDS (all loads of a thread issued before any is used: one memory round trip per pass), thread t
+// then sums its run of SCAN1_SEG consecutive values from LDS, a shuffle scan of the 1024 run totals (64 per wave, 16
+// waves through LDS) gives each run its offset, the prefix sums go back to LDS and out with coalesced stores, and
+// the pass total carries into the next pass.  LDS index k + k/32 spreads a run's strided reads over the banks.
 #define SCAN1_PT 64
+#define SCAN1_SEG 8
 __global__ __launch_bounds__(1024) void k_scan_one(const int* __restrict__ in, int* __restrict__ out, int n) {
+  __shared__ int buf[1024 * SCAN1_SEG + 1024 * SCAN1_SEG / 32];
   __shared__ int wsum[16];
   __shared__ int carry_s;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) { carry_s = 0; out[0] = 0; }
-  __syncthreads();
-  for (int c0 = 0; c0 < n; c0 += 1024) {
-    const int k = c0 + t;
-    int v = k < n ? in[k] : 0;
+  for (int c0 = 0; c0 < n; c0 += 1024 * SCAN1_SEG) {
+    int v[SCAN1_SEG];
+#pragma unroll
+    for (int q = 0; q < SCAN1_SEG; q++) {
+      const int k = c0 + q * 1024 + t;
+      v[q] = k < n ? in[k] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < SCAN1_SEG; q++) {
+      const int k = q * 1024 + t;
+      buf[k + (k >> 5)] = v[q];
+    }
+    __syncthreads();
+    int run = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN1_SEG; q++) {
+      const int k = t * SCAN1_SEG + q;
+      run += buf[k + (k >> 5)];
+      v[q] = run;  // inclusive within the run
+    }
+    int x = run;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(v, o, 64);
-      if (lane >= o) v += y;
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
-    if (lane == 63) wsum[w] = v;
+    if (lane == 63) wsum[w] = x;
     __syncthreads();
-    int off = carry_s;
+    int off = carry_s + x - run;
     for (int q = 0; q < w; q++) off += wsum[q];
-    if (k < n) out[k + 1] = off + v;
+#pragma unroll
+    for (int q = 0; q < SCAN1_SEG; q++) {
+      const int k = t * SCAN1_SEG + q;
+      buf[k + (k >> 5)] = off + v[q];
+    }
     __syncthreads();
-    if (t == 1023) carry_s = off + v;
+#pragma unroll
+    for (int q = 0; q < SCAN1_SEG; q++) {
+      const int k = q * 1024 + t;
+      if (c0 + k < n) out[c0 + k + 1] = buf[k + (k >> 5)];
+    }
+    if (t == 1023) carry_s = off + run;
     __syncthreads();
   }
 }
@@ -195,6 +229,7 @@ __global__ void k_prepare(int N, const double* __restrict__ xin, const int* __re
   x[3 * i] = X[0]; x[3 * i + 1] = X[1]; x[3 * i + 2] = X[2];
   int id = st->bin0 + (b[0] * st->nb[1] + b[1]) * st->nb[2] + b[2];
   bin[i] = id;
+  bin[N + i] = b[0] | (b[1] << 10) | (b[2] << 20);  // the bin's coordinates (nb <= 1024 per direction)
   atomicAdd(&bin_count[id], 1);
 }
 
@@ -283,18 +318,46 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
   const int K0 = st->K[0], K1 = st->K[1], K2 = st->K[2];
   const int w1n = 2 * K1 + 1, w2n = 2 * K2 + 1;
   const int nsten = (2 * K0 + 1) * w1n * w2n;
-  const int local = bin[i] - st->bin0;
-  const int b2 = local % nb2, b1 = (local / nb2) % nb1, b0 = local / (nb1 * nb2);
+  if (nsten >= (1 << 21)) {  // a cell thinner than rc/64: beyond the index arithmetic below; no row, error raised
+    if (lane == 0) {
+      misc[1] = 1;
+      if (!FILL || padded) {
+        cnt[4 * i] = cnt[4 * i + 1] = cnt[4 * i + 2] = cnt[4 * i + 3] = 0;
+        rowcnt[i] = 0;
+        if (FILL) row_off[i] = i * cap_row;
+      }
+    }
+    return;
+  }
+  const float rw1 = 1.0f / (float)w1n, rw2 = 1.0f / (float)w2n;
+  float rfn0, rfn1, rfn2;  // 1.0f / nb_q
+  const int bq = bin[N + i];
+  const int b0 = bq & 1023, b1 = (bq >> 10) & 1023, b2 = bq >> 20;
   const unsigned img0 = pack_img(0, 0, 0);
   // stencil pruning: a lower bound on the distance from atom i to every point of a stencil bin.  Along lattice
   // direction q the bin is the slab of fractional coordinates [(b_q + d_q)/nb_q, (b_q + d_q + 1)/nb_q); its gap to
   // the atom's fractional coordinate times the plane spacing 1/|R_q| bounds the distance.  The largest of the three
   // bounds holds for any cell; for mutually orthogonal lattice rows the slabs are orthogonal and the bounds add in
   // quadrature.  Bins whose bound exceeds rc (with a relative margin far above rounding) hold no neighbour.
-  double fi[3], sp[3];
-  for (int q = 0; q < 3; q++) {
-    fi[q] = xi0 * st->R[3 * q] + xi1 * st->R[3 * q + 1] + xi2 * st->R[3 * q + 2];
-    sp[q] = 1.0 / sqrt(st->R[3 * q] * st->R[3 * q] + st->R[3 * q + 1] * st->R[3 * q + 1] + st->R[3 * q + 2] * st->R[3 * q + 2]);
+  // per-direction constants computed once, lane q for direction q, then broadcast: plane spacing 1/|R_q|, 1/nb_q,
+  // and the float reciprocals for the stencil index arithmetic
+  double fi[3], sp[3], rnb[3];
+  {
+    double spl = 0.0, rnl = 0.0;
+    float rfl = 0.0f;
+    if (lane < 3) {
+      const double* Rq = st->R + 3 * lane;
+      spl = 1.0 / sqrt(Rq[0] * Rq[0] + Rq[1] * Rq[1] + Rq[2] * Rq[2]);
+      const int nq = lane == 0 ? nb0 : (lane == 1 ? nb1 : nb2);
+      rnl = 1.0 / (double)nq;
+      rfl = 1.0f / (float)nq;
+    }
+    for (int q = 0; q < 3; q++) {
+      fi[q] = xi0 * st->R[3 * q] + xi1 * st->R[3 * q + 1] + xi2 * st->R[3 * q + 2];
+      sp[q] = __shfl(spl, q, WAVE);
+      rnb[q] = __shfl(rnl, q, WAVE);
+    }
+    rfn0 = __shfl(rfl, 0, WAVE); rfn1 = __shfl(rfl, 1, WAVE); rfn2 = __shfl(rfl, 2, WAVE);
   }
   bool orth = true;
   for (int q = 0; q < 3; q++) {
@@ -311,23 +374,26 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
     const int t = base + lane;
     int count = 0;
     if (t < nsten) {
-      const int d2 = t % w2n - K2, d1 = (t / w2n) % w1n - K1, d0 = t / (w2n * w1n) - K0;
-      const int dd[3] = {d0, d1, d2}, bb[3] = {b0, b1, b2}, nn[3] = {nb0, nb1, nb2};
+      // t = (d0 * w1n + d1) * w2n + d2 (offsets by K): quotients through float reciprocals, exact for t < 2^21
+      const int t2 = floordiv_f(t, rw2), t1 = floordiv_f(t2, rw1);
+      const int d2 = t - t2 * w2n - K2, d1 = t2 - t1 * w1n - K1, d0 = t1 - K0;
+      const int dd[3] = {d0, d1, d2}, bb[3] = {b0, b1, b2};
       double lb[3];
       for (int q = 0; q < 3; q++) {
+        // (b + d) / nb by the reciprocal: within an ulp of the quotient, far inside the bound's margin
         double g = 0.0;
-        if (dd[q] > 0) g = (double)(bb[q] + dd[q]) / nn[q] - fi[q];
-        else if (dd[q] < 0) g = fi[q] - (double)(bb[q] + dd[q] + 1) / nn[q];
+        if (dd[q] > 0) g = (double)(bb[q] + dd[q]) * rnb[q] - fi[q];
+        else if (dd[q] < 0) g = fi[q] - (double)(bb[q] + dd[q] + 1) * rnb[q];
         lb[q] = g > 0.0 ? g * sp[q] : 0.0;
       }
       const double lb2 = orth ? lb[0] * lb[0] + lb[1] * lb[1] + lb[2] * lb[2]
                               : fmax(lb[0] * lb[0], fmax(lb[1] * lb[1], lb[2] * lb[2]));
       const bool far = lb2 > rc2m;
-      int w0 = b0 + d0, ic0 = floordiv_i(w0, nb0);
+      int w0 = b0 + d0, ic0 = floordiv_f(w0, rfn0);
       w0 -= ic0 * nb0;
-      int w1 = b1 + d1, ic1 = floordiv_i(w1, nb1);
+      int w1 = b1 + d1, ic1 = floordiv_f(w1, rfn1);
       w1 -= ic1 * nb1;
-      int w2 = b2 + d2, ic2 = floordiv_i(w2, nb2);
+      int w2 = b2 + d2, ic2 = floordiv_f(w2, rfn2);
       w2 -= ic2 * nb2;
       const int w = st->bin0 + (w0 * nb1 + w1) * nb2 + w2;
       e_start[lane] = bin_start[w];
