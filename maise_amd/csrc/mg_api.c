@@ -120,6 +120,16 @@ static size_t ensure_pairs(mg_ctx *c, int alloc_row, int use_row) {
   return pair_stride(use_row);
 }
 
+/* the reverse-edge hash tables for rows of up to row entries */
+static int ensure_rhash(mg_ctx *c, int row) {
+  const size_t need = (size_t)c->N * (size_t)mgk_rev_slots(row);
+  if (need <= c->cap_rhash) return MG_OK;
+  const size_t n = need + need / 4 + 1024;
+  ALLOC(c->d_rhash, n * sizeof(uint16_t));
+  c->cap_rhash = n;
+  return MG_OK;
+}
+
 static int ensure_edges(mg_ctx *c, size_t M) {
   if (M <= c->cap_edges) return MG_OK;
   size_t n = M + M / 4 + 64;
@@ -185,7 +195,7 @@ void mg_ctx_free(mg_ctx *c) {
   hipSetDevice(c->device);
   void *ptrs[] = {c->d_basis, c->d_mlp, c->d_str, c->d_atom_str, c->d_spc, c->d_xin, c->d_x, c->d_bin, c->d_bin_count,
                   c->d_bin_start, c->d_bin_rec, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
-                  c->d_key, c->d_vec, c->d_rev, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
+                  c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
                   c->d_vir, c->d_fself, c->d_fe, c->d_gpl, c->d_npl, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
                   c->d_snap_dyn, c->d_cellv, c->d_snap_cellv, c->d_snap_str};
@@ -476,6 +486,7 @@ int mgh_graph_caps(mg_ctx *c, int need_edges, int need_row) {
   if (ne > 0x7fffffff) return mg_fail(MG_ERR_ARG, "too many neighbour edge slots (%zu)", ne);
   int r;
   if ((r = ensure_edges(c, ne))) return r;
+  if ((r = ensure_rhash(c, row))) return r;
   c->gcap_edges = (int)ne;
   c->gcap_row = row;
   c->gpstride = ensure_pairs(c, c->gcap_row, c->gcap_row);
@@ -585,16 +596,17 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
       if ((r = ensure_edges(c, ne))) return r;
     }
     cap_row = c->maxrow;
+    if ((r = ensure_rhash(c, cap_row + (c->graph_headroom > 0.0 ? 8 : 0)))) return r;
     pstride = forces ? ensure_pairs(c, cap_row + (c->graph_headroom > 0.0 ? 8 : 0), cap_row) : 0;
     KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->nedges, cap_row, st));
     KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
-                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_misc, cap_row, 0, st));
+                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, 0, st));
   } else {
     /* capacities known: one pass, row i at i * gcap_row (the edge buffers hold N * gcap_row slots) */
     cap_row = c->gcap_row;
     pstride = forces ? c->gpstride : 0;
     KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
-                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_misc, cap_row, 1, st));
+                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, 1, st));
   }
   if (timing) HIPCHK(hipEventRecord(ev[EV_NBR], st));
   /* descriptors, MLP, forces */

@@ -86,6 +86,8 @@ struct mg_ctx {
   int *d_cnt, *d_rowcnt, *d_row_off, *d_misc;
   void *d_key, *d_vec;
   int *d_rev;
+  uint16_t *d_rhash;        /* per-row hash of neighbour (atom, image) -> position + 1, mgk_rev_slots entries per row */
+  size_t cap_rhash;
   float *d_G, *d_dEdG, *d_Ei, *d_erep, *d_vir;
   void *d_fself, *d_fe;
   unsigned *d_gpl;          /* stored G4 pair lists, pstride entries per atom (j | k << 12 | tag << 24) */
@@ -130,7 +132,8 @@ int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, c
                   const int *bin_start, const void *brec, int *cnt, int *rowcnt, int *misc, hipStream_t st);
 int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
                  const int *bin_start, const void *brec, int *row_off, int *rowcnt, int *cnt, void *key, void *vec,
-                 int *rev, int *misc, int maxrow, int padded, hipStream_t st);
+                 int *rev, uint16_t *rhash, int *misc, int maxrow, int padded, hipStream_t st);
+int mgk_rev_slots(int maxrow);
 int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *rowcnt, const int *cnt,
              const void *vec,
              const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, unsigned *gpl, int *npl,

@@ -41,6 +41,28 @@ __device__ __forceinline__ u64 make_key(int s, int j, unsigned img) {
   return ((u64)(unsigned)s << 60) | ((u64)(unsigned)j << 30) | (u64)img;
 }
 __device__ __forceinline__ int key_j(u64 k) { return (int)((k >> 30) & 0x3fffffffull); }
+// Home slot of neighbour (atom j, packed image img) in a row's reverse-lookup hash table of hmask + 1 slots
+__device__ __forceinline__ int rev_slot(int j, unsigned img, int hmask) {
+  unsigned h = (unsigned)j * 0x9E3779B1u ^ img * 0x85EBCA77u;
+  h ^= h >> 15;
+  return (int)(h & (unsigned)hmask);
+}
+// Insert val (1..65535) at the first empty 16-bit slot from h on (linear probing) in an LDS table of 16-bit slots
+// packed two per word.  A compare-and-swap that fails means another lane filled the other half of the word (or this
+// slot): the slot is read again.  The table is at most half full, so an empty slot lies within hslots / 2 probes, and
+// each failed swap is another lane's insertion: the bound below is never reached.
+__device__ __forceinline__ void htab_insert(unsigned* tab, int h, int hmask, unsigned val) {
+  for (int it = 0; it < 2 * (hmask + 1) + 2 * WAVE; it++) {
+    unsigned* w = tab + (h >> 1);
+    const int sh = (h & 1) * 16;
+    const unsigned old = __atomic_load_n(w, __ATOMIC_RELAXED);
+    if ((old >> sh) & 0xffffu) {
+      h = (h + 1) & hmask;
+      continue;
+    }
+    if (atomicCAS(w, old, old | (val << sh)) == old) return;
+  }
+}
 
 // ---- scans, clears and the capacity check (graph-capturable: kernels only)
 #define SCAN_TILE 1024  // values per block: 256 threads x 4
@@ -285,6 +307,8 @@ __device__ __forceinline__ void wave_sync() {
 // through row_off and rowcnt).  A row longer than cap_row raises the overflow flag misc[3] and is not written;
 // misc[5] keeps the longest row seen, to size the next capacities.
 #define NBR_CHUNK 64
+// per-wave LDS of the stencil chunk tables (shifts, bin starts, images, prefix), a multiple of 16 bytes
+#define NBR_LDS_CHUNK ((NBR_CHUNK * (3 * 8 + 4 + 4 + 4) + 4 + 15) / 16 * 16)
 #define NBR_WPB 4  // most atoms (waves) per block; XCD-aware block order, so neighbouring atoms share bins in one L2
 template <bool FILL>
 __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __restrict__ x, const int* __restrict__ spc,
@@ -293,24 +317,25 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
                                             const BinRec* __restrict__ brec, int* __restrict__ cnt,
                                             int* __restrict__ rowcnt, int* __restrict__ misc,
                                             int* __restrict__ row_off, u64* __restrict__ key,
-                                            float4* __restrict__ vec, int* __restrict__ rev, int cap, int padded,
+                                            float4* __restrict__ vec, int* __restrict__ rev,
+                                            unsigned short* __restrict__ rhash, int hslots, int cap, int padded,
                                             int cap_row, int wpb) {
 #pragma clang fp contract(off)
   if (FILL && !padded && misc[3]) return;  // rows exceed the capacities this launch was sized for
-  __shared__ int e_start_s[NBR_WPB][NBR_CHUNK], e_pre_s[NBR_WPB][NBR_CHUNK + 1];
-  __shared__ unsigned e_img_s[NBR_WPB][NBR_CHUNK];
-  __shared__ double e_sh_s[NBR_WPB][NBR_CHUNK][3];
-  extern __shared__ unsigned char smem_n[];
+  extern __shared__ __align__(16) unsigned char smem_n[];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = xcd_block(blockIdx.x, gridDim.x) * wpb + wv;
   const int lane = threadIdx.x & 63;
   if (i >= N) return;
-  int* e_start = e_start_s[wv];
-  int* e_pre = e_pre_s[wv];
-  unsigned* e_img = e_img_s[wv];
-  double (*e_sh)[3] = e_sh_s[wv];
-  u64* rk = (u64*)smem_n + (size_t)wv * 3 * cap;  // [cap] keys of the row (FILL)
-  float4* rv = (float4*)(rk + cap);                // [cap] vectors
+  // the wave's LDS: the stencil chunk tables (NBR_LDS_CHUNK bytes), then with FILL the row's keys, vectors and hash
+  unsigned char* wl = smem_n + (size_t)wv * (NBR_LDS_CHUNK + 24 * (size_t)cap + 2 * (size_t)hslots);
+  double (*e_sh)[3] = (double (*)[3])wl;                          // [NBR_CHUNK] image shifts
+  int* e_start = (int*)(wl + NBR_CHUNK * 3 * sizeof(double));    // [NBR_CHUNK] first record of each stencil bin
+  unsigned* e_img = (unsigned*)(e_start + NBR_CHUNK);             // [NBR_CHUNK] packed images
+  int* e_pre = (int*)(e_img + NBR_CHUNK);                         // [NBR_CHUNK + 1] candidate prefix
+  u64* rk = (u64*)(wl + NBR_LDS_CHUNK);                           // [cap] keys of the row (FILL)
+  float4* rv = (float4*)(rk + cap);                                // [cap] vectors
+  unsigned* htab = (unsigned*)(rv + cap);                          // [hslots / 2] words: 16-bit hash slots (FILL)
   const mg_struct_dev* st = &str[atom_str[i]];
   const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
   const double RC = st->rc2;
@@ -481,6 +506,13 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
   // rank sort: the keys of a row are distinct, so the number of smaller keys is an entry's place in the sorted row;
   // every lane counts against broadcast reads of the whole row and writes its entries to their places
   const int off = padded ? i * cap_row : row_off[i];
+  // the row's reverse-lookup table: (neighbour atom, image) -> sorted position + 1, open addressing with linear
+  // probing in hslots >= 2 * cap slots (at most half full, so every probe sequence ends at an empty slot).  Only the
+  // entries another row will look up go in: those whose reverse edge owns the pair (k_rev), i.e. j < i, or j == i with
+  // the larger packed image.
+  const int hmask = hslots - 1;
+  for (int q = lane; q < hslots / 2; q += WAVE) htab[q] = 0u;
+  wave_sync();
   for (int t0 = 0; t0 < M; t0 += 2 * WAVE) {
     const int ta = t0 + lane, tb = t0 + WAVE + lane;
     const u64 ka = ta < M ? rk[ta] : ~0ull, kb = tb < M ? rk[tb] : ~0ull;
@@ -491,30 +523,47 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
       ra += ku < ka ? 1 : 0;
       rb += ku < kb ? 1 : 0;
     }
-    if (ta < M) { key[off + ra] = ka; vec[off + ra] = rv[ta]; rev[off + ta] = -1; }
-    if (tb < M) { key[off + rb] = kb; vec[off + rb] = rv[tb]; rev[off + tb] = -1; }
+    if (ta < M) {
+      key[off + ra] = ka; vec[off + ra] = rv[ta]; rev[off + ta] = -1;
+      const int ja = key_j(ka);
+      const unsigned ia = (unsigned)(ka & 0x3fffffffull);
+      if (ja < i || (ja == i && neg_img(ia) < ia)) htab_insert(htab, rev_slot(ja, ia, hmask), hmask, (unsigned)(ra + 1));
+    }
+    if (tb < M) {
+      key[off + rb] = kb; vec[off + rb] = rv[tb]; rev[off + tb] = -1;
+      const int jb = key_j(kb);
+      const unsigned ib = (unsigned)(kb & 0x3fffffffull);
+      if (jb < i || (jb == i && neg_img(ib) < ib)) htab_insert(htab, rev_slot(jb, ib, hmask), hmask, (unsigned)(rb + 1));
+    }
   }
+  wave_sync();
+  unsigned* ht = (unsigned*)(rhash + (size_t)i * hslots);  // hslots is even: the row's table is word aligned
+  for (int q = lane; q < hslots / 2; q += WAVE) ht[q] = htab[q];
 }
 
-// Reverse edge of e = (i -> j, img): the entry (j -> i, -img) of row j, found by binary search on keys.
-// Edges come in mutual pairs e = (i -> j, img), e' = (j -> i, -img), with rev[e] = e' and rev[e'] = e, so only
-// the edge of each pair with the smaller (i, img) looks the other up and writes both.  One wave per atom, four atoms
-// per block: the wave compacts the slots it owns into LDS, then each lane looks up two of them at once with
-// branchless lower-bound searches whose loads are independent of each other (a row of up to 128 neighbours costs
-// ceil(log2(row)) + 1 dependent loads).  Every load stays inside row j (an empty row searches the lane's own edge
-// instead, and the missing reverse edge is flagged).
-__device__ __forceinline__ int rev_lookup_base(const int* __restrict__ row_off, const int* __restrict__ rowcnt, int j,
-                                               int own, int& n) {
-  const int b = row_off[j];
-  n = rowcnt[j];
-  if (n > 0) return b;
-  n = 1;
-  return own;
-}
-
+// Reverse edge of e = (i -> j, img): the entry (j -> i, -img) of row j.  Edges come in mutual pairs, rev[e] = e' and
+// rev[e'] = e, so only the edge that owns its pair (j > i, or j == i with the smaller packed image) looks the other
+// up and writes both.  One wave per atom, four atoms per block: the wave compacts the slots it owns into LDS, then
+// each lane probes row j's hash table (built with the row by the fill): a slot value v > 0 names position v - 1, which
+// is checked against the key (one table load and one key load per probe, about 1.3 probes at the table's load of at
+// most one half, against log2(row) + 1 dependent key loads for a search of the sorted row).  Every load stays inside
+// row j's table and row (positions above rowcnt[j] are rejected); an empty slot, or no match, is a missing partner
+// (error flag, rev = self).
 #define REV_TILE 256
+__device__ __forceinline__ int rev_find(const unsigned short* __restrict__ tab, const u64* __restrict__ key, int base,
+                                        int n, u64 want, int h, int hmask) {
+  for (int pr = 0; pr <= hmask; pr++) {
+    const int v = tab[h];
+    if (v == 0 || v > n) return -1;
+    if (key[base + v - 1] == want) return v - 1;
+    h = (h + 1) & hmask;
+  }
+  return -1;
+}
+
 __global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc, const int* __restrict__ row_off,
                                              const int* __restrict__ rowcnt, const u64* __restrict__ key,
+                                             const unsigned short* __restrict__ rhash, int hslots,
                                              int* __restrict__ rev, int* __restrict__ err, const int* __restrict__ ovf) {
   __shared__ int olist[4][REV_TILE];
   if (*ovf) return;
@@ -524,9 +573,10 @@ __global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc,
   const int lane = threadIdx.x & 63;
   const int off = row_off[i], M = rowcnt[i];
   const int si = spc[i];
+  const int hmask = hslots - 1;
   int* ol = olist[wv];
   for (int c0 = 0; c0 < M; c0 += REV_TILE) {
-    // slots of this tile whose edge owns its pair: j > i, or j == i with the smaller packed image
+    // slots of this tile whose edge owns its pair
     int no = 0;
     for (int t0 = c0; t0 < c0 + REV_TILE && t0 < M; t0 += WAVE) {
       const int t = t0 + lane;
@@ -543,36 +593,16 @@ __global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc,
       no += __popcll(bm);
     }
     wave_sync();
-    for (int q0 = 0; q0 < no; q0 += 2 * WAVE) {
-      const int qa = q0 + lane, qb = q0 + WAVE + lane;
-      const bool va = qa < no, vb = qb < no;
-      const int ea = off + (va ? ol[qa] : 0), eb = off + (vb ? ol[qb] : 0);
-      const u64 ka = key[ea], kb = key[eb];
-      const u64 xa = make_key(si, i, neg_img((unsigned)(ka & 0x3fffffffull)));
-      const u64 xb = make_key(si, i, neg_img((unsigned)(kb & 0x3fffffffull)));
-      int na, nb;
-      const int ba = rev_lookup_base(row_off, rowcnt, key_j(ka), ea, na);
-      const int bb = rev_lookup_base(row_off, rowcnt, key_j(kb), eb, nb);
-      // lower bounds: pa = number of keys of row j below the target
-      int pa = 0, pb = 0;
-      const int nmax = na > nb ? na : nb;
-      for (int step = 1 << (31 - __clz(nmax)); step > 0; step >>= 1) {
-        const int ya_i = pa + step, yb_i = pb + step;
-        const u64 ya = key[ba + (ya_i < na ? ya_i : na) - 1];
-        const u64 yb = key[bb + (yb_i < nb ? yb_i : nb) - 1];
-        if (ya_i <= na && ya < xa) pa = ya_i;
-        if (yb_i <= nb && yb < xb) pb = yb_i;
-      }
-      const u64 za = key[ba + (pa < na ? pa : na - 1)];
-      const u64 zb = key[bb + (pb < nb ? pb : nb - 1)];
-      if (va) {
-        if (pa < na && za == xa) { rev[ea] = ba + pa; rev[ba + pa] = ea; }
-        else { atomicOr(err, 1); rev[ea] = ea; }
-      }
-      if (vb) {
-        if (pb < nb && zb == xb) { rev[eb] = bb + pb; rev[bb + pb] = eb; }
-        else { atomicOr(err, 1); rev[eb] = eb; }
-      }
+    for (int q = lane; q < no; q += WAVE) {
+      const int e = off + ol[q];
+      const u64 k = key[e];
+      const int j = key_j(k);
+      const unsigned nim = neg_img((unsigned)(k & 0x3fffffffull));
+      const int base = row_off[j], n = rowcnt[j];
+      const int p = rev_find(rhash + (size_t)j * hslots, key, base, n, make_key(si, i, nim), rev_slot(i, nim, hmask),
+                             hmask);
+      if (p >= 0) { rev[e] = base + p; rev[base + p] = e; }
+      else { atomicOr(err, 1); rev[e] = e; }
     }
     wave_sync();
   }
@@ -1365,33 +1395,46 @@ int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_co
 int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
                   const int* bin_start, const void* brec, int* cnt, int* rowcnt, int* misc, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_nbr<false>, dim3((N + NBR_WPB - 1) / NBR_WPB), dim3(64 * NBR_WPB), 0, st, N, x, spc, atom_str, str,
+  hipLaunchKernelGGL(k_nbr<false>, dim3((N + NBR_WPB - 1) / NBR_WPB), dim3(64 * NBR_WPB), (size_t)NBR_WPB * NBR_LDS_CHUNK,
+                     st, N, x, spc, atom_str, str,
                      bin, bin_start, (const BinRec*)brec, cnt, rowcnt, misc, (int*)nullptr, (u64*)nullptr,
-                     (float4*)nullptr, (int*)nullptr, 0, 0, 0, NBR_WPB);
+                     (float4*)nullptr, (int*)nullptr, (unsigned short*)nullptr, 0, 0, 0, 0, NBR_WPB);
   return mgk_status();
 }
 
 // Fill and sort the rows (LDS tile of cap entries: a power of two >= the longest row) and build the reverse-edge
 // index.  padded: no count pass, row i at i * maxrow.  Rows longer than 2048 neighbours are rejected (-2).
+// slots of a row's reverse-lookup hash table: twice the row's LDS tile (a power of two >= the longest row, >= 64)
+int mgk_rev_slots(int maxrow) {
+  int cap = 64;
+  while (cap < maxrow) cap <<= 1;
+  return 2 * cap;
+}
+
 int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
                  const int* bin_start, const void* brec, int* row_off, int* rowcnt, int* cnt, void* key, void* vec,
-                 int* rev, int* misc, int maxrow, int padded, hipStream_t st) {
+                 int* rev, uint16_t* rhash, int* misc, int maxrow, int padded, hipStream_t st) {
   if (N == 0) return 0;
   int cap = 64;
   while (cap < maxrow) cap <<= 1;
   if (cap > 2048) return -2;
-  // per wave: keys + vectors (3 u64 per slot); fewer waves per block when long rows would not fit 64 KiB
+  // per wave: the stencil chunk tables, keys + vectors (3 u64 per slot) and the 2 * cap 16-bit slots of the hash
+  // table; fewer waves per block when long rows would not fit 64 KiB (a row of 2,048 fits one wave: 59,664 bytes)
+  const int hslots = mgk_rev_slots(maxrow);
+  const size_t per_wave = NBR_LDS_CHUNK + (size_t)cap * (sizeof(u64) + sizeof(float4)) + (size_t)hslots * sizeof(uint16_t);
+  const size_t budget = 64 * 1024;
   int wpb = NBR_WPB;
-  while (wpb > 1 && (size_t)wpb * cap * (sizeof(u64) + sizeof(float4)) > 64 * 1024) wpb >>= 1;
-  const size_t sm = (size_t)wpb * cap * (sizeof(u64) + sizeof(float4));
-  if (sm > 64 * 1024 &&
+  while (wpb > 1 && (size_t)wpb * per_wave > budget) wpb >>= 1;
+  const size_t sm = (size_t)wpb * per_wave;
+  if (sm > budget &&
       hipFuncSetAttribute((const void*)&k_nbr<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
     return -1;
   hipLaunchKernelGGL(k_nbr<true>, dim3((N + wpb - 1) / wpb), dim3(64 * wpb), sm, st, N, x, spc, atom_str,
                      str, bin, bin_start, (const BinRec*)brec,
-                     cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev, cap, padded, maxrow, wpb);
+                     cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev, (unsigned short*)rhash, hslots, cap,
+                     padded, maxrow, wpb);
   hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, (const int*)row_off, (const int*)rowcnt,
-                     (const u64*)key, rev, misc + 1, (const int*)(misc + 3));
+                     (const u64*)key, (const unsigned short*)rhash, hslots, rev, misc + 1, (const int*)(misc + 3));
   return mgk_status();
 }
 

@@ -204,3 +204,41 @@ def test_bf16_mlp_error_bounds(ref_cases, name):
         assert de < etol
         assert df < ftol
         ctx.close()
+
+
+@pytest.mark.parametrize("a", [2.4, 1.9])
+def test_long_rows_match_oracle(a):
+    """Compressed Cu fcc (2x2x2 cells, jittered) with rows of about 270 and 540 neighbours: LDS tiles of 512 and
+    1,024 entries, two and one wave per block in the neighbour fill, reverse-lookup tables of 1,024 and 2,048 slots.
+    Neighbour sets bit-exact against the oracle's LIST restatement; descriptors, energy and forces against the
+    oracle in FP64 (FP32 sums over several hundred neighbours: relative bars)."""
+    import maise_amd
+    from oracle import pyoracle as po
+    print(f"[gpu case] long rows Cu fcc a={a}", flush=True)
+    rng = np.random.default_rng(17)
+    cell = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    frac = np.concatenate([(cell + np.array([i, j, k])) / 2.0 for i in range(2) for j in range(2) for k in range(2)])
+    lat = np.eye(3) * 2 * a
+    x = frac @ lat + rng.normal(0, 0.02, frac.shape)
+    spc = np.zeros(len(x), np.int32)
+    path = os.path.join(GOLDEN, "models", "nn_Cu_d3_v1")
+    m = maise_amd.Model(path)
+    c = maise_amd.Context(m)
+    c.set_structure(spc, lat, x)
+    c.eval(forces=True, keep_g=True)
+    r = c.result()
+    G = c.descriptors().astype(np.float64)
+    row, j, img = c.neighbors()
+    c.close()
+    om = po.OracleModel(path)
+    cnt, ent, _ = po.neighbors(lat, x, om.rc)
+    counts = np.diff(row)
+    print("row lengths", counts.min(), counts.max(), flush=True)
+    assert counts.max() > (250 if a > 2 else 500)
+    np.testing.assert_array_equal(counts, cnt)
+    assert nbr_sets(counts, np.concatenate([j[:, None], img], axis=1)) == nbr_sets(cnt, ent)
+    o = om.evaluate(spc, lat, x, want_g=True, threshold=False)
+    assert (np.abs(G - o["G"]) / (1.0 + np.abs(o["G"]))).max() < 1e-4
+    N = len(x)
+    assert abs(r["E"] - o["E"]) / N < 1e-4 * (1.0 + abs(o["E"]) / N)
+    assert np.abs(r["F"] - o["F"]).max() < 1e-4 * (1.0 + np.abs(o["F"]).max())
