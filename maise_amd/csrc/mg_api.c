@@ -621,9 +621,10 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   if (timing) HIPCHK(hipEventRecord(ev[EV_MLP], st));
   if (forces) {
     KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
-                   c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, pstride ? c->d_gpl : NULL, c->d_npl, pstride, ovf, st));
+                   c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, pstride ? c->d_gpl : NULL, c->d_npl, pstride, c->d_rev,
+                   c->d_misc + 1, ovf, st));
     if (timing) HIPCHK(hipEventRecord(ev[EV_FORCE], st));
-    KCHK(mgk_gather(N, c->d_row_off, c->d_rowcnt, c->d_rev, c->d_misc + 1, c->d_fself, c->d_fe, c->d_F, ovf, st));
+    KCHK(mgk_gather(N, c->d_row_off, c->d_rowcnt, c->d_fself, c->d_fe, c->d_F, ovf, st));
   } else if (timing) {
     HIPCHK(hipEventRecord(ev[EV_FORCE], st));
   }

@@ -146,9 +146,10 @@ int mgk_mlp(int N, const int *spc, const float *G, const float *W, int wstride, 
 int mgk_force(int N, int S, int variant, const int *spc, const int *row_off, const int *rowcnt, const int *cnt,
               const void *vec,
               const mg_basis_dev *bp, const float *dEdG, void *fself, void *fe, float *vir, float urep, int maxrow,
-              int wpb, const unsigned *gpl, const int *npl, size_t pstride, const int *ovf, hipStream_t st);
-int mgk_gather(int N, const int *row_off, const int *rowcnt, const int *rev, int *err, const void *fself, const void *fe,
-               double *F, const int *ovf, hipStream_t st);
+              int wpb, const unsigned *gpl, const int *npl, size_t pstride, const int *rev, int *err, const int *ovf,
+              hipStream_t st);
+int mgk_gather(int N, const int *row_off, const int *rowcnt, const void *fself, const void *fe, double *F,
+               const int *ovf, hipStream_t st);
 int mgk_clear(int *a, int na, int *b, int nb, int *c, int nc, hipStream_t st);
 int mgk_check_caps(int N, const int *row_off, int *misc, int cap_edges, int cap_row, hipStream_t st);
 

@@ -1078,7 +1078,8 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
                                                     float4* __restrict__ fself, float4* __restrict__ fe,
                                                     float* __restrict__ vir, float urep, int cap,
                                                     const unsigned* __restrict__ gpl, const int* __restrict__ npl,
-                                                    size_t pstride, const int* __restrict__ ovf) {
+                                                    size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
+                                                    const int* __restrict__ ovf) {
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   constexpr int ZL = MG_MAXZ * MG_MAXL;
@@ -1091,6 +1092,9 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
   const int D = bp->D;
   RowInfo ri;
   row_info(ri, i, spc, row_off, rowcnt, cnt, S);
+  // the reverse slots of the first two batches of the row, loaded now so the final scatter does not wait on them
+  const int rq0 = lane < ri.M ? rev[ri.off + lane] : -1;
+  const int rq1 = lane + WAVE < ri.M ? rev[ri.off + lane + WAVE] : -1;
   float4* rec = smem_f + (size_t)wv * ((R + 2) * cap + MG_GRID / 4 + PCH / 4);
   float4* gts4 = rec + R * cap;                  // [MG_GRID / 4]
   float4* fnb4 = gts4 + MG_GRID / 4;             // [cap] force on each neighbour slot (pair terms, triplet j side)
@@ -1235,7 +1239,9 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
       wave_sync();
     }
   // every term of T_i depends on the neighbour vectors d_e only, and fnb[e] = -dT_i/dd_e summed over them: so
-  // F_i = -sum_e fnb[e] and the virial U = -sum_e dT/dd_e (x) d_e = sum_e fnb[e] (x) d_e (symmetric)
+  // F_i = -sum_e fnb[e] and the virial U = -sum_e dT/dd_e (x) d_e = sum_e fnb[e] (x) d_e (symmetric).  The force on
+  // neighbour j of edge e = (i -> j) goes to the slot of the reverse edge rev[e] = (j -> i) in row j, so each atom's
+  // received forces lie in its own row (a missing reverse edge raises the error flag)
   wave_sync();
   for (int t = lane; t < ri.M; t += WAVE) {
     const float4 g1 = fnb4[t], g2 = fnk4[t];
@@ -1244,7 +1250,9 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
     fx -= gx; fy -= gy; fz -= gz;
     u0 += gx * v.x; u1 += gy * v.y; u2 += gz * v.z;
     u3 += 0.5f * (gx * v.y + gy * v.x); u4 += 0.5f * (gy * v.z + gz * v.y); u5 += 0.5f * (gz * v.x + gx * v.z);
-    fe[ri.off + t] = make_float4(gx, gy, gz, 0.0f);
+    const int q = t < WAVE ? rq0 : (t < 2 * WAVE ? rq1 : rev[ri.off + t]);
+    if (q >= 0) fe[q] = make_float4(gx, gy, gz, 0.0f);
+    else atomicOr(err, 1);
   }
   fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
   u0 = wave_sum(u0); u1 = wave_sum(u1); u2 = wave_sum(u2);
@@ -1256,9 +1264,9 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
   }
 }
 
-// F_a = own term + sum of the reverse edges' neighbour terms; 4 lanes per atom, fixed order, FP64 accumulation
+// Force on atom a: its own term plus what its neighbours' centres sent it, which the force kernel wrote into a's row
+// (slot e of row a holds the force from the centre at the far end of e): four lanes per atom stream the row.
 __global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ row_off, const int* __restrict__ rowcnt,
-                                                const int* __restrict__ rev, int* __restrict__ err,
                                                 const float4* __restrict__ fself, const float4* __restrict__ fe,
                                                 double* __restrict__ F, const int* __restrict__ ovf) {
   if (*ovf) return;
@@ -1268,9 +1276,7 @@ __global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ r
   if (a < N) {
     const int e1 = row_off[a] + rowcnt[a];
     for (int e = row_off[a] + sub; e < e1; e += 4) {
-      const int q = rev[e];
-      if (q < 0) { atomicOr(err, 1); continue; }  // reverse edge missing (flagged; the evaluation fails)
-      float4 v = fe[q];
+      const float4 v = fe[e];
       fx += v.x; fy += v.y; fz += v.z;
     }
   }
@@ -1491,20 +1497,21 @@ int mgk_mlp(int N, const int* spc, const float* G, const float* W, int wstride, 
 int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, const int* rowcnt, const int* cnt,
               const void* vec,
               const mg_basis_dev* bp, const float* dEdG, void* fself, void* fe, float* vir, float urep, int maxrow,
-              int wpb, const unsigned* gpl, const int* npl, size_t pstride, const int* ovf, hipStream_t st) {
+              int wpb, const unsigned* gpl, const int* npl, size_t pstride, const int* rev, int* err, const int* ovf,
+              hipStream_t st) {
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * (R + 2) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
   DISPATCH_PU(k_force, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
-              urep, cap, gpl, npl, pstride, ovf);
+              urep, cap, gpl, npl, pstride, rev, err, ovf);
   return mgk_status();
 }
 
-int mgk_gather(int N, const int* row_off, const int* rowcnt, const int* rev, int* err, const void* fself, const void* fe,
-               double* F, const int* ovf, hipStream_t st) {
+int mgk_gather(int N, const int* row_off, const int* rowcnt, const void* fself, const void* fe, double* F,
+               const int* ovf, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rowcnt, rev, err, (const float4*)fself,
+  hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rowcnt, (const float4*)fself,
                      (const float4*)fe, F, ovf);
   return mgk_status();
 }
