@@ -952,7 +952,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
       const int blk = bp->ind4[ds1][ds2];
       const int pjk = sa * MG_MAXSPC + sb;
       const float rc = bp->rc[pjk];
-      const float rinv = 1.0f / rc;
+      const float rinv = 1.0f / rc, rinvh = 0.5f * rinv;
       float ejk[NE];
 #pragma unroll
       for (int e = 0; e < NE; e++) ejk[e] = -(float)MG_LOG2E * bp->eta4[pjk][e];
@@ -966,9 +966,9 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         float r3s = cx * cx + cy * cy + cz * cz;
         float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
         float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
-        float fc3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
+        float fc3x2 = __builtin_amdgcn_cosf(r3 * rinvh) + 1.0f;  // 2 fc(r_jk); the 2 cancels the G4 factor 2
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * ia.x * ib.x;
-        float F = ia.y * ib.y * fc3;  // the factor 2 of the G4 definition is applied with the zeta prefactor
+        float F = ia.y * ib.y * fc3x2;  // 2 fc fc fc: the factor 2 of the G4 definition
         float P[MG_MAXZ * MG_MAXL];
         angular<PAT>(bp, cs, P, nullptr);
         float Bq[NE];
@@ -994,9 +994,8 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         gout += np;
         if (lane == 0) npl[(size_t)i * MG_NCLS + cls] = np;
       }
-      // lane l: grid cell l = (e*MAXZ + z)*MAXL + lambda; the zeta prefactor 2^(1 - zeta) and the factor 2 of the
-      // G4 definition go on here
-      float mine = reduce_scatter64(V, lane) * (2.0f * bp->pref[(lane / MG_MAXL) % MG_MAXZ]);
+      // lane l: grid cell l = (e*MAXZ + z)*MAXL + lambda; the zeta prefactor 2^(1 - zeta) goes on here
+      float mine = reduce_scatter64(V, lane) * bp->pref[(lane / MG_MAXL) % MG_MAXZ];
       int cell = lane < bp->ng4 ? bp->g4cell[lane] : 0;
       float v = __shfl(mine, cell, WAVE);
       if (lane < bp->ng4) Gi[blk + lane] = v;
@@ -1154,7 +1153,7 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
       const int blk = bp->ind4[ds1][ds2];
       const int pjk = sa * MG_MAXSPC + sb, pij = ri.si * MG_MAXSPC + sa, pik = ri.si * MG_MAXSPC + sb;
       const float rc = bp->rc[pjk];
-      const float rinv = 1.0f / rc;
+      const float rinv = 1.0f / rc, rinvh = 0.5f * rinv;
       const float dfk = -0.5f * 3.14159265358979f * rinv;
       {
         // grid cell lane, with the zeta prefactor 2^(1 - zeta) and the factor 2 of the G4 definition folded in
@@ -1179,8 +1178,9 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
         float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
         float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
         float i3 = __builtin_amdgcn_rcpf(r3);
-        float f3 = 0.5f * (cospi_(r3 * rinv) + 1.0f);
-        float d3 = dfk * sinpi_(r3 * rinv);
+        const float th = r3 * rinvh;  // cos(pi r / rc) = cos(2 pi th) in the hardware's revolutions
+        float f3 = __builtin_fmaf(__builtin_amdgcn_cosf(th), 0.5f, 0.5f);
+        float d3 = dfk * __builtin_amdgcn_sinf(th);
         float i1 = ia.x, i2 = ib.x, f1 = ia.y, f2 = ib.y, d1 = ia.z, d2 = ib.z;
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * i1 * i2;
         float P[ZL], Pd[ZL];
@@ -1587,6 +1587,7 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   const int sp = tl.x, start = tl.y, count = tl.z;
   const int lane = threadIdx.x, col = lane & 31, h = lane >> 5;
   const int atom = col < count ? perm[start + col] : -1;
+  extern __shared__ float gsm[];  // [32][Kmax + 1] staged inputs
   const int D = f.D;
   const size_t so = (size_t)sp * f.stride_slots;
   const float* fa = frag + so * 64;
@@ -1598,7 +1599,6 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   const float* w2 = b1 + HB2 * 32;
   const float* xo = w2 + HB2 * 32 + 1;
   const float b2 = w2[HB2 * 32];
-  const float* g = atom >= 0 ? G + (size_t)atom * D : nullptr;
 
   // ---- layer 1: a1[u][atom] = b0 + W0' x
   f32x16 a1[HB1];
@@ -1606,15 +1606,43 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   for (int rb = 0; rb < HB1; rb++)
 #pragma unroll
     for (int r = 0; r < 16; r++) a1[rb][r] = b0[rb * 32 + acc_row(r, h)];
-  for (int st = 0; st < f.k1; st++) {
-    float xb[KS];
+  // the tile's inputs x - xoff staged in LDS first ([32][Kmax + 1], zero past D and for missing atoms): 32 rows of
+  // coalesced loads all in flight at once, instead of two dependent memory round trips per K step
+  const int k1 = f.k1;
+  const int Kmax = BF16 ? 16 * k1 : 2 * k1;
+  const int RS = Kmax + 1;
+  for (int k0 = 0; k0 < Kmax; k0 += WAVE) {
+    const int k = k0 + lane, kc = k < D ? k : D - 1;
+    const float xk = xo[kc];
+    float vals[32];
 #pragma unroll
-    for (int v = 0; v < KS; v++) {
-      const int k = BF16 ? 16 * st + 8 * h + v : 2 * st + h;
-      xb[v] = (g && k < D) ? g[k] - xo[k] : 0.0f;
+    for (int c = 0; c < 32; c++) {
+      const int ac = __shfl(atom, c, WAVE);
+      const float keep = (ac >= 0 && k < D) ? 1.0f : 0.0f;  // a product, not a select: the loads stay unconditional
+      vals[c] = (G[(size_t)(ac >= 0 ? ac : 0) * D + kc] - xk) * keep;
+    }
+    if (k < Kmax)
+#pragma unroll
+      for (int c = 0; c < 32; c++) gsm[c * RS + k] = vals[c];
+  }
+  wave_sync();
+  const float* gr = gsm + col * RS;
+  // four steps at a time (steps past k1 multiply the zero padding): their weight loads are in flight together
+  for (int st0 = 0; st0 < k1; st0 += 4) {
+    float xs[4][KS];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int su = st0 + u < k1 ? st0 + u : k1 - 1;
+#pragma unroll
+      for (int v = 0; v < KS; v++) xs[u][v] = st0 + u < k1 ? gr[BF16 ? 16 * su + 8 * h + v : 2 * su + h] : 0.0f;
     }
 #pragma unroll
-    for (int rb = 0; rb < HB1; rb++) a1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l1 * 64 + (rb * f.k1 + st) * 64 + lane, xb, a1[rb]);
+    for (int u = 0; u < 4; u++) {
+      const int st = st0 + u < k1 ? st0 + u : k1 - 1;
+#pragma unroll
+      for (int rb = 0; rb < HB1; rb++)
+        a1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l1 * 64 + (rb * k1 + st) * 64 + lane, xs[u], a1[rb]);
+    }
   }
 #pragma unroll
   for (int rb = 0; rb < HB1; rb++)
@@ -1714,7 +1742,9 @@ extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, cons
   const int4* t = (const int4*)tiles;
   const __bf16* fb = (const __bf16*)frag_bf;
   const __bf16* fl = (const __bf16*)frag_lo;
-#define MLP_L(P, H1B, H2B) hipLaunchKernelGGL((k_mlp_mfma<P, H1B, H2B>), dim3(ntiles), dim3(64), 0, st, t, perm, G, frag, fb, fl, vecs, f, Ei, dEdG)
+  const size_t sm = (size_t)32 * ((bf16 ? 16 : 2) * f.k1 + 1) * sizeof(float);  // staged inputs of a tile
+  if (sm > 64 * 1024) return -2;
+#define MLP_L(P, H1B, H2B) hipLaunchKernelGGL((k_mlp_mfma<P, H1B, H2B>), dim3(ntiles), dim3(64), sm, st, t, perm, G, frag, fb, fl, vecs, f, Ei, dEdG)
 #define MLP_P(P)                                  \
   do {                                            \
     if (f.hb1 == 1 && f.hb2 == 1) MLP_L(P, 1, 1); \
