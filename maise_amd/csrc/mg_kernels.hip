@@ -1408,8 +1408,9 @@ int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, c
   return mgk_status();
 }
 
-// Fill and sort the rows (LDS tile of cap entries: a power of two >= the longest row) and build the reverse-edge
-// index.  padded: no count pass, row i at i * maxrow.  Rows longer than 2048 neighbours are rejected (-2).
+// Fill and sort the rows (LDS tile of cap entries: a power of two >= the longest row) with their reverse-lookup hash
+// tables; mgk_rev then builds the reverse-edge index.  padded: no count pass, row i at i * maxrow.  Rows longer than
+// 2048 neighbours are rejected (-2).
 // slots of a row's reverse-lookup hash table: twice the row's LDS tile (a power of two >= the longest row, >= 64)
 int mgk_rev_slots(int maxrow) {
   int cap = 64;
@@ -1439,8 +1440,16 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
                      str, bin, bin_start, (const BinRec*)brec,
                      cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev, (unsigned short*)rhash, hslots, cap,
                      padded, maxrow, wpb);
-  hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, (const int*)row_off, (const int*)rowcnt,
-                     (const u64*)key, (const unsigned short*)rhash, hslots, rev, misc + 1, (const int*)(misc + 3));
+  return mgk_status();
+}
+
+// Reverse edges of rows filled by mgk_nbr_fill with the same maxrow (their hash tables have mgk_rev_slots(maxrow)
+// slots); misc[1] takes the error flag, misc[3] (overflow) skips the launch's work.
+int mgk_rev(int N, const int* spc, const int* row_off, const int* rowcnt, const void* key, const uint16_t* rhash,
+            int maxrow, int* rev, int* misc, hipStream_t st) {
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, row_off, rowcnt, (const u64*)key,
+                     (const unsigned short*)rhash, mgk_rev_slots(maxrow), rev, misc + 1, (const int*)(misc + 3));
   return mgk_status();
 }
 
