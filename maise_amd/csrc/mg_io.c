@@ -100,6 +100,7 @@ int mg_poscar_read(const char *path, int nd, int cap, char *title, double *lat, 
   if (ns_sym != ns) { fclose(in); return mg_fail(MG_ERR_ARG, "%s: the species symbols are missing", path); }
   int N = 0;
   for (int k = 0; k < ns; k++) {
+    if (TN[k] < 0 || TN[k] > 100000000 - N) { fclose(in); return mg_fail(MG_ERR_IO, "%s: bad atom counts", path); }
     spcn[k] = TN[k];
     spcz[k] = symbol_z(t[k]);
     if (spcz[k] == 0) { fclose(in); return mg_fail(MG_ERR_ARG, "%s: unknown species %s", path, t[k]); }

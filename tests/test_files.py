@@ -179,6 +179,9 @@ def test_reader_errors(tmp_path):
     bad.write_text("unknown\n1.0\n4 0 0\n0 4 0\n0 0 4\nQq\n1\ndirect\n0 0 0\n")
     with pytest.raises(MaiseError, match="unknown species"):
         pc.read_poscar(str(bad))
+    bad.write_text("negative\n1.0\n4 0 0\n0 4 0\n0 0 4\nCu\n-3\ndirect\n0 0 0\n")
+    with pytest.raises(MaiseError, match="bad atom counts"):
+        pc.read_poscar(str(bad))
 
 
 def test_element_table():
