@@ -159,13 +159,6 @@ int mg_ctx_create(const mg_model *m, int device, mg_ctx **out) {
   c->m = m;
   c->device = device;
   if (hipStreamCreate(&c->st) != hipSuccess) { free(c); return mg_fail(MG_ERR_HIP, "hipStreamCreate failed"); }
-  if (hipStreamCreate(&c->st2) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
-    hipStreamDestroy(c->st);
-    free(c);
-    return mg_fail(MG_ERR_HIP, "second stream or its events could not be created");
-  }
   if (hipHostMalloc((void **)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     hipStreamDestroy(c->st);
     free(c);
@@ -213,10 +206,6 @@ void mg_ctx_free(mg_ctx *c) {
     for (int q = 0; q < MG_TRING; q++)
       for (int k = 0; k < EV_N; k++) hipEventDestroy(c->tev[q][k]);
   if (c->h_flags) hipHostFree(c->h_flags);
-  if (c->st2) hipStreamSynchronize(c->st2);
-  if (c->ev_fork) hipEventDestroy(c->ev_fork);
-  if (c->ev_join) hipEventDestroy(c->ev_join);
-  if (c->st2) hipStreamDestroy(c->st2);
   if (c->st) hipStreamDestroy(c->st);
   free(c->h_str); free(c->h_atom_off); free(c->h_nd); free(c->h_lat);
   free(c);
@@ -619,24 +608,17 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
                       c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, 1, st));
   }
-  /* reverse edges (needed by the force kernel only) on the second stream, beside the descriptors and the MLP */
-  if (forces) {
-    HIPCHK(hipEventRecord(c->ev_fork, st));
-    HIPCHK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
-    KCHK(mgk_rev(N, c->d_spc, c->d_row_off, c->d_rowcnt, c->d_key, c->d_rhash, cap_row, c->d_rev, c->d_misc, c->st2));
-    HIPCHK(hipEventRecord(c->ev_join, c->st2));
-  }
   if (timing) HIPCHK(hipEventRecord(ev[EV_NBR], st));
   /* descriptors, MLP, forces */
   KCHK(mgk_desc(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_G, c->d_erep,
-                urep, cap_row, c->wpb_desc, pstride ? c->d_gpl : NULL, c->d_npl, pstride, ovf, st));
+                urep, cap_row, c->wpb_desc, pstride ? c->d_gpl : NULL, c->d_npl, pstride, c->d_key, c->d_rhash,
+                forces ? c->d_rev : NULL, c->d_misc + 1, ovf, st));
   if (timing) HIPCHK(hipEventRecord(ev[EV_DESC], st));
   if (m->mfma_ok)
     KCHK(mgk_mlp_mfma(c->ntiles, c->d_tiles, c->d_perm, c->d_G, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, m->fdesc,
                       c->prec, c->d_Ei, c->d_dEdG, st));
   else
     KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
-  if (forces) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
   if (timing) HIPCHK(hipEventRecord(ev[EV_MLP], st));
   if (forces) {
     KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4), c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,

@@ -47,8 +47,6 @@ struct mg_ctx {
   const mg_model *m;
   int device;
   hipStream_t st;
-  hipStream_t st2;           /* the reverse-edge lookup runs here, beside the descriptor kernel */
-  hipEvent_t ev_fork, ev_join;
   double urep;
   int prec;
   int timing;
@@ -135,13 +133,12 @@ int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, c
 int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
                  const int *bin_start, const void *brec, int *row_off, int *rowcnt, int *cnt, void *key, void *vec,
                  int *rev, uint16_t *rhash, int *misc, int maxrow, int padded, hipStream_t st);
-int mgk_rev(int N, const int *spc, const int *row_off, const int *rowcnt, const void *key, const uint16_t *rhash,
-            int maxrow, int *rev, int *misc, hipStream_t st);
 int mgk_rev_slots(int maxrow);
 int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *rowcnt, const int *cnt,
              const void *vec,
              const mg_basis_dev *bp, float *G, float *erep, float urep, int maxrow, int wpb, unsigned *gpl, int *npl,
-             size_t pstride, const int *ovf, hipStream_t st);
+             size_t pstride, const void *key, const uint16_t *rhash, int *rev, int *err, const int *ovf,
+             hipStream_t st);
 int mgk_mlp_mfma(int ntiles, const void *tiles, const int *perm, const float *G, const float *frag, const void *frag_bf,
                  const void *frag_lo, const float *vecs, const int *fdesc, int prec, float *Ei, float *dEdG,
                  hipStream_t st);

@@ -4,17 +4,21 @@
 //   k_prepare      CAST_CELL fractional->Cartesian, JAR wrap in FP64 (src/cmod.c:1112-1121, src/cell.c:446-508),
 //                  cell-list bin of every atom
 //   k_bin_fill     counting sort of atoms into bins (order inside a bin is free: rows are sorted later)
+//   k_scan_one     bin offsets (one block, LDS-staged passes)
 //   k_nbr<false>   one wave per atom: neighbour count per species over the bin stencil, FP64 test r < Rc^2 in
 //                  the reference's expression order (src/cell.c:109-123) so the neighbour SETS are bit-exact
-//   k_nbr<true>    fill: 64-bit key (species, j, image) + FP32 (d, r) per edge, row sorted by key in LDS
-//   k_rev          reverse-edge index by binary search in the neighbour's row
+//   k_nbr<true>    fill: 64-bit key (species, j, image) + FP32 (d, r) per edge, row sorted by key in LDS, and
+//                  the row's reverse-lookup hash table
 //   k_desc<NE>     one wave per atom: G2 over neighbours, G4 over neighbour pairs of each species class,
 //                  neighbour tile staged in LDS, (eta, zeta, lambda) grid accumulators in VGPRs,
-//                  reduce-scatter across the wave (src/nprs.c:504-566)
+//                  reduce-scatter across the wave (src/nprs.c:504-566); with forces, the reverse-edge index of
+//                  the row's owned pairs through the partners' hash tables
 //   k_mlp_fp32     per-atom D-H1-H2-1 forward and dE/dG backward (src/nmlp.c:367-431 + chain of 444-478)
+//   k_mlp_mfma     the same on MFMA for hidden layers up to 64 wide (FP32, BF16, compensated BF16)
 //   k_force<NE>    one wave per atom: recomputes pair/triplet geometry, contracts dG/dr with dE/dG in
-//                  registers, forces on neighbour slots via LDS atomics, per-atom virial (no Fn/Sn storage)
-//   k_gather       F_a = own term + sum over reverse edges (deterministic)
+//                  registers, forces on neighbour slots in LDS without atomics, per-atom virial (no Fn/Sn
+//                  storage); each neighbour's force goes to the reverse edge's slot in the neighbour's row
+//   k_gather       F_a = own term + sum over the atom's own row (deterministic)
 //   k_reduce       per-structure FP64 energy and virial sums (deterministic tree)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -508,7 +512,7 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
   const int off = padded ? i * cap_row : row_off[i];
   // the row's reverse-lookup table: (neighbour atom, image) -> sorted position + 1, open addressing with linear
   // probing in hslots >= 2 * cap slots (at most half full, so every probe sequence ends at an empty slot).  Only the
-  // entries another row will look up go in: those whose reverse edge owns the pair (k_rev), i.e. j < i, or j == i with
+  // entries another row will look up go in: those whose reverse edge owns the pair, i.e. j < i, or j == i with
   // the larger packed image.
   const int hmask = hslots - 1;
   for (int q = lane; q < hslots / 2; q += WAVE) htab[q] = 0u;
@@ -543,13 +547,11 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
 
 // Reverse edge of e = (i -> j, img): the entry (j -> i, -img) of row j.  Edges come in mutual pairs, rev[e] = e' and
 // rev[e'] = e, so only the edge that owns its pair (j > i, or j == i with the smaller packed image) looks the other
-// up and writes both.  One wave per atom, four atoms per block: the wave compacts the slots it owns into LDS, then
-// each lane probes row j's hash table (built with the row by the fill): a slot value v > 0 names position v - 1, which
-// is checked against the key (one table load and one key load per probe, about 1.3 probes at the table's load of at
-// most one half, against log2(row) + 1 dependent key loads for a search of the sorted row).  Every load stays inside
-// row j's table and row (positions above rowcnt[j] are rejected); an empty slot, or no match, is a missing partner
-// (error flag, rev = self).
-#define REV_TILE 256
+// up (in the descriptor kernel) and writes both.  The lookup probes row j's hash table (built with the row by the
+// fill): a slot value v > 0 names position v - 1, which is checked against the key (one table load and one key load
+// per probe, about 1.3 probes at the table's load of at most one half, against log2(row) + 1 dependent key loads for a
+// search of the sorted row).  Every load stays inside row j's table and row (positions above rowcnt[j] are
+// rejected); an empty slot, or no match, is a missing partner (error flag, rev = self).
 __device__ __forceinline__ int rev_find(const unsigned short* __restrict__ tab, const u64* __restrict__ key, int base,
                                         int n, u64 want, int h, int hmask) {
   for (int pr = 0; pr <= hmask; pr++) {
@@ -559,53 +561,6 @@ __device__ __forceinline__ int rev_find(const unsigned short* __restrict__ tab, 
     h = (h + 1) & hmask;
   }
   return -1;
-}
-
-__global__ __launch_bounds__(256) void k_rev(int N, const int* __restrict__ spc, const int* __restrict__ row_off,
-                                             const int* __restrict__ rowcnt, const u64* __restrict__ key,
-                                             const unsigned short* __restrict__ rhash, int hslots,
-                                             int* __restrict__ rev, int* __restrict__ err, const int* __restrict__ ovf) {
-  __shared__ int olist[4][REV_TILE];
-  if (*ovf) return;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
-  if (i >= N) return;
-  const int lane = threadIdx.x & 63;
-  const int off = row_off[i], M = rowcnt[i];
-  const int si = spc[i];
-  const int hmask = hslots - 1;
-  int* ol = olist[wv];
-  for (int c0 = 0; c0 < M; c0 += REV_TILE) {
-    // slots of this tile whose edge owns its pair
-    int no = 0;
-    for (int t0 = c0; t0 < c0 + REV_TILE && t0 < M; t0 += WAVE) {
-      const int t = t0 + lane;
-      bool own = false;
-      if (t < M) {
-        const u64 k = key[off + t];
-        const int j = key_j(k);
-        const unsigned im = (unsigned)(k & 0x3fffffffull);
-        own = j > i || (j == i && im < neg_img(im));
-      }
-      const u64 bm = __ballot(own);
-      const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
-      if (own) ol[no + below] = t;
-      no += __popcll(bm);
-    }
-    wave_sync();
-    for (int q = lane; q < no; q += WAVE) {
-      const int e = off + ol[q];
-      const u64 k = key[e];
-      const int j = key_j(k);
-      const unsigned nim = neg_img((unsigned)(k & 0x3fffffffull));
-      const int base = row_off[j], n = rowcnt[j];
-      const int p = rev_find(rhash + (size_t)j * hslots, key, base, n, make_key(si, i, nim), rev_slot(i, nim, hmask),
-                             hmask);
-      if (p >= 0) { rev[e] = base + p; rev[base + p] = e; }
-      else { atomicOr(err, 1); rev[e] = e; }
-    }
-    wave_sync();
-  }
 }
 
 // ---------------------------------------------------------------------------------------------- wave helpers
@@ -887,6 +842,8 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
                                                    const mg_basis_dev* __restrict__ bp, float* __restrict__ G,
                                                    float* __restrict__ erep, float urep, int cap,
                                                    unsigned* __restrict__ gpl, int* __restrict__ npl, size_t pstride,
+                                                   const u64* __restrict__ key, const unsigned short* __restrict__ rhash,
+                                                   int hslots, int* __restrict__ rev, int* __restrict__ err,
                                                    const int* __restrict__ ovf) {
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
@@ -902,6 +859,26 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
   float4* rec = smem_d + (size_t)wv * (R * cap + PCH / 4);
   unsigned* plist = (unsigned*)(rec + R * cap);
   stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
+  // reverse edges (for the force kernel): the edges of this row that own their pair look the partner up in the
+  // partner row's hash table and write both rev entries; the probes' load latency hides under the other waves'
+  // descriptor arithmetic
+  if (rev) {
+    const int hmask = hslots - 1;
+    for (int t = lane; t < ri.M; t += WAVE) {
+      const int e = ri.off + t;
+      const u64 k = key[e];
+      const int j = key_j(k);
+      const unsigned im = (unsigned)(k & 0x3fffffffull);
+      if (j > i || (j == i && im < neg_img(im))) {
+        const unsigned nim = neg_img(im);
+        const int base = row_off[j], n = rowcnt[j];
+        const int p = rev_find(rhash + (size_t)j * hslots, key, base, n, make_key(ri.si, i, nim),
+                               rev_slot(i, nim, hmask), hmask);
+        if (p >= 0) { rev[e] = base + p; rev[base + p] = e; }
+        else { atomicOr(err, 1); rev[e] = e; }
+      }
+    }
+  }
   wave_sync();
   float* Gi = G + (size_t)i * D;
 
@@ -1409,7 +1386,7 @@ int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, c
 }
 
 // Fill and sort the rows (LDS tile of cap entries: a power of two >= the longest row) with their reverse-lookup hash
-// tables; mgk_rev then builds the reverse-edge index.  padded: no count pass, row i at i * maxrow.  Rows longer than
+// tables; the descriptor kernel then builds the reverse-edge index.  padded: no count pass, row i at i * maxrow.  Rows longer than
 // 2048 neighbours are rejected (-2).
 // slots of a row's reverse-lookup hash table: twice the row's LDS tile (a power of two >= the longest row, >= 64)
 int mgk_rev_slots(int maxrow) {
@@ -1440,16 +1417,6 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
                      str, bin, bin_start, (const BinRec*)brec,
                      cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev, (unsigned short*)rhash, hslots, cap,
                      padded, maxrow, wpb);
-  return mgk_status();
-}
-
-// Reverse edges of rows filled by mgk_nbr_fill with the same maxrow (their hash tables have mgk_rev_slots(maxrow)
-// slots); misc[1] takes the error flag, misc[3] (overflow) skips the launch's work.
-int mgk_rev(int N, const int* spc, const int* row_off, const int* rowcnt, const void* key, const uint16_t* rhash,
-            int maxrow, int* rev, int* misc, hipStream_t st) {
-  if (N == 0) return 0;
-  hipLaunchKernelGGL(k_rev, dim3((N + 3) / 4), dim3(256), 0, st, N, spc, row_off, rowcnt, (const u64*)key,
-                     (const unsigned short*)rhash, mgk_rev_slots(maxrow), rev, misc + 1, (const int*)(misc + 3));
   return mgk_status();
 }
 
@@ -1486,12 +1453,15 @@ static int pick_wpb(int wpb, size_t sm_wave) {
 int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* rowcnt, const int* cnt,
              const void* vec,
              const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, int wpb, unsigned* gpl, int* npl,
-             size_t pstride, const int* ovf, hipStream_t st) {
+             size_t pstride, const void* key, const uint16_t* rhash, int* rev, int* err, const int* ovf,
+             hipStream_t st) {
+  const int hslots = mgk_rev_slots(maxrow);
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * R * sizeof(float4) + PCH * sizeof(unsigned);
-  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, G, erep, urep, cap, gpl, npl, pstride, ovf);
+  DISPATCH_PU(k_desc, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, G, erep, urep, cap, gpl, npl, pstride,
+              (const u64*)key, (const unsigned short*)rhash, hslots, rev, err, ovf);
   return mgk_status();
 }
 
