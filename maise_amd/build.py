@@ -23,7 +23,11 @@ HIPCC_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-munsafe
 CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
             f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["mg_kernels.hip", "mg_dyn.hip", "mg_sur.hip"]
+HIP_SOURCES = ["mg_kernels.hip", "mg_mlp.hip", "mg_dyn.hip", "mg_sur.hip"]
+# per-file options.  mg_kernels.hip: MFMA accumulators in ordinary VGPRs (the force kernel reads its results with
+# VALU instructions straight away, which the AGPR form would first copy out one register at a time); the MLP
+# kernels in mg_mlp.hip measured faster with the AGPR form
+HIP_EXTRA = {"mg_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 C_SOURCES = ["mg_model.c", "mg_api.c", "mg_dyn.c", "mg_sur.c", "mg_evo.c", "mg_io.c"]
 
 
@@ -43,7 +47,7 @@ def build(verbose: bool = False) -> str:
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", "_hip.o"))
-        _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        _run([hipcc, *HIPCC_FLAGS, *HIP_EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
     for src in C_SOURCES:
         obj = os.path.join(BUILD, src.replace(".c", ".o"))

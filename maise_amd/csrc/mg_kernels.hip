@@ -14,7 +14,7 @@
 //                  reduce-scatter across the wave (src/nprs.c:504-566); with forces, the reverse-edge index of
 //                  the row's owned pairs through the partners' hash tables
 //   k_mlp_fp32     per-atom D-H1-H2-1 forward and dE/dG backward (src/nmlp.c:367-431 + chain of 444-478)
-//   k_mlp_mfma     the same on MFMA for hidden layers up to 64 wide (FP32, BF16, compensated BF16)
+//   k_mlp_mfma     the same on MFMA for hidden layers up to 64 wide (mg_mlp.hip)
 //   k_force<NE>    one wave per atom: recomputes pair/triplet geometry, contracts dG/dr with dE/dG in
 //                  registers, forces on neighbour slots in LDS without atomics, per-atom virial (no Fn/Sn
 //                  storage); each neighbour's force goes to the reverse edge's slot in the neighbour's row
@@ -24,8 +24,8 @@
 #include <stdint.h>
 
 #include "mg_internal.h"
+#include "mg_dev.h"
 
-#define WAVE 64
 
 typedef unsigned long long u64;
 
@@ -290,12 +290,6 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
-// LDS ordering between the lanes of one wave (several independent waves share a workgroup, so no block barrier)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ---------------------------------------------------------------------------------------------- neighbours
 // One wave per atom.  The stencil of (2K0+1)(2K1+1)(2K2+1) bins around the atom's bin (each with the image
@@ -602,6 +596,17 @@ __device__ __forceinline__ float ipowf(float u, int n) {
 }
 
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+// a = hi + lo for two values at once: hi rounds a to BF16 (one packed conversion), lo rounds the exact residual;
+// the BF16 halves of hi are widened back by shifts
+__device__ __forceinline__ void split_bf16x2(float a, float b, unsigned& hi, unsigned& lo) {
+  const unsigned h = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+  const float ha = __builtin_bit_cast(float, h << 16), hb = __builtin_bit_cast(float, h & 0xffff0000u);
+  hi = h;
+  lo = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a - ha, b - hb}, bf16x2));
+}
 // cos(pi * t) and sin(pi * t) for t in [0, 1]: the hardware takes revolutions, cos(2 pi y)
 __device__ __forceinline__ float cospi_(float t) { return __builtin_amdgcn_cosf(0.5f * t); }
 __device__ __forceinline__ float sinpi_(float t) { return __builtin_amdgcn_sinf(0.5f * t); }
@@ -980,13 +985,6 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
 }
 
 // ---------------------------------------------------------------------------------------------- MLP (FP32 VALU)
-__device__ __forceinline__ float tanh_f(float x) {
-  // tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2|x|): no overflow, full FP32 accuracy
-  float ax = fabsf(x);
-  float e = __expf(-2.0f * ax);
-  float t = (1.0f - e) / (1.0f + e);
-  return copysignf(t, x);
-}
 
 // one thread per atom; hidden activations in LDS [unit][thread]
 __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ spc, const float* __restrict__ G,
@@ -1047,8 +1045,8 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 // Forces: F_j -= dT/da, F_k -= dT/db, F_i += dT/da + dT/db; virial U -= dT/da (x) a + dT/db (x) b.
 // The dE/dG grid g(e,z,l) of the current species block is wave-uniform; it is staged through LDS once per class
 // and kept in registers for the class's triplets.
-template <int PAT, bool UNI>
-__global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+template <int PAT, bool UNI, bool MMA>
+__device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                                     const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
                                                     float4* __restrict__ fself, float4* __restrict__ fe,
@@ -1059,6 +1057,7 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   constexpr int ZL = MG_MAXZ * MG_MAXL;
+  constexpr bool MM = UNI && MMA;  // contraction on the matrix cores (one exp per eta needs shared etas)
   extern __shared__ float4 smem_f[];
   if (*ovf) return;  // rows may exceed the LDS tile of cap entries
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: per-atom data in SGPRs
@@ -1132,13 +1131,46 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
       const float rc = bp->rc[pjk];
       const float rinv = 1.0f / rc, rinvh = 0.5f * rinv;
       const float dfk = -0.5f * 3.14159265358979f * rinv;
+      // the class's etas (wave-uniform)
+      float e3[NE], n3[NE], e1[NE], e2[NE];
+#pragma unroll
+      for (int e = 0; e < NE; e++) {
+        e3[e] = bp->eta4[pjk][e];
+        n3[e] = -(float)MG_LOG2E * e3[e];
+        e1[e] = UNI ? e3[e] : bp->eta4[pij][e];
+        e2[e] = UNI ? e3[e] : bp->eta4[pik][e];
+      }
       {
         // grid cell lane, with the zeta prefactor 2^(1 - zeta) and the factor 2 of the G4 definition folded in
         int n = bp->g4map[lane];
         ((float*)gts4)[lane] = n >= 0 ? gi[blk + n] * (2.0f * bp->pref[(lane / MG_MAXL) % MG_MAXZ]) : 0.0f;
       }
       wave_sync();
-      // the grid of this species block, loop-invariant: kept in registers for the whole class
+      // MM (compensated-BF16 precision, shared etas): the contraction over eta runs on the matrix cores, beside the
+      // VALU work of the other waves; the FP32 precision keeps the VALU contraction.  Per batch of 64 triplets, one
+      // v_mfma_f32_32x32x16_bf16 (three with the split operands) computes, for each triplet t,
+      //   C_q(t) = sum_e g_eq B_e(t)  and  D_q(t) = sum_e eta_e g_eq B_e(t),   B_e = exp(-eta_e (r1^2 + r2^2 + r3^2)),
+      // so that X = sum_e B_e S_e = sum_q C_q P_q, Y = sum_e eta_e B_e S_e = sum_q D_q P_q and
+      // Z = sum_e B_e S'_e = sum_q C_q P'_q.  B operand: lane t gives its B_0..7 as k = 8(t/32) + e of column t%32.
+      // A operand: block diagonal, so that the rows a lane receives (8(r/4) + 4(t/32) + r%4, register r) depend on
+      // its own k half only; register r of every lane then holds output r (C_0..7, D_0..7) of its own triplet.
+      // Both operands are split a = hi + lo (BF16 each) and the three products lo.hi + hi.lo + hi.hi accumulate in
+      // FP32 (relative error ~2^-17 of sum_e |g_eq B_e|; measured forces within ~3e-5 eV/A of FP64).
+      bf16x8 ahi, alo;
+      if (MM) {
+        const int m = lane & 31, h = lane >> 5;
+        const int r = 4 * (m >> 3) + (m & 3);
+        const bool own = ((m >> 2) & 1) == h;
+        const float* gts = (const float*)gts4;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+          float a = 0.f;
+          if (own && v < NE) a = r < ZL ? gts[v * ZL + r] : e3[v < NE ? v : 0] * gts[v * ZL + r - ZL];
+          ahi[v] = (__bf16)a;
+          alo[v] = (__bf16)(a - (float)ahi[v]);
+        }
+      }
+      // the grid of this species block, loop-invariant: kept in registers for the whole class (VALU contraction)
       float gv[NE * ZL];
 #pragma unroll
       for (int e = 0; e < NE; e++) {
@@ -1170,11 +1202,30 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
           Ak[0] = Ak0.x; Ak[1] = Ak0.y; Ak[2] = Ak0.z; Ak[3] = Ak0.w; Ak[4] = Ak1.x; Ak[5] = Ak1.y; Ak[6] = Ak1.z; Ak[7] = Ak1.w;
         }
         const float rsum = ia.w + ib.w + r3s;
+        if (MM) {
+          float B[8];
 #pragma unroll
-        for (int e = 0; e < NE; e++) {
-          const float eta3 = bp->eta4[pjk][e];
-          float B = UNI ? fast_exp2(-(float)MG_LOG2E * eta3 * rsum)
-                        : Aj[e] * Ak[e] * fast_exp2(-(float)MG_LOG2E * eta3 * r3s);
+          for (int e = 0; e < 8; e++) B[e] = e < NE ? fast_exp2(n3[e < NE ? e : 0] * rsum) : 0.f;
+          unsigned h[4], l[4];
+#pragma unroll
+          for (int v = 0; v < 4; v++) split_bf16x2(B[2 * v], B[2 * v + 1], h[v], l[v]);
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, (u32x4){h[0], h[1], h[2], h[3]});
+          const bf16x8 bl = __builtin_bit_cast(bf16x8, (u32x4){l[0], l[1], l[2], l[3]});
+          f32x16 acc = {};
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bl, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bh, acc, 0, 0, 0);
+#pragma unroll
+          for (int q = 0; q < ZL; q++) {
+            X += acc[q] * P[q];
+            Y1 += acc[ZL + q] * P[q];
+            Zc += acc[q] * Pd[q];
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < (MM ? 0 : NE); e++) {
+          const float eta3 = e3[e];
+          float B = UNI ? fast_exp2(n3[e] * rsum) : Aj[e] * Ak[e] * fast_exp2(n3[e] * r3s);
           float Se = 0.f, Sd = 0.f;
 #pragma unroll
           for (int q = 0; q < ZL; q++)
@@ -1184,8 +1235,8 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
           if (UNI) {
             Y1 += eta3 * SB;
           } else {
-            Y1 += bp->eta4[pij][e] * SB;
-            Y2 += bp->eta4[pik][e] * SB;
+            Y1 += e1[e] * SB;
+            Y2 += e2[e] * SB;
             Y3 += eta3 * SB;
           }
           Zc += Sd * B;
@@ -1239,6 +1290,26 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
     float* vv = vir + 6 * (size_t)i;
     vv[0] = u0; vv[1] = u1; vv[2] = u2; vv[3] = u3; vv[4] = u4; vv[5] = u5;
   }
+}
+
+// one wave per atom: k_force contracts on the VALU in FP32, k_force_mm on the matrix cores (compensated BF16)
+template <int PAT, bool UNI>
+__global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+    const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
+    const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG, float4* __restrict__ fself,
+    float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
+    const int* __restrict__ npl, size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
+    const int* __restrict__ ovf) {
+  force_row<PAT, UNI, false>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+}
+template <int PAT, bool UNI>
+__global__ __launch_bounds__(256) void k_force_mm(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+    const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
+    const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG, float4* __restrict__ fself,
+    float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
+    const int* __restrict__ npl, size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
+    const int* __restrict__ ovf) {
+  force_row<PAT, UNI, true>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
 }
 
 // Force on atom a: its own term plus what its neighbours' centres sent it, which the force kernel wrote into a's row
@@ -1449,7 +1520,7 @@ static int pick_wpb(int wpb, size_t sm_wave) {
     else { if (uni_) LAUNCH_PU(KER, 0, true, SM, __VA_ARGS__); else LAUNCH_PU(KER, 0, false, SM, __VA_ARGS__); } \
   } while (0)
 
-// variant = G4 pattern (bits 0-3) | shared-eta flag (bit 4)
+// variant = G4 pattern (bits 0-3) | shared-eta flag (bit 4) | matrix-core force contraction (bit 5, force only)
 int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* rowcnt, const int* cnt,
              const void* vec,
              const mg_basis_dev* bp, float* G, float* erep, float urep, int maxrow, int wpb, unsigned* gpl, int* npl,
@@ -1482,6 +1553,11 @@ int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, con
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * (R + 2) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
+  if ((variant >> 5) & 1) {
+    DISPATCH_PU(k_force_mm, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself,
+                (float4*)fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+    return mgk_status();
+  }
   DISPATCH_PU(k_force, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
               urep, cap, gpl, npl, pstride, rev, err, ovf);
   return mgk_status();
@@ -1504,237 +1580,3 @@ int mgk_reduce(int nstr, int nchunk, const int* chunk_str, const mg_struct_dev* 
 }
 
 }  // extern "C"
-
-// ---------------------------------------------------------------------------------------------- MLP on MFMA
-// One wave per tile of up to 32 atoms of one species (atoms reached through a species-sorted permutation).
-// Layers are GEMMs with hidden units as rows and atoms as columns: out[u][atom] = W[u][k] x[k][atom], on
-// v_mfma_f32_32x32x2_f32 (FP32) or v_mfma_f32_32x32x16_bf16 (BF16 inputs, FP32 accumulation).
-// Accumulator layout (32x32 blocks): lane l, register r holds row 8(r/4) + 4(l/32) + r%4, column l%32.  Each
-// lane therefore keeps all hidden units of one atom spread over its registers, and those registers are fed
-// straight back as the next layer's B operand; the host packs the weight fragments (A operand) in the same
-// k order, so no data movement is needed between layers.  Forward tanh layers, E_i = W2.h2 + b2, and the
-// backward chain dE/dG = W0'^T[(W1^T[W2 (1-h2^2)]) (1-h1^2)] (src/nmlp.c:367-431 and the chain of 433-478).
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ int acc_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
-
-// PREC 0: FP32 MFMA (K=2).  PREC 1: BF16 MFMA (K=16), one pass.  PREC 2: compensated BF16 with split operands
-// a = a_hi + a_lo, b = b_hi + b_lo (each a BF16 value) and three passes a_lo b_hi + a_hi b_lo + a_hi b_hi into the
-// FP32 accumulator: single-pass BF16 rounds the weights to 2^-9 relative, which alone moves energies by ~1e-3
-// eV/atom on the 51-10-10-1 workload; the split keeps the error near FP32 (~1e-6 eV/atom).
-template <int PREC>
-__device__ __forceinline__ f32x16 mfma_step(const float* __restrict__ afrag, const __bf16* __restrict__ afrag_bf,
-                                            const __bf16* __restrict__ afrag_lo, int idx, const float* bvals,
-                                            f32x16 acc) {
-  if (PREC == 0) {
-    return __builtin_amdgcn_mfma_f32_32x32x2f32(afrag[idx], bvals[0], acc, 0, 0, 0);
-  } else {
-    const bf16x8 a = *(const bf16x8*)(afrag_bf + 8 * (size_t)idx);
-    bf16x8 b;
-#pragma unroll
-    for (int v = 0; v < 8; v++) b[v] = (__bf16)bvals[v];
-    if (PREC == 2) {
-      const bf16x8 al = *(const bf16x8*)(afrag_lo + 8 * (size_t)idx);
-      bf16x8 bl;
-#pragma unroll
-      for (int v = 0; v < 8; v++) bl[v] = (__bf16)(bvals[v] - (float)b[v]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl, acc, 0, 0, 0);
-    }
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-  }
-}
-
-// Fragment offsets (in A-operand slots of 64 lanes; BF16 slots hold 8 values) of one species, see mg_model.c
-struct MlpFrag {
-  int D, H1, H2, hb1, hb2, db, k1;   // k1: layer-1 steps
-  int o_l1, o_l2, o_b1, o_g0;        // slot offsets of the four A-operand sets
-  int o_vec;                          // float offset of b0[hb1*32], b1[hb2*32], w2[hb2*32], b2, xoff[D]
-  int stride_slots, stride_vec;       // per-species strides
-};
-
-template <int PREC, int HB1, int HB2>
-__global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles, const int* __restrict__ perm,
-                                                 const float* __restrict__ G, const float* __restrict__ frag,
-                                                 const __bf16* __restrict__ frag_bf, const __bf16* __restrict__ frag_lo,
-                                                 const float* __restrict__ vecs, MlpFrag f, float* __restrict__ Ei,
-                                                 float* __restrict__ dEdG) {
-  constexpr bool BF16 = PREC != 0;
-  constexpr int KS = BF16 ? 8 : 1;   // B values per lane per step
-  const int4 tl = tiles[blockIdx.x];
-  const int sp = tl.x, start = tl.y, count = tl.z;
-  const int lane = threadIdx.x, col = lane & 31, h = lane >> 5;
-  const int atom = col < count ? perm[start + col] : -1;
-  extern __shared__ float gsm[];  // [32][Kmax + 1] staged inputs
-  const int D = f.D;
-  const size_t so = (size_t)sp * f.stride_slots;
-  const float* fa = frag + so * 64;
-  const __bf16* fb = frag_bf + so * 64 * 8;
-  const __bf16* fl = frag_lo + so * 64 * 8;
-  const float* vv = vecs + (size_t)sp * f.stride_vec;
-  const float* b0 = vv;
-  const float* b1 = b0 + HB1 * 32;
-  const float* w2 = b1 + HB2 * 32;
-  const float* xo = w2 + HB2 * 32 + 1;
-  const float b2 = w2[HB2 * 32];
-
-  // ---- layer 1: a1[u][atom] = b0 + W0' x
-  f32x16 a1[HB1];
-#pragma unroll
-  for (int rb = 0; rb < HB1; rb++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) a1[rb][r] = b0[rb * 32 + acc_row(r, h)];
-  // the tile's inputs x - xoff staged in LDS first ([32][Kmax + 1], zero past D and for missing atoms): 32 rows of
-  // coalesced loads all in flight at once, instead of two dependent memory round trips per K step
-  const int k1 = f.k1;
-  const int Kmax = BF16 ? 16 * k1 : 2 * k1;
-  const int RS = Kmax + 1;
-  for (int k0 = 0; k0 < Kmax; k0 += WAVE) {
-    const int k = k0 + lane, kc = k < D ? k : D - 1;
-    const float xk = xo[kc];
-    float vals[32];
-#pragma unroll
-    for (int c = 0; c < 32; c++) {
-      const int ac = __shfl(atom, c, WAVE);
-      const float keep = (ac >= 0 && k < D) ? 1.0f : 0.0f;  // a product, not a select: the loads stay unconditional
-      vals[c] = (G[(size_t)(ac >= 0 ? ac : 0) * D + kc] - xk) * keep;
-    }
-    if (k < Kmax)
-#pragma unroll
-      for (int c = 0; c < 32; c++) gsm[c * RS + k] = vals[c];
-  }
-  wave_sync();
-  const float* gr = gsm + col * RS;
-  // four steps at a time (steps past k1 multiply the zero padding): their weight loads are in flight together
-  for (int st0 = 0; st0 < k1; st0 += 4) {
-    float xs[4][KS];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int su = st0 + u < k1 ? st0 + u : k1 - 1;
-#pragma unroll
-      for (int v = 0; v < KS; v++) xs[u][v] = st0 + u < k1 ? gr[BF16 ? 16 * su + 8 * h + v : 2 * su + h] : 0.0f;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int st = st0 + u < k1 ? st0 + u : k1 - 1;
-#pragma unroll
-      for (int rb = 0; rb < HB1; rb++)
-        a1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l1 * 64 + (rb * k1 + st) * 64 + lane, xs[u], a1[rb]);
-    }
-  }
-#pragma unroll
-  for (int rb = 0; rb < HB1; rb++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) a1[rb][r] = tanh_f(a1[rb][r]);
-
-  // ---- layer 2: a2 = tanh(b1 + W1 h1); energy; delta2 = W2 (1 - h2^2)
-  constexpr int S2 = BF16 ? 2 : 16;  // steps per 32-unit block
-  f32x16 a2[HB2];
-#pragma unroll
-  for (int rb = 0; rb < HB2; rb++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) a2[rb][r] = b1[rb * 32 + acc_row(r, h)];
-#pragma unroll
-  for (int rb1 = 0; rb1 < HB1; rb1++)
-#pragma unroll
-    for (int q = 0; q < S2; q++) {
-      float xb[KS];
-#pragma unroll
-      for (int v = 0; v < KS; v++) xb[v] = a1[rb1][BF16 ? 8 * q + v : q];
-      const int st = rb1 * S2 + q;
-#pragma unroll
-      for (int rb = 0; rb < HB2; rb++)
-        a2[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l2 * 64 + (rb * HB1 * S2 + st) * 64 + lane, xb, a2[rb]);
-    }
-  float e = 0.0f;
-#pragma unroll
-  for (int rb = 0; rb < HB2; rb++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const float w = w2[rb * 32 + acc_row(r, h)];
-      const float t = tanh_f(a2[rb][r]);
-      e += w * t;
-      a2[rb][r] = w * (1.0f - t * t);  // delta2
-    }
-  e += __shfl_xor(e, 32, WAVE);
-  if (atom >= 0 && h == 0) Ei[atom] = e + b2;
-
-  // ---- delta1 = (W1^T delta2) (1 - h1^2)
-  f32x16 d1[HB1];
-#pragma unroll
-  for (int rb = 0; rb < HB1; rb++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) d1[rb][r] = 0.0f;
-#pragma unroll
-  for (int rb2 = 0; rb2 < HB2; rb2++)
-#pragma unroll
-    for (int q = 0; q < S2; q++) {
-      float xb[KS];
-#pragma unroll
-      for (int v = 0; v < KS; v++) xb[v] = a2[rb2][BF16 ? 8 * q + v : q];
-      const int st = rb2 * S2 + q;
-#pragma unroll
-      for (int rb = 0; rb < HB1; rb++)
-        d1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_b1 * 64 + (rb * HB2 * S2 + st) * 64 + lane, xb, d1[rb]);
-    }
-#pragma unroll
-  for (int rb = 0; rb < HB1; rb++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) d1[rb][r] *= 1.0f - a1[rb][r] * a1[rb][r];
-
-  // ---- dE/dG = W0'^T delta1, 32 features per block
-  float* out = atom >= 0 ? dEdG + (size_t)atom * D : nullptr;
-  for (int rbd = 0; rbd < f.db; rbd++) {
-    f32x16 gacc;
-#pragma unroll
-    for (int r = 0; r < 16; r++) gacc[r] = 0.0f;
-#pragma unroll
-    for (int rb1 = 0; rb1 < HB1; rb1++)
-#pragma unroll
-      for (int q = 0; q < S2; q++) {
-        float xb[KS];
-#pragma unroll
-        for (int v = 0; v < KS; v++) xb[v] = d1[rb1][BF16 ? 8 * q + v : q];
-        gacc = mfma_step<PREC>(fa, fb, fl, f.o_g0 * 64 + (rbd * HB1 * S2 + rb1 * S2 + q) * 64 + lane, xb, gacc);
-      }
-    if (out)
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int k = rbd * 32 + acc_row(r, h);
-        if (k < D) out[k] = gacc[r];
-      }
-  }
-}
-
-extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, const float* G, const float* frag,
-                            const void* frag_bf, const void* frag_lo, const float* vecs, const int* fdesc, int prec,
-                            float* Ei, float* dEdG, hipStream_t st) {
-  if (ntiles == 0) return 0;
-  const int bf16 = prec != 0;
-  MlpFrag f;
-  f.D = fdesc[0]; f.H1 = fdesc[1]; f.H2 = fdesc[2]; f.hb1 = fdesc[3]; f.hb2 = fdesc[4]; f.db = fdesc[5];
-  f.k1 = bf16 ? fdesc[6] : fdesc[7];
-  const int* o = bf16 ? fdesc + 8 : fdesc + 14;
-  f.o_l1 = o[0]; f.o_l2 = o[1]; f.o_b1 = o[2]; f.o_g0 = o[3]; f.stride_slots = o[4]; f.o_vec = 0;
-  f.stride_vec = o[5];
-  const int4* t = (const int4*)tiles;
-  const __bf16* fb = (const __bf16*)frag_bf;
-  const __bf16* fl = (const __bf16*)frag_lo;
-  const size_t sm = (size_t)32 * ((bf16 ? 16 : 2) * f.k1 + 1) * sizeof(float);  // staged inputs of a tile
-  if (sm > 64 * 1024) return -2;
-#define MLP_L(P, H1B, H2B) hipLaunchKernelGGL((k_mlp_mfma<P, H1B, H2B>), dim3(ntiles), dim3(64), sm, st, t, perm, G, frag, fb, fl, vecs, f, Ei, dEdG)
-#define MLP_P(P)                                  \
-  do {                                            \
-    if (f.hb1 == 1 && f.hb2 == 1) MLP_L(P, 1, 1); \
-    else if (f.hb1 == 1) MLP_L(P, 1, 2);          \
-    else if (f.hb2 == 1) MLP_L(P, 2, 1);          \
-    else MLP_L(P, 2, 2);                          \
-  } while (0)
-  if (prec == 2) MLP_P(2);
-  else if (prec == 1) MLP_P(1);
-  else MLP_P(0);
-#undef MLP_P
-#undef MLP_L
-  return mgk_status();
-}

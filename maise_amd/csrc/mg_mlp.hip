@@ -1,0 +1,241 @@
+// mg_mlp.hip -- the MAISE network (D-H1-H2-1, tanh) on gfx950 matrix cores.
+//
+// Built without the VGPR-form MFMA option of mg_kernels.hip: these kernels keep their accumulators in AGPRs.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mg_internal.h"
+#include "mg_dev.h"
+
+extern "C" int mgk_status(void);
+
+// ---------------------------------------------------------------------------------------------- MLP on MFMA
+// One wave per tile of up to 32 atoms of one species (atoms reached through a species-sorted permutation).
+// Layers are GEMMs with hidden units as rows and atoms as columns: out[u][atom] = W[u][k] x[k][atom], on
+// v_mfma_f32_32x32x2_f32 (FP32) or v_mfma_f32_32x32x16_bf16 (BF16 inputs, FP32 accumulation).
+// Accumulator layout (32x32 blocks): lane l, register r holds row 8(r/4) + 4(l/32) + r%4, column l%32.  Each
+// lane therefore keeps all hidden units of one atom spread over its registers, and those registers are fed
+// straight back as the next layer's B operand; the host packs the weight fragments (A operand) in the same
+// k order, so no data movement is needed between layers.  Forward tanh layers, E_i = W2.h2 + b2, and the
+// backward chain dE/dG = W0'^T[(W1^T[W2 (1-h2^2)]) (1-h1^2)] (src/nmlp.c:367-431 and the chain of 433-478).
+__device__ __forceinline__ int acc_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// PREC 0: FP32 MFMA (K=2).  PREC 1: BF16 MFMA (K=16), one pass.  PREC 2: compensated BF16 with split operands
+// a = a_hi + a_lo, b = b_hi + b_lo (each a BF16 value) and three passes a_lo b_hi + a_hi b_lo + a_hi b_hi into the
+// FP32 accumulator: single-pass BF16 rounds the weights to 2^-9 relative, which alone moves energies by ~1e-3
+// eV/atom on the 51-10-10-1 workload; the split keeps the error near FP32 (~1e-6 eV/atom).
+template <int PREC>
+__device__ __forceinline__ f32x16 mfma_step(const float* __restrict__ afrag, const __bf16* __restrict__ afrag_bf,
+                                            const __bf16* __restrict__ afrag_lo, int idx, const float* bvals,
+                                            f32x16 acc) {
+  if (PREC == 0) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(afrag[idx], bvals[0], acc, 0, 0, 0);
+  } else {
+    const bf16x8 a = *(const bf16x8*)(afrag_bf + 8 * (size_t)idx);
+    bf16x8 b;
+#pragma unroll
+    for (int v = 0; v < 8; v++) b[v] = (__bf16)bvals[v];
+    if (PREC == 2) {
+      const bf16x8 al = *(const bf16x8*)(afrag_lo + 8 * (size_t)idx);
+      bf16x8 bl;
+#pragma unroll
+      for (int v = 0; v < 8; v++) bl[v] = (__bf16)(bvals[v] - (float)b[v]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl, acc, 0, 0, 0);
+    }
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+}
+
+// Fragment offsets (in A-operand slots of 64 lanes; BF16 slots hold 8 values) of one species, see mg_model.c
+struct MlpFrag {
+  int D, H1, H2, hb1, hb2, db, k1;   // k1: layer-1 steps
+  int o_l1, o_l2, o_b1, o_g0;        // slot offsets of the four A-operand sets
+  int o_vec;                          // float offset of b0[hb1*32], b1[hb2*32], w2[hb2*32], b2, xoff[D]
+  int stride_slots, stride_vec;       // per-species strides
+};
+
+template <int PREC, int HB1, int HB2>
+__global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles, const int* __restrict__ perm,
+                                                 const float* __restrict__ G, const float* __restrict__ frag,
+                                                 const __bf16* __restrict__ frag_bf, const __bf16* __restrict__ frag_lo,
+                                                 const float* __restrict__ vecs, MlpFrag f, float* __restrict__ Ei,
+                                                 float* __restrict__ dEdG) {
+  constexpr bool BF16 = PREC != 0;
+  constexpr int KS = BF16 ? 8 : 1;   // B values per lane per step
+  const int4 tl = tiles[blockIdx.x];
+  const int sp = tl.x, start = tl.y, count = tl.z;
+  const int lane = threadIdx.x, col = lane & 31, h = lane >> 5;
+  const int atom = col < count ? perm[start + col] : -1;
+  extern __shared__ float gsm[];  // [32][Kmax + 1] staged inputs
+  const int D = f.D;
+  const size_t so = (size_t)sp * f.stride_slots;
+  const float* fa = frag + so * 64;
+  const __bf16* fb = frag_bf + so * 64 * 8;
+  const __bf16* fl = frag_lo + so * 64 * 8;
+  const float* vv = vecs + (size_t)sp * f.stride_vec;
+  const float* b0 = vv;
+  const float* b1 = b0 + HB1 * 32;
+  const float* w2 = b1 + HB2 * 32;
+  const float* xo = w2 + HB2 * 32 + 1;
+  const float b2 = w2[HB2 * 32];
+
+  // ---- layer 1: a1[u][atom] = b0 + W0' x
+  f32x16 a1[HB1];
+#pragma unroll
+  for (int rb = 0; rb < HB1; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) a1[rb][r] = b0[rb * 32 + acc_row(r, h)];
+  // the tile's inputs x - xoff staged in LDS first ([32][Kmax + 1], zero past D and for missing atoms): 32 rows of
+  // coalesced loads all in flight at once, instead of two dependent memory round trips per K step
+  const int k1 = f.k1;
+  const int Kmax = BF16 ? 16 * k1 : 2 * k1;
+  const int RS = Kmax + 1;
+  for (int k0 = 0; k0 < Kmax; k0 += WAVE) {
+    const int k = k0 + lane, kc = k < D ? k : D - 1;
+    const float xk = xo[kc];
+    float vals[32];
+#pragma unroll
+    for (int c = 0; c < 32; c++) {
+      const int ac = __shfl(atom, c, WAVE);
+      const float keep = (ac >= 0 && k < D) ? 1.0f : 0.0f;  // a product, not a select: the loads stay unconditional
+      vals[c] = (G[(size_t)(ac >= 0 ? ac : 0) * D + kc] - xk) * keep;
+    }
+    if (k < Kmax)
+#pragma unroll
+      for (int c = 0; c < 32; c++) gsm[c * RS + k] = vals[c];
+  }
+  wave_sync();
+  const float* gr = gsm + col * RS;
+  // four steps at a time (steps past k1 multiply the zero padding): their weight loads are in flight together
+  for (int st0 = 0; st0 < k1; st0 += 4) {
+    float xs[4][KS];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int su = st0 + u < k1 ? st0 + u : k1 - 1;
+#pragma unroll
+      for (int v = 0; v < KS; v++) xs[u][v] = st0 + u < k1 ? gr[BF16 ? 16 * su + 8 * h + v : 2 * su + h] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int st = st0 + u < k1 ? st0 + u : k1 - 1;
+#pragma unroll
+      for (int rb = 0; rb < HB1; rb++)
+        a1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l1 * 64 + (rb * k1 + st) * 64 + lane, xs[u], a1[rb]);
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < HB1; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) a1[rb][r] = tanh_f(a1[rb][r]);
+
+  // ---- layer 2: a2 = tanh(b1 + W1 h1); energy; delta2 = W2 (1 - h2^2)
+  constexpr int S2 = BF16 ? 2 : 16;  // steps per 32-unit block
+  f32x16 a2[HB2];
+#pragma unroll
+  for (int rb = 0; rb < HB2; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) a2[rb][r] = b1[rb * 32 + acc_row(r, h)];
+#pragma unroll
+  for (int rb1 = 0; rb1 < HB1; rb1++)
+#pragma unroll
+    for (int q = 0; q < S2; q++) {
+      float xb[KS];
+#pragma unroll
+      for (int v = 0; v < KS; v++) xb[v] = a1[rb1][BF16 ? 8 * q + v : q];
+      const int st = rb1 * S2 + q;
+#pragma unroll
+      for (int rb = 0; rb < HB2; rb++)
+        a2[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l2 * 64 + (rb * HB1 * S2 + st) * 64 + lane, xb, a2[rb]);
+    }
+  float e = 0.0f;
+#pragma unroll
+  for (int rb = 0; rb < HB2; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const float w = w2[rb * 32 + acc_row(r, h)];
+      const float t = tanh_f(a2[rb][r]);
+      e += w * t;
+      a2[rb][r] = w * (1.0f - t * t);  // delta2
+    }
+  e += __shfl_xor(e, 32, WAVE);
+  if (atom >= 0 && h == 0) Ei[atom] = e + b2;
+
+  // ---- delta1 = (W1^T delta2) (1 - h1^2)
+  f32x16 d1[HB1];
+#pragma unroll
+  for (int rb = 0; rb < HB1; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) d1[rb][r] = 0.0f;
+#pragma unroll
+  for (int rb2 = 0; rb2 < HB2; rb2++)
+#pragma unroll
+    for (int q = 0; q < S2; q++) {
+      float xb[KS];
+#pragma unroll
+      for (int v = 0; v < KS; v++) xb[v] = a2[rb2][BF16 ? 8 * q + v : q];
+      const int st = rb2 * S2 + q;
+#pragma unroll
+      for (int rb = 0; rb < HB1; rb++)
+        d1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_b1 * 64 + (rb * HB2 * S2 + st) * 64 + lane, xb, d1[rb]);
+    }
+#pragma unroll
+  for (int rb = 0; rb < HB1; rb++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) d1[rb][r] *= 1.0f - a1[rb][r] * a1[rb][r];
+
+  // ---- dE/dG = W0'^T delta1, 32 features per block
+  float* out = atom >= 0 ? dEdG + (size_t)atom * D : nullptr;
+  for (int rbd = 0; rbd < f.db; rbd++) {
+    f32x16 gacc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) gacc[r] = 0.0f;
+#pragma unroll
+    for (int rb1 = 0; rb1 < HB1; rb1++)
+#pragma unroll
+      for (int q = 0; q < S2; q++) {
+        float xb[KS];
+#pragma unroll
+        for (int v = 0; v < KS; v++) xb[v] = d1[rb1][BF16 ? 8 * q + v : q];
+        gacc = mfma_step<PREC>(fa, fb, fl, f.o_g0 * 64 + (rbd * HB1 * S2 + rb1 * S2 + q) * 64 + lane, xb, gacc);
+      }
+    if (out)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int k = rbd * 32 + acc_row(r, h);
+        if (k < D) out[k] = gacc[r];
+      }
+  }
+}
+
+extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, const float* G, const float* frag,
+                            const void* frag_bf, const void* frag_lo, const float* vecs, const int* fdesc, int prec,
+                            float* Ei, float* dEdG, hipStream_t st) {
+  if (ntiles == 0) return 0;
+  const int bf16 = prec != 0;
+  MlpFrag f;
+  f.D = fdesc[0]; f.H1 = fdesc[1]; f.H2 = fdesc[2]; f.hb1 = fdesc[3]; f.hb2 = fdesc[4]; f.db = fdesc[5];
+  f.k1 = bf16 ? fdesc[6] : fdesc[7];
+  const int* o = bf16 ? fdesc + 8 : fdesc + 14;
+  f.o_l1 = o[0]; f.o_l2 = o[1]; f.o_b1 = o[2]; f.o_g0 = o[3]; f.stride_slots = o[4]; f.o_vec = 0;
+  f.stride_vec = o[5];
+  const int4* t = (const int4*)tiles;
+  const __bf16* fb = (const __bf16*)frag_bf;
+  const __bf16* fl = (const __bf16*)frag_lo;
+  const size_t sm = (size_t)32 * ((bf16 ? 16 : 2) * f.k1 + 1) * sizeof(float);  // staged inputs of a tile
+  if (sm > 64 * 1024) return -2;
+#define MLP_L(P, H1B, H2B) hipLaunchKernelGGL((k_mlp_mfma<P, H1B, H2B>), dim3(ntiles), dim3(64), sm, st, t, perm, G, frag, fb, fl, vecs, f, Ei, dEdG)
+#define MLP_P(P)                                  \
+  do {                                            \
+    if (f.hb1 == 1 && f.hb2 == 1) MLP_L(P, 1, 1); \
+    else if (f.hb1 == 1) MLP_L(P, 1, 2);          \
+    else if (f.hb2 == 1) MLP_L(P, 2, 1);          \
+    else MLP_L(P, 2, 2);                          \
+  } while (0)
+  if (prec == 2) MLP_P(2);
+  else if (prec == 1) MLP_P(1);
+  else MLP_P(0);
+#undef MLP_P
+#undef MLP_L
+  return mgk_status();
+}

@@ -124,18 +124,21 @@ def test_call_maise_dropin(ref_cases, tmp_path):
         np.testing.assert_allclose(S, c["U"], atol=U_TOL * N)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("name", ["cuag", "mgca", "cupdag_tri", "relax", "species_generic", "species_std", "config1"])
-def test_stored_pairs_match_enumeration(ref_cases, name):
+def test_stored_pairs_match_enumeration(ref_cases, name, prec):
     """The force kernel reading the descriptor kernel's stored pair lists visits the same pairs in the same order
     as enumerating them itself: bit-identical energies, forces and virials (43- and 22-function tables, generic
-    grid, shared and per-pair etas, 1-3 species)."""
+    grid, shared and per-pair etas, 1-3 species), with the VALU contraction (FP32) and with the matrix-core one
+    (compensated BF16: a triplet's terms must not depend on the other triplets of its batch)."""
     import maise_amd
     case = ref_cases[name]
+    precision = maise_amd.MG_PREC_FP32 if prec == "fp32" else maise_amd.MG_PREC_BF16X3
     out = []
     for store in (True, False):
-        print(f"[gpu case] {name} pair_store={store}", flush=True)
+        print(f"[gpu case] {name} {prec} pair_store={store}", flush=True)
         m = maise_amd.Model(model_path(case["model"]))
-        c = maise_amd.Context(m, urep=float(case["urep"]))
+        c = maise_amd.Context(m, urep=float(case["urep"]), precision=precision)
         c.set_pair_store(store)
         c.set_structure(case["spc"], case["lat"], case["x"], nd=int(case["nd"]), xt=int(case["xt"]))
         c.eval(forces=True)
