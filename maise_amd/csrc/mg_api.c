@@ -621,8 +621,13 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     KCHK(mgk_mlp(N, c->d_spc, c->d_G, c->d_mlp, m->wstride, m->D, m->H1, m->H2, c->d_Ei, c->d_dEdG, st));
   if (timing) HIPCHK(hipEventRecord(ev[EV_MLP], st));
   if (forces) {
-    /* the BF16 precisions also contract the force kernel's triplet terms on the matrix cores (bit 5) */
-    KCHK(mgk_force(N, m->nspc, m->pat | (m->uni << 4) | ((c->prec != MG_PREC_FP32) << 5), c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
+    /* shared etas: the force kernel contracts the triplet terms on the matrix cores, with split FP16 operands in
+       FP32 precision (bit 6: needs every radial factor <= 1, so no negative eta) or split BF16 in the BF16
+       precisions (bit 5); otherwise on the VALU */
+    int fvar = m->pat | (m->uni << 4);
+    if (c->prec != MG_PREC_FP32) fvar |= 1 << 5;
+    else if (m->eta_nonneg) fvar |= 1 << 6;
+    KCHK(mgk_force(N, m->nspc, fvar, c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
                    c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, pstride ? c->d_gpl : NULL, c->d_npl, pstride, c->d_rev,
                    c->d_misc + 1, ovf, st));
     if (timing) HIPCHK(hipEventRecord(ev[EV_FORCE], st));

@@ -20,6 +20,7 @@ struct mg_model {
   int wstride;          /* floats per species block (multiple of 4) */
   int pat;              /* G4 kernel specialisation: 0 generic, 1 43-function, 2 22-function table */
   int uni;              /* 1 when every species pair has the same G4 etas */
+  int eta_nonneg;       /* 1 when no G4 eta is negative (every radial factor exp(-eta r^2) <= 1) */
   /* MFMA MLP: weight fragments in the kernel's operand order (FP32 and BF16 sets) and per-unit vectors */
   int mfma_ok;          /* H1, H2 <= 64 */
   int fdesc[20];

@@ -18,6 +18,7 @@
 //   k_force<NE>    one wave per atom: recomputes pair/triplet geometry, contracts dG/dr with dE/dG in
 //                  registers, forces on neighbour slots in LDS without atomics, per-atom virial (no Fn/Sn
 //                  storage); each neighbour's force goes to the reverse edge's slot in the neighbour's row
+//   k_force_mm     the same with the triplet contraction over eta on the matrix cores (BF16 precisions)
 //   k_gather       F_a = own term + sum over the atom's own row (deterministic)
 //   k_reduce       per-structure FP64 energy and virial sums (deterministic tree)
 #include <hip/hip_runtime.h>
@@ -599,6 +600,16 @@ __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_ex
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// a = hi + lo in FP16 for two values at once (packed RNE conversions)
+__device__ __forceinline__ void split_f16x2(float a, float b, unsigned& hi, unsigned& lo) {
+  const f16x2 h = __builtin_convertvector((f32x2){a, b}, f16x2);
+  const f32x2 hf = __builtin_convertvector(h, f32x2);
+  hi = __builtin_bit_cast(unsigned, h);
+  lo = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a - hf[0], b - hf[1]}, f16x2));
+}
 // a = hi + lo for two values at once: hi rounds a to BF16 (one packed conversion), lo rounds the exact residual;
 // the BF16 halves of hi are widened back by shifts
 __device__ __forceinline__ void split_bf16x2(float a, float b, unsigned& hi, unsigned& lo) {
@@ -686,13 +697,16 @@ template <> struct Pat<2> {
 // P_zl(c) = 2^(1-zeta) (1 + lambda c)^zeta and dP/dc, cell q = z*2 + l
 // (1 + lambda c)^zeta and its derivative in c, cell q = z*2 + l.  The prefactors 2^(1 - zeta) are applied once
 // outside the pair loops (to the reduced descriptors, and folded into the dE/dG grid in the force kernel).
-template <int PAT>
+// HALF: the same on w = u / 2, i.e. P = w^zeta = 2^(1 - zeta) P / 2 and P' = 2^(1 - zeta) P' / 2, the zeta prefactor
+// taken into the powers (one scale for every cell)
+template <int PAT, bool HALF = false>
 __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, float cs, float* P, float* Pd) {
 #pragma unroll
   for (int l = 0; l < MG_MAXL; l++) {
     // the two shipped tables have lambda = {-1, +1} (checked when the model is loaded): constants here
     const float lam = Pat<PAT>::ZSTD ? (l == 0 ? -1.0f : 1.0f) : bp->lam[l];
-    const float u = 1.0f + lam * cs;
+    const float u = HALF ? __builtin_fmaf(0.5f * lam, cs, 0.5f) : 1.0f + lam * cs;
+    const float dk = HALF ? 0.5f : 1.0f;
     if (Pat<PAT>::ZSTD) {
       const float u2 = u * u, u4 = u2 * u2, u8 = u4 * u4, u16 = u8 * u8;
       P[0 + l] = u;
@@ -700,10 +714,10 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
       P[4 + l] = u4;
       P[6 + l] = u16;
       if (Pd) {
-        Pd[0 + l] = lam;
-        Pd[2 + l] = 2.0f * lam * u;
-        Pd[4 + l] = 4.0f * lam * u2 * u;
-        Pd[6 + l] = 16.0f * lam * (u8 * u4) * (u2 * u);
+        Pd[0 + l] = dk * lam;
+        Pd[2 + l] = (dk * 2.0f * lam) * u;
+        Pd[4 + l] = (dk * 4.0f * lam) * u2 * u;
+        Pd[6 + l] = (dk * 16.0f * lam) * (u8 * u4) * (u2 * u);
       }
     } else {
 #pragma unroll
@@ -711,7 +725,7 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
         int zt = bp->zeta[z];
         float pm1 = ipowf(u, zt - 1);
         P[z * MG_MAXL + l] = pm1 * u;
-        if (Pd) Pd[z * MG_MAXL + l] = (float)zt * lam * pm1;
+        if (Pd) Pd[z * MG_MAXL + l] = dk * (float)zt * lam * pm1;
       }
     }
   }
@@ -1045,7 +1059,7 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 // Forces: F_j -= dT/da, F_k -= dT/db, F_i += dT/da + dT/db; virial U -= dT/da (x) a + dT/db (x) b.
 // The dE/dG grid g(e,z,l) of the current species block is wave-uniform; it is staged through LDS once per class
 // and kept in registers for the class's triplets.
-template <int PAT, bool UNI, bool MMA>
+template <int PAT, bool UNI, int MMA>
 __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                                     const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
@@ -1057,7 +1071,8 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
   constexpr int NE = Pat<PAT>::NE;
   constexpr int R = UNI ? 2 : 4;
   constexpr int ZL = MG_MAXZ * MG_MAXL;
-  constexpr bool MM = UNI && MMA;  // contraction on the matrix cores (one exp per eta needs shared etas)
+  constexpr bool MM = UNI && MMA != 0;  // contraction on the matrix cores (one exp per eta needs shared etas)
+  constexpr bool MH = UNI && MMA == 2;  // FP16 hi/lo operands (FP32 precision) instead of BF16
   extern __shared__ float4 smem_f[];
   if (*ovf) return;  // rows may exceed the LDS tile of cap entries
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: per-atom data in SGPRs
@@ -1157,7 +1172,37 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
       // Both operands are split a = hi + lo (BF16 each) and the three products lo.hi + hi.lo + hi.hi accumulate in
       // FP32 (relative error ~2^-17 of sum_e |g_eq B_e|; measured forces within ~3e-5 eV/A of FP64).
       bf16x8 ahi, alo;
-      if (MM) {
+      f16x8 hhi, hlo;
+      float oscale = 1.0f;
+      if (MH) {
+        // FP16 form: A = 2 g / pref (the zeta prefactor goes into the powers of w = u/2, so every cell's
+        // coefficient keeps its own size), scaled by a power of two to below 2^14; B is scaled to (0, 2^14]
+        // inside the exp2; the hi/lo split then stays clear of FP16 overflow and subnormals
+        const int m = lane & 31, h = lane >> 5;
+        const int r = 4 * (m >> 3) + (m & 3);
+        const int qq = r < ZL ? r : r - ZL;
+        const bool own = ((m >> 2) & 1) == h;
+        const float* gts = (const float*)gts4;
+        const float ip = 2.0f / bp->pref[(qq / MG_MAXL) % MG_MAXZ];
+        float a[8], amax = 0.f;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+          a[v] = 0.f;
+          if (own && v < NE) a[v] = (r < ZL ? gts[v * ZL + qq] : e3[v < NE ? v : 0] * gts[v * ZL + qq]) * ip;
+          amax = fmaxf(amax, fabsf(a[v]));
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, WAVE));
+        int ex = 0;
+        if (amax > 0.f) (void)frexpf(amax, &ex);
+        oscale = ldexpf(1.0f, ex - 28);
+        unsigned hh[4], hl[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) split_f16x2(ldexpf(a[2 * v], 14 - ex), ldexpf(a[2 * v + 1], 14 - ex), hh[v], hl[v]);
+        hhi = __builtin_bit_cast(f16x8, (u32x4){hh[0], hh[1], hh[2], hh[3]});
+        hlo = __builtin_bit_cast(f16x8, (u32x4){hl[0], hl[1], hl[2], hl[3]});
+      }
+      if (MM && !MH) {
         const int m = lane & 31, h = lane >> 5;
         const int r = 4 * (m >> 3) + (m & 3);
         const bool own = ((m >> 2) & 1) == h;
@@ -1193,7 +1238,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         float i1 = ia.x, i2 = ib.x, f1 = ia.y, f2 = ib.y, d1 = ia.z, d2 = ib.z;
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * i1 * i2;
         float P[ZL], Pd[ZL];
-        angular<PAT>(bp, cs, P, Pd);
+        angular<PAT, MH>(bp, cs, P, Pd);
         float X = 0.f, Y1 = 0.f, Y2 = 0.f, Y3 = 0.f, Zc = 0.f;
         float Aj[8], Ak[8];
         if (!UNI) {
@@ -1205,22 +1250,35 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         if (MM) {
           float B[8];
 #pragma unroll
-          for (int e = 0; e < 8; e++) B[e] = e < NE ? fast_exp2(n3[e < NE ? e : 0] * rsum) : 0.f;
+          for (int e = 0; e < 8; e++)
+            B[e] = e < NE ? fast_exp2(MH ? __builtin_fmaf(n3[e < NE ? e : 0], rsum, 14.0f) : n3[e < NE ? e : 0] * rsum)
+                          : 0.f;
           unsigned h[4], l[4];
-#pragma unroll
-          for (int v = 0; v < 4; v++) split_bf16x2(B[2 * v], B[2 * v + 1], h[v], l[v]);
-          const bf16x8 bh = __builtin_bit_cast(bf16x8, (u32x4){h[0], h[1], h[2], h[3]});
-          const bf16x8 bl = __builtin_bit_cast(bf16x8, (u32x4){l[0], l[1], l[2], l[3]});
           f32x16 acc = {};
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bl, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bh, acc, 0, 0, 0);
+          if (MH) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) split_f16x2(B[2 * v], B[2 * v + 1], h[v], l[v]);
+            const f16x8 bh = __builtin_bit_cast(f16x8, (u32x4){h[0], h[1], h[2], h[3]});
+            const f16x8 bl = __builtin_bit_cast(f16x8, (u32x4){l[0], l[1], l[2], l[3]});
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(hlo, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(hhi, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(hhi, bh, acc, 0, 0, 0);
+          } else {
+#pragma unroll
+            for (int v = 0; v < 4; v++) split_bf16x2(B[2 * v], B[2 * v + 1], h[v], l[v]);
+            const bf16x8 bh = __builtin_bit_cast(bf16x8, (u32x4){h[0], h[1], h[2], h[3]});
+            const bf16x8 bl = __builtin_bit_cast(bf16x8, (u32x4){l[0], l[1], l[2], l[3]});
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bh, acc, 0, 0, 0);
+          }
 #pragma unroll
           for (int q = 0; q < ZL; q++) {
             X += acc[q] * P[q];
             Y1 += acc[ZL + q] * P[q];
             Zc += acc[q] * Pd[q];
           }
+          if (MH) { X *= oscale; Y1 *= oscale; Zc *= oscale; }
         }
 #pragma unroll
         for (int e = 0; e < (MM ? 0 : NE); e++) {
@@ -1300,7 +1358,7 @@ __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restri
     float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
     const int* __restrict__ npl, size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
     const int* __restrict__ ovf) {
-  force_row<PAT, UNI, false>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+  force_row<PAT, UNI, 0>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
 }
 template <int PAT, bool UNI>
 __global__ __launch_bounds__(256) void k_force_mm(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
@@ -1309,7 +1367,16 @@ __global__ __launch_bounds__(256) void k_force_mm(int N, int S, const int* __res
     float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
     const int* __restrict__ npl, size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
     const int* __restrict__ ovf) {
-  force_row<PAT, UNI, true>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+  force_row<PAT, UNI, 1>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+}
+template <int PAT, bool UNI>
+__global__ __launch_bounds__(256) void k_force_mh(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+    const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
+    const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG, float4* __restrict__ fself,
+    float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
+    const int* __restrict__ npl, size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
+    const int* __restrict__ ovf) {
+  force_row<PAT, UNI, 2>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
 }
 
 // Force on atom a: its own term plus what its neighbours' centres sent it, which the force kernel wrote into a's row
@@ -1553,6 +1620,11 @@ int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, con
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
   size_t sm = (size_t)cap * (R + 2) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
+  if ((variant >> 6) & 1) {
+    DISPATCH_PU(k_force_mh, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself,
+                (float4*)fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+    return mgk_status();
+  }
   if ((variant >> 5) & 1) {
     DISPATCH_PU(k_force_mm, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself,
                 (float4*)fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);

@@ -443,6 +443,10 @@ int mg_model_load(const char *path, mg_model **out) {
       for (int s1 = 0; s1 < m->nspc; s1++)
         for (int e = 0; e < B->ne && e < MG_MAXE; e++)
           if (B->eta4[s0 * MG_MAXSPC + s1][e] != B->eta4[0][e]) m->uni = 0;
+    m->eta_nonneg = 1;
+    for (int p = 0; p < MG_MAXSPC * MG_MAXSPC; p++)
+      for (int e = 0; e < B->ne && e < MG_MAXE; e++)
+        if (B->eta4[p][e] < 0.0f) m->eta_nonneg = 0;
     for (int s = 0; s < m->nspc; s++) B->urep_rm[s] = (m->spcz[s] >= 0 && m->spcz[s] < 96) ? (float)MG_MINDIST[m->spcz[s]] : 1.0f;
   }
   free(bs);
