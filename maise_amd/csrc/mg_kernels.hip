@@ -1161,16 +1161,18 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         ((float*)gts4)[lane] = n >= 0 ? gi[blk + n] * (2.0f * bp->pref[(lane / MG_MAXL) % MG_MAXZ]) : 0.0f;
       }
       wave_sync();
-      // MM (compensated-BF16 precision, shared etas): the contraction over eta runs on the matrix cores, beside the
-      // VALU work of the other waves; the FP32 precision keeps the VALU contraction.  Per batch of 64 triplets, one
-      // v_mfma_f32_32x32x16_bf16 (three with the split operands) computes, for each triplet t,
+      // MM (shared etas): the contraction over eta runs on the matrix cores, beside the VALU work of the other waves
+      // (MH: split FP16 operands, the FP32 precision; otherwise split BF16, the BF16 precisions; per-pair etas keep
+      // the VALU contraction below).  Per batch of 64 triplets, one v_mfma_f32_32x32x16 (three with the split
+      // operands) computes, for each triplet t,
       //   C_q(t) = sum_e g_eq B_e(t)  and  D_q(t) = sum_e eta_e g_eq B_e(t),   B_e = exp(-eta_e (r1^2 + r2^2 + r3^2)),
       // so that X = sum_e B_e S_e = sum_q C_q P_q, Y = sum_e eta_e B_e S_e = sum_q D_q P_q and
       // Z = sum_e B_e S'_e = sum_q C_q P'_q.  B operand: lane t gives its B_0..7 as k = 8(t/32) + e of column t%32.
       // A operand: block diagonal, so that the rows a lane receives (8(r/4) + 4(t/32) + r%4, register r) depend on
       // its own k half only; register r of every lane then holds output r (C_0..7, D_0..7) of its own triplet.
-      // Both operands are split a = hi + lo (BF16 each) and the three products lo.hi + hi.lo + hi.hi accumulate in
-      // FP32 (relative error ~2^-17 of sum_e |g_eq B_e|; measured forces within ~3e-5 eV/A of FP64).
+      // Both operands are split a = hi + lo and the three products lo.hi + hi.lo + hi.hi accumulate in FP32: with
+      // BF16 halves the relative error is ~2^-17 of sum_e |g_eq B_e| (forces measured within ~3e-5 eV/A of FP64),
+      // with FP16 halves ~2^-22 (forces at the FP32 VALU contraction's level, a few 1e-6 eV/A).
       bf16x8 ahi, alo;
       f16x8 hhi, hlo;
       float oscale = 1.0f;
