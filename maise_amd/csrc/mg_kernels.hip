@@ -1371,8 +1371,10 @@ __global__ __launch_bounds__(256) void k_force_mm(int N, int S, const int* __res
     const int* __restrict__ ovf) {
   force_row<PAT, UNI, 1>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
 }
+// five waves per SIMD: 96 VGPRs (from 101), the few spilled values are saved once outside the triplet loop
+// (-5% kernel time in same-box timing)
 template <int PAT, bool UNI>
-__global__ __launch_bounds__(256) void k_force_mh(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_force_mh(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
     const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG, float4* __restrict__ fself,
     float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
