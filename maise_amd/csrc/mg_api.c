@@ -515,6 +515,9 @@ static int chain_flags(mg_ctx *c, int mode) {
     if (mode != MG_CHAIN_SYNC) return MG_ERR_NEIGHBOR_CAP;  /* outgrew the capacities: repeat in sync mode */
     return mg_fail(MG_ERR_HIP, "edge capacity check failed in a synchronised evaluation");
   }
+  if (f[1] & 2)
+    return mg_fail(MG_ERR_NEIGHBOR, "neighbour list error: an atom has more than 1024 neighbours of one species beside "
+                                    "neighbours of another (the pair enumeration's limit)");
   if (f[1]) return mg_fail(MG_ERR_NEIGHBOR, "neighbour list error: a reverse edge is missing, or a cell is too thin for the cutoff");
   c->nedges = mode == MG_CHAIN_SYNC ? f[6] : -1;  /* padded rows: the total is summed when it is asked for */
   c->maxrow = f[0];
@@ -623,10 +626,10 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
   if (forces) {
     /* shared etas: the force kernel contracts the triplet terms on the matrix cores, with split FP16 operands in
        FP32 precision (bit 6: needs every radial factor <= 1, so no negative eta) or split BF16 in the BF16
-       precisions (bit 5); otherwise on the VALU */
+       precisions (bit 5); otherwise (per-pair etas) on the VALU in k_force, whose register budget is not capped */
     int fvar = m->pat | (m->uni << 4);
-    if (c->prec != MG_PREC_FP32) fvar |= 1 << 5;
-    else if (m->eta_nonneg) fvar |= 1 << 6;
+    if (m->uni && c->prec != MG_PREC_FP32) fvar |= 1 << 5;
+    else if (m->uni && m->eta_nonneg) fvar |= 1 << 6;
     KCHK(mgk_force(N, m->nspc, fvar, c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
                    c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, pstride ? c->d_gpl : NULL, c->d_npl, pstride, c->d_rev,
                    c->d_misc + 1, ovf, st));

@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "mg_internal.h"
 #include "mg_dev.h"
 
@@ -735,86 +737,100 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
 // species) in circulant order (j, j + d mod m), d = 1 .. m/2, each pair once; rectangle (a x b) with the longer
 // side as the lane-fast index.  Passing pairs are compacted into an LDS list (ballot + lane prefix count) and
 // processed with all 64 lanes busy; about half of the enumerated pairs fail the r_jk test in bulk metals.
-#define PCH 512
-// Enumerate the neighbour pairs (j, k) of one species class with r_jk < rc, 64 candidates per step, compacting
-// the survivors into plist and handing them to body(j, k, tag, valid) in batches of 64 (valid = false on the
-// lanes past the end of the last batch).
-//   same species (tri): circulant diagonals, pair (f, f + sl mod m) for sl = 1..m/2 (the last diagonal of an even
-//     m only for f < m/2), so every unordered pair appears once;
-//   two species: the longer side's slot f against the shorter side's slot (f mod l + sl) mod l, sl = 0..l-1.
-// tag (8 bits) numbers the groups of pairs in which no slot repeats on either side: a triangle diagonal, or a run of
-// l consecutive f on one rectangle diagonal.  The force kernel accumulates one group at a time without atomics.
-// Packing: j | k << 12 | tag << 24 (slots < 2048).  With gout, the survivors are also written there in order (the
-// pair list of the class, which the force kernel reads instead of enumerating again); returns their number.
+#define PRING 128  // per-wave ring of surviving pairs (a power of two >= 2 * WAVE)
+// Pair words: j | k << 11 | tag << 22 (slots < 2048, tag < 1024).  tag numbers the group of the pair, a set of pairs in
+// which no slot repeats on either side, so the force kernel can accumulate one group at a time without atomics.
+#define PK_J(p) ((int)((p) & 0x7ffu))
+#define PK_K(p) ((int)(((p) >> 11) & 0x7ffu))
+#define PK_TAG(p) ((int)((p) >> 22))
+// lane / n for 0 <= lane < 64, 0 < n <= 64: lane + 0.5 is exact and the two roundings move (lane + 0.5) / n by less
+// than 2^-17 / n, under the 0.5 / n that separates it from an integer
+__device__ __forceinline__ int lane_div(int lane, int n) { return (int)(((float)lane + 0.5f) * (1.0f / (float)n)); }
+
+// Enumerate the neighbour pairs (j, k) of one species class with r_jk < rc, compacting the survivors into a ring in
+// LDS and handing them to body(j, k, tag, valid) in batches of 64 as soon as 64 are waiting (valid = false on the
+// lanes past the end of the class's last batch), so every batch but the class's last is full.  The pairs are laid out
+// on circulant diagonals, each a conflict-free group (tag = the diagonal):
+//   same species (m slots): diagonal q = 0..m/2-1 holds the pairs (f, f + q + 1 mod m), f = 0..m-1 (the last
+//     diagonal of an even m only f < m/2), so every unordered pair appears once;
+//   two species (long side a >= short side b): diagonal q = 0..a-1 holds (u, u + q mod a) for the short side's slots
+//     u = 0..b-1 (a <= 1024 is required: a longer side raises the error flag, bit 1).
+// Candidate t = q * n + f (n = m, or b) is taken 64 at a time in order, so a batch of 64 survivors spans only a few
+// diagonals; each lane steps its own (q, f) by 64 candidates with one add and one wrap (no division per candidate),
+// and the wraps are unsigned minima (x mod n = min(x, x - n) for x < 2n).  Only the class's last step checks the
+// candidate range.  With gout, the survivors are also written there in order (the pair list of the class, which the
+// force kernel reads instead of enumerating again); returns their number.  The batches are the consecutive runs of 64
+// of that list, the same batches the stored-list reader forms.
+template <int MODE>
+struct PairMode : std::integral_constant<int, MODE> {};
+
 template <int R, typename F>
-__device__ __forceinline__ int for_pairs(const float4* rec, unsigned* plist, bool tri, int oa, int ma, int ob, int mb,
-                                         float rc2, int lane, unsigned* __restrict__ gout, F&& body) {
-  int mf, l, ntot;  // diagonal length, group length (shorter side), candidates
+__device__ __forceinline__ int for_pairs(const float4* rec, unsigned* ring, bool tri, int oa, int ma, int ob, int mb,
+                                         float rc2, int lane, unsigned* __restrict__ gout, int* __restrict__ err,
+                                         F&& body) {
+  int wp = 0, rp = 0;  // ring write / read positions: survivors found so far, survivors handed to body
+  // mode 0: one species; 1: two species, long side a; 2: two species, long side b
+  auto run = [&](auto MODEC, int n, int nl, int ol, int os) {
+    constexpr int mode = decltype(MODEC)::value;
+    // n: slots per diagonal; nl: the wrap of the partner slot (m, or the long side); ol, os: block offsets of the
+    // partner (long) side and of the lane's (short) side
+    const int ntot = mode == 0 ? ((n >> 1) - 1) * n + ((n & 1) ? n : n >> 1) : nl * n;
+    const int dq = WAVE / n, df = WAVE - dq * n;  // wave-uniform step of (q, f) per 64 candidates
+    int q = lane_div(lane, n);
+    unsigned f = (unsigned)(lane - q * n);
+    const unsigned qoff = mode == 0 ? 1u : 0u;
+    for (int base = 0; base < ntot; base += WAVE) {
+      const bool tail = base + WAVE > ntot;  // wave-uniform
+      // partner slot p = f + q (+1) mod nl; lanes past the last candidate are masked before any read
+      const unsigned p0 = f + (unsigned)q + qoff;
+      const unsigned p = min(p0, p0 - (unsigned)nl);
+      const int js = mode == 0 ? os + (int)f : (mode == 1 ? ol + (int)p : os + (int)f);
+      const int ks = mode == 0 ? os + (int)p : (mode == 1 ? os + (int)f : ol + (int)p);
+      bool ok = false;
+      if (!tail || lane < ntot - base) {
+        const float4 a = rec[R * js], b = rec[R * ks];
+        const float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
+        ok = cx * cx + cy * cy + cz * cz < rc2;
+      }
+      const unsigned pk = (unsigned)js | ((unsigned)ks << 11) | ((unsigned)q << 22);
+      const unsigned f1 = f + (unsigned)df;
+      const bool c = f1 >= (unsigned)n;
+      f = c ? f1 - (unsigned)n : f1;
+      q += dq + (c ? 1 : 0);
+      const unsigned long long bm = __ballot(ok);
+      const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+      if (ok) ring[(wp + below) & (PRING - 1)] = pk;
+      wp += __popcll(bm);
+      if (wp - rp >= WAVE || (tail || base + WAVE == ntot) && wp > rp) {
+        const bool last = tail || base + WAVE == ntot;
+        // at most 127 survivors wait here (fewer than 64 before this step's 64), so the ring never overwrites an
+        // unread entry; the second sync orders the batch's ring reads before the next step's writes
+        wave_sync();
+        while (wp - rp >= WAVE || (last && wp > rp)) {
+          const int tt = rp + lane;
+          const bool valid = tt < wp;
+          const unsigned pw = valid ? ring[tt & (PRING - 1)] : 0u;
+          if (gout && valid) gout[tt] = pw;
+          body(PK_J(pw), PK_K(pw), PK_TAG(pw), valid);
+          rp += WAVE;
+        }
+        wave_sync();
+      }
+    }
+  };
   if (tri) {
     if (ma < 2) return 0;
-    mf = ma; l = ma;
-    const int ms = ma >> 1;
-    ntot = (ms - 1) * ma + ((ma & 1) ? ma : ms);  // diagonals 1..ms; an even m's last one holds f < m/2
-  } else {
-    if (ma == 0 || mb == 0) return 0;
-    mf = ma >= mb ? ma : mb; l = ma >= mb ? mb : ma;
-    ntot = mf * l;
+    run(PairMode<0>{}, ma, ma, oa, oa);
+    return wp;
   }
-  const int nblk = (mf + l - 1) / l;  // groups per rectangle diagonal
-  const float rmf = 1.0f / (float)mf, rl = 1.0f / (float)l;
-  int cnt = 0, tot = 0;
-  for (int base = 0; base < ntot; base += WAVE) {
-    const int t = base + lane;  // candidate t = diagonal * mf + f
-    bool ok = false;
-    unsigned pk = 0;
-    if (t < ntot) {
-      // t / mf through the reciprocal: t + 0.5 is exact and the two roundings move (t + 0.5)/mf by less than
-      // (t + 0.5) 2^-23 / mf, under the 0.5/mf that separates it from an integer while t < 2^22 (rows are at most
-      // 2048 slots, so t < 2048 * 2047 / 2 for a triangle and t < 1024^2 for a rectangle)
-      const int q = (int)(((float)t + 0.5f) * rmf);
-      const int f = t - q * mf;
-      int jj, kk, tag;
-      if (tri) {
-        const int sl = q + 1;
-        int k = f + sl;
-        if (k >= ma) k -= ma;
-        jj = oa + f; kk = oa + k;
-        tag = sl;
-      } else {
-        const int fq = (int)(((float)f + 0.5f) * rl);  // exact as above (f < 2048)
-        const int fr = f - fq * l;
-        int u = fr + q;
-        if (u >= l) u -= l;
-        if (ma >= mb) { jj = oa + f; kk = ob + u; } else { jj = oa + u; kk = ob + f; }
-        tag = q * nblk + fq;
-      }
-      const float4 a = rec[R * jj], b = rec[R * kk];
-      const float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
-      ok = cx * cx + cy * cy + cz * cz < rc2;
-      pk = (unsigned)jj | ((unsigned)kk << 12) | ((unsigned)(tag & 0xff) << 24);
-    }
-    const unsigned long long bm = __ballot(ok);
-    const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
-    if (ok) {
-      plist[cnt + below] = pk;
-      if (gout) gout[tot + below] = pk;
-    }
-    cnt += __popcll(bm);
-    tot += __popcll(bm);
-    const bool last = base + WAVE >= ntot;
-    if (cnt > PCH - WAVE || last) {
-      wave_sync();
-      for (int tb = 0; tb < cnt; tb += WAVE) {
-        const int tt = tb + lane;
-        const bool valid = tt < cnt;
-        const unsigned p = valid ? plist[tt] : 0u;
-        body((int)(p & 0xfffu), (int)((p >> 12) & 0xfffu), (int)(p >> 24), valid);
-      }
-      wave_sync();
-      cnt = 0;
-    }
+  if (ma == 0 || mb == 0) return 0;
+  if ((ma >= mb ? ma : mb) > 1024) {  // tags would not fit 10 bits
+    if (lane == 0) atomicOr(err, 2);
+    return 0;
   }
-  return tot;
+  if (ma >= mb) run(PairMode<1>{}, mb, ma, oa, ob);  // lanes over species b's slots, partners in a
+  else run(PairMode<2>{}, ma, mb, ob, oa);            // lanes over species a's slots, partners in b
+  return wp;
 }
 
 // The stored pair list of one class (np pairs from pl), handed to body in batches of 64 like for_pairs; the next
@@ -830,7 +846,7 @@ __device__ __forceinline__ void for_stored_pairs(const unsigned* __restrict__ pl
     const bool valid = tb + lane < np;
     const int tn = tb + WAVE + lane;
     pn = pl[tn < np ? tn : last];
-    body((int)(p & 0xfffu), (int)((p >> 12) & 0xfffu), (int)(p >> 24), valid);
+    body(PK_J(p), PK_K(p), PK_TAG(p), valid);
   }
 }
 
@@ -875,7 +891,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
   const int D = bp->D;
   RowInfo ri;
   row_info(ri, i, spc, row_off, rowcnt, cnt, S);
-  float4* rec = smem_d + (size_t)wv * (R * cap + PCH / 4);
+  float4* rec = smem_d + (size_t)wv * (R * cap + PRING / 4);
   unsigned* plist = (unsigned*)(rec + R * cap);
   stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
   // reverse edges (for the force kernel): the edges of this row that own their pair look the partner up in the
@@ -900,10 +916,15 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
   }
   wave_sync();
   float* Gi = G + (size_t)i * D;
+  // The shipped G4 tables leave the grid cells of the etas past NE free (8 cells for the 43-function table, 16 for
+  // the 22-function one): there the G2 sums of neighbour species s ride along in the reduce-scatter of class (s, s)
+  // instead of one wave reduction per G2 function.
+  constexpr int G2OFF = NE * MG_MAXZ * MG_MAXL;
+  const bool g2grid = PAT != 0 && bp->ng2 <= MG_GRID - G2OFF;
+  float eu = 0.0f;  // UREP energy of the atom (CELL_UREP, src/cmod.c:70-86)
 
-  // ---- G2 (+ UREP energy) per neighbour species block
-  float eu = 0.0f;
-  for (int s = 0; s < S; s++) {
+  // ---- G2 (+ UREP energy) per neighbour species block (generic grid)
+  for (int s = 0; s < (g2grid ? 0 : S); s++) {
     const int blk = bp->ind2[(s - ri.si + S) % S];
     if (ri.m(s) == 0) {
       if (lane < bp->ng2) Gi[blk + lane] = 0.0f;
@@ -933,10 +954,6 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         if (lane == 0) Gi[blk + c] = v;
       }
   }
-  if (erep) {
-    float v = wave_sum(eu);
-    if (lane == 0) erep[i] = v;
-  }
 
   // ---- G4 per species class (sa <= sb); with gpl, the surviving pairs of each class are stored for the force
   // kernel (class after class from gpl + i * pstride; npl holds each class's count)
@@ -955,6 +972,23 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
       float V[MG_GRID];
 #pragma unroll
       for (int q = 0; q < MG_GRID; q++) V[q] = 0.0f;
+      const bool g2here = g2grid && sa == sb;
+      if (g2here) {
+        // G2 of neighbour species sa (and its UREP energy) into the free cells G2OFF + c
+        const int p = ri.si * MG_MAXSPC + sa;
+        const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[sa];
+        for (int t = ri.o(sa) + lane; t < ri.o(sa) + ri.m(sa); t += WAVE) {
+          const float r = rec[R * t].w;
+          const float fc = rec[R * t + 1].y;
+#pragma unroll
+          for (int c = 0; c < MG_GRID - G2OFF; c++)
+            if (c < MG_MAXG2 && c < bp->ng2) {
+              const float dr = r - bp->rs2[p][c];
+              V[G2OFF + c] += fc * fast_exp2(-(float)MG_LOG2E * bp->eta2[p][c] * dr * dr);
+            }
+          if (urep > 0.0f && r < Rrep) eu += urep * 0.5f * (1.0f + cospi_(r / Rrep)) * Rrep / r;
+        }
+      }
       auto body = [&](int j, int k, int, bool valid) {
         if (!valid) return;
         float4 a = rec[R * j], b = rec[R * k];
@@ -985,17 +1019,24 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
           for (int q = 0; q < MG_MAXZ * MG_MAXL; q++)
             if (Pat<PAT>::mask(e) & (1u << q)) V[e * MG_MAXZ * MG_MAXL + q] += Bq[e] * P[q];
       };
-      const int np = for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, gout, body);
+      const int np = for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, gout, err, body);
       if (gout) {
         gout += np;
         if (lane == 0) npl[(size_t)i * MG_NCLS + cls] = np;
       }
       // lane l: grid cell l = (e*MAXZ + z)*MAXL + lambda; the zeta prefactor 2^(1 - zeta) goes on here
-      float mine = reduce_scatter64(V, lane) * bp->pref[(lane / MG_MAXL) % MG_MAXZ];
+      const float raw = reduce_scatter64(V, lane);
+      float mine = raw * bp->pref[(lane / MG_MAXL) % MG_MAXZ];
       int cell = lane < bp->ng4 ? bp->g4cell[lane] : 0;
       float v = __shfl(mine, cell, WAVE);
       if (lane < bp->ng4) Gi[blk + lane] = v;
+      if (g2here && lane >= G2OFF && lane - G2OFF < bp->ng2) Gi[bp->ind2[ds1] + lane - G2OFF] = raw;
     }
+  if (erep) {
+    // no UREP term (urep = 0, the usual case) leaves eu = 0 on every lane: no reduction needed
+    const float v = urep > 0.0f ? wave_sum(eu) : 0.0f;
+    if (lane == 0) erep[i] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------- MLP (FP32 VALU)
@@ -1059,7 +1100,7 @@ __global__ __launch_bounds__(64) void k_mlp_fp32(int N, const int* __restrict__ 
 // Forces: F_j -= dT/da, F_k -= dT/db, F_i += dT/da + dT/db; virial U -= dT/da (x) a + dT/db (x) b.
 // The dE/dG grid g(e,z,l) of the current species block is wave-uniform; it is staged through LDS once per class
 // and kept in registers for the class's triplets.
-template <int PAT, bool UNI, int MMA>
+template <int PAT, bool UNI, int MMA, bool ST>
 __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
                                                     const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
                                                     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG,
@@ -1085,18 +1126,27 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
   // the reverse slots of the first two batches of the row, loaded now so the final scatter does not wait on them
   const int rq0 = lane < ri.M ? rev[ri.off + lane] : -1;
   const int rq1 = lane + WAVE < ri.M ? rev[ri.off + lane + WAVE] : -1;
-  float4* rec = smem_f + (size_t)wv * ((R + 2) * cap + MG_GRID / 4 + PCH / 4);
+  float4* rec = smem_f + (size_t)wv * ((R + 2) * cap + MG_GRID / 4 + PRING / 4);
   float4* gts4 = rec + R * cap;                  // [MG_GRID / 4]
   float4* fnb4 = gts4 + MG_GRID / 4;             // [cap] force on each neighbour slot (pair terms, triplet j side)
   float4* fnk4 = fnb4 + cap;                     // [cap] triplet k side
   float* fnb = (float*)fnb4;
-  unsigned* plist = (unsigned*)(fnk4 + cap);     // [PCH]
+  unsigned* plist = (unsigned*)(fnk4 + cap);     // [PRING]
   stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
   for (int t = lane; t < ri.M; t += WAVE) {
     fnb4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
     fnk4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   wave_sync();
+  // MH: lb = 2 min_j r_ij^2, a lower bound of r_ij^2 + r_ik^2 + r_jk^2 over the row's triplets (wave-uniform)
+  float rlb = 0.0f;
+  if (MH) {
+    float mn = 3.0e38f;
+    for (int t = lane; t < ri.M; t += WAVE) mn = fminf(mn, rec[R * t + 1].w);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o, WAVE));
+    rlb = ri.M >= 2 ? 2.0f * mn : 0.0f;
+  }
   const float* gi = dEdG + (size_t)i * D;
   float fx = 0.f, fy = 0.f, fz = 0.f;
   float u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f, u4 = 0.f, u5 = 0.f;
@@ -1178,19 +1228,26 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
       float oscale = 1.0f;
       if (MH) {
         // FP16 form: A = 2 g / pref (the zeta prefactor goes into the powers of w = u/2, so every cell's
-        // coefficient keeps its own size), scaled by a power of two to below 2^14; B is scaled to (0, 2^14]
-        // inside the exp2; the hi/lo split then stays clear of FP16 overflow and subnormals
+        // coefficient keeps its own size).  B_e = exp(-eta_e rsum) is taken as exp(-eta_e (rsum - lb)) 2^14 with
+        // lb = 2 min_j r_ij^2 <= rsum, so every B lies in (0, 2^14] and the triplets that dominate a row's sum
+        // (rsum near lb) stay far above the FP16 subnormals whatever the eta; the factor exp(-eta_e lb) moves
+        // into A_e.  A is then scaled by one power of two to below 2^14, so the hi/lo split of both operands stays
+        // clear of FP16 overflow, and only terms smaller than 2^-28 of the largest lose their low half.  Unused
+        // zeta slots (pref = 0) have no coefficient.
         const int m = lane & 31, h = lane >> 5;
         const int r = 4 * (m >> 3) + (m & 3);
         const int qq = r < ZL ? r : r - ZL;
         const bool own = ((m >> 2) & 1) == h;
         const float* gts = (const float*)gts4;
-        const float ip = 2.0f / bp->pref[(qq / MG_MAXL) % MG_MAXZ];
+        const float pz = bp->pref[(qq / MG_MAXL) % MG_MAXZ];
+        const float ip = pz > 0.0f ? 2.0f / pz : 0.0f;
         float a[8], amax = 0.f;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
           a[v] = 0.f;
-          if (own && v < NE) a[v] = (r < ZL ? gts[v * ZL + qq] : e3[v < NE ? v : 0] * gts[v * ZL + qq]) * ip;
+          if (own && v < NE)
+            a[v] = (r < ZL ? gts[v * ZL + qq] : e3[v < NE ? v : 0] * gts[v * ZL + qq]) * ip *
+                   fast_exp2(n3[v < NE ? v : 0] * rlb);
           amax = fmaxf(amax, fabsf(a[v]));
         }
 #pragma unroll
@@ -1253,7 +1310,8 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
           float B[8];
 #pragma unroll
           for (int e = 0; e < 8; e++)
-            B[e] = e < NE ? fast_exp2(MH ? __builtin_fmaf(n3[e < NE ? e : 0], rsum, 14.0f) : n3[e < NE ? e : 0] * rsum)
+            B[e] = e < NE ? fast_exp2(MH ? __builtin_fmaf(n3[e < NE ? e : 0], rsum - rlb, 14.0f)
+                                         : n3[e < NE ? e : 0] * rsum)
                           : 0.f;
           unsigned h[4], l[4];
           f32x16 acc = {};
@@ -1317,12 +1375,14 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         float gbx = cbb * b.x + cb * a.x + c3c * cx, gby = cbb * b.y + cb * a.y + c3c * cy, gbz = cbb * b.z + cb * a.z + c3c * cz;
         group_accum(fnb4, fnk4, j, -gax, -gay, -gaz, k, -gbx, -gby, -gbz, tag, valid);
       };
-      if (pl) {
+      // ST: the pairs come from the stored list (the usual case; compiled apart from the enumeration, whose live
+      // state would otherwise share the register budget of the triplet arithmetic)
+      if (ST) {
         const int np = npl[(size_t)i * MG_NCLS + cls];
         for_stored_pairs(pl, np, lane, body);
         pl += np;
       } else {
-        for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, nullptr, body);
+        for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, nullptr, err, body);
       }
       wave_sync();
     }
@@ -1353,34 +1413,34 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
 }
 
 // one wave per atom: k_force contracts on the VALU in FP32, k_force_mm on the matrix cores (compensated BF16)
-template <int PAT, bool UNI>
+template <int PAT, bool UNI, bool ST>
 __global__ __launch_bounds__(256) void k_force(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
     const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG, float4* __restrict__ fself,
     float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
     const int* __restrict__ npl, size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
     const int* __restrict__ ovf) {
-  force_row<PAT, UNI, 0>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+  force_row<PAT, UNI, 0, ST>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
 }
-template <int PAT, bool UNI>
+template <int PAT, bool UNI, bool ST>
 __global__ __launch_bounds__(256) void k_force_mm(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
     const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG, float4* __restrict__ fself,
     float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
     const int* __restrict__ npl, size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
     const int* __restrict__ ovf) {
-  force_row<PAT, UNI, 1>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+  force_row<PAT, UNI, 1, ST>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
 }
 // five waves per SIMD: 96 VGPRs (from 101), the few spilled values are saved once outside the triplet loop
 // (-5% kernel time in same-box timing)
-template <int PAT, bool UNI>
+template <int PAT, bool UNI, bool ST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_force_mh(int N, int S, const int* __restrict__ spc, const int* __restrict__ row_off,
     const int* __restrict__ rowcnt, const int* __restrict__ cnt, const float4* __restrict__ vec,
     const mg_basis_dev* __restrict__ bp, const float* __restrict__ dEdG, float4* __restrict__ fself,
     float4* __restrict__ fe, float* __restrict__ vir, float urep, int cap, const unsigned* __restrict__ gpl,
     const int* __restrict__ npl, size_t pstride, const int* __restrict__ rev, int* __restrict__ err,
     const int* __restrict__ ovf) {
-  force_row<PAT, UNI, 2>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
+  force_row<PAT, UNI, 2, ST>(N, S, spc, row_off, rowcnt, cnt, vec, bp, dEdG, fself, fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
 }
 
 // Force on atom a: its own term plus what its neighbours' centres sent it, which the force kernel wrote into a's row
@@ -1591,6 +1651,32 @@ static int pick_wpb(int wpb, size_t sm_wave) {
     else { if (uni_) LAUNCH_PU(KER, 0, true, SM, __VA_ARGS__); else LAUNCH_PU(KER, 0, false, SM, __VA_ARGS__); } \
   } while (0)
 
+// the force kernels also take ST: stored pair lists (gpl != NULL) or their own enumeration
+#define LAUNCH_PUS(KER, PAT, UNI, ST, SM, ...)                                                                        \
+  do {                                                                                                               \
+    if ((size_t)(SM) * wpb_ > MG_LDS_BLOCK &&                                                                        \
+        hipFuncSetAttribute((const void*)&KER<PAT, UNI, ST>, hipFuncAttributeMaxDynamicSharedMemorySize,             \
+                            (int)((SM) * wpb_)) != hipSuccess)                                                       \
+      return -1;                                                                                                     \
+    hipLaunchKernelGGL((KER<PAT, UNI, ST>), dim3((N + wpb_ - 1) / wpb_), dim3(64 * wpb_), (SM) * wpb_, st, __VA_ARGS__); \
+  } while (0)
+#define LAUNCH_PU2(KER, PAT, UNI, SM, ...)                                        \
+  do {                                                                            \
+    if (st_) LAUNCH_PUS(KER, PAT, UNI, true, SM, __VA_ARGS__);                    \
+    else LAUNCH_PUS(KER, PAT, UNI, false, SM, __VA_ARGS__);                       \
+  } while (0)
+#define DISPATCH_PUS(KER, SM, STORED, ...)                         \
+  do {                                                             \
+    const int wpb_ = pick_wpb(wpb, SM);                            \
+    if (wpb_ < 0) return -3;                                       \
+    const int pat_ = variant & 15;                                 \
+    const bool uni_ = (variant >> 4) & 1;                          \
+    const bool st_ = (STORED);                                     \
+    if (pat_ == 1) { if (uni_) LAUNCH_PU2(KER, 1, true, SM, __VA_ARGS__); else LAUNCH_PU2(KER, 1, false, SM, __VA_ARGS__); } \
+    else if (pat_ == 2) { if (uni_) LAUNCH_PU2(KER, 2, true, SM, __VA_ARGS__); else LAUNCH_PU2(KER, 2, false, SM, __VA_ARGS__); } \
+    else { if (uni_) LAUNCH_PU2(KER, 0, true, SM, __VA_ARGS__); else LAUNCH_PU2(KER, 0, false, SM, __VA_ARGS__); } \
+  } while (0)
+
 // variant = G4 pattern (bits 0-3) | shared-eta flag (bit 4) | matrix-core force contraction (bit 5, force only)
 int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, const int* rowcnt, const int* cnt,
              const void* vec,
@@ -1601,7 +1687,7 @@ int mgk_desc(int N, int S, int variant, const int* spc, const int* row_off, cons
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
-  size_t sm = (size_t)cap * R * sizeof(float4) + PCH * sizeof(unsigned);
+  size_t sm = (size_t)cap * R * sizeof(float4) + PRING * sizeof(unsigned);
   DISPATCH_PU(k_desc, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, G, erep, urep, cap, gpl, npl, pstride,
               (const u64*)key, (const unsigned short*)rhash, hslots, rev, err, ovf);
   return mgk_status();
@@ -1623,18 +1709,18 @@ int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, con
   if (N == 0) return 0;
   const int cap = maxrow < 1 ? 1 : maxrow;
   const int R = ((variant >> 4) & 1) ? 2 : 4;
-  size_t sm = (size_t)cap * (R + 2) * sizeof(float4) + MG_GRID * sizeof(float) + PCH * sizeof(unsigned);
+  size_t sm = (size_t)cap * (R + 2) * sizeof(float4) + MG_GRID * sizeof(float) + PRING * sizeof(unsigned);
   if ((variant >> 6) & 1) {
-    DISPATCH_PU(k_force_mh, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself,
+    DISPATCH_PUS(k_force_mh, sm, gpl != nullptr, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself,
                 (float4*)fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
     return mgk_status();
   }
   if ((variant >> 5) & 1) {
-    DISPATCH_PU(k_force_mm, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself,
+    DISPATCH_PUS(k_force_mm, sm, gpl != nullptr, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself,
                 (float4*)fe, vir, urep, cap, gpl, npl, pstride, rev, err, ovf);
     return mgk_status();
   }
-  DISPATCH_PU(k_force, sm, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
+  DISPATCH_PUS(k_force, sm, gpl != nullptr, N, S, spc, row_off, rowcnt, cnt, (const float4*)vec, bp, dEdG, (float4*)fself, (float4*)fe, vir,
               urep, cap, gpl, npl, pstride, rev, err, ovf);
   return mgk_status();
 }
