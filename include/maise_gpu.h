@@ -85,6 +85,19 @@ int mg_get(mg_ctx *c, int s, double *E, double *EA, double *F, double *U, double
 /* Batch results: E[S], U[6S], F[3*Ntot] (NULL skips). */
 int mg_get_batch(mg_ctx *c, double *E, double *U, double *F);
 
+/* One FP32 energy + forces + virial evaluation that allocates nothing: no counterpart in the reference; it is for
+ * callers that cannot allocate device memory (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same inputs and outputs as mg_set_structure + mg_eval(MG_EVAL_FORCES) + mg_get (host arrays:
+ * spc[N], lat[9], x[3N] with xt = 0 fractional / 1 Cartesian, nd 0 or 3; E, EA[N], F[3N], U[6], NULL skips), on
+ * the current device and its null stream, synchronously.  Every intermediate buffer is a slice of one fixed
+ * 128 MiB device array per device, which the library holds until the process ends; the model tables are copied in
+ * by each call.  Limits: N <= 32768 atoms, neighbour rows <= 256, and everything must fit the array (about 20,000
+ * atoms at 80 neighbours and a 51-wide descriptor); beyond them the call fails with MG_ERR_ARG or MG_ERR_NEIGHBOR
+ * and a message.  Calls share the array and a host work area: calls that overlap (threads, or devices driven from
+ * several threads) race. */
+int mg_eval_static(const mg_model *m, int N, int nd, int xt, const int *spc, const double *lat, const double *x,
+                   double urep, double *E, double *EA, double *F, double *U);
+
 /* Neighbour list of the last evaluation: row_off[N+1]; j[M], img[3M] (image indices), M = row_off[N].
  * Call with j == NULL to get M via *nedges only. */
 int mg_get_neighbors(mg_ctx *c, int *nedges, int *row_off, int *j, int *img);

@@ -62,6 +62,7 @@ def lib():
             "mg_eval": ([vp, C.c_int], C.c_int),
             "mg_get": ([vp, C.c_int, dp, dp, dp, dp, dp], C.c_int),
             "mg_get_batch": ([vp, dp, dp, dp], C.c_int),
+            "mg_eval_static": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp, C.c_double, dp, dp, dp, dp], C.c_int),
             "mg_get_neighbors": ([vp, ip, ip, ip, ip], C.c_int),
             "mg_get_descriptors": ([vp, fp], C.c_int),
             "mg_get_timing": ([vp, fp], C.c_int),
@@ -149,6 +150,22 @@ class Model:
                 self._h = C.c_void_p()
         except Exception:
             pass
+
+
+def eval_static(model: "Model", spc, lat, x, nd: int = 3, xt: int = 1, urep: float = 0.0) -> dict:
+    """FP32 E/F/virial of one structure through mg_eval_static, the entry point that allocates nothing (all
+    intermediate buffers in the library's fixed per-device scratch array)."""
+    spc = np.ascontiguousarray(spc, dtype=np.int32)
+    lat = np.ascontiguousarray(lat, dtype=np.float64).reshape(9)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    N = spc.size
+    E = np.zeros(1)
+    EA = np.zeros(N)
+    F = np.zeros((N, 3))
+    U = np.zeros(6)
+    _check(lib().mg_eval_static(model._h, N, nd, xt, _ip(spc), _dp(lat), _dp(x), float(urep), _dp(E), _dp(EA), _dp(F),
+                                _dp(U)), "mg_eval_static")
+    return {"E": float(E[0]), "EA": EA, "F": F, "U": U}
 
 
 class Context:

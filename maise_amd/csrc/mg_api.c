@@ -34,7 +34,27 @@ int mg_dev_alloc(void **p, size_t bytes) {
   return MG_OK;
 }
 
-#define ALLOC(ptr, bytes) do { int r_ = mg_dev_alloc((void **)&(ptr), (bytes)); if (r_) return r_; } while (0)
+#define ALLOC(ptr, bytes)                                                                                        \
+  do {                                                                                                           \
+    if (c->no_alloc)                                                                                             \
+      return mg_fail(MG_ERR_ARG, "fixed scratch: a buffer of %zu bytes (%s) does not fit", (size_t)(bytes), #ptr); \
+    int r_ = mg_dev_alloc((void **)&(ptr), (bytes));                                                             \
+    if (r_) return r_;                                                                                           \
+  } while (0)
+
+/* host temporaries of an upload: malloc, or a slice of the context's host work area (fixed-scratch evaluation);
+ * *used tracks the slices taken; NULL when the work area is too small */
+static void *htmp(mg_ctx *c, size_t *used, size_t bytes) {
+  if (!c->h_work) return malloc(bytes > 0 ? bytes : 1);
+  const size_t b = (bytes + 15) & ~(size_t)15;
+  if (*used + b > c->h_work_bytes) return NULL;
+  void *p = c->h_work + *used;
+  *used += b;
+  return p;
+}
+static void hfree(mg_ctx *c, void *p) {
+  if (!c->h_work) free(p);
+}
 
 static int ensure_atoms(mg_ctx *c, size_t N) {
   if (N <= c->cap_atoms) return MG_OK;
@@ -352,7 +372,9 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
     if (spc[i] < 0 || spc[i] >= c->m->nspc) return mg_fail(MG_ERR_ARG, "atom %d: species %d outside the model's %d", i, spc[i], c->m->nspc);
   int r;
   if ((r = ensure_str(c, S)) || (r = ensure_atoms(c, N))) return r;
-  int *atom_str = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+  size_t hused = 0;
+  int *atom_str = (int *)htmp(c, &hused, sizeof(int) * (N > 0 ? N : 1));
+  if (!atom_str) return mg_fail(MG_ERR_ARG, "fixed scratch: %d atoms exceed the host work area", N);
   int bin0 = 0, nchunk = 0;
   const double rc = c->m->rc_list;
   for (int s = 0; s < S; s++) {
@@ -365,7 +387,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
     d->natom = atom_off[s + 1] - atom_off[s];
     d->nd = nd[s];
     d->xt = xt[s];
-    if (d->nd != 0 && d->nd != 3) { free(atom_str); return mg_fail(MG_ERR_ARG, "structure %d: ND must be 0 or 3", s); }
+    if (d->nd != 0 && d->nd != 3) { hfree(c, atom_str); return mg_fail(MG_ERR_ARG, "structure %d: ND must be 0 or 3", s); }
     plan_bins(d, rc, d->natom);
     d->bin0 = bin0;
     bin0 += d->nbins;
@@ -380,23 +402,26 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
   c->h_atom_off[S] = N;
   c->nbins = bin0;
   if ((size_t)nchunk > c->cap_chunk) {
+    if (c->no_alloc) return mg_fail(MG_ERR_ARG, "fixed scratch: %d reduction chunks do not fit", nchunk);
     if ((r = mg_dev_alloc((void **)&c->d_chunk_str, (nchunk + 16) * sizeof(int))) ||
-        (r = mg_dev_alloc((void **)&c->d_part, (nchunk + 16) * 7 * sizeof(double)))) { free(atom_str); return r; }
+        (r = mg_dev_alloc((void **)&c->d_part, (nchunk + 16) * 7 * sizeof(double)))) { hfree(c, atom_str); return r; }
     c->cap_chunk = nchunk + 16;
   }
   {
-    int *cs = (int *)malloc(sizeof(int) * (nchunk > 0 ? nchunk : 1));
+    int *cs = (int *)htmp(c, &hused, sizeof(int) * (nchunk > 0 ? nchunk : 1));
+    if (!cs) return mg_fail(MG_ERR_ARG, "fixed scratch: the host work area is too small");
     for (int s = 0; s < S; s++)
       for (int k = 0; k < c->h_str[s].nchunk; k++) cs[c->h_str[s].chunk0 + k] = s;
     hipError_t e = hipMemcpy(c->d_chunk_str, cs, sizeof(int) * nchunk, hipMemcpyHostToDevice);
-    free(cs);
-    if (e != hipSuccess) { free(atom_str); return mg_fail(MG_ERR_HIP, "chunk table: %s", hipGetErrorString(e)); }
+    hfree(c, cs);
+    if (e != hipSuccess) { hfree(c, atom_str); return mg_fail(MG_ERR_HIP, "chunk table: %s", hipGetErrorString(e)); }
   }
   c->nchunk = nchunk;
   /* species-sorted permutation and MLP tiles of <= 32 same-species atoms */
   {
     const int S_ = c->m->nspc;
-    int *perm = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+    int *perm = (int *)htmp(c, &hused, sizeof(int) * (N > 0 ? N : 1));
+    if (!perm) return mg_fail(MG_ERR_ARG, "fixed scratch: the host work area is too small");
     int cntS[MG_MAXSPC] = {0, 0, 0, 0}, off[MG_MAXSPC + 1];
     for (int i = 0; i < N; i++) cntS[spc[i]]++;
     off[0] = 0;
@@ -406,7 +431,8 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
     for (int i = 0; i < N; i++) perm[pos[spc[i]]++] = i;
     int nt = 0;
     for (int q = 0; q < S_; q++) nt += (cntS[q] + 31) / 32;
-    int *tiles = (int *)malloc(sizeof(int) * 4 * (nt > 0 ? nt : 1));
+    int *tiles = (int *)htmp(c, &hused, sizeof(int) * 4 * (nt > 0 ? nt : 1));
+    if (!tiles) return mg_fail(MG_ERR_ARG, "fixed scratch: the host work area is too small");
     nt = 0;
     for (int q = 0; q < S_; q++)
       for (int a = off[q]; a < off[q + 1]; a += 32) {
@@ -414,27 +440,28 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
         nt++;
       }
     if ((size_t)nt > c->cap_tiles || (size_t)N > c->cap_tiles * 32 || !c->d_perm) {
+      if (c->no_alloc) return mg_fail(MG_ERR_ARG, "fixed scratch: %d MLP tiles do not fit", nt);
       size_t cap = (size_t)nt + N / 32 + 64;
       if ((r = mg_dev_alloc((void **)&c->d_tiles, cap * 4 * sizeof(int))) ||
-          (r = mg_dev_alloc((void **)&c->d_perm, (cap * 32 + 64) * sizeof(int)))) { free(perm); free(tiles); free(atom_str); return r; }
+          (r = mg_dev_alloc((void **)&c->d_perm, (cap * 32 + 64) * sizeof(int)))) { hfree(c, perm); hfree(c, tiles); hfree(c, atom_str); return r; }
       c->cap_tiles = cap;
     }
     hipError_t e1 = hipMemcpy(c->d_perm, perm, sizeof(int) * N, hipMemcpyHostToDevice);
     hipError_t e2 = hipMemcpy(c->d_tiles, tiles, sizeof(int) * 4 * nt, hipMemcpyHostToDevice);
-    free(perm);
-    free(tiles);
-    if (e1 != hipSuccess || e2 != hipSuccess) { free(atom_str); return mg_fail(MG_ERR_HIP, "tile upload failed"); }
+    hfree(c, perm);
+    hfree(c, tiles);
+    if (e1 != hipSuccess || e2 != hipSuccess) { hfree(c, atom_str); return mg_fail(MG_ERR_HIP, "tile upload failed"); }
     c->ntiles = nt;
   }
-  if ((r = ensure_bins(c, bin0))) { free(atom_str); return r; }
+  if ((r = ensure_bins(c, bin0))) { hfree(c, atom_str); return r; }
   int nscan = N > bin0 ? N : bin0;
-  if ((r = ensure_scan_tmp(c, nscan))) { free(atom_str); return r; }
+  if ((r = ensure_scan_tmp(c, nscan))) { hfree(c, atom_str); return r; }
   HIPCHK(hipMemcpyAsync(c->d_str, c->h_str, S * sizeof(mg_struct_dev), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_atom_str, atom_str, N * sizeof(int), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_spc, spc, N * sizeof(int), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_xin, x, 3 * N * sizeof(double), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  free(atom_str);
+  hfree(c, atom_str);
   c->S = S;
   c->N = N;
   c->have_forces = c->have_g = 0;
@@ -863,4 +890,154 @@ void maise_gpu_free(LNK *L) {
   }
   L->storage_[0] = NULL;
   L->B = 0;
+}
+
+/* ------------------------------------------------------------------ fixed-scratch evaluation */
+/* An evaluation that allocates nothing: the device buffers of a throw-away context are slices of the library's
+ * static device array (mgs_scratch), the host temporaries come from a static host area, the chain runs on the null
+ * stream in its synchronised form, and any buffer the structure would need beyond the scratch fails with an error
+ * instead of being allocated.  Calls that overlap on one device (or on the host area) race. */
+#define MG_STATIC_MAX_ATOMS 32768
+#define MG_STATIC_MAX_ROW 256    /* longest neighbour row the reverse-lookup tables are laid out for */
+static unsigned char g_static_hwork[16 * MG_STATIC_MAX_ATOMS + 4096];
+
+typedef struct { unsigned char *base; size_t used, size; } mg_carve;
+static void *carve(mg_carve *a, size_t bytes) {
+  const size_t off = (a->used + 255) & ~(size_t)255;
+  if (off + bytes > a->size) return NULL;
+  a->used = off + bytes;
+  return a->base + off;
+}
+
+int mg_eval_static(const mg_model *m, int N, int nd, int xt, const int *spc, const double *lat, const double *x,
+                   double urep, double *E, double *EA, double *F, double *U) {
+  if (!m || N < 1 || !spc || !lat || !x) return mg_fail(MG_ERR_ARG, "mg_eval_static: bad argument");
+  if (N > MG_STATIC_MAX_ATOMS)
+    return mg_fail(MG_ERR_ARG, "mg_eval_static: %d atoms exceed the fixed-scratch limit of %d", N, MG_STATIC_MAX_ATOMS);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  void *base = NULL;
+  size_t bytes = 0;
+  if (mgs_scratch(dev, &base, &bytes) != 0)
+    return mg_fail(MG_ERR_HIP, "mg_eval_static: the scratch array of device %d could not be looked up", dev);
+  mg_ctx cc;
+  memset(&cc, 0, sizeof cc);
+  mg_ctx *c = &cc;
+  int flags[16];
+  mg_struct_dev h_str[1];
+  int h_atom_off[2], h_nd[1];
+  double h_lat[9];
+  c->m = m;
+  c->device = dev;
+  c->st = 0;  /* the null stream */
+  c->urep = urep;
+  c->prec = MG_PREC_FP32;
+  c->no_alloc = 1;
+  c->pair_store = 0;  /* the force kernel enumerates the pairs itself: no O(N M^2) pair lists in the scratch */
+  c->graph_enable = 0;
+  c->graph_headroom = 0.0;
+  c->h_flags = flags;
+  c->h_str = h_str;
+  c->h_atom_off = h_atom_off;
+  c->h_nd = h_nd;
+  c->h_lat = h_lat;
+  c->cap_str = 1;
+  c->h_work = g_static_hwork;
+  c->h_work_bytes = sizeof g_static_hwork;
+  mg_carve a = {(unsigned char *)base, 0, bytes};
+#define CARVE(ptr, nb)                                                                                              \
+  do {                                                                                                              \
+    void *p_ = carve(&a, (nb));                                                                                     \
+    if (!p_) return mg_fail(MG_ERR_ARG, "mg_eval_static: %d atoms do not fit the %zu-byte fixed scratch", N, bytes); \
+    *(void **)&(ptr) = p_;                                                                                          \
+  } while (0)
+  const size_t n = (size_t)N, D = (size_t)m->D;
+  /* model tables, copied in by every call (the scratch keeps whatever the last call left) */
+  CARVE(c->d_basis, sizeof(mg_basis_dev));
+  CARVE(c->d_mlp, (size_t)m->wstride * m->nspc * sizeof(float));
+  if (m->mfma_ok) {
+    CARVE(c->d_frag, m->frag_n * sizeof(float));
+    CARVE(c->d_frag_bf, m->frag_bf_n * sizeof(uint16_t));
+    CARVE(c->d_frag_lo, m->frag_bf_n * sizeof(uint16_t));
+    CARVE(c->d_fvec, m->fvec_n * sizeof(float));
+  }
+  CARVE(c->d_misc, 16 * sizeof(int));
+  CARVE(c->d_str, sizeof(mg_struct_dev));
+  CARVE(c->d_E, sizeof(double));
+  CARVE(c->d_U, 6 * sizeof(double));
+  /* per atom (the buffers ensure_atoms would allocate) */
+  CARVE(c->d_atom_str, n * sizeof(int));
+  CARVE(c->d_spc, n * sizeof(int));
+  CARVE(c->d_xin, n * 3 * sizeof(double));
+  CARVE(c->d_x, n * 3 * sizeof(double));
+  CARVE(c->d_bin, 2 * n * sizeof(int));
+  CARVE(c->d_bin_rec, n * 32);
+  CARVE(c->d_cnt, n * 4 * sizeof(int));
+  CARVE(c->d_npl, n * MG_NCLS * sizeof(int));
+  CARVE(c->d_rowcnt, n * sizeof(int));
+  CARVE(c->d_row_off, (n + 1) * sizeof(int));
+  CARVE(c->d_G, n * D * sizeof(float));
+  CARVE(c->d_dEdG, n * D * sizeof(float));
+  CARVE(c->d_Ei, n * sizeof(float));
+  CARVE(c->d_erep, n * sizeof(float));
+  CARVE(c->d_vir, n * 6 * sizeof(float));
+  CARVE(c->d_fself, n * 16);
+  CARVE(c->d_F, n * 3 * sizeof(double));
+  c->cap_atoms = n;
+  /* reduction chunks, MLP tiles, bins (at most 4 N + 64 for one structure) and the scan workspace */
+  const size_t nchunk = (n + MG_RCHUNK - 1) / MG_RCHUNK + 16;
+  CARVE(c->d_chunk_str, nchunk * sizeof(int));
+  CARVE(c->d_part, nchunk * 7 * sizeof(double));
+  c->cap_chunk = nchunk;
+  const size_t ntile = n / 32 + MG_MAXSPC + 64;
+  CARVE(c->d_tiles, ntile * 4 * sizeof(int));
+  CARVE(c->d_perm, (ntile * 32 + 64) * sizeof(int));
+  c->cap_tiles = ntile;
+  const size_t nbins = 4 * n + 64 + 16;
+  CARVE(c->d_bin_count, (nbins + 1) * sizeof(int));
+  CARVE(c->d_bin_start, (nbins + 1) * sizeof(int));
+  c->cap_bins = nbins;
+  {
+    size_t need = 0;
+    if (mgk_scan(NULL, NULL, (int)(nbins > n ? nbins : n), NULL, &need, 0) != 0)
+      return mg_fail(MG_ERR_HIP, "mg_eval_static: scan size query failed");
+    CARVE(c->d_tmp, need);
+    c->tmp_bytes = need;
+  }
+  /* reverse-lookup tables for rows of up to MG_STATIC_MAX_ROW, then the edge buffers take the rest (44 bytes per edge:
+   * key, vector, reverse index, force) */
+  const size_t slots = (size_t)mgk_rev_slots(MG_STATIC_MAX_ROW);
+  CARVE(c->d_rhash, n * slots * sizeof(uint16_t));
+  c->cap_rhash = n * slots;
+  {
+    const size_t left = a.size > a.used + 4 * 256 ? a.size - a.used - 4 * 256 : 0;
+    size_t ne = left / 44;
+    if (ne > 0x7fffffff) ne = 0x7fffffff;
+    CARVE(c->d_key, ne * 8);
+    CARVE(c->d_vec, ne * 16);
+    CARVE(c->d_rev, ne * sizeof(int));
+    CARVE(c->d_fe, ne * 16);
+    c->cap_edges = ne;
+  }
+#undef CARVE
+  HIPCHK(hipMemcpyAsync(c->d_basis, &m->basis, sizeof(mg_basis_dev), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_mlp, m->mlp, (size_t)m->wstride * m->nspc * sizeof(float), hipMemcpyHostToDevice, c->st));
+  if (m->mfma_ok) {
+    HIPCHK(hipMemcpyAsync(c->d_frag, m->frag, m->frag_n * sizeof(float), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_frag_bf, m->frag_bf, m->frag_bf_n * sizeof(uint16_t), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_frag_lo, m->frag_lo, m->frag_bf_n * sizeof(uint16_t), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_fvec, m->fvec, m->fvec_n * sizeof(float), hipMemcpyHostToDevice, c->st));
+  }
+  int off[2] = {0, N}, r;
+  if ((r = mgh_upload(c, 1, off, &nd, &xt, lat, spc, x)) != MG_OK) return r;
+  if ((r = mgh_eval_chain(c, 1, MG_CHAIN_SYNC)) != MG_OK) return r;
+  if (E) HIPCHK(hipMemcpy(E, c->d_E, sizeof(double), hipMemcpyDeviceToHost));
+  if (U) HIPCHK(hipMemcpy(U, c->d_U, 6 * sizeof(double), hipMemcpyDeviceToHost));
+  if (F) HIPCHK(hipMemcpy(F, c->d_F, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  if (EA) {
+    float *tmp = (float *)g_static_hwork;  /* the upload's temporaries are no longer needed */
+    HIPCHK(hipMemcpy(tmp, c->d_Ei, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < N; i++) EA[i] = tmp[i];
+  }
+  return MG_OK;
 }

@@ -107,6 +107,11 @@ struct mg_ctx {
   size_t cap_atoms, cap_str, cap_bins, cap_edges;
   int nedges, maxrow;
   int have_forces, have_g;
+  /* fixed-scratch evaluation (mg_eval_static): every device buffer is a slice of the library's static device array
+   * and the host temporaries of the upload come from h_work; any attempt to grow a buffer fails instead */
+  int no_alloc;
+  unsigned char *h_work;
+  size_t h_work_bytes;
   /* dynamics state (mg_dyn.c) */
   double *d_v;          /* velocities [3N] (A/fs) */
   double *d_mass;       /* per-atom mass [N] (eV fs^2 / A^2) */
@@ -121,6 +126,8 @@ struct mg_ctx {
 
 
 int mg_fail(int code, const char *fmt, ...);
+/* per-device address and size of the static scratch array (mg_static.hip) */
+int mgs_scratch(int device, void **ptr, size_t *bytes);
 int mg_dev_alloc(void **p, size_t bytes);
 
 /* kernel launchers (mg_kernels.hip) */

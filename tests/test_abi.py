@@ -68,3 +68,14 @@ def test_model_reader_rejects_bad_files(tmp_path):
     p.write_text("not a model\n")
     with pytest.raises(maise_amd.MaiseError):
         maise_amd.Model(str(p))
+
+
+def test_static_eval_rejects_oversize_before_any_gpu_work():
+    """The evaluation that allocates nothing refuses structures beyond its fixed scratch with a clear message
+    (checked before any device call, so this runs without a GPU)."""
+    import numpy as np
+    import maise_amd
+    m = maise_amd.Model(os.path.join(ROOT, "maise_amd", "potentials", "cu_51_10_10.nn"))
+    N = 32769
+    with pytest.raises(maise_amd.MaiseError, match="fixed-scratch limit"):
+        maise_amd.eval_static(m, np.zeros(N, np.int32), np.eye(3) * 100.0, np.zeros((N, 3)))
