@@ -7,11 +7,16 @@ region covers neighbour build, descriptors, MLP forward/backward, forces/virial 
 K evaluations, inputs resident in HBM.
 
 The JSON line also carries
-  roofline      the dominant kernel (force/virial backward, FP32 ALU bound) priced against the MI355X FP32 peak:
-                algorithmic flops per launch (exact neighbour and triplet counts of the structure) / mean
-                kernel time measured with HIP events on the library's stream over the timed region;
-  cpu_baseline  the reference's own CPU path (compiled from its sources, oracle/_ref/maise_ref, OpenMP over the
-                host cores) timed on a bounded sample of the same workload (Cu fcc 6x6x6, 864 atoms);
+  roofline      the dominant kernel (the descriptor or the force/virial kernel, FP32 vector-ALU bound) priced against
+                the MI355X FP32 peak: algorithmic flops per launch (exact neighbour and triplet counts of the
+                structure) / mean kernel time measured with HIP events on the library's stream over the timed region;
+                beside it the roofline model of the whole evaluation (T_roofline / T_measured with neighbour bytes
+                against HBM, descriptor+force flops and transcendentals against the FP32 peak and the transcendental
+                rate measured on the device by the library's microbenchmark, MLP flops against the FP32 MFMA peak)
+                and each stage's fraction of its own bound;
+  cpu_baseline  the reference's own CPU path (compiled from its sources, oracle/_ref/maise_ref) timed on bounded
+                samples of the same workload: Cu fcc 10x10x10 (4,000 atoms) on the host's cores (at most the 16-core
+                share of one GPU) and Cu fcc 4x4x4 (256 atoms) on one core, with nproc and the CPU model;
   error_fp64    max |dE|/atom and |dF| of the FP32 path against the FP64 oracle on the 256-atom cell.
 """
 from __future__ import annotations
@@ -65,24 +70,95 @@ def flops_model(M, K, ng2=8, neta=7, ng4=43):
     return desc, force
 
 
+def host_cpu_info():
+    """(logical CPUs this process may run on, CPU model name) of the host."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return nproc, model
+
+
 def cpu_baseline(threads):
-    """Reference CPU path on a bounded sample (864 atoms of the same lattice/potential)."""
+    """The reference's own CPU path (CALL_MAISE compiled from its sources, oracle/_ref) on bounded samples of the same
+    lattice and potential: Cu fcc 10x10x10 (4,000 atoms) on `threads` OpenMP threads, and Cu fcc 4x4x4 (256 atoms) on
+    one thread.  Each figure is the mean of 2 calls after one warm-up, model load excluded."""
     from maise_amd import workloads as wl
     from oracle import pyoracle as po
-    s, l, x = wl.cu_fcc(6, 3.61, 0.05, seed=1)
+    nproc, cpu_model = host_cpu_info()
+    info = {"nproc": nproc, "cpu_model": cpu_model}
     if po.have_reference():
-        r = po.run_reference(POT, s, l, x, nm=200, repeats=2, threads=threads, timeout=600)
-        return {"value": len(s) / r["time"], "unit": "atom E+F evals/s", "cores": threads, "kind": "reference",
-                "sample": f"reference CALL_MAISE (compiled from its sources, OpenMP {threads} threads) on Cu fcc "
-                          f"6x6x6 = {len(s)} atoms, same potential and lattice; mean of 2 calls after one warm-up "
-                          "(model load excluded)"}
+        s, l, x = wl.cu_fcc(10, 3.61, 0.05, seed=1)
+        r = po.run_reference(POT, s, l, x, nm=200, repeats=2, threads=threads, timeout=900)
+        s1, l1, x1 = wl.cu_fcc(4, 3.61, 0.05, seed=1)
+        r1 = po.run_reference(POT, s1, l1, x1, nm=200, repeats=2, threads=1, timeout=900)
+        out = {"value": len(s) / r["time"], "unit": "atom E+F evals/s", "cores": threads, "kind": "reference",
+               "sample": f"reference CALL_MAISE (compiled from its sources, OpenMP {threads} threads) on Cu fcc "
+                         f"10x10x10 = {len(s)} atoms, same potential and lattice; mean of 2 calls after one warm-up "
+                         "(model load excluded)",
+               "single_core": {"value": len(s1) / r1["time"], "unit": "atom E+F evals/s", "cores": 1,
+                               "sample": f"the same on Cu fcc 4x4x4 = {len(s1)} atoms, one thread"}}
+        out.update(info)
+        return out
     m = po.OracleModel(POT)
+    s, l, x = wl.cu_fcc(6, 3.61, 0.05, seed=1)
     m.evaluate(s, l, x)
     t0 = time.perf_counter()
     m.evaluate(s, l, x)
     t = time.perf_counter() - t0
-    return {"value": len(s) / t, "unit": "atom E+F evals/s", "cores": po.num_threads(), "kind": "port",
-            "sample": f"FP64 oracle on Cu fcc 6x6x6 = {len(s)} atoms"}
+    out = {"value": len(s) / t, "unit": "atom E+F evals/s", "cores": po.num_threads(), "kind": "port",
+           "sample": f"FP64 oracle on Cu fcc 6x6x6 = {len(s)} atoms"}
+    out.update(info)
+    return out
+
+
+def measured_rates(maise_amd):
+    """Issue rates measured on the device by the library's microbenchmark (lane operations per second)."""
+    names = ("fma", "exp2", "cos", "sqrt", "rcp")
+    return {n: maise_amd.calibrate_rate(k) for k, n in enumerate(names)}
+
+
+def roofline_model(N, M, K, stages, ms, model, rates, ng2=8, neta=7, ng4=43):
+    """Roofline model v1 of the evaluation: T_roofline = neighbour bytes / HBM bandwidth + max(descriptor and force
+    flops / FP32 peak, their transcendentals / the measured transcendental rate) + MLP flops / FP32 MFMA peak, and the
+    fraction of each stage's own bound reached at its measured time."""
+    D, H1, H2 = model.D, model.H1, model.H2
+    nbr_bytes = N * (28 + 4) + 8 * M
+    fl_desc, fl_force = flops_model(M, K, ng2, neta, ng4)
+    fl_df = fl_desc + fl_force
+    trans = M * (5 + 2 * ng2) + K * (12 + 3 * neta)
+    fl_mlp = N * (2 * (D * H1 + H1 * H2 + H2) + 2 * (H2 * H1 + H1 * D) + 2 * H2)
+    p_trans = rates["exp2"]
+    t_nbr = nbr_bytes / (HBM_PEAK_GBS * 1e9)
+    t_df = max(fl_df / (FP32_PEAK_TFLOPS * 1e12), trans / p_trans)
+    t_mlp = fl_mlp / (FP32_PEAK_TFLOPS * 1e12)
+    t_roof = t_nbr + t_df + t_mlp
+    s_nbr, s_desc, s_mlp, s_force = (stages[0] * 1e-3, stages[1] * 1e-3, stages[2] * 1e-3, stages[3] * 1e-3)
+    return {
+        "t_roofline_ms": round(t_roof * 1e3, 4),
+        "t_measured_ms": round(ms, 4),
+        "t_roofline_over_t_measured": round(t_roof * 1e3 / ms, 4),
+        "per_stage": {
+            "neighbours": {"bound": "hbm", "bytes": nbr_bytes, "achieved_GBs": round(nbr_bytes / s_nbr / 1e9, 1),
+                           "peak_GBs": HBM_PEAK_GBS, "frac": round(nbr_bytes / s_nbr / 1e9 / HBM_PEAK_GBS, 4)},
+            "descriptors_forces": {"bound": "valu", "flops": fl_df, "transcendentals": trans,
+                                   "achieved_TFLOPs": round(fl_df / (s_desc + s_force) / 1e12, 2),
+                                   "frac_fp32_peak": round(fl_df / (s_desc + s_force) / 1e12 / FP32_PEAK_TFLOPS, 4),
+                                   "achieved_trans_per_s": trans / (s_desc + s_force),
+                                   "frac_trans_rate": round(trans / (s_desc + s_force) / p_trans, 4)},
+            "mlp": {"bound": "mfma", "flops": fl_mlp, "achieved_TFLOPs": round(fl_mlp / s_mlp / 1e12, 2),
+                    "peak_TFLOPs": FP32_PEAK_TFLOPS, "frac": round(fl_mlp / s_mlp / 1e12 / FP32_PEAK_TFLOPS, 4)},
+        },
+        "peaks": {"hbm_GBs": HBM_PEAK_GBS, "fp32_TFLOPs": FP32_PEAK_TFLOPS, "fp32_mfma_TFLOPs": FP32_PEAK_TFLOPS,
+                  "trans_per_s_measured": p_trans},
+        "measured_rates_per_s": {k: float(v) for k, v in rates.items()},
+    }
 
 
 RELAX_POT = os.path.join(ROOT, "tests", "golden", "models", "nn_CuPdAg_d3_v1")
@@ -212,6 +288,8 @@ def main():
     ap.add_argument("--ga-gens", type=int, default=30, help="generations of the screened search (0 skips it)")
     ap.add_argument("--ga-pop", type=int, default=512, help="population of the screened search")
     ap.add_argument("--ga-single", type=int, default=32, help="offspring relaxed one at a time for the plain rate")
+    ap.add_argument("--rates-only", action="store_true",
+                    help="only run the library's issue-rate microbenchmark (FP32 FMA, exp2, cos, sqrt, rcp) and print it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (5 times): the reference point that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profiling run to bytes")
@@ -237,6 +315,11 @@ def main():
 
     import maise_amd
     from maise_amd import workloads as wl
+    if args.rates_only:
+        r = measured_rates(maise_amd)
+        print(json.dumps({"ops_per_s": r, "fma_TFLOPs": 2 * r["fma"] / 1e12,
+                          "exp2_over_fma": r["exp2"] / r["fma"], "cos_over_fma": r["cos"] / r["fma"]}))
+        return
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -281,14 +364,19 @@ def main():
             cal = tj["calibration"]
             traffic = int(1024 * (kk["FETCH_SIZE"] * cal["fetch_scale"] + kk["WRITE_SIZE"] * cal["write_scale"]))
             traffic_src = "profiles/traffic_cu32k_w64_fp32.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
-    roof = {"bound": "mfma",
-            "bound_detail": "FP32 vector ALU issue (neither HBM nor the matrix cores bound it; the force kernel's "
-                            "triplet contraction runs on the matrix cores beside it); priced at the published FP32 "
-                            "peak, which counts packed FMA; without packed FP32 (not used) the issue limit is half",
+    rates = measured_rates(maise_amd)  # after the timed region: the microbenchmark runs on its own
+    model_roof = roofline_model(N, M, K, stages, ms, model, rates)
+    roof = {"bound": "valu",
+            "bound_detail": "FP32 vector ALU issue, FMA and transcendental (neither HBM nor the matrix cores bound "
+                            "the descriptor and force kernels; the force kernel's triplet contraction runs on the "
+                            "matrix cores beside it); priced at the published FP32 peak, which counts packed FMA "
+                            "(packed FP32 is not used; the measured non-packed FMA issue limit is in "
+                            "measured_rates_per_s)",
             "kernel": kernel, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_source": traffic_src,
-            "frac_of_unpacked_fp32": round(achieved / (FP32_PEAK_TFLOPS / 2), 4),
+            "frac_of_measured_fma_issue": round(achieved / (2 * rates["fma"] / 1e12), 4),
             "flops_per_launch": fl, "kernel_ms": round(t * 1e3, 4), "sum_M": M, "sum_K": K}
+    roof.update(model_roof)
 
     # FP32 path against the reference's own FP64 output (CALL_MAISE compiled from its sources, committed as a golden
     # vector) on the 256-atom cell with the same potential
@@ -350,7 +438,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        # the host cores this process may use, at most the 16-core share of one GPU on a shared node
+        threads = min(host_cpu_info()[0], 16)
         try:
             cpu = cpu_baseline(threads)
         except Exception as e:

@@ -107,6 +107,10 @@ int mg_get_descriptors(mg_ctx *c, float *G);
  * was enabled or last read (reading resets the average): t[0] neighbour build, t[1] descriptor kernel, t[2] MLP,
  * t[3] force/virial kernel, t[4] force gather + FP64 reductions, t[5] total.  t must hold 6 floats. */
 int mg_get_timing(mg_ctx *c, float *t);
+/* Issue rate of one vector ALU operation on the device, in operations (lanes) per second: op 0 FP32 FMA (non-packed),
+ * 1 exp2, 2 cos, 3 sqrt (one add issued per sqrt), 4 rcp.  A microbenchmark with no counterpart in the reference:
+ * it measures the rates that bound the descriptor and force kernels, which price their roofline. */
+int mg_calibrate_rate(int device, int op, double *ops_per_s);
 /* Enable (1) / disable (0) per-stage event timing (small overhead; evaluations stay asynchronous). */
 int mg_ctx_set_timing(mg_ctx *c, int on);
 /* Launch shape of the wave-per-atom descriptor and force kernels: atoms (64-lane waves) per workgroup, 1, 2 or 4;

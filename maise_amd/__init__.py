@@ -62,6 +62,7 @@ def lib():
             "mg_eval": ([vp, C.c_int], C.c_int),
             "mg_get": ([vp, C.c_int, dp, dp, dp, dp, dp], C.c_int),
             "mg_get_batch": ([vp, dp, dp, dp], C.c_int),
+            "mg_calibrate_rate": ([C.c_int, C.c_int, dp], C.c_int),
             "mg_eval_static": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp, C.c_double, dp, dp, dp, dp], C.c_int),
             "mg_get_neighbors": ([vp, ip, ip, ip, ip], C.c_int),
             "mg_get_descriptors": ([vp, fp], C.c_int),
@@ -150,6 +151,13 @@ class Model:
                 self._h = C.c_void_p()
         except Exception:
             pass
+
+
+def calibrate_rate(op: int, device: int = 0) -> float:
+    """Measured issue rate (lane operations/s) of op: 0 FP32 FMA, 1 exp2, 2 cos, 3 sqrt, 4 rcp."""
+    v = C.c_double()
+    _check(lib().mg_calibrate_rate(device, op, C.byref(v)), "mg_calibrate_rate")
+    return v.value
 
 
 def eval_static(model: "Model", spc, lat, x, nd: int = 3, xt: int = 1, urep: float = 0.0) -> dict:

@@ -23,7 +23,7 @@ HIPCC_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-munsafe
 CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
             f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["mg_kernels.hip", "mg_mlp.hip", "mg_dyn.hip", "mg_sur.hip", "mg_static.hip"]
+HIP_SOURCES = ["mg_kernels.hip", "mg_mlp.hip", "mg_dyn.hip", "mg_sur.hip", "mg_static.hip", "mg_calib.hip"]
 # per-file options.  mg_kernels.hip: MFMA accumulators in ordinary VGPRs (the force kernel reads its results with
 # VALU instructions straight away, which the AGPR form would first copy out one register at a time); the MLP
 # kernels in mg_mlp.hip measured faster with the AGPR form
