@@ -41,7 +41,7 @@ typedef struct mg_ctx mg_ctx;
 
 /* MLP precision of mg_ctx_set_precision */
 #define MG_PREC_FP32 0       /* descriptors, MLP and forces in FP32; energy/virial sums in FP64 */
-#define MG_PREC_BF16 1       /* MLP GEMMs in BF16 on MFMA (one pass), FP32 accumulation, FP32 descriptors */
+#define MG_PREC_BF16 1       /* hidden-layer GEMMs (W1 h1, W1^T delta2) in one-pass BF16 on MFMA, FP32 accumulation; FP32 input layer and descriptors */
 #define MG_PREC_BF16X3 2     /* MLP GEMMs in compensated BF16: split operands hi + lo, three MFMA passes, FP32 accumulation */
 
 const char *mg_last_error(void);

@@ -47,32 +47,36 @@ __device__ __forceinline__ f32x16 mfma_step(const float* __restrict__ afrag, con
   }
 }
 
-// Fragment offsets (in A-operand slots of 64 lanes; BF16 slots hold 8 values) of one species, see mg_model.c
+// Fragment offsets (in A-operand slots of 64 lanes; BF16 slots hold 8 values) of one species, see mg_model.c.  The
+// input layer (W0' x and W0'^T delta1) and the hidden layer (W1 h1 and W1^T delta2) may use different sets: FP32
+// (one value per slot, K = 2 per step) or BF16 (eight values, K = 16).
 struct MlpFrag {
-  int D, H1, H2, hb1, hb2, db, k1;   // k1: layer-1 steps
-  int o_l1, o_l2, o_b1, o_g0;        // slot offsets of the four A-operand sets
-  int o_vec;                          // float offset of b0[hb1*32], b1[hb2*32], w2[hb2*32], b2, xoff[D]
-  int stride_slots, stride_vec;       // per-species strides
+  int D, H1, H2, hb1, hb2, db, k1;   // k1: layer-1 steps of the input layer's set
+  int o_l1, o_g0;                     // slot offsets of the input-layer A operands (input layer's set)
+  int o_l2, o_b1;                     // slot offsets of the hidden-layer A operands (hidden layer's set)
+  int sf, sb;                         // per-species strides of the FP32 and BF16 sets (slots)
+  int stride_vec;                     // per-species stride of b0[hb1*32], b1[hb2*32], w2[hb2*32], b2, xoff[D]
 };
 
-template <int PREC, int HB1, int HB2>
+// PIN: precision of the input layer, PHID: of the hidden layer (0 FP32, 1 one-pass BF16, 2 compensated BF16)
+template <int PIN, int PHID, int HB1, int HB2>
 __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles, const int* __restrict__ perm,
                                                  const float* __restrict__ G, const float* __restrict__ frag,
                                                  const __bf16* __restrict__ frag_bf, const __bf16* __restrict__ frag_lo,
                                                  const float* __restrict__ vecs, MlpFrag f, float* __restrict__ Ei,
                                                  float* __restrict__ dEdG) {
-  constexpr bool BF16 = PREC != 0;
-  constexpr int KS = BF16 ? 8 : 1;   // B values per lane per step
+  constexpr bool BIN = PIN != 0, BHID = PHID != 0;
+  constexpr int KS = BIN ? 8 : 1;     // B values per lane per step, input layer
+  constexpr int KSH = BHID ? 8 : 1;   // hidden layer
   const int4 tl = tiles[blockIdx.x];
   const int sp = tl.x, start = tl.y, count = tl.z;
   const int lane = threadIdx.x, col = lane & 31, h = lane >> 5;
   const int atom = col < count ? perm[start + col] : -1;
   extern __shared__ float gsm[];  // [32][Kmax + 1] staged inputs
   const int D = f.D;
-  const size_t so = (size_t)sp * f.stride_slots;
-  const float* fa = frag + so * 64;
-  const __bf16* fb = frag_bf + so * 64 * 8;
-  const __bf16* fl = frag_lo + so * 64 * 8;
+  const float* fa = frag + (size_t)sp * f.sf * 64;
+  const __bf16* fb = frag_bf + (size_t)sp * f.sb * 64 * 8;
+  const __bf16* fl = frag_lo + (size_t)sp * f.sb * 64 * 8;
   const float* vv = vecs + (size_t)sp * f.stride_vec;
   const float* b0 = vv;
   const float* b1 = b0 + HB1 * 32;
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   // the tile's inputs x - xoff staged in LDS first ([32][Kmax + 1], zero past D and for missing atoms): 32 rows of
   // coalesced loads all in flight at once, instead of two dependent memory round trips per K step
   const int k1 = f.k1;
-  const int Kmax = BF16 ? 16 * k1 : 2 * k1;
+  const int Kmax = BIN ? 16 * k1 : 2 * k1;
   const int RS = Kmax + 1;
   for (int k0 = 0; k0 < Kmax; k0 += WAVE) {
     const int k = k0 + lane, kc = k < D ? k : D - 1;
@@ -114,14 +118,14 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
     for (int u = 0; u < 4; u++) {
       const int su = st0 + u < k1 ? st0 + u : k1 - 1;
 #pragma unroll
-      for (int v = 0; v < KS; v++) xs[u][v] = st0 + u < k1 ? gr[BF16 ? 16 * su + 8 * h + v : 2 * su + h] : 0.0f;
+      for (int v = 0; v < KS; v++) xs[u][v] = st0 + u < k1 ? gr[BIN ? 16 * su + 8 * h + v : 2 * su + h] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int st = st0 + u < k1 ? st0 + u : k1 - 1;
 #pragma unroll
       for (int rb = 0; rb < HB1; rb++)
-        a1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l1 * 64 + (rb * k1 + st) * 64 + lane, xs[u], a1[rb]);
+        a1[rb] = mfma_step<PIN>(fa, fb, fl, f.o_l1 * 64 + (rb * k1 + st) * 64 + lane, xs[u], a1[rb]);
     }
   }
 #pragma unroll
@@ -130,7 +134,7 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
     for (int r = 0; r < 16; r++) a1[rb][r] = tanh_f(a1[rb][r]);
 
   // ---- layer 2: a2 = tanh(b1 + W1 h1); energy; delta2 = W2 (1 - h2^2)
-  constexpr int S2 = BF16 ? 2 : 16;  // steps per 32-unit block
+  constexpr int S2 = BHID ? 2 : 16;  // hidden-layer steps per 32-unit block
   f32x16 a2[HB2];
 #pragma unroll
   for (int rb = 0; rb < HB2; rb++)
@@ -140,13 +144,13 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   for (int rb1 = 0; rb1 < HB1; rb1++)
 #pragma unroll
     for (int q = 0; q < S2; q++) {
-      float xb[KS];
+      float xb[KSH];
 #pragma unroll
-      for (int v = 0; v < KS; v++) xb[v] = a1[rb1][BF16 ? 8 * q + v : q];
+      for (int v = 0; v < KSH; v++) xb[v] = a1[rb1][BHID ? 8 * q + v : q];
       const int st = rb1 * S2 + q;
 #pragma unroll
       for (int rb = 0; rb < HB2; rb++)
-        a2[rb] = mfma_step<PREC>(fa, fb, fl, f.o_l2 * 64 + (rb * HB1 * S2 + st) * 64 + lane, xb, a2[rb]);
+        a2[rb] = mfma_step<PHID>(fa, fb, fl, f.o_l2 * 64 + (rb * HB1 * S2 + st) * 64 + lane, xb, a2[rb]);
     }
   float e = 0.0f;
 #pragma unroll
@@ -171,20 +175,21 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   for (int rb2 = 0; rb2 < HB2; rb2++)
 #pragma unroll
     for (int q = 0; q < S2; q++) {
-      float xb[KS];
+      float xb[KSH];
 #pragma unroll
-      for (int v = 0; v < KS; v++) xb[v] = a2[rb2][BF16 ? 8 * q + v : q];
+      for (int v = 0; v < KSH; v++) xb[v] = a2[rb2][BHID ? 8 * q + v : q];
       const int st = rb2 * S2 + q;
 #pragma unroll
       for (int rb = 0; rb < HB1; rb++)
-        d1[rb] = mfma_step<PREC>(fa, fb, fl, f.o_b1 * 64 + (rb * HB2 * S2 + st) * 64 + lane, xb, d1[rb]);
+        d1[rb] = mfma_step<PHID>(fa, fb, fl, f.o_b1 * 64 + (rb * HB2 * S2 + st) * 64 + lane, xb, d1[rb]);
     }
 #pragma unroll
   for (int rb = 0; rb < HB1; rb++)
 #pragma unroll
     for (int r = 0; r < 16; r++) d1[rb][r] *= 1.0f - a1[rb][r] * a1[rb][r];
 
-  // ---- dE/dG = W0'^T delta1, 32 features per block
+  // ---- dE/dG = W0'^T delta1, 32 features per block (input layer's precision)
+  constexpr int SG = BIN ? 2 : 16;
   float* out = atom >= 0 ? dEdG + (size_t)atom * D : nullptr;
   for (int rbd = 0; rbd < f.db; rbd++) {
     f32x16 gacc;
@@ -193,11 +198,11 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
 #pragma unroll
     for (int rb1 = 0; rb1 < HB1; rb1++)
 #pragma unroll
-      for (int q = 0; q < S2; q++) {
+      for (int q = 0; q < SG; q++) {
         float xb[KS];
 #pragma unroll
-        for (int v = 0; v < KS; v++) xb[v] = d1[rb1][BF16 ? 8 * q + v : q];
-        gacc = mfma_step<PREC>(fa, fb, fl, f.o_g0 * 64 + (rbd * HB1 * S2 + rb1 * S2 + q) * 64 + lane, xb, gacc);
+        for (int v = 0; v < KS; v++) xb[v] = d1[rb1][BIN ? 8 * q + v : q];
+        gacc = mfma_step<PIN>(fa, fb, fl, f.o_g0 * 64 + (rbd * HB1 * SG + rb1 * SG + q) * 64 + lane, xb, gacc);
       }
     if (out)
 #pragma unroll
@@ -208,33 +213,38 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   }
 }
 
+// prec: 0 FP32 throughout; 1 (MG_PREC_BF16) one-pass BF16 in the hidden layer (W1 h1 and W1^T delta2), FP32 input
+// layer; 2 (MG_PREC_BF16X3) compensated BF16 in every layer
 extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, const float* G, const float* frag,
                             const void* frag_bf, const void* frag_lo, const float* vecs, const int* fdesc, int prec,
                             float* Ei, float* dEdG, hipStream_t st) {
   if (ntiles == 0) return 0;
-  const int bf16 = prec != 0;
+  const bool bin = prec == 2, bhid = prec != 0;
   MlpFrag f;
   f.D = fdesc[0]; f.H1 = fdesc[1]; f.H2 = fdesc[2]; f.hb1 = fdesc[3]; f.hb2 = fdesc[4]; f.db = fdesc[5];
-  f.k1 = bf16 ? fdesc[6] : fdesc[7];
-  const int* o = bf16 ? fdesc + 8 : fdesc + 14;
-  f.o_l1 = o[0]; f.o_l2 = o[1]; f.o_b1 = o[2]; f.o_g0 = o[3]; f.stride_slots = o[4]; f.o_vec = 0;
-  f.stride_vec = o[5];
+  f.k1 = bin ? fdesc[6] : fdesc[7];
+  const int* oi = bin ? fdesc + 8 : fdesc + 14;   // input layer's set
+  const int* oh = bhid ? fdesc + 8 : fdesc + 14;  // hidden layer's set
+  f.o_l1 = oi[0]; f.o_g0 = oi[3];
+  f.o_l2 = oh[1]; f.o_b1 = oh[2];
+  f.sb = fdesc[12]; f.sf = fdesc[18];
+  f.stride_vec = fdesc[13];
   const int4* t = (const int4*)tiles;
   const __bf16* fb = (const __bf16*)frag_bf;
   const __bf16* fl = (const __bf16*)frag_lo;
-  const size_t sm = (size_t)32 * ((bf16 ? 16 : 2) * f.k1 + 1) * sizeof(float);  // staged inputs of a tile
+  const size_t sm = (size_t)32 * ((bin ? 16 : 2) * f.k1 + 1) * sizeof(float);  // staged inputs of a tile
   if (sm > 64 * 1024) return -2;
-#define MLP_L(P, H1B, H2B) hipLaunchKernelGGL((k_mlp_mfma<P, H1B, H2B>), dim3(ntiles), dim3(64), sm, st, t, perm, G, frag, fb, fl, vecs, f, Ei, dEdG)
-#define MLP_P(P)                                  \
-  do {                                            \
-    if (f.hb1 == 1 && f.hb2 == 1) MLP_L(P, 1, 1); \
-    else if (f.hb1 == 1) MLP_L(P, 1, 2);          \
-    else if (f.hb2 == 1) MLP_L(P, 2, 1);          \
-    else MLP_L(P, 2, 2);                          \
+#define MLP_L(PI, PH, H1B, H2B) hipLaunchKernelGGL((k_mlp_mfma<PI, PH, H1B, H2B>), dim3(ntiles), dim3(64), sm, st, t, perm, G, frag, fb, fl, vecs, f, Ei, dEdG)
+#define MLP_P(PI, PH)                                     \
+  do {                                                    \
+    if (f.hb1 == 1 && f.hb2 == 1) MLP_L(PI, PH, 1, 1);    \
+    else if (f.hb1 == 1) MLP_L(PI, PH, 1, 2);             \
+    else if (f.hb2 == 1) MLP_L(PI, PH, 2, 1);             \
+    else MLP_L(PI, PH, 2, 2);                             \
   } while (0)
-  if (prec == 2) MLP_P(2);
-  else if (prec == 1) MLP_P(1);
-  else MLP_P(0);
+  if (prec == 2) MLP_P(2, 2);
+  else if (prec == 1) MLP_P(0, 1);
+  else MLP_P(0, 0);
 #undef MLP_P
 #undef MLP_L
   return mgk_status();
