@@ -1,6 +1,7 @@
 """In-tree build of libmaise_gpu.so (hipcc for the gfx950 kernels, gcc for the C host code).
 
-Objects go to maise_amd/_build/, the shared library to maise_amd/libmaise_gpu.so.  Every call rebuilds.
+Objects go to maise_amd/_build/, the shared library to maise_amd/libmaise_gpu.so and the same objects as a static
+archive to maise_amd/libmaise_gpu.a.  Every call rebuilds.
 """
 from __future__ import annotations
 
@@ -13,6 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmaise_gpu.so")
+STATIC_LIB = os.path.join(HERE, "libmaise_gpu.a")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("MAISE_GPU_ARCH", "gfx950")
 
@@ -56,7 +58,35 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lm"], verbose)
     os.replace(tmp, LIB)
+    # the same objects as a static archive, for link lines like the reference's (libmaise.a -lm; add the HIP
+    # runtime: -L/opt/rocm/lib -lamdhip64 -lstdc++)
+    tmpa = STATIC_LIB + ".tmp"
+    if os.path.exists(tmpa):
+        os.remove(tmpa)
+    _run(["ar", "rcs", tmpa, *objs], verbose)
+    os.replace(tmpa, STATIC_LIB)
     return LIB
+
+
+REF_INTERFACE = os.environ.get("MAISE_REFERENCE", "/root/reference") + "/interface"
+CALLER_BIN = os.path.join(ROOT, "tests", "drivers", "_bin")
+
+
+def build_reference_callers(verbose: bool = False) -> list:
+    """Compile the reference's own CALL_MAISE callers (interface/c/c2maise.c, interface/cpp/cpp2maise.cpp), unchanged
+    from the reference tree, against include/mlib.h and link them to the static library with the reference's link
+    line plus the HIP runtime; the executables go to tests/drivers/_bin/ (skipped when the tree is absent)."""
+    srcs = [os.path.join(REF_INTERFACE, "c", "c2maise.c"), os.path.join(REF_INTERFACE, "cpp", "cpp2maise.cpp")]
+    if not all(os.path.exists(p) for p in srcs):
+        return []
+    os.makedirs(CALLER_BIN, exist_ok=True)
+    out = []
+    for src in srcs:
+        exe = os.path.join(CALLER_BIN, os.path.splitext(os.path.basename(src))[0])
+        _run(["gcc", "-o", exe, src, "-I", os.path.join(ROOT, "include"), f"-I{ROCM}/include", STATIC_LIB,
+              f"-L{ROCM}/lib", "-lamdhip64", "-lstdc++", "-lm", "-fopenmp", f"-Wl,-rpath,{ROCM}/lib"], verbose)
+        out.append(exe)
+    return out
 
 
 def build_oracle(verbose: bool = False) -> None:

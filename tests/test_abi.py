@@ -79,3 +79,23 @@ def test_static_eval_rejects_oversize_before_any_gpu_work():
     N = 32769
     with pytest.raises(maise_amd.MaiseError, match="fixed-scratch limit"):
         maise_amd.eval_static(m, np.zeros(N, np.int32), np.eye(3) * 100.0, np.zeros((N, 3)))
+
+
+REF_IFACE = os.path.join(os.environ.get("MAISE_REFERENCE", "/root/reference"), "interface")
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_IFACE), reason="reference tree not mounted")
+@pytest.mark.parametrize("lib_kind", ["static", "shared"])
+def test_reference_callers_compile_unchanged(tmp_path, lib_kind):
+    """The reference's own callers, interface/c/c2maise.c and interface/cpp/cpp2maise.cpp, read from the reference
+    tree and compiled unchanged against include/mlib.h, link against libmaise_gpu.a with the reference's link line
+    (libmaise.a -lm -fopenmp, interface/c/makefile:6) plus the HIP runtime, and against libmaise_gpu.so."""
+    lib = os.path.join(ROOT, "maise_amd", "libmaise_gpu.a" if lib_kind == "static" else "libmaise_gpu.so")
+    assert os.path.exists(lib)
+    for src in (os.path.join(REF_IFACE, "c", "c2maise.c"), os.path.join(REF_IFACE, "cpp", "cpp2maise.cpp")):
+        exe = tmp_path / (os.path.basename(src) + "." + lib_kind)
+        cmd = ["gcc", "-o", str(exe), src, "-I", INC, "-I/opt/rocm/include", lib, "-L/opt/rocm/lib", "-lamdhip64",
+               "-lstdc++", "-lm", "-fopenmp", "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath," + os.path.dirname(lib)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        assert exe.exists()

@@ -124,6 +124,31 @@ def test_call_maise_dropin(ref_cases, tmp_path):
         np.testing.assert_allclose(S, c["U"], atol=U_TOL * N)
 
 
+@pytest.mark.parametrize("exe", ["c2maise", "cpp2maise"])
+def test_reference_callers_run_unchanged(ref_cases, tmp_path, exe):
+    """The reference's own interface drivers (interface/c/c2maise.c, interface/cpp/cpp2maise.cpp), compiled unchanged
+    against include/mlib.h and linked to libmaise_gpu.a by build() where the reference tree is present, run on the
+    GPU with the interface's model (a copy: tests/golden/models/nn_CuAg_d3_v0_iface) in their working directory and
+    print the forces and enthalpy of the reference's interface/c/test/OUTCAR (4-atom Cu cluster)."""
+    path = os.path.join(ROOT, "tests", "drivers", "_bin", exe)
+    if not os.path.exists(path):
+        pytest.skip("reference callers not built (the reference tree was absent at build time)")
+    c = ref_cases["iface"]
+    with open(os.path.join(GOLDEN, "models", "nn_CuAg_d3_v0_iface")) as src, open(tmp_path / "model", "w") as dst:
+        dst.write(src.read())
+    print(f"[gpu case] reference caller {exe}", flush=True)
+    env = {k: v for k, v in os.environ.items() if k not in ("MAISE_MODEL", "MAISE_UREP")}
+    out = subprocess.run([path], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    F = np.array([[float(v) for v in l.split()[1:4]] for l in out.stdout.splitlines()
+                  if len(l.split()) == 4 and l.split()[0].isdigit()])
+    H = float(next(l.split()[1] for l in out.stdout.splitlines() if l.startswith("Enthalpy")))
+    print(out.stdout)
+    assert F.shape == (4, 3)
+    assert H == pytest.approx(float(c["H"]), abs=E_TOL * 4)
+    np.testing.assert_allclose(F, c["F"], atol=F_TOL)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("name", ["cuag", "mgca", "cupdag_tri", "relax", "species_generic", "species_std", "config1"])
 def test_stored_pairs_match_enumeration(ref_cases, name, prec):
