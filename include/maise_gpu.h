@@ -168,6 +168,20 @@ int mg_evo_offspring(int npar, int N, int nspc, const int *spc, const double *la
                      int max_tries, int *out_spc, double *out_lat, double *out_x, int *failed);
 /* smallest periodic interatomic distance of one structure (lattice rows L[9], Cartesian x[3n]) */
 double mg_evo_min_distance(const double *L, const double *x, int n);
+/* Mutation after BULK_MUTE with SHKE_CL (src/efnc.c:2316-2386, src/eutl.c:176-218): a parent drawn by weight, its
+ * lattice strained (diagonal 1 + cl g, symmetric off-diagonal 0.5 cl g, g Gaussian; fractional coordinates kept),
+ * every coordinate moved by a Gaussian of ca A, the rows reduced (SHRT_LV), the volume within 50% of the parent's,
+ * then each atom's position swapped with probability cs with that of an atom of another species, and all periodic
+ * distances >= dmin.  Layout and return value as mg_evo_offspring. */
+int mg_evo_mutate(int npar, int N, int nspc, const int *spc, const double *lat, const double *x, const double *weights,
+                  int count, double cl, double ca, double cs, double dmin, unsigned long long seed, int max_tries,
+                  int *out_spc, double *out_lat, double *out_x, int *failed);
+/* Structure fingerprints after RDF(C, 1) (src/cell.c:369-415) for S periodic structures (atom_off[S+1], lat[9S],
+ * Cartesian x): Gaussian-broadened (sigma) radial distributions of every species pair m <= n on nrdf bins up to rmax,
+ * tapered by a cosine from rmin to rmax, same-species pairs halved, scaled to unit length; out[S][npair * nrdf] floats.
+ * The dot product of two fingerprints is CxC (src/cell.c:417-427), the structural part of COMP_CL. */
+int mg_evo_fingerprint(int S, const int *atom_off, int nspc, const int *spc, const double *lat, const double *x,
+                       int nrdf, double rmax, double rmin, double sigma, float *out);
 
 /* --- device-resident dynamics (state stays on the device between steps) --- */
 /* NVT molecular dynamics, replacing Dynamics/Maxwell (src/cmod.c:224-549): velocity Verlet with a Nose-Hoover

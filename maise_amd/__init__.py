@@ -90,6 +90,10 @@ def lib():
             "mg_evo_offspring": ([C.c_int, C.c_int, C.c_int, ip, dp, dp, dp, C.c_int, C.c_double, C.c_double,
                                   C.c_ulonglong, C.c_int, ip, dp, dp, ip], C.c_int),
             "mg_evo_min_distance": ([dp, dp, C.c_int], C.c_double),
+            "mg_evo_mutate": ([C.c_int, C.c_int, C.c_int, ip, dp, dp, dp, C.c_int, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_ulonglong, C.c_int, ip, dp, dp, ip], C.c_int),
+            "mg_evo_fingerprint": ([C.c_int, ip, C.c_int, ip, dp, dp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    fp], C.c_int),
             "mg_poscar_read": ([C.c_char_p, C.c_int, C.c_int, C.c_char_p, dp, ip, ip, ip, ip, dp, ip, dp, ip],
                                C.c_int),
             "mg_poscar_write": ([C.c_char_p, C.c_char_p, C.c_double, dp, C.c_int, C.c_int, ip, ip, C.c_int, dp, ip,

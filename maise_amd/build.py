@@ -31,6 +31,8 @@ HIP_SOURCES = ["mg_kernels.hip", "mg_mlp.hip", "mg_dyn.hip", "mg_sur.hip", "mg_s
 # kernels in mg_mlp.hip measured faster with the AGPR form
 HIP_EXTRA = {"mg_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 C_SOURCES = ["mg_model.c", "mg_api.c", "mg_dyn.c", "mg_sur.c", "mg_evo.c", "mg_io.c"]
+# the search's host operators spread structures over the host cores (OpenMP; libgomp is linked into the library)
+C_EXTRA = {"mg_evo.c": ["-fopenmp"]}
 
 
 def _run(cmd, verbose):
@@ -53,10 +55,10 @@ def build(verbose: bool = False) -> str:
         objs.append(obj)
     for src in C_SOURCES:
         obj = os.path.join(BUILD, src.replace(".c", ".o"))
-        _run(["gcc", *CC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        _run(["gcc", *CC_FLAGS, *C_EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lm"], verbose)
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lgomp", "-lm"], verbose)
     os.replace(tmp, LIB)
     # the same objects as a static archive, for link lines like the reference's (libmaise.a -lm; add the HIP
     # runtime: -L/opt/rocm/lib -lamdhip64 -lstdc++)

@@ -233,3 +233,197 @@ int mg_evo_offspring(int npar, int N, int nspc, const int *spc, const double *la
   free(side);
   return nfail;
 }
+
+/* Gaussian deviate by Marsaglia's polar method (the reference's RANG, src/util.c:169-185, draws the same
+ * distribution from its own generator) */
+static double grand(unsigned long long *s) {
+  double x1, x2, w;
+  do {
+    x1 = 2.0 * urand(s) - 1.0;
+    x2 = 2.0 * urand(s) - 1.0;
+    w = x1 * x1 + x2 * x2;
+  } while (w >= 1.0 || w == 0.0);
+  return x1 * sqrt(-2.0 * log(w) / w);
+}
+
+/* lattice reduction in the manner of SHRT_LV (src/cell.c:514-545): while a row has a projection onto another row of
+ * more than half that row's squared length, subtract the nearest integer multiple of the other row; atoms keep their
+ * Cartesian positions (the lattice still spans the same periodic crystal).  Returns 0 if a row is shorter than lmin. */
+static int reduce_rows(double *L, double lmin) {
+  for (int it = 0; it < 100; it++) {
+    int changed = 0;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        if (a == b) continue;
+        const double *u = L + 3 * b;
+        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double pu = L[3 * a] * u[0] + L[3 * a + 1] * u[1] + L[3 * a + 2] * u[2];
+        if (fabs(pu) > 0.5 * uu * (1.0 + 1e-12)) {
+          const double m = nearbyint(pu / uu);
+          for (int q = 0; q < 3; q++) L[3 * a + q] -= m * u[q];
+          changed = 1;
+        }
+      }
+    if (!changed) break;
+  }
+  for (int a = 0; a < 3; a++)
+    if (sqrt(L[3 * a] * L[3 * a] + L[3 * a + 1] * L[3 * a + 1] + L[3 * a + 2] * L[3 * a + 2]) < lmin) return 0;
+  return 1;
+}
+
+/* count mutants of N atoms each, after BULK_MUTE (src/efnc.c:2316-2386) with SHKE_CL (src/eutl.c:176-218): a parent
+ * drawn with the given weights (roulette), its lattice strained by e = I + cl g (diagonal) and 0.5 cl g (symmetric
+ * off-diagonal), g standard Gaussian, with the fractional coordinates kept; every coordinate then moved by a
+ * Gaussian of ca A; the rows reduced (SHRT_LV) and no row shorter than 1 A; the volume kept within a factor 1.5 of the
+ * parent's; then, with more than one species, each atom's position swapped with probability cs with that of a random
+ * atom of another species (species are grouped); finally every periodic distance must be >= dmin.  A slot whose tries
+ * all fail is flagged in failed[k].  Layout as mg_evo_offspring.  Returns the number of failed slots, or < 0. */
+int mg_evo_mutate(int npar, int N, int nspc, const int *spc, const double *lat, const double *x, const double *weights,
+                  int count, double cl, double ca, double cs, double dmin, unsigned long long seed, int max_tries,
+                  int *out_spc, double *out_lat, double *out_x, int *failed) {
+  if (npar < 1 || N < 1 || nspc < 1 || nspc > MG_MAXSPC || count < 0 || !spc || !lat || !x || !weights || !out_spc ||
+      !out_lat || !out_x || !failed || cl < 0.0 || ca < 0.0 || cs < 0.0 || cs > 1.0)
+    return mg_fail(MG_ERR_ARG, "mg_evo_mutate: bad argument");
+  for (int i = 0; i < npar * N; i++)
+    if (spc[i] < 0 || spc[i] >= nspc) return mg_fail(MG_ERR_ARG, "mg_evo_mutate: species out of range");
+  double *cum = (double *)malloc(sizeof(double) * npar);
+  double tot = 0.0;
+  for (int p = 0; p < npar; p++) {
+    tot += weights[p] > 0.0 ? weights[p] : 0.0;
+    cum[p] = tot;
+  }
+  if (!(tot > 0.0)) { free(cum); return mg_fail(MG_ERR_ARG, "mg_evo_mutate: no parent has positive weight"); }
+  double *f = (double *)malloc(sizeof(double) * 3 * N);
+  int nfail = 0;
+  for (int k = 0; k < count; k++) {
+    unsigned long long rs = seed ^ (0x9E6C63D0676A9A99ull * (unsigned long long)(k + 1));
+    int ok = 0;
+    int *cs_ = out_spc + (size_t)k * N;
+    double *cL = out_lat + 9 * (size_t)k, *cx = out_x + 3 * (size_t)k * N;
+    for (int t = 0; t < max_tries && !ok; t++) {
+      int p;
+      const double r = urand(&rs) * tot;
+      for (p = 0; p < npar - 1 && (cum[p] <= r || !(weights[p] > 0.0)); p++) ;
+      const int *ps = spc + (size_t)p * N;
+      const double *pL = lat + 9 * (size_t)p, *px = x + 3 * (size_t)p * N;
+      memcpy(cs_, ps, sizeof(int) * N);
+      to_frac(pL, px, N, f);
+      /* strain e (symmetric), rows L'_i = sum_k e_ik L_k, fractional coordinates kept */
+      double e[9];
+      e[0] = 1.0 + cl * grand(&rs); e[4] = 1.0 + cl * grand(&rs); e[8] = 1.0 + cl * grand(&rs);
+      e[1] = e[3] = 0.5 * cl * grand(&rs);
+      e[2] = e[6] = 0.5 * cl * grand(&rs);
+      e[5] = e[7] = 0.5 * cl * grand(&rs);
+      for (int i = 0; i < 3; i++)
+        for (int q = 0; q < 3; q++) cL[3 * i + q] = e[3 * i] * pL[q] + e[3 * i + 1] * pL[3 + q] + e[3 * i + 2] * pL[6 + q];
+      for (int i = 0; i < N; i++)
+        for (int q = 0; q < 3; q++)
+          cx[3 * i + q] = f[3 * i] * cL[q] + f[3 * i + 1] * cL[3 + q] + f[3 * i + 2] * cL[6 + q] + ca * grand(&rs);
+      if (!reduce_rows(cL, 1.0)) continue;
+      const double V0 = fabs(det3(pL)), V1 = fabs(det3(cL));
+      if (!(V0 > 0.0) || fabs(V1 / V0 - 1.0) > 0.5) continue;
+      if (nspc > 1)
+        for (int i = 0; i < N; i++)
+          if (urand(&rs) < cs) {
+            int n_i = 0, j;
+            for (int a = 0; a < N; a++) n_i += cs_[a] == cs_[i];
+            if (n_i >= N) continue;
+            for (j = i; cs_[j % N] == cs_[i]; j++) ;  /* first atom after i's species block */
+            j = (j + (int)(urand(&rs) * (double)(N - n_i))) % N;
+            for (int q = 0; q < 3; q++) {
+              const double tmp = cx[3 * i + q];
+              cx[3 * i + q] = cx[3 * j + q];
+              cx[3 * j + q] = tmp;
+            }
+          }
+      ok = mg_evo_min_distance(cL, cx, N) >= dmin;
+    }
+    failed[k] = !ok;
+    nfail += !ok;
+  }
+  free(cum);
+  free(f);
+  return nfail;
+}
+
+/* Structure fingerprints after RDF(C, 1) (src/cell.c:369-415): per structure s (atoms atom_off[s]..atom_off[s+1],
+ * lattice rows lat[9s], Cartesian x, nd = 3), for every ordered species pair m <= n a radial distribution on nrdf bins
+ * of width dr = rmax / nrdf: each pair (i, j image) with spc[j] >= spc[i] at distance r adds
+ * f(x_k) exp(-(r - x_k)^2 / (2 sigma^2)) to the bins x_k = k dr within 3 sigma of r, f = 1 below rmin and
+ * cos(pi/2 (x - rmin)/(rmax - rmin)) above; same-species bins are halved; the whole fingerprint is scaled to unit
+ * length.  out holds nspc(nspc+1)/2 * nrdf floats per structure, pair-major (pairs in the order (0,0), (0,1), ...,
+ * (1,1), ...).  The dot product of two fingerprints is CxC (src/cell.c:417-427). */
+int mg_evo_fingerprint(int S, const int *atom_off, int nspc, const int *spc, const double *lat, const double *x,
+                       int nrdf, double rmax, double rmin, double sigma, float *out) {
+  if (S < 0 || !atom_off || nspc < 1 || nspc > MG_MAXSPC || !spc || !lat || !x || nrdf < 1 || !(rmax > 0.0) ||
+      !(sigma > 0.0) || !(rmin < rmax) || !out)
+    return mg_fail(MG_ERR_ARG, "mg_evo_fingerprint: bad argument");
+  const int npair = nspc * (nspc + 1) / 2;
+  int pidx[MG_MAXSPC][MG_MAXSPC];
+  for (int m = 0, p = 0; m < nspc; m++)
+    for (int n = m; n < nspc; n++) pidx[m][n] = pidx[n][m] = p++;
+  const double dr = rmax / (double)nrdf;
+  const int Do = (int)(3.0 * sigma / dr);
+  const double DR = 0.5 / (sigma * sigma);
+  const double rcut = rmax + 3.0 * sigma + dr;
+  const double cstep = exp(-2.0 * DR * dr * dr);
+  /* structures in parallel on the host cores, each with its own accumulator */
+#pragma omp parallel
+  {
+  double *acc = (double *)malloc(sizeof(double) * (size_t)npair * nrdf);
+#pragma omp for schedule(dynamic, 4)
+  for (int s = 0; s < S; s++) {
+    const int a0 = atom_off[s], n = atom_off[s + 1] - a0;
+    const double *L = lat + 9 * (size_t)s, *X = x + 3 * (size_t)a0;
+    const int *sp = spc + a0;
+    memset(acc, 0, sizeof(double) * (size_t)npair * nrdf);
+    double I[9];
+    inv3(L, I);
+    int K[3];
+    for (int q = 0; q < 3; q++) {  /* images needed: rcut times the length of reciprocal row q (columns of I) */
+      const double rq = sqrt(I[q] * I[q] + I[3 + q] * I[3 + q] + I[6 + q] * I[6 + q]);
+      K[q] = (int)ceil(rcut * rq) + 1;
+    }
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        if (sp[j] < sp[i]) continue;
+        const int p = pidx[sp[i]][sp[j]];
+        for (int a = -K[0]; a <= K[0]; a++)
+          for (int b = -K[1]; b <= K[1]; b++)
+            for (int c = -K[2]; c <= K[2]; c++) {
+              if (i == j && a == 0 && b == 0 && c == 0) continue;
+              double d[3], r2 = 0.0;
+              for (int q = 0; q < 3; q++) {
+                d[q] = X[3 * j + q] - X[3 * i + q] + a * L[q] + b * L[3 + q] + c * L[6 + q];
+                r2 += d[q] * d[q];
+              }
+              if (r2 >= rcut * rcut) continue;
+              const double r = sqrt(r2);
+              const int o = (int)(r / dr);
+              /* exp(-DR (r - x_k)^2) along the bins by recurrence: g_{k+1} = g_k q_k, q_{k+1} = q_k c */
+              const int k0 = o - Do;
+              double xk = (double)k0 * dr;
+              double g = exp(-(r - xk) * (r - xk) * DR), qk = exp(2.0 * DR * dr * (r - xk) - DR * dr * dr);
+              for (int k = k0; k <= o + Do; k++) {
+                if (k >= 0 && k < nrdf) {
+                  xk = (double)k * dr;
+                  const double fk = xk < rmin ? 1.0 : cos(0.5 * 3.14159265358979323846 * (xk - rmin) / (rmax - rmin));
+                  acc[(size_t)p * nrdf + k] += fk * g;
+                }
+                g *= qk;
+                qk *= cstep;
+              }
+            }
+      }
+    for (int m = 0; m < nspc; m++)
+      for (int k = 0; k < nrdf; k++) acc[(size_t)pidx[m][m] * nrdf + k] *= 0.5;
+    double nrm = 0.0;
+    for (size_t q = 0; q < (size_t)npair * nrdf; q++) nrm += acc[q] * acc[q];
+    nrm = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+    float *o = out + (size_t)s * npair * nrdf;
+    for (size_t q = 0; q < (size_t)npair * nrdf; q++) o[q] = (float)(acc[q] * nrm);
+  }
+  free(acc);
+  }
+  return MG_OK;
+}

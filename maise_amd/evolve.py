@@ -8,9 +8,14 @@ The generation loop of the reference's EVOS caller (src/emod.c:484-757) with its
              src/eutl.c:139-172: the second parent's lattice, cyclically permuted to best match the first, with
              probability equal to the first parent's share of atoms) and is rejected when its volume differs by
              10% or more from the parents' mean;
-  mutation   Gaussian displacement of every atom (sigma in A), then the minimum-distance check;
-  selection  the population is ranked by relaxed energy per atom (RANK_TR, src/emod.c:304-444), near-duplicate
-             energies are dropped, and parents are drawn with weights that fall off with rank.
+  mutation   Gaussian displacement of every atom (sigma in A) of each crossover child, then the minimum-distance
+             check; and, for a `mute_frac` share of the offspring, BULK_MUTE's operator (src/efnc.c:2316-2386):
+             a parent drawn by weight, distorted by SHKE_CL (src/eutl.c:176-218: random symmetric lattice strain,
+             atomic displacements), reduced (SHRT_LV), with species swapped at rate `mute_cs`;
+  selection  the population is ranked by relaxed energy per atom (RANK_TR, src/emod.c:304-444); a structure whose
+             COMP_CL similarity (src/eutl.c:394-413: 0.8 CxC of the RDF fingerprints of src/cell.c:369-427 + 0.1 of
+             an energy and 0.1 of a volume closeness) to a lower-energy member exceeds `dup_cut` is a duplicate and
+             dropped, and parents are drawn with weights that fall off with rank.
 All structures are relaxed on the GPU in one batch (mg_relax, positions only, cell fixed).
 
 Surrogate screening: each generation's offspring are evaluated once unrelaxed (one batched descriptor pass),
@@ -29,7 +34,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from . import Context, Model, _check, _dp, _ip, lib
+from . import Context, Model, _check, _dp, _fp, _ip, lib
 from .surrogate import Surrogate
 
 Structure = Tuple[np.ndarray, np.ndarray, np.ndarray]  # (species[N], lattice rows[3,3], Cartesian x[N,3])
@@ -53,6 +58,16 @@ class GAConfig:
     batch: int = 64
     lr: float = 1e-3
     audit: bool = True                         # also relax every offspring to measure recall and error
+    mute_frac: float = 0.0                     # share of offspring made by BULK_MUTE-style mutation (rest: crossover)
+    mute_cl: float = 0.15                      # lattice distortion of the mutation (LDST)
+    mute_ca: float = 0.15                      # atomic displacement of the mutation, A (ADST)
+    mute_cs: float = 0.10                      # species swap rate of the mutation (SDST)
+    dup_cut: float = 0.95                      # COMP_CL above this marks a duplicate (SCUT); >= 1 disables the test
+    fp_nrdf: int = 3000                        # fingerprint: RDF bins (NRDF), up to fp_rmax (RMAX) with a cosine
+    fp_rmax: float = 6.0                       # taper from fp_rmin (RMIN), Gaussian spread fp_sigma (DELR), A
+    fp_rmin: float = 5.8
+    fp_sigma: float = 0.008
+    precision: int = 0                         # MLP precision of the relaxations and evaluations (MG_PREC_*)
 
 
 @dataclass
@@ -145,6 +160,115 @@ def make_offspring(parents: List[Structure], weights: np.ndarray, cfg: GAConfig,
     return out
 
 
+def mutate(parents: List[Structure], weights: np.ndarray, cfg: GAConfig, rng: np.random.Generator, count: int,
+           max_tries: int = 500) -> List[Structure]:
+    """count mutants (native mg_evo_mutate, after BULK_MUTE + SHKE_CL); a slot whose tries all fail gets a fresh
+    random structure."""
+    N = len(parents[0][0])
+    spc = np.ascontiguousarray(np.concatenate([p[0] for p in parents]), dtype=np.int32)
+    lat = np.ascontiguousarray(np.concatenate([np.asarray(p[1]).reshape(9) for p in parents]), dtype=np.float64)
+    x = np.ascontiguousarray(np.concatenate([np.asarray(p[2]).reshape(-1) for p in parents]), dtype=np.float64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    os_ = np.zeros(count * N, dtype=np.int32)
+    ol = np.zeros(count * 9)
+    ox = np.zeros(count * 3 * N)
+    failed = np.zeros(count, dtype=np.int32)
+    rc = lib().mg_evo_mutate(len(parents), N, len(cfg.counts), _ip(spc), _dp(lat), _dp(x), _dp(w), count,
+                             float(cfg.mute_cl), float(cfg.mute_ca), float(cfg.mute_cs), float(cfg.dmin),
+                             int(rng.integers(1 << 62)), int(max_tries), _ip(os_), _dp(ol), _dp(ox), _ip(failed))
+    if rc < 0:
+        _check(rc, "mg_evo_mutate")
+    out: List[Structure] = []
+    for k in range(count):
+        if failed[k]:
+            out.append(random_structure(cfg, rng))
+        else:
+            out.append((os_[k * N:(k + 1) * N].copy(), ol[9 * k:9 * k + 9].reshape(3, 3).copy(),
+                        ox[3 * N * k:3 * N * (k + 1)].reshape(N, 3).copy()))
+    return out
+
+
+def fingerprints(structs: List[Structure], nspc: int, cfg: Optional[GAConfig] = None) -> np.ndarray:
+    """Unit-length RDF fingerprints [S, nspc(nspc+1)/2 * nrdf] (native mg_evo_fingerprint, after RDF(C, 1))."""
+    cfg = cfg or GAConfig()
+    S = len(structs)
+    npair = nspc * (nspc + 1) // 2
+    off = np.zeros(S + 1, dtype=np.int32)
+    off[1:] = np.cumsum([len(s[0]) for s in structs])
+    spc = np.ascontiguousarray(np.concatenate([s[0] for s in structs]), dtype=np.int32)
+    lat = np.ascontiguousarray(np.concatenate([np.asarray(s[1]).reshape(9) for s in structs]), dtype=np.float64)
+    x = np.ascontiguousarray(np.concatenate([np.asarray(s[2]).reshape(-1) for s in structs]), dtype=np.float64)
+    out = np.zeros((S, npair * cfg.fp_nrdf), dtype=np.float32)
+    _check(lib().mg_evo_fingerprint(S, _ip(off), nspc, _ip(spc), _dp(lat), _dp(x), int(cfg.fp_nrdf),
+                                    float(cfg.fp_rmax), float(cfg.fp_rmin), float(cfg.fp_sigma), _fp(out)),
+           "mg_evo_fingerprint")
+    return out
+
+
+def cell_volumes(structs: List[Structure]) -> np.ndarray:
+    return np.array([abs(np.linalg.det(np.asarray(s[1]).reshape(3, 3))) for s in structs])
+
+
+def comp_cl(fa: np.ndarray, fb: np.ndarray, ea, eb, va, vb) -> np.ndarray:
+    """COMP_CL (src/eutl.c:394-413) of structure a against the structures b: 0.8 CxC (fingerprint dot product)
+    + 0.1 exp(-dE^2 / (2 0.05^2)) + 0.1 exp(-dV^2 / (2 0.01^2)), dV = (V_a - V_b) / (2 (V_a + V_b)); at most 1."""
+    fb = np.atleast_2d(fb)
+    cxc = fb @ fa
+    de = np.asarray(eb, dtype=np.float64) - float(ea)
+    dv = 0.5 * (float(va) - np.asarray(vb, dtype=np.float64)) / (float(va) + np.asarray(vb, dtype=np.float64))
+    return 0.8 * cxc + 0.1 * np.exp(-0.5 * de * de / 0.05 ** 2) + 0.1 * np.exp(-0.5 * dv * dv / 0.01 ** 2)
+
+
+# COMP_CL = 0.8 CxC + 0.1 e_E + 0.1 e_V with CxC, e_E, e_V <= 1 exceeds 0.95 only if e_E > 0.05 and e_V > 0.05, i.e.
+# |dE| < 0.05 sqrt(2 ln 20) eV/atom and |dV| < 0.01 sqrt(2 ln 20): the only pairs whose fingerprints need comparing
+_DUP_DE = 0.05 * math.sqrt(2.0 * math.log(20.0))
+_DUP_DV = 0.01 * math.sqrt(2.0 * math.log(20.0))
+
+
+def unique_by_fingerprint(E: np.ndarray, fp: np.ndarray, V: np.ndarray, cut: float, limit: int,
+                          n_unique: int = 0) -> List[int]:
+    """Indices of up to `limit` structures in increasing energy, skipping every structure whose COMP_CL with one
+    already kept exceeds `cut` (RANK_TR's duplicate test, src/emod.c:351-366, which divides such a structure's
+    fitness by ~1000; here it leaves the population).  Infinite energies are dropped.  The first `n_unique`
+    structures are known to be no duplicates of each other (the previous population), so only pairs with at least one
+    later structure are compared: their fingerprint products come from one matrix product, and only pairs inside
+    the energy and volume windows where COMP_CL can exceed 0.95 are scored (exact for cut >= 0.95)."""
+    order = np.argsort(E, kind="stable")
+    test = cut < 1.0 and len(E) > n_unique
+    if test:
+        G = fp[n_unique:] @ fp.T          # [new, all] fingerprint products (CxC)
+    de_win = _DUP_DE if cut >= 0.95 else np.inf
+    dv_win = _DUP_DV if cut >= 0.95 else np.inf
+    keep: List[int] = []
+    kidx = np.zeros(len(E), dtype=np.int64)
+    kE = np.zeros(len(E))
+    kV = np.zeros(len(E))
+    for k in order:
+        if not np.isfinite(E[k]):
+            continue
+        n = len(keep)
+        if test and n:
+            cand = kidx[:n]
+            m = (E[k] - kE[:n] <= de_win) & (np.abs(0.5 * (V[k] - kV[:n]) / (V[k] + kV[:n])) <= dv_win)
+            if k < n_unique:
+                m &= cand >= n_unique       # an old member is only compared with new structures
+            near = cand[m]
+            if len(near):
+                cxc = G[k - n_unique, near] if k >= n_unique else G[near - n_unique, k]
+                de = E[near] - E[k]
+                dv = 0.5 * (V[k] - V[near]) / (V[k] + V[near])
+                w = 0.8 * cxc + 0.1 * np.exp(-0.5 * de * de / 0.05 ** 2) + 0.1 * np.exp(-0.5 * dv * dv / 0.01 ** 2)
+                if w.max() > cut:
+                    continue
+        kidx[n] = k
+        kE[n] = E[k]
+        kV[n] = V[k]
+        keep.append(int(k))
+        if len(keep) == limit:
+            break
+    return keep
+
+
 # ------------------------------------------------------------------ the search
 class SurrogateGA:
     def __init__(self, model_path: str, cfg: GAConfig, device: int = 0):
@@ -157,8 +281,8 @@ class SurrogateGA:
         self.trained = False
         self.X: List[np.ndarray] = []
         self.y: List[np.ndarray] = []
-        self._ctx_eval = Context(self.model, device=device, urep=cfg.urep)
-        self._ctx_relax = Context(self.model, device=device, urep=cfg.urep)
+        self._ctx_eval = Context(self.model, device=device, urep=cfg.urep, precision=cfg.precision)
+        self._ctx_relax = Context(self.model, device=device, urep=cfg.urep, precision=cfg.precision)
 
     def close(self):
         self.sur.close()
@@ -207,22 +331,18 @@ class SurrogateGA:
         extra = self.rng.choice(rest, size=min(nrand, len(rest)), replace=False) if nrand else np.array([], int)
         return np.sort(np.concatenate([top, extra]).astype(int))
 
-    def update_population(self, pop: List[Structure], popE: np.ndarray, new: List[Structure], newE: np.ndarray,
-                          tol: float = 1e-4):
+    def update_population(self, pop: List[Structure], popE: np.ndarray, popFP: Optional[np.ndarray],
+                          new: List[Structure], newE: np.ndarray):
+        """Keep the cfg.pop lowest-energy structures of pop + new that are not duplicates (COMP_CL > dup_cut) of a
+        lower-energy one; returns (structures, energies, fingerprints)."""
         allS = pop + new
         allE = np.concatenate([popE, newE])
-        order = np.argsort(allE, kind="stable")
-        keep, last = [], None
-        for k in order:
-            if not np.isfinite(allE[k]):
-                continue
-            if last is not None and abs(allE[k] - last) < tol:
-                continue                           # near-duplicate energy: treated as the same structure
-            keep.append(k)
-            last = allE[k]
-            if len(keep) == self.cfg.pop:
-                break
-        return [allS[k] for k in keep], allE[keep]
+        newFP = fingerprints(new, len(self.cfg.counts), self.cfg) if new else None
+        parts = [f for f in (popFP, newFP) if f is not None and len(f)]
+        allFP = np.concatenate(parts) if parts else np.zeros((0, 1), np.float32)
+        keep = unique_by_fingerprint(allE, allFP, cell_volumes(allS), self.cfg.dup_cut, self.cfg.pop,
+                                     n_unique=len(pop))
+        return [allS[k] for k in keep], allE[keep], allFP[keep]
 
     def run(self, log=None) -> GAResult:
         cfg = self.cfg
@@ -235,7 +355,7 @@ class SurrogateGA:
         self.X.append(feat)
         self.y.append(epa.astype(np.float32))
         loss = self.train()
-        pop, popE = self.update_population([], np.zeros(0), relaxed, np.where(done, epa, np.inf))
+        pop, popE, popFP = self.update_population([], np.zeros(0), None, relaxed, np.where(done, epa, np.inf))
         res.reports.append(GenerationReport(0, len(init), len(init), time.perf_counter() - t0,
                                             best_eV_atom=float(popE[0]), train_loss=loss))
         if log:
@@ -243,7 +363,10 @@ class SurrogateGA:
         for g in range(1, cfg.generations + 1):
             ranks = np.arange(len(pop))
             weights = np.exp(-ranks / max(1.0, 0.25 * len(pop)))
-            kids = make_offspring(pop, weights, cfg, self.rng, cfg.pop)
+            nm = int(round(cfg.mute_frac * cfg.pop))
+            kids = make_offspring(pop, weights, cfg, self.rng, cfg.pop - nm)
+            if nm:
+                kids += mutate(pop, weights, cfg, self.rng, nm)
             t0 = time.perf_counter()
             feat = self.features(kids)
             pred = self.sur.predict(feat)
@@ -267,7 +390,7 @@ class SurrogateGA:
                     k = max(1, int(round(frac_k * len(kids))))
                     setattr(rep, name, len(sset.intersection(order[:k].tolist())) / k)
                 rep.mae_eV_atom = float(np.mean(np.abs(pred.astype(np.float64) - epa_all)))
-            pop, popE = self.update_population(pop, popE, relaxed, np.where(done, epa, np.inf))
+            pop, popE, popFP = self.update_population(pop, popE, popFP, relaxed, np.where(done, epa, np.inf))
             rep.best_eV_atom = float(popE[0])
             res.reports.append(rep)
             if log:
