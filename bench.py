@@ -229,10 +229,12 @@ GA_POT = os.path.join(ROOT, "tests", "golden", "models", "nn_MgCa_d3_v0")
 
 def ga_leg(maise_amd, args):
     """Surrogate-screened evolutionary search, Mg16Ca8 (24 atoms), population 512, crossover + Gaussian mutation
-    sigma 0.3 A, seeded.  Every generation's offspring are also relaxed in full (audit) for recall and error; a
-    sample of offspring relaxed one structure per context gives the plain-path rate."""
-    from maise_amd.evolve import GAConfig, SurrogateGA
-    cfg = GAConfig(pop=args.ga_pop, generations=args.ga_gens, seed=5)
+    sigma 0.3 A, seeded, FIRE capped at 300 steps per generation (unconverged structures carried over).
+    (1) an audited run: every generation's offspring are also relaxed in full, outside the search clock, for recall
+    and error; (2) the same run in compensated BF16 (BF16x3); (3) the same seeded search with and without screening
+    for equal search time; (4) a sample of offspring relaxed one structure per context gives the plain-path rate."""
+    from maise_amd.evolve import GAConfig, SurrogateGA, compare_screening, make_offspring
+    cfg = GAConfig(pop=args.ga_pop, generations=args.ga_gens, seed=5, relax_cap=300)
     t0 = time.perf_counter()
     ga = SurrogateGA(GA_POT, cfg)
     res = ga.run()
@@ -241,7 +243,6 @@ def ga_leg(maise_amd, args):
     ts = float(np.mean([r.seconds_screened for r in reps]))
     tf = float(np.mean([r.seconds_full for r in reps]))
     # plain path: the offspring of one more generation relaxed one structure per context
-    from maise_amd.evolve import make_offspring
     pop = [res.best] * 2
     rng = np.random.default_rng(77)
     kids = make_offspring(pop, np.ones(2), cfg, rng, args.ga_single)
@@ -253,23 +254,47 @@ def ga_leg(maise_amd, args):
     single = len(kids) / (time.perf_counter() - t1)
     c1.close()
     ga.close()
+    # the same screened search with the MLP in compensated BF16 (BF16x3), no audit
+    import dataclasses
+    cfg_b = dataclasses.replace(cfg, precision=maise_amd.MG_PREC_BF16X3, audit=False)
+    gb = SurrogateGA(GA_POT, cfg_b)
+    rb = gb.run()
+    gb.close()
+    tb = float(np.mean([r.seconds_screened for r in rb.reports[1:]]))
+    cmp_ = None
+    if args.ga_compare_seconds > 0:
+        cmp_ = compare_screening(GA_POT, GAConfig(pop=args.ga_pop, relax_cap=300), args.ga_compare_seconds,
+                                 list(range(1, args.ga_compare_seeds + 1)))
+        for name in cmp_["runs"]:
+            for r in cmp_["runs"][name]:
+                tr = r.pop("trajectory")
+                # improvements only (search clock, best eV/atom)
+                r["improvements"] = [(round(t, 3), round(e, 5)) for k, (t, e) in enumerate(tr)
+                                     if k == 0 or e < tr[k - 1][1] - 1e-9]
     n = cfg.pop
+    br = {"predict": "seconds_predict", "offspring": "seconds_offspring", "relax_kept": "seconds_relax",
+          "train": "seconds_train", "population_update": "seconds_select"}
     return {"workload": f"Mg16Ca8 (24 atoms), shipped potential nn_MgCa_d3_v0 (145-10-10-1, Rc 7.5 A), UREP 100 eV; "
-                        f"population {n}, {cfg.generations} generations, seed {cfg.seed}; surrogate 102-64-64-1 "
-                        "trained online on the GPU; keep top 25% + 10% random; FIRE fmax 0.01 eV/A",
+                        f"population {n}, {cfg.generations} generations, seed {cfg.seed}; surrogate 290-64-64-1 on the "
+                        "per-species mean descriptor rows, trained online on the GPU; keep top 25% + 10% random; "
+                        "FIRE fmax 0.01 eV/A, at most 300 steps per generation (the rest carried over)",
             "offspring_per_s_screened": n / ts, "offspring_per_s_batched_full": n / tf,
+            "offspring_per_s_screened_bf16x3": n / tb,
             "offspring_per_s_single_structure": single,
             "screened_over_single": (n / ts) / single, "batched_full_over_single": (n / tf) / single,
+            "screened_bf16x3_over_single": (n / tb) / single,
             "relaxations_per_generation_screened": reps[0].relaxed if reps else None,
             "recall_top10_mean": float(np.mean([r.recall_top10 for r in reps])),
             "recall_top25_mean": float(np.mean([r.recall_top25 for r in reps])),
+            "recall_top25_last10_mean": float(np.mean([r.recall_top25 for r in reps[-10:]])),
             "recall_random_selection": cfg.keep_top + cfg.keep_random,
             "mae_eV_atom_mean": float(np.mean([r.mae_eV_atom for r in reps])),
-            "seconds_per_generation": {"screened": ts, "full_relax": tf,
-                                       "predict": float(np.mean([r.seconds_predict for r in reps])),
-                                       "relax_kept": float(np.mean([r.seconds_relax for r in reps])),
-                                       "train": float(np.mean([r.seconds_train for r in reps]))},
+            "seconds_per_generation": dict({"screened": ts, "full_relax": tf, "screened_bf16x3": tb},
+                                           **{k: float(np.mean([getattr(r, v) for r in reps])) for k, v in br.items()}),
+            "train_share_of_generation": float(np.mean([r.seconds_train / r.seconds_screened for r in reps])),
+            "carried_per_generation_mean": float(np.mean([r.carried for r in reps])),
             "best_eV_atom_first_last": [res.reports[0].best_eV_atom, res.reports[-1].best_eV_atom],
+            "equal_time_comparison": cmp_,
             "seconds_total": wall}
 
 
@@ -288,6 +313,9 @@ def main():
     ap.add_argument("--ga-gens", type=int, default=30, help="generations of the screened search (0 skips it)")
     ap.add_argument("--ga-pop", type=int, default=512, help="population of the screened search")
     ap.add_argument("--ga-single", type=int, default=32, help="offspring relaxed one at a time for the plain rate")
+    ap.add_argument("--ga-compare-seconds", type=float, default=8.0,
+                    help="search clock of each run of the with/without-screening comparison (0 skips it)")
+    ap.add_argument("--ga-compare-seeds", type=int, default=3, help="seeds of the with/without-screening comparison")
     ap.add_argument("--rates-only", action="store_true",
                     help="only run the library's issue-rate microbenchmark (FP32 FMA, exp2, cos, sqrt, rcp) and print it")
     ap.add_argument("--copy-calibration", action="store_true",

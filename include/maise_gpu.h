@@ -139,7 +139,7 @@ int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out);
  * Inputs are standardised with (mu, sig) and the target with (ymu, ysig) set by mg_sur_set_norm. */
 typedef struct mg_sur mg_sur;
 int mg_sur_nparams(int nin);
-/* nin <= 128; weights ~ N(0, 1/fan_in) from the seed, biases 0, identity normalisation */
+/* nin <= 1024; weights ~ N(0, 1/fan_in) from the seed, biases 0, identity normalisation */
 int mg_sur_create(int device, int nin, unsigned long long seed, mg_sur **out);
 void mg_sur_free(mg_sur *s);
 int mg_sur_get_params(mg_sur *s, float *p);
@@ -153,14 +153,13 @@ int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, i
 int mg_sur_grad(mg_sur *s, const float *X, const float *y, int n, float *grad, float *loss);
 int mg_sur_predict(mg_sur *s, const float *X, int n, float *y);
 /* Features of every structure of the context's last evaluation: per species q, the mean over that structure's
- * atoms of species q of the same-species descriptor block (G2 block at ind2[0], then the G4 block at ind4[0][0];
- * ng2 + ng4 values), zero when q is absent.  feat holds S * nspc * (ng2 + ng4) floats. */
+ * atoms of species q of their descriptor rows (D values), zero when q is absent.  feat holds S * nspc * D floats. */
 int mg_features(mg_ctx *c, float *feat);
 
 /* --- evolutionary operators (host C; no GPU work) ---
  * Cut-and-splice crossover after BULK_MATE (src/efnc.c:2021-2310) with MATE_LT's lattice choice
  * (src/eutl.c:139-172), then a Gaussian displacement of every atom (sigma, A) and a periodic minimum-distance
- * test (>= dmin).  Parents: npar structures of N species-grouped atoms (spc[npar*N], lat[9*npar] rows,
+ * test (>= dmin), after at most N/4 steps of ADJT_CL's soft repulsion (src/eutl.c:97-135) when it fails.  Parents: npar structures of N species-grouped atoms (spc[npar*N], lat[9*npar] rows,
  * x[3*npar*N] Cartesian) drawn with the given weights; count children go to out_* with the same layout.  A slot
  * whose operators fail max_tries times is flagged in failed[count].  Returns the number of failed slots. */
 int mg_evo_offspring(int npar, int N, int nspc, const int *spc, const double *lat, const double *x,
@@ -172,7 +171,7 @@ double mg_evo_min_distance(const double *L, const double *x, int n);
  * lattice strained (diagonal 1 + cl g, symmetric off-diagonal 0.5 cl g, g Gaussian; fractional coordinates kept),
  * every coordinate moved by a Gaussian of ca A, the rows reduced (SHRT_LV), the volume within 50% of the parent's,
  * then each atom's position swapped with probability cs with that of an atom of another species, and all periodic
- * distances >= dmin.  Layout and return value as mg_evo_offspring. */
+ * distances >= dmin (after ADJT_CL's repair, as in mg_evo_offspring).  Layout and return value as mg_evo_offspring. */
 int mg_evo_mutate(int npar, int N, int nspc, const int *spc, const double *lat, const double *x, const double *weights,
                   int count, double cl, double ca, double cs, double dmin, unsigned long long seed, int max_tries,
                   int *out_spc, double *out_lat, double *out_x, int *failed);

@@ -350,10 +350,10 @@ class Context:
         return ep[:nsteps], ek[:nsteps], vol[:nsteps], pi[:nsteps]
 
     def features(self) -> np.ndarray:
-        """Per-structure surrogate inputs of the last evaluation: [S, nspc * (ng2 + ng4)] species means of the
-        same-species descriptor block."""
+        """Per-structure surrogate inputs of the last evaluation: [S, nspc * D], per species the mean of the
+        descriptor rows of that species' atoms."""
         m = self.model
-        f = np.zeros((self.S, m.nspc * (m.ng2 + m.ng4)), dtype=np.float32)
+        f = np.zeros((self.S, m.nspc * m.D), dtype=np.float32)
         _check(lib().mg_features(self._h, _fp(f)), "mg_features")
         return f
 

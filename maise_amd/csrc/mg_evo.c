@@ -88,6 +88,42 @@ double mg_evo_min_distance(const double *L, const double *x, int n) {
   return sqrt(best);
 }
 
+/* Soft repulsion that pushes too-close atoms apart, after ADJT_CL (src/eutl.c:97-135): at most `iters` steps;
+ * before each, give up (return 0) when a pair is closer than 0.7 dmin; a step moves every atom by 0.2 F_i with
+ * F_i = sum_j (x_i - x_j)/R (r_ij/R)^-13 over the periodic images within 2R (R = dmin, the pair's hard-core
+ * distance), |F_i| capped at 2.  Returns 1 when every distance ends >= dmin.  f and F hold 3N doubles. */
+static int push_apart(const double *L, double *x, int N, double dmin, int iters, double *f, double *F) {
+  const double R = dmin, rc2 = 4.0 * R * R;
+  for (int it = 0; it < iters; it++) {
+    if (mg_evo_min_distance(L, x, N) < 0.7 * dmin) return 0;
+    to_frac(L, x, N, f);
+    memset(F, 0, sizeof(double) * 3 * N);
+    for (int i = 0; i < N; i++)
+      for (int j = i + 1; j < N; j++) {
+        double d[3];
+        for (int q = 0; q < 3; q++) { d[q] = f[3 * j + q] - f[3 * i + q]; d[q] -= nearbyint(d[q]); }
+        for (int a = -1; a <= 1; a++)
+          for (int b = -1; b <= 1; b++)
+            for (int c = -1; c <= 1; c++) {
+              const double u = d[0] + a, v = d[1] + b, w = d[2] + c;
+              const double X = u * L[0] + v * L[3] + w * L[6], Y = u * L[1] + v * L[4] + w * L[7],
+                           Z = u * L[2] + v * L[5] + w * L[8];
+              const double r2 = X * X + Y * Y + Z * Z;
+              if (r2 >= rc2 || r2 == 0.0) continue;
+              const double g = pow(sqrt(r2) / R, -13.0) / R;   /* x_j - x_i = (X, Y, Z) */
+              F[3 * i] -= g * X; F[3 * i + 1] -= g * Y; F[3 * i + 2] -= g * Z;
+              F[3 * j] += g * X; F[3 * j + 1] += g * Y; F[3 * j + 2] += g * Z;
+            }
+      }
+    for (int i = 0; i < N; i++) {
+      const double n = sqrt(F[3 * i] * F[3 * i] + F[3 * i + 1] * F[3 * i + 1] + F[3 * i + 2] * F[3 * i + 2]);
+      const double sc = n > 2.0 ? 2.0 / n : 1.0;
+      for (int q = 0; q < 3; q++) x[3 * i + q] += 0.2 * sc * F[3 * i + q];
+    }
+  }
+  return mg_evo_min_distance(L, x, N) >= dmin;
+}
+
 /* random vector in the unit ball (RAND_VC, src/eutl.c:222-234) */
 static void ball(unsigned long long *s, double *a) {
   do {
@@ -181,7 +217,8 @@ static int mate(int N, int nspc, const int *s1, const double *L1, const double *
 
 /* count offspring of N atoms each from npar parents (parent p: spc[p*N..], lat[9p..], x[3pN..]) drawn with the
  * given weights; each child is a crossover, then a Gaussian displacement (sigma, A) of every atom, and must keep
- * all periodic distances >= dmin.  A child whose operators fail max_tries times in a row is flagged in failed[k]
+ * all periodic distances >= dmin, after at most N/4 steps of push_apart as BULK_MATE repairs its children with
+ * ADJT_CL (src/efnc.c:2194).  A child whose operators fail max_tries times in a row is flagged in failed[k]
  * (its slot is left unset).  Returns the number of failed slots, or < 0 on a bad argument. */
 int mg_evo_offspring(int npar, int N, int nspc, const int *spc, const double *lat, const double *x,
                      const double *weights, int count, double sigma, double dmin, unsigned long long seed,
@@ -200,9 +237,14 @@ int mg_evo_offspring(int npar, int N, int nspc, const int *spc, const double *la
     cum[p] = tot;
   }
   if (npos < 2) { free(cum); return mg_fail(MG_ERR_ARG, "mg_evo_offspring: at least two parents need positive weight"); }
-  double *f1 = (double *)malloc(sizeof(double) * 3 * N), *f2 = (double *)malloc(sizeof(double) * 3 * N);
-  int *side = (int *)malloc(sizeof(int) * 2 * N);
   int nfail = 0;
+  /* every child draws from its own stream (seed, k): the result does not depend on the thread count */
+#pragma omp parallel reduction(+ : nfail)
+  {
+  double *f1 = (double *)malloc(sizeof(double) * 3 * N), *f2 = (double *)malloc(sizeof(double) * 3 * N);
+  double *F = (double *)malloc(sizeof(double) * 3 * N);
+  int *side = (int *)malloc(sizeof(int) * 2 * N);
+#pragma omp for schedule(dynamic, 4)
   for (int k = 0; k < count; k++) {
     unsigned long long rs = seed ^ (0xD1B54A32D192ED03ull * (unsigned long long)(k + 1));
     int ok = 0;
@@ -222,15 +264,17 @@ int mg_evo_offspring(int npar, int N, int nspc, const int *spc, const double *la
                 x + 3 * (size_t)k2 * N, &rs, 20, cs, cl, cx, f1, f2, side))
         continue;
       for (int i = 0; i < 3 * N; i++) cx[i] += sigma * nrand(&rs);
-      ok = mg_evo_min_distance(cl, cx, N) >= dmin;
+      ok = mg_evo_min_distance(cl, cx, N) >= dmin || push_apart(cl, cx, N, dmin, N / 4, f1, F);
     }
     failed[k] = !ok;
     nfail += !ok;
   }
-  free(cum);
   free(f1);
   free(f2);
+  free(F);
   free(side);
+  }
+  free(cum);
   return nfail;
 }
 
@@ -276,7 +320,8 @@ static int reduce_rows(double *L, double lmin) {
  * off-diagonal), g standard Gaussian, with the fractional coordinates kept; every coordinate then moved by a
  * Gaussian of ca A; the rows reduced (SHRT_LV) and no row shorter than 1 A; the volume kept within a factor 1.5 of the
  * parent's; then, with more than one species, each atom's position swapped with probability cs with that of a random
- * atom of another species (species are grouped); finally every periodic distance must be >= dmin.  A slot whose tries
+ * atom of another species (species are grouped); finally every periodic distance must be >= dmin, after at most
+ * N/4 steps of push_apart as BULK_MUTE repairs its mutants with ADJT_CL (src/efnc.c:2360).  A slot whose tries
  * all fail is flagged in failed[k].  Layout as mg_evo_offspring.  Returns the number of failed slots, or < 0. */
 int mg_evo_mutate(int npar, int N, int nspc, const int *spc, const double *lat, const double *x, const double *weights,
                   int count, double cl, double ca, double cs, double dmin, unsigned long long seed, int max_tries,
@@ -293,8 +338,11 @@ int mg_evo_mutate(int npar, int N, int nspc, const int *spc, const double *lat, 
     cum[p] = tot;
   }
   if (!(tot > 0.0)) { free(cum); return mg_fail(MG_ERR_ARG, "mg_evo_mutate: no parent has positive weight"); }
-  double *f = (double *)malloc(sizeof(double) * 3 * N);
   int nfail = 0;
+#pragma omp parallel reduction(+ : nfail)
+  {
+  double *f = (double *)malloc(sizeof(double) * 3 * N), *F = (double *)malloc(sizeof(double) * 3 * N);
+#pragma omp for schedule(dynamic, 4)
   for (int k = 0; k < count; k++) {
     unsigned long long rs = seed ^ (0x9E6C63D0676A9A99ull * (unsigned long long)(k + 1));
     int ok = 0;
@@ -336,13 +384,15 @@ int mg_evo_mutate(int npar, int N, int nspc, const int *spc, const double *lat, 
               cx[3 * j + q] = tmp;
             }
           }
-      ok = mg_evo_min_distance(cL, cx, N) >= dmin;
+      ok = mg_evo_min_distance(cL, cx, N) >= dmin || push_apart(cL, cx, N, dmin, N / 4, f, F);
     }
     failed[k] = !ok;
     nfail += !ok;
   }
-  free(cum);
   free(f);
+  free(F);
+  }
+  free(cum);
   return nfail;
 }
 
@@ -367,30 +417,48 @@ int mg_evo_fingerprint(int S, const int *atom_off, int nspc, const int *spc, con
   const double DR = 0.5 / (sigma * sigma);
   const double rcut = rmax + 3.0 * sigma + dr;
   const double cstep = exp(-2.0 * DR * dr * dr);
+  /* the taper f(x_k) of every bin, once */
+  double *fk = (double *)malloc(sizeof(double) * (size_t)nrdf);
+  for (int k = 0; k < nrdf; k++) {
+    const double xk = (double)k * dr;
+    fk[k] = xk < rmin ? 1.0 : cos(0.5 * 3.14159265358979323846 * (xk - rmin) / (rmax - rmin));
+  }
   /* structures in parallel on the host cores, each with its own accumulator */
 #pragma omp parallel
   {
   double *acc = (double *)malloc(sizeof(double) * (size_t)npair * nrdf);
+  double *fr = NULL;
+  int fr_cap = 0;
 #pragma omp for schedule(dynamic, 4)
   for (int s = 0; s < S; s++) {
     const int a0 = atom_off[s], n = atom_off[s + 1] - a0;
     const double *L = lat + 9 * (size_t)s, *X = x + 3 * (size_t)a0;
     const int *sp = spc + a0;
     memset(acc, 0, sizeof(double) * (size_t)npair * nrdf);
-    double I[9];
+    double I[9], rr[3];
     inv3(L, I);
-    int K[3];
-    for (int q = 0; q < 3; q++) {  /* images needed: rcut times the length of reciprocal row q (columns of I) */
-      const double rq = sqrt(I[q] * I[q] + I[3 + q] * I[3 + q] + I[6 + q] * I[6 + q]);
-      K[q] = (int)ceil(rcut * rq) + 1;
+    for (int q = 0; q < 3; q++)  /* a pair closer than rcut has fractional separation below rcut |column q of L^-1| */
+      rr[q] = rcut * sqrt(I[q] * I[q] + I[3 + q] * I[3 + q] + I[6 + q] * I[6 + q]);
+    if (n > fr_cap) {
+      free(fr);
+      fr_cap = n;
+      fr = (double *)malloc(sizeof(double) * 3 * (size_t)fr_cap);
     }
+    to_frac(L, X, n, fr);
     for (int i = 0; i < n; i++)
       for (int j = 0; j < n; j++) {
         if (sp[j] < sp[i]) continue;
         const int p = pidx[sp[i]][sp[j]];
-        for (int a = -K[0]; a <= K[0]; a++)
-          for (int b = -K[1]; b <= K[1]; b++)
-            for (int c = -K[2]; c <= K[2]; c++) {
+        /* only the images whose fractional separation can be below rr (the others are out of range) */
+        int lo[3], hi[3];
+        for (int q = 0; q < 3; q++) {
+          const double df = fr[3 * j + q] - fr[3 * i + q];
+          lo[q] = (int)ceil(-rr[q] - df) - 1;
+          hi[q] = (int)floor(rr[q] - df) + 1;
+        }
+        for (int a = lo[0]; a <= hi[0]; a++)
+          for (int b = lo[1]; b <= hi[1]; b++)
+            for (int c = lo[2]; c <= hi[2]; c++) {
               if (i == j && a == 0 && b == 0 && c == 0) continue;
               double d[3], r2 = 0.0;
               for (int q = 0; q < 3; q++) {
@@ -405,11 +473,7 @@ int mg_evo_fingerprint(int S, const int *atom_off, int nspc, const int *spc, con
               double xk = (double)k0 * dr;
               double g = exp(-(r - xk) * (r - xk) * DR), qk = exp(2.0 * DR * dr * (r - xk) - DR * dr * dr);
               for (int k = k0; k <= o + Do; k++) {
-                if (k >= 0 && k < nrdf) {
-                  xk = (double)k * dr;
-                  const double fk = xk < rmin ? 1.0 : cos(0.5 * 3.14159265358979323846 * (xk - rmin) / (rmax - rmin));
-                  acc[(size_t)p * nrdf + k] += fk * g;
-                }
+                if (k >= 0 && k < nrdf) acc[(size_t)p * nrdf + k] += fk[k] * g;
                 g *= qk;
                 qk *= cstep;
               }
@@ -424,6 +488,8 @@ int mg_evo_fingerprint(int S, const int *atom_off, int nspc, const int *spc, con
     for (size_t q = 0; q < (size_t)npair * nrdf; q++) o[q] = (float)(acc[q] * nrm);
   }
   free(acc);
+  free(fr);
   }
+  free(fk);
   return MG_OK;
 }

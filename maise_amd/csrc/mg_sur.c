@@ -13,12 +13,12 @@
 #include "mg_host.h"
 
 #define SUR_H 64
-#define SUR_MAXIN 128
-#define SUR_CHUNK 4
+#define SUR_MAXIN 1024
+#define SUR_W 4        /* samples per block of the gradient kernel */
 
-int mgk_sur_features(int S, const mg_struct_dev *str, const int *spc, const float *G, int D, int nspc, int ng2,
-                     int ng4, int off2, int off4, float *feat, hipStream_t st);
-int mgk_sur_grad(int B, int chunk, int nin, const int *idx, const float *X, const float *Y, const float *p,
+int mgk_sur_features(int S, const mg_struct_dev *str, const int *spc, const float *G, int D, int nspc, float *feat,
+                     hipStream_t st);
+int mgk_sur_grad(int B, int nblk, int nin, const int *idx, const float *X, const float *Y, const float *p,
                  const float *norm, float *part, float *part_loss, hipStream_t st);
 int mgk_sur_adam(int P, int nblk, const float *part, float *p, float *m, float *v, float *grad, float lr, float b1,
                  float b2, float eps, float c1, float c2, hipStream_t st);
@@ -33,7 +33,7 @@ struct mg_sur {
   size_t cap_blk;
   float *d_X, *d_Y, *d_out;
   int *d_idx;
-  size_t cap_n;
+  size_t cap_n, cap_idx;
 };
 
 static unsigned long long splitmix64(unsigned long long *s) {
@@ -140,20 +140,32 @@ int mg_sur_set_norm(mg_sur *s, const float *mu, const float *sig, float ymu, flo
   return MG_OK;
 }
 
-#define SUR_MAXBLK 256  /* blocks of a full-data gradient pass (chunk grows with n) */
+#define SUR_MAXBLK 256  /* blocks of a full-data gradient pass (each takes more rounds as n grows) */
+#define SUR_IDX_CAP (1 << 22)  /* sample indices uploaded at once: the permutations of that many epochs */
 
-static int ensure_data(mg_sur *s, int n, int batch) {
+static int grad_blocks(int n) {
+  const int b = (n + SUR_W - 1) / SUR_W;
+  return b < SUR_MAXBLK ? b : SUR_MAXBLK;
+}
+
+/* device room for n samples, nidx sample indices and the block partials of batches of `batch` */
+static int ensure_data(mg_sur *s, int n, size_t nidx, int batch) {
   int r;
   if ((size_t)n > s->cap_n) {
     size_t cap = (size_t)n + 256;
+    s->cap_n = 0;
     if ((r = mg_dev_alloc((void **)&s->d_X, sizeof(float) * cap * s->nin)) ||
         (r = mg_dev_alloc((void **)&s->d_Y, sizeof(float) * cap)) ||
-        (r = mg_dev_alloc((void **)&s->d_out, sizeof(float) * cap)) ||
-        (r = mg_dev_alloc((void **)&s->d_idx, sizeof(int) * cap)))
+        (r = mg_dev_alloc((void **)&s->d_out, sizeof(float) * cap)))
       return r;
     s->cap_n = cap;
   }
-  size_t nblk = (size_t)(batch + SUR_CHUNK - 1) / SUR_CHUNK;
+  if (nidx > s->cap_idx) {
+    s->cap_idx = 0;
+    if ((r = mg_dev_alloc((void **)&s->d_idx, sizeof(int) * nidx))) return r;
+    s->cap_idx = nidx;
+  }
+  size_t nblk = (size_t)(batch + SUR_W - 1) / SUR_W;
   if (nblk < SUR_MAXBLK) nblk = SUR_MAXBLK;
   if (nblk > s->cap_blk) {
     if ((r = mg_dev_alloc((void **)&s->d_part, sizeof(float) * nblk * s->P)) ||
@@ -172,10 +184,8 @@ static int upload_xy(mg_sur *s, const float *X, const float *y, int n) {
 
 /* mean squared error (normalised units) of the current parameters over the n samples now on the device */
 static int full_loss(mg_sur *s, int n, float *loss, float *grad) {
-  int chunk = (n + SUR_MAXBLK - 1) / SUR_MAXBLK;
-  if (chunk < SUR_CHUNK) chunk = SUR_CHUNK;
-  const int nblk = (n + chunk - 1) / chunk;
-  if (mgk_sur_grad(n, chunk, s->nin, NULL, s->d_X, s->d_Y, s->d_p, s->d_norm, s->d_part, s->d_ploss, s->st) ||
+  const int nblk = grad_blocks(n);
+  if (mgk_sur_grad(n, nblk, s->nin, NULL, s->d_X, s->d_Y, s->d_p, s->d_norm, s->d_part, s->d_ploss, s->st) ||
       mgk_sur_adam(s->P, nblk, s->d_part, s->d_p, s->d_m, s->d_v, s->d_grad, 0.0f, 0.f, 0.f, 0.f, 1.f, 1.f, s->st))
     return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
   float *pl = (float *)malloc(sizeof(float) * nblk);
@@ -194,7 +204,7 @@ int mg_sur_grad(mg_sur *s, const float *X, const float *y, int n, float *grad, f
   if (!s || !X || !y || n < 1) return mg_fail(MG_ERR_ARG, "mg_sur_grad: bad argument");
   HIPCHK(hipSetDevice(s->device));
   int r;
-  if ((r = ensure_data(s, n, 1)) || (r = upload_xy(s, X, y, n))) return r;
+  if ((r = ensure_data(s, n, 0, 1)) || (r = upload_xy(s, X, y, n))) return r;
   return full_loss(s, n, loss, grad);
 }
 
@@ -205,36 +215,50 @@ int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, i
   HIPCHK(hipSetDevice(s->device));
   int r;
   if (batch > n) batch = n;
-  if ((r = ensure_data(s, n, batch)) || (r = upload_xy(s, X, y, n))) return r;
+  /* the seeded permutations of a group of epochs go up in one copy; the steps then run without a host wait */
+  int group = SUR_IDX_CAP / n;
+  if (group < 1) group = 1;
+  if (group > epochs) group = epochs > 0 ? epochs : 1;
+  if ((r = ensure_data(s, n, (size_t)group * n, batch)) || (r = upload_xy(s, X, y, n))) return r;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  int *perm = (int *)malloc(sizeof(int) * n);
+  int *perm = (int *)malloc(sizeof(int) * (size_t)group * n);
   unsigned long long st = seed;
-  for (int ep = 0; ep < epochs; ep++) {
-    for (int i = 0; i < n; i++) perm[i] = i;
-    for (int i = n - 1; i > 0; i--) { /* Fisher-Yates, seeded */
-      int j = (int)(splitmix64(&st) % (unsigned long long)(i + 1));
-      int t = perm[i]; perm[i] = perm[j]; perm[j] = t;
+  for (int ep0 = 0; ep0 < epochs; ep0 += group) {
+    const int ng = epochs - ep0 < group ? epochs - ep0 : group;
+    /* the previous group's kernels read d_idx and perm's last copy may still be in flight: wait for both */
+    if (hipStreamSynchronize(s->st) != hipSuccess) {
+      free(perm);
+      return mg_fail(MG_ERR_HIP, "mg_sur_train: stream");
     }
-    /* the previous epoch's kernels read d_idx: order the copy after them on the same stream */
-    if (hipMemcpyAsync(s->d_idx, perm, sizeof(int) * n, hipMemcpyHostToDevice, s->st) != hipSuccess ||
-        hipStreamSynchronize(s->st) != hipSuccess) {
+    for (int g = 0; g < ng; g++) {
+      int *pg = perm + (size_t)g * n;
+      for (int i = 0; i < n; i++) pg[i] = i;
+      for (int i = n - 1; i > 0; i--) { /* Fisher-Yates, seeded */
+        int j = (int)(splitmix64(&st) % (unsigned long long)(i + 1));
+        int t = pg[i]; pg[i] = pg[j]; pg[j] = t;
+      }
+    }
+    if (hipMemcpyAsync(s->d_idx, perm, sizeof(int) * (size_t)ng * n, hipMemcpyHostToDevice, s->st) != hipSuccess) {
       free(perm);
       return mg_fail(MG_ERR_HIP, "mg_sur_train: index upload");
     }
-    for (int b0 = 0; b0 < n; b0 += batch) {
-      const int B = n - b0 < batch ? n - b0 : batch;
-      const int nblk = (B + SUR_CHUNK - 1) / SUR_CHUNK;
-      s->step++;
-      const float c1 = 1.0f - powf(b1, (float)s->step), c2 = 1.0f - powf(b2, (float)s->step);
-      if (mgk_sur_grad(B, SUR_CHUNK, s->nin, s->d_idx + b0, s->d_X, s->d_Y, s->d_p, s->d_norm, s->d_part, s->d_ploss, s->st) ||
-          mgk_sur_adam(s->P, nblk, s->d_part, s->d_p, s->d_m, s->d_v, NULL, lr, b1, b2, eps, c1, c2, s->st)) {
-        free(perm);
-        return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
+    for (int g = 0; g < ng; g++)
+      for (int b0 = 0; b0 < n; b0 += batch) {
+        const int B = n - b0 < batch ? n - b0 : batch;
+        const int nblk = (B + SUR_W - 1) / SUR_W;
+        s->step++;
+        const float c1 = 1.0f - powf(b1, (float)s->step), c2 = 1.0f - powf(b2, (float)s->step);
+        if (mgk_sur_grad(B, nblk, s->nin, s->d_idx + (size_t)g * n + b0, s->d_X, s->d_Y, s->d_p, s->d_norm,
+                         s->d_part, s->d_ploss, s->st) ||
+            mgk_sur_adam(s->P, nblk, s->d_part, s->d_p, s->d_m, s->d_v, NULL, lr, b1, b2, eps, c1, c2, s->st)) {
+          free(perm);
+          return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
+        }
       }
-    }
   }
+  r = full_loss(s, n, loss, NULL);  /* ends with a stream synchronisation: perm's copies are complete */
   free(perm);
-  return full_loss(s, n, loss, NULL);
+  return r;
 }
 
 int mg_sur_predict(mg_sur *s, const float *X, int n, float *y) {
@@ -242,7 +266,7 @@ int mg_sur_predict(mg_sur *s, const float *X, int n, float *y) {
   if (n == 0) return MG_OK;
   HIPCHK(hipSetDevice(s->device));
   int r;
-  if ((r = ensure_data(s, n, 1)) || (r = upload_xy(s, X, NULL, n))) return r;
+  if ((r = ensure_data(s, n, 0, 1)) || (r = upload_xy(s, X, NULL, n))) return r;
   if (mgk_sur_predict(n, s->nin, s->d_X, s->d_p, s->d_norm, s->d_out, s->st))
     return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
   HIPCHK(hipMemcpyAsync(y, s->d_out, sizeof(float) * n, hipMemcpyDeviceToHost, s->st));
@@ -256,12 +280,11 @@ int mg_features(mg_ctx *c, float *feat) {
   const mg_model *m = c->m;
   HIPCHK(hipSetDevice(c->device));
   { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
-  const int nf = m->nspc * (m->ng2 + m->ng4);
+  const int nf = m->nspc * m->D;
   float *d_feat = NULL;
   int r;
   if ((r = mg_dev_alloc((void **)&d_feat, sizeof(float) * (size_t)c->S * nf))) return r;
-  if (mgk_sur_features(c->S, c->d_str, c->d_spc, c->d_G, m->D, m->nspc, m->ng2, m->ng4, m->basis.ind2[0],
-                       m->basis.ind4[0][0], d_feat, c->st)) {
+  if (mgk_sur_features(c->S, c->d_str, c->d_spc, c->d_G, m->D, m->nspc, d_feat, c->st)) {
     hipFree(d_feat);
     return mg_fail(MG_ERR_HIP, "feature kernel launch failed");
   }

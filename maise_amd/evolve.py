@@ -19,11 +19,19 @@ The generation loop of the reference's EVOS caller (src/emod.c:484-757) with its
 All structures are relaxed on the GPU in one batch (mg_relax, positions only, cell fixed).
 
 Surrogate screening: each generation's offspring are evaluated once unrelaxed (one batched descriptor pass),
-the surrogate predicts their relaxed energy per atom from the species means of the same-species descriptor
-block, and only the predicted best `keep_top` fraction plus a random `keep_random` fraction of the rest are
-relaxed.  The surrogate is retrained after every generation on all (unrelaxed features, relaxed energy) pairs
-relaxed so far.  With `audit=True` every offspring is also relaxed (separately timed) so that top-k recall and
-the surrogate's energy error are measured against full relaxation.
+the surrogate predicts their relaxed energy per atom from the per-species means of their descriptor rows, and
+only the predicted best `keep_top` fraction plus a random `keep_random` fraction of the rest are relaxed.  The
+surrogate is retrained after every generation on all (unrelaxed features, relaxed energy) pairs relaxed so far.
+With `screen=False` every offspring is relaxed and no surrogate is trained (the method without it).  With
+`audit=True` every offspring is also relaxed (separately timed, outside the search clock) so that top-k recall
+and the surrogate's energy error are measured against full relaxation.
+
+Relaxation tail: the reference gives each relaxation job a wall-time limit and drops the ones that exceed it
+(RELX_TR, src/emod.c:608-660).  Here a generation's batch runs at most `relax_cap` FIRE steps; a structure not
+converged by then is carried, at its current positions, into the next generation's batch (FIRE restarts from
+rest) and joins the population when it converges; one still unconverged after `max_steps` steps in total is
+dropped, as the reference drops a job over its limit.  So the slowest few structures no longer set every
+generation's wall time.
 """
 from __future__ import annotations
 
@@ -68,6 +76,9 @@ class GAConfig:
     fp_rmin: float = 5.8
     fp_sigma: float = 0.008
     precision: int = 0                         # MLP precision of the relaxations and evaluations (MG_PREC_*)
+    screen: bool = True                        # screen offspring with the surrogate (False: relax every offspring)
+    offspring: int = 0                         # offspring per generation (0: pop)
+    relax_cap: int = 0                         # FIRE steps per generation before a structure is carried over (0: none)
 
 
 @dataclass
@@ -75,16 +86,21 @@ class GenerationReport:
     gen: int
     offspring: int
     relaxed: int
-    seconds_screened: float                    # features + prediction + relaxation of the kept + training
+    seconds_screened: float                    # the whole generation: offspring, screening, relaxation, training, update
     seconds_full: Optional[float] = None       # relaxation of every offspring (audit)
     recall_top10: Optional[float] = None
     recall_top25: Optional[float] = None
     mae_eV_atom: Optional[float] = None
     best_eV_atom: float = math.nan
     train_loss: Optional[float] = None
+    seconds_offspring: Optional[float] = None  # crossover and mutation (host)
     seconds_predict: Optional[float] = None    # unrelaxed evaluation + features + prediction + selection
     seconds_relax: Optional[float] = None      # relaxation of the kept offspring
     seconds_train: Optional[float] = None      # retraining
+    seconds_select: Optional[float] = None     # fingerprints and the population update
+    elapsed: float = 0.0                       # search clock at the end of the generation (audit excluded)
+    carried: int = 0                           # unconverged structures carried into the next generation
+    dropped: int = 0                           # structures dropped after max_steps FIRE steps in total
 
 
 @dataclass
@@ -276,7 +292,7 @@ class SurrogateGA:
         self.model = Model(model_path)
         self.device = device
         self.rng = np.random.default_rng(cfg.seed)
-        self.nin = self.model.nspc * (self.model.ng2 + self.model.ng4)
+        self.nin = self.model.nspc * self.model.D
         self.sur = Surrogate(self.nin, device=device, seed=cfg.seed)
         self.trained = False
         self.X: List[np.ndarray] = []
@@ -295,27 +311,41 @@ class SurrogateGA:
         c.eval(forces=False)
         return c.features()
 
-    def relax(self, structs: List[Structure]):
-        """Relax in one batch; returns (relaxed structures, energy per atom, converged flags)."""
+    def relax(self, structs: List[Structure], max_steps: Optional[int] = None):
+        """Relax in one batch; returns (structures at their final positions, energy per atom, converged flags,
+        FIRE steps taken)."""
         c = self._ctx_relax
         c.set_batch(structs)
-        _, done = c.relax(max_steps=self.cfg.max_steps, fmax=self.cfg.fmax)
+        steps, done = c.relax(max_steps=self.cfg.max_steps if max_steps is None else max_steps, fmax=self.cfg.fmax)
         E = c.batch_result(forces=False)["E"]
         out, epa = [], np.zeros(len(structs))
         for k, (spc, lat, _) in enumerate(structs):
             X = c.result(k, forces=False)["X"]
             out.append((spc, lat, X))
             epa[k] = E[k] / len(spc)
-        return out, epa, done
+        return out, epa, done, steps
 
     def train(self) -> float:
-        """A fixed budget of Adam steps over everything relaxed so far, warm-started from the last generation's
-        network; the normalisation is fitted once, on the first (random, fully relaxed) generation."""
-        X = np.concatenate(self.X)
-        y = np.concatenate(self.y)
+        """A fixed budget of train_steps Adam steps, warm-started from the last generation's network, over the
+        newest generation's pairs plus a uniform sample of the earlier ones (all of them while they fit in the
+        budget's train_steps * batch samples, so that the cost per generation stays flat as the data grows); the
+        normalisation is fitted once, on the first (random, fully relaxed) generation."""
+        cap = self.cfg.train_steps * self.cfg.batch
+        X_new, y_new = self.X[-1], self.y[-1]
+        if len(self.X) > 1:
+            X_old = np.concatenate(self.X[:-1])
+            y_old = np.concatenate(self.y[:-1])
+            room = cap - len(y_new)
+            if len(y_old) > room:
+                pick = self.rng.choice(len(y_old), size=max(room, 0), replace=False)
+                X_old, y_old = X_old[pick], y_old[pick]
+            X = np.concatenate([X_old, X_new])
+            y = np.concatenate([y_old, y_new])
+        else:
+            X, y = X_new, y_new
         if not self.trained:
             self.sur.fit_norm(X, y)
-        epochs = max(1, int(round(self.cfg.train_steps * self.cfg.batch / len(y))))
+        epochs = max(1, int(round(cap / len(y))))
         loss = self.sur.train(X, y, epochs=epochs, batch=self.cfg.batch, lr=self.cfg.lr,
                               seed=int(self.rng.integers(1 << 62)))
         self.trained = True
@@ -344,45 +374,93 @@ class SurrogateGA:
                                      n_unique=len(pop))
         return [allS[k] for k in keep], allE[keep], allFP[keep]
 
-    def run(self, log=None) -> GAResult:
+    def _relax_with_carry(self, batch, feats, prior_steps, capped=True):
+        """Relax `batch` for at most relax_cap steps (max_steps without a cap, or with capped=False).  Returns the
+        converged structures with their energies and features, and the carry list of (structure, features, steps
+        so far) for the unconverged ones still under max_steps, plus the number dropped."""
+        cfg = self.cfg
+        cap = cfg.relax_cap if (cfg.relax_cap > 0 and capped) else cfg.max_steps
+        rel, epa, done, steps = self.relax(batch, max_steps=cap)
+        fin, finE, finF, carry, dropped = [], [], [], [], 0
+        for k in range(len(batch)):
+            total = prior_steps[k] + int(steps[k])
+            if done[k]:
+                fin.append(rel[k])
+                finE.append(epa[k])
+                finF.append(feats[k])
+            elif total < cfg.max_steps and cfg.relax_cap > 0:
+                carry.append((rel[k], feats[k], total))
+            else:
+                dropped += 1
+        return fin, np.array(finE), finF, carry, dropped
+
+    def run(self, log=None, time_budget: Optional[float] = None) -> GAResult:
+        """Run up to cfg.generations generations, or until the search clock passes time_budget seconds."""
         cfg = self.cfg
         res = GAResult()
-        # generation 0: random structures, all relaxed, first training set
+        screen = cfg.screen
+        n_off = cfg.offspring if cfg.offspring > 0 else cfg.pop
+        clock = 0.0
+        # generation 0: random structures, all relaxed without the cap (they are the first parents), first
+        # training set
         t0 = time.perf_counter()
         init = [random_structure(cfg, self.rng) for _ in range(cfg.pop)]
-        feat = self.features(init)
-        relaxed, epa, done = self.relax(init)
-        self.X.append(feat)
-        self.y.append(epa.astype(np.float32))
-        loss = self.train()
-        pop, popE, popFP = self.update_population([], np.zeros(0), None, relaxed, np.where(done, epa, np.inf))
-        res.reports.append(GenerationReport(0, len(init), len(init), time.perf_counter() - t0,
-                                            best_eV_atom=float(popE[0]), train_loss=loss))
+        feat = self.features(init) if screen else [None] * len(init)
+        fin, finE, finF, carry, dropped = self._relax_with_carry(init, feat, [0] * len(init), capped=False)
+        loss = None
+        if screen and fin:
+            self.X.append(np.stack(finF))
+            self.y.append(finE.astype(np.float32))
+            loss = self.train()
+        pop, popE, popFP = self.update_population([], np.zeros(0), None, fin, finE)
+        clock += time.perf_counter() - t0
+        res.reports.append(GenerationReport(0, len(init), len(init), clock, best_eV_atom=float(popE[0]),
+                                            train_loss=loss, elapsed=clock, carried=len(carry), dropped=dropped))
         if log:
             log(res.reports[-1])
         for g in range(1, cfg.generations + 1):
+            if time_budget is not None and clock >= time_budget:
+                break
+            t0 = time.perf_counter()
             ranks = np.arange(len(pop))
             weights = np.exp(-ranks / max(1.0, 0.25 * len(pop)))
-            nm = int(round(cfg.mute_frac * cfg.pop))
-            kids = make_offspring(pop, weights, cfg, self.rng, cfg.pop - nm)
+            nm = int(round(cfg.mute_frac * n_off))
+            kids = make_offspring(pop, weights, cfg, self.rng, n_off - nm)
             if nm:
                 kids += mutate(pop, weights, cfg, self.rng, nm)
-            t0 = time.perf_counter()
-            feat = self.features(kids)
-            pred = self.sur.predict(feat)
-            sel = self.select(pred)
+            t_off = time.perf_counter()
+            pred = None
+            if screen:
+                feat = self.features(kids)
+                pred = self.sur.predict(feat)
+                sel = self.select(pred)
+            else:
+                feat = [None] * len(kids)
+                sel = np.arange(len(kids))
             t1 = time.perf_counter()
-            relaxed, epa, done = self.relax([kids[k] for k in sel])
+            batch = [kids[k] for k in sel] + [c[0] for c in carry]
+            bfeat = [feat[k] for k in sel] + [c[1] for c in carry]
+            prior = [0] * len(sel) + [c[2] for c in carry]
+            fin, finE, finF, carry, dropped = self._relax_with_carry(batch, bfeat, prior)
             t2 = time.perf_counter()
-            self.X.append(feat[sel])
-            self.y.append(epa.astype(np.float32))
-            loss = self.train()
+            loss = None
+            if screen and fin:
+                self.X.append(np.stack(finF))
+                self.y.append(finE.astype(np.float32))
+                loss = self.train()
             t3 = time.perf_counter()
-            rep = GenerationReport(g, len(kids), len(sel), t3 - t0, train_loss=loss, seconds_predict=t1 - t0,
-                                   seconds_relax=t2 - t1, seconds_train=t3 - t2)
-            if cfg.audit:
+            pop, popE, popFP = self.update_population(pop, popE, popFP, fin, finE)
+            t4 = time.perf_counter()
+            clock += t4 - t0
+            rep = GenerationReport(g, len(kids), len(sel), t4 - t0, train_loss=loss, seconds_predict=t1 - t0,
+                                   seconds_relax=t2 - t1, seconds_train=t3 - t2, elapsed=clock,
+                                   carried=len(carry), dropped=dropped)
+            rep.seconds_select = t4 - t3
+            rep.seconds_offspring = t_off - t0
+            rep.seconds_predict = t1 - t_off
+            if screen and cfg.audit:
                 t1 = time.perf_counter()
-                _, epa_all, done_all = self.relax(kids)
+                _, epa_all, done_all, _ = self.relax(kids)
                 rep.seconds_full = time.perf_counter() - t1
                 order = np.argsort(epa_all, kind="stable")
                 sset = set(sel.tolist())
@@ -390,7 +468,6 @@ class SurrogateGA:
                     k = max(1, int(round(frac_k * len(kids))))
                     setattr(rep, name, len(sset.intersection(order[:k].tolist())) / k)
                 rep.mae_eV_atom = float(np.mean(np.abs(pred.astype(np.float64) - epa_all)))
-            pop, popE, popFP = self.update_population(pop, popE, popFP, relaxed, np.where(done, epa, np.inf))
             rep.best_eV_atom = float(popE[0])
             res.reports.append(rep)
             if log:
@@ -398,3 +475,51 @@ class SurrogateGA:
         res.population_E = popE
         res.best = pop[0]
         return res
+
+
+def compare_screening(model_path: str, cfg: GAConfig, seconds: float, seeds, variants=None, log=None) -> dict:
+    """The same seeded search with and without surrogate screening, each run for `seconds` of search clock.
+
+    variants maps a name to GAConfig overrides; the default compares the method without the surrogate
+    ("unscreened": every offspring relaxed), the screened search as configured ("screened") and the screened
+    search drawing as many offspring as make its relaxed share equal the unscreened batch ("screened_wide").
+    For each variant and seed the result holds the best energy per atom against the search clock; per seed the
+    target is the lowest energy any variant reached (within 1e-4 eV/atom), and `seconds_to_target` the clock at
+    which a variant first reached it (None if it did not)."""
+    import dataclasses
+    share = cfg.keep_top + cfg.keep_random
+    if variants is None:
+        variants = {"unscreened": {"screen": False},
+                    "screened": {"screen": True},
+                    "screened_wide": {"screen": True, "offspring": int(round(cfg.pop / share))}}
+    runs = {name: [] for name in variants}
+    for seed in seeds:
+        for name, over in variants.items():
+            c = dataclasses.replace(cfg, seed=int(seed), audit=False, generations=10 ** 6, **over)
+            ga = SurrogateGA(model_path, c)
+            try:
+                res = ga.run(time_budget=seconds)
+            finally:
+                ga.close()
+            traj = [(r.elapsed, r.best_eV_atom) for r in res.reports]
+            runs[name].append({"seed": int(seed), "generations": len(res.reports) - 1,
+                               "relaxed_per_generation": res.reports[-1].relaxed if len(res.reports) > 1 else None,
+                               "best_eV_atom": float(res.population_E[0]), "trajectory": traj})
+            if log:
+                log(name, seed, runs[name][-1])
+    for k, seed in enumerate(seeds):
+        target = min(runs[name][k]["best_eV_atom"] for name in variants)
+        for name in variants:
+            hit = [t for t, e in runs[name][k]["trajectory"] if e <= target + 1e-4]
+            runs[name][k]["target_eV_atom"] = target
+            runs[name][k]["seconds_to_target"] = hit[0] if hit else None
+    summary = {}
+    for name in variants:
+        r = runs[name]
+        tt = [x["seconds_to_target"] for x in r]
+        summary[name] = {"best_eV_atom_mean": float(np.mean([x["best_eV_atom"] for x in r])),
+                         "reached_target": int(sum(t is not None for t in tt)),
+                         "seconds_to_target_median": (float(np.median([t if t is not None else np.inf for t in tt]))
+                                                      if tt else None),
+                         "generations_mean": float(np.mean([x["generations"] for x in r]))}
+    return {"seconds": seconds, "seeds": [int(s) for s in seeds], "summary": summary, "runs": runs}

@@ -78,18 +78,16 @@ def adam_step(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
     return p - lr * mh / (np.sqrt(vh) + eps), m, v
 
 
-def features(G, spc, atom_off, nspc, ng2, ng4, off2=0, off4=None):
-    """Per structure and species, the mean of the same-species descriptor block (G2 at ind2[0] = 0, G4 at
-    ind4[0] = NG2, src/nprs.c:484-489) over that species' atoms; zeros for an absent species."""
-    if off4 is None:
-        off4 = ng2
+def features(G, spc, atom_off, nspc):
+    """Per structure and species, the mean of the whole descriptor row over that species' atoms; zeros for an
+    absent species."""
     G = np.asarray(G, dtype=np.float64)
-    cols = list(range(off2, off2 + ng2)) + list(range(off4, off4 + ng4))
+    D = G.shape[1]
     S = len(atom_off) - 1
-    out = np.zeros((S, nspc * len(cols)))
+    out = np.zeros((S, nspc * D))
     for s in range(S):
         for q in range(nspc):
             rows = [i for i in range(atom_off[s], atom_off[s + 1]) if spc[i] == q]
             if rows:
-                out[s, q * len(cols):(q + 1) * len(cols)] = G[np.ix_(rows, cols)].mean(axis=0)
+                out[s, q * D:(q + 1) * D] = G[rows].mean(axis=0)
     return out

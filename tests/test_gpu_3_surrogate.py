@@ -35,16 +35,16 @@ def test_features_match_oracle_means():
     c.set_batch(structs)
     c.eval(forces=False)
     f = c.features()
-    assert f.shape == (6, 2 * (m.ng2 + m.ng4))
+    assert f.shape == (6, 2 * m.D)
     G = c.descriptors()
     spc = np.concatenate([s[0] for s in structs])
     off = np.concatenate([[0], np.cumsum([len(s[0]) for s in structs])])
-    ref32 = so.features(G, spc, off, m.nspc, m.ng2, m.ng4)
+    ref32 = so.features(G, spc, off, m.nspc)
     assert np.allclose(f, ref32, rtol=1e-5, atol=1e-7)
     # and against FP64 oracle descriptors
     om = po.OracleModel(MGCA)
     G64 = np.concatenate([om.evaluate(*s, forces=False, want_g=True)["G"] for s in structs])
-    ref64 = so.features(G64, spc, off, m.nspc, m.ng2, m.ng4)
+    ref64 = so.features(G64, spc, off, m.nspc)
     assert np.allclose(f, ref64, rtol=1e-4, atol=1e-6)
     c.close()
 
@@ -131,3 +131,47 @@ def test_screened_search_short():
     # a selection of 35% of the offspring drawn at random recalls 35% of the top quarter on average
     print("recall@25%", [r.recall_top25 for r in reps[1:]], "MAE", [r.mae_eV_atom for r in reps[1:]], flush=True)
     assert rec > 0.35
+
+
+def test_capped_relaxation_carries_the_tail_over():
+    """With a FIRE cap per generation, unconverged structures move into the next generation's batch instead of
+    holding every generation to the slowest structure; nothing is lost except past max_steps in total."""
+    from maise_amd.evolve import GAConfig, SurrogateGA
+    print("[gpu case] GA with relax_cap 60, Mg16Ca8, pop 64, 4 generations", flush=True)
+    cfg = GAConfig(pop=64, generations=4, seed=23, relax_cap=60, max_steps=400, audit=False)
+    ga = SurrogateGA(MGCA, cfg)
+    res = ga.run()
+    ga.close()
+    reps = res.reports
+    assert any(r.carried > 0 for r in reps)                  # 60 steps leave some structures unconverged
+    assert all(np.isfinite(r.best_eV_atom) for r in reps)
+    assert all(reps[k + 1].best_eV_atom <= reps[k].best_eV_atom + 1e-12 for k in range(4))
+    assert np.isfinite(res.population_E).all()
+
+
+def test_screening_at_equal_search_time():
+    """The same seeded search (population 512, as the search workload) with and without the surrogate for the
+    same search clock.  What screening delivers by construction is asserted: 35% of the offspring relaxed, and so
+    more generations per second at this population (a batch of 179 structures of 24 atoms relaxes in about 60% of
+    the time of 512); the audited run above asserts a recall above random selection's.  Which search reaches the
+    lower energy first is measured and printed, not asserted: it depends on the seed, and on this GPU the
+    unscreened search has reached the lower energy first in most seeds measured, because relaxation time grows
+    far less than linearly with the batch below a few thousand structures."""
+    from maise_amd.evolve import GAConfig, compare_screening
+    print("[gpu case] equal-time search, Mg16Ca8, pop 512, 1 seed x 4 s", flush=True)
+    cfg = GAConfig(pop=512, relax_cap=300)
+    out = compare_screening(MGCA, cfg, 4.0, [31],
+                            variants={"unscreened": {"screen": False}, "screened": {"screen": True}})
+    for name, s in out["summary"].items():
+        print(name, s, flush=True)
+    for name, rr in out["runs"].items():
+        for r in rr:
+            print(name, r["seed"], "generations", r["generations"], "best", round(r["best_eV_atom"], 5),
+                  "seconds to target", r["seconds_to_target"], flush=True)
+    su, ss = out["summary"]["unscreened"], out["summary"]["screened"]
+    assert ss["generations_mean"] > su["generations_mean"]
+    for r in out["runs"]["screened"]:
+        assert r["relaxed_per_generation"] == round(0.25 * 512) + round(0.10 * 512)
+    for name in ("unscreened", "screened"):
+        for r in out["runs"][name]:
+            assert r["generations"] >= 3 and np.isfinite(r["best_eV_atom"])

@@ -69,18 +69,19 @@ def test_oracle_adam_first_step_closed_form():
 
 def test_oracle_features_are_species_means():
     rng = np.random.default_rng(3)
-    ng2, ng4, D = 3, 4, 20
+    D = 20
     atom_off = [0, 5, 9, 12]
-    spc = np.array([0, 0, 1, 1, 1, 0, 0, 0, 0, 1, 1, 1])
+    spc = np.array([0, 0, 1, 1, 1, 0, 0, 0, 0, 1, 1, 1])   # the third structure has no atom of species 0
     G = rng.normal(size=(12, D))
-    f = so.features(G, spc, atom_off, 2, ng2, ng4)
-    cols = np.r_[0:ng2, ng2:ng2 + ng4]
+    f = so.features(G, spc, atom_off, 2)
+    assert f.shape == (3, 2 * D)
     for s in range(3):
         a, b = atom_off[s], atom_off[s + 1]
         for q in range(2):
             sel = spc[a:b] == q
-            exp = G[a:b][sel][:, cols].sum(axis=0) / max(sel.sum(), 1)
-            assert np.allclose(f[s, q * 7:(q + 1) * 7], exp if sel.any() else 0.0)
+            exp = G[a:b][sel].sum(axis=0) / max(sel.sum(), 1)
+            assert np.allclose(f[s, q * D:(q + 1) * D], exp if sel.any() else 0.0)
+    assert not f[2, :D].any()
 
 
 # ------------------------------------------------------------------ evolutionary operators
