@@ -173,11 +173,21 @@ def relax_leg(maise_amd, batch, args):
     c = maise_amd.Context(model, device=0, urep=100.0)
     c.set_batch(batch[:64])
     c.relax(max_steps=50, fmax=fmax)           # warm-up: code objects, allocations
+    c.set_autotune(True)                       # tuned on the full batch before the first capture (timed)
     t0 = time.perf_counter()
     c.set_batch(batch)
     steps, done = c.relax(max_steps=max_steps, fmax=fmax)
     dt = time.perf_counter() - t0
+    shape = c.shape()
+    log = c.relax_log()
     c.close()
+    # wall time by the share of structures still active in each segment between host checks
+    S0 = float(len(batch))
+    split = {}
+    for name, lo, hi in (("active_ge_50pct", 0.5, 2.0), ("active_10_50pct", 0.1, 0.5), ("active_lt_10pct", 0.0, 0.1)):
+        sel = (log[:, 1] / S0 >= lo) & (log[:, 1] / S0 < hi)
+        split[name] = {"seconds": float(log[sel, 3].sum()),
+                       "iterations": int(np.diff(np.concatenate([[0], log[:, 0]]))[sel].sum())}
     nsingle = min(args.relax_single, len(batch))
     c1 = maise_amd.Context(model, device=0, urep=100.0)
     t1 = time.perf_counter()
@@ -194,6 +204,10 @@ def relax_leg(maise_amd, batch, args):
             "relaxed_structures_per_s": float(done.sum()) / dt, "seconds": dt,
             "converged": int(done.sum()), "structures": len(batch),
             "fire_steps_mean": float(steps.mean()), "fire_steps_max": int(steps.max()),
+            "autotuned_shape": {"nbr_wpb": shape[0], "desc_wpb": shape[1], "force_wpb": shape[2],
+                                "pair_store": shape[3]},
+            "wall_by_active_share": split,
+            "tail_share_of_wall": split["active_lt_10pct"]["seconds"] / max(float(log[:, 3].sum()), 1e-12),
             "single_structure_path": {"structures": nsingle, "seconds": dt1,
                                       "relaxed_structures_per_s": nsingle / dt1,
                                       "steps_equal_batched": bool(np.array_equal(np.array(s1), steps[:nsingle]))},

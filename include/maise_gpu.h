@@ -128,9 +128,20 @@ int mg_ctx_set_graph(mg_ctx *c, int enable, double headroom);
  * not depend on this setting.  No counterpart in the reference (PRS_BP enumerates the triplets once per call,
  * src/nprs.c:515-551, and derives forces in the same pass). */
 int mg_ctx_set_pair_store(mg_ctx *c, int enable);
-/* Autotuner: times 1, 2 and 4 waves per workgroup on the structure/batch currently set (reps evaluations each)
- * and keeps the fastest shape per kernel; wpb_out[2] (may be NULL) receives (descriptor, force). */
+/* Autotuner (no counterpart in the reference, which runs one CPU code path): on the structure/batch currently
+ * set, reps timed evaluations per candidate, it picks 1, 2 or 4 waves per workgroup for the neighbour fill, the
+ * descriptor and the force kernel (each its fastest), then stored pair lists or enumeration by the descriptor +
+ * force time.  No choice changes a result bit.  wpb_out[2] (may be NULL) receives (descriptor, force). */
 int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out);
+/* The same; shape[4] receives (neighbour fill wpb, descriptor wpb, force wpb, pair store). */
+int mg_ctx_autotune_shape(mg_ctx *c, int reps, int *shape);
+/* Current launch shape: shape[4] = (neighbour fill wpb, descriptor wpb, force wpb, pair store); 0 = automatic. */
+int mg_ctx_get_shape(mg_ctx *c, int *shape);
+/* Waves per workgroup of the neighbour fill: 1, 2, 4, or 0 (4, fewer when the LDS rows of long lists need it). */
+int mg_ctx_set_wpb_nbr(mg_ctx *c, int wpb);
+/* on != 0: mg_relax, mg_relax_cell and mg_md_run run the autotuner (3 reps) before their first captured iteration,
+ * once per structure/batch set (evaluations at unchanged positions, so the trajectory is the same). */
+int mg_ctx_set_autotune(mg_ctx *c, int on);
 
 /* --- screening surrogate (no counterpart in the reference; its training derivative follows DE_ANN_PARA,
  * src/nmlp.c:292-354, and its training loop replaces the GSL minimiser of src/nmin.c:361-432 with Adam) ---
@@ -206,6 +217,10 @@ int mg_md_run_npt(mg_ctx *c, int nsteps, double dt, double T, double tau, double
  * per-structure displacement norm of one step.  steps[S] and done[S] (NULL skips) report iterations and
  * convergence; the final positions/energies/forces are read with mg_get / mg_get_batch. */
 int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done);
+/* Wall time of the last relaxation by segment between its host checks: up to max records of 4 doubles
+ * (iterations done at the segment's end, active structures, active atoms, seconds) go to rec; returns the number
+ * of records there are.  No counterpart in the reference. */
+int mg_relax_log(mg_ctx *c, int max, double *rec);
 /* Full-cell relaxation (the reference's RLXT 3, CELL_RELX/CELL_MIN src/cmod.c:901-977, src/cmin.c:90-166: enthalpy
  * H = E + pV over positions and lattice, there with GSL BFGS2): batched FIRE over positions and the cell strain
  * (generalized force (U - pV 1)/N), to every |F| < fmax (eV/A) and every |U - pV 1|/V < smax (GPa), at pressure

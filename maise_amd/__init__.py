@@ -55,6 +55,11 @@ def lib():
             "mg_ctx_set_timing": ([vp, C.c_int], C.c_int),
             "mg_ctx_set_wpb": ([vp, C.c_int, C.c_int], C.c_int),
             "mg_ctx_autotune": ([vp, C.c_int, ip], C.c_int),
+            "mg_ctx_autotune_shape": ([vp, C.c_int, ip], C.c_int),
+            "mg_ctx_get_shape": ([vp, ip], C.c_int),
+            "mg_ctx_set_wpb_nbr": ([vp, C.c_int], C.c_int),
+            "mg_ctx_set_autotune": ([vp, C.c_int], C.c_int),
+            "mg_relax_log": ([vp, C.c_int, dp], C.c_int),
             "mg_ctx_set_graph": ([vp, C.c_int, C.c_double], C.c_int),
             "mg_ctx_set_pair_store": ([vp, C.c_int], C.c_int),
             "mg_set_structure": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp], C.c_int),
@@ -229,10 +234,24 @@ class Context:
         _check(lib().mg_ctx_set_pair_store(self._h, 1 if enable else 0), "mg_ctx_set_pair_store")
 
     def autotune(self, reps: int = 3):
-        """Time 1/2/4 waves per workgroup on the current structure or batch; keep the fastest per kernel."""
-        out = np.zeros(2, dtype=np.int32)
-        _check(lib().mg_ctx_autotune(self._h, int(reps), _ip(out)), "mg_ctx_autotune")
-        return int(out[0]), int(out[1])
+        """Tune the launch shape on the current structure or batch (neighbour fill, descriptor and force waves per
+        workgroup, stored pair lists or enumeration); returns (nbr, desc, force, pair_store)."""
+        out = np.zeros(4, dtype=np.int32)
+        _check(lib().mg_ctx_autotune_shape(self._h, int(reps), _ip(out)), "mg_ctx_autotune_shape")
+        return tuple(int(v) for v in out)
+
+    def shape(self):
+        """Current launch shape (nbr, desc, force waves per workgroup, pair_store); 0 = automatic."""
+        out = np.zeros(4, dtype=np.int32)
+        _check(lib().mg_ctx_get_shape(self._h, _ip(out)), "mg_ctx_get_shape")
+        return tuple(int(v) for v in out)
+
+    def set_wpb_nbr(self, wpb: int = 0):
+        _check(lib().mg_ctx_set_wpb_nbr(self._h, int(wpb)), "mg_ctx_set_wpb_nbr")
+
+    def set_autotune(self, on: bool = True):
+        """Relaxation and MD tune the launch shape before their first captured iteration (results unchanged)."""
+        _check(lib().mg_ctx_set_autotune(self._h, 1 if on else 0), "mg_ctx_set_autotune")
 
     def set_structure(self, spc, lat, x, nd: int = 3, xt: int = 1):
         spc = np.ascontiguousarray(spc, dtype=np.int32)
@@ -365,6 +384,16 @@ class Context:
         _check(lib().mg_relax(self._h, int(max_steps), float(fmax), float(dt0), float(dtmax), float(maxstep),
                               _ip(steps), _ip(done)), "mg_relax")
         return steps, done.astype(bool)
+
+    def relax_log(self) -> np.ndarray:
+        """The last relaxation's wall time by segment between host checks: rows of (iterations done, active
+        structures, active atoms, seconds)."""
+        n = lib().mg_relax_log(self._h, 0, None)
+        if n < 0:
+            _check(n, "mg_relax_log")
+        rec = np.zeros(4 * max(n, 1))
+        _check(min(0, lib().mg_relax_log(self._h, n, _dp(rec))), "mg_relax_log")
+        return rec[:4 * n].reshape(n, 4)
 
     def relax_cell(self, p_gpa: float = 0.0, max_steps: int = 3000, fmax: float = 0.01, smax_gpa: float = 0.01,
                    dt0: float = 0.1, dtmax: float = 1.0, maxstep: float = 0.2):

@@ -228,6 +228,7 @@ void mg_ctx_free(mg_ctx *c) {
   if (c->h_flags) hipHostFree(c->h_flags);
   if (c->st) hipStreamDestroy(c->st);
   free(c->h_str); free(c->h_atom_off); free(c->h_nd); free(c->h_lat);
+  free(c->rlog);
   free(c);
 }
 
@@ -269,19 +270,48 @@ int mg_ctx_set_wpb(mg_ctx *c, int wpb_desc, int wpb_force) {
   return MG_OK;
 }
 
-/* Time the descriptor and force kernels at 1, 2 and 4 waves per block on the structure (or batch) now set,
- * reps evaluations each, and keep the fastest per kernel.  Candidates whose LDS does not fit are skipped. */
-int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out) {
+int mg_ctx_set_wpb_nbr(mg_ctx *c, int wpb) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  if (wpb != 0 && wpb != 1 && wpb != 2 && wpb != 4)
+    return mg_fail(MG_ERR_ARG, "waves per block must be 0 (automatic), 1, 2 or 4");
+  c->wpb_nbr = wpb;
+  return MG_OK;
+}
+
+int mg_ctx_set_autotune(mg_ctx *c, int on) {
+  if (!c) return mg_fail(MG_ERR_ARG, "null context");
+  c->autotune = on != 0;
+  c->tuned = 0;
+  return MG_OK;
+}
+
+int mg_ctx_get_shape(mg_ctx *c, int *shape) {
+  if (!c || !shape) return mg_fail(MG_ERR_ARG, "mg_ctx_get_shape: bad argument");
+  shape[0] = c->wpb_nbr;
+  shape[1] = c->wpb_desc;
+  shape[2] = c->wpb_force;
+  shape[3] = c->pair_store;
+  return MG_OK;
+}
+
+/* Launch-shape autotuner on the structure (or batch) now set, reps timed evaluations per candidate after one
+ * untimed: (1) 1, 2 and 4 waves per block for the neighbour fill, the descriptor and the force kernel, each
+ * kernel keeping its fastest (the stages are timed separately, so one sweep serves all three); (2) with those,
+ * the force kernel reading the descriptor kernel's stored pair lists against enumerating them itself, by the
+ * descriptor + force time.  Candidates that do not fit (LDS, pair-list budget) are skipped.  None of these
+ * choices changes a result bit: every atom's arithmetic is the same whatever the block it runs in, and both pair
+ * sources visit the same pairs in the same order. */
+static int autotune_impl(mg_ctx *c, int reps, int *shape) {
   if (!c) return mg_fail(MG_ERR_ARG, "null context");
   if (c->S == 0) return mg_fail(MG_ERR_ARG, "mg_ctx_autotune: no structure set");
   if (reps < 1) reps = 1;
   const int cand[3] = {1, 2, 4};
-  const int timing0 = c->timing, wd0 = c->wpb_desc, wf0 = c->wpb_force;
-  float best_d = 1e30f, best_f = 1e30f, t[6];
-  int wd = 0, wf = 0, r;
+  const int timing0 = c->timing, wn0 = c->wpb_nbr, wd0 = c->wpb_desc, wf0 = c->wpb_force, ps0 = c->pair_store;
+  float best_n = 1e30f, best_d = 1e30f, best_f = 1e30f, t[6];
+  int wn = 0, wd = 0, wf = 0, r;
   if ((r = mg_ctx_set_timing(c, 1)) != MG_OK) return r;
   for (int k = 0; k < 3; k++) {
-    c->wpb_desc = c->wpb_force = cand[k];
+    c->wpb_nbr = c->wpb_desc = c->wpb_force = cand[k];
     if (mgh_eval(c, MG_EVAL_FORCES, 0) != MG_OK) continue;  /* LDS does not fit: skip this shape */
     if ((r = mg_get_timing(c, t)) != MG_OK) return r;       /* drop the warm-up evaluation */
     int ok = 1;
@@ -289,19 +319,53 @@ int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out) {
       if (mgh_eval(c, MG_EVAL_FORCES, 0) != MG_OK) ok = 0;
     if (!ok) continue;
     if ((r = mg_get_timing(c, t)) != MG_OK) return r;       /* stage means over the reps */
+    if (t[0] < best_n) { best_n = t[0]; wn = cand[k]; }
     if (t[1] < best_d) { best_d = t[1]; wd = cand[k]; }
     if (t[3] < best_f) { best_f = t[3]; wf = cand[k]; }
   }
-  if ((r = mg_ctx_set_timing(c, timing0)) != MG_OK) return r;
-  if (!wd || !wf) {
+  if (!wn || !wd || !wf) {
+    c->wpb_nbr = wn0;
     c->wpb_desc = wd0;
     c->wpb_force = wf0;
+    mg_ctx_set_timing(c, timing0);
     return mg_fail(MG_ERR_HIP, "mg_ctx_autotune: no launch shape ran");
   }
+  c->wpb_nbr = wn;
   c->wpb_desc = wd;
   c->wpb_force = wf;
-  if (wpb_out) { wpb_out[0] = wd; wpb_out[1] = wf; }
+  /* stored pair lists against enumeration, at the chosen shape */
+  float tp[2] = {1e30f, 1e30f};
+  for (int store = 0; store < 2; store++) {
+    c->pair_store = store;
+    if (mgh_eval(c, MG_EVAL_FORCES, 0) != MG_OK) continue;
+    if ((r = mg_get_timing(c, t)) != MG_OK) return r;
+    int ok = 1;
+    for (int q = 0; q < reps && ok; q++)
+      if (mgh_eval(c, MG_EVAL_FORCES, 0) != MG_OK) ok = 0;
+    if (!ok) continue;
+    if ((r = mg_get_timing(c, t)) != MG_OK) return r;
+    tp[store] = t[1] + t[3];
+  }
+  c->pair_store = tp[1] < 1e29f && tp[1] <= tp[0] ? 1 : (tp[0] < 1e29f ? 0 : ps0);
+  if ((r = mg_ctx_set_timing(c, timing0)) != MG_OK) return r;
+  c->tuned = 1;
+  if (shape) mg_ctx_get_shape(c, shape);
   return MG_OK;
+}
+
+int mg_ctx_autotune(mg_ctx *c, int reps, int *wpb_out) {
+  int shape[4];
+  const int r = autotune_impl(c, reps, shape);
+  if (r == MG_OK && wpb_out) { wpb_out[0] = shape[1]; wpb_out[1] = shape[2]; }
+  return r;
+}
+
+int mg_ctx_autotune_shape(mg_ctx *c, int reps, int *shape) { return autotune_impl(c, reps, shape); }
+
+/* called by relaxation and MD before their first capture: tune once per context batch when asked to */
+int mgh_autotune_once(mg_ctx *c) {
+  if (!c->autotune || c->tuned) return MG_OK;
+  return autotune_impl(c, 3, NULL);
 }
 
 /* reciprocal rows R = (L1xL2, L2xL0, L0xL1) / (L0.(L1xL2)) with the reference's operation order
@@ -473,6 +537,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
 int mg_set_structure(mg_ctx *c, int N, int nd, int xt, const int *spc, const double *lat, const double *x) {
   if (!c || N < 0 || (N > 0 && (!spc || !x)) || !lat) return mg_fail(MG_ERR_ARG, "mg_set_structure: bad argument");
   int off[2] = {0, N};
+  c->tuned = 0;  /* a new workload: tune again when asked to */
   return mgh_upload(c, 1, off, &nd, &xt, lat, spc, x);
 }
 
@@ -480,6 +545,7 @@ int mg_set_batch(mg_ctx *c, int S, const int *atom_off, const int *nd, const dou
   if (!c || S < 1 || !atom_off || !nd || !lat || !spc || !x) return mg_fail(MG_ERR_ARG, "mg_set_batch: bad argument");
   int *xt = (int *)malloc(sizeof(int) * S);
   for (int s = 0; s < S; s++) xt[s] = 1;
+  c->tuned = 0;
   int r = mgh_upload(c, S, atom_off, nd, xt, lat, spc, x);
   free(xt);
   return r;
@@ -630,13 +696,15 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     pstride = forces ? ensure_pairs(c, cap_row + (c->graph_headroom > 0.0 ? 8 : 0), cap_row) : 0;
     KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->nedges, cap_row, st));
     KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
-                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, 0, st));
+                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, 0, c->wpb_nbr,
+                      st));
   } else {
     /* capacities known: one pass, row i at i * gcap_row (the edge buffers hold N * gcap_row slots) */
     cap_row = c->gcap_row;
     pstride = forces ? c->gpstride : 0;
     KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
-                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, 1, st));
+                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, 1, c->wpb_nbr,
+                      st));
   }
   if (timing) HIPCHK(hipEventRecord(ev[EV_NBR], st));
   /* descriptors, MLP, forces */

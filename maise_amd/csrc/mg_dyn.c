@@ -9,6 +9,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "mg_host.h"
 
@@ -190,6 +191,7 @@ static int md_impl(mg_ctx *c, int nsteps, double dt, double T, double tau, int b
       MCHK(md_step_plain(c, &a));
       s++;
       if (c->graph_enable && s < nsteps) {
+        MCHK(mgh_autotune_once(c));  /* evaluations at the same positions: the state does not change */
         MCHK(mgh_graph_caps(c, c->nedges > need_e ? c->nedges : need_e, c->maxrow > need_r ? c->maxrow : need_r));
         MCHK(graph_capture(c, md_head, md_tail, &a, &gx));
         MCHK(snapshot(c, 0));
@@ -316,6 +318,32 @@ static int fire_tail(mg_ctx *c, void *arg) {
   return e ? mg_fail(MG_ERR_HIP, "kernel launch failed: FIRE step (%s)", mgk_last_error()) : MG_OK;
 }
 
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* one record of the relaxation log (grown on demand; a failed allocation only stops the log) */
+static void rlog_add(mg_ctx *c, int it, int S, int N, double sec) {
+  if (c->rlog_n == c->rlog_cap) {
+    const int cap = c->rlog_cap ? 2 * c->rlog_cap : 256;
+    double *p = (double *)realloc(c->rlog, sizeof(double) * 4 * (size_t)cap);
+    if (!p) return;
+    c->rlog = p;
+    c->rlog_cap = cap;
+  }
+  double *r = c->rlog + 4 * (size_t)c->rlog_n++;
+  r[0] = it; r[1] = S; r[2] = N; r[3] = sec;
+}
+
+int mg_relax_log(mg_ctx *c, int max, double *rec) {
+  if (!c || max < 0 || (max > 0 && !rec)) return mg_fail(MG_ERR_ARG, "mg_relax_log: bad argument");
+  const int n = c->rlog_n < max ? c->rlog_n : max;
+  if (n) memcpy(rec, c->rlog, sizeof(double) * 4 * (size_t)n);
+  return c->rlog_n;
+}
+
 typedef struct {
   int S0, N0;
   int *off0, *nd0, *spc0;
@@ -428,6 +456,8 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
   RHIP(hipMemsetAsync(c->d_cellv, 0, sizeof(double) * 9 * c->S, c->st));
   RKCHK(mgk_fire_init(c->S, c->d_fire, dt0, c->st));
   int it = 0, need_e = 0, need_r = 0;
+  c->rlog_n = 0;
+  double t_seg = now_s();
   while (it < max_steps) {
     int nrun;
     if (c->graph_enable && gx) {
@@ -447,6 +477,7 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
       }
       if (c->graph_enable) {
         it += nrun;
+        RCHK(mgh_autotune_once(c));  /* before the capacities and the capture, which depend on the shape */
         RCHK(mgh_graph_caps(c, c->nedges > need_e ? c->nedges : need_e, c->maxrow > need_r ? c->maxrow : need_r));
         RCHK(graph_capture(c, NULL, fire_tail, &fa, &gx));
         RCHK(snapshot(c, 0));
@@ -460,6 +491,11 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
     RKCHK(mgk_fire_count(c->S, c->d_fire, c->d_misc + 2, c->st));
     RHIP(hipMemcpyAsync(flags, c->d_misc, sizeof flags, hipMemcpyDeviceToHost, c->st));
     RHIP(hipStreamSynchronize(c->st));
+    {
+      const double t1 = now_s();
+      rlog_add(c, it, c->S, c->N, t1 - t_seg);
+      t_seg = t1;
+    }
     if (flags[3]) {
       /* a row or the edge total outgrew the capture: redo these iterations with larger buffers */
       RCHK(snapshot(c, 1));

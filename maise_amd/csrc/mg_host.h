@@ -52,6 +52,9 @@ struct mg_ctx {
   int prec;
   int timing;
   int wpb_desc, wpb_force; /* waves (atoms) per block of the descriptor and force kernels; 0 = by LDS size */
+  int wpb_nbr;              /* waves (atoms) per block of the neighbour fill; 0 = 4, fewer when LDS is short */
+  int autotune;             /* relaxation / MD tune the launch shape before their first capture */
+  int tuned;                /* the shape was tuned on this context's batch */
   int gcap_edges, gcap_row; /* graph-mode capacities: edges allocated, longest row the LDS tiles hold */
   int graph_enable;         /* relaxation / MD replay a captured iteration (default 1) */
   double graph_headroom;    /* fractional headroom of the graph-mode capacities (default 1/8) */
@@ -122,6 +125,10 @@ struct mg_ctx {
   double *d_snap_x, *d_snap_v, *d_snap_fire, *d_snap_dyn, *d_snap_cellv; /* state at the last graph checkpoint */
   mg_struct_dev *d_snap_str;                                             /* (lattices change in cell relaxation) */
   size_t cap_dyn_atoms, cap_dyn_str, cap_trace;
+  /* last relaxation's wall time by segment between host checks: (iterations done, active structures, active
+   * atoms, seconds) per record */
+  double *rlog;
+  int rlog_n, rlog_cap;
 };
 
 
@@ -138,9 +145,10 @@ int mgk_bins(int N, int nbins, const int *bin, const int *bin_start, int *bin_co
              void *brec, hipStream_t st);
 int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
                   const int *bin_start, const void *brec, int *cnt, int *rowcnt, int *misc, hipStream_t st);
+int mgh_autotune_once(struct mg_ctx *c);
 int mgk_nbr_fill(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
                  const int *bin_start, const void *brec, int *row_off, int *rowcnt, int *cnt, void *key, void *vec,
-                 int *rev, uint16_t *rhash, int *misc, int maxrow, int padded, hipStream_t st);
+                 int *rev, uint16_t *rhash, int *misc, int maxrow, int padded, int wpb, hipStream_t st);
 int mgk_rev_slots(int maxrow);
 int mgk_desc(int N, int S, int variant, const int *spc, const int *row_off, const int *rowcnt, const int *cnt,
              const void *vec,

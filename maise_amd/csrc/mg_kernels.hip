@@ -1599,7 +1599,7 @@ int mgk_rev_slots(int maxrow) {
 
 int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
                  const int* bin_start, const void* brec, int* row_off, int* rowcnt, int* cnt, void* key, void* vec,
-                 int* rev, uint16_t* rhash, int* misc, int maxrow, int padded, hipStream_t st) {
+                 int* rev, uint16_t* rhash, int* misc, int maxrow, int padded, int wpb_req, hipStream_t st) {
   if (N == 0) return 0;
   int cap = 64;
   while (cap < maxrow) cap <<= 1;
@@ -1609,7 +1609,8 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
   const int hslots = mgk_rev_slots(maxrow);
   const size_t per_wave = NBR_LDS_CHUNK + (size_t)cap * (sizeof(u64) + sizeof(float4)) + (size_t)hslots * sizeof(uint16_t);
   const size_t budget = 64 * 1024;
-  int wpb = NBR_WPB;
+  // wpb_req 1, 2 or 4 asks for that many waves per block (fewer when they would not fit); 0 starts from NBR_WPB
+  int wpb = (wpb_req == 1 || wpb_req == 2 || wpb_req == 4) ? wpb_req : NBR_WPB;
   while (wpb > 1 && (size_t)wpb * per_wave > budget) wpb >>= 1;
   const size_t sm = (size_t)wpb * per_wave;
   if (sm > budget &&
