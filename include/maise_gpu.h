@@ -139,6 +139,14 @@ int mg_ctx_autotune_shape(mg_ctx *c, int reps, int *shape);
 int mg_ctx_get_shape(mg_ctx *c, int *shape);
 /* Waves per workgroup of the neighbour fill: 1, 2, 4, or 0 (4, fewer when the LDS rows of long lists need it). */
 int mg_ctx_set_wpb_nbr(mg_ctx *c, int wpb);
+/* Skin neighbour lists for fixed-cell runs (mg_relax, mg_md_run without a barostat) in graph mode: candidates
+ * within rc + skin (A, 0 = off, the default; at most 4), rebuilt when an atom has moved more than skin/2 since their
+ * build; every step applies LIST's own FP64 test (src/cell.c:109-123) to the candidates, so the neighbour rows,
+ * and with them every result, are bit-identical to building them from scratch each step.  No counterpart in the
+ * reference, which rebuilds its lists every call. */
+int mg_ctx_set_skin(mg_ctx *c, double skin);
+/* Skin-candidate builds over the context's life (fixed-cell runs with skin lists). */
+int mg_ctx_skin_builds(mg_ctx *c, int *builds);
 /* on != 0: mg_relax, mg_relax_cell and mg_md_run run the autotuner (3 reps) before their first captured iteration,
  * once per structure/batch set (evaluations at unchanged positions, so the trajectory is the same). */
 int mg_ctx_set_autotune(mg_ctx *c, int on);

@@ -60,6 +60,8 @@ def lib():
             "mg_ctx_set_wpb_nbr": ([vp, C.c_int], C.c_int),
             "mg_ctx_set_autotune": ([vp, C.c_int], C.c_int),
             "mg_relax_log": ([vp, C.c_int, dp], C.c_int),
+            "mg_ctx_set_skin": ([vp, C.c_double], C.c_int),
+            "mg_ctx_skin_builds": ([vp, ip], C.c_int),
             "mg_ctx_set_graph": ([vp, C.c_int, C.c_double], C.c_int),
             "mg_ctx_set_pair_store": ([vp, C.c_int], C.c_int),
             "mg_set_structure": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp], C.c_int),
@@ -248,6 +250,18 @@ class Context:
 
     def set_wpb_nbr(self, wpb: int = 0):
         _check(lib().mg_ctx_set_wpb_nbr(self._h, int(wpb)), "mg_ctx_set_wpb_nbr")
+
+    def set_skin(self, skin: float = 1.0):
+        """Skin neighbour lists (A; 0 = off) for fixed-cell relaxation and MD: candidates within rc + skin are kept
+        until an atom has moved skin/2, and every step applies the reference's FP64 neighbour test to them, so the
+        results are bit-identical to rebuilding the lists every step."""
+        _check(lib().mg_ctx_set_skin(self._h, float(skin)), "mg_ctx_set_skin")
+
+    def skin_builds(self) -> int:
+        """Skin-candidate builds over the context's life."""
+        n = np.zeros(1, dtype=np.int32)
+        _check(lib().mg_ctx_skin_builds(self._h, _ip(n)), "mg_ctx_skin_builds")
+        return int(n[0])
 
     def set_autotune(self, on: bool = True):
         """Relaxation and MD tune the launch shape before their first captured iteration (results unchanged)."""

@@ -187,10 +187,11 @@ int mg_ctx_create(const mg_model *m, int device, mg_ctx **out) {
   int r;
   if ((r = mg_dev_alloc((void **)&c->d_basis, sizeof(mg_basis_dev))) ||
       (r = mg_dev_alloc((void **)&c->d_mlp, (size_t)m->wstride * m->nspc * sizeof(float))) ||
-      (r = mg_dev_alloc((void **)&c->d_misc, 16 * sizeof(int)))) {
+      (r = mg_dev_alloc((void **)&c->d_misc, 32 * sizeof(int)))) {
     mg_ctx_free(c);
     return r;
   }
+  HIPCHK(hipMemset(c->d_misc, 0, 32 * sizeof(int)));  /* misc[16..31] are never cleared by the chains */
   HIPCHK(hipMemcpy(c->d_basis, &m->basis, sizeof(mg_basis_dev), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_mlp, m->mlp, (size_t)m->wstride * m->nspc * sizeof(float), hipMemcpyHostToDevice));
   if (m->mfma_ok) {
@@ -218,7 +219,8 @@ void mg_ctx_free(mg_ctx *c) {
                   c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
                   c->d_vir, c->d_fself, c->d_fe, c->d_gpl, c->d_npl, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
-                  c->d_snap_dyn, c->d_cellv, c->d_snap_cellv, c->d_snap_str};
+                  c->d_snap_dyn, c->d_cellv, c->d_snap_cellv, c->d_snap_str, c->d_ckey, c->d_crow, c->d_xb,
+                  c->d_mimg};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
     if (ptrs[k]) hipFree(ptrs[k]);
   if (c->st) hipStreamSynchronize(c->st);
@@ -282,6 +284,28 @@ int mg_ctx_set_autotune(mg_ctx *c, int on) {
   if (!c) return mg_fail(MG_ERR_ARG, "null context");
   c->autotune = on != 0;
   c->tuned = 0;
+  return MG_OK;
+}
+
+static void plan_bins(mg_struct_dev *s, double rc, double skin, int natom);
+
+int mg_ctx_set_skin(mg_ctx *c, double skin) {
+  if (!c || !(skin >= 0.0) || skin > 4.0) return mg_fail(MG_ERR_ARG, "mg_ctx_set_skin: skin must be 0..4 A");
+  if (c->no_alloc && skin > 0.0) return mg_fail(MG_ERR_ARG, "mg_ctx_set_skin: not with the fixed scratch");
+  c->skin = skin;
+  c->ccap_row = 0;
+  if (c->S > 0) {  /* the skin stencils of the structures set */
+    for (int s = 0; s < c->S; s++) plan_bins(&c->h_str[s], c->m->rc_list, skin, c->h_str[s].natom);
+    HIPCHK(hipMemcpy(c->d_str, c->h_str, c->S * sizeof(mg_struct_dev), hipMemcpyHostToDevice));
+  }
+  return MG_OK;
+}
+
+int mg_ctx_skin_builds(mg_ctx *c, int *builds) {
+  if (!c || !builds) return mg_fail(MG_ERR_ARG, "mg_ctx_skin_builds: bad argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(builds, c->d_misc + 16, sizeof(int), hipMemcpyDeviceToHost));
   return MG_OK;
 }
 
@@ -395,11 +419,12 @@ static double cell_volume(const double *L) { /* src/cutl.c:178-188 */
 /* bins and stencil of one structure: bins at least rc/2 thick (perpendicular to each lattice plane), stencil
  * half width K_q >= rc |R_q| nb_q so every pair within rc lies in the stencil (the neighbour kernels skip the
  * stencil bins that lie farther than rc from the atom). */
-static void plan_bins(mg_struct_dev *s, double rc, int natom) {
+static void plan_bins(mg_struct_dev *s, double rc, double skin, int natom) {
   const double eps = 1e-6;
   if (s->nd == 0) {
     s->nb[0] = s->nb[1] = s->nb[2] = 1;
     s->K[0] = s->K[1] = s->K[2] = 0;
+    s->Ks[0] = s->Ks[1] = s->Ks[2] = 0;
   } else {
     double h[3];
     for (int q = 0; q < 3; q++) {
@@ -418,6 +443,9 @@ static void plan_bins(mg_struct_dev *s, double rc, int natom) {
     for (int q = 0; q < 3; q++) {
       int K = (int)ceil(h[q] * s->nb[q]);
       s->K[q] = K < 1 ? 1 : K;
+      /* the skin-candidate radius rc + skin needs a stencil that much wider */
+      int Ks = (int)ceil(h[q] * (rc + skin) / rc * s->nb[q]);
+      s->Ks[q] = Ks < s->K[q] ? s->K[q] : Ks;
     }
   }
   s->nbins = s->nb[0] * s->nb[1] * s->nb[2];
@@ -452,7 +480,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
     d->nd = nd[s];
     d->xt = xt[s];
     if (d->nd != 0 && d->nd != 3) { hfree(c, atom_str); return mg_fail(MG_ERR_ARG, "structure %d: ND must be 0 or 3", s); }
-    plan_bins(d, rc, d->natom);
+    plan_bins(d, rc, c->skin, d->natom);
     d->bin0 = bin0;
     bin0 += d->nbins;
     d->chunk0 = nchunk;
@@ -531,6 +559,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
   c->have_forces = c->have_g = 0;
   c->gcap_edges = c->gcap_row = 0;  /* capacities are sized again by the next synchronised evaluation */
   c->gpstride = 0;
+  c->ccap_row = 0;                   /* and the skin candidates with them */
   return MG_OK;
 }
 
@@ -583,6 +612,30 @@ int mgh_graph_caps(mg_ctx *c, int need_edges, int need_row) {
   c->gcap_edges = (int)ne;
   c->gcap_row = row;
   c->gpstride = ensure_pairs(c, c->gcap_row, c->gcap_row);
+  if (c->skin_on) {
+    /* candidate rows: the neighbour rows scaled by the volume of the larger sphere, with headroom, or the longest
+     * candidate row an overflowed step has seen */
+    const double rc = c->m->rc_list, g = (rc + c->skin) / rc;
+    int cc = (int)ceil(need_row * g * g * g * (1.0 + (h > 0.0 ? h : 0.0))) + 16;
+    if (c->skin_need + 16 > cc) cc = c->skin_need + 16;
+    if (cc > 2048) cc = 2048;
+    const size_t nk = (size_t)c->N * (size_t)cc;
+    if (nk > c->cap_ckey) {
+      ALLOC(c->d_ckey, nk * sizeof(unsigned long long));
+      c->cap_ckey = nk;
+    }
+    if ((size_t)c->N > c->cap_skin_atoms) {
+      const size_t n = (size_t)c->N + 64;
+      ALLOC(c->d_crow, n * sizeof(int));
+      ALLOC(c->d_xb, 3 * n * sizeof(double));
+      ALLOC(c->d_mimg, n * sizeof(unsigned));
+      c->cap_skin_atoms = n;
+    }
+    c->ccap_row = cc;
+    /* candidate rows built before this point may have another stride (or lie in a reallocated buffer): mark the
+     * lists stale so the next chain builds them again */
+    if (mgk_clear(c->d_misc + 8, 1, NULL, 0, NULL, 0, c->st)) return mg_fail(MG_ERR_HIP, "misc clear failed");
+  }
   return MG_OK;
 }
 
@@ -665,8 +718,11 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     KCHK(mgk_clear(c->d_bin_count, c->nbins + 1, NULL, 0, NULL, 0, st));
     c->bins_dirty = 0;
   }
+  /* skin lists: graph-mode chains of fixed-cell runs; any other chain clears misc[0..15], which marks them stale */
+  const int skin = c->skin_on && c->ccap_row > 0 && mode != MG_CHAIN_SYNC;
+  const double half = 0.5 * c->skin * (1.0 - 1e-9);
   KCHK(mgk_prepare(N, c->d_xin, c->d_atom_str, c->d_str, c->d_x, c->d_bin, c->d_bin_count, c->d_misc,
-                   mode == MG_CHAIN_GRAPH ? 1 : 16, st));
+                   mode == MG_CHAIN_GRAPH ? 1 : 16, skin ? c->d_xb : NULL, c->d_mimg, half * half, st));
   KCHK(mgk_scan(c->d_bin_count, c->d_bin_start, c->nbins, c->d_tmp, &c->tmp_bytes, st));
   KCHK(mgk_bins(N, c->nbins, c->d_bin, c->d_bin_start, c->d_bin_count, c->d_x, c->d_spc, c->d_bin_rec, st));
   int cap_row;
@@ -697,6 +753,16 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     KCHK(mgk_check_caps(N, c->d_row_off, c->d_misc, c->nedges, cap_row, st));
     KCHK(mgk_nbr_fill(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_row_off,
                       c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, 0, c->wpb_nbr,
+                      st));
+  } else if (skin) {
+    /* capacities known, skin lists: candidates rebuilt when stale, then LIST's test on the candidates */
+    cap_row = c->gcap_row;
+    pstride = forces ? c->gpstride : 0;
+    KCHK(mgk_nbr_cand(N, c->d_x, c->d_spc, c->d_atom_str, c->d_str, c->d_bin, c->d_bin_start, c->d_bin_rec, c->d_crow,
+                      c->d_ckey, c->d_misc, c->ccap_row, c->skin, st));
+    KCHK(mgk_skin_commit(N, c->d_x, c->d_xb, c->d_mimg, c->d_misc, st));
+    KCHK(mgk_nbr_skin(N, c->d_x, c->d_atom_str, c->d_str, c->d_ckey, c->d_crow, c->ccap_row, c->d_mimg, c->d_row_off,
+                      c->d_rowcnt, c->d_cnt, c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_misc, cap_row, c->wpb_nbr,
                       st));
   } else {
     /* capacities known: one pass, row i at i * gcap_row (the edge buffers hold N * gcap_row slots) */
@@ -1029,7 +1095,7 @@ int mg_eval_static(const mg_model *m, int N, int nd, int xt, const int *spc, con
     CARVE(c->d_frag_lo, m->frag_bf_n * sizeof(uint16_t));
     CARVE(c->d_fvec, m->fvec_n * sizeof(float));
   }
-  CARVE(c->d_misc, 16 * sizeof(int));
+  CARVE(c->d_misc, 32 * sizeof(int));
   CARVE(c->d_str, sizeof(mg_struct_dev));
   CARVE(c->d_E, sizeof(double));
   CARVE(c->d_U, 6 * sizeof(double));

@@ -175,6 +175,8 @@ static int md_impl(mg_ctx *c, int nsteps, double dt, double T, double tau, int b
     c->cap_trace = ntr;
   }
   md_arg a = {dt, MG_KB * T, 3.0 * c->N, tau, nsteps, baro, p, ktt * dt};
+  c->skin_on = !baro && c->skin > 0.0;  /* a fixed cell: skin lists hold */
+  c->skin_need = 0;
   if (baro) {  /* lambda = 1 until the first evaluation has set it */
     const double one = 1.0;
     HIPCHK(hipMemcpy(c->d_dyn + 4, &one, sizeof(double), hipMemcpyHostToDevice));
@@ -206,6 +208,7 @@ static int md_impl(mg_ctx *c, int nsteps, double dt, double T, double tau, int b
     MHIP(hipStreamSynchronize(c->st));
     if (flags[3]) {
       /* restore the checkpoint; the next pass evaluates it synchronised and captures with larger buffers */
+      if (flags[7] > c->skin_need) c->skin_need = flags[7];
       MCHK(snapshot(c, 1));
       s = s_ck;
       need_e = flags[4];
@@ -235,6 +238,7 @@ static int md_impl(mg_ctx *c, int nsteps, double dt, double T, double tau, int b
   r = MG_OK;
 out:
   if (gx) hipGraphExecDestroy(gx);
+  c->skin_on = 0;
   return r;
 #undef MCHK
 #undef MHIP
@@ -457,6 +461,8 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
   RKCHK(mgk_fire_init(c->S, c->d_fire, dt0, c->st));
   int it = 0, need_e = 0, need_r = 0;
   c->rlog_n = 0;
+  c->skin_on = !cell && c->skin > 0.0;  /* positions only: the cells stay fixed, skin lists hold */
+  c->skin_need = 0;
   double t_seg = now_s();
   while (it < max_steps) {
     int nrun;
@@ -486,7 +492,7 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
       }
     }
     it += nrun;
-    int flags[6];
+    int flags[8];
     RKCHK(mgk_clear(c->d_misc + 2, 1, NULL, 0, NULL, 0, c->st));
     RKCHK(mgk_fire_count(c->S, c->d_fire, c->d_misc + 2, c->st));
     RHIP(hipMemcpyAsync(flags, c->d_misc, sizeof flags, hipMemcpyDeviceToHost, c->st));
@@ -498,6 +504,7 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
     }
     if (flags[3]) {
       /* a row or the edge total outgrew the capture: redo these iterations with larger buffers */
+      if (flags[7] > c->skin_need) c->skin_need = flags[7];
       RCHK(snapshot(c, 1));
       it -= nrun;
       need_e = flags[4];
@@ -566,7 +573,9 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
 }
 
 int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done) {
-  return relax_impl(c, 0, 0.0, 0.0, max_steps, fmax, dt0, dtmax, maxstep, steps, done);
+  const int r = relax_impl(c, 0, 0.0, 0.0, max_steps, fmax, dt0, dtmax, maxstep, steps, done);
+  if (c) c->skin_on = 0;  /* skin lists serve this run only (every exit path) */
+  return r;
 }
 
 int mg_relax_cell(mg_ctx *c, double p_gpa, int max_steps, double fmax, double smax_gpa, double dt0, double dtmax,

@@ -54,6 +54,17 @@ struct mg_ctx {
   int wpb_desc, wpb_force; /* waves (atoms) per block of the descriptor and force kernels; 0 = by LDS size */
   int wpb_nbr;              /* waves (atoms) per block of the neighbour fill; 0 = 4, fewer when LDS is short */
   int autotune;             /* relaxation / MD tune the launch shape before their first capture */
+  /* skin neighbour lists (fixed-cell MD and FIRE in graph mode): candidates within rc + skin, rebuilt when an atom
+   * has moved skin/2 since their build; every step filters them with LIST's own test */
+  double skin;              /* A; 0 = off */
+  int skin_on;              /* this run uses them (set by relaxation / MD for fixed cells) */
+  int ccap_row;             /* candidate row capacity (0 = buffers not sized) */
+  int skin_need;            /* longest candidate row seen by an overflowed step */
+  void *d_ckey;             /* u64[N * ccap_row] candidate keys */
+  int *d_crow;              /* int[N] candidate row lengths */
+  double *d_xb;             /* double[3N] wrapped positions at the candidates' build */
+  unsigned *d_mimg;         /* unsigned[N] packed image shifts since the build */
+  size_t cap_skin_atoms, cap_ckey;
   int tuned;                /* the shape was tuned on this context's batch */
   int gcap_edges, gcap_row; /* graph-mode capacities: edges allocated, longest row the LDS tiles hold */
   int graph_enable;         /* relaxation / MD replay a captured iteration (default 1) */
@@ -139,8 +150,16 @@ int mg_dev_alloc(void **p, size_t bytes);
 
 /* kernel launchers (mg_kernels.hip) */
 int mgk_scan(const int *in, int *out, int n, void *tmp, size_t *tmp_bytes, hipStream_t st);
+int mgk_nbr_cand(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,
+                 const int *bin_start, const void *brec, int *crow, void *ckey, int *misc, int ccap, double skin,
+                 hipStream_t st);
+int mgk_skin_commit(int N, const double *x, double *xb, unsigned *mimg, int *misc, hipStream_t st);
+int mgk_nbr_skin(int N, const double *x, const int *atom_str, const mg_struct_dev *str, const void *ckey,
+                 const int *crow, int ccap, const unsigned *mimg, int *row_off, int *rowcnt, int *cnt, void *key,
+                 void *vec, int *rev, uint16_t *rhash, int *misc, int maxrow, int wpb_req, hipStream_t st);
 int mgk_prepare(int N, const double *xin, const int *atom_str, const mg_struct_dev *str, double *x, int *bin,
-                int *bin_count, int *misc, int nclear, hipStream_t st);
+                int *bin_count, int *misc, int nclear, const double *xb, unsigned *mimg, double lim2,
+                hipStream_t st);
 int mgk_bins(int N, int nbins, const int *bin, const int *bin_start, int *bin_count, const double *x, const int *spc,
              void *brec, hipStream_t st);
 int mgk_nbr_count(int N, const double *x, const int *spc, const int *atom_str, const mg_struct_dev *str, const int *bin,

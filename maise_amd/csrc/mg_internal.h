@@ -64,6 +64,7 @@ typedef struct {
   int atom0, natom, nd, xt;
   int nb[3];                   /* bins per lattice direction */
   int K[3];                    /* stencil half width per direction */
+  int Ks[3];                   /* the same for the skin-candidate radius (rc + skin) */
   int bin0, nbins;
   int chunk0, nchunk;           /* reduction chunks (MG_RCHUNK atoms each) of this structure */
 } mg_struct_dev;

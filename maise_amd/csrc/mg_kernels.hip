@@ -5,10 +5,12 @@
 //                  cell-list bin of every atom
 //   k_bin_fill     counting sort of atoms into bins (order inside a bin is free: rows are sorted later)
 //   k_scan_one     bin offsets (one block, LDS-staged passes)
-//   k_nbr<false>   one wave per atom: neighbour count per species over the bin stencil, FP64 test r < Rc^2 in
+//   k_nbr<0>       one wave per atom: neighbour count per species over the bin stencil, FP64 test r < Rc^2 in
 //                  the reference's expression order (src/cell.c:109-123) so the neighbour SETS are bit-exact
-//   k_nbr<true>    fill: 64-bit key (species, j, image) + FP32 (d, r) per edge, row sorted by key in LDS, and
+//   k_nbr<1>       fill: 64-bit key (species, j, image) + FP32 (d, r) per edge, row sorted by key in LDS, and
 //                  the row's reverse-lookup hash table
+//   k_nbr<2>       skin candidates (MD, FIRE): the fill's search at Rc + skin, keys only, when the lists are stale
+//   k_nbr_skin     every step with skin lists: LIST's FP64 test on the candidates only, same rows as the fill
 //   k_desc<NE>     one wave per atom: G2 over neighbours, G4 over neighbour pairs of each species class,
 //                  neighbour tile staged in LDS, (eta, zeta, lambda) grid accumulators in VGPRs,
 //                  reduce-scatter across the wave (src/nprs.c:504-566); with forces, the reverse-edge index of
@@ -215,9 +217,13 @@ __global__ void k_check_caps(int N, const int* __restrict__ row_off, int* __rest
 // ---------------------------------------------------------------------------------------------- prepare
 // Wraps positions and counts atoms per bin (bin_count is zero on entry: the bin fill of the previous evaluation
 // counted it back down).  Also clears the first nclear flags of misc for this evaluation.
+// With skin lists (xb non-null): the image shift m_i that carries the wrapped position at the lists' build, xb_i,
+// next to the current one, m_i = rint((xb_i - x_i) . R) per lattice direction, goes to mimg[i] (packed); when the
+// atom moved more than sqrt(lim2) from there (minimum image) the lists are marked stale (misc[8] = 0).
 __global__ void k_prepare(int N, const double* __restrict__ xin, const int* __restrict__ atom_str,
                           const mg_struct_dev* __restrict__ str, double* __restrict__ x, int* __restrict__ bin,
-                          int* __restrict__ bin_count, int* __restrict__ misc, int nclear) {
+                          int* __restrict__ bin_count, int* __restrict__ misc, int nclear,
+                          const double* __restrict__ xb, unsigned* __restrict__ mimg, double lim2) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nclear) misc[i] = 0;
@@ -256,6 +262,21 @@ __global__ void k_prepare(int N, const double* __restrict__ xin, const int* __re
     }
   }
   x[3 * i] = X[0]; x[3 * i + 1] = X[1]; x[3 * i + 2] = X[2];
+  if (xb) {
+    const double d[3] = {xb[3 * i] - X[0], xb[3 * i + 1] - X[1], xb[3 * i + 2] - X[2]};
+    int mq[3] = {0, 0, 0};
+    if (st->nd > 0)
+      for (int q = 0; q < 3; q++) mq[q] = (int)rint(d[0] * st->R[3 * q] + d[1] * st->R[3 * q + 1] + d[2] * st->R[3 * q + 2]);
+    double u2 = 0.0;
+    for (int q = 0; q < 3; q++) {
+      const double u = -d[q] + mq[0] * st->L[q] + mq[1] * st->L[3 + q] + mq[2] * st->L[6 + q];
+      u2 += u * u;
+    }
+    // a shift beyond the packing range cannot be carried: mark the lists stale as for a long move
+    const bool far = mq[0] < -256 || mq[0] > 255 || mq[1] < -256 || mq[1] > 255 || mq[2] < -256 || mq[2] > 255;
+    if (u2 > lim2 || far) misc[8] = 0;
+    mimg[i] = far ? pack_img(0, 0, 0) : pack_img(mq[0], mq[1], mq[2]);
+  }
   int id = st->bin0 + (b[0] * st->nb[1] + b[1]) * st->nb[2] + b[2];
   bin[i] = id;
   bin[N + i] = b[0] | (b[1] << 10) | (b[2] << 20);  // the bin's coordinates (nb <= 1024 per direction)
@@ -311,7 +332,10 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
 // per-wave LDS of the stencil chunk tables (shifts, bin starts, images, prefix), a multiple of 16 bytes
 #define NBR_LDS_CHUNK ((NBR_CHUNK * (3 * 8 + 4 + 4 + 4) + 4 + 15) / 16 * 16)
 #define NBR_WPB 4  // most atoms (waves) per block; XCD-aware block order, so neighbouring atoms share bins in one L2
-template <bool FILL>
+// MODE 0 counts, 1 fills; MODE 2 fills skin candidates: the search at the radius
+// Rc + skin with the wider stencil Ks, rows of keys only (species, j, image at this build) at i * cap_row in key[]
+// with their lengths in rowcnt[], run only when the lists are stale (misc[8] == 0); its longest row goes to misc[7].
+template <int MODE>
 __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __restrict__ x, const int* __restrict__ spc,
                                             const int* __restrict__ atom_str, const mg_struct_dev* __restrict__ str,
                                             const int* __restrict__ bin, const int* __restrict__ bin_start,
@@ -320,9 +344,12 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
                                             int* __restrict__ row_off, u64* __restrict__ key,
                                             float4* __restrict__ vec, int* __restrict__ rev,
                                             unsigned short* __restrict__ rhash, int hslots, int cap, int padded,
-                                            int cap_row, int wpb) {
+                                            int cap_row, int wpb, double skin) {
 #pragma clang fp contract(off)
+  constexpr bool FILL = MODE != 0;
+  constexpr bool CAND = MODE == 2;
   if (FILL && !padded && misc[3]) return;  // rows exceed the capacities this launch was sized for
+  if (CAND && misc[8]) return;             // the skin lists are current
   extern __shared__ __align__(16) unsigned char smem_n[];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = xcd_block(blockIdx.x, gridDim.x) * wpb + wv;
@@ -339,15 +366,21 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
   unsigned* htab = (unsigned*)(rv + cap);                          // [hslots / 2] words: 16-bit hash slots (FILL)
   const mg_struct_dev* st = &str[atom_str[i]];
   const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
-  const double RC = st->rc2;
+  double RC = st->rc2;
+  if (CAND) {
+    const double rs = sqrt(st->rc2) + skin;
+    RC = rs * rs * (1.0 + 1e-12);
+  }
   const int nb0 = st->nb[0], nb1 = st->nb[1], nb2 = st->nb[2];
-  const int K0 = st->K[0], K1 = st->K[1], K2 = st->K[2];
+  const int K0 = CAND ? st->Ks[0] : st->K[0], K1 = CAND ? st->Ks[1] : st->K[1], K2 = CAND ? st->Ks[2] : st->K[2];
   const int w1n = 2 * K1 + 1, w2n = 2 * K2 + 1;
   const int nsten = (2 * K0 + 1) * w1n * w2n;
   if (nsten >= (1 << 21)) {  // a cell thinner than rc/64: beyond the index arithmetic below; no row, error raised
     if (lane == 0) {
       misc[1] = 1;
-      if (!FILL || padded) {
+      if (CAND) {
+        rowcnt[i] = 0;
+      } else if (!FILL || padded) {
         cnt[4 * i] = cnt[4 * i + 1] = cnt[4 * i + 2] = cnt[4 * i + 3] = 0;
         rowcnt[i] = 0;
         if (FILL) row_off[i] = i * cap_row;
@@ -475,10 +508,10 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
         const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
         if (ok && n + below < cap) {
           rk[n + below] = make_key(sj, j, img);
-          rv[n + below] = make_float4((float)a0, (float)a1, (float)a2, (float)sqrt(r));
+          if (!CAND) rv[n + below] = make_float4((float)a0, (float)a1, (float)a2, (float)sqrt(r));
         }
       }
-      if (!FILL || padded) {
+      if (!CAND && (!FILL || padded)) {
         c0 += __popcll(__ballot(ok && sj == 0));
         c1 += __popcll(__ballot(ok && sj == 1));
         c2 += __popcll(__ballot(ok && sj == 2));
@@ -488,7 +521,14 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
     }
     wave_sync();
   }
-  if (!FILL || padded) {
+  if (CAND) {
+    if (lane == 0) {
+      rowcnt[i] = n < cap_row ? n : cap_row;
+      if (n > __atomic_load_n(misc + 7, __ATOMIC_RELAXED)) atomicMax(misc + 7, n);
+      if (n > cap_row) misc[3] = 1;
+    }
+    if (n > cap_row) return;
+  } else if (!FILL || padded) {
     if (lane == 0) {
       cnt[4 * i] = c0; cnt[4 * i + 1] = c1; cnt[4 * i + 2] = c2; cnt[4 * i + 3] = c3;
       rowcnt[i] = n;
@@ -504,6 +544,18 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
   }
   const int M = n < cap ? n : cap;  // n <= cap by construction (cap >= longest row)
   wave_sync();
+  if (CAND) {  // candidate rows: the keys in sorted order, nothing else
+    const int off = i * cap_row;
+    for (int t0 = 0; t0 < M; t0 += WAVE) {
+      const int ta = t0 + lane;
+      const u64 ka = ta < M ? rk[ta] : ~0ull;
+      int ra = 0;
+#pragma unroll 4
+      for (int u = 0; u < M; u++) ra += rk[u] < ka ? 1 : 0;
+      if (ta < M) key[off + ra] = ka;
+    }
+    return;
+  }
   // rank sort: the keys of a row are distinct, so the number of smaller keys is an entry's place in the sorted row;
   // every lane counts against broadcast reads of the whole row and writes its entries to their places
   const int off = padded ? i * cap_row : row_off[i];
@@ -540,6 +592,154 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
   wave_sync();
   unsigned* ht = (unsigned*)(rhash + (size_t)i * hslots);  // hslots is even: the row's table is word aligned
   for (int q = lane; q < hslots / 2; q += WAVE) ht[q] = htab[q];
+}
+
+// Skin lists, every step: row i of the padded CSR (what k_nbr<1> writes with padded = 1: keys, vectors, rev = -1,
+// species counts, row lengths and offsets, the reverse-lookup hash table, misc[0]/misc[5]/overflow) from the row's
+// skin candidates alone.  A candidate (species, j, image c at the lists' build) lies at image c + m_j - m_i now
+// (m from k_prepare); its vector and distance come from the same FP64 expressions as the fill's (shift
+// sum_q c_q L_q accumulated in that order, then + (x_j - x_i); r = a0^2 + a1^2 + a2^2 accumulated in that order),
+// so the accepted set and the vectors are bit-identical to a fresh fill.  The candidate row is sorted by key and
+// every candidate of one (species, j) moves by the same image shift, which keeps the lexicographic order of the
+// packed images: the accepted entries come out in the fill's sorted order without a sort.  Marks the lists
+// current (misc[8] = 1) for the next step: the stale mark of this step was consumed by k_nbr<2> and k_skin_commit,
+// which ran before this kernel.
+__global__ __launch_bounds__(64 * NBR_WPB) void k_nbr_skin(int N, const double* __restrict__ x,
+                                                  const int* __restrict__ atom_str,
+                                                  const mg_struct_dev* __restrict__ str,
+                                                  const u64* __restrict__ ckey, const int* __restrict__ crow,
+                                                  int ccap, const unsigned* __restrict__ mimg,
+                                                  int* __restrict__ cnt, int* __restrict__ rowcnt,
+                                                  int* __restrict__ misc, int* __restrict__ row_off,
+                                                  u64* __restrict__ key, float4* __restrict__ vec,
+                                                  int* __restrict__ rev, unsigned short* __restrict__ rhash,
+                                                  int hslots, int cap, int cap_row, int wpb) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char smem_s[];
+  if (blockIdx.x == 0 && threadIdx.x == 0) misc[8] = 1;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = xcd_block(blockIdx.x, gridDim.x) * wpb + wv;
+  const int lane = threadIdx.x & 63;
+  if (i >= N) return;
+  unsigned char* wl = smem_s + (size_t)wv * (24 * (size_t)cap + 2 * (size_t)hslots);
+  u64* rk = (u64*)wl;                        // [cap] keys of the row
+  float4* rv = (float4*)(rk + cap);          // [cap] vectors
+  unsigned* htab = (unsigned*)(rv + cap);    // [hslots / 2] words of 16-bit hash slots
+  const mg_struct_dev* st = &str[atom_str[i]];
+  const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+  const double RC = st->rc2;
+  const unsigned mi = mimg[i];
+  const int mi0 = (int)(mi & 1023u) - MG_IMG_OFF, mi1 = (int)((mi >> 10) & 1023u) - MG_IMG_OFF,
+            mi2 = (int)((mi >> 20) & 1023u) - MG_IMG_OFF;
+  const int nc = crow[i];
+  const u64* ck = ckey + (size_t)i * ccap;
+  int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+  int n = 0;
+  // SKIN_ILP chunks of 64 candidates at a time: their key, image-shift and position loads are issued together
+  // (one round of memory latency per group instead of one per chunk); the chunks are then tested and appended in
+  // candidate order
+  constexpr int SKIN_ILP = 4;
+  for (int cb = 0; cb < nc; cb += SKIN_ILP * WAVE) {
+    u64 kq[SKIN_ILP];
+    unsigned mq[SKIN_ILP];
+    double xq[SKIN_ILP][3];
+#pragma unroll
+    for (int q = 0; q < SKIN_ILP; q++) {
+      const int c = cb + q * WAVE + lane;
+      kq[q] = c < nc ? ck[c] : 0ull;
+    }
+#pragma unroll
+    for (int q = 0; q < SKIN_ILP; q++) {
+      const int c = cb + q * WAVE + lane;
+      const int j = key_j(kq[q]);
+      mq[q] = c < nc ? mimg[j] : 0u;
+      xq[q][0] = c < nc ? x[3 * j] : 0.0;
+      xq[q][1] = c < nc ? x[3 * j + 1] : 0.0;
+      xq[q][2] = c < nc ? x[3 * j + 2] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < SKIN_ILP; q++) {
+      const int c = cb + q * WAVE + lane;
+      if (cb + q * WAVE >= nc) break;  // wave-uniform
+      bool ok = false;
+      int j = 0, sj = 0;
+      unsigned img = 0;
+      double a0 = 0, a1 = 0, a2 = 0, r = 0;
+      if (c < nc) {
+        const u64 k = kq[q];
+        sj = (int)(k >> 60);
+        j = key_j(k);
+        // image now = image at the build + m_j - m_i (packed fields carry MG_IMG_OFF each)
+        const unsigned ib = (unsigned)(k & 0x3fffffffull), mj = mq[q];
+        const int ic0 = (int)(ib & 1023u) - MG_IMG_OFF + ((int)(mj & 1023u) - MG_IMG_OFF) - mi0;
+        const int ic1 = (int)((ib >> 10) & 1023u) - MG_IMG_OFF + ((int)((mj >> 10) & 1023u) - MG_IMG_OFF) - mi1;
+        const int ic2 = (int)((ib >> 20) & 1023u) - MG_IMG_OFF + ((int)((mj >> 20) & 1023u) - MG_IMG_OFF) - mi2;
+        img = pack_img(ic0, ic1, ic2);
+        double sh[3];
+        for (int d = 0; d < 3; d++) {
+          double v = 0.0;
+          v += (double)ic0 * st->L[d];
+          v += (double)ic1 * st->L[3 + d];
+          v += (double)ic2 * st->L[6 + d];
+          sh[d] = v;
+        }
+        a0 = sh[0]; a0 += xq[q][0] - xi0;
+        a1 = sh[1]; a1 += xq[q][1] - xi1;
+        a2 = sh[2]; a2 += xq[q][2] - xi2;
+        r = 0.0;
+        r += a0 * a0;
+        r += a1 * a1;
+        r += a2 * a2;
+        ok = r < RC;
+      }
+      const u64 bm = __ballot(ok);
+      const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+      if (ok && n + below < cap) {
+        rk[n + below] = make_key(sj, j, img);
+        rv[n + below] = make_float4((float)a0, (float)a1, (float)a2, (float)sqrt(r));
+      }
+      c0 += __popcll(__ballot(ok && sj == 0));
+      c1 += __popcll(__ballot(ok && sj == 1));
+      c2 += __popcll(__ballot(ok && sj == 2));
+      c3 += __popcll(__ballot(ok && sj == 3));
+      n += __popcll(bm);
+    }
+  }
+  if (lane == 0) {
+    cnt[4 * i] = c0; cnt[4 * i + 1] = c1; cnt[4 * i + 2] = c2; cnt[4 * i + 3] = c3;
+    rowcnt[i] = n;
+    if (n > __atomic_load_n(misc, __ATOMIC_RELAXED)) atomicMax(misc, n);
+    row_off[i] = i * cap_row;
+    if (n > __atomic_load_n(misc + 5, __ATOMIC_RELAXED)) atomicMax(misc + 5, n);
+    if (n > cap_row) misc[3] = 1;
+  }
+  if (n > cap_row) return;  // overflow: nothing of this row is written
+  const int M = n < cap ? n : cap;
+  const int off = i * cap_row;
+  const int hmask = hslots - 1;
+  for (int q = lane; q < hslots / 2; q += WAVE) htab[q] = 0u;
+  wave_sync();
+  for (int ta = lane; ta < M; ta += WAVE) {
+    const u64 ka = rk[ta];
+    key[off + ta] = ka; vec[off + ta] = rv[ta]; rev[off + ta] = -1;
+    const int ja = key_j(ka);
+    const unsigned ia = (unsigned)(ka & 0x3fffffffull);
+    if (ja < i || (ja == i && neg_img(ia) < ia)) htab_insert(htab, rev_slot(ja, ia, hmask), hmask, (unsigned)(ta + 1));
+  }
+  wave_sync();
+  unsigned* ht = (unsigned*)(rhash + (size_t)i * hslots);
+  for (int q = lane; q < hslots / 2; q += WAVE) ht[q] = htab[q];
+}
+
+// After a skin-candidate build (lists stale, misc[8] == 0): the build positions are the current wrapped ones, and
+// every image shift is zero; misc[16] counts the builds over the context's life.
+__global__ void k_skin_commit(int N, const double* __restrict__ x, double* __restrict__ xb, unsigned* __restrict__ mimg,
+                              int* __restrict__ misc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || misc[8]) return;
+  xb[3 * i] = x[3 * i]; xb[3 * i + 1] = x[3 * i + 1]; xb[3 * i + 2] = x[3 * i + 2];
+  mimg[i] = pack_img(0, 0, 0);
+  if (i == 0) misc[16] += 1;
 }
 
 // Reverse edge of e = (i -> j, img): the entry (j -> i, -img) of row j.  Edges come in mutual pairs, rev[e] = e' and
@@ -1561,11 +1761,11 @@ int mgk_check_caps(int N, const int* row_off, int* misc, int cap_edges, int cap_
 }
 
 int mgk_prepare(int N, const double* xin, const int* atom_str, const mg_struct_dev* str, double* x, int* bin,
-                int* bin_count, int* misc, int nclear, hipStream_t st) {
+                int* bin_count, int* misc, int nclear, const double* xb, unsigned* mimg, double lim2, hipStream_t st) {
   const int n = N > nclear ? N : nclear;
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_prepare, dim3((n + 255) / 256), dim3(256), 0, st, N, xin, atom_str, str, x, bin, bin_count, misc,
-                     nclear);
+                     nclear, xb, mimg, lim2);
   return mgk_status();
 }
 
@@ -1580,10 +1780,10 @@ int mgk_bins(int N, int nbins, const int* bin, const int* bin_start, int* bin_co
 int mgk_nbr_count(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
                   const int* bin_start, const void* brec, int* cnt, int* rowcnt, int* misc, hipStream_t st) {
   if (N == 0) return 0;
-  hipLaunchKernelGGL(k_nbr<false>, dim3((N + NBR_WPB - 1) / NBR_WPB), dim3(64 * NBR_WPB), (size_t)NBR_WPB * NBR_LDS_CHUNK,
+  hipLaunchKernelGGL(k_nbr<0>, dim3((N + NBR_WPB - 1) / NBR_WPB), dim3(64 * NBR_WPB), (size_t)NBR_WPB * NBR_LDS_CHUNK,
                      st, N, x, spc, atom_str, str,
                      bin, bin_start, (const BinRec*)brec, cnt, rowcnt, misc, (int*)nullptr, (u64*)nullptr,
-                     (float4*)nullptr, (int*)nullptr, (unsigned short*)nullptr, 0, 0, 0, 0, NBR_WPB);
+                     (float4*)nullptr, (int*)nullptr, (unsigned short*)nullptr, 0, 0, 0, 0, NBR_WPB, 0.0);
   return mgk_status();
 }
 
@@ -1614,12 +1814,62 @@ int mgk_nbr_fill(int N, const double* x, const int* spc, const int* atom_str, co
   while (wpb > 1 && (size_t)wpb * per_wave > budget) wpb >>= 1;
   const size_t sm = (size_t)wpb * per_wave;
   if (sm > budget &&
-      hipFuncSetAttribute((const void*)&k_nbr<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
+      hipFuncSetAttribute((const void*)&k_nbr<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
     return -1;
-  hipLaunchKernelGGL(k_nbr<true>, dim3((N + wpb - 1) / wpb), dim3(64 * wpb), sm, st, N, x, spc, atom_str,
+  hipLaunchKernelGGL(k_nbr<1>, dim3((N + wpb - 1) / wpb), dim3(64 * wpb), sm, st, N, x, spc, atom_str,
                      str, bin, bin_start, (const BinRec*)brec,
                      cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev, (unsigned short*)rhash, hslots, cap,
-                     padded, maxrow, wpb);
+                     padded, maxrow, wpb, 0.0);
+  return mgk_status();
+}
+
+// Skin candidates at Rc + skin (rows of ccap keys; the stencil half widths Ks of each structure cover that radius),
+// only when the lists are stale.  ccap <= 2048.
+int mgk_nbr_cand(int N, const double* x, const int* spc, const int* atom_str, const mg_struct_dev* str, const int* bin,
+                 const int* bin_start, const void* brec, int* crow, void* ckey, int* misc, int ccap, double skin,
+                 hipStream_t st) {
+  if (N == 0) return 0;
+  int cap = 64;
+  while (cap < ccap) cap <<= 1;
+  if (cap > 2048) return -2;
+  const size_t per_wave = NBR_LDS_CHUNK + (size_t)cap * (sizeof(u64) + sizeof(float4));
+  int wpb = NBR_WPB;
+  while (wpb > 1 && (size_t)wpb * per_wave > 64 * 1024) wpb >>= 1;
+  const size_t sm = (size_t)wpb * per_wave;
+  if (sm > 64 * 1024 &&
+      hipFuncSetAttribute((const void*)&k_nbr<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(k_nbr<2>, dim3((N + wpb - 1) / wpb), dim3(64 * wpb), sm, st, N, x, spc, atom_str, str, bin,
+                     bin_start, (const BinRec*)brec, (int*)nullptr, crow, misc, (int*)nullptr, (u64*)ckey,
+                     (float4*)nullptr, (int*)nullptr, (unsigned short*)nullptr, 0, cap, 1, ccap, wpb, skin);
+  return mgk_status();
+}
+
+int mgk_skin_commit(int N, const double* x, double* xb, unsigned* mimg, int* misc, hipStream_t st) {
+  if (N == 0) return 0;
+  hipLaunchKernelGGL(k_skin_commit, dim3((N + 255) / 256), dim3(256), 0, st, N, x, xb, mimg, misc);
+  return mgk_status();
+}
+
+// The padded rows (row i at i * maxrow, as mgk_nbr_fill with padded = 1) from the skin candidates.
+int mgk_nbr_skin(int N, const double* x, const int* atom_str, const mg_struct_dev* str, const void* ckey,
+                 const int* crow, int ccap, const unsigned* mimg, int* row_off, int* rowcnt, int* cnt, void* key,
+                 void* vec, int* rev, uint16_t* rhash, int* misc, int maxrow, int wpb_req, hipStream_t st) {
+  if (N == 0) return 0;
+  int cap = 64;
+  while (cap < maxrow) cap <<= 1;
+  if (cap > 2048) return -2;
+  const int hslots = mgk_rev_slots(maxrow);
+  const size_t per_wave = (size_t)cap * (sizeof(u64) + sizeof(float4)) + (size_t)hslots * sizeof(uint16_t);
+  int wpb = (wpb_req == 1 || wpb_req == 2 || wpb_req == 4) ? wpb_req : NBR_WPB;
+  while (wpb > 1 && (size_t)wpb * per_wave > 64 * 1024) wpb >>= 1;
+  const size_t sm = (size_t)wpb * per_wave;
+  if (sm > 64 * 1024 &&
+      hipFuncSetAttribute((const void*)&k_nbr_skin, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(k_nbr_skin, dim3((N + wpb - 1) / wpb), dim3(64 * wpb), sm, st, N, x, atom_str, str,
+                     (const u64*)ckey, crow, ccap, mimg, cnt, rowcnt, misc, row_off, (u64*)key, (float4*)vec, rev,
+                     (unsigned short*)rhash, hslots, cap, maxrow, wpb);
   return mgk_status();
 }
 

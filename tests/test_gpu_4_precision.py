@@ -11,10 +11,11 @@ The workload's own size, 20x20x20 (32,000 atoms), with snapshots at steps 0, 50,
 about half a minute per snapshot on the host cores).  MAISE_FULL_REPORT=1 takes snapshots every 20 steps;
 MAISE_REPORT_FILE=<path> also writes the table there.
 
-Bars (FP64 reference): FP32 1e-5 eV/atom and 1e-4 eV/A; BF16x3 1e-3 eV/atom and 1e-2 eV/A (the stated BF16
-bound).  One-pass BF16 in the hidden layer (MG_PREC_BF16) is reported, not gated: on the 10-wide shipped models
-rounding the hidden layer's weights or activations to 8 mantissa bits alone moves the energy by 1-4 meV/atom
-(emulated on the CPU with the same rounding), so the compensated form is the BF16 path held to the bound.
+Bars (FP64 reference): FP32 1e-5 eV/atom and 1e-4 eV/A; both BF16 paths at the stated BF16 bound, 1e-3 eV/atom
+and 1e-2 eV/A: BF16x3 (compensated) and one-pass BF16 in the 64x64 hidden-layer product (MG_PREC_BF16; measured
+about 0.6 meV/atom and 4 meV/A on this workload).  On the 10-wide shipped models one-pass BF16 misses the bound
+(rounding a 10-wide hidden layer's weights or activations to 8 mantissa bits moves the energy by 1-4 meV/atom,
+emulated on the CPU with the same rounding) and is reported there, not gated (tests/test_gpu_5_config_errors.py).
 """
 import os
 
@@ -97,3 +98,4 @@ def test_precision_report_nvt_900k():
     md.close()
     assert agg["FP32"][0] < 1e-5 and agg["FP32"][1] < 1e-4
     assert agg["BF16x3"][0] < 1e-3 and agg["BF16x3"][1] < 1e-2
+    assert agg["BF16"][0] < 1e-3 and agg["BF16"][1] < 1e-2
