@@ -174,6 +174,7 @@ def relax_leg(maise_amd, batch, args):
     c.set_batch(batch[:64])
     c.relax(max_steps=50, fmax=fmax)           # warm-up: code objects, allocations
     c.set_autotune(True)                       # tuned on the full batch before the first capture (timed)
+    c.set_skin(1.0)                            # skin neighbour lists (results bit-identical to rebuilding each step)
     t0 = time.perf_counter()
     c.set_batch(batch)
     steps, done = c.relax(max_steps=max_steps, fmax=fmax)
@@ -216,24 +217,41 @@ def relax_leg(maise_amd, batch, args):
 
 def md_leg(maise_amd, args):
     """NVT molecular dynamics of the Cu-Ag 50/50 fcc 12x12x12 alloy (6912 atoms, a = 3.8 A), 600 K
-    Maxwell-Boltzmann start (seed 1), 1 fs steps, state on the device."""
+    Maxwell-Boltzmann start (seed 1), 1 fs steps, state on the device; with skin neighbour lists (1 A; the
+    trajectory is bit-identical to rebuilding the lists every step, which is timed too)."""
     from maise_amd import workloads as wl
     s, l, x = wl.alloy_fcc(12, 3.8, 2, 0.0, seed=2)
-    c = maise_amd.Context(maise_amd.Model(MD_POT), device=0)
-    c.set_structure(s, l, x)
-    c.md_init(600.0, seed=1)
-    c.md_run(10, dt=1.0, T=600.0, tau=50.0)     # warm-up
-    c.md_init(600.0, seed=1)
-    t0 = time.perf_counter()
-    ep, ek = c.md_run(args.md_steps, dt=1.0, T=600.0, tau=50.0)
-    dt = time.perf_counter() - t0
-    c.close()
+    model = maise_amd.Model(MD_POT)
+
+    def run(skin):
+        c = maise_amd.Context(model, device=0)
+        c.set_skin(skin)
+        c.set_structure(s, l, x)
+        c.md_init(600.0, seed=1)
+        c.md_run(10, dt=1.0, T=600.0, tau=50.0)     # warm-up
+        c.md_init(600.0, seed=1)
+        t0 = time.perf_counter()
+        ep, ek = c.md_run(args.md_steps, dt=1.0, T=600.0, tau=50.0)
+        dt = time.perf_counter() - t0
+        builds = c.skin_builds()
+        c.close()
+        return ep, ek, dt, builds
+
+    if args.md_skin is not None:  # one variant only (profiling)
+        ep, ek, dt, builds = run(args.md_skin)
+        ep0, ek0, dt0 = ep, ek, dt
+    else:
+        ep0, ek0, dt0, _ = run(0.0)
+        ep, ek, dt, builds = run(1.0)
     T = ek / (1.5 * len(s) / 11604.0)
     n = len(s)
     return {"workload": f"Cu-Ag 50/50 fcc 12x12x12 ({n} atoms), a=3.8 A, NVT 600 K, dt 1 fs, "
                         f"{args.md_steps} steps; shipped potential nn_CuAg_d3_v1 (145-10-10-1, Rc 7.5 A)",
             "steps_per_s": args.md_steps / dt, "atom_steps_per_s": n * args.md_steps / dt,
             "ms_per_step": 1e3 * dt / args.md_steps,
+            "skin_A": 1.0 if args.md_skin is None else args.md_skin, "skin_builds": builds,
+            "ms_per_step_lists_every_step": 1e3 * dt0 / args.md_steps,
+            "trajectory_equal_to_lists_every_step": bool(np.array_equal(ep0, ep) and np.array_equal(ek0, ek)),
             "T_mean_last_half_K": float(T[len(T) // 2:].mean()),
             "epot_per_atom_first_last_eV": [float(ep[0] / n), float(ep[-1] / n)]}
 
@@ -330,6 +348,11 @@ def main():
     ap.add_argument("--ga-compare-seconds", type=float, default=8.0,
                     help="search clock of each run of the with/without-screening comparison (0 skips it)")
     ap.add_argument("--ga-compare-seeds", type=int, default=3, help="seeds of the with/without-screening comparison")
+    ap.add_argument("--md-skin", type=float, default=None,
+                    help="run the MD leg with this skin only (default: 0 and 1.0 A, compared)")
+    ap.add_argument("--fire-sweep", type=int, nargs="*", default=None,
+                    help="only time FIRE iterations on batches of this many Mg16Ca8 cells (e.g. 64 179 512 2048)")
+    ap.add_argument("--fire-iters", type=int, default=300, help="FIRE iterations per batch of --fire-sweep")
     ap.add_argument("--rates-only", action="store_true",
                     help="only run the library's issue-rate microbenchmark (FP32 FMA, exp2, cos, sqrt, rcp) and print it")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -357,6 +380,26 @@ def main():
 
     import maise_amd
     from maise_amd import workloads as wl
+    if args.fire_sweep:
+        # FIRE iterations of batches of Mg16Ca8 cells (24 atoms): wall time per iteration against batch size, every
+        # structure active throughout (fmax below reach)
+        from maise_amd.evolve import GAConfig, random_structure
+        cfg = GAConfig()
+        rng = np.random.default_rng(3)
+        cells = [random_structure(cfg, rng) for _ in range(max(args.fire_sweep))]
+        c = maise_amd.Context(maise_amd.Model(GA_POT), device=0, urep=100.0)
+        c.set_batch(cells[:16])
+        c.relax(max_steps=20, fmax=0.01)
+        out = {}
+        for n in args.fire_sweep:
+            c.set_batch(cells[:n])
+            t0 = time.perf_counter()
+            c.relax(max_steps=args.fire_iters, fmax=1e-9)
+            out[n] = 1e6 * (time.perf_counter() - t0) / args.fire_iters
+        c.close()
+        print(json.dumps({"workload": f"FIRE on batches of random Mg16Ca8 cells, nn_MgCa_d3_v0, UREP 100 eV, "
+                                      f"{args.fire_iters} iterations", "us_per_iteration_by_cells": out}))
+        return
     if args.rates_only:
         r = measured_rates(maise_amd)
         print(json.dumps({"ops_per_s": r, "fma_TFLOPs": 2 * r["fma"] / 1e12,
