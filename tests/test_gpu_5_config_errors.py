@@ -14,7 +14,7 @@
 Bars against FP64: FP32 path 1e-5 eV/atom, 1e-4 eV/A, 1e-4 eV/atom virial; compensated BF16 (BF16x3, the bounded
 BF16 path) 1e-3 eV/atom, 1e-2 eV/A, 1e-3 eV/atom virial.  One-pass BF16 in the hidden layer (MG_PREC_BF16) is
 reported, not gated: rounding a 10-wide hidden layer's weights or activations to 8 mantissa bits moves these
-shipped potentials' energies by 1-4 meV/atom (emulated on the CPU with the same rounding), above the bound.
+shipped potentials' energies by 1-17 meV/atom (emulated on the CPU with the same rounding), above the bound.
 MAISE_REPORT_FILE=<path> appends the tables there.
 """
 import os
