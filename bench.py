@@ -333,8 +333,9 @@ def ga_leg(maise_amd, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50,
+                    help="untimed evaluations first: the clocks ramp over the first few tens of milliseconds of load")
     ap.add_argument("--n", type=int, default=20, help="fcc supercell edge (4 n^3 atoms)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--relax", type=int, default=4096,

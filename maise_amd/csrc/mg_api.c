@@ -357,10 +357,13 @@ static int autotune_impl(mg_ctx *c, int reps, int *shape) {
   c->wpb_nbr = wn;
   c->wpb_desc = wd;
   c->wpb_force = wf;
-  /* stored pair lists against enumeration, at the chosen shape */
+  /* stored pair lists against enumeration, at the chosen shape; switching them re-sizes the capacities (as
+   * mg_ctx_set_pair_store does), so the next evaluation stores the lists or not as asked */
   float tp[2] = {1e30f, 1e30f};
   for (int store = 0; store < 2; store++) {
     c->pair_store = store;
+    c->gcap_edges = c->gcap_row = 0;
+    c->gpstride = 0;
     if (mgh_eval(c, MG_EVAL_FORCES, 0) != MG_OK) continue;
     if ((r = mg_get_timing(c, t)) != MG_OK) return r;
     int ok = 1;
@@ -371,6 +374,8 @@ static int autotune_impl(mg_ctx *c, int reps, int *shape) {
     tp[store] = t[1] + t[3];
   }
   c->pair_store = tp[1] < 1e29f && tp[1] <= tp[0] ? 1 : (tp[0] < 1e29f ? 0 : ps0);
+  c->gcap_edges = c->gcap_row = 0;  /* the next evaluation sizes the capacities for the chosen setting */
+  c->gpstride = 0;
   if ((r = mg_ctx_set_timing(c, timing0)) != MG_OK) return r;
   c->tuned = 1;
   if (shape) mg_ctx_get_shape(c, shape);
