@@ -98,3 +98,27 @@ def test_fire_with_skin_lists_is_bit_identical_small_cells():
     b = _relax_pair(MGCA, structs, 0.5)
     print("skin builds:", b, flush=True)
     assert b >= 1
+
+
+def test_cell_changing_runs_ignore_the_skin_setting():
+    """NPT MD and cell relaxation move the lattice every step, so they rebuild the lists every step whatever the
+    skin: with a skin set they reproduce the runs without one bit for bit and build no candidates."""
+    import maise_amd
+    from maise_amd import workloads as wl
+    print("[gpu case] NPT MD and cell relaxation with skin 0 / 1.0 A, Cu 108 atoms", flush=True)
+    m = maise_amd.Model(os.path.join(GOLDEN, "models", "nn_Cu_d3_v1"))
+    s, l, x = wl.cu_fcc(3, 3.55, 0.02, seed=5)
+    out = []
+    for skin in (0.0, 1.0):
+        c = maise_amd.Context(m)
+        c.set_skin(skin)
+        c.set_structure(s, l, x)
+        c.md_init(300.0, seed=3)
+        ep, ek, vol, pint = c.md_run_npt(60, dt=1.0, T=300.0, tau=50.0, p_gpa=0.0, ktt=2e-3)
+        c.set_structure(s, l, x)
+        steps, done = c.relax_cell(p_gpa=0.0, max_steps=200, fmax=0.01, smax_gpa=0.05)
+        out.append((ep, ek, vol, pint, steps, c.lattice(), c.result(0)["X"], c.skin_builds()))
+        c.close()
+    for a, b in zip(out[0][:7], out[1][:7]):
+        np.testing.assert_array_equal(a, b)
+    assert out[1][7] == 0
