@@ -85,6 +85,11 @@ def host_cpu_info():
     return nproc, model
 
 
+def progress(msg):
+    """One line per finished step of the long legs, on stderr (the result stays the one JSON line on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(threads):
     """The reference's own CPU path (CALL_MAISE compiled from its sources, oracle/_ref) on bounded samples of the same
     lattice and potential: Cu fcc 10x10x10 (4,000 atoms) on `threads` OpenMP threads, and Cu fcc 4x4x4 (256 atoms) on
@@ -264,12 +269,14 @@ def ga_leg(maise_amd, args):
     sigma 0.3 A, seeded, FIRE capped at 300 steps per generation (unconverged structures carried over).
     (1) an audited run: every generation's offspring are also relaxed in full, outside the search clock, for recall
     and error; (2) the same run in compensated BF16 (BF16x3); (3) the same seeded search with and without screening
-    for equal search time; (4) a sample of offspring relaxed one structure per context gives the plain-path rate."""
+    for equal search time, at population 512 and at 2048; (4) a sample of offspring relaxed one structure per context
+    gives the plain-path rate."""
     from maise_amd.evolve import GAConfig, SurrogateGA, compare_screening, make_offspring
     cfg = GAConfig(pop=args.ga_pop, generations=args.ga_gens, seed=5, relax_cap=300)
     t0 = time.perf_counter()
     ga = SurrogateGA(GA_POT, cfg)
-    res = ga.run()
+    res = ga.run(log=lambda r: progress(f"audited search generation {r.gen}: best {r.best_eV_atom:.5f} eV/atom")
+                 if r.gen % 10 == 0 else None)
     wall = time.perf_counter() - t0
     reps = res.reports[1:]
     ts = float(np.mean([r.seconds_screened for r in reps]))
@@ -293,16 +300,28 @@ def ga_leg(maise_amd, args):
     rb = gb.run()
     gb.close()
     tb = float(np.mean([r.seconds_screened for r in rb.reports[1:]]))
-    cmp_ = None
-    if args.ga_compare_seconds > 0:
-        cmp_ = compare_screening(GA_POT, GAConfig(pop=args.ga_pop, relax_cap=300), args.ga_compare_seconds,
-                                 list(range(1, args.ga_compare_seeds + 1)))
+    def trim(cmp_):
         for name in cmp_["runs"]:
             for r in cmp_["runs"][name]:
                 tr = r.pop("trajectory")
                 # improvements only (search clock, best eV/atom)
                 r["improvements"] = [(round(t, 3), round(e, 5)) for k, (t, e) in enumerate(tr)
                                      if k == 0 or e < tr[k - 1][1] - 1e-9]
+        return cmp_
+
+    both = {"unscreened": {"screen": False}, "screened": {"screen": True}}
+
+    def cmp_log(name, seed, r):
+        progress(f"comparison run {name} seed {seed}: {r['generations']} generations, best {r['best_eV_atom']:.5f}")
+    cmp_ = cmp_large = None
+    if args.ga_compare_seconds > 0:
+        cmp_ = trim(compare_screening(GA_POT, GAConfig(pop=args.ga_pop, relax_cap=300), args.ga_compare_seconds,
+                                      list(range(1, args.ga_compare_seeds + 1)), variants=both, log=cmp_log))
+    if args.ga_compare_large_pop > 0:
+        # a population whose relaxation batches fill the GPU (relaxation time grows with the batch there)
+        cmp_large = trim(compare_screening(GA_POT, GAConfig(pop=args.ga_compare_large_pop, relax_cap=300),
+                                           args.ga_compare_large_seconds, list(range(11, 11 + args.ga_compare_seeds + 1)),
+                                           variants=both, log=cmp_log))
     n = cfg.pop
     br = {"predict": "seconds_predict", "offspring": "seconds_offspring", "relax_kept": "seconds_relax",
           "train": "seconds_train", "population_update": "seconds_select"}
@@ -327,6 +346,7 @@ def ga_leg(maise_amd, args):
             "carried_per_generation_mean": float(np.mean([r.carried for r in reps])),
             "best_eV_atom_first_last": [res.reports[0].best_eV_atom, res.reports[-1].best_eV_atom],
             "equal_time_comparison": cmp_,
+            "equal_time_comparison_large_population": cmp_large,
             "seconds_total": wall}
 
 
@@ -349,6 +369,10 @@ def main():
     ap.add_argument("--ga-compare-seconds", type=float, default=8.0,
                     help="search clock of each run of the with/without-screening comparison (0 skips it)")
     ap.add_argument("--ga-compare-seeds", type=int, default=3, help="seeds of the with/without-screening comparison")
+    ap.add_argument("--ga-compare-large-pop", type=int, default=2048,
+                    help="population of a second with/without-screening comparison (0 skips it; one more seed)")
+    ap.add_argument("--ga-compare-large-seconds", type=float, default=15.0,
+                    help="search clock of each run of the large-population comparison")
     ap.add_argument("--md-skin", type=float, default=None,
                     help="run the MD leg with this skin only (default: 0 and 1.0 A, compared)")
     ap.add_argument("--fire-sweep", type=int, nargs="*", default=None,
@@ -503,22 +527,26 @@ def main():
     except Exception as e:
         bf = {"unavailable": str(e)[:200]}
 
+    progress(f"headline done: {value:.4g} atom E+F evals/s, {ms:.4f} ms per evaluation")
     relax = None
     if batch is not None:
         try:
             relax = relax_leg(maise_amd, batch, args)
+            progress(f"relax leg done: {relax['relaxed_structures_per_s']:.1f} relaxed structures/s")
         except Exception as e:
             relax = {"unavailable": str(e)[:300]}
     md = None
     if args.md_steps > 0:
         try:
             md = md_leg(maise_amd, args)
+            progress(f"md leg done: {md['ms_per_step']:.4f} ms per step")
         except Exception as e:
             md = {"unavailable": str(e)[:300]}
     ga = None
     if args.ga_gens > 0:
         try:
             ga = ga_leg(maise_amd, args)
+            progress("search leg done")
         except Exception as e:
             ga = {"unavailable": str(e)[:300]}
 
@@ -528,6 +556,7 @@ def main():
         threads = min(host_cpu_info()[0], 16)
         try:
             cpu = cpu_baseline(threads)
+            progress(f"cpu baseline done: {cpu['value']:.1f} atom E+F evals/s on {threads} threads")
         except Exception as e:
             cpu = {"unavailable": str(e)[:200]}
 
