@@ -123,6 +123,43 @@ def cpu_baseline(threads):
     return out
 
 
+def leg_cpu_baselines(threads, batch):
+    """CPU figures beside the legs, on bounded samples: MD = the reference's CALL_MAISE (compiled from its sources)
+    on the Cu-Ag alloy at 8x8x8 (2,048 atoms, same lattice and potential; an MD step costs one E+F evaluation), mean
+    of 2 calls after a warm-up; batched FIRE and the search = FIRE on FP64 oracle forces (the reference relaxes with
+    GSL's BFGS, absent here), one structure after another, on the first 2 structures of the Cu-Pd-Ag batch and on 4
+    random Mg16Ca8 cells of the search's initial population."""
+    from maise_amd import workloads as wl
+    from maise_amd.evolve import GAConfig, random_structure
+    from oracle import pyoracle as po
+    out = {}
+    if po.have_reference():
+        s, l, x = wl.alloy_fcc(8, 3.8, 2, 0.0, seed=2)
+        r = po.run_reference(MD_POT, s, l, x, nm=300, repeats=2, threads=threads, timeout=900)
+        out["md"] = {"value": len(s) / r["time"], "unit": "atom-steps/s", "cores": threads, "kind": "reference",
+                     "sample": f"reference CALL_MAISE, Cu-Ag 50/50 fcc 8x8x8 ({len(s)} atoms), one E+F evaluation "
+                               "per MD step, mean of 2 calls after a warm-up"}
+    if batch is not None:
+        m = po.OracleModel(RELAX_POT)
+        t0 = time.perf_counter()
+        for s, l, x in batch[:2]:
+            po.fire_relax(m, s, l, x, urep=100.0)
+        out["relax"] = {"value": 2 / (time.perf_counter() - t0), "unit": "relaxed structures/s",
+                        "cores": po.num_threads(), "kind": "port",
+                        "sample": "FIRE on FP64 oracle forces, the first 2 structures of the Cu-Pd-Ag batch"}
+    cfg = GAConfig()
+    rng = np.random.default_rng(cfg.seed)
+    cells = [random_structure(cfg, rng) for _ in range(4)]
+    m = po.OracleModel(GA_POT)
+    t0 = time.perf_counter()
+    for s, l, x in cells:
+        po.fire_relax(m, s, l, x, urep=cfg.urep)
+    out["search"] = {"value": 4 / (time.perf_counter() - t0), "unit": "relaxed offspring/s",
+                     "cores": po.num_threads(), "kind": "port",
+                     "sample": "FIRE on FP64 oracle forces, 4 random Mg16Ca8 cells of the initial population"}
+    return out
+
+
 def measured_rates(maise_amd):
     """Issue rates measured on the device by the library's microbenchmark (lane operations per second)."""
     names = ("fma", "exp2", "cos", "sqrt", "rcp")
@@ -358,6 +395,7 @@ def main():
                     help="untimed evaluations first: the clocks ramp over the first few tens of milliseconds of load")
     ap.add_argument("--n", type=int, default=20, help="fcc supercell edge (4 n^3 atoms)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-leg-cpu", action="store_true", help="skip the CPU figures beside the MD, relax and search legs")
     ap.add_argument("--relax", type=int, default=4096,
                     help="structures in the batched FIRE leg (0 skips it)")
     ap.add_argument("--relax-single", type=int, default=48,
@@ -559,6 +597,15 @@ def main():
             progress(f"cpu baseline done: {cpu['value']:.1f} atom E+F evals/s on {threads} threads")
         except Exception as e:
             cpu = {"unavailable": str(e)[:200]}
+        if not args.no_leg_cpu:
+            try:
+                legs = leg_cpu_baselines(threads, batch if relax is not None else None)
+                for key, leg in (("md", md), ("relax", relax), ("search", ga)):
+                    if key in legs and isinstance(leg, dict):
+                        leg["cpu_baseline"] = legs[key]
+                progress("leg CPU baselines done")
+            except Exception as e:
+                progress(f"leg CPU baselines unavailable: {str(e)[:200]}")
 
     out = {
         "metric": "atom E+F evals/s", "value": value, "unit": "atom E+F evals/s", "n_gpus": 1,
