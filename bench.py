@@ -221,6 +221,7 @@ def relax_leg(maise_amd, batch, args):
     c.set_batch(batch)
     steps, done = c.relax(max_steps=max_steps, fmax=fmax)
     dt = time.perf_counter() - t0
+    fres = c.batch_result()
     shape = c.shape()
     log = c.relax_log()
     c.close()
@@ -231,6 +232,28 @@ def relax_leg(maise_amd, batch, args):
         sel = (log[:, 1] / S0 >= lo) & (log[:, 1] / S0 < hi)
         split[name] = {"seconds": float(log[sel, 3].sum()),
                        "iterations": int(np.diff(np.concatenate([[0], log[:, 0]]))[sel].sum())}
+    # the same batch with the device L-BFGS (the reference minimises with GSL's BFGS2; no line search here)
+    c = maise_amd.Context(model, device=0, urep=100.0)
+    c.set_batch(batch[:64])
+    c.relax_lbfgs(max_steps=50, fmax=fmax)     # warm-up
+    c.set_autotune(True)
+    c.set_skin(1.0)
+    t0 = time.perf_counter()
+    c.set_batch(batch)
+    lsteps, ldone = c.relax_lbfgs(max_steps=max_steps, fmax=fmax)
+    ldt = time.perf_counter() - t0
+    lres = c.batch_result()
+    c.close()
+    nat = np.array([len(b[0]) for b in batch], dtype=np.float64)
+    ef, el = fres["E"] / nat, lres["E"] / nat
+    lbfgs = {"relaxed_structures_per_s": float(ldone.sum()) / ldt, "seconds": ldt, "converged": int(ldone.sum()),
+             "steps_mean": float(lsteps.mean()), "steps_max": int(lsteps.max()), "memory": 20,
+             "speedup_over_fire": (float(ldone.sum()) / ldt) / (float(done.sum()) / dt),
+             "relaxed_energy_per_atom_mean_eV": {"fire": float(ef.mean()), "lbfgs": float(el.mean())},
+             "share_lower_than_fire_by_1meV": float(np.mean(el < ef - 1e-3)),
+             "share_higher_than_fire_by_1meV": float(np.mean(el > ef + 1e-3))}
+    progress(f"relax: L-BFGS {lbfgs['relaxed_structures_per_s']:.0f} structures/s, steps mean "
+             f"{lbfgs['steps_mean']:.1f} max {lbfgs['steps_max']}")
     nsingle = min(args.relax_single, len(batch))
     c1 = maise_amd.Context(model, device=0, urep=100.0)
     t1 = time.perf_counter()
@@ -254,7 +277,8 @@ def relax_leg(maise_amd, batch, args):
             "single_structure_path": {"structures": nsingle, "seconds": dt1,
                                       "relaxed_structures_per_s": nsingle / dt1,
                                       "steps_equal_batched": bool(np.array_equal(np.array(s1), steps[:nsingle]))},
-            "batched_over_single": (float(done.sum()) / dt) / (nsingle / dt1)}
+            "batched_over_single": (float(done.sum()) / dt) / (nsingle / dt1),
+            "lbfgs": lbfgs}
 
 
 def md_leg(maise_amd, args):

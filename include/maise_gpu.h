@@ -225,6 +225,16 @@ int mg_md_run_npt(mg_ctx *c, int nsteps, double dt, double T, double tau, double
  * per-structure displacement norm of one step.  steps[S] and done[S] (NULL skips) report iterations and
  * convergence; the final positions/energies/forces are read with mg_get / mg_get_batch. */
 int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done);
+/* Batched L-BFGS relaxation of every structure of the context (positions only), replacing the same caller with the
+ * quasi-Newton method the reference minimises with (CELL_MIN's default MINT 0, GSL vector_bfgs2, src/cmin.c:203-245),
+ * in its limited-memory form without a line search: per structure the two-loop recursion over the last `memory`
+ * (1..32) step / force-change pairs (pairs without positive curvature are skipped), gamma = s.y / y.y scaling, h0
+ * (A^2/eV) as the initial inverse Hessian, steepest descent with the memory dropped when the direction is not a
+ * descent direction, and every step scaled so no atom moves more than maxstep (A).  Stops a structure when
+ * max_i |F_i| < fmax (eV/A).  Same batching, compaction, graph replay and skin lists as mg_relax; steps/done as
+ * there. */
+int mg_relax_lbfgs(mg_ctx *c, int max_steps, double fmax, double maxstep, int memory, double h0, int *steps,
+                   int *done);
 /* Wall time of the last relaxation by segment between its host checks: up to max records of 4 doubles
  * (iterations done at the segment's end, active structures, active atoms, seconds) go to rec; returns the number
  * of records there are.  No counterpart in the reference. */

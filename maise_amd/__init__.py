@@ -81,6 +81,7 @@ def lib():
             "mg_md_run_npt": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp],
                               C.c_int),
             "mg_relax": ([vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, ip, ip], C.c_int),
+            "mg_relax_lbfgs": ([vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, ip, ip], C.c_int),
             "mg_relax_cell": ([vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                ip, ip], C.c_int),
             "mg_get_lattice": ([vp, C.c_int, dp], C.c_int),
@@ -397,6 +398,16 @@ class Context:
         done = np.zeros(self.S, dtype=np.int32)
         _check(lib().mg_relax(self._h, int(max_steps), float(fmax), float(dt0), float(dtmax), float(maxstep),
                               _ip(steps), _ip(done)), "mg_relax")
+        return steps, done.astype(bool)
+
+    def relax_lbfgs(self, max_steps: int = 1000, fmax: float = 0.01, maxstep: float = 0.2, memory: int = 20,
+                    h0: float = 1.0 / 70.0):
+        """Batched L-BFGS over all structures of the context (memory pairs per structure, initial inverse Hessian h0
+        in A^2/eV, no atom moving more than maxstep A per step); returns (steps[S], converged[S])."""
+        steps = np.zeros(self.S, dtype=np.int32)
+        done = np.zeros(self.S, dtype=np.int32)
+        _check(lib().mg_relax_lbfgs(self._h, int(max_steps), float(fmax), float(maxstep), int(memory), float(h0),
+                                    _ip(steps), _ip(done)), "mg_relax_lbfgs")
         return steps, done.astype(bool)
 
     def relax_log(self) -> np.ndarray:

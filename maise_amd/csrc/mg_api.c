@@ -220,7 +220,8 @@ void mg_ctx_free(mg_ctx *c) {
                   c->d_vir, c->d_fself, c->d_fe, c->d_gpl, c->d_npl, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
                   c->d_snap_dyn, c->d_cellv, c->d_snap_cellv, c->d_snap_str, c->d_ckey, c->d_crow, c->d_xb,
-                  c->d_mimg};
+                  c->d_mimg, c->d_lb_s, c->d_lb_y, c->d_lb_f0, c->d_lb_rho, c->d_lb_tmp, c->d_snap_lb_s,
+                  c->d_snap_lb_y, c->d_snap_lb_f0, c->d_snap_lb_rho, c->d_lb_map};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++)
     if (ptrs[k]) hipFree(ptrs[k]);
   if (c->st) hipStreamSynchronize(c->st);

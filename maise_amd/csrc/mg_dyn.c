@@ -1,6 +1,6 @@
 /*
  * mg_dyn.c -- host sequencing of the device-resident dynamics (mg_dyn.hip): NVT molecular dynamics and
- * batched FIRE relaxation.  Positions, velocities, forces and integrator state stay on the device; each step
+ * batched FIRE and L-BFGS relaxation.  Positions, velocities, forces and integrator state stay on the device; each step
  * is a fixed chain of launches on the context stream.
  *
  * Units: A, fs, eV.  Masses m = atomic mass (amu) * 103.6427 eV fs^2/A^2 and kB = 1/11604 eV/K, the
@@ -42,6 +42,12 @@ int mgk_fire_count(int S, const double *state, int *out, hipStream_t st);
 int mgk_fire_cell(int S, mg_struct_dev *str, const double *F, const double *U, double *v, double *cellv,
                   const double *x, double *xin, double *state, double p, double fmax, double smax, double dtmax,
                   double maxstep, hipStream_t st);
+int mgk_lbfgs_init(int S, double *state, double h0, hipStream_t st);
+int mgk_lbfgs(int S, int maxn, const mg_struct_dev *str, const double *F, double *dr, double *f0, double *hs, double *hy,
+              double *rho, int m, size_t hstride, const double *x, double *xin, double *state, double fmax,
+              double maxstep, double h0, hipStream_t st);
+int mgk_gather_rows(int nrows, const int *src, int width, int nslab, size_t in_stride, size_t out_stride,
+                    const double *in, double *out, hipStream_t st);
 
 #define KCHK(x) do { if ((x) != 0) return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error()); } while (0)
 
@@ -296,6 +302,13 @@ static int snapshot(mg_ctx *c, int restore) {
     HIPCHK(hipMemcpyAsync(c->d_snap_dyn, c->d_dyn, 16 * sizeof(double), k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_snap_cellv, c->d_cellv, sizeof(double) * 9 * c->S, k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_snap_str, c->d_str, sizeof(mg_struct_dev) * c->S, k, c->st));
+    if (c->lb_on) {
+      const size_t nr = sizeof(double) * c->lb_m * c->lb_stride;
+      HIPCHK(hipMemcpyAsync(c->d_snap_lb_s, c->d_lb_s, nr, k, c->st));
+      HIPCHK(hipMemcpyAsync(c->d_snap_lb_y, c->d_lb_y, nr, k, c->st));
+      HIPCHK(hipMemcpyAsync(c->d_snap_lb_f0, c->d_lb_f0, nx, k, c->st));
+      HIPCHK(hipMemcpyAsync(c->d_snap_lb_rho, c->d_lb_rho, sizeof(double) * c->lb_m * c->S, k, c->st));
+    }
   } else {
     HIPCHK(hipMemcpyAsync(c->d_xin, c->d_snap_x, nx, k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_v, c->d_snap_v, nx, k, c->st));
@@ -303,7 +316,71 @@ static int snapshot(mg_ctx *c, int restore) {
     HIPCHK(hipMemcpyAsync(c->d_dyn, c->d_snap_dyn, 16 * sizeof(double), k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_cellv, c->d_snap_cellv, sizeof(double) * 9 * c->S, k, c->st));
     HIPCHK(hipMemcpyAsync(c->d_str, c->d_snap_str, sizeof(mg_struct_dev) * c->S, k, c->st));
+    if (c->lb_on) {
+      const size_t nr = sizeof(double) * c->lb_m * c->lb_stride;
+      HIPCHK(hipMemcpyAsync(c->d_lb_s, c->d_snap_lb_s, nr, k, c->st));
+      HIPCHK(hipMemcpyAsync(c->d_lb_y, c->d_snap_lb_y, nr, k, c->st));
+      HIPCHK(hipMemcpyAsync(c->d_lb_f0, c->d_snap_lb_f0, nx, k, c->st));
+      HIPCHK(hipMemcpyAsync(c->d_lb_rho, c->d_snap_lb_rho, sizeof(double) * c->lb_m * c->S, k, c->st));
+    }
   }
+  return MG_OK;
+}
+
+/* L-BFGS buffers for the current batch with m pairs per structure (laid out for 3N; compaction only shrinks N) */
+static int ensure_lbfgs(mg_ctx *c, int m) {
+  int r;
+  const size_t ring = (size_t)m * 3 * (size_t)c->N + 16;
+  if (c->cap_lb_ring < ring || !c->d_lb_s) {
+    if ((r = mg_dev_alloc((void **)&c->d_lb_s, ring * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_lb_y, ring * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_lb_tmp, ring * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_lb_s, ring * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_lb_y, ring * sizeof(double))))
+      return r;
+    c->cap_lb_ring = ring;
+  }
+  if (c->cap_lb_atoms < (size_t)c->N + 16 || !c->d_lb_f0) {
+    const size_t n = (size_t)c->N + 16;
+    if ((r = mg_dev_alloc((void **)&c->d_lb_f0, n * 3 * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_lb_f0, n * 3 * sizeof(double))))
+      return r;
+    c->cap_lb_atoms = n;
+  }
+  if (c->cap_lb_str < (size_t)m * (c->S + 4) || !c->d_lb_rho) {
+    const size_t n = (size_t)m * (c->S + 4);
+    if ((r = mg_dev_alloc((void **)&c->d_lb_rho, n * sizeof(double))) ||
+        (r = mg_dev_alloc((void **)&c->d_snap_lb_rho, n * sizeof(double))))
+      return r;
+    c->cap_lb_str = n;
+  }
+  /* index maps of a re-pack: N + S ints at most (the map buffer is re-sized with the ring) */
+  if ((r = mg_dev_alloc((void **)&c->d_lb_map, ((size_t)c->N + (size_t)c->S + 16) * sizeof(int)))) return r;
+  c->lb_m = m;
+  c->lb_stride = 3 * (size_t)c->N;
+  return MG_OK;
+}
+
+/* After a re-pack: carry the L-BFGS rings, previous forces and 1/(y.s) of the kept structures into the new layout.
+ * amap[i'] = the old index of new atom i' (N1 of them), smap[k'] = the old slot of new structure k' (S1). */
+static int lbfgs_repack(mg_ctx *c, const int *amap, int N1, const int *smap, int S1, size_t old_stride) {
+  const int m = c->lb_m;
+  int *d_amap = c->d_lb_map, *d_smap = c->d_lb_map + N1;
+  HIPCHK(hipMemcpyAsync(d_amap, amap, sizeof(int) * N1, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(d_smap, smap, sizeof(int) * S1, hipMemcpyHostToDevice, c->st));
+  const size_t new_stride = 3 * (size_t)N1;
+  double *t;
+  KCHK(mgk_gather_rows(N1, d_amap, 3, m, old_stride, new_stride, c->d_lb_s, c->d_lb_tmp, c->st));
+  t = c->d_lb_s; c->d_lb_s = c->d_lb_tmp; c->d_lb_tmp = t;
+  KCHK(mgk_gather_rows(N1, d_amap, 3, m, old_stride, new_stride, c->d_lb_y, c->d_lb_tmp, c->st));
+  t = c->d_lb_y; c->d_lb_y = c->d_lb_tmp; c->d_lb_tmp = t;
+  KCHK(mgk_gather_rows(N1, d_amap, 3, 1, 0, 0, c->d_lb_f0, c->d_lb_tmp, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_lb_f0, c->d_lb_tmp, sizeof(double) * 3 * N1, hipMemcpyDeviceToDevice, c->st));
+  KCHK(mgk_gather_rows(S1, d_smap, m, 1, 0, 0, c->d_lb_rho, c->d_lb_tmp, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_lb_rho, c->d_lb_tmp, sizeof(double) * m * S1, hipMemcpyDeviceToDevice, c->st));
+  /* the maps are read by the gathers above before the host arrays go: wait here */
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->lb_stride = new_stride;
   return MG_OK;
 }
 
@@ -311,10 +388,18 @@ typedef struct {
   double fmax, dtmax, maxstep;
   int cell;            /* also relax the lattice (enthalpy at pressure p) */
   double p, smax;      /* eV/A^3 */
+  int lbfgs;           /* L-BFGS (positions only) instead of FIRE */
+  double h0;           /* its initial inverse Hessian (A^2/eV) */
+  int maxn;            /* most atoms in one structure of the batch (block size of the L-BFGS kernel) */
 } fire_arg;
 
 static int fire_tail(mg_ctx *c, void *arg) {
   const fire_arg *a = (const fire_arg *)arg;
+  if (a->lbfgs) {
+    const int e = mgk_lbfgs(c->S, a->maxn, c->d_str, c->d_F, c->d_v, c->d_lb_f0, c->d_lb_s, c->d_lb_y, c->d_lb_rho,
+                            c->lb_m, c->lb_stride, c->d_x, c->d_xin, c->d_fire, a->fmax, a->maxstep, a->h0, c->st);
+    return e ? mg_fail(MG_ERR_HIP, "kernel launch failed: L-BFGS step (%s)", mgk_last_error()) : MG_OK;
+  }
   const int e = a->cell ? mgk_fire_cell(c->S, c->d_str, c->d_F, c->d_U, c->d_v, c->d_cellv, c->d_x, c->d_xin,
                                         c->d_fire, a->p, a->fmax, a->smax, a->dtmax, a->maxstep, c->st)
                         : mgk_fire(c->S, c->d_str, c->d_F, c->d_v, c->d_x, c->d_xin, c->d_fire, a->fmax, a->dtmax,
@@ -391,10 +476,15 @@ static int compact(mg_ctx *c, relax_book *b, int repack) {
     double *lat = (double *)malloc(sizeof(double) * 9 * S1), *xn = (double *)malloc(sizeof(double) * 3 * b->N0);
     double *vn = (double *)malloc(sizeof(double) * 3 * b->N0), *sn = (double *)malloc(sizeof(double) * 5 * S1);
     double *cvn = (double *)malloc(sizeof(double) * 9 * (S1 > 0 ? S1 : 1));
+    /* L-BFGS: old atom of each new atom, old slot of each new structure */
+    int *amap = (int *)malloc(sizeof(int) * (N > 0 ? N : 1)), *smap = (int *)malloc(sizeof(int) * (S > 0 ? S : 1));
+    const size_t old_stride = 3 * (size_t)N;
     int kk = 0;
     for (int k = 0; k < S; k++) {
       if (st[5 * k + 3] != 0.0) continue;
       const int o = b->act[kk], n = b->off0[o + 1] - b->off0[o];
+      smap[kk] = k;
+      for (int a = 0; a < n; a++) amap[N1 + a] = c->h_atom_off[k] + a;
       off[kk] = N1;
       nd[kk] = b->nd0[o];
       xt[kk] = 1;
@@ -415,8 +505,10 @@ static int compact(mg_ctx *c, relax_book *b, int repack) {
           hipMemcpy(c->d_cellv, cvn, sizeof(double) * 9 * S1, hipMemcpyHostToDevice) != hipSuccess)
         r = mg_fail(MG_ERR_HIP, "mg_relax: state upload failed");
     }
+    if (r == MG_OK && c->lb_on) r = lbfgs_repack(c, amap, N1, smap, S1, old_stride);
     b->S = S1;
     free(off); free(nd); free(xt); free(spc); free(lat); free(xn); free(vn); free(sn); free(cvn);
+    free(amap); free(smap);
   }
 out:
   free(x); free(v); free(st); free(cv); free(sd);
@@ -424,8 +516,9 @@ out:
 }
 
 static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps, double fmax, double dt0, double dtmax,
-                      double maxstep, int *steps, int *done) {
-  if (!c || c->S < 1 || max_steps < 0 || fmax <= 0.0 || (cell && smax <= 0.0))
+                      double maxstep, int lbfgs, int mem, int *steps, int *done) {
+  if (!c || c->S < 1 || max_steps < 0 || fmax <= 0.0 || (cell && smax <= 0.0) ||
+      (lbfgs && (cell || mem < 1 || mem > 32 || dt0 <= 0.0 || maxstep <= 0.0)))
     return mg_fail(MG_ERR_ARG, "mg_relax: bad argument");
   HIPCHK(hipSetDevice(c->device));
   int r;
@@ -455,10 +548,21 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
 #define RKCHK(x) do { if ((x) != 0) { if (gx) hipGraphExecDestroy(gx); book_free(&b); return mg_fail(MG_ERR_HIP, "kernel launch failed: %s (%s)", #x, mgk_last_error()); } } while (0)
 #define RHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { if (gx) hipGraphExecDestroy(gx); book_free(&b); return mg_fail(MG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
   hipGraphExec_t gx = NULL;
-  fire_arg fa = {fmax, dtmax, maxstep, cell, p, smax};
+  int maxn = 0;
+  for (int s = 0; s < c->S; s++)
+    if (c->h_atom_off[s + 1] - c->h_atom_off[s] > maxn) maxn = c->h_atom_off[s + 1] - c->h_atom_off[s];
+  /* L-BFGS: dt0 is the initial inverse Hessian h0 (A^2/eV) */
+  fire_arg fa = {fmax, dtmax, maxstep, cell, p, smax, lbfgs, dt0, maxn};
   RHIP(hipMemsetAsync(c->d_v, 0, sizeof(double) * 3 * c->N, c->st));
   RHIP(hipMemsetAsync(c->d_cellv, 0, sizeof(double) * 9 * c->S, c->st));
-  RKCHK(mgk_fire_init(c->S, c->d_fire, dt0, c->st));
+  if (lbfgs) {
+    RCHK(ensure_lbfgs(c, mem));
+    RHIP(hipMemsetAsync(c->d_lb_rho, 0, sizeof(double) * mem * c->S, c->st));
+    RKCHK(mgk_lbfgs_init(c->S, c->d_fire, dt0, c->st));
+    c->lb_on = 1;
+  } else {
+    RKCHK(mgk_fire_init(c->S, c->d_fire, dt0, c->st));
+  }
   int it = 0, need_e = 0, need_r = 0;
   c->rlog_n = 0;
   c->skin_on = !cell && c->skin > 0.0;  /* positions only: the cells stay fixed, skin lists hold */
@@ -573,13 +677,20 @@ static int relax_impl(mg_ctx *c, int cell, double p, double smax, int max_steps,
 }
 
 int mg_relax(mg_ctx *c, int max_steps, double fmax, double dt0, double dtmax, double maxstep, int *steps, int *done) {
-  const int r = relax_impl(c, 0, 0.0, 0.0, max_steps, fmax, dt0, dtmax, maxstep, steps, done);
+  const int r = relax_impl(c, 0, 0.0, 0.0, max_steps, fmax, dt0, dtmax, maxstep, 0, 0, steps, done);
   if (c) c->skin_on = 0;  /* skin lists serve this run only (every exit path) */
+  return r;
+}
+
+int mg_relax_lbfgs(mg_ctx *c, int max_steps, double fmax, double maxstep, int memory, double h0, int *steps,
+                   int *done) {
+  const int r = relax_impl(c, 0, 0.0, 0.0, max_steps, fmax, h0, 0.0, maxstep, 1, memory, steps, done);
+  if (c) { c->skin_on = 0; c->lb_on = 0; }  /* this run only (every exit path) */
   return r;
 }
 
 int mg_relax_cell(mg_ctx *c, double p_gpa, int max_steps, double fmax, double smax_gpa, double dt0, double dtmax,
                   double maxstep, int *steps, int *done) {
   const double ev2gpa = 160.2176565; /* eV/A^3 in GPa, the reference's eV2GPa */
-  return relax_impl(c, 1, p_gpa / ev2gpa, smax_gpa / ev2gpa, max_steps, fmax, dt0, dtmax, maxstep, steps, done);
+  return relax_impl(c, 1, p_gpa / ev2gpa, smax_gpa / ev2gpa, max_steps, fmax, dt0, dtmax, maxstep, 0, 0, steps, done);
 }

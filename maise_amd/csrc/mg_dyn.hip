@@ -1,12 +1,12 @@
 // mg_dyn.hip -- device-resident dynamics: NVT molecular dynamics (velocity Verlet with a Nose-Hoover
-// thermostat) and batched FIRE relaxation.  State (positions, velocities, thermostat, FIRE scalars) never
-// leaves the device between steps; the host only sequences launches on the context stream.
+// thermostat) and batched FIRE and L-BFGS relaxation.  State (positions, velocities, thermostat, optimiser state)
+// never leaves the device between steps; the host only sequences launches on the context stream.
 //
 // Reference callers replaced: Dynamics/Maxwell/CELL_KIN (src/cmod.c:12-23,224-280,283-549) and the relaxation
 // driver CELL_RELX/CELL_MIN (src/cmod.c:901-977, src/cmin.c:177-301).  The reference integrates NVT with one
 // force call per step (x += v dt + f dt^2/2, v += f dt, f = F/m - xi v, xi += dt (2K/(3N kT) - 1)/tau^2) and
 // relaxes with GSL BFGS2; this build uses velocity Verlet with the same thermostat equation of motion
-// (half-step splitting) and FIRE (Bitzek et al., PRL 97, 170201 (2006)).
+// (half-step splitting), FIRE (Bitzek et al., PRL 97, 170201 (2006)) and limited-memory BFGS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -419,6 +419,195 @@ __global__ void k_fire_count(int S, const double* __restrict__ state, int* __res
   if (s < S && state[5 * s + 3] != 0.0) atomicAdd(out, 1);
 }
 
+// ---------------------------------------------------------------------------------------------- L-BFGS
+// Limited-memory BFGS (Nocedal, Math. Comp. 35 (1980) 773; two-loop recursion as Nocedal & Wright, Numerical
+// Optimization, Alg. 7.4), one block of 64 or 256 threads per structure, a thread per atom, every sum in FP64 in a
+// fixed order (wave butterfly, then the waves in order), so a run does not depend on how it is launched.
+// state[s] = {gamma, head, count, done, steps}: gamma = s.y / y.y of the newest pair (the initial inverse Hessian
+// of the recursion; h0 before the first pair), head = the ring slot the next pair goes to, count = stored pairs.
+// History ring: hs/hy[slot * hstride + 3 i + q] (hstride = 3 N), rho[s * m + slot] = 1 / (y.s).  dr holds the
+// previous step (it doubles as the recursion's work vector), f0 the forces at the previous positions.
+// A pair with y.s <= 1e-10 |y||s| (no positive curvature along the step) is not stored; a direction that is not a
+// descent direction (p.F <= 0) drops the memory and steps along h0 F.  The step is scaled so no atom moves more than
+// maxstep (A).
+#define LB_MAXM 32
+__device__ __forceinline__ double lb_wsum(double v) {
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, 64);
+  return v;
+}
+__device__ __forceinline__ double lb_wmax(double v) {
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) v = fmax(v, __shfl_xor(v, k, 64));
+  return v;
+}
+// block sum / max: red[] holds one partial per wave; the leading barrier keeps the previous call's readers ahead of
+// this call's writers
+__device__ __forceinline__ double lb_bsum(double v, double* red, int t, int nw) {
+  v = lb_wsum(v);
+  if (nw == 1) return v;
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < nw; w++) r += red[w];
+  return r;
+}
+__device__ __forceinline__ double lb_bmax(double v, double* red, int t, int nw) {
+  v = lb_wmax(v);
+  if (nw == 1) return v;
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int w = 1; w < nw; w++) r = fmax(r, red[w]);
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_lbfgs(int S, const mg_struct_dev* __restrict__ str, const double* __restrict__ F,
+                                               double* __restrict__ dr, double* __restrict__ f0, double* __restrict__ hs,
+                                               double* __restrict__ hy, double* __restrict__ rho, int m, size_t hstride,
+                                               const double* __restrict__ x, double* __restrict__ xin,
+                                               double* __restrict__ state, double ftol, double maxstep, double h0) {
+  __shared__ double red[4];
+  __shared__ double a_sh[LB_MAXM], rho_sh[LB_MAXM];
+  const int s = blockIdx.x, t = threadIdx.x, nt = blockDim.x, nw = nt >> 6;
+  const int a0 = str[s].atom0, n = str[s].natom;
+  double* st = state + 5 * s;
+  if (st[3] != 0.0) {  // converged: positions kept as they are (see k_fire)
+    for (int i = a0 + t; i < a0 + n; i += nt)
+      for (int q = 0; q < 3; q++) dr[3 * i + q] = 0.0;
+    return;
+  }
+  double fm = 0.0;
+  for (int i = a0 + t; i < a0 + n; i += nt) {
+    const double* f = F + 3 * i;
+    fm = fmax(fm, f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  }
+  fm = lb_bmax(fm, red, t, nw);
+  if (fm < ftol * ftol) {
+    for (int i = a0 + t; i < a0 + n; i += nt)
+      for (int q = 0; q < 3; q++) { xin[3 * i + q] = x[3 * i + q]; dr[3 * i + q] = 0.0; }
+    if (t == 0) st[3] = 1.0;
+    return;
+  }
+  double gam = st[0];
+  int head = (int)st[1], cnt = (int)st[2];
+  const double steps = st[4];
+  double* rs = rho + (size_t)s * m;
+  for (int k = t; k < m; k += nt) rho_sh[k] = rs[k];
+  // the last step and the change of the forces along it become the newest pair (positive curvature only)
+  if (steps > 0.0) {
+    double ys = 0.0, yy = 0.0, ss = 0.0;
+    for (int i = a0 + t; i < a0 + n; i += nt)
+      for (int q = 0; q < 3; q++) {
+        const double y = f0[3 * i + q] - F[3 * i + q], d = dr[3 * i + q];
+        ys += y * d; yy += y * y; ss += d * d;
+      }
+    ys = lb_bsum(ys, red, t, nw);
+    yy = lb_bsum(yy, red, t, nw);
+    ss = lb_bsum(ss, red, t, nw);
+    if (yy > 0.0 && ys > 1e-10 * sqrt(yy * ss)) {
+      double* ps = hs + (size_t)head * hstride;
+      double* py = hy + (size_t)head * hstride;
+      for (int i = a0 + t; i < a0 + n; i += nt)
+        for (int q = 0; q < 3; q++) {
+          ps[3 * i + q] = dr[3 * i + q];
+          py[3 * i + q] = f0[3 * i + q] - F[3 * i + q];
+        }
+      __syncthreads();  // rho_sh: loaded above by some threads, written here by one
+      if (t == 0) rho_sh[head] = 1.0 / ys;
+      head = head + 1 == m ? 0 : head + 1;
+      cnt = cnt < m ? cnt + 1 : m;
+      gam = ys / yy;
+    }
+  }
+  __syncthreads();
+  // two-loop recursion on q = F, the negative gradient, so it ends with the search direction p = H F (kept in dr;
+  // each thread touches only its own atoms' components)
+  for (int i = a0 + t; i < a0 + n; i += nt)
+    for (int q = 0; q < 3; q++) dr[3 * i + q] = F[3 * i + q];
+  for (int k = 0; k < cnt; k++) {
+    int slot = head - 1 - k;
+    if (slot < 0) slot += m;
+    const double* ps = hs + (size_t)slot * hstride;
+    const double* py = hy + (size_t)slot * hstride;
+    double d = 0.0;
+    for (int i = a0 + t; i < a0 + n; i += nt)
+      for (int q = 0; q < 3; q++) d += ps[3 * i + q] * dr[3 * i + q];
+    const double a = rho_sh[slot] * lb_bsum(d, red, t, nw);
+    if (t == 0) a_sh[k] = a;
+    for (int i = a0 + t; i < a0 + n; i += nt)
+      for (int q = 0; q < 3; q++) dr[3 * i + q] -= a * py[3 * i + q];
+  }
+  const double g = cnt > 0 ? gam : h0;
+  for (int i = a0 + t; i < a0 + n; i += nt)
+    for (int q = 0; q < 3; q++) dr[3 * i + q] *= g;
+  __syncthreads();  // a_sh
+  for (int k = cnt - 1; k >= 0; k--) {
+    int slot = head - 1 - k;
+    if (slot < 0) slot += m;
+    const double* ps = hs + (size_t)slot * hstride;
+    const double* py = hy + (size_t)slot * hstride;
+    double d = 0.0;
+    for (int i = a0 + t; i < a0 + n; i += nt)
+      for (int q = 0; q < 3; q++) d += py[3 * i + q] * dr[3 * i + q];
+    const double c = a_sh[k] - rho_sh[slot] * lb_bsum(d, red, t, nw);
+    for (int i = a0 + t; i < a0 + n; i += nt)
+      for (int q = 0; q < 3; q++) dr[3 * i + q] += c * ps[3 * i + q];
+  }
+  // dr = p = H F; not a descent direction (p.F <= 0): steepest descent, memory dropped
+  double pf = 0.0;
+  for (int i = a0 + t; i < a0 + n; i += nt)
+    for (int q = 0; q < 3; q++) pf += dr[3 * i + q] * F[3 * i + q];
+  pf = lb_bsum(pf, red, t, nw);
+  const bool sd = !(pf > 0.0);
+  if (sd) {
+    cnt = 0;
+    gam = h0;
+    for (int i = a0 + t; i < a0 + n; i += nt)
+      for (int q = 0; q < 3; q++) dr[3 * i + q] = h0 * F[3 * i + q];
+  }
+  double l2 = 0.0;
+  for (int i = a0 + t; i < a0 + n; i += nt) {
+    const double* d = dr + 3 * i;
+    l2 = fmax(l2, d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  }
+  l2 = lb_bmax(l2, red, t, nw);
+  const double sc = l2 > maxstep * maxstep ? maxstep / sqrt(l2) : 1.0;
+  for (int i = a0 + t; i < a0 + n; i += nt)
+    for (int q = 0; q < 3; q++) {
+      const double d = sc * dr[3 * i + q];
+      dr[3 * i + q] = d;
+      xin[3 * i + q] = x[3 * i + q] + d;
+      f0[3 * i + q] = F[3 * i + q];
+    }
+  __syncthreads();  // rho_sh
+  for (int k = t; k < m; k += nt) rs[k] = rho_sh[k];
+  if (t == 0) {
+    st[0] = gam; st[1] = (double)head; st[2] = (double)cnt; st[4] = steps + 1.0;
+  }
+}
+
+__global__ void k_lbfgs_init(int S, double* __restrict__ state, double h0) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double* st = state + 5 * s;
+  st[0] = h0; st[1] = 0.0; st[2] = 0.0; st[3] = 0.0; st[4] = 0.0;
+}
+
+// out[k * out_stride + r * width + w] = in[k * in_stride + src[r] * width + w] for k < nslab, r < nrows (the
+// optimiser's per-atom and per-structure arrays carried over when a batch is re-packed)
+__global__ void k_gather_rows(int nrows, const int* __restrict__ src, int width, int nslab, size_t in_stride,
+                              size_t out_stride, const double* __restrict__ in, double* __restrict__ out) {
+  const size_t per = (size_t)nrows * width;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= per * nslab) return;
+  const size_t k = idx / per, e = idx - k * per;
+  const int r = (int)(e / width), w = (int)(e - (size_t)r * width);
+  out[k * out_stride + (size_t)r * width + w] = in[k * in_stride + (size_t)src[r] * width + w];
+}
+
 extern "C" {
 
 int mgk_md_maxwell(int N, unsigned long long seed, const double* mass, double kT, double* v, hipStream_t st) {
@@ -487,6 +676,31 @@ int mgk_fire_cell(int S, mg_struct_dev* str, const double* F, const double* U, d
 
 int mgk_fire_count(int S, const double* state, int* out, hipStream_t st) {
   hipLaunchKernelGGL(k_fire_count, dim3((S + 255) / 256), dim3(256), 0, st, S, state, out);
+  return mgk_status();
+}
+
+int mgk_lbfgs_init(int S, double* state, double h0, hipStream_t st) {
+  hipLaunchKernelGGL(k_lbfgs_init, dim3((S + 255) / 256), dim3(256), 0, st, S, state, h0);
+  return mgk_status();
+}
+
+// block of one wave when every structure has at most 64 atoms (no barriers in the sums), else four waves
+int mgk_lbfgs(int S, int maxn, const mg_struct_dev* str, const double* F, double* dr, double* f0, double* hs, double* hy,
+              double* rho, int m, size_t hstride, const double* x, double* xin, double* state, double fmax,
+              double maxstep, double h0, hipStream_t st) {
+  if (m < 1 || m > LB_MAXM) return -1;
+  const int nt = maxn <= 64 ? 64 : 256;
+  hipLaunchKernelGGL(k_lbfgs, dim3(S), dim3(nt), 0, st, S, str, F, dr, f0, hs, hy, rho, m, hstride, x, xin, state,
+                     fmax, maxstep, h0);
+  return mgk_status();
+}
+
+int mgk_gather_rows(int nrows, const int* src, int width, int nslab, size_t in_stride, size_t out_stride,
+                    const double* in, double* out, hipStream_t st) {
+  const size_t tot = (size_t)nrows * width * nslab;
+  if (tot == 0) return 0;
+  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nrows, src, width, nslab,
+                     in_stride, out_stride, in, out);
   return mgk_status();
 }
 

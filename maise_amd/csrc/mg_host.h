@@ -136,6 +136,15 @@ struct mg_ctx {
   double *d_snap_x, *d_snap_v, *d_snap_fire, *d_snap_dyn, *d_snap_cellv; /* state at the last graph checkpoint */
   mg_struct_dev *d_snap_str;                                             /* (lattices change in cell relaxation) */
   size_t cap_dyn_atoms, cap_dyn_str, cap_trace;
+  /* L-BFGS relaxation (mg_relax_lbfgs): history ring [m][3N] of steps (d_lb_s) and force changes (d_lb_y), forces
+   * at the previous positions [3N], 1/(y.s) [S][m], their snapshots at the last graph checkpoint, a gather buffer
+   * and an index map for re-packing the batch; lb_stride = 3N of the batch the rings are laid out for */
+  double *d_lb_s, *d_lb_y, *d_lb_f0, *d_lb_rho, *d_lb_tmp;
+  double *d_snap_lb_s, *d_snap_lb_y, *d_snap_lb_f0, *d_snap_lb_rho;
+  int *d_lb_map;
+  size_t cap_lb_ring, cap_lb_atoms, cap_lb_str;
+  int lb_on, lb_m;
+  size_t lb_stride;
   /* last relaxation's wall time by segment between host checks: (iterations done, active structures, active
    * atoms, seconds) per record */
   double *rlog;

@@ -225,3 +225,62 @@ def fire_relax(model: "OracleModel", spc, lat, x, fmax=0.01, max_steps=2000, dt0
         x = x + dr
     o = model.evaluate(spc, lat, x, urep=urep)
     return o["X"], o["E"], max_steps, False
+
+
+def lbfgs_direction(S, Y, gamma, q):
+    """H q by the L-BFGS two-loop recursion (Nocedal & Wright, Numerical Optimization, Alg. 7.4): pairs (s, y) oldest
+    first, rho = 1 / (y.s), initial inverse Hessian gamma I."""
+    q = np.array(q, dtype=np.float64).copy()
+    a = np.zeros(len(S))
+    for i in range(len(S) - 1, -1, -1):
+        a[i] = (S[i] @ q) / (Y[i] @ S[i])
+        q -= a[i] * Y[i]
+    z = gamma * q
+    for i in range(len(S)):
+        b = (Y[i] @ z) / (Y[i] @ S[i])
+        z += S[i] * (a[i] - b)
+    return z
+
+
+def lbfgs_relax(model: "OracleModel", spc, lat, x, fmax=0.01, max_steps=2000, maxstep=0.2, memory=20, h0=1.0 / 70.0,
+                urep=0.0, return_x_after=None):
+    """Limited-memory BFGS without a line search on FP64 oracle forces, the restatement the GPU batched L-BFGS
+    (mg_relax_lbfgs) is checked against: the newest (s, y) = (last step, F_old - F_new) pair is kept when
+    y.s > 1e-10 |y||s| (at most `memory` pairs), gamma = s.y / y.y of the newest kept pair (h0 before any), direction
+    p = H F; p.F <= 0 drops the memory and takes p = h0 F; the step is scaled so no atom moves more than maxstep.
+    Returns (x, E, steps, converged); with return_x_after = k, the (unwrapped) positions after k steps instead."""
+    x = np.array(x, dtype=np.float64).reshape(-1, 3).copy()
+    S, Y = [], []
+    gamma = h0
+    f0 = dr = None
+    for it in range(max_steps):
+        if return_x_after is not None and it == return_x_after:
+            return x
+        o = model.evaluate(spc, lat, x, urep=urep, threshold=False)
+        F = o["F"].reshape(-1)
+        if np.max(np.sum(o["F"] * o["F"], axis=1)) < fmax * fmax:
+            return o["X"], o["E"], it, True
+        if f0 is not None:
+            y = f0 - F
+            ys, yy, ss = float(y @ dr), float(y @ y), float(dr @ dr)
+            if yy > 0.0 and ys > 1e-10 * np.sqrt(yy * ss):
+                S.append(dr.copy())
+                Y.append(y)
+                if len(S) > memory:
+                    S.pop(0)
+                    Y.pop(0)
+                gamma = ys / yy
+        p = lbfgs_direction(S, Y, gamma if S else h0, F)
+        if not float(p @ F) > 0.0:
+            S, Y = [], []
+            gamma = h0
+            p = h0 * F
+        l2 = float(np.max(np.sum(p.reshape(-1, 3) ** 2, axis=1)))
+        if l2 > maxstep * maxstep:
+            p = p * (maxstep / np.sqrt(l2))
+        x = x + p.reshape(-1, 3)
+        dr, f0 = p, F
+    if return_x_after is not None:
+        return x
+    o = model.evaluate(spc, lat, x, urep=urep)
+    return o["X"], o["E"], max_steps, False
