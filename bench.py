@@ -227,11 +227,16 @@ def relax_leg(maise_amd, batch, args):
     c.close()
     # wall time by the share of structures still active in each segment between host checks
     S0 = float(len(batch))
-    split = {}
-    for name, lo, hi in (("active_ge_50pct", 0.5, 2.0), ("active_10_50pct", 0.1, 0.5), ("active_lt_10pct", 0.0, 0.1)):
-        sel = (log[:, 1] / S0 >= lo) & (log[:, 1] / S0 < hi)
-        split[name] = {"seconds": float(log[sel, 3].sum()),
-                       "iterations": int(np.diff(np.concatenate([[0], log[:, 0]]))[sel].sum())}
+
+    def wall_split(log):
+        split = {}
+        for name, lo, hi in (("active_ge_50pct", 0.5, 2.0), ("active_10_50pct", 0.1, 0.5),
+                             ("active_lt_10pct", 0.0, 0.1)):
+            sel = (log[:, 1] / S0 >= lo) & (log[:, 1] / S0 < hi)
+            split[name] = {"seconds": float(log[sel, 3].sum()),
+                           "iterations": int(np.diff(np.concatenate([[0], log[:, 0]]))[sel].sum())}
+        return split
+    split = wall_split(log)
     # the same batch with the device L-BFGS (the reference minimises with GSL's BFGS2; no line search here)
     c = maise_amd.Context(model, device=0, urep=100.0)
     c.set_batch(batch[:64])
@@ -243,6 +248,7 @@ def relax_leg(maise_amd, batch, args):
     lsteps, ldone = c.relax_lbfgs(max_steps=max_steps, fmax=fmax)
     ldt = time.perf_counter() - t0
     lres = c.batch_result()
+    lsplit = wall_split(c.relax_log())
     c.close()
     nat = np.array([len(b[0]) for b in batch], dtype=np.float64)
     ef, el = fres["E"] / nat, lres["E"] / nat
@@ -251,7 +257,9 @@ def relax_leg(maise_amd, batch, args):
              "speedup_over_fire": (float(ldone.sum()) / ldt) / (float(done.sum()) / dt),
              "relaxed_energy_per_atom_mean_eV": {"fire": float(ef.mean()), "lbfgs": float(el.mean())},
              "share_lower_than_fire_by_1meV": float(np.mean(el < ef - 1e-3)),
-             "share_higher_than_fire_by_1meV": float(np.mean(el > ef + 1e-3))}
+             "share_higher_than_fire_by_1meV": float(np.mean(el > ef + 1e-3)),
+             "wall_by_active_share": lsplit,
+             "tail_share_of_wall": lsplit["active_lt_10pct"]["seconds"] / max(ldt, 1e-12)}
     progress(f"relax: L-BFGS {lbfgs['relaxed_structures_per_s']:.0f} structures/s, steps mean "
              f"{lbfgs['steps_mean']:.1f} max {lbfgs['steps_max']}")
     nsingle = min(args.relax_single, len(batch))
@@ -331,7 +339,8 @@ def ga_leg(maise_amd, args):
     (1) an audited run: every generation's offspring are also relaxed in full, outside the search clock, for recall
     and error; (2) the same run in compensated BF16 (BF16x3); (3) the same seeded search with and without screening
     for equal search time, at population 512 and at 2048; (4) a sample of offspring relaxed one structure per context
-    gives the plain-path rate."""
+    gives the plain-path rate; (5) the search with the device L-BFGS in place of FIRE (screened, screened in BF16x3,
+    every offspring relaxed) and the same sample relaxed one structure per context with it."""
     from maise_amd.evolve import GAConfig, SurrogateGA, compare_screening, make_offspring
     cfg = GAConfig(pop=args.ga_pop, generations=args.ga_gens, seed=5, relax_cap=300)
     t0 = time.perf_counter()
@@ -352,6 +361,11 @@ def ga_leg(maise_amd, args):
         c1.set_batch([k])
         c1.relax(max_steps=cfg.max_steps, fmax=cfg.fmax)
     single = len(kids) / (time.perf_counter() - t1)
+    t1 = time.perf_counter()
+    for k in kids:
+        c1.set_batch([k])
+        c1.relax_lbfgs(max_steps=cfg.max_steps, fmax=cfg.fmax)
+    lb_single = len(kids) / (time.perf_counter() - t1)
     c1.close()
     ga.close()
     # the same screened search with the MLP in compensated BF16 (BF16x3), no audit
@@ -361,6 +375,15 @@ def ga_leg(maise_amd, args):
     rb = gb.run()
     gb.close()
     tb = float(np.mean([r.seconds_screened for r in rb.reports[1:]]))
+    # with the device L-BFGS in place of FIRE: screened in FP32, screened in BF16x3, and every offspring relaxed
+    lb = {}
+    for name, over in (("screened", {}), ("screened_bf16x3", {"precision": maise_amd.MG_PREC_BF16X3}),
+                       ("batched_full", {"screen": False})):
+        gl = SurrogateGA(GA_POT, dataclasses.replace(cfg, optimizer="lbfgs", audit=False, **over))
+        rl = gl.run()
+        gl.close()
+        lb[name] = cfg.pop / float(np.mean([r.seconds_screened for r in rl.reports[1:]]))
+        progress(f"search with L-BFGS, {name}: {lb[name]:.0f} offspring/s")
     def trim(cmp_):
         for name in cmp_["runs"]:
             for r in cmp_["runs"][name]:
@@ -406,6 +429,9 @@ def ga_leg(maise_amd, args):
             "train_share_of_generation": float(np.mean([r.seconds_train / r.seconds_screened for r in reps])),
             "carried_per_generation_mean": float(np.mean([r.carried for r in reps])),
             "best_eV_atom_first_last": [res.reports[0].best_eV_atom, res.reports[-1].best_eV_atom],
+            "lbfgs": {"offspring_per_s": lb, "offspring_per_s_single_structure": lb_single,
+                      "screened_bf16x3_over_plain_fire_single": lb["screened_bf16x3"] / single,
+                      "screened_bf16x3_over_plain_lbfgs_single": lb["screened_bf16x3"] / lb_single},
             "equal_time_comparison": cmp_,
             "equal_time_comparison_large_population": cmp_large,
             "seconds_total": wall}

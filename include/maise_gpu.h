@@ -168,6 +168,12 @@ int mg_sur_set_norm(mg_sur *s, const float *mu, const float *sig, float ymu, flo
 /* epochs of seeded-shuffle mini-batch Adam over X[n][nin], y[n]; loss = final mean squared error (normalised) */
 int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, int batch, float lr,
                  unsigned long long seed, float *loss);
+/* The same run split in two: _begin copies the samples and enqueues every step on the surrogate's stream and returns
+ * (the caller's arrays are free at once), _end waits for it and gives the final loss.  Any other call on the
+ * surrogate waits for a run in flight first.  No counterpart in the reference. */
+int mg_sur_train_begin(mg_sur *s, const float *X, const float *y, int n, int epochs, int batch, float lr,
+                       unsigned long long seed);
+int mg_sur_train_end(mg_sur *s, float *loss);
 /* gradient (mg_sur_nparams floats) and value of the mean squared error over all n samples, parameters unchanged */
 int mg_sur_grad(mg_sur *s, const float *X, const float *y, int n, float *grad, float *loss);
 int mg_sur_predict(mg_sur *s, const float *X, int n, float *y);
@@ -200,6 +206,14 @@ int mg_evo_mutate(int npar, int N, int nspc, const int *spc, const double *lat, 
  * The dot product of two fingerprints is CxC (src/cell.c:417-427), the structural part of COMP_CL. */
 int mg_evo_fingerprint(int S, const int *atom_off, int nspc, const int *spc, const double *lat, const double *x,
                        int nrdf, double rmax, double rmin, double sigma, float *out);
+/* The duplicate walk of the population update (RANK_TR's test with COMP_CL, src/emod.c:351-366,
+ * src/eutl.c:394-413): the n structures in increasing energy E (ties by index; non-finite skipped) are kept unless
+ * 0.8 CxC + 0.1 exp(-dE^2/(2 0.05^2)) + 0.1 exp(-dV^2/(2 0.01^2)) > cut against one already kept, where
+ * dV = (V_a - V_b)/(2 (V_a + V_b)); pairs outside E_k - E_kept <= de_win, |dV| <= dv_win are not scored.  The first
+ * n_unique structures are never compared with each other.  G[(k - n_unique) n + j] = CxC of structure k >= n_unique
+ * with j (may be NULL when cut >= 1).  Up to limit indices go to keep; returns their number. */
+int mg_evo_unique(int n, const double *E, const double *V, const float *G, int n_unique, double cut, double de_win,
+                  double dv_win, int limit, int *keep);
 
 /* --- device-resident dynamics (state stays on the device between steps) --- */
 /* NVT molecular dynamics, replacing Dynamics/Maxwell (src/cmod.c:224-549): velocity Verlet with a Nose-Hoover

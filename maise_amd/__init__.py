@@ -92,6 +92,8 @@ def lib():
             "mg_sur_set_params": ([vp, fp], C.c_int),
             "mg_sur_set_norm": ([vp, fp, fp, C.c_float, C.c_float], C.c_int),
             "mg_sur_train": ([vp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp], C.c_int),
+            "mg_sur_train_begin": ([vp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong], C.c_int),
+            "mg_sur_train_end": ([vp, fp], C.c_int),
             "mg_sur_grad": ([vp, fp, fp, C.c_int, fp, fp], C.c_int),
             "mg_sur_predict": ([vp, fp, C.c_int, fp], C.c_int),
             "mg_features": ([vp, fp], C.c_int),
@@ -102,6 +104,8 @@ def lib():
                                C.c_double, C.c_ulonglong, C.c_int, ip, dp, dp, ip], C.c_int),
             "mg_evo_fingerprint": ([C.c_int, ip, C.c_int, ip, dp, dp, C.c_int, C.c_double, C.c_double, C.c_double,
                                     fp], C.c_int),
+            "mg_evo_unique": ([C.c_int, dp, dp, fp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, ip],
+                              C.c_int),
             "mg_poscar_read": ([C.c_char_p, C.c_int, C.c_int, C.c_char_p, dp, ip, ip, ip, ip, dp, ip, dp, ip],
                                C.c_int),
             "mg_poscar_write": ([C.c_char_p, C.c_char_p, C.c_double, dp, C.c_int, C.c_int, ip, ip, C.c_int, dp, ip,

@@ -493,3 +493,53 @@ int mg_evo_fingerprint(int S, const int *atom_off, int nspc, const int *spc, con
   free(fk);
   return MG_OK;
 }
+
+/* Duplicate walk of the population update: structures in increasing energy (ties by index, non-finite energies
+ * skipped); a structure is kept unless COMP_CL (src/eutl.c:394-413) against one already kept exceeds cut:
+ *   w = 0.8 CxC + 0.1 exp(-dE^2 / (2 0.05^2)) + 0.1 exp(-dV^2 / (2 0.01^2)),  dV = (V_a - V_b) / (2 (V_a + V_b)),
+ * scored only for pairs with E_k - E_kept <= de_win and |dV| <= dv_win (outside them w cannot exceed cut when
+ * cut >= 0.95).  The first n_unique structures are free of duplicates among themselves, so two of them are never
+ * compared.  G[(k - n_unique) * n + j] = CxC of structure k >= n_unique with structure j.  Stops at limit kept;
+ * returns their number (indices in keep, in the order kept). */
+typedef struct { double e; int k; } ek_pair;
+static int ek_cmp(const void *a, const void *b) {
+  const ek_pair *x = (const ek_pair *)a, *y = (const ek_pair *)b;
+  if (x->e < y->e) return -1;
+  if (x->e > y->e) return 1;
+  return x->k < y->k ? -1 : (x->k > y->k);
+}
+
+int mg_evo_unique(int n, const double *E, const double *V, const float *G, int n_unique, double cut, double de_win,
+                  double dv_win, int limit, int *keep) {
+  if (n < 0 || (n > 0 && (!E || !V || !keep)) || n_unique < 0 || n_unique > n || limit < 0)
+    return mg_fail(MG_ERR_ARG, "mg_evo_unique: bad argument");
+  const int test = cut < 1.0 && n > n_unique;
+  if (test && !G) return mg_fail(MG_ERR_ARG, "mg_evo_unique: fingerprint products missing");
+  ek_pair *ord = (ek_pair *)malloc(sizeof(ek_pair) * (n > 0 ? n : 1));
+  if (!ord) return mg_fail(MG_ERR_ALLOC, "mg_evo_unique: out of memory");
+  int m = 0;
+  for (int k = 0; k < n; k++)
+    if (isfinite(E[k])) { ord[m].e = E[k]; ord[m].k = k; m++; }
+  qsort(ord, m, sizeof(ek_pair), ek_cmp);
+  int nk = 0;
+  const double se = 1.0 / (0.05 * 0.05), sv = 1.0 / (0.01 * 0.01);
+  for (int t = 0; t < m && nk < limit; t++) {
+    const int k = ord[t].k;
+    int dup = 0;
+    if (test)
+      for (int c = 0; c < nk && !dup; c++) {
+        const int j = keep[c];
+        if (k < n_unique && j < n_unique) continue;  /* two members of the duplicate-free block */
+        const double de = E[j] - E[k];
+        const double dv = 0.5 * (V[k] - V[j]) / (V[k] + V[j]);
+        if (-de > de_win || fabs(dv) > dv_win) continue;
+        const double cxc = k >= n_unique ? (double)G[(size_t)(k - n_unique) * n + j]
+                                         : (double)G[(size_t)(j - n_unique) * n + k];
+        const double w = 0.8 * cxc + 0.1 * exp(-0.5 * de * de * se) + 0.1 * exp(-0.5 * dv * dv * sv);
+        if (w > cut) dup = 1;
+      }
+    if (!dup) keep[nk++] = k;
+  }
+  free(ord);
+  return nk;
+}

@@ -3,8 +3,9 @@
  * initialisation, normalisation, mini-batch Adam training, full-batch gradients and prediction, and the
  * per-structure descriptor features drawn from a context's last evaluation.
  *
- * Training data lives on the device for the duration of one mg_sur_train call; the parameters, the Adam
- * moments and the normalisation stay on the device between calls.
+ * Training data lives on the device for the duration of one training run; the parameters, the Adam moments and
+ * the normalisation stay on the device between calls.  A run can be left in flight (mg_sur_train_begin) while the
+ * host does other work; every other call on the surrogate waits for it first.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -34,7 +35,16 @@ struct mg_sur {
   float *d_X, *d_Y, *d_out;
   int *d_idx;
   size_t cap_n, cap_idx;
+  /* a training run in flight (mg_sur_train_begin): its samples, sample indices and loss partials in pinned host
+   * memory until mg_sur_train_end has waited for it */
+  int pending, pend_n, pend_nblk;
+  float *h_X, *h_Y, *h_ploss;
+  int *h_perm;
+  size_t cap_hx, cap_hperm, cap_hploss;
+  float last_loss;
 };
+
+static int settle(mg_sur *s);
 
 static unsigned long long splitmix64(unsigned long long *s) {
   unsigned long long z = (*s += 0x9E3779B97F4A7C15ull);
@@ -99,6 +109,10 @@ int mg_sur_create(int device, int nin, unsigned long long seed, mg_sur **out) {
 void mg_sur_free(mg_sur *s) {
   if (!s) return;
   hipSetDevice(s->device);
+  (void)settle(s);
+  if (s->h_X) hipHostFree(s->h_X);
+  if (s->h_perm) hipHostFree(s->h_perm);
+  if (s->h_ploss) hipHostFree(s->h_ploss);
   float *bufs[] = {s->d_p, s->d_m, s->d_v, s->d_norm, s->d_grad, s->d_part, s->d_ploss, s->d_X, s->d_Y, s->d_out};
   for (size_t k = 0; k < sizeof bufs / sizeof bufs[0]; k++)
     if (bufs[k]) hipFree(bufs[k]);
@@ -110,6 +124,7 @@ void mg_sur_free(mg_sur *s) {
 int mg_sur_get_params(mg_sur *s, float *p) {
   if (!s || !p) return mg_fail(MG_ERR_ARG, "mg_sur_get_params: bad argument");
   HIPCHK(hipSetDevice(s->device));
+  { const int r_ = settle(s); if (r_) return r_; }
   HIPCHK(hipMemcpy(p, s->d_p, sizeof(float) * s->P, hipMemcpyDeviceToHost));
   return MG_OK;
 }
@@ -117,6 +132,7 @@ int mg_sur_get_params(mg_sur *s, float *p) {
 int mg_sur_set_params(mg_sur *s, const float *p) {
   if (!s || !p) return mg_fail(MG_ERR_ARG, "mg_sur_set_params: bad argument");
   HIPCHK(hipSetDevice(s->device));
+  { const int r_ = settle(s); if (r_) return r_; }
   HIPCHK(hipMemcpy(s->d_p, p, sizeof(float) * s->P, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(s->d_m, 0, sizeof(float) * s->P));
   HIPCHK(hipMemset(s->d_v, 0, sizeof(float) * s->P));
@@ -126,6 +142,8 @@ int mg_sur_set_params(mg_sur *s, const float *p) {
 
 int mg_sur_set_norm(mg_sur *s, const float *mu, const float *sig, float ymu, float ysig) {
   if (!s || !mu || !sig || !(ysig > 0.0f)) return mg_fail(MG_ERR_ARG, "mg_sur_set_norm: bad argument");
+  HIPCHK(hipSetDevice(s->device));
+  { const int r_ = settle(s); if (r_) return r_; }
   for (int k = 0; k < s->nin; k++)
     if (!(sig[k] > 0.0f)) return mg_fail(MG_ERR_ARG, "mg_sur_set_norm: sig[%d] must be positive", k);
   float *norm = (float *)malloc(sizeof(float) * (2 * s->nin + 2));
@@ -182,54 +200,91 @@ static int upload_xy(mg_sur *s, const float *X, const float *y, int n) {
   return MG_OK;
 }
 
-/* mean squared error (normalised units) of the current parameters over the n samples now on the device */
-static int full_loss(mg_sur *s, int n, float *loss, float *grad) {
+/* pinned host buffer of at least `bytes` (grown on demand) */
+static int ensure_pinned(void **p, size_t *cap, size_t bytes) {
+  if (*cap >= bytes && *p) return MG_OK;
+  if (*p) hipHostFree(*p);
+  *p = NULL;
+  *cap = 0;
+  HIPCHK(hipHostMalloc(p, bytes, 0));
+  *cap = bytes;
+  return MG_OK;
+}
+
+/* mean squared error (normalised units) of the current parameters over the n samples now on the device: the
+ * kernels and the copy of the per-block partials to pinned memory are enqueued; loss_finish sums them */
+static int loss_enqueue(mg_sur *s, int n, float *grad) {
   const int nblk = grad_blocks(n);
+  int r;
+  if ((r = ensure_pinned((void **)&s->h_ploss, &s->cap_hploss, sizeof(float) * nblk))) return r;
   if (mgk_sur_grad(n, nblk, s->nin, NULL, s->d_X, s->d_Y, s->d_p, s->d_norm, s->d_part, s->d_ploss, s->st) ||
       mgk_sur_adam(s->P, nblk, s->d_part, s->d_p, s->d_m, s->d_v, s->d_grad, 0.0f, 0.f, 0.f, 0.f, 1.f, 1.f, s->st))
     return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
-  float *pl = (float *)malloc(sizeof(float) * nblk);
-  hipError_t e = hipMemcpyAsync(pl, s->d_ploss, sizeof(float) * nblk, hipMemcpyDeviceToHost, s->st);
-  if (e == hipSuccess && grad) e = hipMemcpyAsync(grad, s->d_grad, sizeof(float) * s->P, hipMemcpyDeviceToHost, s->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
-  double sum = 0.0;
-  for (int b = 0; b < nblk; b++) sum += pl[b];
-  free(pl);
-  if (e != hipSuccess) return mg_fail(MG_ERR_HIP, "surrogate loss: %s", hipGetErrorString(e));
-  if (loss) *loss = (float)(sum / n);
+  HIPCHK(hipMemcpyAsync(s->h_ploss, s->d_ploss, sizeof(float) * nblk, hipMemcpyDeviceToHost, s->st));
+  if (grad) HIPCHK(hipMemcpyAsync(grad, s->d_grad, sizeof(float) * s->P, hipMemcpyDeviceToHost, s->st));
+  s->pend_n = n;
+  s->pend_nblk = nblk;
   return MG_OK;
+}
+
+static int loss_finish(mg_sur *s, float *loss) {
+  HIPCHK(hipStreamSynchronize(s->st));
+  double sum = 0.0;
+  for (int b = 0; b < s->pend_nblk; b++) sum += s->h_ploss[b];  /* fixed order */
+  if (loss) *loss = (float)(sum / s->pend_n);
+  return MG_OK;
+}
+
+static int full_loss(mg_sur *s, int n, float *loss, float *grad) {
+  int r;
+  if ((r = loss_enqueue(s, n, grad))) return r;
+  return loss_finish(s, loss);
+}
+
+/* wait for a training run still in flight; its final loss is kept for mg_sur_train_end */
+static int settle(mg_sur *s) {
+  if (!s->pending) return MG_OK;
+  s->pending = 0;
+  return loss_finish(s, &s->last_loss);
 }
 
 int mg_sur_grad(mg_sur *s, const float *X, const float *y, int n, float *grad, float *loss) {
   if (!s || !X || !y || n < 1) return mg_fail(MG_ERR_ARG, "mg_sur_grad: bad argument");
   HIPCHK(hipSetDevice(s->device));
   int r;
+  if ((r = settle(s))) return r;
   if ((r = ensure_data(s, n, 0, 1)) || (r = upload_xy(s, X, y, n))) return r;
   return full_loss(s, n, loss, grad);
 }
 
-int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, int batch, float lr,
-                 unsigned long long seed, float *loss) {
+int mg_sur_train_begin(mg_sur *s, const float *X, const float *y, int n, int epochs, int batch, float lr,
+                       unsigned long long seed) {
   if (!s || !X || !y || n < 1 || epochs < 0 || batch < 1 || !(lr > 0.0f))
     return mg_fail(MG_ERR_ARG, "mg_sur_train: bad argument");
   HIPCHK(hipSetDevice(s->device));
   int r;
+  if ((r = settle(s))) return r;
   if (batch > n) batch = n;
   /* the seeded permutations of a group of epochs go up in one copy; the steps then run without a host wait */
   int group = SUR_IDX_CAP / n;
   if (group < 1) group = 1;
   if (group > epochs) group = epochs > 0 ? epochs : 1;
-  if ((r = ensure_data(s, n, (size_t)group * n, batch)) || (r = upload_xy(s, X, y, n))) return r;
+  if ((r = ensure_data(s, n, (size_t)group * n, batch))) return r;
+  /* the samples go through pinned memory owned by the surrogate, so the caller's arrays are free on return */
+  if ((r = ensure_pinned((void **)&s->h_X, &s->cap_hx, sizeof(float) * (size_t)n * (s->nin + 1))) ||
+      (r = ensure_pinned((void **)&s->h_perm, &s->cap_hperm, sizeof(int) * (size_t)group * n)))
+    return r;
+  s->h_Y = s->h_X + (size_t)n * s->nin;
+  memcpy(s->h_X, X, sizeof(float) * (size_t)n * s->nin);
+  memcpy(s->h_Y, y, sizeof(float) * n);
+  if ((r = upload_xy(s, s->h_X, s->h_Y, n))) return r;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  int *perm = (int *)malloc(sizeof(int) * (size_t)group * n);
+  int *perm = s->h_perm;
   unsigned long long st = seed;
   for (int ep0 = 0; ep0 < epochs; ep0 += group) {
     const int ng = epochs - ep0 < group ? epochs - ep0 : group;
-    /* the previous group's kernels read d_idx and perm's last copy may still be in flight: wait for both */
-    if (hipStreamSynchronize(s->st) != hipSuccess) {
-      free(perm);
-      return mg_fail(MG_ERR_HIP, "mg_sur_train: stream");
-    }
+    /* the previous group's kernels read d_idx and its index copy reads perm: wait for both */
+    if (ep0 > 0) HIPCHK(hipStreamSynchronize(s->st));
     for (int g = 0; g < ng; g++) {
       int *pg = perm + (size_t)g * n;
       for (int i = 0; i < n; i++) pg[i] = i;
@@ -238,10 +293,7 @@ int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, i
         int t = pg[i]; pg[i] = pg[j]; pg[j] = t;
       }
     }
-    if (hipMemcpyAsync(s->d_idx, perm, sizeof(int) * (size_t)ng * n, hipMemcpyHostToDevice, s->st) != hipSuccess) {
-      free(perm);
-      return mg_fail(MG_ERR_HIP, "mg_sur_train: index upload");
-    }
+    HIPCHK(hipMemcpyAsync(s->d_idx, perm, sizeof(int) * (size_t)ng * n, hipMemcpyHostToDevice, s->st));
     for (int g = 0; g < ng; g++)
       for (int b0 = 0; b0 < n; b0 += batch) {
         const int B = n - b0 < batch ? n - b0 : batch;
@@ -250,15 +302,29 @@ int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, i
         const float c1 = 1.0f - powf(b1, (float)s->step), c2 = 1.0f - powf(b2, (float)s->step);
         if (mgk_sur_grad(B, nblk, s->nin, s->d_idx + (size_t)g * n + b0, s->d_X, s->d_Y, s->d_p, s->d_norm,
                          s->d_part, s->d_ploss, s->st) ||
-            mgk_sur_adam(s->P, nblk, s->d_part, s->d_p, s->d_m, s->d_v, NULL, lr, b1, b2, eps, c1, c2, s->st)) {
-          free(perm);
+            mgk_sur_adam(s->P, nblk, s->d_part, s->d_p, s->d_m, s->d_v, NULL, lr, b1, b2, eps, c1, c2, s->st))
           return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");
-        }
       }
   }
-  r = full_loss(s, n, loss, NULL);  /* ends with a stream synchronisation: perm's copies are complete */
-  free(perm);
-  return r;
+  if ((r = loss_enqueue(s, n, NULL))) return r;
+  s->pending = 1;
+  return MG_OK;
+}
+
+int mg_sur_train_end(mg_sur *s, float *loss) {
+  if (!s) return mg_fail(MG_ERR_ARG, "mg_sur_train_end: bad argument");
+  HIPCHK(hipSetDevice(s->device));
+  int r;
+  if ((r = settle(s))) return r;
+  if (loss) *loss = s->last_loss;
+  return MG_OK;
+}
+
+int mg_sur_train(mg_sur *s, const float *X, const float *y, int n, int epochs, int batch, float lr,
+                 unsigned long long seed, float *loss) {
+  int r;
+  if ((r = mg_sur_train_begin(s, X, y, n, epochs, batch, lr, seed))) return r;
+  return mg_sur_train_end(s, loss);
 }
 
 int mg_sur_predict(mg_sur *s, const float *X, int n, float *y) {
@@ -266,6 +332,8 @@ int mg_sur_predict(mg_sur *s, const float *X, int n, float *y) {
   if (n == 0) return MG_OK;
   HIPCHK(hipSetDevice(s->device));
   int r;
+  /* a training run in flight is waited for first: the prediction uses its parameters and its sample buffer */
+  if ((r = settle(s))) return r;
   if ((r = ensure_data(s, n, 0, 1)) || (r = upload_xy(s, X, NULL, n))) return r;
   if (mgk_sur_predict(n, s->nin, s->d_X, s->d_p, s->d_norm, s->d_out, s->st))
     return mg_fail(MG_ERR_HIP, "surrogate kernel launch failed");

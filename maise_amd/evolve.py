@@ -79,6 +79,7 @@ class GAConfig:
     screen: bool = True                        # screen offspring with the surrogate (False: relax every offspring)
     offspring: int = 0                         # offspring per generation (0: pop)
     relax_cap: int = 0                         # FIRE steps per generation before a structure is carried over (0: none)
+    optimizer: str = "fire"                    # relaxation method: "fire" (mg_relax) or "lbfgs" (mg_relax_lbfgs)
 
 
 @dataclass
@@ -96,7 +97,7 @@ class GenerationReport:
     seconds_offspring: Optional[float] = None  # crossover and mutation (host)
     seconds_predict: Optional[float] = None    # unrelaxed evaluation + features + prediction + selection
     seconds_relax: Optional[float] = None      # relaxation of the kept offspring
-    seconds_train: Optional[float] = None      # retraining
+    seconds_train: Optional[float] = None      # retraining: host time to start it plus any wait for it after the update
     seconds_select: Optional[float] = None     # fingerprints and the population update
     elapsed: float = 0.0                       # search clock at the end of the generation (audit excluded)
     carried: int = 0                           # unconverged structures carried into the next generation
@@ -247,42 +248,22 @@ def unique_by_fingerprint(E: np.ndarray, fp: np.ndarray, V: np.ndarray, cut: flo
     already kept exceeds `cut` (RANK_TR's duplicate test, src/emod.c:351-366, which divides such a structure's
     fitness by ~1000; here it leaves the population).  Infinite energies are dropped.  The first `n_unique`
     structures are known to be no duplicates of each other (the previous population), so only pairs with at least one
-    later structure are compared: their fingerprint products come from one matrix product, and only pairs inside
-    the energy and volume windows where COMP_CL can exceed 0.95 are scored (exact for cut >= 0.95)."""
-    order = np.argsort(E, kind="stable")
-    test = cut < 1.0 and len(E) > n_unique
-    if test:
-        G = fp[n_unique:] @ fp.T          # [new, all] fingerprint products (CxC)
+    later structure are compared: their fingerprint products come from one matrix product, and the walk itself
+    (native mg_evo_unique) scores only pairs inside the energy and volume windows where COMP_CL can exceed 0.95
+    (exact for cut >= 0.95)."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    n = len(E)
+    test = cut < 1.0 and n > n_unique
+    G = np.ascontiguousarray(fp[n_unique:] @ fp.T, dtype=np.float32) if test else np.zeros(1, np.float32)
     de_win = _DUP_DE if cut >= 0.95 else np.inf
     dv_win = _DUP_DV if cut >= 0.95 else np.inf
-    keep: List[int] = []
-    kidx = np.zeros(len(E), dtype=np.int64)
-    kE = np.zeros(len(E))
-    kV = np.zeros(len(E))
-    for k in order:
-        if not np.isfinite(E[k]):
-            continue
-        n = len(keep)
-        if test and n:
-            cand = kidx[:n]
-            m = (E[k] - kE[:n] <= de_win) & (np.abs(0.5 * (V[k] - kV[:n]) / (V[k] + kV[:n])) <= dv_win)
-            if k < n_unique:
-                m &= cand >= n_unique       # an old member is only compared with new structures
-            near = cand[m]
-            if len(near):
-                cxc = G[k - n_unique, near] if k >= n_unique else G[near - n_unique, k]
-                de = E[near] - E[k]
-                dv = 0.5 * (V[k] - V[near]) / (V[k] + V[near])
-                w = 0.8 * cxc + 0.1 * np.exp(-0.5 * de * de / 0.05 ** 2) + 0.1 * np.exp(-0.5 * dv * dv / 0.01 ** 2)
-                if w.max() > cut:
-                    continue
-        kidx[n] = k
-        kE[n] = E[k]
-        kV[n] = V[k]
-        keep.append(int(k))
-        if len(keep) == limit:
-            break
-    return keep
+    keep = np.zeros(max(n, 1), dtype=np.int32)
+    nk = lib().mg_evo_unique(n, _dp(E), _dp(V), _fp(G), int(n_unique), float(cut), float(de_win), float(dv_win),
+                             int(limit), _ip(keep))
+    if nk < 0:
+        _check(nk, "mg_evo_unique")
+    return [int(k) for k in keep[:nk]]
 
 
 # ------------------------------------------------------------------ the search
@@ -312,11 +293,17 @@ class SurrogateGA:
         return c.features()
 
     def relax(self, structs: List[Structure], max_steps: Optional[int] = None):
-        """Relax in one batch; returns (structures at their final positions, energy per atom, converged flags,
-        FIRE steps taken)."""
+        """Relax in one batch with cfg.optimizer; returns (structures at their final positions, energy per atom,
+        converged flags, steps taken)."""
         c = self._ctx_relax
         c.set_batch(structs)
-        steps, done = c.relax(max_steps=self.cfg.max_steps if max_steps is None else max_steps, fmax=self.cfg.fmax)
+        ms = self.cfg.max_steps if max_steps is None else max_steps
+        if self.cfg.optimizer == "lbfgs":
+            steps, done = c.relax_lbfgs(max_steps=ms, fmax=self.cfg.fmax)
+        elif self.cfg.optimizer == "fire":
+            steps, done = c.relax(max_steps=ms, fmax=self.cfg.fmax)
+        else:
+            raise ValueError(f"unknown optimizer {self.cfg.optimizer!r}")
         E = c.batch_result(forces=False)["E"]
         out, epa = [], np.zeros(len(structs))
         for k, (spc, lat, _) in enumerate(structs):
@@ -326,10 +313,19 @@ class SurrogateGA:
         return out, epa, done, steps
 
     def train(self) -> float:
+        self.train_begin()
+        return self.train_end()
+
+    def train_end(self) -> float:
+        """Wait for the training run started by train_begin; returns its final loss."""
+        return self.sur.train_end()
+
+    def train_begin(self) -> None:
         """A fixed budget of train_steps Adam steps, warm-started from the last generation's network, over the
         newest generation's pairs plus a uniform sample of the earlier ones (all of them while they fit in the
         budget's train_steps * batch samples, so that the cost per generation stays flat as the data grows); the
-        normalisation is fitted once, on the first (random, fully relaxed) generation."""
+        normalisation is fitted once, on the first (random, fully relaxed) generation.  The run is left in flight
+        on the device (the host goes on with the population update meanwhile)."""
         cap = self.cfg.train_steps * self.cfg.batch
         X_new, y_new = self.X[-1], self.y[-1]
         if len(self.X) > 1:
@@ -346,10 +342,9 @@ class SurrogateGA:
         if not self.trained:
             self.sur.fit_norm(X, y)
         epochs = max(1, int(round(cap / len(y))))
-        loss = self.sur.train(X, y, epochs=epochs, batch=self.cfg.batch, lr=self.cfg.lr,
-                              seed=int(self.rng.integers(1 << 62)))
+        self.sur.train_begin(X, y, epochs=epochs, batch=self.cfg.batch, lr=self.cfg.lr,
+                             seed=int(self.rng.integers(1 << 62)))
         self.trained = True
-        return loss
 
     def select(self, pred: np.ndarray) -> np.ndarray:
         n = len(pred)
@@ -444,16 +439,21 @@ class SurrogateGA:
             fin, finE, finF, carry, dropped = self._relax_with_carry(batch, bfeat, prior)
             t2 = time.perf_counter()
             loss = None
+            training = False
             if screen and fin:
                 self.X.append(np.stack(finF))
                 self.y.append(finE.astype(np.float32))
-                loss = self.train()
+                self.train_begin()             # runs on the device during the population update
+                training = True
             t3 = time.perf_counter()
             pop, popE, popFP = self.update_population(pop, popE, popFP, fin, finE)
             t4 = time.perf_counter()
-            clock += t4 - t0
-            rep = GenerationReport(g, len(kids), len(sel), t4 - t0, train_loss=loss, seconds_predict=t1 - t0,
-                                   seconds_relax=t2 - t1, seconds_train=t3 - t2, elapsed=clock,
+            if training:
+                loss = self.train_end()
+            t5 = time.perf_counter()
+            clock += t5 - t0
+            rep = GenerationReport(g, len(kids), len(sel), t5 - t0, train_loss=loss, seconds_predict=t1 - t0,
+                                   seconds_relax=t2 - t1, seconds_train=(t3 - t2) + (t5 - t4), elapsed=clock,
                                    carried=len(carry), dropped=dropped)
             rep.seconds_select = t4 - t3
             rep.seconds_offspring = t_off - t0

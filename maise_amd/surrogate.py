@@ -73,6 +73,18 @@ class Surrogate:
                                   C.byref(loss)), "mg_sur_train")
         return loss.value
 
+    def train_begin(self, X, y, epochs: int = 20, batch: int = 64, lr: float = 1e-3, seed: int = 1) -> None:
+        """The same training run left in flight on the device; train_end() waits for it and returns the loss."""
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, self.nin)
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        _check(lib().mg_sur_train_begin(self._h, _fp(X), _fp(y), len(y), int(epochs), int(batch), float(lr),
+                                        int(seed)), "mg_sur_train_begin")
+
+    def train_end(self) -> float:
+        loss = C.c_float()
+        _check(lib().mg_sur_train_end(self._h, C.byref(loss)), "mg_sur_train_end")
+        return loss.value
+
     def grad(self, X, y):
         """(loss, gradient) of the mean-squared error over all of (X, y); parameters unchanged."""
         X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, self.nin)

@@ -177,3 +177,46 @@ def test_duplicate_walk_shortcut_agrees_with_the_full_walk():
     fast = ev.unique_by_fingerprint(E, fp, V, 0.95, 50, n_unique=40)
     assert full == fast
     assert not (set(full) & {60, 61, 62})
+
+
+def _walk_reference(E, fp, V, cut, limit, n_unique=0):
+    """The duplicate walk written out pair by pair (every kept structure scored with COMP_CL, no windows)."""
+    order = np.argsort(E, kind="stable")
+    keep = []
+    for k in order:
+        if not np.isfinite(E[k]):
+            continue
+        dup = False
+        for j in keep:
+            if k < n_unique and j < n_unique:
+                continue
+            w = ev.comp_cl(fp[k], fp[j][None], E[k], np.array([E[j]]), V[k], np.array([V[j]]))[0]
+            if w > cut:
+                dup = True
+                break
+        if not dup:
+            keep.append(int(k))
+        if len(keep) == limit:
+            break
+    return keep
+
+
+@pytest.mark.parametrize("n_unique", [0, 30])
+def test_native_duplicate_walk_equals_pairwise_comp_cl(n_unique):
+    """mg_evo_unique (energy/volume windows, native walk) against COMP_CL evaluated for every kept pair, on
+    structures with planted duplicates, tied energies and an infinite energy."""
+    cfg = ev.GAConfig(counts=(16, 8))
+    rng = np.random.default_rng(21)
+    base = [ev.random_structure(cfg, rng) for _ in range(45)]
+    dups = [(s[0], s[1], s[2] + rng.normal(0.0, 0.002, s[2].shape)) for s in base[:8]]
+    structs = base + dups
+    E = np.concatenate([rng.normal(-1.7, 0.01, 45), np.zeros(8)])
+    E[45:] = E[:8] + rng.uniform(-0.002, 0.002, 8)
+    E[10] = E[11]            # a tie
+    E[20] = np.inf           # dropped
+    fp = ev.fingerprints(structs, 2, cfg)
+    V = ev.cell_volumes(structs)
+    got = ev.unique_by_fingerprint(E, fp, V, 0.95, 40, n_unique=n_unique)
+    want = _walk_reference(E, fp, V, 0.95, 40, n_unique=n_unique)
+    assert got == want
+    assert 20 not in got and len(set(range(45, 53)) & set(got)) + len(set(range(8)) & set(got)) <= 8
