@@ -217,7 +217,7 @@ void mg_ctx_free(mg_ctx *c) {
   void *ptrs[] = {c->d_basis, c->d_mlp, c->d_str, c->d_atom_str, c->d_spc, c->d_xin, c->d_x, c->d_bin, c->d_bin_count,
                   c->d_bin_start, c->d_bin_rec, c->d_cnt, c->d_rowcnt, c->d_row_off, c->d_misc,
                   c->d_key, c->d_vec, c->d_rev, c->d_rhash, c->d_G, c->d_dEdG, c->d_Ei, c->d_erep,
-                  c->d_vir, c->d_fself, c->d_fe, c->d_gpl, c->d_npl, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
+                  c->d_vir, c->d_fself, c->d_fe, c->d_gpl, c->d_npl, c->d_F, c->d_E, c->d_U, c->d_tmp, c->d_part, c->d_chunk_str, c->d_chunk_cnt, c->d_frag, c->d_frag_bf, c->d_frag_lo, c->d_fvec, c->d_perm, c->d_tiles,
                   c->d_v, c->d_mass, c->d_dyn, c->d_fire, c->d_trace, c->d_snap_x, c->d_snap_v, c->d_snap_fire,
                   c->d_snap_dyn, c->d_cellv, c->d_snap_cellv, c->d_snap_str, c->d_ckey, c->d_crow, c->d_xb,
                   c->d_mimg, c->d_lb_s, c->d_lb_y, c->d_lb_f0, c->d_lb_rho, c->d_lb_tmp, c->d_snap_lb_s,
@@ -502,6 +502,7 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
   if ((size_t)nchunk > c->cap_chunk) {
     if (c->no_alloc) return mg_fail(MG_ERR_ARG, "fixed scratch: %d reduction chunks do not fit", nchunk);
     if ((r = mg_dev_alloc((void **)&c->d_chunk_str, (nchunk + 16) * sizeof(int))) ||
+        (r = mg_dev_alloc((void **)&c->d_chunk_cnt, (nchunk + 16) * sizeof(int))) ||
         (r = mg_dev_alloc((void **)&c->d_part, (nchunk + 16) * 7 * sizeof(double)))) { hfree(c, atom_str); return r; }
     c->cap_chunk = nchunk + 16;
   }
@@ -511,6 +512,8 @@ int mgh_upload(mg_ctx *c, int S, const int *atom_off, const int *nd, const int *
     for (int s = 0; s < S; s++)
       for (int k = 0; k < c->h_str[s].nchunk; k++) cs[c->h_str[s].chunk0 + k] = s;
     hipError_t e = hipMemcpy(c->d_chunk_str, cs, sizeof(int) * nchunk, hipMemcpyHostToDevice);
+    /* the assembly kernel's per-structure counters start at zero (each launch leaves them so) */
+    if (e == hipSuccess) e = hipMemset(c->d_chunk_cnt, 0, sizeof(int) * nchunk);
     hfree(c, cs);
     if (e != hipSuccess) { hfree(c, atom_str); return mg_fail(MG_ERR_HIP, "chunk table: %s", hipGetErrorString(e)); }
   }
@@ -800,13 +803,12 @@ int mgh_eval_chain(mg_ctx *c, int forces, int mode) {
     KCHK(mgk_force(N, m->nspc, fvar, c->d_spc, c->d_row_off, c->d_rowcnt, c->d_cnt, c->d_vec, c->d_basis, c->d_dEdG, c->d_fself,
                    c->d_fe, c->d_vir, urep, cap_row, c->wpb_force, pstride ? c->d_gpl : NULL, c->d_npl, pstride, c->d_rev,
                    c->d_misc + 1, ovf, st));
-    if (timing) HIPCHK(hipEventRecord(ev[EV_FORCE], st));
-    KCHK(mgk_gather(N, c->d_row_off, c->d_rowcnt, c->d_fself, c->d_fe, c->d_F, ovf, st));
-  } else if (timing) {
-    HIPCHK(hipEventRecord(ev[EV_FORCE], st));
   }
-  KCHK(mgk_reduce(c->S, c->nchunk, c->d_chunk_str, c->d_str, c->d_Ei, c->d_erep, c->d_vir, forces, c->d_part, c->d_E,
-                  c->d_U, st));
+  if (timing) HIPCHK(hipEventRecord(ev[EV_FORCE], st));
+  /* force gather and the FP64 energy/virial reductions in one launch */
+  KCHK(mgk_assemble(N, forces, c->d_row_off, c->d_rowcnt, c->d_fself, c->d_fe, c->d_F, ovf, c->S, c->nchunk,
+                    c->d_chunk_str, c->d_chunk_cnt, c->d_str, c->d_Ei, c->d_erep, c->d_vir, c->d_part, c->d_E, c->d_U,
+                    st));
   if (timing) HIPCHK(hipEventRecord(ev[EV_END], st));
   if (mode == MG_CHAIN_GRAPH) return MG_OK;
   /* misc[0..7] (row maximum, error, overflow, edge total) to pinned host memory; ASYNC leaves the check to
@@ -1127,6 +1129,7 @@ int mg_eval_static(const mg_model *m, int N, int nd, int xt, const int *spc, con
   /* reduction chunks, MLP tiles, bins (at most 4 N + 64 for one structure) and the scan workspace */
   const size_t nchunk = (n + MG_RCHUNK - 1) / MG_RCHUNK + 16;
   CARVE(c->d_chunk_str, nchunk * sizeof(int));
+  CARVE(c->d_chunk_cnt, nchunk * sizeof(int));
   CARVE(c->d_part, nchunk * 7 * sizeof(double));
   c->cap_chunk = nchunk;
   const size_t ntile = n / 32 + MG_MAXSPC + 64;

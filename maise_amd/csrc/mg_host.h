@@ -114,6 +114,7 @@ struct mg_ctx {
   int *d_perm, *d_tiles, ntiles;
   size_t cap_tiles;
   int *d_chunk_str, nchunk;
+  int *d_chunk_cnt;         /* per structure (at its first chunk): finished chunk blocks of k_assemble, zero between */
   size_t cap_chunk;
   void *d_tmp;
   size_t tmp_bytes;
@@ -193,8 +194,6 @@ int mgk_force(int N, int S, int variant, const int *spc, const int *row_off, con
               const mg_basis_dev *bp, const float *dEdG, void *fself, void *fe, float *vir, float urep, int maxrow,
               int wpb, const unsigned *gpl, const int *npl, size_t pstride, const int *rev, int *err, const int *ovf,
               hipStream_t st);
-int mgk_gather(int N, const int *row_off, const int *rowcnt, const void *fself, const void *fe, double *F,
-               const int *ovf, hipStream_t st);
 int mgk_clear(int *a, int na, int *b, int nb, int *c, int nc, hipStream_t st);
 int mgk_check_caps(int N, const int *row_off, int *misc, int cap_edges, int cap_row, hipStream_t st);
 
@@ -215,7 +214,9 @@ int mgh_eval(struct mg_ctx *c, int flags, int async);
 int mgh_settle(struct mg_ctx *c);
 /* size the graph-mode capacities from the last evaluation with headroom, allocating edge buffers */
 int mgh_graph_caps(struct mg_ctx *c, int need_edges, int need_row);
-int mgk_reduce(int nstr, int nchunk, const int *chunk_str, const mg_struct_dev *str, const float *Ei, const float *erep,
-               const float *vir, int have_vir, double *part, double *E, double *U, hipStream_t st);
+int mgk_assemble(int N, int forces, const int *row_off, const int *rowcnt, const void *fself, const void *fe, double *F,
+                 const int *ovf, int nstr, int nchunk, const int *chunk_str, int *chunk_cnt, const mg_struct_dev *str,
+                 const float *Ei, const float *erep, const float *vir, double *part, double *E, double *U,
+                 hipStream_t st);
 
 #endif

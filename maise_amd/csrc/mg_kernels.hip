@@ -21,8 +21,9 @@
 //                  registers, forces on neighbour slots in LDS without atomics, per-atom virial (no Fn/Sn
 //                  storage); each neighbour's force goes to the reverse edge's slot in the neighbour's row
 //   k_force_mm     the same with the triplet contraction over eta on the matrix cores (BF16 precisions)
-//   k_gather       F_a = own term + sum over the atom's own row (deterministic)
-//   k_reduce       per-structure FP64 energy and virial sums (deterministic tree)
+//   k_assemble     one launch: F_a = own term + sum over the atom's own row (deterministic), and the
+//                  per-structure FP64 energy and virial sums (chunk sums, then a fixed tree by a structure's last
+//                  chunk block)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -1645,37 +1646,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
 
 // Force on atom a: its own term plus what its neighbours' centres sent it, which the force kernel wrote into a's row
 // (slot e of row a holds the force from the centre at the far end of e): four lanes per atom stream the row.
-__global__ __launch_bounds__(256) void k_gather(int N, const int* __restrict__ row_off, const int* __restrict__ rowcnt,
-                                                const float4* __restrict__ fself, const float4* __restrict__ fe,
-                                                double* __restrict__ F, const int* __restrict__ ovf) {
-  if (*ovf) return;
-  const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int a = g >> 2, sub = g & 3;
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  if (a < N) {
-    const int e1 = row_off[a] + rowcnt[a];
-    for (int e = row_off[a] + sub; e < e1; e += 4) {
-      const float4 v = fe[e];
-      fx += v.x; fy += v.y; fz += v.z;
-    }
-  }
-  fx += __shfl_xor(fx, 1, 4); fy += __shfl_xor(fy, 1, 4); fz += __shfl_xor(fz, 1, 4);
-  fx += __shfl_xor(fx, 2, 4); fy += __shfl_xor(fy, 2, 4); fz += __shfl_xor(fz, 2, 4);
-  if (a < N && sub == 0) {
-    float4 s = fself[a];
-    F[3 * a] = fx + s.x; F[3 * a + 1] = fy + s.y; F[3 * a + 2] = fz + s.z;
-  }
+// One launch for the end of an evaluation: blocks [0, gb) gather the forces (four lanes per atom, XCD-aware block
+// order) and blocks [gb, gb + nchunk) sum energy and virial over one reduction chunk of MG_RCHUNK atoms each, in FP64
+// in a fixed order.  A structure of one chunk gets its sums from that block directly; for a structure of several,
+// each block writes its partial and counts itself in the structure's counter (cnt[chunk0], zero between launches),
+// and the block that completes the count sums the partials in a fixed order (lane l takes chunks l, l + 64, ...,
+// then a fixed butterfly) and clears the counter.  The partials are released and acquired at device scope around
+// the counter, so that block sees every partial whatever XCD wrote it.
+__device__ __forceinline__ double dwave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
 }
-
-// Per-structure FP64 energy and virial in two fixed-order passes: chunk partials, then a sum over the chunks.
-__global__ __launch_bounds__(256) void k_reduce_part(int nchunk, const int* __restrict__ chunk_str,
-                                                     const mg_struct_dev* __restrict__ str, const float* __restrict__ Ei,
-                                                     const float* __restrict__ erep, const float* __restrict__ vir,
-                                                     int have_vir, double* __restrict__ part) {
-  __shared__ double sh[7][256];
-  const int ch = blockIdx.x, t = threadIdx.x;
-  const mg_struct_dev* st = &str[chunk_str[ch]];
-  const int a0 = st->atom0 + (ch - st->chunk0) * MG_RCHUNK;
+__global__ __launch_bounds__(256) void k_assemble(int N, int gb, const int* __restrict__ row_off,
+                                                  const int* __restrict__ rowcnt, const float4* __restrict__ fself,
+                                                  const float4* __restrict__ fe, double* __restrict__ F,
+                                                  const int* __restrict__ ovf, const int* __restrict__ chunk_str,
+                                                  int* __restrict__ cnt, const mg_struct_dev* __restrict__ str,
+                                                  const float* __restrict__ Ei, const float* __restrict__ erep,
+                                                  const float* __restrict__ vir, int have_vir,
+                                                  double* __restrict__ part, double* __restrict__ Eout,
+                                                  double* __restrict__ Uout) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if ((int)blockIdx.x < gb) {
+    if (*ovf) return;
+    const int g = xcd_block(blockIdx.x, gb) * blockDim.x + t;
+    const int a = g >> 2, sub = g & 3;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (a < N) {
+      const int e1 = row_off[a] + rowcnt[a];
+      for (int e = row_off[a] + sub; e < e1; e += 4) {
+        const float4 v = fe[e];
+        fx += v.x; fy += v.y; fz += v.z;
+      }
+    }
+    fx += __shfl_xor(fx, 1, 4); fy += __shfl_xor(fy, 1, 4); fz += __shfl_xor(fz, 1, 4);
+    fx += __shfl_xor(fx, 2, 4); fy += __shfl_xor(fy, 2, 4); fz += __shfl_xor(fz, 2, 4);
+    if (a < N && sub == 0) {
+      float4 s = fself[a];
+      F[3 * a] = fx + s.x; F[3 * a + 1] = fy + s.y; F[3 * a + 2] = fz + s.z;
+    }
+    return;
+  }
+  __shared__ double sh[4][7];
+  __shared__ int last;
+  const int ch = blockIdx.x - gb;
+  const int s = chunk_str[ch];
+  const mg_struct_dev* st = &str[s];
+  const int a0 = st->atom0 + (ch - st->chunk0) * MG_RCHUNK, nc = st->nchunk;
   int a1 = a0 + MG_RCHUNK;
   if (a1 > st->atom0 + st->natom) a1 = st->atom0 + st->natom;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1684,31 +1702,37 @@ __global__ __launch_bounds__(256) void k_reduce_part(int nchunk, const int* __re
     if (have_vir)
       for (int q = 0; q < 6; q++) acc[1 + q] += (double)vir[6 * (size_t)a + q];
   }
-  for (int q = 0; q < 7; q++) sh[q][t] = acc[q];
+#pragma unroll
+  for (int q = 0; q < 7; q++) acc[q] = dwave_sum(acc[q]);
+  if (lane == 0)
+    for (int q = 0; q < 7; q++) sh[wv][q] = acc[q];
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w)
-      for (int q = 0; q < 7; q++) sh[q][t] += sh[q][t + w];
-    __syncthreads();
+  if (nc == 1) {  // the structure's sums are this block's
+    if (t < 7) {
+      const double v = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
+      if (t == 0) Eout[s] = v;
+      else Uout[6 * s + t - 1] = v;
+    }
+    return;
   }
-  if (t < 7) part[7 * (size_t)ch + t] = sh[t][0];
-}
-
-// one wave per structure: lane l sums chunks l, l + 64, ... of the structure, then a fixed butterfly tree
-__global__ __launch_bounds__(256) void k_reduce_final(int nstr, const mg_struct_dev* __restrict__ str,
-                                                      const double* __restrict__ part, double* __restrict__ Eout,
-                                                      double* __restrict__ Uout) {
-  const int s = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (s >= nstr) return;
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  const int c0 = str[s].chunk0, nc = str[s].nchunk;
+  if (t < 7) {
+    part[7 * (size_t)ch + t] = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
+    __threadfence();  // each writer releases its partial at device scope
+  }
+  __syncthreads();
+  if (t == 0) last = atomicAdd(cnt + st->chunk0, 1) == nc - 1;
+  __syncthreads();
+  if (!last || wv != 0) return;
+  __threadfence();  // acquire the structure's partials
+  double r[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int c = lane; c < nc; c += WAVE)
-    for (int q = 0; q < 7; q++) acc[q] += part[7 * (size_t)(c0 + c) + q];
-  for (int o = 32; o > 0; o >>= 1)
-    for (int q = 0; q < 7; q++) acc[q] += __shfl_xor(acc[q], o, WAVE);
+    for (int q = 0; q < 7; q++) r[q] += part[7 * (size_t)(st->chunk0 + c) + q];
+#pragma unroll
+  for (int q = 0; q < 7; q++) r[q] = dwave_sum(r[q]);
   if (lane == 0) {
-    Eout[s] = acc[0];
-    for (int q = 0; q < 6; q++) Uout[6 * s + q] = acc[1 + q];
+    Eout[s] = r[0];
+    for (int q = 0; q < 6; q++) Uout[6 * s + q] = r[1 + q];
+    cnt[st->chunk0] = 0;
   }
 }
 
@@ -1976,19 +2000,14 @@ int mgk_force(int N, int S, int variant, const int* spc, const int* row_off, con
   return mgk_status();
 }
 
-int mgk_gather(int N, const int* row_off, const int* rowcnt, const void* fself, const void* fe, double* F,
-               const int* ovf, hipStream_t st) {
-  if (N == 0) return 0;
-  hipLaunchKernelGGL(k_gather, dim3((4 * N + 255) / 256), dim3(256), 0, st, N, row_off, rowcnt, (const float4*)fself,
-                     (const float4*)fe, F, ovf);
-  return mgk_status();
-}
-
-int mgk_reduce(int nstr, int nchunk, const int* chunk_str, const mg_struct_dev* str, const float* Ei, const float* erep,
-               const float* vir, int have_vir, double* part, double* E, double* U, hipStream_t st) {
-  if (nstr == 0) return 0;
-  hipLaunchKernelGGL(k_reduce_part, dim3(nchunk), dim3(256), 0, st, nchunk, chunk_str, str, Ei, erep, vir, have_vir, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3((nstr + 3) / 4), dim3(256), 0, st, nstr, str, part, E, U);
+int mgk_assemble(int N, int forces, const int* row_off, const int* rowcnt, const void* fself, const void* fe, double* F,
+                 const int* ovf, int nstr, int nchunk, const int* chunk_str, int* chunk_cnt, const mg_struct_dev* str,
+                 const float* Ei, const float* erep, const float* vir, double* part, double* E, double* U,
+                 hipStream_t st) {
+  if (nstr == 0 || nchunk == 0) return 0;
+  const int gb = forces && N > 0 ? (4 * N + 255) / 256 : 0;
+  hipLaunchKernelGGL(k_assemble, dim3(gb + nchunk), dim3(256), 0, st, N, gb, row_off, rowcnt, (const float4*)fself,
+                     (const float4*)fe, F, ovf, chunk_str, chunk_cnt, str, Ei, erep, vir, forces, part, E, U);
   return mgk_status();
 }
 
