@@ -422,7 +422,7 @@ __global__ void k_fire_count(int S, const double* __restrict__ state, int* __res
 // ---------------------------------------------------------------------------------------------- L-BFGS
 // Limited-memory BFGS (Nocedal, Math. Comp. 35 (1980) 773; two-loop recursion as Nocedal & Wright, Numerical
 // Optimization, Alg. 7.4), one block of 64 or 256 threads per structure, a thread per atom, every sum in FP64 in a
-// fixed order (wave butterfly, then the waves in order), so a run does not depend on how it is launched.
+// fixed order (a fixed DPP tree per wave, then the waves in order), so a run does not depend on how it is launched.
 // state[s] = {gamma, head, count, done, steps}: gamma = s.y / y.y of the newest pair (the initial inverse Hessian
 // of the recursion; h0 before the first pair), head = the ring slot the next pair goes to, count = stored pairs.
 // History ring: hs/hy[slot * hstride + 3 i + q] (hstride = 3 N), rho[s * m + slot] = 1 / (y.s).  dr holds the
@@ -431,15 +431,39 @@ __global__ void k_fire_count(int S, const double* __restrict__ state, int* __res
 // descent direction (p.F <= 0) drops the memory and steps along h0 F.  The step is scaled so no atom moves more than
 // maxstep (A).
 #define LB_MAXM 32
-__device__ __forceinline__ double lb_wsum(double v) {
-#pragma unroll
-  for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, 64);
-  return v;
+// Wave sums and maxima through DPP moves in the VALU rather than LDS permutes: xor 1 and xor 2 inside quads, rotations
+// by 4 and 8 inside rows of 16 (every lane then holds its row's value), row 0 into row 1 and row 2 into row 3
+// (row_bcast:15), rows 0-1 into rows 2-3 (row_bcast:31); lane 63 holds the wave's value, read into a scalar register
+// and so the same in every lane.  Lanes outside the row mask keep 0 (sum) or their own value (max).
+template <int CTRL, int RM>
+__device__ __forceinline__ double lb_dpp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RM, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RM, 0xf, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
 }
-__device__ __forceinline__ double lb_wmax(double v) {
-#pragma unroll
-  for (int k = 32; k >= 1; k >>= 1) v = fmax(v, __shfl_xor(v, k, 64));
-  return v;
+__device__ __forceinline__ double lb_lane63(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double lb_wsum(double v) {
+  v += lb_dpp<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += lb_dpp<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += lb_dpp<0x124, 0xf>(v);  // row_ror:4
+  v += lb_dpp<0x128, 0xf>(v);  // row_ror:8
+  v += lb_dpp<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  v += lb_dpp<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+  return lb_lane63(v);
+}
+__device__ __forceinline__ double lb_wmax(double v) {  // v >= 0 on every lane (squared norms)
+  v = fmax(v, lb_dpp<0xB1, 0xf>(v));
+  v = fmax(v, lb_dpp<0x4E, 0xf>(v));
+  v = fmax(v, lb_dpp<0x124, 0xf>(v));
+  v = fmax(v, lb_dpp<0x128, 0xf>(v));
+  v = fmax(v, lb_dpp<0x142, 0xa>(v));
+  v = fmax(v, lb_dpp<0x143, 0xc>(v));
+  return lb_lane63(v);
 }
 // block sum / max: red[] holds one partial per wave; the leading barrier keeps the previous call's readers ahead of
 // this call's writers
@@ -589,6 +613,136 @@ __global__ __launch_bounds__(256) void k_lbfgs(int S, const mg_struct_dev* __res
   }
 }
 
+// The same step for batches whose structures have at most one atom per thread (n <= blockDim.x): the thread's
+// forces, last step, previous forces and recursion vector stay in registers, and each ring slot's step and force
+// change are loaded one slot ahead of the sum that needs them, so the recursion's chain of block sums is not also a
+// chain of memory round trips.  The same recursion and sums as k_lbfgs (the compiler may contract a product and a
+// sum into one FMA differently, so the two kernels can differ in the last bits; each is deterministic).
+__device__ __forceinline__ void lb_load3(const double* __restrict__ p, bool on, double* v) {
+  v[0] = on ? p[0] : 0.0; v[1] = on ? p[1] : 0.0; v[2] = on ? p[2] : 0.0;
+}
+__global__ __launch_bounds__(256) void k_lbfgs1(int S, const mg_struct_dev* __restrict__ str, const double* __restrict__ F,
+                                                double* __restrict__ dr, double* __restrict__ f0, double* __restrict__ hs,
+                                                double* __restrict__ hy, double* __restrict__ rho, int m, size_t hstride,
+                                                const double* __restrict__ x, double* __restrict__ xin,
+                                                double* __restrict__ state, double ftol, double maxstep, double h0) {
+  __shared__ double red[4];
+  __shared__ double a_sh[LB_MAXM], rho_sh[LB_MAXM];
+  const int s = blockIdx.x, t = threadIdx.x, nt = blockDim.x, nw = nt >> 6;
+  const int a0 = str[s].atom0, n = str[s].natom;
+  const bool on = t < n;
+  const int i = a0 + (on ? t : 0);
+  double* st = state + 5 * s;
+  if (st[3] != 0.0) {
+    if (on) { dr[3 * i] = 0.0; dr[3 * i + 1] = 0.0; dr[3 * i + 2] = 0.0; }
+    return;
+  }
+  double f[3], d0[3], fo[3];
+  lb_load3(F + 3 * i, on, f);
+  lb_load3(dr + 3 * i, on, d0);
+  lb_load3(f0 + 3 * i, on, fo);
+  const double fm = lb_bmax(f[0] * f[0] + f[1] * f[1] + f[2] * f[2], red, t, nw);
+  if (fm < ftol * ftol) {
+    if (on) {
+      for (int q = 0; q < 3; q++) { xin[3 * i + q] = x[3 * i + q]; dr[3 * i + q] = 0.0; }
+    }
+    if (t == 0) st[3] = 1.0;
+    return;
+  }
+  double gam = st[0];
+  int head = (int)st[1], cnt = (int)st[2];
+  const double steps = st[4];
+  double* rs = rho + (size_t)s * m;
+  for (int k = t; k < m; k += nt) rho_sh[k] = rs[k];
+  if (steps > 0.0) {
+    double y[3] = {fo[0] - f[0], fo[1] - f[1], fo[2] - f[2]};
+    double ys = 0.0, yy = 0.0, ss = 0.0;
+    for (int q = 0; q < 3; q++) { ys += y[q] * d0[q]; yy += y[q] * y[q]; ss += d0[q] * d0[q]; }
+    ys = lb_bsum(ys, red, t, nw);
+    yy = lb_bsum(yy, red, t, nw);
+    ss = lb_bsum(ss, red, t, nw);
+    if (yy > 0.0 && ys > 1e-10 * sqrt(yy * ss)) {
+      if (on) {
+        double* ps = hs + (size_t)head * hstride + 3 * i;
+        double* py = hy + (size_t)head * hstride + 3 * i;
+        for (int q = 0; q < 3; q++) { ps[q] = d0[q]; py[q] = y[q]; }
+      }
+      __syncthreads();
+      if (t == 0) rho_sh[head] = 1.0 / ys;
+      head = head + 1 == m ? 0 : head + 1;
+      cnt = cnt < m ? cnt + 1 : m;
+      gam = ys / yy;
+    }
+  }
+  __syncthreads();
+  // two-loop recursion on q = F, newest pair first; the next slot's values are in flight during each block sum
+  double q3[3] = {f[0], f[1], f[2]};
+  double sv[3], yv[3], sn[3], yn[3];
+  {
+    int slot = head - 1;
+    if (slot < 0) slot += m;
+    lb_load3(hs + (size_t)slot * hstride + 3 * i, on && cnt > 0, sn);
+    lb_load3(hy + (size_t)slot * hstride + 3 * i, on && cnt > 0, yn);
+  }
+  for (int k = 0; k < cnt; k++) {
+    int slot = head - 1 - k;
+    if (slot < 0) slot += m;
+    for (int q = 0; q < 3; q++) { sv[q] = sn[q]; yv[q] = yn[q]; }
+    int nslot = slot - 1;
+    if (nslot < 0) nslot += m;
+    lb_load3(hs + (size_t)nslot * hstride + 3 * i, on && k + 1 < cnt, sn);
+    lb_load3(hy + (size_t)nslot * hstride + 3 * i, on && k + 1 < cnt, yn);
+    double dd = 0.0;
+    for (int q = 0; q < 3; q++) dd += sv[q] * q3[q];
+    const double a = rho_sh[slot] * lb_bsum(dd, red, t, nw);
+    if (t == 0) a_sh[k] = a;
+    for (int q = 0; q < 3; q++) q3[q] -= a * yv[q];
+  }
+  const double g = cnt > 0 ? gam : h0;
+  for (int q = 0; q < 3; q++) q3[q] *= g;
+  __syncthreads();  // a_sh
+  {
+    int slot = head - cnt;
+    if (slot < 0) slot += m;
+    lb_load3(hs + (size_t)slot * hstride + 3 * i, on && cnt > 0, sn);
+    lb_load3(hy + (size_t)slot * hstride + 3 * i, on && cnt > 0, yn);
+  }
+  for (int k = cnt - 1; k >= 0; k--) {
+    int slot = head - 1 - k;
+    if (slot < 0) slot += m;
+    for (int q = 0; q < 3; q++) { sv[q] = sn[q]; yv[q] = yn[q]; }
+    const int nslot = slot + 1 == m ? 0 : slot + 1;
+    lb_load3(hs + (size_t)nslot * hstride + 3 * i, on && k > 0, sn);
+    lb_load3(hy + (size_t)nslot * hstride + 3 * i, on && k > 0, yn);
+    double dd = 0.0;
+    for (int q = 0; q < 3; q++) dd += yv[q] * q3[q];
+    const double c = a_sh[k] - rho_sh[slot] * lb_bsum(dd, red, t, nw);
+    for (int q = 0; q < 3; q++) q3[q] += c * sv[q];
+  }
+  double pf = 0.0;
+  for (int q = 0; q < 3; q++) pf += q3[q] * f[q];
+  pf = lb_bsum(pf, red, t, nw);
+  if (!(pf > 0.0)) {
+    cnt = 0;
+    gam = h0;
+    for (int q = 0; q < 3; q++) q3[q] = h0 * f[q];
+  }
+  const double l2 = lb_bmax(q3[0] * q3[0] + q3[1] * q3[1] + q3[2] * q3[2], red, t, nw);
+  const double sc = l2 > maxstep * maxstep ? maxstep / sqrt(l2) : 1.0;
+  if (on)
+    for (int q = 0; q < 3; q++) {
+      const double d = sc * q3[q];
+      dr[3 * i + q] = d;
+      xin[3 * i + q] = x[3 * i + q] + d;
+      f0[3 * i + q] = f[q];
+    }
+  __syncthreads();  // rho_sh
+  for (int k = t; k < m; k += nt) rs[k] = rho_sh[k];
+  if (t == 0) {
+    st[0] = gam; st[1] = (double)head; st[2] = (double)cnt; st[4] = steps + 1.0;
+  }
+}
+
 __global__ void k_lbfgs_init(int S, double* __restrict__ state, double h0) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -684,14 +838,19 @@ int mgk_lbfgs_init(int S, double* state, double h0, hipStream_t st) {
   return mgk_status();
 }
 
-// block of one wave when every structure has at most 64 atoms (no barriers in the sums), else four waves
+// block of one wave when every structure has at most 64 atoms (no barriers in the sums), else four waves; the
+// register form when no structure has more atoms than the block has threads
 int mgk_lbfgs(int S, int maxn, const mg_struct_dev* str, const double* F, double* dr, double* f0, double* hs, double* hy,
               double* rho, int m, size_t hstride, const double* x, double* xin, double* state, double fmax,
               double maxstep, double h0, hipStream_t st) {
   if (m < 1 || m > LB_MAXM) return -1;
   const int nt = maxn <= 64 ? 64 : 256;
-  hipLaunchKernelGGL(k_lbfgs, dim3(S), dim3(nt), 0, st, S, str, F, dr, f0, hs, hy, rho, m, hstride, x, xin, state,
-                     fmax, maxstep, h0);
+  if (maxn <= nt)  // one atom per thread: registers and prefetched ring slots
+    hipLaunchKernelGGL(k_lbfgs1, dim3(S), dim3(nt), 0, st, S, str, F, dr, f0, hs, hy, rho, m, hstride, x, xin, state,
+                       fmax, maxstep, h0);
+  else
+    hipLaunchKernelGGL(k_lbfgs, dim3(S), dim3(nt), 0, st, S, str, F, dr, f0, hs, hy, rho, m, hstride, x, xin, state,
+                       fmax, maxstep, h0);
   return mgk_status();
 }
 
