@@ -40,6 +40,62 @@ typedef unsigned long long u64;
 // (|a| + 0.5) 2^-23 / b, less than that while |a| < 2^21
 __device__ __forceinline__ int floordiv_f(int a, float rb) { return (int)floorf(((float)a + 0.5f) * rb); }
 
+// Wave64 scans and reductions through DPP moves in the VALU (no LDS round trips).  Inclusive scan: row shifts by 1, 2,
+// 4, 8 inside rows of 16 lanes, then row 0's last lane into row 1 and row 2's into row 3 (row_bcast:15) and row 1's
+// into rows 2 and 3 (row_bcast:31).  Sum / min / max: xor 1 and xor 2 inside quads, rotations by 4 and 8 inside rows
+// (every lane then holds its row's value), the two broadcasts, and lane 63 read into a scalar register (the same
+// value in every lane).  Lanes a move leaves without a source take 0 (scan, sum) or keep their own value (min, max).
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp0(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, RM, 0xf, false); }
+template <int CTRL, int RM>
+__device__ __forceinline__ float dppf0(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, RM, 0xf, false));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ float dppf_self(float v) {
+  const int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, CTRL, RM, 0xf, false));
+}
+__device__ __forceinline__ int wave_scan_incl(int v) {
+  v += dpp0<0x111, 0xf>(v);
+  v += dpp0<0x112, 0xf>(v);
+  v += dpp0<0x114, 0xf>(v);
+  v += dpp0<0x118, 0xf>(v);
+  v += dpp0<0x142, 0xa>(v);
+  v += dpp0<0x143, 0xc>(v);
+  return v;
+}
+__device__ __forceinline__ float lane63f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dppf0<0xB1, 0xf>(v);
+  v += dppf0<0x4E, 0xf>(v);
+  v += dppf0<0x124, 0xf>(v);
+  v += dppf0<0x128, 0xf>(v);
+  v += dppf0<0x142, 0xa>(v);
+  v += dppf0<0x143, 0xc>(v);
+  return lane63f(v);
+}
+__device__ __forceinline__ float wave_min(float v) {
+  v = fminf(v, dppf_self<0xB1, 0xf>(v));
+  v = fminf(v, dppf_self<0x4E, 0xf>(v));
+  v = fminf(v, dppf_self<0x124, 0xf>(v));
+  v = fminf(v, dppf_self<0x128, 0xf>(v));
+  v = fminf(v, dppf_self<0x142, 0xa>(v));
+  v = fminf(v, dppf_self<0x143, 0xc>(v));
+  return lane63f(v);
+}
+__device__ __forceinline__ float wave_max(float v) {
+  v = fmaxf(v, dppf_self<0xB1, 0xf>(v));
+  v = fmaxf(v, dppf_self<0x4E, 0xf>(v));
+  v = fmaxf(v, dppf_self<0x124, 0xf>(v));
+  v = fmaxf(v, dppf_self<0x128, 0xf>(v));
+  v = fmaxf(v, dppf_self<0x142, 0xa>(v));
+  v = fmaxf(v, dppf_self<0x143, 0xc>(v));
+  return lane63f(v);
+}
+
 __device__ __forceinline__ unsigned pack_img(int a, int b, int c) {
   return (unsigned)(a + MG_IMG_OFF) | ((unsigned)(b + MG_IMG_OFF) << 10) | ((unsigned)(c + MG_IMG_OFF) << 20);
 }
@@ -170,12 +226,7 @@ __global__ __launch_bounds__(1024) void k_scan_one(const int* __restrict__ in, i
       run += buf[k + (k >> 5)];
       v[q] = run;  // inclusive within the run
     }
-    int x = run;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
+    const int x = wave_scan_incl(run);
     if (lane == 63) wsum[w] = x;
     __syncthreads();
     int off = carry_s + x - run;
@@ -304,6 +355,13 @@ __global__ void k_bin_fill(int N, const int* __restrict__ bin, const int* __rest
   brec[p] = r;
 }
 
+// a lane's double in every lane (scalar read; lane must be wave-uniform)
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // ---------------------------------------------------------------------------------------------- block order, wave sync
 // The 8 XCDs of the chip take workgroups round robin (block b runs on XCD b mod 8), each with its own L2.  Kernels
 // that read the rows of an atom's neighbours remap blocks so every XCD works through one contiguous range of
@@ -414,10 +472,12 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
     }
     for (int q = 0; q < 3; q++) {
       fi[q] = xi0 * st->R[3 * q] + xi1 * st->R[3 * q + 1] + xi2 * st->R[3 * q + 2];
-      sp[q] = __shfl(spl, q, WAVE);
-      rnb[q] = __shfl(rnl, q, WAVE);
+      sp[q] = readlane_d(spl, q);
+      rnb[q] = readlane_d(rnl, q);
     }
-    rfn0 = __shfl(rfl, 0, WAVE); rfn1 = __shfl(rfl, 1, WAVE); rfn2 = __shfl(rfl, 2, WAVE);
+    rfn0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, rfl), 0));
+    rfn1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, rfl), 1));
+    rfn2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, rfl), 2));
   }
   bool orth = true;
   for (int q = 0; q < 3; q++) {
@@ -467,15 +527,10 @@ __global__ __launch_bounds__(64 * NBR_WPB) void k_nbr(int N, const double* __res
         e_sh[lane][q] = v;
       }
     }
-    int inc = count;  // inclusive scan of the counts across the wave
-#pragma unroll
-    for (int k = 1; k < WAVE; k <<= 1) {
-      int y = __shfl_up(inc, k, WAVE);
-      if (lane >= k) inc += y;
-    }
+    const int inc = wave_scan_incl(count);  // inclusive scan of the counts across the wave
     e_pre[lane + 1] = inc;
     if (lane == 0) e_pre[0] = 0;
-    const int total = __shfl(inc, WAVE - 1, WAVE);
+    const int total = __builtin_amdgcn_readlane(inc, WAVE - 1);
     wave_sync();
     for (int cb = 0; cb < total; cb += WAVE) {
       const int c = cb + lane;
@@ -762,11 +817,6 @@ __device__ __forceinline__ int rev_find(const unsigned short* __restrict__ tab, 
 }
 
 // ---------------------------------------------------------------------------------------------- wave helpers
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, WAVE);
-  return v;
-}
 
 // Reduce-scatter of 64 per-lane accumulators: afterwards lane l holds the wave sum of accumulator l.
 template <int A>
@@ -1058,7 +1108,7 @@ __device__ __forceinline__ void group_accum(float4* fj, float4* fk, int j, float
                                             float ky, float kz, int tag, bool valid) {
   unsigned long long rem = __ballot(valid);
   while (rem) {
-    const int cur = __shfl(tag, (int)__builtin_ctzll(rem), WAVE);
+    const int cur = __builtin_amdgcn_readlane(tag, (int)__builtin_ctzll(rem));  // scalar read, no LDS round trip
     const bool mine = valid && tag == cur;
     if (mine) {
       float4 v = fj[j], w = fk[k];
@@ -1344,8 +1394,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
   if (MH) {
     float mn = 3.0e38f;
     for (int t = lane; t < ri.M; t += WAVE) mn = fminf(mn, rec[R * t + 1].w);
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o, WAVE));
+    mn = wave_min(mn);
     rlb = ri.M >= 2 ? 2.0f * mn : 0.0f;
   }
   const float* gi = dEdG + (size_t)i * D;
@@ -1451,8 +1500,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
                    fast_exp2(n3[v < NE ? v : 0] * rlb);
           amax = fmaxf(amax, fabsf(a[v]));
         }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, WAVE));
+        amax = wave_max(amax);
         int ex = 0;
         if (amax > 0.f) (void)frexpf(amax, &ex);
         oscale = ldexpf(1.0f, ex - 28);

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
     float vals[32];
 #pragma unroll
     for (int c = 0; c < 32; c++) {
-      const int ac = __shfl(atom, c, WAVE);
+      const int ac = __builtin_amdgcn_readlane(atom, c);  // tile atom c, scalar
       const float keep = (ac >= 0 && k < D) ? 1.0f : 0.0f;  // a product, not a select: the loads stay unconditional
       vals[c] = (G[(size_t)(ac >= 0 ? ac : 0) * D + kc] - xk) * keep;
     }

@@ -115,6 +115,31 @@ def test_training_fits_a_smooth_function():
     s.close()
 
 
+def test_training_left_in_flight_matches_blocking_training():
+    """mg_sur_train_begin/_end (the run enqueued, the caller's arrays released at once, host work in between, a
+    prediction that waits for the run) gives the parameters and loss of mg_sur_train bit for bit."""
+    print("[gpu case] surrogate training begin/end vs blocking", flush=True)
+    rng = np.random.default_rng(15)
+    nin, n = 24, 700
+    X = rng.normal(size=(n, nin)).astype(np.float32)
+    y = (np.cos(X[:, 0]) + 0.2 * X[:, 1]).astype(np.float32)
+    a, b = _sur(nin, seed=9), _sur(nin, seed=9)
+    for s in (a, b):
+        s.fit_norm(X, y)
+    la = a.train(X, y, epochs=7, batch=32, lr=1e-3, seed=4)
+    Xc, yc = X.copy(), y.copy()
+    b.train_begin(Xc, yc, epochs=7, batch=32, lr=1e-3, seed=4)
+    Xc[:] = 0.0                                    # the caller's buffers are free once begin returns
+    yc[:] = 0.0
+    pb = b.predict(X[:10])                         # waits for the run in flight
+    lb = b.train_end()
+    assert lb == la
+    assert np.array_equal(a.params, b.params)
+    assert np.array_equal(a.predict(X[:10]), pb)
+    a.close()
+    b.close()
+
+
 def test_screened_search_short():
     from maise_amd.evolve import GAConfig, SurrogateGA
     print("[gpu case] screened GA, Mg16Ca8, pop 96, 3 generations", flush=True)
