@@ -818,24 +818,61 @@ __device__ __forceinline__ int rev_find(const unsigned short* __restrict__ tab, 
 
 // ---------------------------------------------------------------------------------------------- wave helpers
 
-// Reduce-scatter of 64 per-lane accumulators: afterwards lane l holds the wave sum of accumulator l.
-template <int A>
-__device__ __forceinline__ void rs_step(float* v, int lane, int k) {
-  const bool hi = (lane & k) != 0;
+// Reduce-scatter of 64 per-lane accumulators: afterwards lane l holds the wave sum of accumulator l.  Each stage k
+// halves the accumulators: a lane keeps the half its bit k selects and adds the partner's (lane ^ k) copy of it.  The
+// exchanges stay in the VALU: stages 32 and 16 by the lane-half swaps (v_permlane32/16_swap: a swap of the two
+// halves of a pair (v[p], v[p + A/2]) leaves v[p] of both lanes in one register and v[p + A/2] of both in the other,
+// so one add finishes a pair), stage 8 by a row rotation, stage 4 by row shifts either way under bank masks, stages 2
+// and 1 by quad permutes.  Every move runs with all lanes active (a move never reads a lane a branch has switched off).  The sums are those of the xor exchange, term for term.
+// the swap in inline assembly: this compiler's builtin for it returns the first result twice; two wait states
+// between the instruction that writes an operand and the swap
+__device__ __forceinline__ float fpair_sum32(float a, float b) {
+  unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
+  asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(ua), "+v"(ub));
+  return __builtin_bit_cast(float, ua) + __builtin_bit_cast(float, ub);
+}
+__device__ __forceinline__ float fpair_sum16(float a, float b) {
+  unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
+  asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(ua), "+v"(ub));
+  return __builtin_bit_cast(float, ua) + __builtin_bit_cast(float, ub);
+}
+template <int A, int K>
+__device__ __forceinline__ void rs_step(float* v, int lane) {
+  const bool hi = (lane & K) != 0;
 #pragma unroll
   for (int p = 0; p < A / 2; p++) {
-    float send = hi ? v[p] : v[p + A / 2];
-    float keep = hi ? v[p + A / 2] : v[p];
-    v[p] = keep + __shfl_xor(send, k, WAVE);
+    if (K == 32) {
+      v[p] = fpair_sum32(v[p], v[p + A / 2]);
+    } else if (K == 16) {
+      v[p] = fpair_sum16(v[p], v[p + A / 2]);
+    } else {
+      const float send = hi ? v[p] : v[p + A / 2];
+      const float keep = hi ? v[p + A / 2] : v[p];
+      float recv;
+      if (K == 8) {
+        recv = dppf0<0x128, 0xf>(send);  // row_ror:8 = lane ^ 8
+      } else if (K == 4) {
+        // lane ^ 4 by two moves with bank masks (banks = 4-lane groups of a row), no select: row_shl:4 (lane + 4)
+        // into banks 0 and 2, then row_shr:4 (lane - 4) into banks 1 and 3; both with every lane active
+        const int sb = __builtin_bit_cast(int, send);
+        const int up = __builtin_amdgcn_update_dpp(0, sb, 0x104, 0xf, 0x5, false);
+        recv = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(up, sb, 0x114, 0xf, 0xa, false));
+      } else if (K == 2) {
+        recv = dppf0<0x4E, 0xf>(send);   // quad_perm [2,3,0,1]
+      } else {
+        recv = dppf0<0xB1, 0xf>(send);   // quad_perm [1,0,3,2]
+      }
+      v[p] = keep + recv;
+    }
   }
 }
 __device__ __forceinline__ float reduce_scatter64(float* v, int lane) {
-  rs_step<64>(v, lane, 32);
-  rs_step<32>(v, lane, 16);
-  rs_step<16>(v, lane, 8);
-  rs_step<8>(v, lane, 4);
-  rs_step<4>(v, lane, 2);
-  rs_step<2>(v, lane, 1);
+  rs_step<64, 32>(v, lane);
+  rs_step<32, 16>(v, lane);
+  rs_step<16, 8>(v, lane);
+  rs_step<8, 4>(v, lane);
+  rs_step<4, 2>(v, lane);
+  rs_step<2, 1>(v, lane);
   return v[0];
 }
 
