@@ -1028,8 +1028,16 @@ __device__ __forceinline__ void angular(const mg_basis_dev* __restrict__ bp, flo
 #define PRING 128  // per-wave ring of surviving pairs (a power of two >= 2 * WAVE)
 // Pair words: j | k << 11 | tag << 22 (slots < 2048, tag < 1024).  tag numbers the group of the pair, a set of pairs in
 // which no slot repeats on either side, so the force kernel can accumulate one group at a time without atomics.
-#define PK_J(p) ((int)((p) & 0x7ffu))
-#define PK_K(p) ((int)(((p) >> 11) & 0x7ffu))
+// The fields are taken out with v_bfe_u32 written out: the compiler turns (p & 0x7ff) * 32 + base into a shift, a
+// mask and an add, where a field extract and one shift-add suffice.
+template <int OFF, int W>
+__device__ __forceinline__ unsigned ubfe_v(unsigned x) {
+  unsigned r;
+  asm("v_bfe_u32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "i"(OFF), "i"(W));
+  return r;
+}
+#define PK_J(p) ((int)ubfe_v<0, 11>(p))
+#define PK_K(p) ((int)ubfe_v<11, 11>(p))
 #define PK_TAG(p) ((int)((p) >> 22))
 // lane / n for 0 <= lane < 64, 0 < n <= 64: lane + 0.5 is exact and the two roundings move (lane + 0.5) / n by less
 // than 2^-17 / n, under the 0.5 / n that separates it from an integer
@@ -1038,7 +1046,7 @@ __device__ __forceinline__ int lane_div(int lane, int n) { return (int)(((float)
 // Enumerate the neighbour pairs (j, k) of one species class with r_jk < rc, compacting the survivors into a ring in
 // LDS and handing them to body(j, k, tag, valid) in batches of 64 as soon as 64 are waiting (valid = false on the
 // lanes past the end of the class's last batch), so every batch but the class's last is full.  The pairs are laid out
-// on circulant diagonals, each a conflict-free group (tag = the diagonal):
+// on circulant diagonals, each a conflict-free group (tag = the diagonal's q, q + 1 for one species):
 //   same species (m slots): diagonal q = 0..m/2-1 holds the pairs (f, f + q + 1 mod m), f = 0..m-1 (the last
 //     diagonal of an even m only f < m/2), so every unordered pair appears once;
 //   two species (long side a >= short side b): diagonal q = 0..a-1 holds (u, u + q mod a) for the short side's slots
@@ -1063,34 +1071,44 @@ __device__ __forceinline__ int for_pairs(const float4* rec, unsigned* ring, bool
     // n: slots per diagonal; nl: the wrap of the partner slot (m, or the long side); ol, os: block offsets of the
     // partner (long) side and of the lane's (short) side
     const int ntot = mode == 0 ? ((n >> 1) - 1) * n + ((n & 1) ? n : n >> 1) : nl * n;
-    const int dq = WAVE / n, df = WAVE - dq * n;  // wave-uniform step of (q, f) per 64 candidates
-    int q = lane_div(lane, n);
-    unsigned f = (unsigned)(lane - q * n);
-    const unsigned qoff = mode == 0 ? 1u : 0u;
-    for (int base = 0; base < ntot; base += WAVE) {
-      const bool tail = base + WAVE > ntot;  // wave-uniform
-      // partner slot p = f + q (+1) mod nl; lanes past the last candidate are masked before any read
-      const unsigned p0 = f + (unsigned)q + qoff;
-      const unsigned p = min(p0, p0 - (unsigned)nl);
-      const int js = mode == 0 ? os + (int)f : (mode == 1 ? ol + (int)p : os + (int)f);
-      const int ks = mode == 0 ? os + (int)p : (mode == 1 ? os + (int)f : ol + (int)p);
+    // wave-uniform step of (q, f) per 64 candidates (df pinned in a scalar register: left as 64 - dq n, the compiler
+    // spends two vector instructions per step on f + df)
+    const int dq = WAVE / n;
+    int df = WAVE - dq * n;
+    asm("" : "+s"(df));
+    // lane state: f, the short-side slot, and qq = q (+ 1 for one species), the partner's offset along the diagonal;
+    // the partner slot is p = f + qq mod nl, and qq is also the pair's tag (a diagonal number, distinct per diagonal)
+    int qq = lane_div(lane, n);
+    unsigned f = (unsigned)(lane - qq * n);
+    if (mode == 0) qq += 1;
+    const float4* recf = rec + R * os;  // the short side's records
+    const float4* recp = rec + R * ol;  // the partner side's records
+    // the pair word j | k << 11 | tag << 22 with block slots: j is the short side's slot except in mode 1
+    const unsigned pkc = mode == 1 ? (unsigned)ol | ((unsigned)os << 11) : (unsigned)os | ((unsigned)ol << 11);
+    // one step of 64 candidates; GUARD: the class's last, partial step, where lanes past the last candidate skip
+    // the reads (elsewhere f < n and f + qq < 2 nl hold on every lane, so both slots lie in the class's blocks)
+    auto step = [&](auto GUARD, int base) {
+      constexpr bool guard = decltype(GUARD)::value;
+      const unsigned t = f + (unsigned)qq;
+      const unsigned p = min(t, t - (unsigned)nl);
       bool ok = false;
-      if (!tail || lane < ntot - base) {
-        const float4 a = rec[R * js], b = rec[R * ks];
+      if (!guard || lane < ntot - base) {
+        const float4 a = recf[R * f], b = recp[R * p];
         const float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         ok = cx * cx + cy * cy + cz * cz < rc2;
       }
-      const unsigned pk = (unsigned)js | ((unsigned)ks << 11) | ((unsigned)q << 22);
+      const unsigned pk = (mode == 1 ? (p | (f << 11)) : (f | (p << 11))) + (((unsigned)qq << 22) + pkc);
       const unsigned f1 = f + (unsigned)df;
       const bool c = f1 >= (unsigned)n;
       f = c ? f1 - (unsigned)n : f1;
-      q += dq + (c ? 1 : 0);
+      qq += dq + (c ? 1 : 0);
       const unsigned long long bm = __ballot(ok);
-      const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
-      if (ok) ring[(wp + below) & (PRING - 1)] = pk;
+      // wp + the survivors on lower lanes
+      const unsigned at = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, (unsigned)wp));
+      if (ok) ring[at & (PRING - 1)] = pk;
       wp += __popcll(bm);
-      if (wp - rp >= WAVE || (tail || base + WAVE == ntot) && wp > rp) {
-        const bool last = tail || base + WAVE == ntot;
+      const bool last = guard || base + WAVE == ntot;
+      if (wp - rp >= WAVE || last && wp > rp) {
         // at most 127 survivors wait here (fewer than 64 before this step's 64), so the ring never overwrites an
         // unread entry; the second sync orders the batch's ring reads before the next step's writes
         wave_sync();
@@ -1104,7 +1122,10 @@ __device__ __forceinline__ int for_pairs(const float4* rec, unsigned* ring, bool
         }
         wave_sync();
       }
-    }
+    };
+    int base = 0;
+    for (; base + WAVE <= ntot; base += WAVE) step(std::false_type{}, base);
+    if (base < ntot) step(std::true_type{}, base);
   };
   if (tri) {
     if (ma < 2) return 0;
@@ -1284,16 +1305,19 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         float r3s = cx * cx + cy * cy + cz * cz;
         float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
         float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
-        float fc3x2 = __builtin_amdgcn_cosf(r3 * rinvh) + 1.0f;  // 2 fc(r_jk); the 2 cancels the G4 factor 2
+        // fc(r_jk), clamped to [0, 1] by the instruction (the hardware cosine may step past -1 by an ulp)
+        float fc3 = __builtin_amdgcn_fmed3f(__builtin_fmaf(__builtin_amdgcn_cosf(r3 * rinvh), 0.5f, 0.5f), 0.0f, 1.0f);
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * ia.x * ib.x;
-        float F = ia.y * ib.y * fc3x2;  // 2 fc fc fc: the factor 2 of the G4 definition
+        float F = ia.y * ib.y * fc3;  // fc fc fc; the factor 2 of the G4 definition goes on with the zeta prefactors
         float P[MG_MAXZ * MG_MAXL];
         angular<PAT>(bp, cs, P, nullptr);
         float Bq[NE];
         if (UNI) {
+          // F exp2(c rs) = exp2(c rs + log2 F): one logarithm in place of NE products (F = 0 gives -inf, then 0)
           const float rs = ia.w + ib.w + r3s;
+          const float lF = __builtin_amdgcn_logf(F);
 #pragma unroll
-          for (int e = 0; e < NE; e++) Bq[e] = F * fast_exp2(ejk[e] * rs);
+          for (int e = 0; e < NE; e++) Bq[e] = fast_exp2(__builtin_fmaf(ejk[e], rs, lF));
         } else {
           float4 Aj0 = rec[R * j + 2], Aj1 = rec[R * j + 3], Ak0 = rec[R * k + 2], Ak1 = rec[R * k + 3];
           float Aj[8] = {Aj0.x, Aj0.y, Aj0.z, Aj0.w, Aj1.x, Aj1.y, Aj1.z, Aj1.w};
@@ -1314,7 +1338,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
       }
       // lane l: grid cell l = (e*MAXZ + z)*MAXL + lambda; the zeta prefactor 2^(1 - zeta) goes on here
       const float raw = reduce_scatter64(V, lane);
-      float mine = raw * bp->pref[(lane / MG_MAXL) % MG_MAXZ];
+      float mine = raw * (2.0f * bp->pref[(lane / MG_MAXL) % MG_MAXZ]);
       int cell = lane < bp->ng4 ? bp->g4cell[lane] : 0;
       float v = __shfl(mine, cell, WAVE);
       if (lane < bp->ng4) Gi[blk + lane] = v;
