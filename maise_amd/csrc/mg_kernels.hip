@@ -935,8 +935,10 @@ __device__ __forceinline__ void row_info(RowInfo& ri, int i, const int* __restri
 
 // Stage the row of atom i into LDS.  Record per neighbour: (dx, dy, dz, r), (1/r, fc, dfc, r^2) and, unless
 // every species pair shares the G4 etas (UNI), A[0..7] = exp(-eta4(i,j)_e r^2).  With shared etas a triplet's
-// radial factor is one exp(-eta_e (r_ij^2 + r_ik^2 + r_jk^2)) and the A tables are not needed.
-template <int NE, bool UNI>
+// radial factor is one exp(-eta_e (r_ij^2 + r_ik^2 + r_jk^2)) and the A tables are not needed.  DESC: the
+// descriptor kernel's form (dx, dy, dz, r^2), (dx/r, dy/r, dz/r, fc): a triplet's cosine is one dot product of unit
+// vectors and both records are single 16-byte reads (that kernel reads r itself from the edge vectors).
+template <int NE, bool UNI, bool DESC = false>
 __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const float4* __restrict__ vec,
                                           const mg_basis_dev* __restrict__ bp, int lane, int S) {
   constexpr int R = UNI ? 2 : 4;
@@ -949,10 +951,16 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
     float rc = bp->rc[p];
     float tt = v.w / rc;
     float fc = v.w < rc ? 0.5f * (cospi_(tt) + 1.0f) : 0.0f;
-    float dfc = v.w < rc ? -0.5f * 3.14159265358979f / rc * sinpi_(tt) : 0.0f;
     float r2 = v.w * v.w;
-    rec[R * t + 0] = v;
-    rec[R * t + 1] = make_float4(__builtin_amdgcn_rcpf(v.w), fc, dfc, r2);
+    const float ir = __builtin_amdgcn_rcpf(v.w);
+    if (DESC) {
+      rec[R * t + 0] = make_float4(v.x, v.y, v.z, r2);
+      rec[R * t + 1] = make_float4(v.x * ir, v.y * ir, v.z * ir, fc);
+    } else {
+      float dfc = v.w < rc ? -0.5f * 3.14159265358979f / rc * sinpi_(tt) : 0.0f;
+      rec[R * t + 0] = v;
+      rec[R * t + 1] = make_float4(ir, fc, dfc, r2);
+    }
     if (!UNI) {
       float A[8];
 #pragma unroll
@@ -1113,10 +1121,13 @@ __device__ __forceinline__ int for_pairs(const float4* rec, unsigned* ring, bool
         // unread entry; the second sync orders the batch's ring reads before the next step's writes
         wave_sync();
         while (wp - rp >= WAVE || (last && wp > rp)) {
-          const int tt = rp + lane;
-          const bool valid = tt < wp;
-          const unsigned pw = valid ? ring[tt & (PRING - 1)] : 0u;
-          if (gout && valid) gout[tt] = pw;
+          const unsigned tt = (unsigned)(rp + lane);
+          const bool valid = tt < (unsigned)wp;
+          // read on every lane: past wp the ring holds older pairs of this row (or stale words), whose slots the
+          // body masks; only the valid lanes write
+          const unsigned pw = ring[tt & (PRING - 1)];
+          // a 32-bit byte offset from the row's base (rows hold far fewer than 2^30 pairs): a scalar-based store
+          if (gout && valid) *(unsigned*)((char*)gout + (tt << 2)) = pw;
           body(PK_J(pw), PK_K(pw), PK_TAG(pw), valid);
           rp += WAVE;
         }
@@ -1202,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
   row_info(ri, i, spc, row_off, rowcnt, cnt, S);
   float4* rec = smem_d + (size_t)wv * (R * cap + PRING / 4);
   unsigned* plist = (unsigned*)(rec + R * cap);
-  stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
+  stage_row<NE, UNI, true>(rec, ri, vec, bp, lane, S);
   // reverse edges (for the force kernel): the edges of this row that own their pair look the partner up in the
   // partner row's hash table and write both rev entries; the probes' load latency hides under the other waves'
   // descriptor arithmetic
@@ -1225,6 +1236,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
   }
   wave_sync();
   float* Gi = G + (size_t)i * D;
+  const float* vr = (const float*)(vec + ri.off) + 3;  // vr[4 t] = r of neighbour t (the G2 sums read it here)
   // The shipped G4 tables leave the grid cells of the etas past NE free (8 cells for the 43-function table, 16 for
   // the 22-function one): there the G2 sums of neighbour species s ride along in the reduce-scatter of class (s, s)
   // instead of one wave reduction per G2 function.
@@ -1245,9 +1257,8 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
     for (int c = 0; c < MG_MAXG2; c++) acc[c] = 0.0f;
     const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[s];
     for (int t = ri.o(s) + lane; t < ri.o(s) + ri.m(s); t += WAVE) {
-      float4 v = rec[R * t];
-      float fc = rec[R * t + 1].y;
-      float r = v.w;
+      const float r = vr[4 * t];
+      const float fc = rec[R * t + 1].w;
 #pragma unroll
       for (int c = 0; c < MG_MAXG2; c++)
         if (c < bp->ng2) {
@@ -1287,8 +1298,8 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         const int p = ri.si * MG_MAXSPC + sa;
         const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[sa];
         for (int t = ri.o(sa) + lane; t < ri.o(sa) + ri.m(sa); t += WAVE) {
-          const float r = rec[R * t].w;
-          const float fc = rec[R * t + 1].y;
+          const float r = vr[4 * t];
+          const float fc = rec[R * t + 1].w;
 #pragma unroll
           for (int c = 0; c < MG_GRID - G2OFF; c++)
             if (c < MG_MAXG2 && c < bp->ng2) {
@@ -1300,21 +1311,21 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
       }
       auto body = [&](int j, int k, int, bool valid) {
         if (!valid) return;
-        float4 a = rec[R * j], b = rec[R * k];
+        float4 a = rec[R * j], b = rec[R * k];  // (d, r^2)
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
-        float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
+        float4 ua = rec[R * j + 1], ub = rec[R * k + 1];  // (d / r, fc)
         float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
         // fc(r_jk), clamped to [0, 1] by the instruction (the hardware cosine may step past -1 by an ulp)
         float fc3 = __builtin_amdgcn_fmed3f(__builtin_fmaf(__builtin_amdgcn_cosf(r3 * rinvh), 0.5f, 0.5f), 0.0f, 1.0f);
-        float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * ia.x * ib.x;
-        float F = ia.y * ib.y * fc3;  // fc fc fc; the factor 2 of the G4 definition goes on with the zeta prefactors
+        float cs = ua.x * ub.x + ua.y * ub.y + ua.z * ub.z;
+        float F = ua.w * ub.w * fc3;  // fc fc fc; the factor 2 of the G4 definition goes on with the zeta prefactors
         float P[MG_MAXZ * MG_MAXL];
         angular<PAT>(bp, cs, P, nullptr);
         float Bq[NE];
         if (UNI) {
           // F exp2(c rs) = exp2(c rs + log2 F): one logarithm in place of NE products (F = 0 gives -inf, then 0)
-          const float rs = ia.w + ib.w + r3s;
+          const float rs = a.w + b.w + r3s;
           const float lF = __builtin_amdgcn_logf(F);
 #pragma unroll
           for (int e = 0; e < NE; e++) Bq[e] = fast_exp2(__builtin_fmaf(ejk[e], rs, lF));
