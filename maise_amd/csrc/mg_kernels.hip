@@ -935,13 +935,16 @@ __device__ __forceinline__ void row_info(RowInfo& ri, int i, const int* __restri
 
 // Stage the row of atom i into LDS.  Record per neighbour: (dx, dy, dz, r), (1/r, fc, dfc, r^2) and, unless
 // every species pair shares the G4 etas (UNI), A[0..7] = exp(-eta4(i,j)_e r^2).  With shared etas a triplet's
-// radial factor is one exp(-eta_e (r_ij^2 + r_ik^2 + r_jk^2)) and the A tables are not needed.  DESC: the
-// descriptor kernel's form (dx, dy, dz, r^2), (dx/r, dy/r, dz/r, fc): a triplet's cosine is one dot product of unit
-// vectors and both records are single 16-byte reads (that kernel reads r itself from the edge vectors).
+// radial factor is one exp(-eta_e (r_ij^2 + r_ik^2 + r_jk^2)) and the A tables are not needed.  The records of a
+// neighbour lie side by side (R float4 per slot).  DESC: the descriptor kernel's form, in planes of cap float4
+// (record q of slot t at rec[q cap + t], so consecutive slots are consecutive 16-byte words and a wave's reads of
+// consecutive slots meet no bank conflict): (dx, dy, dz, r^2), (dx/r, dy/r, dz/r, fc), then the A tables; a
+// triplet's cosine is one dot product of unit vectors (that kernel reads r itself from the edge vectors).
 template <int NE, bool UNI, bool DESC = false>
 __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const float4* __restrict__ vec,
-                                          const mg_basis_dev* __restrict__ bp, int lane, int S) {
+                                          const mg_basis_dev* __restrict__ bp, int lane, int S, int cap = 0) {
   constexpr int R = UNI ? 2 : 4;
+  const int ss = DESC ? 1 : R, pl = DESC ? cap : 1;  // slot and record strides
   for (int t = lane; t < ri.M; t += WAVE) {
     float4 v = vec[ri.off + t];
     int s = 0;
@@ -954,8 +957,8 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
     float r2 = v.w * v.w;
     const float ir = __builtin_amdgcn_rcpf(v.w);
     if (DESC) {
-      rec[R * t + 0] = make_float4(v.x, v.y, v.z, r2);
-      rec[R * t + 1] = make_float4(v.x * ir, v.y * ir, v.z * ir, fc);
+      rec[t] = make_float4(v.x, v.y, v.z, r2);
+      rec[pl + t] = make_float4(v.x * ir, v.y * ir, v.z * ir, fc);
     } else {
       float dfc = v.w < rc ? -0.5f * 3.14159265358979f / rc * sinpi_(tt) : 0.0f;
       rec[R * t + 0] = v;
@@ -965,8 +968,8 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
       float A[8];
 #pragma unroll
       for (int e = 0; e < 8; e++) A[e] = e < NE ? fast_exp2(-(float)MG_LOG2E * bp->eta4[p][e] * r2) : 0.0f;
-      rec[R * t + 2] = make_float4(A[0], A[1], A[2], A[3]);
-      rec[R * t + 3] = make_float4(A[4], A[5], A[6], A[7]);
+      rec[ss * t + 2 * pl] = make_float4(A[0], A[1], A[2], A[3]);
+      rec[ss * t + 3 * pl] = make_float4(A[4], A[5], A[6], A[7]);
     }
   }
 }
@@ -1064,7 +1067,8 @@ __device__ __forceinline__ int lane_div(int lane, int n) { return (int)(((float)
 // and the wraps are unsigned minima (x mod n = min(x, x - n) for x < 2n).  Only the class's last step checks the
 // candidate range.  With gout, the survivors are also written there in order (the pair list of the class, which the
 // force kernel reads instead of enumerating again); returns their number.  The batches are the consecutive runs of 64
-// of that list, the same batches the stored-list reader forms.
+// of that list, the same batches the stored-list reader forms.  R: float4 from one slot's position record to the
+// next (records side by side, or 1 for the descriptor kernel's planes).
 template <int MODE>
 struct PairMode : std::integral_constant<int, MODE> {};
 
@@ -1213,7 +1217,10 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
   row_info(ri, i, spc, row_off, rowcnt, cnt, S);
   float4* rec = smem_d + (size_t)wv * (R * cap + PRING / 4);
   unsigned* plist = (unsigned*)(rec + R * cap);
-  stage_row<NE, UNI, true>(rec, ri, vec, bp, lane, S);
+  stage_row<NE, UNI, true>(rec, ri, vec, bp, lane, S, cap);
+  const float4* rec1 = rec + cap;  // (d / r, fc) plane
+  const float4* rec2 = rec + 2 * cap;  // A tables (without UNI)
+  const float4* rec3 = rec + 3 * cap;
   // reverse edges (for the force kernel): the edges of this row that own their pair look the partner up in the
   // partner row's hash table and write both rev entries; the probes' load latency hides under the other waves'
   // descriptor arithmetic
@@ -1258,7 +1265,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
     const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[s];
     for (int t = ri.o(s) + lane; t < ri.o(s) + ri.m(s); t += WAVE) {
       const float r = vr[4 * t];
-      const float fc = rec[R * t + 1].w;
+      const float fc = rec1[t].w;
 #pragma unroll
       for (int c = 0; c < MG_MAXG2; c++)
         if (c < bp->ng2) {
@@ -1299,7 +1306,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
         const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[sa];
         for (int t = ri.o(sa) + lane; t < ri.o(sa) + ri.m(sa); t += WAVE) {
           const float r = vr[4 * t];
-          const float fc = rec[R * t + 1].w;
+          const float fc = rec1[t].w;
 #pragma unroll
           for (int c = 0; c < MG_GRID - G2OFF; c++)
             if (c < MG_MAXG2 && c < bp->ng2) {
@@ -1311,10 +1318,10 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
       }
       auto body = [&](int j, int k, int, bool valid) {
         if (!valid) return;
-        float4 a = rec[R * j], b = rec[R * k];  // (d, r^2)
+        float4 a = rec[j], b = rec[k];  // (d, r^2)
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
-        float4 ua = rec[R * j + 1], ub = rec[R * k + 1];  // (d / r, fc)
+        float4 ua = rec1[j], ub = rec1[k];  // (d / r, fc)
         float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
         // fc(r_jk), clamped to [0, 1] by the instruction (the hardware cosine may step past -1 by an ulp)
         float fc3 = __builtin_amdgcn_fmed3f(__builtin_fmaf(__builtin_amdgcn_cosf(r3 * rinvh), 0.5f, 0.5f), 0.0f, 1.0f);
@@ -1330,7 +1337,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
 #pragma unroll
           for (int e = 0; e < NE; e++) Bq[e] = fast_exp2(__builtin_fmaf(ejk[e], rs, lF));
         } else {
-          float4 Aj0 = rec[R * j + 2], Aj1 = rec[R * j + 3], Ak0 = rec[R * k + 2], Ak1 = rec[R * k + 3];
+          float4 Aj0 = rec2[j], Aj1 = rec3[j], Ak0 = rec2[k], Ak1 = rec3[k];
           float Aj[8] = {Aj0.x, Aj0.y, Aj0.z, Aj0.w, Aj1.x, Aj1.y, Aj1.z, Aj1.w};
           float Ak[8] = {Ak0.x, Ak0.y, Ak0.z, Ak0.w, Ak1.x, Ak1.y, Ak1.z, Ak1.w};
 #pragma unroll
@@ -1342,7 +1349,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
           for (int q = 0; q < MG_MAXZ * MG_MAXL; q++)
             if (Pat<PAT>::mask(e) & (1u << q)) V[e * MG_MAXZ * MG_MAXL + q] += Bq[e] * P[q];
       };
-      const int np = for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, gout, err, body);
+      const int np = for_pairs<1>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, gout, err, body);
       if (gout) {
         gout += np;
         if (lane == 0) npl[(size_t)i * MG_NCLS + cls] = np;
