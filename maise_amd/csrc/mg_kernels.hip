@@ -935,16 +935,15 @@ __device__ __forceinline__ void row_info(RowInfo& ri, int i, const int* __restri
 
 // Stage the row of atom i into LDS.  Record per neighbour: (dx, dy, dz, r), (1/r, fc, dfc, r^2) and, unless
 // every species pair shares the G4 etas (UNI), A[0..7] = exp(-eta4(i,j)_e r^2).  With shared etas a triplet's
-// radial factor is one exp(-eta_e (r_ij^2 + r_ik^2 + r_jk^2)) and the A tables are not needed.  The records of a
-// neighbour lie side by side (R float4 per slot).  DESC: the descriptor kernel's form, in planes of cap float4
-// (record q of slot t at rec[q cap + t], so consecutive slots are consecutive 16-byte words and a wave's reads of
-// consecutive slots meet no bank conflict): (dx, dy, dz, r^2), (dx/r, dy/r, dz/r, fc), then the A tables; a
-// triplet's cosine is one dot product of unit vectors (that kernel reads r itself from the edge vectors).
+// radial factor is one exp(-eta_e (r_ij^2 + r_ik^2 + r_jk^2)) and the A tables are not needed.  The records lie
+// in planes of cap float4 (record q of slot t at rec[q cap + t]), so consecutive slots are consecutive 16-byte words
+// and a wave's reads of consecutive slots (the pairs of one diagonal) meet no bank conflict.  DESC: the descriptor
+// kernel's form (dx, dy, dz, r^2), (dx/r, dy/r, dz/r, fc): a triplet's cosine is one dot product of unit vectors
+// (that kernel reads r itself from the edge vectors).
 template <int NE, bool UNI, bool DESC = false>
 __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const float4* __restrict__ vec,
-                                          const mg_basis_dev* __restrict__ bp, int lane, int S, int cap = 0) {
-  constexpr int R = UNI ? 2 : 4;
-  const int ss = DESC ? 1 : R, pl = DESC ? cap : 1;  // slot and record strides
+                                          const mg_basis_dev* __restrict__ bp, int lane, int S, int cap) {
+  const int pl = cap;  // plane stride
   for (int t = lane; t < ri.M; t += WAVE) {
     float4 v = vec[ri.off + t];
     int s = 0;
@@ -961,15 +960,15 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
       rec[pl + t] = make_float4(v.x * ir, v.y * ir, v.z * ir, fc);
     } else {
       float dfc = v.w < rc ? -0.5f * 3.14159265358979f / rc * sinpi_(tt) : 0.0f;
-      rec[R * t + 0] = v;
-      rec[R * t + 1] = make_float4(ir, fc, dfc, r2);
+      rec[t] = v;
+      rec[pl + t] = make_float4(ir, fc, dfc, r2);
     }
     if (!UNI) {
       float A[8];
 #pragma unroll
       for (int e = 0; e < 8; e++) A[e] = e < NE ? fast_exp2(-(float)MG_LOG2E * bp->eta4[p][e] * r2) : 0.0f;
-      rec[ss * t + 2 * pl] = make_float4(A[0], A[1], A[2], A[3]);
-      rec[ss * t + 3 * pl] = make_float4(A[4], A[5], A[6], A[7]);
+      rec[2 * pl + t] = make_float4(A[0], A[1], A[2], A[3]);
+      rec[3 * pl + t] = make_float4(A[4], A[5], A[6], A[7]);
     }
   }
 }
@@ -1068,7 +1067,7 @@ __device__ __forceinline__ int lane_div(int lane, int n) { return (int)(((float)
 // candidate range.  With gout, the survivors are also written there in order (the pair list of the class, which the
 // force kernel reads instead of enumerating again); returns their number.  The batches are the consecutive runs of 64
 // of that list, the same batches the stored-list reader forms.  R: float4 from one slot's position record to the
-// next (records side by side, or 1 for the descriptor kernel's planes).
+// next (1 for the records' planes).
 template <int MODE>
 struct PairMode : std::integral_constant<int, MODE> {};
 
@@ -1462,7 +1461,10 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
   float4* fnk4 = fnb4 + cap;                     // [cap] triplet k side
   float* fnb = (float*)fnb4;
   unsigned* plist = (unsigned*)(fnk4 + cap);     // [PRING]
-  stage_row<NE, UNI>(rec, ri, vec, bp, lane, S);
+  stage_row<NE, UNI>(rec, ri, vec, bp, lane, S, cap);
+  const float4* rec1 = rec + cap;  // (1/r, fc, dfc, r^2) plane
+  const float4* rec2 = rec + 2 * cap;  // A tables (without UNI)
+  const float4* rec3 = rec + 3 * cap;
   for (int t = lane; t < ri.M; t += WAVE) {
     fnb4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
     fnk4[t] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1472,7 +1474,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
   float rlb = 0.0f;
   if (MH) {
     float mn = 3.0e38f;
-    for (int t = lane; t < ri.M; t += WAVE) mn = fminf(mn, rec[R * t + 1].w);
+    for (int t = lane; t < ri.M; t += WAVE) mn = fminf(mn, rec1[t].w);
     mn = wave_min(mn);
     rlb = ri.M >= 2 ? 2.0f * mn : 0.0f;
   }
@@ -1487,8 +1489,8 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
     const int blk = bp->ind2[(s - ri.si + S) % S];
     const float Rrep = bp->urep_rm[ri.si] + bp->urep_rm[s];
     for (int t = ri.o(s) + lane; t < ri.o(s) + ri.m(s); t += WAVE) {
-      float4 v = rec[R * t];
-      float4 w = rec[R * t + 1];
+      float4 v = rec[t];
+      float4 w = rec1[t];
       float r = v.w, fc = w.y, dfc = w.z;
       float dT = 0.0f;
 #pragma unroll
@@ -1613,10 +1615,10 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
       }
       auto body = [&](int j, int k, int tag, bool valid) {
         if (!valid) { j = 0; k = 1; }  // a harmless slot pair; its gradient is discarded below
-        float4 a = rec[R * j], b = rec[R * k];
+        float4 a = rec[j], b = rec[k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
-        float4 ia = rec[R * j + 1], ib = rec[R * k + 1];
+        float4 ia = rec1[j], ib = rec1[k];
         float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
         float i3 = __builtin_amdgcn_rcpf(r3);
         const float th = r3 * rinvh;  // cos(pi r / rc) = cos(2 pi th) in the hardware's revolutions
@@ -1629,7 +1631,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         float X = 0.f, Y1 = 0.f, Y2 = 0.f, Y3 = 0.f, Zc = 0.f;
         float Aj[8], Ak[8];
         if (!UNI) {
-          float4 Aj0 = rec[R * j + 2], Aj1 = rec[R * j + 3], Ak0 = rec[R * k + 2], Ak1 = rec[R * k + 3];
+          float4 Aj0 = rec2[j], Aj1 = rec3[j], Ak0 = rec2[k], Ak1 = rec3[k];
           Aj[0] = Aj0.x; Aj[1] = Aj0.y; Aj[2] = Aj0.z; Aj[3] = Aj0.w; Aj[4] = Aj1.x; Aj[5] = Aj1.y; Aj[6] = Aj1.z; Aj[7] = Aj1.w;
           Ak[0] = Ak0.x; Ak[1] = Ak0.y; Ak[2] = Ak0.z; Ak[3] = Ak0.w; Ak[4] = Ak1.x; Ak[5] = Ak1.y; Ak[6] = Ak1.z; Ak[7] = Ak1.w;
         }
@@ -1710,7 +1712,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         for_stored_pairs(pl, np, lane, body);
         pl += np;
       } else {
-        for_pairs<R>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, nullptr, err, body);
+        for_pairs<1>(rec, plist, sa == sb, ri.o(sa), ri.m(sa), ri.o(sb), ri.m(sb), rc * rc, lane, nullptr, err, body);
       }
       wave_sync();
     }
@@ -1722,7 +1724,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
   for (int t = lane; t < ri.M; t += WAVE) {
     const float4 g1 = fnb4[t], g2 = fnk4[t];
     const float gx = g1.x + g2.x, gy = g1.y + g2.y, gz = g1.z + g2.z;
-    const float4 v = rec[R * t];
+    const float4 v = rec[t];
     fx -= gx; fy -= gy; fz -= gz;
     u0 += gx * v.x; u1 += gy * v.y; u2 += gz * v.z;
     u3 += 0.5f * (gx * v.y + gy * v.x); u4 += 0.5f * (gy * v.z + gz * v.y); u5 += 0.5f * (gz * v.x + gx * v.z);
