@@ -213,6 +213,209 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   }
 }
 
+// ---- two waves per tile.  The single-wave kernel above leaves one wave per SIMD on most of the chip (a tile is 32
+// atoms), so each wave's load latencies and its chain of matrix-core and tanh work run alone.  Here the two waves
+// of a workgroup share a tile: wave w owns hidden block w of a 64-wide layer (HB == 2), half of the input layer's
+// K steps when that layer is one block wide (HB1 == 1; the two partial sums are added in the same order by both
+// waves), and every second output block of dE/dG.  Activations, deltas and partial sums cross between the waves
+// through LDS (regions of [2 waves][16 registers][64 lanes] floats, one per exchange, so no region is rewritten
+// while the other wave may still read it).  Same fragments, same arithmetic per block as the single-wave kernel.
+template <int PIN, int PHID, int HB1, int HB2>
+__global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tiles, const int* __restrict__ perm,
+                                                   const float* __restrict__ G, const float* __restrict__ frag,
+                                                   const __bf16* __restrict__ frag_bf, const __bf16* __restrict__ frag_lo,
+                                                   const float* __restrict__ vecs, MlpFrag f, float* __restrict__ Ei,
+                                                   float* __restrict__ dEdG) {
+  constexpr bool BIN = PIN != 0, BHID = PHID != 0;
+  constexpr int KS = BIN ? 8 : 1;
+  constexpr int KSH = BHID ? 8 : 1;
+  constexpr int S2 = BHID ? 2 : 16;  // hidden-layer steps per 32-unit block
+  constexpr int SG = BIN ? 2 : 16;   // dE/dG steps per 32-unit block of d1
+  const int4 tl = tiles[blockIdx.x];
+  const int sp = tl.x, start = tl.y, count = tl.z;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5;
+  const int atom = col < count ? perm[start + col] : -1;
+  extern __shared__ float gsm[];
+  const int D = f.D;
+  const float* fa = frag + (size_t)sp * f.sf * 64;
+  const __bf16* fb = frag_bf + (size_t)sp * f.sb * 64 * 8;
+  const __bf16* fl = frag_lo + (size_t)sp * f.sb * 64 * 8;
+  const float* vv = vecs + (size_t)sp * f.stride_vec;
+  const float* b0 = vv;
+  const float* b1 = b0 + HB1 * 32;
+  const float* w2 = b1 + HB2 * 32;
+  const float* xo = w2 + HB2 * 32 + 1;
+  const float b2 = w2[HB2 * 32];
+  const int k1 = f.k1;
+  const int Kmax = BIN ? 16 * k1 : 2 * k1;
+  const int RS = Kmax + 1;
+  float* xa = gsm + 32 * RS;      // h1 (HB1 == 2) or the layer-1 partial sums (HB1 == 1)
+  float* xb2 = xa + 2 * 16 * 64;  // delta2 (HB2 == 2) and the energy partials
+  float* xc = xb2 + 2 * 16 * 64 + 2 * 64;  // d1 (HB1 == 2)
+  float* eb = xb2 + 2 * 16 * 64;
+
+  // the tile's inputs x - xoff in LDS, wave w staging tile atoms 16 w .. 16 w + 15
+  for (int k0 = 0; k0 < Kmax; k0 += WAVE) {
+    const int k = k0 + lane, kc = k < D ? k : D - 1;
+    const float xk = xo[kc];
+    float vals[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      const int ac = __builtin_amdgcn_readlane(atom, 16 * w + c);
+      const float keep = (ac >= 0 && k < D) ? 1.0f : 0.0f;
+      vals[c] = (G[(size_t)(ac >= 0 ? ac : 0) * D + kc] - xk) * keep;
+    }
+    if (k < Kmax)
+#pragma unroll
+      for (int c = 0; c < 16; c++) gsm[(16 * w + c) * RS + k] = vals[c];
+  }
+  __syncthreads();
+  const float* gr = gsm + col * RS;
+
+  // ---- layer 1: a1 = tanh(b0 + W0' x), HB1 blocks, all held by both waves afterwards
+  f32x16 a1[HB1];
+  {
+    f32x16 acc;
+    const int rb = HB1 == 2 ? w : 0;
+    const bool bias = HB1 == 2 || w == 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = bias ? b0[rb * 32 + acc_row(r, h)] : 0.0f;
+    const int half = (k1 + 1) >> 1;
+    const int sa = HB1 == 2 ? 0 : w * half;
+    const int se = HB1 == 2 ? k1 : (w == 0 ? half : k1);
+    for (int st0 = sa; st0 < se; st0 += 4) {
+      float xs[4][KS];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int su = st0 + u < se ? st0 + u : se - 1;
+#pragma unroll
+        for (int v = 0; v < KS; v++) xs[u][v] = st0 + u < se ? gr[BIN ? 16 * su + 8 * h + v : 2 * su + h] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int st = st0 + u < se ? st0 + u : se - 1;
+        acc = mfma_step<PIN>(fa, fb, fl, f.o_l1 * 64 + (rb * k1 + st) * 64 + lane, xs[u], acc);
+      }
+    }
+    if (HB1 == 2) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) xa[(w * 16 + r) * 64 + lane] = tanh_f(acc[r]);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < HB1; q++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) a1[q][r] = xa[(q * 16 + r) * 64 + lane];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; r++) xa[(w * 16 + r) * 64 + lane] = acc[r];
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 16; r++) a1[0][r] = tanh_f(xa[r * 64 + lane] + xa[(16 + r) * 64 + lane]);
+    }
+  }
+
+  // ---- layer 2: a2 = tanh(b1 + W1 h1); energy; delta2 = W2 (1 - h2^2): wave w's block when HB2 == 2, else both
+  // waves compute the single block
+  f32x16 a2[HB2];
+  {
+    f32x16 acc;
+    const int rb = HB2 == 2 ? w : 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = b1[rb * 32 + acc_row(r, h)];
+#pragma unroll
+    for (int rb1 = 0; rb1 < HB1; rb1++)
+#pragma unroll
+      for (int q = 0; q < S2; q++) {
+        float xb[KSH];
+#pragma unroll
+        for (int v = 0; v < KSH; v++) xb[v] = a1[rb1][BHID ? 8 * q + v : q];
+        const int st = rb1 * S2 + q;
+        acc = mfma_step<PHID>(fa, fb, fl, f.o_l2 * 64 + (rb * HB1 * S2 + st) * 64 + lane, xb, acc);
+      }
+    float e = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const float wr = w2[rb * 32 + acc_row(r, h)];
+      const float t = tanh_f(acc[r]);
+      e += wr * t;
+      acc[r] = wr * (1.0f - t * t);  // delta2
+    }
+    e += __shfl_xor(e, 32, WAVE);
+    if (HB2 == 2) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) xb2[(w * 16 + r) * 64 + lane] = acc[r];
+      eb[w * 64 + lane] = e;
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < HB2; q++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) a2[q][r] = xb2[(q * 16 + r) * 64 + lane];
+      e = eb[lane] + eb[64 + lane];
+    } else {
+      a2[0] = acc;
+    }
+    if (w == 0 && atom >= 0 && h == 0) Ei[atom] = e + b2;
+  }
+
+  // ---- delta1 = (W1^T delta2) (1 - h1^2): wave w's block when HB1 == 2, else both waves
+  f32x16 d1[HB1];
+  {
+    f32x16 acc;
+    const int rb = HB1 == 2 ? w : 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+#pragma unroll
+    for (int rb2 = 0; rb2 < HB2; rb2++)
+#pragma unroll
+      for (int q = 0; q < S2; q++) {
+        float xb[KSH];
+#pragma unroll
+        for (int v = 0; v < KSH; v++) xb[v] = a2[rb2][BHID ? 8 * q + v : q];
+        const int st = rb2 * S2 + q;
+        acc = mfma_step<PHID>(fa, fb, fl, f.o_b1 * 64 + (rb * HB2 * S2 + st) * 64 + lane, xb, acc);
+      }
+    if (HB1 == 2) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const float hv = w == 0 ? a1[0][r] : a1[HB1 - 1][r];
+        xc[(w * 16 + r) * 64 + lane] = acc[r] * (1.0f - hv * hv);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < HB1; q++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) d1[q][r] = xc[(q * 16 + r) * 64 + lane];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; r++) d1[0][r] = acc[r] * (1.0f - a1[0][r] * a1[0][r]);
+    }
+  }
+
+  // ---- dE/dG = W0'^T delta1, 32 features per block, blocks alternating between the waves
+  float* out = atom >= 0 ? dEdG + (size_t)atom * D : nullptr;
+  for (int rbd = w; rbd < f.db; rbd += 2) {
+    f32x16 gacc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) gacc[r] = 0.0f;
+#pragma unroll
+    for (int rb1 = 0; rb1 < HB1; rb1++)
+#pragma unroll
+      for (int q = 0; q < SG; q++) {
+        float xb[KS];
+#pragma unroll
+        for (int v = 0; v < KS; v++) xb[v] = d1[rb1][BIN ? 8 * q + v : q];
+        gacc = mfma_step<PIN>(fa, fb, fl, f.o_g0 * 64 + (rbd * HB1 * SG + rb1 * SG + q) * 64 + lane, xb, gacc);
+      }
+    if (out)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int k = rbd * 32 + acc_row(r, h);
+        if (k < D) out[k] = gacc[r];
+      }
+  }
+}
+
 // prec: 0 FP32 throughout; 1 (MG_PREC_BF16) one-pass BF16 in the hidden layer (W1 h1 and W1^T delta2), FP32 input
 // layer; 2 (MG_PREC_BF16X3) compensated BF16 in every layer
 extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, const float* G, const float* frag,
@@ -234,7 +437,18 @@ extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, cons
   const __bf16* fl = (const __bf16*)frag_lo;
   const size_t sm = (size_t)32 * ((bin ? 16 : 2) * f.k1 + 1) * sizeof(float);  // staged inputs of a tile
   if (sm > 64 * 1024) return -2;
-#define MLP_L(PI, PH, H1B, H2B) hipLaunchKernelGGL((k_mlp_mfma<PI, PH, H1B, H2B>), dim3(ntiles), dim3(64), sm, st, t, perm, G, frag, fb, fl, vecs, f, Ei, dEdG)
+  // two waves per tile where the exchange regions fit beside the staged inputs (every shipped width does)
+  const size_t sm2 = sm + (size_t)(3 * 2 * 16 * 64 + 2 * 64) * sizeof(float);
+  const bool two = sm2 <= 64 * 1024;
+#define MLP_L(PI, PH, H1B, H2B)                                                                                      \
+  do {                                                                                                               \
+    if (two)                                                                                                         \
+      hipLaunchKernelGGL((k_mlp_mfma2<PI, PH, H1B, H2B>), dim3(ntiles), dim3(128), sm2, st, t, perm, G, frag, fb, fl, \
+                         vecs, f, Ei, dEdG);                                                                         \
+    else                                                                                                             \
+      hipLaunchKernelGGL((k_mlp_mfma<PI, PH, H1B, H2B>), dim3(ntiles), dim3(64), sm, st, t, perm, G, frag, fb, fl,   \
+                         vecs, f, Ei, dEdG);                                                                         \
+  } while (0)
 #define MLP_P(PI, PH)                                     \
   do {                                                    \
     if (f.hb1 == 1 && f.hb2 == 1) MLP_L(PI, PH, 1, 1);    \
