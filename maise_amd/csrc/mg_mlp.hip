@@ -213,6 +213,44 @@ __global__ __launch_bounds__(64) void k_mlp_mfma(const int4* __restrict__ tiles,
   }
 }
 
+// The A operand of one MFMA step (one FP32 value, or eight BF16 values and their low halves), loaded ahead of the
+// step that uses it, and the step itself (mfma_step with the load taken out).
+template <int PREC> struct AFrag { float a; };
+template <> struct AFrag<1> { bf16x8 a; };
+template <> struct AFrag<2> { bf16x8 a, lo; };
+
+template <int PREC>
+__device__ __forceinline__ AFrag<PREC> afrag_load(const float* __restrict__ afrag, const __bf16* __restrict__ afrag_bf,
+                                                  const __bf16* __restrict__ afrag_lo, int idx) {
+  AFrag<PREC> r;
+  if constexpr (PREC == 0) {
+    r.a = afrag[idx];
+  } else {
+    r.a = *(const bf16x8*)(afrag_bf + 8 * (size_t)idx);
+    if constexpr (PREC == 2) r.lo = *(const bf16x8*)(afrag_lo + 8 * (size_t)idx);
+  }
+  return r;
+}
+
+template <int PREC>
+__device__ __forceinline__ f32x16 mfma_frag(const AFrag<PREC>& A, const float* bvals, f32x16 acc) {
+  if constexpr (PREC == 0) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(A.a, bvals[0], acc, 0, 0, 0);
+  } else {
+    bf16x8 b;
+#pragma unroll
+    for (int v = 0; v < 8; v++) b[v] = (__bf16)bvals[v];
+    if constexpr (PREC == 2) {
+      bf16x8 bl;
+#pragma unroll
+      for (int v = 0; v < 8; v++) bl[v] = (__bf16)(bvals[v] - (float)b[v]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.lo, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.a, bl, acc, 0, 0, 0);
+    }
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.a, b, acc, 0, 0, 0);
+  }
+}
+
 // ---- two waves per tile.  The single-wave kernel above leaves one wave per SIMD on most of the chip (a tile is 32
 // atoms), so each wave's load latencies and its chain of matrix-core and tanh work run alone.  Here the two waves
 // of a workgroup share a tile: wave w owns hidden block w of a 64-wide layer (HB == 2), half of the input layer's
@@ -273,31 +311,44 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
   __syncthreads();
   const float* gr = gsm + col * RS;
 
-  // ---- layer 1: a1 = tanh(b0 + W0' x), HB1 blocks, all held by both waves afterwards
+  // ---- layer 1: a1 = tanh(b0 + W0' x), HB1 blocks, all held by both waves afterwards.  The A operands (weight
+  // fragments) are loaded a group of U steps ahead of the steps that use them, and each later layer's fragments
+  // are loaded before the exchange that precedes it, so their latency passes under the barrier.
+  constexpr int U = BIN ? 4 : 8;
+  constexpr int N2 = HB1 * S2, N3 = HB2 * S2, N4 = HB1 * SG;  // steps of layer 2, delta1 and a dE/dG block
+  const int rb1w = HB1 == 2 ? w : 0, rb2w = HB2 == 2 ? w : 0;  // the blocks this wave computes
   f32x16 a1[HB1];
+  AFrag<PHID> f2[N2];
   {
     f32x16 acc;
-    const int rb = HB1 == 2 ? w : 0;
     const bool bias = HB1 == 2 || w == 0;
 #pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = bias ? b0[rb * 32 + acc_row(r, h)] : 0.0f;
+    for (int r = 0; r < 16; r++) acc[r] = bias ? b0[rb1w * 32 + acc_row(r, h)] : 0.0f;
     const int half = (k1 + 1) >> 1;
     const int sa = HB1 == 2 ? 0 : w * half;
     const int se = HB1 == 2 ? k1 : (w == 0 ? half : k1);
-    for (int st0 = sa; st0 < se; st0 += 4) {
-      float xs[4][KS];
+    const int i0 = f.o_l1 * 64 + rb1w * k1 * 64 + lane;
+    AFrag<PIN> fc[U];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < U; u++) fc[u] = afrag_load<PIN>(fa, fb, fl, i0 + max(min(sa + u, se - 1), 0) * 64);
+    for (int st0 = sa; st0 < se; st0 += U) {
+      AFrag<PIN> fn[U];  // the next group (past the end: the last step again, unused)
+#pragma unroll
+      for (int u = 0; u < U; u++) fn[u] = afrag_load<PIN>(fa, fb, fl, i0 + min(st0 + U + u, se - 1) * 64);
+      float xs[U][KS];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
         const int su = st0 + u < se ? st0 + u : se - 1;
 #pragma unroll
         for (int v = 0; v < KS; v++) xs[u][v] = st0 + u < se ? gr[BIN ? 16 * su + 8 * h + v : 2 * su + h] : 0.0f;
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int st = st0 + u < se ? st0 + u : se - 1;
-        acc = mfma_step<PIN>(fa, fb, fl, f.o_l1 * 64 + (rb * k1 + st) * 64 + lane, xs[u], acc);
-      }
+      for (int u = 0; u < U; u++) acc = mfma_frag<PIN>(fc[u], xs[u], acc);
+#pragma unroll
+      for (int u = 0; u < U; u++) fc[u] = fn[u];
     }
+#pragma unroll
+    for (int s = 0; s < N2; s++) f2[s] = afrag_load<PHID>(fa, fb, fl, f.o_l2 * 64 + (rb2w * N2 + s) * 64 + lane);
     if (HB1 == 2) {
 #pragma unroll
       for (int r = 0; r < 16; r++) xa[(w * 16 + r) * 64 + lane] = tanh_f(acc[r]);
@@ -318,11 +369,11 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
   // ---- layer 2: a2 = tanh(b1 + W1 h1); energy; delta2 = W2 (1 - h2^2): wave w's block when HB2 == 2, else both
   // waves compute the single block
   f32x16 a2[HB2];
+  AFrag<PHID> f3[N3];
   {
     f32x16 acc;
-    const int rb = HB2 == 2 ? w : 0;
 #pragma unroll
-    for (int r = 0; r < 16; r++) acc[r] = b1[rb * 32 + acc_row(r, h)];
+    for (int r = 0; r < 16; r++) acc[r] = b1[rb2w * 32 + acc_row(r, h)];
 #pragma unroll
     for (int rb1 = 0; rb1 < HB1; rb1++)
 #pragma unroll
@@ -330,13 +381,14 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
         float xb[KSH];
 #pragma unroll
         for (int v = 0; v < KSH; v++) xb[v] = a1[rb1][BHID ? 8 * q + v : q];
-        const int st = rb1 * S2 + q;
-        acc = mfma_step<PHID>(fa, fb, fl, f.o_l2 * 64 + (rb * HB1 * S2 + st) * 64 + lane, xb, acc);
+        acc = mfma_frag<PHID>(f2[rb1 * S2 + q], xb, acc);
       }
+#pragma unroll
+    for (int s = 0; s < N3; s++) f3[s] = afrag_load<PHID>(fa, fb, fl, f.o_b1 * 64 + (rb1w * N3 + s) * 64 + lane);
     float e = 0.0f;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-      const float wr = w2[rb * 32 + acc_row(r, h)];
+      const float wr = w2[rb2w * 32 + acc_row(r, h)];
       const float t = tanh_f(acc[r]);
       e += wr * t;
       acc[r] = wr * (1.0f - t * t);  // delta2
@@ -360,9 +412,9 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
 
   // ---- delta1 = (W1^T delta2) (1 - h1^2): wave w's block when HB1 == 2, else both waves
   f32x16 d1[HB1];
+  AFrag<PIN> f4[N4];
   {
     f32x16 acc;
-    const int rb = HB1 == 2 ? w : 0;
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = 0.0f;
 #pragma unroll
@@ -372,9 +424,12 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
         float xb[KSH];
 #pragma unroll
         for (int v = 0; v < KSH; v++) xb[v] = a2[rb2][BHID ? 8 * q + v : q];
-        const int st = rb2 * S2 + q;
-        acc = mfma_step<PHID>(fa, fb, fl, f.o_b1 * 64 + (rb * HB2 * S2 + st) * 64 + lane, xb, acc);
+        acc = mfma_frag<PHID>(f3[rb2 * S2 + q], xb, acc);
       }
+    // the first dE/dG block's fragments (block w; a wave without a block loads block 0, unused)
+    const int rbd0 = w < f.db ? w : 0;
+#pragma unroll
+    for (int s = 0; s < N4; s++) f4[s] = afrag_load<PIN>(fa, fb, fl, f.o_g0 * 64 + (rbd0 * N4 + s) * 64 + lane);
     if (HB1 == 2) {
 #pragma unroll
       for (int r = 0; r < 16; r++) {
@@ -392,9 +447,14 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
     }
   }
 
-  // ---- dE/dG = W0'^T delta1, 32 features per block, blocks alternating between the waves
+  // ---- dE/dG = W0'^T delta1, 32 features per block, blocks alternating between the waves (the next block's
+  // fragments loaded before this block's steps)
   float* out = atom >= 0 ? dEdG + (size_t)atom * D : nullptr;
   for (int rbd = w; rbd < f.db; rbd += 2) {
+    AFrag<PIN> fn[N4];
+    const int rn = rbd + 2 < f.db ? rbd + 2 : rbd;
+#pragma unroll
+    for (int s = 0; s < N4; s++) fn[s] = afrag_load<PIN>(fa, fb, fl, f.o_g0 * 64 + (rn * N4 + s) * 64 + lane);
     f32x16 gacc;
 #pragma unroll
     for (int r = 0; r < 16; r++) gacc[r] = 0.0f;
@@ -405,7 +465,7 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
         float xb[KS];
 #pragma unroll
         for (int v = 0; v < KS; v++) xb[v] = d1[rb1][BIN ? 8 * q + v : q];
-        gacc = mfma_step<PIN>(fa, fb, fl, f.o_g0 * 64 + (rbd * HB1 * SG + rb1 * SG + q) * 64 + lane, xb, gacc);
+        gacc = mfma_frag<PIN>(f4[rb1 * SG + q], xb, gacc);
       }
     if (out)
 #pragma unroll
@@ -413,6 +473,8 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
         const int k = rbd * 32 + acc_row(r, h);
         if (k < D) out[k] = gacc[r];
       }
+#pragma unroll
+    for (int s = 0; s < N4; s++) f4[s] = fn[s];
   }
 }
 
