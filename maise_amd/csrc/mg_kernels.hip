@@ -1787,6 +1787,7 @@ __device__ __forceinline__ double dwave_sum(double v) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, WAVE);
   return v;
 }
+#define ASM_L 8  // lanes per atom in the force gather
 __global__ __launch_bounds__(256) void k_assemble(int N, int gb, const int* __restrict__ row_off,
                                                   const int* __restrict__ rowcnt, const float4* __restrict__ fself,
                                                   const float4* __restrict__ fe, double* __restrict__ F,
@@ -1799,18 +1800,21 @@ __global__ __launch_bounds__(256) void k_assemble(int N, int gb, const int* __re
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if ((int)blockIdx.x < gb) {
     if (*ovf) return;
+    // ASM_L lanes per atom: each load of the row's group covers ASM_L consecutive 16-byte edge forces (full lines)
     const int g = xcd_block(blockIdx.x, gb) * blockDim.x + t;
-    const int a = g >> 2, sub = g & 3;
+    const int a = g / ASM_L, sub = g % ASM_L;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (a < N) {
       const int e1 = row_off[a] + rowcnt[a];
-      for (int e = row_off[a] + sub; e < e1; e += 4) {
+      for (int e = row_off[a] + sub; e < e1; e += ASM_L) {
         const float4 v = fe[e];
         fx += v.x; fy += v.y; fz += v.z;
       }
     }
-    fx += __shfl_xor(fx, 1, 4); fy += __shfl_xor(fy, 1, 4); fz += __shfl_xor(fz, 1, 4);
-    fx += __shfl_xor(fx, 2, 4); fy += __shfl_xor(fy, 2, 4); fz += __shfl_xor(fz, 2, 4);
+#pragma unroll
+    for (int m = 1; m < ASM_L; m <<= 1) {
+      fx += __shfl_xor(fx, m, ASM_L); fy += __shfl_xor(fy, m, ASM_L); fz += __shfl_xor(fz, m, ASM_L);
+    }
     if (a < N && sub == 0) {
       float4 s = fself[a];
       F[3 * a] = fx + s.x; F[3 * a + 1] = fy + s.y; F[3 * a + 2] = fz + s.z;
@@ -2134,7 +2138,7 @@ int mgk_assemble(int N, int forces, const int* row_off, const int* rowcnt, const
                  const float* Ei, const float* erep, const float* vir, double* part, double* E, double* U,
                  hipStream_t st) {
   if (nstr == 0 || nchunk == 0) return 0;
-  const int gb = forces && N > 0 ? (4 * N + 255) / 256 : 0;
+  const int gb = forces && N > 0 ? (ASM_L * N + 255) / 256 : 0;
   hipLaunchKernelGGL(k_assemble, dim3(gb + nchunk), dim3(256), 0, st, N, gb, row_off, rowcnt, (const float4*)fself,
                      (const float4*)fe, F, ovf, chunk_str, chunk_cnt, str, Ei, erep, vir, forces, part, E, U);
   return mgk_status();
