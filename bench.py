@@ -464,8 +464,9 @@ def main():
     ap.add_argument("--md-skin", type=float, default=None,
                     help="run the MD leg with this skin only (default: 0 and 1.0 A, compared)")
     ap.add_argument("--fire-sweep", type=int, nargs="*", default=None,
-                    help="only time FIRE iterations on batches of this many Mg16Ca8 cells (e.g. 64 179 512 2048)")
-    ap.add_argument("--fire-iters", type=int, default=300, help="FIRE iterations per batch of --fire-sweep")
+                    help="only time FIRE and L-BFGS iterations on batches of this many Mg16Ca8 cells (e.g. 64 179 512 "
+                         "2048), and both to convergence on up to 179 cells and offspring")
+    ap.add_argument("--fire-iters", type=int, default=300, help="iterations per batch of --fire-sweep")
     ap.add_argument("--rates-only", action="store_true",
                     help="only run the library's issue-rate microbenchmark (FP32 FMA, exp2, cos, sqrt, rcp) and print it")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -504,14 +505,35 @@ def main():
         c.set_batch(cells[:16])
         c.relax(max_steps=20, fmax=0.01)
         out = {}
+        c.relax_lbfgs(max_steps=20, fmax=0.01)
+        out, out_lb = {}, {}
         for n in args.fire_sweep:
-            c.set_batch(cells[:n])
-            t0 = time.perf_counter()
-            c.relax(max_steps=args.fire_iters, fmax=1e-9)
-            out[n] = 1e6 * (time.perf_counter() - t0) / args.fire_iters
+            for opt, dst in ((c.relax, out), (c.relax_lbfgs, out_lb)):
+                c.set_batch(cells[:n])
+                t0 = time.perf_counter()
+                opt(max_steps=args.fire_iters, fmax=1e-9)
+                dst[n] = 1e6 * (time.perf_counter() - t0) / args.fire_iters
+        # to convergence (fmax 0.01 eV/A), FIRE against L-BFGS: random cells, and crossover offspring of 8 relaxed
+        # parents (the search's operators, seed 5)
+        from maise_amd.evolve import make_offspring
+        nconv = min(179, max(args.fire_sweep))
+        c.set_batch(cells[:nconv])
+        c.relax(max_steps=3000, fmax=0.01)
+        par = [(cells[k][0], cells[k][1], c.result(k, forces=False)["X"]) for k in range(min(8, nconv))]
+        kids = make_offspring(par, np.ones(len(par)), cfg, np.random.default_rng(5), nconv)
+        conv = {}
+        for name, batch in (("random", cells[:nconv]), ("offspring", kids)):
+            for opt in ("fire", "lbfgs"):
+                c.set_batch(batch)
+                t0 = time.perf_counter()
+                st, dn = (c.relax if opt == "fire" else c.relax_lbfgs)(max_steps=3000, fmax=0.01)
+                conv[f"{name}_{opt}"] = {"seconds": time.perf_counter() - t0, "steps_mean": float(np.mean(st)),
+                                         "steps_max": int(np.max(st)), "converged": int(np.sum(dn))}
         c.close()
-        print(json.dumps({"workload": f"FIRE on batches of random Mg16Ca8 cells, nn_MgCa_d3_v0, UREP 100 eV, "
-                                      f"{args.fire_iters} iterations", "us_per_iteration_by_cells": out}))
+        print(json.dumps({"workload": f"FIRE and L-BFGS (m = 20) on batches of random Mg16Ca8 cells, nn_MgCa_d3_v0, "
+                                      f"UREP 100 eV, {args.fire_iters} iterations",
+                          "us_per_iteration_by_cells": out, "lbfgs_us_per_iteration_by_cells": out_lb,
+                          "to_convergence_cells": nconv, "to_convergence": conv}))
         return
     if args.rates_only:
         r = measured_rates(maise_amd)

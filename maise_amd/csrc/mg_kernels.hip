@@ -12,15 +12,16 @@
 //   k_nbr<2>       skin candidates (MD, FIRE): the fill's search at Rc + skin, keys only, when the lists are stale
 //   k_nbr_skin     every step with skin lists: LIST's FP64 test on the candidates only, same rows as the fill
 //   k_desc<NE>     one wave per atom: G2 over neighbours, G4 over neighbour pairs of each species class,
-//                  neighbour tile staged in LDS, (eta, zeta, lambda) grid accumulators in VGPRs,
+//                  neighbour tile staged in LDS (record planes), (eta, zeta, lambda) grid accumulators in VGPRs,
 //                  reduce-scatter across the wave (src/nprs.c:504-566); with forces, the reverse-edge index of
 //                  the row's owned pairs through the partners' hash tables
 //   k_mlp_fp32     per-atom D-H1-H2-1 forward and dE/dG backward (src/nmlp.c:367-431 + chain of 444-478)
-//   k_mlp_mfma     the same on MFMA for hidden layers up to 64 wide (mg_mlp.hip)
+//   k_mlp_mfma2    the same on MFMA for hidden layers up to 64 wide, two waves per tile (mg_mlp.hip)
 //   k_force<NE>    one wave per atom: recomputes pair/triplet geometry, contracts dG/dr with dE/dG in
 //                  registers, forces on neighbour slots in LDS without atomics, per-atom virial (no Fn/Sn
 //                  storage); each neighbour's force goes to the reverse edge's slot in the neighbour's row
-//   k_force_mm     the same with the triplet contraction over eta on the matrix cores (BF16 precisions)
+//   k_force_mh/_mm the same with the triplet contraction over eta on the matrix cores (split FP16 in FP32, split
+//                  BF16 in the BF16 precisions)
 //   k_assemble     one launch: F_a = own term + sum over the atom's own row (deterministic), and the
 //                  per-structure FP64 energy and virial sums (chunk sums, then a fixed tree by a structure's last
 //                  chunk block)
