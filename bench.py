@@ -340,7 +340,8 @@ def ga_leg(maise_amd, args):
     and error; (2) the same run in compensated BF16 (BF16x3); (3) the same seeded search with and without screening
     for equal search time, at population 512 and at 2048; (4) a sample of offspring relaxed one structure per context
     gives the plain-path rate; (5) the search with the device L-BFGS in place of FIRE (screened, screened in BF16x3,
-    every offspring relaxed) and the same sample relaxed one structure per context with it."""
+    every offspring relaxed) and the same sample relaxed one structure per context with it; (6) an audited run that
+    ranks offspring after a few FIRE steps of each."""
     from maise_amd.evolve import GAConfig, SurrogateGA, compare_screening, make_offspring
     cfg = GAConfig(pop=args.ga_pop, generations=args.ga_gens, seed=5, relax_cap=300)
     t0 = time.perf_counter()
@@ -375,6 +376,23 @@ def ga_leg(maise_amd, args):
     rb = gb.run()
     gb.close()
     tb = float(np.mean([r.seconds_screened for r in rb.reports[1:]]))
+    # ranking pre-relaxed offspring: every offspring takes args.ga_prerelax FIRE steps before the surrogate ranks
+    # it (audited, half the generations)
+    pre = None
+    if args.ga_prerelax > 0:
+        gp = SurrogateGA(GA_POT, dataclasses.replace(cfg, prerelax=args.ga_prerelax,
+                                                     generations=max(1, cfg.generations // 2)))
+        rp = gp.run()
+        gp.close()
+        rr = rp.reports[1:]
+        pre = {"prerelax_steps": args.ga_prerelax, "generations": len(rr),
+               "recall_top10_mean": float(np.mean([r.recall_top10 for r in rr])),
+               "recall_top25_mean": float(np.mean([r.recall_top25 for r in rr])),
+               "mae_eV_atom_mean": float(np.mean([r.mae_eV_atom for r in rr])),
+               "seconds_per_generation": float(np.mean([r.seconds_screened for r in rr])),
+               "seconds_prerelax": float(np.mean([r.seconds_prerelax for r in rr])),
+               "seconds_relax_kept": float(np.mean([r.seconds_relax for r in rr]))}
+        progress(f"search ranking pre-relaxed offspring: recall@25% {pre['recall_top25_mean']:.2f}")
     # with the device L-BFGS in place of FIRE: screened in FP32, screened in BF16x3, and every offspring relaxed
     lb = {}
     for name, over in (("screened", {}), ("screened_bf16x3", {"precision": maise_amd.MG_PREC_BF16X3}),
@@ -432,6 +450,7 @@ def ga_leg(maise_amd, args):
             "lbfgs": {"offspring_per_s": lb, "offspring_per_s_single_structure": lb_single,
                       "screened_bf16x3_over_plain_fire_single": lb["screened_bf16x3"] / single,
                       "screened_bf16x3_over_plain_lbfgs_single": lb["screened_bf16x3"] / lb_single},
+            "screened_prerelaxed": pre,
             "equal_time_comparison": cmp_,
             "equal_time_comparison_large_population": cmp_large,
             "seconds_total": wall}
@@ -453,6 +472,8 @@ def main():
     ap.add_argument("--md-steps", type=int, default=1000, help="NVT steps of the MD leg (0 skips it)")
     ap.add_argument("--ga-gens", type=int, default=30, help="generations of the screened search (0 skips it)")
     ap.add_argument("--ga-pop", type=int, default=512, help="population of the screened search")
+    ap.add_argument("--ga-prerelax", type=int, default=40,
+                    help="FIRE steps of every offspring before ranking, in an extra audited run (0 skips it)")
     ap.add_argument("--ga-single", type=int, default=32, help="offspring relaxed one at a time for the plain rate")
     ap.add_argument("--ga-compare-seconds", type=float, default=8.0,
                     help="search clock of each run of the with/without-screening comparison (0 skips it)")

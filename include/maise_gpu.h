@@ -84,6 +84,10 @@ int mg_eval(mg_ctx *c, int flags);
 int mg_get(mg_ctx *c, int s, double *E, double *EA, double *F, double *U, double *X);
 /* Batch results: E[S], U[6S], F[3*Ntot] (NULL skips). */
 int mg_get_batch(mg_ctx *c, double *E, double *U, double *F);
+/* Batch positions: X[3*Ntot], the wrapped positions of every structure in batch order, in one copy (mg_get per
+ * structure takes a copy per field and structure).  No counterpart in the reference, whose callers hold one
+ * structure: it is for batched relaxation drivers (the evolutionary search reads its relaxed offspring with it). */
+int mg_get_batch_x(mg_ctx *c, double *X);
 
 /* One FP32 energy + forces + virial evaluation that allocates nothing: no counterpart in the reference; it is for
  * callers that cannot allocate device memory (HIP graph capture, tight serving loops, harnesses that forbid

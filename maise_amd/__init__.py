@@ -69,6 +69,7 @@ def lib():
             "mg_eval": ([vp, C.c_int], C.c_int),
             "mg_get": ([vp, C.c_int, dp, dp, dp, dp, dp], C.c_int),
             "mg_get_batch": ([vp, dp, dp, dp], C.c_int),
+            "mg_get_batch_x": ([vp, dp], C.c_int),
             "mg_calibrate_rate": ([C.c_int, C.c_int, dp], C.c_int),
             "mg_eval_static": ([vp, C.c_int, C.c_int, C.c_int, ip, dp, dp, C.c_double, dp, dp, dp, dp], C.c_int),
             "mg_get_neighbors": ([vp, ip, ip, ip, ip], C.c_int),
@@ -334,6 +335,12 @@ class Context:
             out["U"] = U.reshape(self.S, 6)
             out["F"] = F.reshape(self.N, 3)
         return out
+
+    def batch_positions(self) -> np.ndarray:
+        """Wrapped positions [N, 3] of every structure of the batch, in batch order (one device copy)."""
+        X = np.zeros(3 * self.N)
+        _check(lib().mg_get_batch_x(self._h, _dp(X)), "mg_get_batch_x")
+        return X.reshape(self.N, 3)
 
     def neighbors(self):
         """(row_off[N+1], j[M], img[M,3]) of the last evaluation; rows sorted by (species, j, image)."""

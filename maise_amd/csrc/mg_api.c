@@ -906,6 +906,15 @@ int mg_get_batch(mg_ctx *c, double *E, double *U, double *F) {
   return MG_OK;
 }
 
+int mg_get_batch_x(mg_ctx *c, double *X) {
+  if (!c || !X) return mg_fail(MG_ERR_ARG, "mg_get_batch_x: bad argument");
+  if (!c->have_g) return mg_fail(MG_ERR_ARG, "mg_get_batch_x: the batch has not been evaluated yet");
+  HIPCHK(hipSetDevice(c->device));
+  { int rs_ = mgh_settle(c); if (rs_ != MG_OK) return rs_; }
+  if (c->N > 0) HIPCHK(hipMemcpy(X, c->d_x, 3 * (size_t)c->N * sizeof(double), hipMemcpyDeviceToHost));
+  return MG_OK;
+}
+
 int mg_get_neighbors(mg_ctx *c, int *nedges, int *row_off, int *j, int *img) {
   if (!c || !nedges) return mg_fail(MG_ERR_ARG, "mg_get_neighbors: bad argument");
   HIPCHK(hipSetDevice(c->device));

@@ -80,6 +80,7 @@ class GAConfig:
     offspring: int = 0                         # offspring per generation (0: pop)
     relax_cap: int = 0                         # FIRE steps per generation before a structure is carried over (0: none)
     optimizer: str = "fire"                    # relaxation method: "fire" (mg_relax) or "lbfgs" (mg_relax_lbfgs)
+    prerelax: int = 0                          # screening: relaxation steps of every offspring before it is ranked
 
 
 @dataclass
@@ -95,6 +96,7 @@ class GenerationReport:
     best_eV_atom: float = math.nan
     train_loss: Optional[float] = None
     seconds_offspring: Optional[float] = None  # crossover and mutation (host)
+    seconds_prerelax: Optional[float] = None   # the first prerelax steps of every offspring (cfg.prerelax)
     seconds_predict: Optional[float] = None    # unrelaxed evaluation + features + prediction + selection
     seconds_relax: Optional[float] = None      # relaxation of the kept offspring
     seconds_train: Optional[float] = None      # retraining: host time to start it plus any wait for it after the update
@@ -305,11 +307,14 @@ class SurrogateGA:
         else:
             raise ValueError(f"unknown optimizer {self.cfg.optimizer!r}")
         E = c.batch_result(forces=False)["E"]
+        Xall = c.batch_positions()
         out, epa = [], np.zeros(len(structs))
+        a = 0
         for k, (spc, lat, _) in enumerate(structs):
-            X = c.result(k, forces=False)["X"]
-            out.append((spc, lat, X))
-            epa[k] = E[k] / len(spc)
+            n = len(spc)
+            out.append((spc, lat, Xall[a:a + n]))
+            epa[k] = E[k] / n
+            a += n
         return out, epa, done, steps
 
     def train(self) -> float:
@@ -425,17 +430,25 @@ class SurrogateGA:
                 kids += mutate(pop, weights, cfg, self.rng, nm)
             t_off = time.perf_counter()
             pred = None
+            ranked, pre_steps = kids, [0] * len(kids)
+            if screen and cfg.prerelax > 0:
+                # every offspring takes a few relaxation steps first, in one batch: the surrogate then ranks
+                # structures that have begun to settle (and is trained on such structures' features), and the kept
+                # ones go on from there
+                ranked, _, _, st_pre = self.relax(kids, max_steps=cfg.prerelax)
+                pre_steps = [int(v) for v in st_pre]
+            t_pre = time.perf_counter()
             if screen:
-                feat = self.features(kids)
+                feat = self.features(ranked)
                 pred = self.sur.predict(feat)
                 sel = self.select(pred)
             else:
                 feat = [None] * len(kids)
                 sel = np.arange(len(kids))
             t1 = time.perf_counter()
-            batch = [kids[k] for k in sel] + [c[0] for c in carry]
+            batch = [ranked[k] for k in sel] + [c[0] for c in carry]
             bfeat = [feat[k] for k in sel] + [c[1] for c in carry]
-            prior = [0] * len(sel) + [c[2] for c in carry]
+            prior = [pre_steps[k] for k in sel] + [c[2] for c in carry]
             fin, finE, finF, carry, dropped = self._relax_with_carry(batch, bfeat, prior)
             t2 = time.perf_counter()
             loss = None
@@ -457,7 +470,8 @@ class SurrogateGA:
                                    carried=len(carry), dropped=dropped)
             rep.seconds_select = t4 - t3
             rep.seconds_offspring = t_off - t0
-            rep.seconds_predict = t1 - t_off
+            rep.seconds_prerelax = t_pre - t_off
+            rep.seconds_predict = t1 - t_pre
             if screen and cfg.audit:
                 t1 = time.perf_counter()
                 _, epa_all, done_all, _ = self.relax(kids)

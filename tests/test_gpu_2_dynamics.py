@@ -181,10 +181,14 @@ def test_fire_batch_relaxes_and_matches_oracle():
     steps, done = c.relax(max_steps=2000, fmax=0.01)
     assert done.all(), steps
     res = c.batch_result()
+    Xall = c.batch_positions()  # one copy of every structure's positions: the same values as per structure
     om = po.OracleModel(path)
+    a = 0
     for k, (spc, lat, _) in enumerate(structs):
         n = len(spc)
         X = c.result(k)["X"]
+        np.testing.assert_array_equal(Xall[a:a + n], X)
+        a += n
         o = om.evaluate(spc, lat, X, urep=100.0)
         assert np.abs(o["F"]).max() < 0.011
         assert res["E"][k] == pytest.approx(o["E"], abs=1e-5 * n)

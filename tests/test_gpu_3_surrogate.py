@@ -158,6 +158,27 @@ def test_screened_search_short():
     assert rec > 0.35
 
 
+def test_screened_search_ranks_prerelaxed_offspring():
+    """prerelax: every offspring takes a few FIRE steps (one batch) before the surrogate ranks it, and the kept ones
+    go on from there.  The kept share is unchanged, the search stays elitist, and the audit (every offspring relaxed
+    from scratch) scores the ranking; on the 512-population workload 40 steps raised recall@25% from 0.39 to 0.53
+    (measured in the bench's search leg), here the recall is printed."""
+    from maise_amd.evolve import GAConfig, SurrogateGA
+    print("[gpu case] screened GA with prerelax 20, Mg16Ca8, pop 96, 3 generations", flush=True)
+    cfg = GAConfig(pop=96, generations=3, seed=21, max_steps=1500, prerelax=20)
+    ga = SurrogateGA(MGCA, cfg)
+    res = ga.run()
+    ga.close()
+    reps = res.reports
+    assert len(reps) == 4
+    assert reps[1].relaxed == round(0.25 * 96) + round(0.10 * 96)
+    assert all(r.seconds_prerelax > 0 for r in reps[1:])
+    assert all(np.isfinite(r.best_eV_atom) for r in reps)
+    assert all(reps[k + 1].best_eV_atom <= reps[k].best_eV_atom + 1e-12 for k in range(3))
+    print("recall@25% after 20 steps", [r.recall_top25 for r in reps[1:]], flush=True)
+    assert all(0.0 <= r.recall_top25 <= 1.0 for r in reps[1:])
+
+
 def test_capped_relaxation_carries_the_tail_over():
     """With a FIRE cap per generation, unconverged structures move into the next generation's batch instead of
     holding every generation to the slowest structure; nothing is lost except past max_steps in total."""
@@ -176,12 +197,13 @@ def test_capped_relaxation_carries_the_tail_over():
 
 def test_screening_at_equal_search_time():
     """The same seeded search (population 512, as the search workload) with and without the surrogate for the
-    same search clock.  What screening delivers by construction is asserted: 35% of the offspring relaxed, and so
-    more generations per second at this population (a batch of 179 structures of 24 atoms relaxes in about 60% of
-    the time of 512); the audited run above asserts a recall above random selection's.  Which search reaches the
-    lower energy first is measured and printed, not asserted: it depends on the seed, and on this GPU the
-    unscreened search has reached the lower energy first in most seeds measured, because relaxation time grows
-    far less than linearly with the batch below a few thousand structures."""
+    same search clock.  What screening delivers by construction is asserted: 35% of the offspring relaxed.  The
+    generation rate is printed, not asserted: a batch of 179 structures of 24 atoms relaxes in about 60% of the
+    time of 512, and the surrogate's evaluation and training take about what that saves, so at this population the
+    two searches run about as many generations (the bench's comparison at population 2,048, where relaxation time
+    grows with the batch, gives the screened search 1.6x the generations).  Which search reaches the lower energy
+    first is measured and printed, not asserted: it depends on the seed; the audited run above asserts a recall
+    above random selection's."""
     from maise_amd.evolve import GAConfig, compare_screening
     print("[gpu case] equal-time search, Mg16Ca8, pop 512, 1 seed x 4 s", flush=True)
     cfg = GAConfig(pop=512, relax_cap=300)
@@ -193,8 +215,6 @@ def test_screening_at_equal_search_time():
         for r in rr:
             print(name, r["seed"], "generations", r["generations"], "best", round(r["best_eV_atom"], 5),
                   "seconds to target", r["seconds_to_target"], flush=True)
-    su, ss = out["summary"]["unscreened"], out["summary"]["screened"]
-    assert ss["generations_mean"] > su["generations_mean"]
     for r in out["runs"]["screened"]:
         assert r["relaxed_per_generation"] == round(0.25 * 512) + round(0.10 * 512)
     for name in ("unscreened", "screened"):
