@@ -4,7 +4,9 @@
   shared-eta matrix-core force contraction with unused zeta slots (their 2^(1 - zeta) prefactor slot is 0);
 * the 43-function table with G4 etas up to 2.0 A^-2: radial factors exp(-eta (r_ij^2 + r_ik^2 + r_jk^2)) down to
   ~1e-30, far below the FP16 normal range the split matrix-core operands use;
-* hidden layers wider than 64 (51-96-96-1), which run on the FP32 VALU MLP kernel instead of MFMA.
+* hidden layers wider than 64 (51-96-96-1), which run on the FP32 VALU MLP kernel instead of MFMA;
+* hidden layers of one and two 32-unit blocks mixed (64-10 on the binary alloy, 10-48 on Cu): the two-wave MFMA
+  MLP kernel's layouts where one layer is split by output block between the waves and the other is not.
 
 Each synthetic model has seeded weights and input ranges set from the oracle's descriptors of the test
 structure, so the network is neither saturated nor trivially linear.  Bars are the FP32 path's: energy 1e-5
@@ -47,6 +49,8 @@ def _variants():
         "generic_shared_z13_cuag": ([29, 47], (10, 10), mf.generic_basis(zetas=(1.0, 3.0))),
         "large_eta_cu": ([29], (10, 10), mf.standard_basis(eta4=np.geomspace(0.05, 2.0, 7))),
         "wide_mlp_cu": ([29], (96, 96), None),
+        "mlp_64_10_cuag": ([29, 47], (64, 10), None),
+        "mlp_10_48_cu": ([29], (10, 48), None),
     }
 
 
@@ -56,7 +60,8 @@ def built(tmp_path_factory):
     return {name: _model(tmp, name, *spec) for name, spec in _variants().items()}
 
 
-@pytest.mark.parametrize("name", ["generic_shared_z13_cu", "generic_shared_z13_cuag", "large_eta_cu", "wide_mlp_cu"])
+@pytest.mark.parametrize("name", ["generic_shared_z13_cu", "generic_shared_z13_cuag", "large_eta_cu", "wide_mlp_cu",
+                                  "mlp_64_10_cuag", "mlp_10_48_cu"])
 def test_variant_matches_oracle(built, name):
     import maise_amd
     from oracle import pyoracle as po
@@ -71,6 +76,8 @@ def test_variant_matches_oracle(built, name):
         assert m.g4_variant == 1 and m.shared_eta
     if name == "wide_mlp_cu":
         assert not m.mfma
+    if name.startswith("mlp_"):
+        assert m.mfma
     c = maise_amd.Context(m)
     c.set_structure(spc, lat, x)
     r = c.evaluate()
