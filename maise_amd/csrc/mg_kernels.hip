@@ -952,7 +952,7 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
     for (int q = 1; q < MG_MAXSPC; q++) s += (q < S && t >= ri.o(q)) ? 1 : 0;
     int p = ri.si * MG_MAXSPC + s;
     float rc = bp->rc[p];
-    float tt = v.w / rc;
+    float tt = v.w * __builtin_amdgcn_rcpf(rc);  // r / rc within 1.5 ulp: no IEEE division sequence per neighbour
     float fc = v.w < rc ? 0.5f * (cospi_(tt) + 1.0f) : 0.0f;
     float r2 = v.w * v.w;
     const float ir = __builtin_amdgcn_rcpf(v.w);
@@ -1050,9 +1050,12 @@ __device__ __forceinline__ unsigned ubfe_v(unsigned x) {
 #define PK_J(p) ((int)ubfe_v<0, 11>(p))
 #define PK_K(p) ((int)ubfe_v<11, 11>(p))
 #define PK_TAG(p) ((int)((p) >> 22))
-// lane / n for 0 <= lane < 64, 0 < n <= 64: lane + 0.5 is exact and the two roundings move (lane + 0.5) / n by less
-// than 2^-17 / n, under the 0.5 / n that separates it from an integer
-__device__ __forceinline__ int lane_div(int lane, int n) { return (int)(((float)lane + 0.5f) * (1.0f / (float)n)); }
+// lane / n for 0 <= lane < 64, 0 < n <= 2048: lane + 0.5 is exact, and the hardware reciprocal (1 ulp) and the
+// product move (lane + 0.5) / n by less than 64 * 1.5 * 2^-23 = 1.2e-5, under the 0.5 / n >= 2.4e-4 that separates
+// it from an integer
+__device__ __forceinline__ int lane_div(int lane, int n) {
+  return (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)n));
+}
 
 // Enumerate the neighbour pairs (j, k) of one species class with r_jk < rc, compacting the survivors into a ring in
 // LDS and handing them to body(j, k, tag, valid) in batches of 64 as soon as 64 are waiting (valid = false on the
@@ -1292,7 +1295,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
       const int blk = bp->ind4[ds1][ds2];
       const int pjk = sa * MG_MAXSPC + sb;
       const float rc = bp->rc[pjk];
-      const float rinv = 1.0f / rc, rinvh = 0.5f * rinv;
+      const float rinv = __builtin_amdgcn_rcpf(rc), rinvh = 0.5f * rinv;  // hardware reciprocal (1 ulp)
       float ejk[NE];
 #pragma unroll
       for (int e = 0; e < NE; e++) ejk[e] = -(float)MG_LOG2E * bp->eta4[pjk][e];
@@ -1526,7 +1529,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
       const int blk = bp->ind4[ds1][ds2];
       const int pjk = sa * MG_MAXSPC + sb, pij = ri.si * MG_MAXSPC + sa, pik = ri.si * MG_MAXSPC + sb;
       const float rc = bp->rc[pjk];
-      const float rinv = 1.0f / rc, rinvh = 0.5f * rinv;
+      const float rinv = __builtin_amdgcn_rcpf(rc), rinvh = 0.5f * rinv;  // hardware reciprocal (1 ulp)
       const float dfk = -0.5f * 3.14159265358979f * rinv;
       // the class's etas (wave-uniform)
       float e3[NE], n3[NE], e1[NE], e2[NE];
@@ -1572,7 +1575,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         const bool own = ((m >> 2) & 1) == h;
         const float* gts = (const float*)gts4;
         const float pz = bp->pref[(qq / MG_MAXL) % MG_MAXZ];
-        const float ip = pz > 0.0f ? 2.0f / pz : 0.0f;
+        const float ip = pz > 0.0f ? 2.0f * __builtin_amdgcn_rcpf(pz) : 0.0f;  // pz = 2^(1 - zeta): exact for integer zeta
         float a[8], amax = 0.f;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
