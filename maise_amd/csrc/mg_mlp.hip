@@ -251,6 +251,13 @@ __device__ __forceinline__ f32x16 mfma_frag(const AFrag<PREC>& A, const float* b
   }
 }
 
+// floats of the two-wave kernel's exchange regions: layer 1's (always), delta2's and the energy partials (a two-block
+// second layer), delta1's (a two-block first layer).  Only what the layout uses is allocated, so a one-block network
+// on a wide input keeps more tiles resident per CU.
+static inline int mlp2_exchange_floats(int hb1, int hb2) {
+  return 2 * 16 * 64 + (hb2 == 2 ? 2 * 16 * 64 + 2 * 64 : 0) + (hb1 == 2 ? 2 * 16 * 64 : 0);
+}
+
 // ---- two waves per tile.  The single-wave kernel above leaves one wave per SIMD on most of the chip (a tile is 32
 // atoms), so each wave's load latencies and its chain of matrix-core and tanh work run alone.  Here the two waves
 // of a workgroup share a tile: wave w owns hidden block w of a 64-wide layer (HB == 2), half of the input layer's
@@ -288,10 +295,11 @@ __global__ __launch_bounds__(128) void k_mlp_mfma2(const int4* __restrict__ tile
   const int k1 = f.k1;
   const int Kmax = BIN ? 16 * k1 : 2 * k1;
   const int RS = Kmax + 1;
-  float* xa = gsm + 32 * RS;      // h1 (HB1 == 2) or the layer-1 partial sums (HB1 == 1)
-  float* xb2 = xa + 2 * 16 * 64;  // delta2 (HB2 == 2) and the energy partials
-  float* xc = xb2 + 2 * 16 * 64 + 2 * 64;  // d1 (HB1 == 2)
-  float* eb = xb2 + 2 * 16 * 64;
+  // exchange regions after the staged inputs, only those the block layout uses (mlp2_exchange_floats)
+  float* xa = gsm + 32 * RS;                             // h1 (HB1 == 2) or the layer-1 partial sums (HB1 == 1)
+  float* xb2 = xa + 2 * 16 * 64;                         // delta2 (HB2 == 2)
+  float* eb = xb2 + 2 * 16 * 64;                         // the energy partials (HB2 == 2)
+  float* xc = HB2 == 2 ? eb + 2 * 64 : xa + 2 * 16 * 64;  // d1 (HB1 == 2)
 
   // the tile's inputs x - xoff in LDS, wave w staging tile atoms 16 w .. 16 w + 15
   for (int k0 = 0; k0 < Kmax; k0 += WAVE) {
@@ -500,7 +508,7 @@ extern "C" int mgk_mlp_mfma(int ntiles, const void* tiles, const int* perm, cons
   const size_t sm = (size_t)32 * ((bin ? 16 : 2) * f.k1 + 1) * sizeof(float);  // staged inputs of a tile
   if (sm > 64 * 1024) return -2;
   // two waves per tile where the exchange regions fit beside the staged inputs (every shipped width does)
-  const size_t sm2 = sm + (size_t)(3 * 2 * 16 * 64 + 2 * 64) * sizeof(float);
+  const size_t sm2 = sm + (size_t)mlp2_exchange_floats(f.hb1, f.hb2) * sizeof(float);
   const bool two = sm2 <= 64 * 1024;
 #define MLP_L(PI, PH, H1B, H2B)                                                                                      \
   do {                                                                                                               \
