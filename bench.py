@@ -216,7 +216,7 @@ def relax_leg(maise_amd, batch, args):
     c.set_batch(batch[:64])
     c.relax(max_steps=50, fmax=fmax)           # warm-up: code objects, allocations
     c.set_autotune(True)                       # tuned on the full batch before the first capture (timed)
-    c.set_skin(1.0)                            # skin neighbour lists (results bit-identical to rebuilding each step)
+    c.set_skin(args.relax_skin)                # skin neighbour lists (results bit-identical to rebuilding each step)
     t0 = time.perf_counter()
     c.set_batch(batch)
     steps, done = c.relax(max_steps=max_steps, fmax=fmax)
@@ -242,7 +242,7 @@ def relax_leg(maise_amd, batch, args):
     c.set_batch(batch[:64])
     c.relax_lbfgs(max_steps=50, fmax=fmax)     # warm-up
     c.set_autotune(True)
-    c.set_skin(1.0)
+    c.set_skin(args.relax_skin)
     t0 = time.perf_counter()
     c.set_batch(batch)
     lsteps, ldone = c.relax_lbfgs(max_steps=max_steps, fmax=fmax)
@@ -276,7 +276,7 @@ def relax_leg(maise_amd, batch, args):
                         "random triclinic cells, 15-20 A^3/atom, dmin 2.0 A, seed 3; shipped potential "
                         "nn_CuPdAg_d3_v1 (282-10-10-1, Rc 7.5 A), UREP 100 eV; FIRE to fmax 0.01 eV/A, cap 3000",
             "relaxed_structures_per_s": float(done.sum()) / dt, "seconds": dt,
-            "converged": int(done.sum()), "structures": len(batch),
+            "converged": int(done.sum()), "structures": len(batch), "skin_A": args.relax_skin,
             "fire_steps_mean": float(steps.mean()), "fire_steps_max": int(steps.max()),
             "autotuned_shape": {"nbr_wpb": shape[0], "desc_wpb": shape[1], "force_wpb": shape[2],
                                 "pair_store": shape[3]},
@@ -470,6 +470,7 @@ def main():
     ap.add_argument("--relax-single", type=int, default=48,
                     help="structures relaxed one at a time for the unbatched comparison")
     ap.add_argument("--md-steps", type=int, default=1000, help="NVT steps of the MD leg (0 skips it)")
+    ap.add_argument("--relax-skin", type=float, default=1.0, help="skin (A) of the relax leg's neighbour lists (0 = off)")
     ap.add_argument("--ga-gens", type=int, default=30, help="generations of the screened search (0 skips it)")
     ap.add_argument("--ga-pop", type=int, default=512, help="population of the screened search")
     ap.add_argument("--ga-prerelax", type=int, default=40,
