@@ -901,6 +901,12 @@ __device__ __forceinline__ void split_f16x2(float a, float b, unsigned& hi, unsi
   hi = __builtin_bit_cast(unsigned, h);
   lo = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a - hf[0], b - hf[1]}, f16x2));
 }
+// the same for one value in the low half, the high half zero (an odd count of etas)
+__device__ __forceinline__ void split_f16x1(float a, unsigned& hi, unsigned& lo) {
+  const _Float16 h = (_Float16)a;
+  hi = (unsigned)__builtin_bit_cast(unsigned short, h);
+  lo = (unsigned)__builtin_bit_cast(unsigned short, (_Float16)(a - (float)h));
+}
 // a = hi + lo for two values at once: hi rounds a to BF16 (one packed conversion), lo rounds the exact residual;
 // the BF16 halves of hi are widened back by shifts
 __device__ __forceinline__ void split_bf16x2(float a, float b, unsigned& hi, unsigned& lo) {
@@ -908,6 +914,11 @@ __device__ __forceinline__ void split_bf16x2(float a, float b, unsigned& hi, uns
   const float ha = __builtin_bit_cast(float, h << 16), hb = __builtin_bit_cast(float, h & 0xffff0000u);
   hi = h;
   lo = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a - ha, b - hb}, bf16x2));
+}
+__device__ __forceinline__ void split_bf16x1(float a, unsigned& hi, unsigned& lo) {
+  const __bf16 h = (__bf16)a;
+  hi = (unsigned)__builtin_bit_cast(unsigned short, h);
+  lo = (unsigned)__builtin_bit_cast(unsigned short, (__bf16)(a - (float)h));
 }
 // cos(pi * t) and sin(pi * t) for t in [0, 1]: the hardware takes revolutions, cos(2 pi y)
 __device__ __forceinline__ float cospi_(float t) { return __builtin_amdgcn_cosf(0.5f * t); }
@@ -1185,15 +1196,16 @@ __device__ __forceinline__ void group_accum(float4* fj, float4* fk, int j, float
   unsigned long long rem = __ballot(valid);
   while (rem) {
     const int cur = __builtin_amdgcn_readlane(tag, (int)__builtin_ctzll(rem));  // scalar read, no LDS round trip
-    const bool mine = valid && tag == cur;
-    if (mine) {
+    const bool same = tag == cur;
+    const unsigned long long hit = __ballot(same);  // rem holds valid lanes only, so this clears what is done
+    if (valid && same) {
       float4 v = fj[j], w = fk[k];
       v.x += jx; v.y += jy; v.z += jz;
       w.x += kx; w.y += ky; w.z += kz;
       fj[j] = v;
       fk[k] = w;
     }
-    rem &= ~__ballot(mine);
+    rem &= ~hit;
   }
 }
 
@@ -1550,8 +1562,8 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
       // (MH: split FP16 operands, the FP32 precision; otherwise split BF16, the BF16 precisions; per-pair etas keep
       // the VALU contraction below).  Per batch of 64 triplets, one v_mfma_f32_32x32x16 (three with the split
       // operands) computes, for each triplet t,
-      //   C_q(t) = sum_e g_eq B_e(t)  and  D_q(t) = sum_e eta_e g_eq B_e(t),   B_e = exp(-eta_e (r1^2 + r2^2 + r3^2)),
-      // so that X = sum_e B_e S_e = sum_q C_q P_q, Y = sum_e eta_e B_e S_e = sum_q D_q P_q and
+      //   C_q(t) = sum_e g_eq B_e(t)  and  D_q(t) = -2 sum_e eta_e g_eq B_e(t),   B_e = exp(-eta_e (r1^2 + r2^2 + r3^2)),
+      // so that X = sum_e B_e S_e = sum_q C_q P_q, -2 Y = -2 sum_e eta_e B_e S_e = sum_q D_q P_q and
       // Z = sum_e B_e S'_e = sum_q C_q P'_q.  B operand: lane t gives its B_0..7 as k = 8(t/32) + e of column t%32.
       // A operand: block diagonal, so that the rows a lane receives (8(r/4) + 4(t/32) + r%4, register r) depend on
       // its own k half only; register r of every lane then holds output r (C_0..7, D_0..7) of its own triplet.
@@ -1581,7 +1593,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         for (int v = 0; v < 8; v++) {
           a[v] = 0.f;
           if (own && v < NE)
-            a[v] = (r < ZL ? gts[v * ZL + qq] : e3[v < NE ? v : 0] * gts[v * ZL + qq]) * ip *
+            a[v] = (r < ZL ? gts[v * ZL + qq] : -2.0f * e3[v < NE ? v : 0] * gts[v * ZL + qq]) * ip *
                    fast_exp2(n3[v < NE ? v : 0] * rlb);
           amax = fmaxf(amax, fabsf(a[v]));
         }
@@ -1603,11 +1615,14 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
 #pragma unroll
         for (int v = 0; v < 8; v++) {
           float a = 0.f;
-          if (own && v < NE) a = r < ZL ? gts[v * ZL + r] : e3[v < NE ? v : 0] * gts[v * ZL + r - ZL];
+          if (own && v < NE) a = r < ZL ? gts[v * ZL + r] : -2.0f * e3[v < NE ? v : 0] * gts[v * ZL + r - ZL];
           ahi[v] = (__bf16)a;
           alo[v] = (__bf16)(a - (float)ahi[v]);
         }
       }
+      // the matrix-core sums come out scaled by 1/oscale; every term of dT is linear in f3 or d3, so the scale is
+      // taken there (a power of two: exact) instead of on X, Y and Z
+      const float h3 = 0.5f * oscale, dfo = dfk * oscale;
       // the grid of this species block, loop-invariant: kept in registers for the whole class (VALU contraction)
       float gv[NE * ZL];
 #pragma unroll
@@ -1618,7 +1633,9 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         gv[e * ZL + 4] = g1.x; gv[e * ZL + 5] = g1.y; gv[e * ZL + 6] = g1.z; gv[e * ZL + 7] = g1.w;
       }
       auto body = [&](int j, int k, int tag, bool valid) {
-        if (!valid) { j = 0; k = 1; }  // a harmless slot pair; its gradient is discarded below
+        // a harmless slot pair for the lanes past the batch (their gradient is discarded below); the stored list's
+        // reader already gives them the row's last pair
+        if (!ST && !valid) { j = 0; k = 1; }
         float4 a = rec[j], b = rec[k];
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
@@ -1626,8 +1643,8 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         float r3 = __builtin_amdgcn_sqrtf(r3s);  // hardware square root (1 ulp): no IEEE fix-up sequence
         float i3 = __builtin_amdgcn_rcpf(r3);
         const float th = r3 * rinvh;  // cos(pi r / rc) = cos(2 pi th) in the hardware's revolutions
-        float f3 = __builtin_fmaf(__builtin_amdgcn_cosf(th), 0.5f, 0.5f);
-        float d3 = dfk * __builtin_amdgcn_sinf(th);
+        float f3 = __builtin_fmaf(__builtin_amdgcn_cosf(th), h3, h3);
+        float d3 = dfo * __builtin_amdgcn_sinf(th);
         float i1 = ia.x, i2 = ib.x, f1 = ia.y, f2 = ib.y, d1 = ia.z, d2 = ib.z;
         float cs = (a.x * b.x + a.y * b.y + a.z * b.z) * i1 * i2;
         float P[ZL], Pd[ZL];
@@ -1651,7 +1668,10 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
           f32x16 acc = {};
           if (MH) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) split_f16x2(B[2 * v], B[2 * v + 1], h[v], l[v]);
+            for (int v = 0; v < 4; v++)
+              if (2 * v + 1 < NE) split_f16x2(B[2 * v], B[2 * v + 1], h[v], l[v]);
+              else if (2 * v < NE) split_f16x1(B[2 * v], h[v], l[v]);
+              else h[v] = l[v] = 0u;
             const f16x8 bh = __builtin_bit_cast(f16x8, (u32x4){h[0], h[1], h[2], h[3]});
             const f16x8 bl = __builtin_bit_cast(f16x8, (u32x4){l[0], l[1], l[2], l[3]});
             acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(hlo, bh, acc, 0, 0, 0);
@@ -1659,7 +1679,10 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
             acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(hhi, bh, acc, 0, 0, 0);
           } else {
 #pragma unroll
-            for (int v = 0; v < 4; v++) split_bf16x2(B[2 * v], B[2 * v + 1], h[v], l[v]);
+            for (int v = 0; v < 4; v++)
+              if (2 * v + 1 < NE) split_bf16x2(B[2 * v], B[2 * v + 1], h[v], l[v]);
+              else if (2 * v < NE) split_bf16x1(B[2 * v], h[v], l[v]);
+              else h[v] = l[v] = 0u;
             const bf16x8 bh = __builtin_bit_cast(bf16x8, (u32x4){h[0], h[1], h[2], h[3]});
             const bf16x8 bl = __builtin_bit_cast(bf16x8, (u32x4){l[0], l[1], l[2], l[3]});
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bh, acc, 0, 0, 0);
@@ -1672,7 +1695,6 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
             Y1 += acc[ZL + q] * P[q];
             Zc += acc[q] * Pd[q];
           }
-          if (MH) { X *= oscale; Y1 *= oscale; Zc *= oscale; }
         }
 #pragma unroll
         for (int e = 0; e < (MM ? 0 : NE); e++) {
@@ -1696,9 +1718,10 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         if (UNI) { Y2 = Y1; Y3 = Y1; }
         float r1 = a.w, r2 = b.w;
         float Fp = f1 * f2 * f3;
-        float dT1 = -2.0f * r1 * Fp * Y1 + d1 * f2 * f3 * X;
-        float dT2 = -2.0f * r2 * Fp * Y2 + f1 * d2 * f3 * X;
-        float dT3 = -2.0f * r3 * Fp * Y3 + f1 * f2 * d3 * X;
+        constexpr float m2 = MM ? 1.0f : -2.0f;  // the matrix-core D sums carry the -2 already
+        float dT1 = m2 * r1 * Fp * Y1 + d1 * f2 * f3 * X;
+        float dT2 = m2 * r2 * Fp * Y2 + f1 * d2 * f3 * X;
+        float dT3 = m2 * r3 * Fp * Y3 + f1 * f2 * d3 * X;
         float dTc = Fp * Zc;
         float i12 = i1 * i2;
         float ca = dT1 * i1 - dTc * cs * i1 * i1;  // coefficient of a in dT/da
