@@ -1061,6 +1061,9 @@ __device__ __forceinline__ unsigned ubfe_v(unsigned x) {
 #define PK_J(p) ((int)ubfe_v<0, 11>(p))
 #define PK_K(p) ((int)ubfe_v<11, 11>(p))
 #define PK_TAG(p) ((int)((p) >> 22))
+// j and k renamed between the reads of two record planes, so that each plane's address is one shift-add of the slot
+// (shared, the shifted slot costs a shift and one add per plane)
+__device__ __forceinline__ void fresh2(int& j, int& k) { asm("" : "+v"(j), "+v"(k)); }
 // lane / n for 0 <= lane < 64, 0 < n <= 2048: lane + 0.5 is exact, and the hardware reciprocal (1 ulp) and the
 // product move (lane + 0.5) / n by less than 64 * 1.5 * 2^-23 = 1.2e-5, under the 0.5 / n >= 2.4e-4 that separates
 // it from an integer
@@ -1177,13 +1180,16 @@ __device__ __forceinline__ int for_pairs(const float4* rec, unsigned* ring, bool
 template <typename F>
 __device__ __forceinline__ void for_stored_pairs(const unsigned* __restrict__ pl, int np, int lane, F&& body) {
   if (np <= 0) return;
-  const int last = np - 1;
-  unsigned pn = pl[lane < np ? lane : last];
+  // a lane's pair as a 32-bit byte offset from the class's (scalar) list base, clamped to the last pair
+  const char* base = (const char*)pl;
+  const unsigned endb = 4u * (unsigned)np, lastb = endb - 4u;
+  unsigned ob = 4u * (unsigned)lane;
+  unsigned pn = *(const unsigned*)(base + (ob < lastb ? ob : lastb));
   for (int tb = 0; tb < np; tb += WAVE) {
     const unsigned p = pn;
-    const bool valid = tb + lane < np;
-    const int tn = tb + WAVE + lane;
-    pn = pl[tn < np ? tn : last];
+    const bool valid = ob < endb;
+    ob += 4u * WAVE;
+    pn = *(const unsigned*)(base + (ob < lastb ? ob : lastb));
     body(PK_J(p), PK_K(p), PK_TAG(p), valid);
   }
 }
@@ -1334,6 +1340,7 @@ __global__ __launch_bounds__(256) void k_desc(int N, int S, const int* __restric
       auto body = [&](int j, int k, int, bool valid) {
         if (!valid) return;
         float4 a = rec[j], b = rec[k];  // (d, r^2)
+        fresh2(j, k);
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
         float4 ua = rec1[j], ub = rec1[k];  // (d / r, fc)
@@ -1637,6 +1644,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
         // reader already gives them the row's last pair
         if (!ST && !valid) { j = 0; k = 1; }
         float4 a = rec[j], b = rec[k];
+        fresh2(j, k);
         float cx = b.x - a.x, cy = b.y - a.y, cz = b.z - a.z;
         float r3s = cx * cx + cy * cy + cz * cz;
         float4 ia = rec1[j], ib = rec1[k];
