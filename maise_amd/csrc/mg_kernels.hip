@@ -945,7 +945,7 @@ __device__ __forceinline__ void row_info(RowInfo& ri, int i, const int* __restri
   ri.m3 = S > 3 ? cnt[4 * i + 3] : 0;
 }
 
-// Stage the row of atom i into LDS.  Record per neighbour: (dx, dy, dz, r), (1/r, fc, dfc, r^2) and, unless
+// Stage the row of atom i into LDS.  Record per neighbour: (dx, dy, dz, r), (1/r, fc, fc'/r, r^2) and, unless
 // every species pair shares the G4 etas (UNI), A[0..7] = exp(-eta4(i,j)_e r^2).  With shared etas a triplet's
 // radial factor is one exp(-eta_e (r_ij^2 + r_ik^2 + r_jk^2)) and the A tables are not needed.  The records lie
 // in planes of cap float4 (record q of slot t at rec[q cap + t]), so consecutive slots are consecutive 16-byte words
@@ -973,7 +973,7 @@ __device__ __forceinline__ void stage_row(float4* rec, const RowInfo& ri, const 
     } else {
       float dfc = v.w < rc ? -0.5f * 3.14159265358979f / rc * sinpi_(tt) : 0.0f;
       rec[t] = v;
-      rec[pl + t] = make_float4(ir, fc, dfc, r2);
+      rec[pl + t] = make_float4(ir, fc, dfc * ir, r2);
     }
     if (!UNI) {
       float A[8];
@@ -1485,7 +1485,7 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
   float* fnb = (float*)fnb4;
   unsigned* plist = (unsigned*)(fnk4 + cap);     // [PRING]
   stage_row<NE, UNI>(rec, ri, vec, bp, lane, S, cap);
-  const float4* rec1 = rec + cap;  // (1/r, fc, dfc, r^2) plane
+  const float4* rec1 = rec + cap;  // (1/r, fc, fc'/r, r^2) plane
   const float4* rec2 = rec + 2 * cap;  // A tables (without UNI)
   const float4* rec3 = rec + 3 * cap;
   for (int t = lane; t < ri.M; t += WAVE) {
@@ -1514,17 +1514,17 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
     for (int t = ri.o(s) + lane; t < ri.o(s) + ri.m(s); t += WAVE) {
       float4 v = rec[t];
       float4 w = rec1[t];
-      float r = v.w, fc = w.y, dfc = w.z;
-      float dT = 0.0f;
+      // sc = dT/dr / r from fc/r and fc'/r
+      float r = v.w, fcr = w.y * w.x, dfr = w.z;
+      float sc = 0.0f;
 #pragma unroll
       for (int c = 0; c < MG_MAXG2; c++)
         if (c < bp->ng2) {
           float eta = bp->eta2[p][c];
           float dr = r - bp->rs2[p][c];
           float ex = fast_exp2(-(float)MG_LOG2E * eta * dr * dr);
-          dT += gi[blk + c] * ex * (-2.0f * eta * dr * fc + dfc);
+          sc += gi[blk + c] * ex * (-2.0f * eta * dr * fcr + dfr);
         }
-      float sc = dT * w.x;
       fnb[4 * t] -= sc * v.x; fnb[4 * t + 1] -= sc * v.y; fnb[4 * t + 2] -= sc * v.z;
       if (urep > 0.0f && r < Rrep) {
         // per ordered pair, acting on atom i only, as CELL_UREP (src/cmod.c:95-116)
@@ -1701,7 +1701,20 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
           for (int q = 0; q < ZL; q++) {
             X += acc[q] * P[q];
             Y1 += acc[ZL + q] * P[q];
-            Zc += acc[q] * Pd[q];
+            if (!Pat<PAT>::ZSTD) Zc += acc[q] * Pd[q];
+          }
+          if (Pat<PAT>::ZSTD) {
+            // zeta = 1, 2, 4, 16 on u_l (cell 2z + l): sum_z C_zl dP_zl/dc = dk lam_l (C_0l + 2u (C_1l + 2u^2 (C_2l
+            // + 4 u^12 C_3l))), with dk = 1/2 for the halved powers; lam = -1, +1
+            float zl[2];
+#pragma unroll
+            for (int l = 0; l < 2; l++) {
+              const float u = P[l], u2 = P[2 + l], u12 = P[4 + l] * (P[4 + l] * P[4 + l]);
+              const float s2 = __builtin_fmaf(acc[6 + l] * u12, 4.0f, acc[4 + l]);
+              const float s1 = __builtin_fmaf(u2 * s2, 2.0f, acc[2 + l]);
+              zl[l] = __builtin_fmaf(u * s1, 2.0f, acc[l]);
+            }
+            Zc = (MH ? 0.5f : 1.0f) * (zl[1] - zl[0]);
           }
         }
 #pragma unroll
@@ -1724,18 +1737,19 @@ __device__ __forceinline__ void force_row(int N, int S, const int* __restrict__ 
           Zc += Sd * B;
         }
         if (UNI) { Y2 = Y1; Y3 = Y1; }
-        float r1 = a.w, r2 = b.w;
-        float Fp = f1 * f2 * f3;
+        // dT/dr_n / r_n = -2 Fp Y_n + (fc_n'/r_n) (the other two cutoffs) X: the r_n of d(r_n^2)/dr_n cancels the
+        // 1/r_n of the direction (d1, d2 are fc'/r from the records; d3 / r3 here)
+        const float f12 = f1 * f2, Fp = f12 * f3;
         constexpr float m2 = MM ? 1.0f : -2.0f;  // the matrix-core D sums carry the -2 already
-        float dT1 = m2 * r1 * Fp * Y1 + d1 * f2 * f3 * X;
-        float dT2 = m2 * r2 * Fp * Y2 + f1 * d2 * f3 * X;
-        float dT3 = m2 * r3 * Fp * Y3 + f1 * f2 * d3 * X;
-        float dTc = Fp * Zc;
-        float i12 = i1 * i2;
-        float ca = dT1 * i1 - dTc * cs * i1 * i1;  // coefficient of a in dT/da
-        float cb = dTc * i12;                      // coefficient of b in dT/da (and of a in dT/db)
-        float c3c = dT3 * i3;                      // coefficient of c
-        float cbb = dT2 * i2 - dTc * cs * i2 * i2; // coefficient of b in dT/db
+        const float FY1 = m2 * Fp * Y1;
+        const float FY2 = UNI ? FY1 : m2 * Fp * Y2, FY3 = UNI ? FY1 : m2 * Fp * Y3;
+        const float c1 = __builtin_fmaf(d1, f2 * f3 * X, FY1);
+        const float c2 = __builtin_fmaf(d2, f1 * f3 * X, FY2);
+        const float c3c = __builtin_fmaf(d3 * i3, f12 * X, FY3);  // coefficient of c
+        float dTc = Fp * Zc, dcs = dTc * cs;
+        float ca = c1 - dcs * (i1 * i1);   // coefficient of a in dT/da
+        float cb = dTc * (i1 * i2);        // coefficient of b in dT/da (and of a in dT/db)
+        float cbb = c2 - dcs * (i2 * i2);  // coefficient of b in dT/db
         float gax = ca * a.x + cb * b.x - c3c * cx, gay = ca * a.y + cb * b.y - c3c * cy, gaz = ca * a.z + cb * b.z - c3c * cz;
         float gbx = cbb * b.x + cb * a.x + c3c * cx, gby = cbb * b.y + cb * a.y + c3c * cy, gbz = cbb * b.z + cb * a.z + c3c * cz;
         group_accum(fnb4, fnk4, j, -gax, -gay, -gaz, k, -gbx, -gby, -gbz, tag, valid);
